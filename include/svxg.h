@@ -1,0 +1,231 @@
+/* svxg — the fragment-graph elastic-equilibrium stress solve on one MI355X (gfx950).
+ *
+ * C ABI of the device library libsvxg.so. Plain structs, pointers and sizes: no C++ types, no
+ * torch types. Every call returns an int status (SVXG_OK = 0); nothing aborts.
+ *
+ * The data model mirrors the reference's (core/include/svx/stress/stress.hpp:21-51 SNode/SBond,
+ * core/include/svx/material/material.hpp:47-86 Material) field for field, as plain structs. The
+ * entry points replace the member functions of svx::StressProblem
+ * (core/include/svx/stress/stress.hpp:74-160) and the free functions bond_strength /
+ * bond_utilization (stress.hpp:57,66); each declaration cites the function it replaces.
+ *
+ * Threading: calls on distinct problems may come from distinct host threads (each problem
+ * owns its stream); calls on one problem must not overlap. svxg_solve_batch runs many
+ * problems in one set of launches (the preferred entry for many problems a tick).
+ */
+#ifndef SVXG_H
+#define SVXG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+  SVXG_OK = 0,
+  SVXG_BAD_INPUT = 1,    /* invalid argument (index out of range, size mismatch, null) */
+  SVXG_OOM_BUDGET = 2,   /* the context's HBM budget would be exceeded */
+  SVXG_DEVICE_ERROR = 3, /* a HIP call failed (the message: svxg_last_error) */
+  SVXG_NOT_ASSEMBLED = 4,/* the problem has no operator (assemble failed or was never run) */
+  SVXG_SINGULAR = 5      /* assemble: a diagonal block stayed singular after regularisation */
+};
+
+/* ---- precision of a solve ---------------------------------------------------------------- */
+enum {
+  /* FP64 PCG, FP64 K products, FP64 V-cycle: the plain path */
+  SVXG_FP64 = 0,
+  /* FP64 iterative refinement (FP64 residual recomputation, FP64 dots, FP64 coarse solve)
+     around an inner PCG whose K products and V-cycle run in FP32 */
+  SVXG_MIXED = 1
+};
+
+/* ---- data model -------------------------------------------------------------------------- */
+
+/* A fragment: rigid, 6 DOFs (u, theta about c). stress.hpp:21-26 (SNode). */
+typedef struct svxg_node {
+  double c[3];     /* centre (the solve's frame) */
+  double mass;
+  uint8_t fixed;   /* pinned: no DOFs */
+  uint8_t gone;    /* retired: an identity row, no bonds */
+  uint8_t pad_[6];
+} svxg_node;       /* 40 bytes */
+
+/* An interface between two fragments (b < 0: a fixed support). stress.hpp:28-51 (SBond). */
+typedef struct svxg_bond {
+  int32_t a, b;
+  uint8_t broken;     /* severed */
+  uint8_t in_k;       /* its stiffness is in the assembled K (kept by the library) */
+  uint8_t ma, mb;     /* material ids of the two sides */
+  uint8_t sectioned;  /* section strengths below are used (else the weaker material's) */
+  uint8_t pad0_[3];
+  double n[3], t1[3], t2[3]; /* local frame: n from a towards b */
+  double area;               /* m^2 */
+  double p[3];               /* section centroid */
+  double s1, s2;             /* second moments of area about p along t1, t2 */
+  double c1, c2;             /* extreme fibre distances along t1, t2 */
+  double rmax;               /* largest distance of the section from p (torsion) */
+  double la, lb;             /* lengths along n: node a to section, section to node b */
+  double strength;           /* design strength multiplier */
+  float ft, fb, fc, coh, mu; /* section strengths (Pa) when sectioned */
+  int32_t faces, tag;
+  int32_t pad1_;
+} svxg_bond;                 /* 216 bytes */
+
+/* Material properties a solve and the checks read. material.hpp:47-57 (Material). */
+typedef struct svxg_material {
+  double E, G, rho;
+  double ft, fb, fc, cohesion, friction;
+} svxg_material;             /* 64 bytes */
+
+/* AmgOptions (core/include/svx/solve/amg.hpp:46-69). The device V-cycle smooths with Chebyshev
+   polynomials of degree cheb_degree in D^-1 A (amg.hpp:53-55 names that smoother; it runs
+   row-parallel and is symmetric), not with the reference's hybrid Gauss-Seidel. */
+typedef struct svxg_amg_options {
+  double strength;           /* 0.1: aggregate across ||A_ij|| >= s sqrt(||A_ii|| ||A_jj||) */
+  double coarse_scale;       /* 0.6: scaling of unsmoothed Galerkin coarse operators */
+  int32_t coarse_max;        /* 48: dense solve at or below this many nodes */
+  int32_t smoothed;          /* 1: smoothed aggregation P = (I - w D^-1 A) P_rigid */
+  int32_t max_levels;        /* 16 */
+  int32_t sweeps;            /* 1: smoother applications before and after the coarse correction */
+  int32_t cheb_degree;       /* device smoother degree (default 2) */
+  int32_t coarsen_dense;     /* 1 */
+  int32_t reaggregate;       /* 1 */
+  int32_t pad_;
+} svxg_amg_options;
+
+/* StressOptions (stress.hpp:68-72). */
+typedef struct svxg_stress_options {
+  double rtol;               /* 2e-3: relative residual of a converged solve */
+  int32_t amg_min_nodes;     /* 160: smaller graphs are block-Jacobi preconditioned */
+  int32_t precision;         /* SVXG_FP64 or SVXG_MIXED */
+  svxg_amg_options amg;
+} svxg_stress_options;
+
+/* PcgResult (amg.hpp:120-125). */
+typedef struct svxg_pcg_result {
+  int32_t iters;
+  int32_t converged;
+  int32_t breakdown;
+  int32_t refinements;       /* SVXG_MIXED: outer refinement steps (0 for SVXG_FP64) */
+  double rel_res;
+} svxg_pcg_result;
+
+/* BondLoad (stress.hpp:61-63): N (tension > 0), shear V1, V2, torsion T, bending M1, M2. */
+typedef struct svxg_bond_load {
+  double N, V1, V2, T, M1, M2;
+} svxg_bond_load;
+
+typedef struct svxg_ctx svxg_ctx;
+typedef struct svxg_problem svxg_problem;
+
+/* ---- context ------------------------------------------------------------------------------ */
+
+/* A device context: one GPU, an HBM budget (bytes; 0 = 200 GB) that every problem's device
+   memory is admitted against. No reference counterpart (the reference has no device). */
+int svxg_ctx_create(int device, int64_t hbm_budget, svxg_ctx** out);
+int svxg_ctx_destroy(svxg_ctx* ctx);
+/* Bytes of device memory held by the context's problems. */
+int64_t svxg_ctx_bytes_in_use(const svxg_ctx* ctx);
+/* The message of the last failing call on this thread. */
+const char* svxg_last_error(void);
+/* Default options: StressOptions{} / AmgOptions{} (stress.hpp:68-72, amg.hpp:46-69). */
+void svxg_default_options(svxg_stress_options* out);
+
+/* ---- problems ----------------------------------------------------------------------------- */
+
+/* A problem holds the graph (copied: nodes, bonds, the material table) and, once assembled,
+   its device operator and preconditioner. StressProblem's public members nodes / bonds / mats
+   (stress.hpp:76-79) become these arrays; nmats <= 127 (material.hpp:45). */
+int svxg_problem_create(svxg_ctx* ctx, const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m,
+                        const svxg_material* mats, int32_t nmats, svxg_problem** out);
+int svxg_problem_destroy(svxg_problem* p);
+
+/* StressProblem::assemble (stress.hpp:83; stress.cpp:134-176): K from the intact bonds on the
+   device, then block-Jacobi (nfree < amg_min_nodes) or the multigrid hierarchy, built on the
+   device. Returns SVXG_SINGULAR where the reference's assemble returns false. */
+int svxg_problem_assemble(svxg_problem* p, const svxg_stress_options* opt);
+/* StressProblem::release (stress.hpp:99; stress.cpp:326-339). */
+int svxg_problem_release(svxg_problem* p);
+/* StressProblem::invalidate (stress.hpp:85-88). */
+int svxg_problem_invalidate(svxg_problem* p);
+
+/* StressProblem::solve (stress.hpp:122; stress.cpp:348-360). f, u: 6 per node over all nodes
+   (host memory); u is the warm start when warm != 0, the solution on return. maxit < 0: no cap.
+   The precision is the options' (assemble). */
+int svxg_problem_solve(svxg_problem* p, const double* f, double* u, double rtol, int32_t maxit, int32_t warm,
+                       svxg_pcg_result* res);
+/* StressProblem::begin / iterate / current (stress.hpp:126-130; stress.cpp:362-475): the same
+   solve spread over calls. u0 may be null (cold start). */
+int svxg_problem_begin(svxg_problem* p, const double* f, const double* u0);
+int svxg_problem_iterate(svxg_problem* p, int32_t maxit, double rtol, svxg_pcg_result* res);
+int svxg_problem_current(svxg_problem* p, double* u);
+
+/* StressProblem::remove_bond (stress.hpp:108; stress.cpp:260-284): the bonds break and their
+   stiffness leaves K in place (never twice). */
+int svxg_problem_remove_bonds(svxg_problem* p, const int32_t* ids, int32_t count);
+/* StressProblem::retire_nodes (stress.hpp:109; stress.cpp:286-310). */
+int svxg_problem_retire_nodes(svxg_problem* p, const int32_t* ids, int32_t count);
+/* Restores bond b's `broken` flag to 0 without putting it back into K (the caller-kept topology of
+   a retired chip, as test_stress.cpp:248-249 does through the public SBond member). */
+int svxg_problem_set_bond_broken(svxg_problem* p, int32_t b, int32_t broken);
+
+/* StressProblem::bond_load for every bond (stress.hpp:132; stress.cpp:477-502), from u (host,
+   6 per node) or, when u is null, from the device-resident solution of the last solve. */
+int svxg_problem_bond_loads(svxg_problem* p, const double* u, svxg_bond_load* out);
+/* bond_utilization of every bond (stress.hpp:57,66; stress.cpp:14-57) after its bond_load:
+   phi (1 = at strength) and the FailMode (0 None, 1 Tension, 2 Crush, 3 Shear). u as above. */
+int svxg_problem_bond_checks(svxg_problem* p, const double* u, double fragility, double* phi, uint8_t* mode);
+
+/* StressProblem::add_force (stress.hpp:134; stress.cpp:504-513), on a host load vector. */
+void svxg_add_force(double* f, int32_t node, const double c[3], const double F[3], const double at[3]);
+
+/* Problem facts: free nodes, K blocks, levels, per-level sizes and blocks (up to 16 each),
+   StressProblem::work_per_iteration / assembly_work (stress.hpp:95,119), device bytes. */
+typedef struct svxg_problem_info {
+  int32_t nodes, bonds, nfree, levels;
+  int32_t block_jacobi, assembled, preconditioner_current, pad_;
+  int64_t blocks;
+  int64_t work_per_iteration, assembly_work;
+  int64_t device_bytes;
+  int32_t level_rows[16];
+  int64_t level_blocks[16];
+  int64_t level_pblocks[16];
+} svxg_problem_info;
+int svxg_problem_info_get(const svxg_problem* p, svxg_problem_info* out);
+
+/* The graph as the library holds it (broken / in_k flags move with remove_bonds / retire_nodes). */
+int svxg_problem_get_bonds(const svxg_problem* p, svxg_bond* out);
+int svxg_problem_get_nodes(const svxg_problem* p, svxg_node* out);
+
+/* ---- batches ------------------------------------------------------------------------------ */
+
+/* Solves `count` assembled problems in one set of device launches (a batch of the tick's
+   problems; the reference runs them one by one, world_pieces.cpp:1452-1463, or in turn,
+   world.cpp:986-1129). f[i], u[i]: host vectors of problem i (6 per node); a null f[i] reuses the
+   problem's resident load. Each problem's answer is the same alone and in a batch (bitwise:
+   every reduction runs in a fixed order over the problem's own rows). */
+int svxg_solve_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_t count, const double* const* f,
+                     double* const* u, double rtol, int32_t maxit, int32_t warm, svxg_pcg_result* results);
+
+/* ---- device-resident views (tests and benchmarks) ----------------------------------------- */
+
+/* K as assembled, in the reference's Bsr6 layout (amg.hpp:20-27): rowptr[nfree+1], col[blocks],
+   val[36 blocks] row-major 6x6, diagonal block first in each row. Null pointers: skipped. */
+int svxg_problem_get_matrix(const svxg_problem* p, int32_t* rowptr, int32_t* col, double* val);
+/* y = K x on the device (Bsr6::apply, amg.cpp:174-183) over free DOFs (6 nfree each), FP64 or,
+   with fp32 != 0, through the FP32 copy of K. */
+int svxg_problem_apply_matrix(svxg_problem* p, const double* x, double* y, int32_t fp32);
+/* z = M^-1 r: one application of the preconditioner (Amg::apply, amg.cpp:718-731; block-Jacobi,
+   stress.cpp:312-324) over free DOFs, in the options' precision. */
+int svxg_problem_apply_preconditioner(svxg_problem* p, const double* r, double* z);
+/* Milliseconds of device time of the last solve's solve phase (events on the problem's stream),
+   and its count of outer K products. */
+int svxg_problem_last_timing(const svxg_problem* p, double* solve_ms, double* setup_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVXG_H */

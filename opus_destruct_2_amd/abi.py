@@ -1,0 +1,114 @@
+"""numpy / ctypes mirrors of the plain structs in include/svxg.h.
+
+The graph is handed to the C ABI as numpy structured arrays whose layout is the C struct's
+(svxg_node 40 B, svxg_bond 216 B, svxg_material 64 B), so a problem is built without a copy on the
+Python side. Field names follow the reference's SNode / SBond / Material
+(core/include/svx/stress/stress.hpp:21-51, core/include/svx/material/material.hpp:47-57).
+"""
+import ctypes
+
+import numpy as np
+
+NODE = np.dtype([
+    ("c", "<f8", (3,)), ("mass", "<f8"), ("fixed", "u1"), ("gone", "u1"), ("pad_", "u1", (6,)),
+], align=True)
+
+BOND = np.dtype([
+    ("a", "<i4"), ("b", "<i4"),
+    ("broken", "u1"), ("in_k", "u1"), ("ma", "u1"), ("mb", "u1"), ("sectioned", "u1"), ("pad0_", "u1", (3,)),
+    ("n", "<f8", (3,)), ("t1", "<f8", (3,)), ("t2", "<f8", (3,)),
+    ("area", "<f8"), ("p", "<f8", (3,)),
+    ("s1", "<f8"), ("s2", "<f8"), ("c1", "<f8"), ("c2", "<f8"), ("rmax", "<f8"),
+    ("la", "<f8"), ("lb", "<f8"), ("strength", "<f8"),
+    ("ft", "<f4"), ("fb", "<f4"), ("fc", "<f4"), ("coh", "<f4"), ("mu", "<f4"),
+    ("faces", "<i4"), ("tag", "<i4"), ("pad1_", "<i4"),
+], align=True)
+
+MATERIAL = np.dtype([
+    ("E", "<f8"), ("G", "<f8"), ("rho", "<f8"),
+    ("ft", "<f8"), ("fb", "<f8"), ("fc", "<f8"), ("cohesion", "<f8"), ("friction", "<f8"),
+], align=True)
+
+BOND_LOAD = np.dtype([("N", "<f8"), ("V1", "<f8"), ("V2", "<f8"), ("T", "<f8"), ("M1", "<f8"), ("M2", "<f8")])
+
+assert NODE.itemsize == 40 and BOND.itemsize == 216 and MATERIAL.itemsize == 64
+
+SVXG_OK, SVXG_BAD_INPUT, SVXG_OOM_BUDGET, SVXG_DEVICE_ERROR, SVXG_NOT_ASSEMBLED, SVXG_SINGULAR = range(6)
+SVXG_FP64, SVXG_MIXED = 0, 1
+
+
+class AmgOptions(ctypes.Structure):
+    """AmgOptions (core/include/svx/solve/amg.hpp:46-69)."""
+    _fields_ = [
+        ("strength", ctypes.c_double), ("coarse_scale", ctypes.c_double),
+        ("coarse_max", ctypes.c_int32), ("smoothed", ctypes.c_int32), ("max_levels", ctypes.c_int32),
+        ("sweeps", ctypes.c_int32), ("cheb_degree", ctypes.c_int32), ("coarsen_dense", ctypes.c_int32),
+        ("reaggregate", ctypes.c_int32), ("pad_", ctypes.c_int32),
+    ]
+
+
+class StressOptions(ctypes.Structure):
+    """StressOptions (core/include/svx/stress/stress.hpp:68-72) plus the solve's precision."""
+    _fields_ = [("rtol", ctypes.c_double), ("amg_min_nodes", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("amg", AmgOptions)]
+
+
+class PcgResult(ctypes.Structure):
+    """PcgResult (core/include/svx/solve/amg.hpp:120-125)."""
+    _fields_ = [("iters", ctypes.c_int32), ("converged", ctypes.c_int32), ("breakdown", ctypes.c_int32),
+                ("refinements", ctypes.c_int32), ("rel_res", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"iters": self.iters, "converged": bool(self.converged), "breakdown": bool(self.breakdown),
+                "refinements": self.refinements, "rel_res": self.rel_res}
+
+
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [
+        ("nodes", ctypes.c_int32), ("bonds", ctypes.c_int32), ("nfree", ctypes.c_int32), ("levels", ctypes.c_int32),
+        ("block_jacobi", ctypes.c_int32), ("assembled", ctypes.c_int32), ("preconditioner_current", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("blocks", ctypes.c_int64), ("work_per_iteration", ctypes.c_int64), ("assembly_work", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("level_rows", ctypes.c_int32 * 16), ("level_blocks", ctypes.c_int64 * 16), ("level_pblocks", ctypes.c_int64 * 16),
+    ]
+
+    def as_dict(self):
+        L = self.levels
+        return {
+            "nodes": self.nodes, "bonds": self.bonds, "nfree": self.nfree, "levels": L,
+            "block_jacobi": bool(self.block_jacobi), "assembled": bool(self.assembled),
+            "preconditioner_current": bool(self.preconditioner_current), "blocks": self.blocks,
+            "work_per_iteration": self.work_per_iteration, "assembly_work": self.assembly_work,
+            "device_bytes": self.device_bytes,
+            "level_rows": list(self.level_rows[:L]), "level_blocks": list(self.level_blocks[:L]),
+            "level_pblocks": list(self.level_pblocks[:L]),
+        }
+
+
+def ptr(a):
+    """The data pointer of a numpy array (None for None)."""
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def reference_options(**kw):
+    """StressOptions{} with AmgOptions{} as the reference defaults them."""
+    o = StressOptions()
+    o.rtol = 2e-3
+    o.amg_min_nodes = 160
+    o.precision = SVXG_FP64
+    o.amg.strength = 0.1
+    o.amg.coarse_scale = 0.6
+    o.amg.coarse_max = 48
+    o.amg.smoothed = 1
+    o.amg.max_levels = 16
+    o.amg.sweeps = 1
+    o.amg.cheb_degree = 2
+    o.amg.coarsen_dense = 1
+    o.amg.reaggregate = 1
+    for k, v in kw.items():
+        if hasattr(o.amg, k) and k not in ("rtol",):
+            setattr(o.amg, k, v)
+        else:
+            setattr(o, k, v)
+    return o

@@ -1,0 +1,225 @@
+"""The checker: ctypes access to the CPU restatement (liboracle.so) and to the reference's own
+stress path compiled from its sources (_ref/libsvxref.so).
+
+TEST INFRASTRUCTURE ONLY. Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg; never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from opus_destruct_2_amd.abi import BOND, BOND_LOAD, NODE, PcgResult, ProblemInfo, StressOptions, reference_options
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libsvxref.so")
+
+
+def build(ref=True):
+    """Compile liboracle.so (and _ref/libsvxref.so when the reference sources are present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"] + (["ref"] if ref else []), check=True)
+
+
+def _bind(path, pre):
+    L = ctypes.CDLL(path)
+    vp, i32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    sig = {
+        "create": ([vp, i32, vp, i32, vp, i32], vp),
+        "destroy": ([vp], None),
+        "assemble": ([vp, vp], ctypes.c_int),
+        "release": ([vp], None),
+        "solve": ([vp, vp, vp, f64, i32, i32, vp], ctypes.c_int),
+        "begin": ([vp, vp, vp], None),
+        "iterate": ([vp, i32, f64, vp], None),
+        "current": ([vp, vp], None),
+        "remove_bond": ([vp, i32], None),
+        "retire_nodes": ([vp, vp, i32], None),
+        "set_bond_broken": ([vp, i32, i32], None),
+        "bond_load": ([vp, i32, vp, vp], None),
+        "bond_utilization": ([vp, vp, f64, vp, i32, vp], f64),
+        "bond_strength": ([vp, f64, vp, i32, vp], None),
+        "info": ([vp, vp], None),
+        "get_bonds": ([vp, vp], None),
+        "set_threads": ([i32], None),
+    }
+    if pre == "orc_":
+        sig.update({"get_matrix": ([vp, vp, vp, vp], None), "apply_matrix": ([vp, vp, vp], None),
+                    "apply_preconditioner": ([vp, vp, vp], None), "get_nodes": ([vp, vp], None)})
+    for name, (args, res) in sig.items():
+        fn = getattr(L, pre + name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+_LIBS = {}
+
+
+def library(kind="oracle"):
+    """kind: 'oracle' (the restatement) or 'ref' (the reference's own sources)."""
+    if kind not in _LIBS:
+        path, pre = (ORACLE_SO, "orc_") if kind == "oracle" else (REF_SO, "ref_")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        _LIBS[kind] = (_bind(path, pre), pre)
+    return _LIBS[kind]
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+class CpuProblem:
+    """A StressProblem (core/include/svx/stress/stress.hpp:74-160) on the CPU: the restatement or
+    the reference itself, through the same entry points."""
+
+    def __init__(self, nodes, bonds, mats, kind="oracle"):
+        self.L, self.pre = library(kind)
+        self.kind = kind
+        self.nodes = np.ascontiguousarray(nodes, NODE)
+        self.bonds = np.ascontiguousarray(bonds, BOND)
+        self.mats = np.ascontiguousarray(mats)
+        self.h = self._f("create")(self.nodes.ctypes.data, len(self.nodes), self.bonds.ctypes.data, len(self.bonds),
+                                   self.mats.ctypes.data, len(self.mats))
+
+    def _f(self, name):
+        return getattr(self.L, self.pre + name)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._f("destroy")(h)
+            self.h = None
+
+    def assemble(self, opt=None):
+        o = opt if opt is not None else reference_options()
+        return bool(self._f("assemble")(self.h, ctypes.byref(o)))
+
+    def release(self):
+        self._f("release")(self.h)
+
+    def solve(self, f, u=None, rtol=1e-8, maxit=-1, warm=False):
+        n6 = 6 * len(self.nodes)
+        f = np.ascontiguousarray(f, np.float64)
+        u = np.zeros(n6) if u is None else np.array(u, np.float64)
+        r = PcgResult()
+        self._f("solve")(self.h, f.ctypes.data, u.ctypes.data, rtol, maxit, int(warm), ctypes.byref(r))
+        return u, r.as_dict()
+
+    def begin(self, f, u0=None):
+        f = np.ascontiguousarray(f, np.float64)
+        u0 = None if u0 is None else np.ascontiguousarray(u0, np.float64)
+        self._f("begin")(self.h, f.ctypes.data, None if u0 is None else u0.ctypes.data)
+
+    def iterate(self, maxit, rtol):
+        r = PcgResult()
+        self._f("iterate")(self.h, maxit, rtol, ctypes.byref(r))
+        return r.as_dict()
+
+    def current(self):
+        u = np.zeros(6 * len(self.nodes))
+        self._f("current")(self.h, u.ctypes.data)
+        return u
+
+    def remove_bond(self, b):
+        self._f("remove_bond")(self.h, int(b))
+
+    def retire_nodes(self, ids):
+        ids = np.ascontiguousarray(ids, np.int32)
+        self._f("retire_nodes")(self.h, ids.ctypes.data, len(ids))
+
+    def set_bond_broken(self, b, broken):
+        self._f("set_bond_broken")(self.h, int(b), int(bool(broken)))
+
+    def bond_load(self, b, u):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros(1, BOND_LOAD)
+        self._f("bond_load")(self.h, int(b), u.ctypes.data, out.ctypes.data)
+        return out[0]
+
+    def bond_loads(self, u):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros(len(self.bonds), BOND_LOAD)
+        fn = self._f("bond_load")
+        for b in range(len(self.bonds)):
+            fn(self.h, b, u.ctypes.data, out[b:b + 1].ctypes.data)
+        return out
+
+    def bond_checks(self, u, fragility=1.0):
+        """phi and FailMode of every bond (bond_utilization after bond_load)."""
+        loads = self.bond_loads(u)
+        bonds = self.get_bonds()
+        phi = np.zeros(len(bonds))
+        mode = np.zeros(len(bonds), np.uint8)
+        fn = self._f("bond_utilization")
+        md = ctypes.c_uint8()
+        for b in range(len(bonds)):
+            phi[b] = fn(bonds[b:b + 1].ctypes.data, loads[b:b + 1].ctypes.data, fragility, self.mats.ctypes.data,
+                        len(self.mats), ctypes.byref(md))
+            mode[b] = md.value
+        return phi, mode
+
+    def info(self):
+        i = ProblemInfo()
+        self._f("info")(self.h, ctypes.byref(i))
+        return i.as_dict()
+
+    def get_bonds(self):
+        out = self.bonds.copy()
+        self._f("get_bonds")(self.h, out.ctypes.data)
+        return out
+
+    def matrix(self):
+        """(rowptr, col, val[nb, 6, 6]) of K (the restatement only)."""
+        inf = self.info()
+        rowptr = np.zeros(inf["nfree"] + 1, np.int32)
+        col = np.zeros(inf["blocks"], np.int32)
+        val = np.zeros((inf["blocks"], 6, 6))
+        self._f("get_matrix")(self.h, rowptr.ctypes.data, col.ctypes.data, val.ctypes.data)
+        return rowptr, col, val
+
+    def apply_matrix(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        self._f("apply_matrix")(self.h, x.ctypes.data, y.ctypes.data)
+        return y
+
+    def apply_preconditioner(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        self._f("apply_preconditioner")(self.h, r.ctypes.data, z.ctypes.data)
+        return z
+
+
+def set_ref_threads(n):
+    L, pre = library("ref")
+    L.ref_set_threads(int(n))
+
+
+def bond_utilization(bond, load, fragility, mats, kind="oracle"):
+    """bond_utilization (core/src/stress/stress.cpp:23-57) of one bond: (phi, mode)."""
+    L, pre = library(kind)
+    b = np.ascontiguousarray(np.atleast_1d(bond), BOND)
+    ld = np.zeros(1, BOND_LOAD)
+    for k in ("N", "V1", "V2", "T", "M1", "M2"):
+        ld[k] = load.get(k, 0.0) if isinstance(load, dict) else load[k]
+    mats = np.ascontiguousarray(mats)
+    md = ctypes.c_uint8()
+    phi = getattr(L, pre + "bond_utilization")(b.ctypes.data, ld.ctypes.data, fragility, mats.ctypes.data, len(mats),
+                                               ctypes.byref(md))
+    return phi, md.value
+
+
+def bond_strength(bond, fragility, mats, kind="oracle"):
+    L, pre = library(kind)
+    b = np.ascontiguousarray(np.atleast_1d(bond), BOND)
+    mats = np.ascontiguousarray(mats)
+    out = np.zeros(5)
+    getattr(L, pre + "bond_strength")(b.ctypes.data, fragility, mats.ctypes.data, len(mats), out.ctypes.data)
+    return out
+
+
+__all__ = ["CpuProblem", "build", "library", "ref_available", "bond_utilization", "bond_strength",
+           "reference_options", "StressOptions", "set_ref_threads"]

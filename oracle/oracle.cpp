@@ -1,0 +1,1202 @@
+// oracle — CPU restatement of the reference's fragment-graph stress solve.
+//
+// TEST INFRASTRUCTURE ONLY: the checker of the device library (tests/, __graft_entry__.smoke(),
+// bench.py's cpu_baseline leg). Never loaded by the product path.
+//
+// It restates the reference's algorithm step for step, in the same floating-point operation order,
+// so that on the same input it produces the same bits as the reference compiled with
+// -ffp-contract=off -fno-fast-math (CMakeLists.txt:16-17). Each part cites what it follows:
+//   block helpers            core/src/solve/amg.cpp:32-172 (inv6, fro6, rigid_block, atbd_add,
+//                            mul6 / mul6_add row sums, the FP32 bsub / badd lane sums)
+//   Bsr6 / Bsr6Builder       amg.cpp:174-224, amg.hpp:18-44
+//   Amg::build / coarsen     amg.cpp:228-533 (finalize, estimate_lmax, aggregation, P, RAP)
+//   coarse Cholesky          amg.cpp:535-592
+//   FP32 copies and V-cycle  amg.cpp:594-731 (make_fast, smooth_f, cycle_f, apply)
+//   StressProblem            core/src/stress/stress.cpp:59-475 (bond_matrices, build_matrix,
+//                            assemble, remove_bond, retire_nodes, precondition, begin, iterate)
+//   bond_load / checks       stress.cpp:14-57, 477-502
+//   ordered chunk sums       core/src/base/parallel.cpp:208-227 (dots in chunks of 4096 entries)
+//
+// Pinning (tests/test_oracle_*.py): every function here is compared bit for bit with the
+// reference compiled from its own sources (oracle/_ref) on the reference's test problems and the
+// synthetic configs, through committed fixtures written by tests/golden/make_golden.py, and
+// against the closed forms of tests/core/test_stress.cpp (cantilever statics, chain tension,
+// support reactions, the failure-mode known answers) and brute-force dense solves.
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+namespace {
+
+using f32 = float;
+using f64 = double;
+using i32 = int32_t;
+using i64 = int64_t;
+
+struct V3 {
+  f64 x = 0, y = 0, z = 0;
+};
+inline V3 v3(const double* p) { return V3{p[0], p[1], p[2]}; }
+inline V3 vsub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline f64 vdot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 vcross(const V3& a, const V3& b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline f64 vget(const V3& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+constexpr i64 kGrain = 512;  // rows per chunk (amg.hpp:29-31)
+
+// ---- 6x6 block arithmetic (row-major f64) --------------------------------------------------
+
+bool inv6(const f64* A, f64* out) {
+  f64 W[6][12];
+  f64 big = 0.0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 12; ++c) {
+      W[r][c] = c < 6 ? A[r * 6 + c] : (c - 6 == r ? 1.0 : 0.0);
+      if (c < 6) big = std::max(big, std::abs(W[r][c]));
+    }
+  if (big == 0.0) return false;
+  for (int c = 0; c < 6; ++c) {
+    int best = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (std::abs(W[r][c]) > std::abs(W[best][c])) best = r;
+    if (std::abs(W[best][c]) <= 1e-15 * big) return false;
+    if (best != c)
+      for (int k = 0; k < 12; ++k) std::swap(W[c][k], W[best][k]);
+    const f64 d = 1.0 / W[c][c];
+    for (int k = 0; k < 12; ++k) W[c][k] *= d;
+    for (int r = 0; r < 6; ++r) {
+      if (r == c) continue;
+      const f64 m = W[r][c];
+      if (m == 0.0) continue;
+      for (int k = 0; k < 12; ++k) W[r][k] -= m * W[c][k];
+    }
+  }
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) out[r * 6 + c] = W[r][6 + c];
+  return true;
+}
+
+f64 fro6(const f64* M) {
+  f64 s = 0.0;
+  for (int i = 0; i < 36; ++i) s += M[i] * M[i];
+  return std::sqrt(s);
+}
+
+// the map of a rigid motion (U, Theta) about X to a point at X + r: [[I, -[r]x], [0, I]]
+void rigid(const V3& r, f64* P) {
+  for (int i = 0; i < 36; ++i) P[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  P[4] = r.z;
+  P[5] = -r.y;
+  P[9] = -r.z;
+  P[11] = r.x;
+  P[15] = r.y;
+  P[16] = -r.x;
+}
+
+// C += s * A^T (B D)
+void atbd_add(const f64* A, const f64* B, const f64* D, f64 s, f64* C) {
+  f64 T[36];
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      f64 acc = 0.0;
+      for (int k = 0; k < 6; ++k) acc += B[r * 6 + k] * D[k * 6 + c];
+      T[r * 6 + c] = acc;
+    }
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      f64 acc = 0.0;
+      for (int k = 0; k < 6; ++k) acc += A[k * 6 + r] * T[k * 6 + c];
+      C[r * 6 + c] += s * acc;
+    }
+}
+
+// y = M x (each row summed left to right)
+void mv6(const f64* M, const f64* x, f64* y) {
+  for (int r = 0; r < 6; ++r) {
+    const f64* m = M + 6 * r;
+    y[r] = m[0] * x[0] + m[1] * x[1] + m[2] * x[2] + m[3] * x[3] + m[4] * x[4] + m[5] * x[5];
+  }
+}
+// y += (M x): the row's sum first, then added
+void mv6_add(const f64* M, const f64* x, f64* y) {
+  for (int r = 0; r < 6; ++r) {
+    const f64* m = M + 6 * r;
+    y[r] += m[0] * x[0] + m[1] * x[1] + m[2] * x[2] + m[3] * x[3] + m[4] * x[4] + m[5] * x[5];
+  }
+}
+// C = A B, every element summed over q from zero
+void mul6(const f64* A, const f64* B, f64* C) {
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      f64 s = 0.0;
+      for (int q = 0; q < 6; ++q) s += A[r * 6 + q] * B[q * 6 + c];
+      C[r * 6 + c] = s;
+    }
+}
+// C += (A B) or C += (A^T B)
+void mul6_add(const f64* A, const f64* B, f64* C, bool trans) {
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      f64 s = 0.0;
+      for (int q = 0; q < 6; ++q) s += (trans ? A[q * 6 + r] : A[r * 6 + q]) * B[q * 6 + c];
+      C[r * 6 + c] = C[r * 6 + c] + s;
+    }
+}
+// FP32 lane sums: y -= B x or y += B x accumulated column by column into y itself
+inline void fsub(const f32* B, const f32* x, f32* y) {
+  for (int j = 0; j < 6; ++j)
+    for (int r = 0; r < 6; ++r) y[r] = y[r] - B[r * 6 + j] * x[j];
+}
+inline void fadd(const f32* B, const f32* x, f32* y) {
+  for (int j = 0; j < 6; ++j)
+    for (int r = 0; r < 6; ++r) y[r] = y[r] + B[r * 6 + j] * x[j];
+}
+
+// ---- block-sparse matrices -------------------------------------------------------------------
+
+struct Bsr {
+  i32 n = 0;
+  std::vector<i32> rowptr, col;
+  std::vector<f64> val;
+  i64 blocks() const { return static_cast<i64>(col.size()); }
+  void apply(const f64* x, f64* y) const {
+    for (i32 i = 0; i < n; ++i) {
+      f64* yi = y + 6 * size_t(i);
+      for (int q = 0; q < 6; ++q) yi[q] = 0.0;
+      for (i32 k = rowptr[size_t(i)]; k < rowptr[size_t(i) + 1]; ++k) mv6_add(&val[36 * size_t(k)], x + 6 * size_t(col[size_t(k)]), yi);
+    }
+  }
+  const f64* diag(i32 i) const { return &val[36 * size_t(rowptr[size_t(i)])]; }
+  f64* diag(i32 i) { return &val[36 * size_t(rowptr[size_t(i)])]; }
+  f64* find(i32 r, i32 c) {
+    for (i32 k = rowptr[size_t(r)]; k < rowptr[size_t(r) + 1]; ++k)
+      if (col[size_t(k)] == c) return &val[36 * size_t(k)];
+    return nullptr;
+  }
+};
+
+// rows of (column, storage slot); blocks zero on first touch; finish(): diagonal first, then
+// ascending columns (a missing diagonal is added as a zero block)
+class Builder {
+ public:
+  explicit Builder(i32 n) : n_(n), rows_(size_t(n)) {}
+  f64* at(i32 r, i32 c) {
+    for (const auto& e : rows_[size_t(r)])
+      if (e.first == c) return &store_[36 * size_t(e.second)];
+    const i32 s = static_cast<i32>(store_.size() / 36);
+    store_.resize(store_.size() + 36, 0.0);
+    rows_[size_t(r)].push_back({c, s});
+    return &store_[36 * size_t(s)];
+  }
+  Bsr finish() {
+    Bsr A;
+    A.n = n_;
+    A.rowptr.assign(size_t(n_) + 1, 0);
+    for (i32 r = 0; r < n_; ++r) {
+      auto& row = rows_[size_t(r)];
+      std::sort(row.begin(), row.end(), [r](const std::pair<i32, i32>& a, const std::pair<i32, i32>& b) {
+        if ((a.first == r) != (b.first == r)) return a.first == r;
+        return a.first < b.first;
+      });
+      if (row.empty() || row.front().first != r) row.insert(row.begin(), {r, -1});
+      A.rowptr[size_t(r) + 1] = A.rowptr[size_t(r)] + static_cast<i32>(row.size());
+    }
+    const size_t nb = size_t(A.rowptr[size_t(n_)]);
+    A.col.resize(nb);
+    A.val.assign(36 * nb, 0.0);
+    size_t k = 0;
+    for (i32 r = 0; r < n_; ++r)
+      for (const auto& e : rows_[size_t(r)]) {
+        A.col[k] = e.first;
+        if (e.second >= 0) std::memcpy(&A.val[36 * k], &store_[36 * size_t(e.second)], sizeof(f64) * 36);
+        ++k;
+      }
+    return A;
+  }
+
+ private:
+  i32 n_;
+  std::vector<std::vector<std::pair<i32, i32>>> rows_;
+  std::vector<f64> store_;
+};
+
+// ---- smoothed-aggregation multigrid ------------------------------------------------------------
+
+struct AmgOpt {
+  i32 coarse_max = 48;
+  f64 strength = 0.1;
+  bool smoothed = true;
+  f64 coarse_scale = 0.6;
+  int max_levels = 16;
+  int sweeps = 1;
+  bool coarsen_dense = true;
+  bool reaggregate = true;
+};
+
+struct Level {
+  Bsr A;
+  std::vector<f64> Dinv;
+  std::vector<V3> pos, off;
+  std::vector<i32> agg;
+  std::vector<i32> Prow, Pcol;
+  std::vector<f64> Pval;
+  std::vector<i32> Rrow;
+  std::vector<std::pair<i32, i32>> Rent;
+  // FP32 copies (row-major 36-float blocks; the arithmetic order is the lane order of the FP32
+  // sweeps, whatever the storage)
+  std::vector<f32> Af, Df, Pf, Rf;
+  mutable std::vector<f32> xf, bf, rf, of, cf;
+};
+
+class Multigrid {
+ public:
+  bool build(const Bsr& A, const std::vector<V3>& pos, const AmgOpt& o) {
+    opt_ = o;
+    lv_.clear();
+    chol_.clear();
+    chol_n_ = 0;
+    work_ = 0;
+    build_work_ = 0;
+    if (A.n == 0) return true;
+    lv_.emplace_back();
+    lv_[0].A = A;
+    lv_[0].pos = pos;
+    if (!finalize(lv_[0])) {
+      lv_.clear();
+      return false;
+    }
+    while (static_cast<int>(lv_.size()) < opt_.max_levels && lv_.back().A.n > opt_.coarse_max) {
+      const Bsr& L = lv_.back().A;
+      const bool dense = lv_.size() > 1 && L.blocks() > 80 * static_cast<i64>(L.n);
+      if (dense && !opt_.coarsen_dense) break;
+      if (!coarsen(lv_.size() - 1, !dense && opt_.smoothed)) break;
+      if (dense) break;
+    }
+    factor();
+    to_single();
+    for (size_t l = 0; l < lv_.size(); ++l) {
+      const bool last = l + 1 == lv_.size();
+      const i64 nd = chol_n_ / 6;
+      const i64 coarsest = chol_n_ > 0 ? std::max<i64>(lv_[l].A.blocks(), 2 * nd * nd) : 8 * 2 * lv_[l].A.blocks();
+      work_ += last ? coarsest : lv_[l].A.blocks() * (4 * opt_.sweeps + 2);
+    }
+    return true;
+  }
+  bool built() const { return !lv_.empty(); }
+  i64 work_per_apply() const { return work_; }
+  i64 build_work() const { return build_work_; }
+  size_t levels() const { return lv_.size(); }
+  const Level& level(size_t l) const { return lv_[l]; }
+
+  void apply(const f64* r, f64* z) const {
+    if (lv_.empty()) return;
+    const Level& L = lv_[0];
+    const i32 n = L.A.n;
+    for (size_t k = 0; k < 6 * size_t(n); ++k) L.bf[k] = static_cast<f32>(r[k]);
+    cycle(0, L.bf.data(), L.xf.data());
+    for (size_t k = 0; k < 6 * size_t(n); ++k) z[k] = L.xf[k];
+  }
+
+ private:
+  bool finalize(Level& L) {
+    const i32 n = L.A.n;
+    L.Dinv.assign(36 * size_t(n), 0.0);
+    for (i32 i = 0; i < n; ++i) {
+      const f64* D = L.A.diag(i);
+      if (!inv6(D, &L.Dinv[36 * size_t(i)])) {
+        f64 R[36];
+        std::memcpy(R, D, sizeof R);
+        f64 tr = 0.0;
+        for (int q = 0; q < 6; ++q) tr += std::abs(R[q * 7]);
+        const f64 eps = tr > 0 ? 1e-9 * tr : 1.0;
+        for (int q = 0; q < 6; ++q) R[q * 7] += eps;
+        if (!inv6(R, &L.Dinv[36 * size_t(i)])) return false;
+      }
+    }
+    build_work_ += 12 * static_cast<i64>(n);
+    return true;
+  }
+
+  f64 lmax(const Level& L) {
+    const i32 n = L.A.n;
+    build_work_ += 15 * (L.A.blocks() + n);
+    const size_t m = 6 * size_t(n);
+    std::vector<f64> v(m), w(m);
+    for (size_t k = 0; k < m; ++k) v[k] = 1.0 + 0.001 * static_cast<f64>(k % 7);
+    f64 lam = 1.0;
+    for (int it = 0; it < 15; ++it) {
+      L.A.apply(v.data(), w.data());
+      for (i32 i = 0; i < n; ++i) mv6(&L.Dinv[36 * size_t(i)], &w[6 * size_t(i)], &v[6 * size_t(i)]);
+      f64 s = 0.0;
+      for (f64 q : v) s += q * q;
+      s = std::sqrt(s);
+      if (!(s > 0)) break;
+      lam = s;
+      for (f64& q : v) q /= s;
+    }
+    return 1.1 * lam;
+  }
+
+  bool coarsen(size_t l, bool smoothed) {
+    Level& F = lv_[l];
+    const Bsr& A = F.A;
+    const i32 n = A.n;
+    std::vector<f64> dn(static_cast<size_t>(n));
+    for (i32 i = 0; i < n; ++i) dn[size_t(i)] = fro6(A.diag(i));
+    f64 theta = opt_.strength;
+    auto strong = [&](i32 i, i32 k) {
+      const i32 j = A.col[size_t(k)];
+      return j != i && fro6(&A.val[36 * size_t(k)]) >= theta * std::sqrt(dn[size_t(i)] * dn[size_t(j)]);
+    };
+    auto lonely = [&](i32 i) { return A.rowptr[size_t(i) + 1] - A.rowptr[size_t(i)] <= 1; };
+    std::vector<i32> agg(size_t(n), -1);
+    i32 na = 0;
+    for (int round = 0;; ++round) {
+      std::fill(agg.begin(), agg.end(), -1);
+      na = 0;
+      // roots: every strong neighbour still unaggregated
+      for (i32 i = 0; i < n; ++i) {
+        if (agg[size_t(i)] >= 0 || lonely(i)) continue;
+        bool ok = true;
+        for (i32 k = A.rowptr[size_t(i)]; k < A.rowptr[size_t(i) + 1] && ok; ++k)
+          if (strong(i, k) && agg[size_t(A.col[size_t(k)])] >= 0) ok = false;
+        if (!ok) continue;
+        agg[size_t(i)] = na;
+        for (i32 k = A.rowptr[size_t(i)]; k < A.rowptr[size_t(i) + 1]; ++k)
+          if (strong(i, k)) agg[size_t(A.col[size_t(k)])] = na;
+        ++na;
+      }
+      // the rest join the aggregate of their strongest aggregated neighbour (as of the roots)
+      const std::vector<i32> snap = agg;
+      for (i32 i = 0; i < n; ++i) {
+        if (snap[size_t(i)] >= 0) continue;
+        f64 best = -1.0;
+        i32 pick = -1;
+        for (i32 k = A.rowptr[size_t(i)]; k < A.rowptr[size_t(i) + 1]; ++k) {
+          const i32 j = A.col[size_t(k)];
+          if (j == i || snap[size_t(j)] < 0) continue;
+          const f64 s = fro6(&A.val[36 * size_t(k)]);
+          if (s > best) {
+            best = s;
+            pick = snap[size_t(j)];
+          }
+        }
+        agg[size_t(i)] = pick;
+      }
+      // leftovers: coupled ones alone, uncoupled ones eight to an aggregate
+      i32 open = -1, fill = 0;
+      for (i32 i = 0; i < n; ++i) {
+        if (agg[size_t(i)] >= 0) continue;
+        if (!lonely(i)) {
+          agg[size_t(i)] = na++;
+          continue;
+        }
+        if (open < 0 || fill >= 8) {
+          open = na++;
+          fill = 0;
+        }
+        agg[size_t(i)] = open;
+        ++fill;
+      }
+      if (!opt_.reaggregate || round == 3 || 2 * static_cast<i64>(na) <= n) break;
+      theta *= 0.5;
+    }
+    if (na >= n || static_cast<f64>(na) > 0.85 * n) return false;
+
+    Level C;
+    C.pos.assign(size_t(na), V3{});
+    std::vector<i32> cnt(size_t(na), 0);
+    for (i32 i = 0; i < n; ++i) {
+      V3& q = C.pos[size_t(agg[size_t(i)])];
+      q.x += F.pos[size_t(i)].x;
+      q.y += F.pos[size_t(i)].y;
+      q.z += F.pos[size_t(i)].z;
+      ++cnt[size_t(agg[size_t(i)])];
+    }
+    for (i32 a = 0; a < na; ++a) {
+      const f64 s = 1.0 / cnt[size_t(a)];
+      C.pos[size_t(a)].x *= s;
+      C.pos[size_t(a)].y *= s;
+      C.pos[size_t(a)].z *= s;
+    }
+    F.agg = agg;
+    F.off.resize(size_t(n));
+    for (i32 i = 0; i < n; ++i) F.off[size_t(i)] = vsub(F.pos[size_t(i)], C.pos[size_t(agg[size_t(i)])]);
+
+    // prolongation: each aggregate's rigid motion, smoothed by one damped block-Jacobi step
+    using Entry = std::pair<i32, std::array<f64, 36>>;
+    {
+      const f64 omega = smoothed ? (4.0 / 3.0) / std::max(1e-300, lmax(F)) : 0.0;
+      std::vector<std::vector<Entry>> rows(static_cast<size_t>(n));
+      std::vector<i32> slot(size_t(na), -1);
+      for (i32 i = 0; i < n; ++i) {
+        auto& row = rows[size_t(i)];
+        auto add = [&](i32 c, const f64* blk, f64 sc) {
+          i32& at = slot[size_t(c)];
+          if (at < 0) {
+            at = static_cast<i32>(row.size());
+            row.push_back({c, {}});
+          }
+          auto& b = row[size_t(at)].second;
+          for (int q = 0; q < 36; ++q) b[size_t(q)] += sc * blk[q];
+        };
+        f64 Pi[36];
+        rigid(F.off[size_t(i)], Pi);
+        add(agg[size_t(i)], Pi, 1.0);
+        const f64* Di = &F.Dinv[36 * size_t(i)];
+        const i32 kend = omega != 0.0 ? A.rowptr[size_t(i) + 1] : A.rowptr[size_t(i)];
+        for (i32 k = A.rowptr[size_t(i)]; k < kend; ++k) {
+          const i32 j = A.col[size_t(k)];
+          f64 Pj[36], T[36], U[36];
+          rigid(F.off[size_t(j)], Pj);
+          mul6(&A.val[36 * size_t(k)], Pj, T);
+          mul6(Di, T, U);
+          add(agg[size_t(j)], U, -omega);
+        }
+        for (const auto& e : row) slot[size_t(e.first)] = -1;
+      }
+      F.Prow.assign(size_t(n) + 1, 0);
+      F.Pcol.clear();
+      F.Pval.clear();
+      for (i32 i = 0; i < n; ++i) {
+        auto& row = rows[size_t(i)];
+        std::sort(row.begin(), row.end(), [](const Entry& a, const Entry& b) { return a.first < b.first; });
+        for (const auto& e : row) {
+          F.Pcol.push_back(e.first);
+          F.Pval.insert(F.Pval.end(), e.second.begin(), e.second.end());
+        }
+        F.Prow[size_t(i) + 1] = static_cast<i32>(F.Pcol.size());
+      }
+    }
+    if (smoothed) build_work_ += 12 * A.blocks();
+    for (i32 i = 0; i < n; ++i)
+      for (i32 k = A.rowptr[size_t(i)]; k < A.rowptr[size_t(i) + 1]; ++k) {
+        const i32 j = A.col[size_t(k)];
+        build_work_ += 6 * static_cast<i64>(F.Prow[size_t(j) + 1] - F.Prow[size_t(j)]);
+      }
+    // P^T by coarse node: (fine node, P entry), fine nodes ascending
+    F.Rrow.assign(size_t(na) + 1, 0);
+    for (i32 J : F.Pcol) ++F.Rrow[size_t(J) + 1];
+    for (i32 a = 0; a < na; ++a) F.Rrow[size_t(a) + 1] += F.Rrow[size_t(a)];
+    F.Rent.assign(F.Pcol.size(), {0, 0});
+    {
+      std::vector<i32> put(F.Rrow.begin(), F.Rrow.end() - 1);
+      for (i32 i = 0; i < n; ++i)
+        for (i32 e = F.Prow[size_t(i)]; e < F.Prow[size_t(i) + 1]; ++e) F.Rent[size_t(put[size_t(F.Pcol[size_t(e)])]++)] = {i, e};
+    }
+    // Galerkin product: A P by fine row, then P^T (A P) by coarse row
+    std::vector<std::vector<Entry>> ap(static_cast<size_t>(n));
+    {
+      std::vector<i32> slot(size_t(na), -1);
+      for (i32 i = 0; i < n; ++i) {
+        auto& row = ap[size_t(i)];
+        for (i32 k = A.rowptr[size_t(i)]; k < A.rowptr[size_t(i) + 1]; ++k) {
+          const i32 j = A.col[size_t(k)];
+          for (i32 e = F.Prow[size_t(j)]; e < F.Prow[size_t(j) + 1]; ++e) {
+            const i32 J = F.Pcol[size_t(e)];
+            i32& at = slot[size_t(J)];
+            if (at < 0) {
+              at = static_cast<i32>(row.size());
+              row.push_back({J, {}});
+            }
+            mul6_add(&A.val[36 * size_t(k)], &F.Pval[36 * size_t(e)], row[size_t(at)].second.data(), false);
+          }
+        }
+        for (const auto& x : row) slot[size_t(x.first)] = -1;
+      }
+    }
+    for (i32 i = 0; i < n; ++i) build_work_ += 6 * static_cast<i64>(F.Prow[size_t(i) + 1] - F.Prow[size_t(i)]) * static_cast<i64>(ap[size_t(i)].size());
+    std::vector<std::vector<Entry>> crow(static_cast<size_t>(na));
+    {
+      std::vector<i32> slot(size_t(na), -1);
+      for (i32 I = 0; I < na; ++I) {
+        auto& out = crow[size_t(I)];
+        for (i32 r = F.Rrow[size_t(I)]; r < F.Rrow[size_t(I) + 1]; ++r) {
+          const i32 i = F.Rent[size_t(r)].first, e = F.Rent[size_t(r)].second;
+          for (const auto& x : ap[size_t(i)]) {
+            i32& at = slot[size_t(x.first)];
+            if (at < 0) {
+              at = static_cast<i32>(out.size());
+              out.push_back({x.first, {}});
+            }
+            mul6_add(&F.Pval[36 * size_t(e)], x.second.data(), out[size_t(at)].second.data(), true);
+          }
+        }
+        for (const auto& y : out) slot[size_t(y.first)] = -1;
+      }
+    }
+    ap.clear();
+    Builder B(na);
+    for (i32 I = 0; I < na; ++I)
+      for (const auto& y : crow[size_t(I)]) {
+        f64* dst = B.at(I, y.first);
+        for (int q = 0; q < 36; ++q) dst[q] += y.second[size_t(q)];
+      }
+    crow.clear();
+    C.A = B.finish();
+    if (!smoothed && opt_.coarse_scale != 1.0)
+      for (f64& v : C.A.val) v *= opt_.coarse_scale;
+    if (!finalize(C)) return false;
+    lv_.push_back(std::move(C));
+    return true;
+  }
+
+  void factor() {
+    const Level& L = lv_.back();
+    if (L.A.n > 4 * opt_.coarse_max) {
+      chol_n_ = 0;
+      chol_.clear();
+      return;
+    }
+    const i32 m = 6 * L.A.n;
+    chol_n_ = m;
+    build_work_ += static_cast<i64>(m) * m * m / 108;
+    chol_.assign(size_t(m) * size_t(m), 0.0);
+    auto at = [&](i32 r, i32 c) -> f64& { return chol_[size_t(r) * size_t(m) + size_t(c)]; };
+    for (i32 i = 0; i < L.A.n; ++i)
+      for (i32 k = L.A.rowptr[size_t(i)]; k < L.A.rowptr[size_t(i) + 1]; ++k) {
+        const i32 j = L.A.col[size_t(k)];
+        const f64* b = &L.A.val[36 * size_t(k)];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 6; ++c) at(6 * i + r, 6 * j + c) += b[r * 6 + c];
+      }
+    f64 dmax = 0.0;
+    for (i32 i = 0; i < m; ++i) dmax = std::max(dmax, std::abs(at(i, i)));
+    for (i32 j = 0; j < m; ++j) {
+      f64 d = at(j, j);
+      for (i32 k = 0; k < j; ++k) d -= at(j, k) * at(j, k);
+      if (!(d > 1e-12 * dmax)) d = std::max(1e-12 * dmax, 1e-300);
+      const f64 s = std::sqrt(d);
+      at(j, j) = s;
+      for (i32 i = j + 1; i < m; ++i) {
+        f64 v = 0.5 * (at(i, j) + at(j, i));
+        for (i32 k = 0; k < j; ++k) v -= at(i, k) * at(j, k);
+        at(i, j) = v / s;
+      }
+    }
+    for (i32 i = 0; i < m; ++i)
+      for (i32 k = i + 1; k < m; ++k) at(i, k) = at(k, i);
+  }
+
+  void solve_dense(const f64* b, f64* x) const {
+    const i32 m = chol_n_;
+    auto at = [&](i32 r, i32 c) { return chol_[size_t(r) * size_t(m) + size_t(c)]; };
+    for (i32 i = 0; i < m; ++i) {
+      f64 v = b[i];
+      for (i32 k = 0; k < i; ++k) v -= at(i, k) * x[k];
+      x[i] = v / at(i, i);
+    }
+    for (i32 i = m - 1; i >= 0; --i) {
+      f64 v = x[i];
+      for (i32 k = i + 1; k < m; ++k) v -= at(i, k) * x[k];
+      x[i] = v / at(i, i);
+    }
+  }
+
+  static void single_block(const f64* B, f32* out, bool transpose) {
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) out[r * 6 + c] = static_cast<f32>(transpose ? B[c * 6 + r] : B[r * 6 + c]);
+  }
+
+  void to_single() {
+    for (Level& L : lv_) {
+      const size_t nb = size_t(L.A.blocks());
+      L.Af.assign(36 * nb, 0.0f);
+      for (size_t k = 0; k < nb; ++k) single_block(&L.A.val[36 * k], &L.Af[36 * k], false);
+      L.Df.assign(36 * size_t(L.A.n), 0.0f);
+      for (i32 i = 0; i < L.A.n; ++i) single_block(&L.Dinv[36 * size_t(i)], &L.Df[36 * size_t(i)], false);
+      L.Pf.assign(36 * L.Pcol.size(), 0.0f);
+      for (size_t e = 0; e < L.Pcol.size(); ++e) single_block(&L.Pval[36 * e], &L.Pf[36 * e], false);
+      L.Rf.assign(36 * L.Rent.size(), 0.0f);
+      for (size_t r = 0; r < L.Rent.size(); ++r) single_block(&L.Pval[36 * size_t(L.Rent[r].second)], &L.Rf[36 * r], true);
+      const size_t m = 6 * size_t(L.A.n);
+      L.xf.assign(m, 0.0f);
+      L.bf.assign(m, 0.0f);
+      L.rf.assign(m, 0.0f);
+      L.of.assign(m, 0.0f);
+      L.cf.assign(m, 0.0f);
+    }
+  }
+
+  // hybrid Gauss-Seidel: sequential inside fixed 512-row blocks, the other blocks' values from
+  // before the sweep; `fresh` (x = 0) skips what is still zero
+  void sweep(const Level& L, f32* x, const f32* b, bool fresh, bool backward) const {
+    const Bsr& A = L.A;
+    const i32 n = A.n;
+    if (!fresh) std::copy(x, x + 6 * size_t(n), L.of.begin());
+    const f32* xo = L.of.data();
+    for (i64 r0 = 0; r0 < n; r0 += kGrain) {
+      const i64 r1 = std::min<i64>(n, r0 + kGrain);
+      for (i64 t = 0; t < r1 - r0; ++t) {
+        const i64 i = backward ? r1 - 1 - t : r0 + t;
+        f32 s[6];
+        for (int q = 0; q < 6; ++q) s[q] = b[6 * i + q];
+        for (i32 k = A.rowptr[size_t(i)] + 1; k < A.rowptr[size_t(i) + 1]; ++k) {
+          const i64 j = A.col[size_t(k)];
+          if (j >= r0 && j < r1) {
+            if (fresh && j > i) continue;
+            fsub(&L.Af[36 * size_t(k)], x + 6 * j, s);
+          } else if (!fresh) {
+            fsub(&L.Af[36 * size_t(k)], xo + 6 * j, s);
+          }
+        }
+        f32 y[6] = {0, 0, 0, 0, 0, 0};
+        fadd(&L.Df[36 * size_t(i)], s, y);
+        for (int q = 0; q < 6; ++q) x[6 * i + q] = y[q];
+      }
+    }
+  }
+
+  void cycle(size_t l, const f32* b, f32* x) const {
+    const Level& L = lv_[l];
+    const i32 n = L.A.n;
+    if (l + 1 == lv_.size() && chol_n_ == 0) {
+      for (int s = 0; s < 4; ++s) sweep(L, x, b, s == 0, false);
+      for (int s = 0; s < 4; ++s) sweep(L, x, b, false, true);
+      return;
+    }
+    if (l + 1 == lv_.size()) {
+      std::vector<f64> bd(6 * size_t(n)), xd(6 * size_t(n));
+      for (size_t k = 0; k < 6 * size_t(n); ++k) bd[k] = b[k];
+      solve_dense(bd.data(), xd.data());
+      for (size_t k = 0; k < 6 * size_t(n); ++k) x[k] = static_cast<f32>(xd[k]);
+      return;
+    }
+    for (int s = 0; s < opt_.sweeps; ++s) sweep(L, x, b, s == 0, false);
+    const Bsr& A = L.A;
+    f32* r = L.rf.data();
+    for (i32 i = 0; i < n; ++i) {
+      f32 s[6];
+      for (int q = 0; q < 6; ++q) s[q] = b[6 * size_t(i) + q];
+      for (i32 k = A.rowptr[size_t(i)]; k < A.rowptr[size_t(i) + 1]; ++k) fsub(&L.Af[36 * size_t(k)], x + 6 * size_t(A.col[size_t(k)]), s);
+      for (int q = 0; q < 6; ++q) r[6 * size_t(i) + q] = s[q];
+    }
+    const Level& C = lv_[l + 1];
+    f32* bc = C.bf.data();
+    for (i32 a = 0; a < C.A.n; ++a) {
+      f32 y[6] = {0, 0, 0, 0, 0, 0};
+      for (i32 e = L.Rrow[size_t(a)]; e < L.Rrow[size_t(a) + 1]; ++e) fadd(&L.Rf[36 * size_t(e)], r + 6 * size_t(L.Rent[size_t(e)].first), y);
+      for (int q = 0; q < 6; ++q) bc[6 * size_t(a) + q] = y[q];
+    }
+    f32* xc = C.xf.data();
+    cycle(l + 1, bc, xc);
+    for (i32 i = 0; i < n; ++i) {
+      f32 y[6];
+      for (int q = 0; q < 6; ++q) y[q] = x[6 * size_t(i) + q];
+      for (i32 e = L.Prow[size_t(i)]; e < L.Prow[size_t(i) + 1]; ++e) fadd(&L.Pf[36 * size_t(e)], xc + 6 * size_t(L.Pcol[size_t(e)]), y);
+      for (int q = 0; q < 6; ++q) x[6 * size_t(i) + q] = y[q];
+    }
+    for (int s = 0; s < opt_.sweeps; ++s) sweep(L, x, b, false, true);
+  }
+
+  AmgOpt opt_;
+  std::vector<Level> lv_;
+  std::vector<f64> chol_;
+  i32 chol_n_ = 0;
+  i64 work_ = 0, build_work_ = 0;
+};
+
+// ---- the stress problem ------------------------------------------------------------------------
+
+const svxg_material& mat_of(const std::vector<svxg_material>& mats, int id) {
+  return (id >= 0 && size_t(id) < mats.size()) ? mats[size_t(id)] : mats[0];
+}
+
+// sum of a[k] b[k] in chunks of 4096 entries, the chunks' sums added in chunk order
+f64 chunked_dot(const std::vector<f64>& a, const std::vector<f64>& b) {
+  const i64 m = static_cast<i64>(a.size());
+  if (m <= 0) return 0.0;
+  const i64 g = 8 * kGrain;
+  const i64 nc = (m + g - 1) / g;
+  auto part = [&](i64 k0, i64 k1) {
+    f64 s = 0.0;
+    for (i64 k = k0; k < k1; ++k) s += a[size_t(k)] * b[size_t(k)];
+    return s;
+  };
+  if (nc == 1) return part(0, m);
+  f64 s = 0.0;
+  for (i64 c = 0; c < nc; ++c) s += part(c * g, std::min(m, (c + 1) * g));
+  return s;
+}
+
+}  // namespace
+
+struct orc_problem {
+  std::vector<svxg_node> nodes;
+  std::vector<svxg_bond> bonds;
+  std::vector<svxg_material> mats;
+  svxg_stress_options opt{};
+  bool assembled = false, pc_current = false, jacobi_only = false;
+  i64 assembly_work = 0;
+  i32 pc_n = 0, nfree = 0;
+  f64 id_scale = 1.0;
+  std::vector<f64> jinv;
+  std::vector<i32> dof;
+  Bsr K;
+  Multigrid amg;
+  struct Run {
+    bool active = false;
+    std::vector<f64> x, r, z, p, q;
+    f64 rz = 0.0, bn = 0.0;
+    int iters = 0;
+  } run;
+
+  void bond_matrices(const svxg_bond& b, f64 D[36], f64 Ba[36], f64 Bb[36]) const {
+    const svxg_material& A = mat_of(mats, b.ma);
+    const svxg_material& B = mat_of(mats, b.mb);
+    const f64 la = std::max(b.la, 1e-4), lb = std::max(b.lb, 1e-4);
+    const f64 invE = la / A.E + lb / B.E;
+    const f64 invG = la / A.G + lb / B.G;
+    const f64 kt[3] = {b.area / invE, (5.0 / 6.0) * b.area / invG, (5.0 / 6.0) * b.area / invG};
+    const f64 kr[3] = {0.8 * (b.s1 + b.s2) / invG, b.s2 / invE, b.s1 / invE};
+    const V3 Q[3] = {v3(b.n), v3(b.t1), v3(b.t2)};
+    for (int q = 0; q < 36; ++q) D[q] = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        f64 st = 0.0, sr = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          st += vget(Q[k], r) * kt[k] * vget(Q[k], c);
+          sr += vget(Q[k], r) * kr[k] * vget(Q[k], c);
+        }
+        D[r * 6 + c] = st;
+        D[(3 + r) * 6 + 3 + c] = sr;
+      }
+    rigid(vsub(v3(b.p), v3(nodes[size_t(b.a)].c)), Ba);
+    if (b.b >= 0) rigid(vsub(v3(b.p), v3(nodes[size_t(b.b)].c)), Bb);
+  }
+
+  void build_matrix() {
+    const i32 n = static_cast<i32>(nodes.size());
+    dof.assign(size_t(n), -1);
+    nfree = 0;
+    for (i32 i = 0; i < n; ++i)
+      if (!nodes[size_t(i)].fixed) dof[size_t(i)] = nfree++;
+    Builder B(nfree);
+    for (i32 i = 0; i < n; ++i)
+      if (dof[size_t(i)] >= 0) B.at(dof[size_t(i)], dof[size_t(i)]);
+    f64 D[36], Ba[36], Bb[36];
+    for (svxg_bond& b : bonds) {
+      b.in_k = 0;
+      if (b.broken || nodes[size_t(b.a)].gone || (b.b >= 0 && nodes[size_t(b.b)].gone)) continue;
+      b.in_k = 1;
+      bond_matrices(b, D, Ba, Bb);
+      const i32 ia = dof[size_t(b.a)];
+      const i32 ib = b.b >= 0 ? dof[size_t(b.b)] : -1;
+      if (ia >= 0) atbd_add(Ba, D, Ba, 1.0, B.at(ia, ia));
+      if (ib >= 0) atbd_add(Bb, D, Bb, 1.0, B.at(ib, ib));
+      if (ia >= 0 && ib >= 0) {
+        atbd_add(Ba, D, Bb, -1.0, B.at(ia, ib));
+        atbd_add(Bb, D, Ba, -1.0, B.at(ib, ia));
+      }
+    }
+    K = B.finish();
+    f64 tr = 0.0;
+    i32 cnt = 0;
+    for (i32 i = 0; i < nfree; ++i) {
+      const f64* d = K.diag(i);
+      const f64 t = d[0] + d[7] + d[14];
+      if (t > 0) {
+        tr += t / 3.0;
+        ++cnt;
+      }
+    }
+    id_scale = cnt ? tr / cnt : 1.0;
+    for (i32 i = 0; i < n; ++i) {
+      const i32 d = dof[size_t(i)];
+      if (d < 0) continue;
+      f64* Dg = K.diag(d);
+      const bool empty = Dg[0] == 0.0 && Dg[7] == 0.0 && Dg[14] == 0.0;
+      if (nodes[size_t(i)].gone || empty)
+        for (int q = 0; q < 6; ++q) Dg[q * 7] = id_scale;
+    }
+  }
+
+  bool assemble(const svxg_stress_options& o) {
+    opt = o;
+    assembled = false;
+    pc_current = true;
+    run.active = false;
+    build_matrix();
+    assembly_work = 0;
+    for (const svxg_bond& b : bonds)
+      if (b.in_k) assembly_work += 48;
+    std::vector<V3> pos(static_cast<size_t>(nfree));
+    for (size_t i = 0; i < nodes.size(); ++i)
+      if (dof[i] >= 0) pos[size_t(dof[i])] = v3(nodes[i].c);
+    jacobi_only = nfree < opt.amg_min_nodes;
+    if (jacobi_only) {
+      amg = Multigrid{};
+      pc_n = 0;
+      jinv.assign(36 * size_t(nfree), 0.0);
+      for (i32 i = 0; i < nfree; ++i)
+        if (!inv6(K.diag(i), &jinv[36 * size_t(i)]))
+          for (int q = 0; q < 6; ++q) jinv[36 * size_t(i) + size_t(q * 7)] = 1.0 / id_scale;
+      assembly_work += 12 * static_cast<i64>(nfree);
+      assembled = true;
+      return true;
+    }
+    AmgOpt ao;
+    ao.coarse_max = opt.amg.coarse_max;
+    ao.strength = opt.amg.strength;
+    ao.smoothed = opt.amg.smoothed != 0;
+    ao.coarse_scale = opt.amg.coarse_scale;
+    ao.max_levels = opt.amg.max_levels;
+    ao.sweeps = opt.amg.sweeps;
+    ao.coarsen_dense = opt.amg.coarsen_dense != 0;
+    ao.reaggregate = opt.amg.reaggregate != 0;
+    if (!amg.build(K, pos, ao)) return false;
+    assembly_work += amg.build_work();
+    pc_n = nfree;
+    jinv.clear();
+    assembled = true;
+    return true;
+  }
+
+  void release() {
+    assembled = pc_current = jacobi_only = false;
+    pc_n = nfree = 0;
+    run = Run{};
+    K = Bsr{};
+    amg = Multigrid{};
+    jinv.clear();
+    dof.clear();
+    for (svxg_bond& b : bonds) b.in_k = 0;
+  }
+
+  void remove_bond(i32 bi) {
+    svxg_bond& b = bonds[size_t(bi)];
+    b.broken = 1;
+    if (!assembled || !b.in_k) return;
+    b.in_k = 0;
+    f64 D[36], Ba[36], Bb[36];
+    bond_matrices(b, D, Ba, Bb);
+    const i32 ia = dof[size_t(b.a)];
+    const i32 ib = b.b >= 0 ? dof[size_t(b.b)] : -1;
+    if (ia >= 0) atbd_add(Ba, D, Ba, -1.0, K.find(ia, ia));
+    if (ib >= 0) atbd_add(Bb, D, Bb, -1.0, K.find(ib, ib));
+    if (ia >= 0 && ib >= 0) {
+      if (f64* k = K.find(ia, ib)) atbd_add(Ba, D, Bb, 1.0, k);
+      if (f64* k = K.find(ib, ia)) atbd_add(Bb, D, Ba, 1.0, k);
+    }
+    if (!jacobi_only) pc_current = false;
+    if (jacobi_only)
+      for (i32 d : {ia, ib}) {
+        if (d < 0) continue;
+        if (!inv6(K.diag(d), &jinv[36 * size_t(d)]))
+          for (int q = 0; q < 36; ++q) jinv[36 * size_t(d) + size_t(q)] = (q % 7 == 0) ? 1.0 / id_scale : 0.0;
+      }
+    run.active = false;
+  }
+
+  void retire_nodes(const i32* ids, i32 count) {
+    if (count <= 0) return;
+    std::vector<uint8_t> now(nodes.size(), 0);
+    for (i32 t = 0; t < count; ++t) {
+      const i32 i = ids[t];
+      now[size_t(i)] = nodes[size_t(i)].gone ? 0 : 1;
+      nodes[size_t(i)].gone = 1;
+    }
+    for (i32 b = 0; b < static_cast<i32>(bonds.size()); ++b) {
+      const svxg_bond& B = bonds[size_t(b)];
+      if (now[size_t(B.a)] || (B.b >= 0 && now[size_t(B.b)])) remove_bond(b);
+    }
+    if (assembled)
+      for (i32 t = 0; t < count; ++t) {
+        const i32 i = ids[t];
+        const i32 d = dof[size_t(i)];
+        if (d < 0) continue;
+        if (now[size_t(i)] && !jacobi_only) pc_current = false;
+        f64* Dg = K.find(d, d);
+        for (int q = 0; q < 36; ++q) Dg[q] = (q % 7 == 0) ? id_scale : 0.0;
+        if (jacobi_only)
+          for (int q = 0; q < 36; ++q) jinv[36 * size_t(d) + size_t(q)] = (q % 7 == 0) ? 1.0 / id_scale : 0.0;
+      }
+    run.active = false;
+  }
+
+  void precondition(const f64* r, f64* z) const {
+    if (!jacobi_only) {
+      amg.apply(r, z);
+      return;
+    }
+    for (i32 i = 0; i < nfree; ++i) mv6(&jinv[36 * size_t(i)], r + 6 * size_t(i), z + 6 * size_t(i));
+  }
+
+  void begin(const f64* f, const f64* u0) {
+    run.active = false;
+    if (!assembled && !assemble(opt)) return;
+    const size_t n = nodes.size();
+    const size_t m = 6 * size_t(nfree);
+    run.x.assign(m, 0.0);
+    std::vector<f64> fb(m, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+      const i32 d = dof[i];
+      if (d < 0) continue;
+      for (int q = 0; q < 6; ++q) {
+        fb[6 * size_t(d) + size_t(q)] = f[6 * i + size_t(q)];
+        if (u0) run.x[6 * size_t(d) + size_t(q)] = u0[6 * i + size_t(q)];
+      }
+    }
+    run.r.assign(m, 0.0);
+    run.z.assign(m, 0.0);
+    run.p.assign(m, 0.0);
+    run.q.assign(m, 0.0);
+    K.apply(run.x.data(), run.q.data());
+    f64 bn = 0.0;
+    for (size_t k = 0; k < m; ++k) {
+      run.r[k] = fb[k] - run.q[k];
+      bn += fb[k] * fb[k];
+    }
+    run.bn = std::sqrt(bn);
+    if (!(bn > 0.0)) {
+      std::fill(run.x.begin(), run.x.end(), 0.0);
+      std::fill(run.r.begin(), run.r.end(), 0.0);
+    }
+    precondition(run.r.data(), run.z.data());
+    run.p = run.z;
+    run.rz = 0.0;
+    for (size_t k = 0; k < m; ++k) run.rz += run.r[k] * run.z[k];
+    run.iters = 0;
+    run.active = true;
+  }
+
+  svxg_pcg_result iterate(int maxit, f64 rtol) {
+    svxg_pcg_result res{};
+    if (!run.active) return res;
+    const size_t m = run.x.size();
+    f64 rn = std::sqrt(chunked_dot(run.r, run.r));
+    const f64 target = rtol * run.bn;
+    if (!std::isfinite(run.bn) || !std::isfinite(rn)) {
+      res.rel_res = INFINITY;
+      res.breakdown = 1;
+      run.active = false;
+      return res;
+    }
+    res.rel_res = run.bn > 0 ? rn / run.bn : 0.0;
+    if (rn <= target || !(run.bn > 0) || m == 0) {
+      res.converged = 1;
+      return res;
+    }
+    for (int it = 0; it < maxit; ++it) {
+      K.apply(run.p.data(), run.q.data());
+      const f64 pq = chunked_dot(run.p, run.q);
+      ++res.iters;
+      ++run.iters;
+      if (!(pq > 0)) {
+        res.converged = 0;
+        res.breakdown = 1;
+        break;
+      }
+      const f64 alpha = run.rz / pq;
+      for (size_t k = 0; k < m; ++k) {
+        run.x[k] += alpha * run.p[k];
+        run.r[k] -= alpha * run.q[k];
+      }
+      rn = std::sqrt(chunked_dot(run.r, run.r));
+      res.rel_res = rn / run.bn;
+      if (rn <= target) {
+        res.converged = 1;
+        break;
+      }
+      precondition(run.r.data(), run.z.data());
+      const f64 rz1 = chunked_dot(run.r, run.z);
+      if (!(rz1 > 0)) {
+        res.breakdown = 1;
+        break;
+      }
+      const f64 beta = rz1 / run.rz;
+      run.rz = rz1;
+      for (size_t k = 0; k < m; ++k) run.p[k] = run.z[k] + beta * run.p[k];
+    }
+    return res;
+  }
+
+  void current(f64* u) const {
+    const size_t n = nodes.size();
+    std::fill(u, u + 6 * n, 0.0);
+    if (run.x.size() != 6 * size_t(nfree) || dof.size() != n) return;
+    for (size_t i = 0; i < n; ++i) {
+      const i32 d = dof[i];
+      if (d < 0) continue;
+      for (int q = 0; q < 6; ++q) u[6 * i + size_t(q)] = run.x[6 * size_t(d) + size_t(q)];
+    }
+  }
+
+  svxg_bond_load bond_load(i32 bi, const f64* u) const {
+    const svxg_bond& b = bonds[size_t(bi)];
+    const svxg_material& A = mat_of(mats, b.ma);
+    const svxg_material& B = mat_of(mats, b.mb);
+    f64 Ba[36], Bb[36];
+    rigid(vsub(v3(b.p), v3(nodes[size_t(b.a)].c)), Ba);
+    f64 xa[6] = {0, 0, 0, 0, 0, 0}, xb[6] = {0, 0, 0, 0, 0, 0};
+    if (!nodes[size_t(b.a)].fixed) mv6(Ba, &u[6 * size_t(b.a)], xa);
+    if (b.b >= 0) {
+      rigid(vsub(v3(b.p), v3(nodes[size_t(b.b)].c)), Bb);
+      if (!nodes[size_t(b.b)].fixed) mv6(Bb, &u[6 * size_t(b.b)], xb);
+    }
+    const V3 dt{xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
+    const V3 dr{xb[3] - xa[3], xb[4] - xa[4], xb[5] - xa[5]};
+    const f64 la = std::max(b.la, 1e-4), lb = std::max(b.lb, 1e-4);
+    const f64 invE = la / A.E + lb / B.E;
+    const f64 invG = la / A.G + lb / B.G;
+    svxg_bond_load L;
+    L.N = b.area / invE * vdot(dt, v3(b.n));
+    L.V1 = (5.0 / 6.0) * b.area / invG * vdot(dt, v3(b.t1));
+    L.V2 = (5.0 / 6.0) * b.area / invG * vdot(dt, v3(b.t2));
+    L.T = 0.8 * (b.s1 + b.s2) / invG * vdot(dr, v3(b.n));
+    L.M1 = b.s2 / invE * vdot(dr, v3(b.t1));
+    L.M2 = b.s1 / invE * vdot(dr, v3(b.t2));
+    return L;
+  }
+};
+
+namespace {
+
+void strengths(const svxg_bond& b, f64 fragility, const svxg_material* mats, i32 nmats, f64 S[5]) {
+  const f64 s = b.strength / std::max(1e-6, fragility);
+  if (b.sectioned) {
+    S[0] = s * b.ft;
+    S[1] = s * b.fb;
+    S[2] = s * b.fc;
+    S[3] = s * b.coh;
+    S[4] = static_cast<f64>(b.mu);
+    return;
+  }
+  const svxg_material& A = (b.ma < nmats) ? mats[b.ma] : mats[0];
+  const svxg_material& B = (b.mb < nmats) ? mats[b.mb] : mats[0];
+  S[0] = s * std::min(A.ft, B.ft);
+  S[1] = s * std::min(A.fb, B.fb);
+  S[2] = s * std::min(A.fc, B.fc);
+  S[3] = s * std::min(A.cohesion, B.cohesion);
+  S[4] = std::min(A.friction, B.friction);
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_problem* orc_create(const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m, const svxg_material* mats,
+                        int32_t nmats) {
+  orc_problem* h = new orc_problem;
+  h->nodes.assign(nodes, nodes + n);
+  h->bonds.assign(bonds, bonds + m);
+  h->mats.assign(mats, mats + nmats);
+  orc_default_options(&h->opt);
+  return h;
+}
+
+void orc_destroy(orc_problem* h) { delete h; }
+
+int orc_assemble(orc_problem* h, const svxg_stress_options* opt) { return h->assemble(*opt) ? 1 : 0; }
+void orc_release(orc_problem* h) { h->release(); }
+
+int orc_solve(orc_problem* h, const double* f, double* u, double rtol, int32_t maxit, int32_t warm, svxg_pcg_result* res) {
+  *res = svxg_pcg_result{};
+  if (!h->assembled && !h->assemble(h->opt)) {
+    res->rel_res = INFINITY;
+    res->breakdown = 1;
+    return 0;
+  }
+  h->begin(f, warm ? u : nullptr);
+  *res = h->iterate(maxit < 0 ? 100000 : maxit, rtol);
+  h->current(u);
+  h->run.active = false;
+  return 1;
+}
+
+void orc_begin(orc_problem* h, const double* f, const double* u0) { h->begin(f, u0); }
+void orc_iterate(orc_problem* h, int32_t maxit, double rtol, svxg_pcg_result* res) { *res = h->iterate(maxit, rtol); }
+void orc_current(orc_problem* h, double* u) { h->current(u); }
+void orc_remove_bond(orc_problem* h, int32_t b) { h->remove_bond(b); }
+void orc_retire_nodes(orc_problem* h, const int32_t* ids, int32_t count) { h->retire_nodes(ids, count); }
+void orc_set_bond_broken(orc_problem* h, int32_t b, int32_t broken) { h->bonds[size_t(b)].broken = broken ? 1 : 0; }
+void orc_bond_load(orc_problem* h, int32_t b, const double* u, svxg_bond_load* out) { *out = h->bond_load(b, u); }
+
+void orc_bond_strength(const svxg_bond* b, double fragility, const svxg_material* mats, int32_t nmats, double out[5]) {
+  strengths(*b, fragility, mats, nmats, out);
+}
+
+double orc_bond_utilization(const svxg_bond* bp, const svxg_bond_load* Lp, double fragility, const svxg_material* mats,
+                            int32_t nmats, uint8_t* mode) {
+  const svxg_bond& b = *bp;
+  const svxg_bond_load& L = *Lp;
+  f64 S[5];
+  strengths(b, fragility, mats, nmats, S);
+  constexpr f64 kWeak = 1e-9;
+  const f64 ft = std::max(S[0], kWeak), fb = std::max(S[1], kWeak), fc = std::max(S[2], kWeak);
+  const f64 A = std::max(b.area, 1e-12);
+  const f64 sN = L.N / A;
+  const f64 sb = (b.s2 > 0 ? std::abs(L.M1) * b.c2 / b.s2 : 0.0) + (b.s1 > 0 ? std::abs(L.M2) * b.c1 / b.s1 : 0.0);
+  const f64 phi_t = sN >= 0.0 ? sN / ft + sb / fb : std::max(0.0, sb + sN) / fb;
+  const f64 phi_c = std::max(0.0, sb - sN) / fc;
+  const f64 J = std::max(b.s1 + b.s2, 1e-18);
+  const f64 tau = 1.5 * std::sqrt(L.V1 * L.V1 + L.V2 * L.V2) / A + std::abs(L.T) * b.rmax / J;
+  const f64 cap = std::max(S[3] + S[4] * std::max(0.0, -sN), kWeak);
+  const f64 phi_s = tau / cap;
+  f64 phi = phi_t;
+  uint8_t md = 1;
+  if (phi_c > phi) {
+    phi = phi_c;
+    md = 2;
+  }
+  if (phi_s > phi) {
+    phi = phi_s;
+    md = 3;
+  }
+  if (mode) *mode = phi > 0 ? md : 0;
+  return phi;
+}
+
+void orc_info(orc_problem* h, svxg_problem_info* out) {
+  *out = svxg_problem_info{};
+  out->nodes = static_cast<int32_t>(h->nodes.size());
+  out->bonds = static_cast<int32_t>(h->bonds.size());
+  out->nfree = h->nfree;
+  out->block_jacobi = h->jacobi_only;
+  out->assembled = h->assembled;
+  out->preconditioner_current = h->assembled && h->pc_current;
+  out->blocks = h->K.blocks();
+  out->work_per_iteration = 2 * h->K.blocks() + h->amg.work_per_apply();
+  out->assembly_work = h->assembly_work;
+  out->levels = static_cast<int32_t>(h->amg.levels());
+  for (size_t l = 0; l < h->amg.levels() && l < 16; ++l) {
+    out->level_rows[l] = h->amg.level(l).A.n;
+    out->level_blocks[l] = h->amg.level(l).A.blocks();
+    out->level_pblocks[l] = static_cast<int64_t>(h->amg.level(l).Pcol.size());
+  }
+}
+
+void orc_get_matrix(orc_problem* h, int32_t* rowptr, int32_t* col, double* val) {
+  if (rowptr) std::copy(h->K.rowptr.begin(), h->K.rowptr.end(), rowptr);
+  if (col) std::copy(h->K.col.begin(), h->K.col.end(), col);
+  if (val) std::copy(h->K.val.begin(), h->K.val.end(), val);
+}
+
+void orc_apply_matrix(orc_problem* h, const double* x, double* y) { h->K.apply(x, y); }
+void orc_apply_preconditioner(orc_problem* h, const double* r, double* z) { h->precondition(r, z); }
+void orc_get_bonds(orc_problem* h, svxg_bond* out) { std::copy(h->bonds.begin(), h->bonds.end(), out); }
+void orc_get_nodes(orc_problem* h, svxg_node* out) { std::copy(h->nodes.begin(), h->nodes.end(), out); }
+void orc_set_threads(int32_t) {}
+
+// StressOptions{} / AmgOptions{} defaults (stress.hpp:68-72, amg.hpp:46-69).
+void orc_default_options(svxg_stress_options* o) {
+  o->rtol = 2e-3;
+  o->amg_min_nodes = 160;
+  o->precision = SVXG_FP64;
+  o->amg.strength = 0.1;
+  o->amg.coarse_scale = 0.6;
+  o->amg.coarse_max = 48;
+  o->amg.smoothed = 1;
+  o->amg.max_levels = 16;
+  o->amg.sweeps = 1;
+  o->amg.cheb_degree = 3;
+  o->amg.coarsen_dense = 1;
+  o->amg.reaggregate = 1;
+}
+
+}  // extern "C"

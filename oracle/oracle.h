@@ -1,0 +1,52 @@
+/* oracle — CPU restatement of the reference's fragment-graph stress solve.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the device library is compared with. Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the product path
+ * (libsvxg.so and the opus_destruct_2_amd package) never does.
+ *
+ * The same entry points, prefixed ref_ instead of orc_, are exported by oracle/_ref/libsvxref.so:
+ * the reference's own sources (core/src/stress/stress.cpp, core/src/solve/amg.cpp,
+ * core/src/base/parallel.cpp, core/src/material/material.cpp) compiled by oracle/Makefile behind
+ * oracle/ref_shim.cpp.
+ */
+#ifndef SVX_ORACLE_H
+#define SVX_ORACLE_H
+
+#include "svxg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_problem orc_problem;
+
+orc_problem* orc_create(const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m, const svxg_material* mats,
+                        int32_t nmats);
+void orc_destroy(orc_problem* h);
+int orc_assemble(orc_problem* h, const svxg_stress_options* opt);
+void orc_release(orc_problem* h);
+int orc_solve(orc_problem* h, const double* f, double* u, double rtol, int32_t maxit, int32_t warm, svxg_pcg_result* res);
+void orc_begin(orc_problem* h, const double* f, const double* u0);
+void orc_iterate(orc_problem* h, int32_t maxit, double rtol, svxg_pcg_result* res);
+void orc_current(orc_problem* h, double* u);
+void orc_remove_bond(orc_problem* h, int32_t b);
+void orc_retire_nodes(orc_problem* h, const int32_t* ids, int32_t count);
+void orc_set_bond_broken(orc_problem* h, int32_t b, int32_t broken);
+void orc_bond_load(orc_problem* h, int32_t b, const double* u, svxg_bond_load* out);
+double orc_bond_utilization(const svxg_bond* b, const svxg_bond_load* L, double fragility, const svxg_material* mats,
+                            int32_t nmats, uint8_t* mode);
+void orc_bond_strength(const svxg_bond* b, double fragility, const svxg_material* mats, int32_t nmats, double out[5]);
+void orc_info(orc_problem* h, svxg_problem_info* out);
+void orc_get_matrix(orc_problem* h, int32_t* rowptr, int32_t* col, double* val);
+void orc_apply_matrix(orc_problem* h, const double* x, double* y);
+void orc_apply_preconditioner(orc_problem* h, const double* r, double* z);
+void orc_get_bonds(orc_problem* h, svxg_bond* out);
+void orc_get_nodes(orc_problem* h, svxg_node* out);
+void orc_set_threads(int32_t n);
+void orc_default_options(svxg_stress_options* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
