@@ -1,0 +1,182 @@
+"""Benchmark of the stress solve on one MI355X.
+
+Workload (the metric's single-GPU configuration): one irregular block of 64 x 64 x 48 fragments with
+15 % voids (about 167k nodes, 1.0M DOFs, stiffness ratio up to 20:1), solved to a relative residual
+of 1e-8 and bond-checked. A step is one solve from a cold start plus the bond checks on the
+device-resident solution. The headline runs the mixed path (FP32 K products and V-cycle inside FP64
+iterative refinement); the plain FP64 path is measured beside it.
+
+value: solved DOFs / (device time of the solve + wall time of the checks), inputs resident.
+roofline: the dominant kernel's algorithmic bytes per launch / its average launch time, measured
+live with HIP events on the library's stream during the timed steps.
+cpu_baseline: the reference's own solver (compiled from its sources into oracle/_ref) on the same
+workload, one assemble + solve, all host threads (up to 16).
+
+  python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PCG-solved DOFs/s at 1e-8 residual, batched vs plain; p50/p99 tick latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(g, mats, precision, steps, warmup, ctx):
+    from opus_destruct_2_amd import stress
+    p = stress.StressProblem(g.nodes, g.bonds, mats, ctx)
+    t = time.perf_counter()
+    assert p.assemble(stress.default_options(precision=precision))
+    setup_ms = (time.perf_counter() - t) * 1e3
+    info = p.info()
+    log(f"  precision={precision} setup {setup_ms:.1f} ms levels {info['level_rows']} blocks {info['level_blocks']}")
+    for _ in range(warmup):
+        u, r = p.solve(g.f, rtol=1e-8, maxit=5000)
+        p.bond_checks(None)
+    ctx.set_profiling(True)
+    for k in range(4):
+        ctx.kernel_stats(k)
+    dev_ms, wall_ms, chk_ms, iters = [], [], [], []
+    for s in range(steps):
+        t0 = time.perf_counter()
+        u, r = p.solve(g.f, rtol=1e-8, maxit=5000)
+        t1 = time.perf_counter()
+        phi, mode = p.bond_checks(None)
+        t2 = time.perf_counter()
+        assert r["converged"], r
+        dev_ms.append(p.last_timing()["solve_ms"])
+        wall_ms.append((t1 - t0) * 1e3)
+        chk_ms.append((t2 - t1) * 1e3)
+        iters.append(r["iters"])
+        log(f"  step {s}: solve {dev_ms[-1]:.2f} ms device ({wall_ms[-1]:.2f} wall), checks {chk_ms[-1]:.2f} ms, "
+            f"iters {r['iters']} refinements {r['refinements']} rel_res {r['rel_res']:.2e}")
+    ctx.set_profiling(False)
+    kstats = {k: ctx.kernel_stats(k) for k in range(4)}
+    out = {
+        "setup_ms": setup_ms, "solve_ms": float(np.mean(dev_ms)), "wall_ms": float(np.mean(wall_ms)),
+        "checks_ms": float(np.mean(chk_ms)), "iters": int(np.median(iters)), "refinements": r["refinements"],
+        "rel_res": r["rel_res"], "u": u, "phi": phi, "kstats": kstats, "info": info,
+    }
+    p.close()
+    return out
+
+
+def roofline(res, precision):
+    names = {0: "k_pcg_spmv<double>", 1: "k_pcg_spmv<float>", 2: "k_cheb_step<double> (level 0)",
+             3: "k_cheb_step<float> (level 0)"}
+    kinds = (1, 3) if precision == "mixed" else (0, 2)
+    best = max(kinds, key=lambda k: res["kstats"][k][1])
+    n, ms, byts = res["kstats"][best]
+    if n == 0 or ms <= 0:
+        return None
+    avg_s = ms / n / 1e3
+    per_launch = byts / n
+    achieved = per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": names[best],
+            "launches": n, "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": int(per_launch)}
+
+
+def cpu_baseline(g, mats):
+    """The reference's own solve (oracle/_ref: its sources compiled here) on the same workload."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    kind = "reference" if oracle.ref_available() else "port"
+    if kind == "reference":
+        oracle.set_ref_threads(threads)
+    else:
+        threads = 1
+    p = oracle.CpuProblem(g.nodes, g.bonds, mats, "ref" if kind == "reference" else "oracle")
+    t = time.perf_counter()
+    assert p.assemble(oracle.reference_options())
+    setup_s = time.perf_counter() - t
+    t = time.perf_counter()
+    u, r = p.solve(g.f, rtol=1e-8, maxit=5000)
+    solve_s = time.perf_counter() - t
+    phi, mode = p.bond_checks(u)
+    return {"value": round(g.dofs / solve_s, 1), "unit": "DOF/s", "cores": threads, "kind": kind,
+            "sample": f"one solve of the same block to 1e-8 ({r['iters']} iterations, {solve_s:.2f} s; "
+                      f"assemble + multigrid build {setup_s:.2f} s)",
+            "u": u, "phi": phi, "setup_s": setup_s, "solve_s": solve_s, "iters": r["iters"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--ny", type=int, default=64)
+    ap.add_argument("--nz", type=int, default=48)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the stress solve here is measured on one MI355X's share; "
+              "--gpus above 1 is not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    from opus_destruct_2_amd import gen, stress
+    from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
+    mats = gen.materials()
+    g = gen.block(args.nx, args.ny, args.nz, 0.5, 0.15, seed=1)
+    log(f"workload {g.name}: {g.n} nodes, {g.m} bonds, {g.dofs} DOFs")
+    ctx = stress.default_context()
+    head_prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
+    other_prec = SVXG_FP64 if head_prec == SVXG_MIXED else SVXG_MIXED
+    head = run_path(g, mats, head_prec, args.steps, args.warmup, ctx)
+    other = run_path(g, mats, other_prec, max(1, args.steps // 2), 1, ctx)
+    plain = other if head_prec == SVXG_MIXED else head
+    step_s = (head["solve_ms"] + head["checks_ms"]) / 1e3
+    value = g.dofs / step_s
+    rl = roofline(head, args.precision)
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if head_prec == SVXG_MIXED else "f64", "data": "synthetic",
+        "config": {"workload": f"single large structure: block {args.nx}x{args.ny}x{args.nz}, 15% voids, "
+                               f"{g.n} nodes, {g.dofs} DOFs, AMG-PCG to 1e-8 + bond checks",
+                   "precision": args.precision, "iters": head["iters"], "refinements": head["refinements"],
+                   "setup_ms": round(head["setup_ms"], 1), "solve_ms": round(head["solve_ms"], 3),
+                   "checks_ms": round(head["checks_ms"], 3), "wall_ms_pcie_inclusive": round(head["wall_ms"], 3)},
+        "plain_fp64": {"value": round(g.dofs / ((plain["solve_ms"] + plain["checks_ms"]) / 1e3), 1),
+                       "solve_ms": round(plain["solve_ms"], 3), "iters": plain["iters"],
+                       "setup_ms": round(plain["setup_ms"], 1)},
+        "roofline": rl,
+    }
+    ref_u = plain["u"]
+    mixed_res = head if head_prec == SVXG_MIXED else other
+    scale = np.abs(ref_u).max()
+    line["mixed_vs_fp64"] = {"u_rel_err": float(np.abs(mixed_res["u"] - ref_u).max() / scale),
+                             "phi_max_abs_err": float(np.abs(mixed_res["phi"] - plain["phi"]).max())}
+    if not args.no_cpu:
+        log("cpu baseline: the reference's solver on the same workload ...")
+        cb = cpu_baseline(g, mats)
+        su = np.abs(cb["u"]).max()
+        line["error_vs_cpu_reference"] = {
+            "fp64_u_rel": float(np.abs(plain["u"] - cb["u"]).max() / su),
+            "mixed_u_rel": float(np.abs(mixed_res["u"] - cb["u"]).max() / su),
+            "fp64_phi_abs": float(np.abs(plain["phi"] - cb["phi"]).max()),
+            "failed_bonds_gpu": int((head["phi"] >= 1).sum()), "failed_bonds_cpu": int((cb["phi"] >= 1).sum()),
+        }
+        for k in ("u", "phi", "setup_s", "solve_s", "iters"):
+            cb.pop(k)
+        line["cpu_baseline"] = cb
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -220,9 +220,17 @@ int svxg_problem_apply_matrix(svxg_problem* p, const double* x, double* y, int32
 /* z = M^-1 r: one application of the preconditioner (Amg::apply, amg.cpp:718-731; block-Jacobi,
    stress.cpp:312-324) over free DOFs, in the options' precision. */
 int svxg_problem_apply_preconditioner(svxg_problem* p, const double* r, double* z);
-/* Milliseconds of device time of the last solve's solve phase (events on the problem's stream),
-   and its count of outer K products. */
+/* Milliseconds of device time of the last solve (HIP events on the context's stream around the
+   solve) and wall milliseconds of the last assemble. */
 int svxg_problem_last_timing(const svxg_problem* p, double* solve_ms, double* setup_ms);
+
+/* Kernel timing (no reference counterpart; for benchmarks): with profiling on, the solve drivers
+   bracket every launch of the profiled kernels with HIP events on the context's stream.
+   kind 0 / 1: the outer K product, FP64 / FP32; kind 2 / 3: a level-0 Chebyshev smoothing step
+   (an A product fused with the smoother update), FP64 / FP32. Stats: launches, summed device
+   milliseconds and summed algorithmic bytes since the last call (which resets them). */
+int svxg_ctx_set_profiling(svxg_ctx* ctx, int32_t on);
+int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,505 @@
+// libsvxg host side: the device arena, the assembly plan, the batched multigrid setup, the PCG and
+// mixed-precision refinement drivers and the C ABI (include/svxg.h).
+//
+// The reference's StressProblem (core/src/stress/stress.cpp) runs every step on the CPU; here the
+// host keeps only integer bookkeeping (the DOF map, K's block pattern and each block's list of bond
+// contributions - the reference's Bsr6Builder rows, amg.cpp:185-224) and drives the device: every
+// floating-point value of K, of the hierarchy and of the solve is computed by the kernels.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "host_common.hpp"
+
+namespace svxg {
+
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& msg) { g_error = msg; }
+
+// ---- arena ---------------------------------------------------------------------------------------
+
+static int size_class(size_t bytes) {
+  size_t s = 256;
+  int c = 0;
+  while (s < bytes) {
+    s <<= 1;
+    ++c;
+  }
+  return c;
+}
+
+Arena::~Arena() {
+  (void)hipSetDevice(device_);
+  for (void* p : slabs_) (void)hipFree(p);
+  for (void* p : big_) (void)hipFree(p);
+}
+
+int Arena::alloc(size_t bytes, void** out, size_t* held) {
+  std::lock_guard<std::mutex> lk(mu_);
+  *out = nullptr;
+  if (bytes >= kBig) {
+    const size_t sz = (bytes + 4095) & ~size_t(4095);
+    if (in_use_ + int64_t(sz) > budget_) {
+      set_error("HBM budget exceeded");
+      return SVXG_OOM_BUDGET;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+      return SVXG_DEVICE_ERROR;
+    }
+    big_.push_back(p);
+    in_use_ += int64_t(sz);
+    *out = p;
+    *held = sz;
+    return SVXG_OK;
+  }
+  const int c = size_class(bytes);
+  const size_t sz = size_t(256) << c;
+  if (in_use_ + int64_t(sz) > budget_) {
+    set_error("HBM budget exceeded");
+    return SVXG_OOM_BUDGET;
+  }
+  if (!free_[size_t(c)].empty()) {
+    *out = free_[size_t(c)].back();
+    free_[size_t(c)].pop_back();
+  } else {
+    if (bump_left_ < sz) {
+      void* p = nullptr;
+      hipError_t e = hipMalloc(&p, kSlab);
+      if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return SVXG_DEVICE_ERROR;
+      }
+      slabs_.push_back(p);
+      bump_ = static_cast<char*>(p);
+      bump_left_ = kSlab;
+    }
+    *out = bump_;
+    bump_ += sz;
+    bump_left_ -= sz;
+  }
+  in_use_ += int64_t(sz);
+  *held = sz;
+  return SVXG_OK;
+}
+
+void Arena::release(void* p, size_t held) {
+  std::lock_guard<std::mutex> lk(mu_);
+  in_use_ -= int64_t(held);
+  if (held >= kBig) {
+    for (size_t k = 0; k < big_.size(); ++k)
+      if (big_[k] == p) {
+        (void)hipFree(p);
+        big_[k] = big_.back();
+        big_.pop_back();
+        return;
+      }
+    return;
+  }
+  free_[size_t(size_class(held))].push_back(p);
+}
+
+// ---- tiles -----------------------------------------------------------------------------------------
+
+static void add_tiles(std::vector<Tile>& t, int g, int n, int per) {
+  for (int r = 0; r < n; r += per) t.push_back(Tile{g, r, std::min(per, n - r), int32_t(t.size())});
+}
+
+// (synchronous: the host vector may go out of scope when this returns)
+template <typename T>
+static int upload(DevBuf& d, Arena* a, const std::vector<T>& h, hipStream_t s) {
+  SVXG_TRY(d.make(a, std::max<size_t>(1, h.size()) * sizeof(T)));
+  if (!h.empty()) {
+    SVXG_HIP(hipMemcpyAsync(d.as<void>(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    SVXG_HIP(hipStreamSynchronize(s));
+  }
+  return SVXG_OK;
+}
+template <typename T>
+static int upload_into(const DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  if (!h.empty()) {
+    SVXG_HIP(hipMemcpyAsync(d.as<void>(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    SVXG_HIP(hipStreamSynchronize(s));
+  }
+  return SVXG_OK;
+}
+
+template <typename T>
+static int download(std::vector<T>& h, const DevBuf& d, size_t count, hipStream_t s) {
+  h.resize(count);
+  if (count) SVXG_HIP(hipMemcpyAsync(h.data(), d.as<void>(), count * sizeof(T), hipMemcpyDeviceToHost, s));
+  SVXG_HIP(hipStreamSynchronize(s));
+  return SVXG_OK;
+}
+
+}  // namespace svxg
+
+using namespace svxg;
+
+int64_t svxg_problem::device_bytes() const {
+  int64_t bytes = 0;
+  for (const DevBuf* d : {&d_nodes, &d_nodec, &d_bonds, &d_mats, &d_dof, &d_bond_slot, &d_cptr, &d_cidx, &d_row_gone,
+                          &d_id_scale, &x, &r, &z, &p, &q, &b, &ix, &ir, &iz, &ip, &iq, &u_all})
+    bytes += int64_t(d->held());
+  for (const HostLevel& L : lv)
+    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32,
+                            &L.Rrow, &L.Rfine, &L.Rent, &L.R64, &L.R32, &L.Minv, &L.x64[0], &L.x64[1], &L.b64,
+                            &L.r64, &L.d64, &L.x32[0], &L.x32[1], &L.b32, &L.r32, &L.d32, &L.pos, &L.lam})
+      bytes += int64_t(d->held());
+  return bytes;
+}
+
+namespace svxg {
+
+// ---- batches ----------------------------------------------------------------------------------------
+
+struct Batch {
+  std::vector<svxg_problem*> probs;
+  int G = 0;
+  int maxlev = 1;
+  std::vector<GraphDev> hG;
+  DevBuf dG, dstate;
+  std::vector<PcgState> hstate;
+  DevBuf t0;
+  int nt0 = 0;
+  DevBuf gtile;
+  DevBuf tj;
+  int ntj = 0;
+  struct Lv {
+    DevBuf ts, tc, tstall, dense;
+    int nts = 0, ntc = 0, ntstall = 0, ndense = 0;
+  };
+  std::vector<Lv> lvl;
+  DevBuf partial;
+  bool any_amg = false;
+};
+
+static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
+  std::memset(&g, 0, sizeof g);
+  g.nlev = P->jacobi_only ? 1 : int32_t(P->lv.size());
+  g.nfree = P->nfree;
+  g.jacobi = P->jacobi_only ? 1 : 0;
+  g.st = st;
+  g.x = P->x.as<double>();
+  g.r = P->r.as<double>();
+  g.z = P->z.as<double>();
+  g.p = P->p.as<double>();
+  g.q = P->q.as<double>();
+  g.b = P->b.as<double>();
+  g.ix = P->ix.as<float>();
+  g.ir = P->ir.as<float>();
+  g.iz = P->iz.as<float>();
+  g.ip = P->ip.as<float>();
+  g.iq = P->iq.as<float>();
+  for (size_t l = 0; l < P->lv.size() && l < size_t(kMaxLevels); ++l) {
+    HostLevel& H = P->lv[l];
+    LevelDev& L = g.lv[l];
+    L.n = H.n;
+    L.nc = H.nc;
+    L.terminal = H.terminal ? 1 : 0;
+    L.dense_m = H.dense_m;
+    L.nnzb = H.nnzb;
+    L.np = H.np;
+    L.rowptr = H.rowptr.as<int32_t>();
+    L.col = H.col.as<int32_t>();
+    L.Prow = H.Prow.as<int32_t>();
+    L.Pcol = H.Pcol.as<int32_t>();
+    L.Rrow = H.Rrow.as<int32_t>();
+    L.Rfine = H.Rfine.as<int32_t>();
+    L.Minv = H.Minv.as<double>();
+    L.lmax = H.lmax;
+    L.cheb_degree = P->opt.amg.cheb_degree;
+    L.f64.A = H.A64.as<double>();
+    L.f64.Dinv = H.D64.as<double>();
+    L.f64.P = H.P64.as<double>();
+    L.f64.R = H.R64.as<double>();
+    L.f64.x[0] = H.x64[0].as<double>();
+    L.f64.x[1] = l == 0 ? P->z.as<double>() : H.x64[1].as<double>();
+    L.f64.b = l == 0 ? P->r.as<double>() : H.b64.as<double>();
+    L.f64.r = H.r64.as<double>();
+    L.f64.d = H.d64.as<double>();
+    L.f32.A = H.A32.as<float>();
+    L.f32.Dinv = H.D32.as<float>();
+    L.f32.P = H.P32.as<float>();
+    L.f32.R = H.R32.as<float>();
+    L.f32.x[0] = H.x32[0].as<float>();
+    L.f32.x[1] = l == 0 ? P->iz.as<float>() : H.x32[1].as<float>();
+    L.f32.b = l == 0 ? P->ir.as<float>() : H.b32.as<float>();
+    L.f32.r = H.r32.as<float>();
+    L.f32.d = H.d32.as<float>();
+  }
+}
+
+static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A, hipStream_t s) {
+  B.probs.assign(probs, probs + count);
+  B.G = count;
+  SVXG_TRY(B.dstate.make(A, sizeof(PcgState) * size_t(std::max(1, count))));
+  B.hstate.assign(size_t(count), PcgState{});
+  B.hG.resize(size_t(count));
+  B.maxlev = 1;
+  B.any_amg = false;
+  for (int g = 0; g < count; ++g) {
+    fill_desc(probs[g], B.hG[size_t(g)], B.dstate.as<PcgState>() + g);
+    if (!probs[g]->jacobi_only) {
+      B.maxlev = std::max(B.maxlev, int(probs[g]->lv.size()));
+      B.any_amg = true;
+    }
+  }
+  SVXG_TRY(upload(B.dG, A, B.hG, s));
+  std::vector<Tile> t0, tj;
+  std::vector<int32_t> gt(static_cast<size_t>(count) + 1, 0);
+  for (int g = 0; g < count; ++g) {
+    gt[size_t(g)] = int32_t(t0.size());
+    add_tiles(t0, g, probs[g]->nfree, kTileRows);
+    if (probs[g]->jacobi_only) add_tiles(tj, g, probs[g]->nfree, kTileRows);
+  }
+  gt[size_t(count)] = int32_t(t0.size());
+  B.nt0 = int(t0.size());
+  B.ntj = int(tj.size());
+  SVXG_TRY(upload(B.t0, A, t0, s));
+  SVXG_TRY(upload(B.tj, A, tj, s));
+  SVXG_TRY(upload(B.gtile, A, gt, s));
+  SVXG_TRY(B.partial.make(A, sizeof(double) * 2 * size_t(std::max(1, B.nt0))));
+  B.lvl.clear();
+  B.lvl.resize(size_t(B.maxlev));
+  for (int l = 0; l < B.maxlev; ++l) {
+    std::vector<Tile> ts, tc, tstall;
+    std::vector<int32_t> dense;
+    for (int g = 0; g < count; ++g) {
+      svxg_problem* P = probs[g];
+      if (P->jacobi_only || l >= int(P->lv.size())) continue;
+      const HostLevel& H = P->lv[size_t(l)];
+      if (!H.terminal) {
+        add_tiles(ts, g, H.n, kTileRows);
+        add_tiles(tc, g, P->lv[size_t(l) + 1].n, kTileRows);
+      } else if (H.dense_m > 0) {
+        dense.push_back(g);
+      } else {
+        add_tiles(tstall, g, H.n, kTileRows);
+      }
+    }
+    Batch::Lv& V = B.lvl[size_t(l)];
+    V.nts = int(ts.size());
+    V.ntc = int(tc.size());
+    V.ntstall = int(tstall.size());
+    V.ndense = int(dense.size());
+    SVXG_TRY(upload(V.ts, A, ts, s));
+    SVXG_TRY(upload(V.tc, A, tc, s));
+    SVXG_TRY(upload(V.tstall, A, tstall, s));
+    SVXG_TRY(upload(V.dense, A, dense, s));
+  }
+  return SVXG_OK;
+}
+
+// ---- kernel timing ----------------------------------------------------------------------------------
+
+// algorithmic bytes of one launch over the batch: the outer K product (K's blocks and column
+// indices, row pointers, p gathered once, q written) and a level-0 Chebyshev step (the same matrix
+// traffic, plus b, D^-1, d read and written, x written)
+static double prof_bytes(const Batch& B, int kind) {
+  const double w = (kind & 1) ? 4.0 : 8.0;
+  double s = 0.0;
+  for (const svxg_problem* P : B.probs) {
+    if (P->lv.empty()) continue;
+    const double nb = double(P->lv[0].nnzb), n = double(P->lv[0].n);
+    const double mat = nb * (36.0 * w + 4.0) + 4.0 * (n + 1.0);
+    s += kind < 2 ? mat + 2.0 * 6.0 * n * w : mat + 36.0 * n * w + 5.0 * 6.0 * n * w;
+  }
+  return s;
+}
+
+struct ProfScope {
+  svxg_ctx* c;
+  int kind;
+  hipStream_t s;
+  hipEvent_t e1 = nullptr;
+  ProfScope(const Batch& B, int k, hipStream_t st) : c(B.probs[0]->ctx), kind(k), s(st) {
+    if (!c->profiling || kind < 0) return;
+    auto& P = c->prof[kind];
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!P.spare.empty()) {
+      ev = P.spare.back();
+      P.spare.pop_back();
+    } else {
+      (void)hipEventCreate(&ev.first);
+      (void)hipEventCreate(&ev.second);
+    }
+    (void)hipEventRecord(ev.first, s);
+    P.pending.push_back(ev);
+    P.pending_bytes += prof_bytes(B, kind);
+    e1 = ev.second;
+  }
+  ~ProfScope() {
+    if (e1) (void)hipEventRecord(e1, s);
+  }
+};
+
+// after a synchronisation: the pending pairs' times added up
+static void prof_collect(svxg_ctx* c) {
+  if (!c->profiling) return;
+  for (auto& P : c->prof) {
+    for (auto& ev : P.pending) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+        P.ms += ms;
+        ++P.launches;
+      }
+      P.spare.push_back(ev);
+    }
+    P.bytes += P.pending_bytes;
+    P.pending_bytes = 0.0;
+    P.pending.clear();
+  }
+}
+
+// ---- the V-cycle -----------------------------------------------------------------------------------
+
+constexpr int kStallDegree = 8;  // Chebyshev steps of a coarsest level left without a dense solve
+
+template <typename T>
+static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
+  const GraphDev* G = B.dG.as<GraphDev>();
+  if (B.ntj) launch_jacobi<T>(s, G, B.tj.as<Tile>(), B.ntj, inner);
+  if (!B.any_amg) return;
+  const int L = B.maxlev;
+  for (int l = 0; l < L; ++l) {
+    Batch::Lv& V = B.lvl[size_t(l)];
+    if (V.nts) {
+      for (int st = 0; st < degree; ++st) {
+        ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
+        launch_cheb_step<T>(s, G, V.ts.as<Tile>(), V.nts, l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr);
+      }
+      launch_residual<T>(s, G, V.ts.as<Tile>(), V.nts, l, (degree - 1) & 1, inner);
+      launch_restrict<T>(s, G, V.tc.as<Tile>(), V.ntc, l, inner);
+    }
+    if (V.ndense) launch_dense_solve<T>(s, G, V.dense.as<int32_t>(), V.ndense, l, inner);
+    if (V.ntstall)
+      for (int st = 0; st < kStallDegree; ++st)
+        launch_cheb_step<T>(s, G, V.tstall.as<Tile>(), V.ntstall, l, st, (st + 1) & 1, st & 1, st == 0, inner,
+                            nullptr);
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    Batch::Lv& V = B.lvl[size_t(l)];
+    if (!V.nts) continue;
+    const int cur = (degree - 1) & 1;
+    launch_prolong<T>(s, G, V.ts.as<Tile>(), V.nts, l, cur, inner);
+    for (int j = 0; j < degree; ++j) {
+      const int gs = degree + j;
+      ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
+      launch_cheb_step<T>(s, G, V.ts.as<Tile>(), V.nts, l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr);
+    }
+  }
+}
+
+static void prof_collect(svxg_ctx* c);
+static int read_states(Batch& B, hipStream_t s) {
+  SVXG_HIP(hipMemcpyAsync(B.hstate.data(), B.dstate.as<void>(), sizeof(PcgState) * size_t(B.G), hipMemcpyDeviceToHost, s));
+  SVXG_HIP(hipStreamSynchronize(s));
+  if (!B.probs.empty()) prof_collect(B.probs[0]->ctx);
+  return SVXG_OK;
+}
+
+static bool any_running(const Batch& B, bool inner) {
+  for (const PcgState& st : B.hstate)
+    if (st.state == ST_RUNNING && (!inner || st.inner_state == ST_RUNNING)) return true;
+  return false;
+}
+
+constexpr int kPollEvery = 8;
+
+static void drop_self_batch_ptrs(svxg_problem* P);
+
+// the FP64 PCG iterations of StressProblem::iterate (stress.cpp:403-464) over a batch (after
+// run_begin), each graph with its own scalars and stopping point
+static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
+  const GraphDev* G = B.dG.as<GraphDev>();
+  const Tile* t0 = B.t0.as<Tile>();
+  const int32_t* gt = B.gtile.as<int32_t>();
+  double* part = B.partial.as<double>();
+  const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
+  for (int it = 0;; ++it) {
+    if (it % kPollEvery == 0) {
+      SVXG_TRY(read_states(B, s));
+      if (!any_running(B, false)) break;
+    }
+    {
+      ProfScope ps(B, 0, s);
+      launch_pcg_spmv<double>(s, G, t0, B.nt0, 0, part);
+    }
+    launch_finish(s, FinishKind::Alpha, G, gt, B.G, 0, part);
+    launch_axpy_rr<double>(s, G, t0, B.nt0, 0, part);
+    launch_finish(s, FinishKind::Rn, G, gt, B.G, 0, part);
+    vcycle<double>(B, s, 0, degree);
+    launch_dot_rz<double>(s, G, t0, B.nt0, 0, part);
+    launch_finish(s, FinishKind::Beta, G, gt, B.G, 0, part);
+    launch_update_p<double>(s, G, t0, B.nt0, 0);
+  }
+  SVXG_HIP(hipGetLastError());
+  return SVXG_OK;
+}
+
+// FP64 iterative refinement around an FP32 PCG (FP32 K products and V-cycle, FP64 dots): each
+// outer step recomputes r = b - K x in FP64, solves K d = r to max(eta |r|, target / 2) in FP32 and
+// adds d to x
+static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
+  const GraphDev* G = B.dG.as<GraphDev>();
+  const Tile* t0 = B.t0.as<Tile>();
+  const int32_t* gt = B.gtile.as<int32_t>();
+  double* part = B.partial.as<double>();
+  const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
+  for (int outer = 0; outer < 100; ++outer) {
+    launch_outer_residual(s, G, t0, B.nt0, part);
+    launch_finish(s, FinishKind::Outer, G, gt, B.G, 0, part);
+    SVXG_TRY(read_states(B, s));
+    if (!any_running(B, false)) break;
+    launch_inner_init(s, G, t0, B.nt0);
+    vcycle<float>(B, s, 1, degree);
+    launch_start_p<float>(s, G, t0, B.nt0, 1, part);
+    launch_finish(s, FinishKind::Start, G, gt, B.G, 1, part);
+    for (int it = 0;; ++it) {
+      if (it % kPollEvery == 0) {
+        SVXG_TRY(read_states(B, s));
+        if (!any_running(B, true)) break;
+      }
+      {
+        ProfScope ps(B, 1, s);
+        launch_pcg_spmv<float>(s, G, t0, B.nt0, 1, part);
+      }
+      launch_finish(s, FinishKind::Alpha, G, gt, B.G, 1, part);
+      launch_axpy_rr<float>(s, G, t0, B.nt0, 1, part);
+      launch_finish(s, FinishKind::Rn, G, gt, B.G, 1, part);
+      vcycle<float>(B, s, 1, degree);
+      launch_dot_rz<float>(s, G, t0, B.nt0, 1, part);
+      launch_finish(s, FinishKind::Beta, G, gt, B.G, 1, part);
+      launch_update_p<float>(s, G, t0, B.nt0, 1);
+    }
+    launch_inner_apply(s, G, t0, B.nt0);
+    launch_inner_close(s, G, B.G);
+  }
+  SVXG_HIP(hipGetLastError());
+  return SVXG_OK;
+}
+
+// the head of StressProblem::iterate on a resumed run: |r| against the (new) target
+static int recheck_residual(Batch& B, hipStream_t s) {
+  launch_dot_rr<double>(s, B.dG.as<GraphDev>(), B.t0.as<Tile>(), B.nt0, 0, B.partial.as<double>());
+  launch_finish(s, FinishKind::Rn, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.G, 0, B.partial.as<double>());
+  SVXG_HIP(hipGetLastError());
+  return SVXG_OK;
+}
+
+}  // namespace svxg
+
+// the rest of the host side (plan, setup, ABI) is in host_setup.hip and host_abi.hip
+#include "host_setup.inc"
+#include "host_abi.inc"
+
+namespace svxg {
+// (a kept batch-of-one holds device pointers: rebuilt when K's arrays change)
+static void drop_self_batch_ptrs(svxg_problem* P) { drop_self_batch(P); }
+}  // namespace svxg

@@ -1,0 +1,1257 @@
+// Setup-phase kernels of the svxg stress solve, and the bond checks.
+//
+// Assembly follows StressProblem::build_matrix / bond_matrices (core/src/stress/stress.cpp:59-132)
+// and blk6::atbd_add / rigid_block (core/src/solve/amg.cpp:143-170) in the same floating-point
+// operation order (this file is compiled with -ffp-contract=off), and sums each block's
+// contributions in bond order as Bsr6Builder does: K is bit-identical to the reference's.
+//
+// The multigrid setup is the reference's smoothed aggregation (amg.cpp:228-533) made parallel:
+//   strength        ||A_ij|| >= theta sqrt(||A_ii|| ||A_jj||) (symmetrised: max of ij and ji)
+//   aggregation     roots are a distance-2 maximal independent set of the strong graph (the
+//                   parallel form of the reference's greedy pass 1, which also keeps roots three
+//                   links apart); members join their root; pass 2 joins the strongest aggregated
+//                   neighbour (amg.cpp:350-366); pass 3 makes singletons and groups uncoupled rows
+//                   eight at a time (amg.cpp:367-384); theta halves up to three times while
+//                   2 na > n (amg.cpp:385-388)
+//   prolongator     P = (I - w D^-1 A) P_rigid, w = 4/3 / lambda_max (amg.cpp:391-452), the
+//                   reference's block product order
+//   Galerkin        A P by fine row, then P^T (A P) by coarse row, each block summed in the
+//                   reference's order (amg.cpp:472-529)
+//   coarsest        dense Cholesky with the reference's symmetrisation and pivot floor
+//                   (amg.cpp:535-578), then its explicit inverse for a parallel apply
+// Bond checks: bond_load + bond_utilization (stress.cpp:14-57, 477-502), same operation order.
+#include "svxg_kernels.hpp"
+
+namespace svxg {
+
+namespace {
+
+constexpr int kBondsPerWG = 7;  // 36 lanes a bond / a block
+
+__device__ __forceinline__ const svxg_material& mat_of(const svxg_material* m, int nm, int id) {
+  return (id >= 0 && id < nm) ? m[id] : m[0];
+}
+
+// element (r, c) of rigid_block(r): [[I, -[r]x], [0, I]] (amg.cpp:143-153)
+__device__ __forceinline__ double rigid_at(const double r[3], int R, int C) {
+  if (R == C) return 1.0;
+  const int e = R * 6 + C;
+  switch (e) {
+    case 4: return r[2];
+    case 5: return -r[1];
+    case 9: return -r[2];
+    case 11: return r[0];
+    case 15: return r[1];
+    case 16: return -r[0];
+    default: return 0.0;
+  }
+}
+
+// the bond's local stiffness in the global frame, element (R, C) (stress.cpp:59-79)
+__device__ double bond_D(const svxg_bond& b, const svxg_material& A, const svxg_material& B, int R, int C) {
+  const bool t = R < 3 && C < 3, rr = R >= 3 && C >= 3;
+  if (!t && !rr) return 0.0;
+  const double la = fmax(b.la, 1e-4), lb = fmax(b.lb, 1e-4);
+  const double invE = la / A.E + lb / B.E;
+  const double invG = la / A.G + lb / B.G;
+  double k[3];
+  if (t) {
+    k[0] = b.area / invE;
+    k[1] = (5.0 / 6.0) * b.area / invG;
+    k[2] = (5.0 / 6.0) * b.area / invG;
+  } else {
+    k[0] = 0.8 * (b.s1 + b.s2) / invG;
+    k[1] = b.s2 / invE;
+    k[2] = b.s1 / invE;
+  }
+  const int r = t ? R : R - 3, c = t ? C : C - 3;
+  const double* Q[3] = {b.n, b.t1, b.t2};
+  double s = 0.0;
+  for (int q = 0; q < 3; ++q) s += Q[q][r] * k[q] * Q[q][c];
+  return s;
+}
+
+// element (R, C) of s * L^T (M X) (blk6::atbd_add, amg.cpp:155-170) from 6x6 blocks in LDS
+__device__ __forceinline__ double atbd_elem(const double* L, const double* M, const double* X, int R, int C) {
+  double T[6];
+  for (int k = 0; k < 6; ++k) {
+    double acc = 0.0;
+    for (int j = 0; j < 6; ++j) acc += M[k * 6 + j] * X[j * 6 + C];
+    T[k] = acc;
+  }
+  double acc = 0.0;
+  for (int k = 0; k < 6; ++k) acc += L[k * 6 + R] * T[k];
+  return acc;
+}
+
+// a bond's D, Ba, Bb into LDS (lane e of the bond's 36 computes element e of each)
+__device__ void bond_blocks_lds(const svxg_bond& b, const double* nodec, const svxg_material* mats, int nmats, int e,
+                                double* sD, double* sA, double* sB) {
+  const svxg_material& MA = mat_of(mats, nmats, b.ma);
+  const svxg_material& MB = mat_of(mats, nmats, b.mb);
+  const int R = e / 6, C = e % 6;
+  sD[e] = bond_D(b, MA, MB, R, C);
+  const double* ca = nodec + 3 * size_t(b.a);
+  const double ra[3] = {b.p[0] - ca[0], b.p[1] - ca[1], b.p[2] - ca[2]};
+  sA[e] = rigid_at(ra, R, C);
+  if (b.b >= 0) {
+    const double* cb = nodec + 3 * size_t(b.b);
+    const double rb[3] = {b.p[0] - cb[0], b.p[1] - cb[1], b.p[2] - cb[2]};
+    sB[e] = rigid_at(rb, R, C);
+  } else {
+    sB[e] = 0.0;
+  }
+}
+
+// the 6x6 Gauss-Jordan inverse with partial pivoting of blk6::inv6 (amg.cpp:105-135)
+__device__ bool inv6(const double* A, double* out) {
+  double W[6][12];
+  double big = 0.0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 12; ++c) {
+      W[r][c] = c < 6 ? A[r * 6 + c] : (c - 6 == r ? 1.0 : 0.0);
+      if (c < 6) big = fmax(big, fabs(W[r][c]));
+    }
+  if (big == 0.0) return false;
+  for (int c = 0; c < 6; ++c) {
+    int best = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(W[r][c]) > fabs(W[best][c])) best = r;
+    if (fabs(W[best][c]) <= 1e-15 * big) return false;
+    if (best != c)
+      for (int k = 0; k < 12; ++k) {
+        const double t = W[c][k];
+        W[c][k] = W[best][k];
+        W[best][k] = t;
+      }
+    const double d = 1.0 / W[c][c];
+    for (int k = 0; k < 12; ++k) W[c][k] *= d;
+    for (int r = 0; r < 6; ++r) {
+      if (r == c) continue;
+      const double m = W[r][c];
+      if (m == 0.0) continue;
+      for (int k = 0; k < 12; ++k) W[r][k] -= m * W[c][k];
+    }
+  }
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) out[r * 6 + c] = W[r][6 + c];
+  return true;
+}
+
+// Frobenius norm summed in an order symmetric under transposition (diagonal, then each pair
+// (rc, cr) together): ||A|| and ||A^T|| come out bitwise equal
+__device__ __forceinline__ double fro_sym(const double* M) {
+  double s = 0.0;
+  for (int r = 0; r < 6; ++r) s += M[r * 7] * M[r * 7];
+  for (int r = 0; r < 6; ++r)
+    for (int c = r + 1; c < 6; ++c) s += M[r * 6 + c] * M[r * 6 + c] + M[c * 6 + r] * M[c * 6 + r];
+  return sqrt(s);
+}
+
+// position of column c in row r (diagonal first, then ascending), -1 if absent
+__device__ __forceinline__ int find_block(const int32_t* rowptr, const int32_t* col, int r, int c) {
+  int lo = rowptr[r], hi = rowptr[r + 1];
+  if (lo >= hi) return -1;
+  if (col[lo] == c) return lo;
+  ++lo;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (col[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < rowptr[r + 1] && col[lo] == c) ? lo : -1;
+}
+
+__device__ __forceinline__ void sort_ints(int32_t* a, int n) {
+  for (int i = 1; i < n; ++i) {
+    const int32_t v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = v;
+  }
+}
+
+__device__ __forceinline__ int unique_sorted(int32_t* a, int n) {
+  if (n == 0) return 0;
+  int u = 1;
+  for (int i = 1; i < n; ++i)
+    if (a[i] != a[u - 1]) a[u++] = a[i];
+  return u;
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+constexpr int32_t MIS_UND = 0, MIS_ROOT = 1, MIS_EXC = 2;
+
+__device__ __forceinline__ uint64_t mis_tuple(int state, int i) {
+  if (state == MIS_EXC) return 0ull;
+  const uint64_t rank = state == MIS_ROOT ? 3ull : 2ull;
+  const uint64_t pr = hash32(uint32_t(i) * 2654435761U + 12345U) & 0x7fffffffU;
+  return (rank << 62) | (pr << 31) | uint64_t(uint32_t(i));
+}
+
+// exclusive scan of a[0..len) in place, a[len] = total, one workgroup
+__device__ void block_scan(int32_t* a, int len, int32_t* total_out) {
+  __shared__ int32_t buf[kWG];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < len; base += kWG) {
+    const int i = base + int(threadIdx.x);
+    const int32_t v = i < len ? a[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kWG; o <<= 1) {
+      const int32_t t = threadIdx.x >= unsigned(o) ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < len) a[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == kWG - 1) carry += buf[kWG - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a[len] = carry;
+    if (total_out) *total_out = carry;
+  }
+}
+
+}  // namespace
+
+// ---- assembly -------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kWG) k_bond_products(const AssemblyDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                       int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const AssemblyDev& a = J[tl.g];
+  __shared__ double sD[kBondsPerWG][36], sA[kBondsPerWG][36], sB[kBondsPerWG][36];
+  const int lb = threadIdx.x / 36, e = threadIdx.x % 36;
+  const bool on = lb < kBondsPerWG && lb < tl.nrows;
+  const int bi = tl.row0 + (on ? lb : 0);
+  const int32_t* sl = a.bond_slot + 4 * size_t(bi);
+  const bool used = on && (sl[0] >= 0 || sl[1] >= 0 || sl[2] >= 0 || sl[3] >= 0);
+  if (used) bond_blocks_lds(a.bonds[bi], a.nodec, a.mats, a.nmats, e, sD[lb], sA[lb], sB[lb]);
+  __syncthreads();
+  if (!used) return;
+  const int R = e / 6, C = e % 6;
+  double* out = a.prod + 144 * size_t(bi);
+  if (sl[0] >= 0) out[e] = 1.0 * atbd_elem(sA[lb], sD[lb], sA[lb], R, C);
+  if (sl[1] >= 0) out[36 + e] = 1.0 * atbd_elem(sB[lb], sD[lb], sB[lb], R, C);
+  if (sl[2] >= 0) out[72 + e] = -1.0 * atbd_elem(sA[lb], sD[lb], sB[lb], R, C);
+  if (sl[3] >= 0) out[108 + e] = -1.0 * atbd_elem(sB[lb], sD[lb], sA[lb], R, C);
+}
+
+// each block: the sum of its contributions in bond order, from zero (Bsr6Builder)
+__global__ void __launch_bounds__(kWG) k_gather_blocks(const AssemblyDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                       int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const AssemblyDev& a = J[tl.g];
+  const int lb = threadIdx.x / 36, e = threadIdx.x % 36;
+  if (lb >= kBondsPerWG || lb >= tl.nrows) return;
+  const int k = tl.row0 + lb;
+  double v = 0.0;
+  for (int c = a.cptr[k]; c < a.cptr[k + 1]; ++c) {
+    const int32_t ci = a.cidx[c];
+    v += a.prod[144 * size_t(ci >> 2) + 36 * size_t(ci & 3) + e];
+  }
+  a.K[36 * size_t(k) + e] = v;
+}
+
+__global__ void k_diag_trace(const AssemblyDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const AssemblyDev& a = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const double* d = a.K + 36 * size_t(a.rowptr[i]);
+  a.trace[i] = d[0] + d[7] + d[14];
+}
+
+// id_scale = mean of tr(K_ii[0:3,0:3]) / 3 over rows with a positive trace, in row order
+__global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= count) return;
+  const AssemblyDev& a = J[g];
+  double tr = 0.0;
+  int cnt = 0;
+  for (int i = 0; i < a.nfree; ++i) {
+    const double t = a.trace[i];
+    if (t > 0) {
+      tr += t / 3.0;
+      ++cnt;
+    }
+  }
+  *a.id_scale = cnt ? tr / cnt : 1.0;
+}
+
+__global__ void k_identity_rows(const AssemblyDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const AssemblyDev& a = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  double* d = a.K + 36 * size_t(a.rowptr[i]);
+  const bool empty = d[0] == 0.0 && d[7] == 0.0 && d[14] == 0.0;
+  if (a.row_gone[i] || empty) {
+    const double s = *a.id_scale;
+    for (int q = 0; q < 6; ++q) d[q * 7] = s;
+  }
+}
+
+// bonds leave K in list order (one workgroup a graph): each one's products subtracted in place
+__global__ void __launch_bounds__(kWG) k_remove_bonds(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ ids,
+                                                      int count) {
+  const AssemblyDev& a = J[0];
+  __shared__ double sD[36], sA[36], sB[36];
+  const int e = threadIdx.x;
+  for (int t = 0; t < count; ++t) {
+    const int bi = ids[t];
+    const int32_t* sl = a.bond_slot + 4 * size_t(bi);
+    if (e < 36) bond_blocks_lds(a.bonds[bi], a.nodec, a.mats, a.nmats, e, sD, sA, sB);
+    __syncthreads();
+    if (e < 36) {
+      const int R = e / 6, C = e % 6;
+      // (stress.cpp:269-274: aa, bb subtracted, ab, ba added back: C + (-1) acc, C + (+1) acc)
+      if (sl[0] >= 0) a.K[36 * size_t(sl[0]) + e] += -1.0 * atbd_elem(sA, sD, sA, R, C);
+      if (sl[1] >= 0) a.K[36 * size_t(sl[1]) + e] += -1.0 * atbd_elem(sB, sD, sB, R, C);
+      if (sl[2] >= 0) a.K[36 * size_t(sl[2]) + e] += 1.0 * atbd_elem(sA, sD, sB, R, C);
+      if (sl[3] >= 0) a.K[36 * size_t(sl[3]) + e] += 1.0 * atbd_elem(sB, sD, sA, R, C);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_retire_rows(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ rows, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 36 * count) return;
+  const AssemblyDev& a = J[0];
+  const int i = rows[t / 36], e = t % 36;
+  a.K[36 * size_t(a.rowptr[i]) + e] = (e % 7 == 0) ? *a.id_scale : 0.0;
+}
+
+__global__ void k_inv_diag(const DiagDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const DiagDev& a = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const double* D = a.A + 36 * size_t(a.rowptr[i]);
+  double* out = a.Dinv + 36 * size_t(i);
+  double Dl[36], O[36];
+  for (int q = 0; q < 36; ++q) Dl[q] = D[q];
+  if (inv6(Dl, O)) {
+    for (int q = 0; q < 36; ++q) out[q] = O[q];
+    return;
+  }
+  if (a.mode == 1) {
+    const double s = 1.0 / *a.id_scale;
+    for (int q = 0; q < 36; ++q) out[q] = (q % 7 == 0) ? s : 0.0;
+    return;
+  }
+  double tr = 0.0;
+  for (int q = 0; q < 6; ++q) tr += fabs(Dl[q * 7]);
+  const double eps = tr > 0 ? 1e-9 * tr : 1.0;
+  for (int q = 0; q < 6; ++q) Dl[q * 7] += eps;
+  if (inv6(Dl, O)) {
+    for (int q = 0; q < 36; ++q) out[q] = O[q];
+  } else {
+    for (int q = 0; q < 36; ++q) out[q] = 0.0;
+    atomicAdd(a.fail, 1);
+  }
+}
+
+// ---- aggregation ------------------------------------------------------------------------------------
+
+// block strengths (and the diagonal's)
+__global__ void k_strength(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) c.s[k] = fro_sym(c.A + 36 * size_t(k));
+}
+
+// strong couplings (symmetric), initial MIS states (uncoupled rows: excluded, grouped later)
+__global__ void k_strong(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
+  const double di = c.s[k0];
+  for (int k = k0; k < k1; ++k) {
+    const int j = c.col[k];
+    if (j == i) {
+      c.strong[k] = 0;
+      continue;
+    }
+    const int kt = find_block(c.rowptr, c.col, j, i);
+    const double sk = kt >= 0 ? fmax(c.s[k], c.s[kt]) : c.s[k];
+    const double dj = c.s[c.rowptr[j]];
+    c.strong[k] = sk >= c.theta * sqrt(di * dj) ? 1 : 0;
+  }
+  c.state[i] = (k1 - k0 <= 1) ? MIS_EXC : MIS_UND;
+}
+
+__global__ void k_mis_t1(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  uint64_t m = mis_tuple(c.state[i], i);
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k)
+    if (c.strong[k]) {
+      const int j = c.col[k];
+      const uint64_t t = mis_tuple(c.state[j], j);
+      m = t > m ? t : m;
+    }
+  c.t1[i] = m;
+}
+
+__global__ void k_mis_t2(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  uint64_t m = c.t1[i];
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k)
+    if (c.strong[k]) {
+      const uint64_t t = c.t1[c.col[k]];
+      m = t > m ? t : m;
+    }
+  c.t2[i] = m;
+}
+
+__global__ void k_mis_update(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  __shared__ int und;
+  if (threadIdx.x == 0) und = 0;
+  __syncthreads();
+  if (int(threadIdx.x) < tl.nrows) {
+    const int i = tl.row0 + threadIdx.x;
+    if (c.state[i] == MIS_UND) {
+      const uint64_t m = c.t2[i];
+      if (m == mis_tuple(MIS_UND, i)) c.state[i] = MIS_ROOT;
+      else if ((m >> 62) == 3ull) c.state[i] = MIS_EXC;
+      else atomicAdd(&und, 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && und) atomicAdd(&c.counts[0], und);
+}
+
+// pass 1: a root is its own aggregate's anchor; its strong neighbours join it
+__global__ void k_agg_pass1(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  int root = -1;
+  if (c.state[i] == MIS_ROOT) {
+    root = i;
+  } else {
+    uint64_t best = 0;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      if (c.strong[k] && c.state[j] == MIS_ROOT) {
+        const uint64_t t = mis_tuple(MIS_ROOT, j);
+        if (t > best) {
+          best = t;
+          root = j;
+        }
+      }
+    }
+  }
+  c.rootof[i] = root;
+}
+
+// pass 2 (on pass 1's result): the strongest aggregated neighbour's aggregate; then the anchor
+// (roots, singletons) and uncoupled-row flags for the numbering scans
+__global__ void k_agg_pass2(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
+  const bool lonely = k1 - k0 <= 1;
+  int root = c.rootof[i];
+  if (root < 0 && !lonely) {
+    double best = -1.0;
+    for (int k = k0; k < k1; ++k) {
+      const int j = c.col[k];
+      if (j == i || c.rootof[j] < 0) continue;
+      if (c.s[k] > best) {
+        best = c.s[k];
+        root = c.rootof[j];
+      }
+    }
+  }
+  c.rootof2[i] = root;
+  c.scan_a[i] = (c.state[i] == MIS_ROOT || (root < 0 && !lonely)) ? 1 : 0;
+  c.scan_b[i] = lonely ? 1 : 0;
+}
+
+__global__ void k_scan_pair(const CoarsenDev* __restrict__ J, int count) {
+  if (int(blockIdx.x) >= count) return;
+  const CoarsenDev& c = J[blockIdx.x];
+  block_scan(c.scan_a, c.n, &c.counts[1]);
+  __syncthreads();
+  block_scan(c.scan_b, c.n, &c.counts[2]);
+}
+
+// aggregate ids: anchors in row order, then the uncoupled rows' groups of eight
+__global__ void k_assign(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int nanchor = c.scan_a[c.n];
+  int a;
+  if (c.scan_b[i + 1] - c.scan_b[i]) a = nanchor + c.scan_b[i] / 8;
+  else if (c.scan_a[i + 1] - c.scan_a[i]) a = c.scan_a[i];
+  else a = c.scan_a[c.rootof2[i]];
+  c.agg[i] = a;
+}
+
+__global__ void k_member_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  atomicAdd(&c.acnt[c.agg[tl.row0 + threadIdx.x]], 1);
+}
+
+__global__ void k_scan_acnt(const CoarsenDev* __restrict__ J, int count) {
+  if (int(blockIdx.x) >= count) return;
+  const CoarsenDev& c = J[blockIdx.x];
+  block_scan(c.acnt, c.na, nullptr);
+}
+
+__global__ void k_member_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int a = c.agg[i];
+  const int at = atomicAdd(&c.cursor[a], 1);
+  c.amem[c.acnt[a] + at] = i;
+}
+
+// coarse positions: the members' mean (summed in row order, times 1 / count: amg.cpp:391-398)
+__global__ void k_coarse_pos(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int a = tl.row0 + threadIdx.x;
+  int32_t* m = c.amem + c.acnt[a];
+  const int cnt = c.acnt[a + 1] - c.acnt[a];
+  sort_ints(m, cnt);
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int t = 0; t < cnt; ++t) {
+    const double* p = c.pos + 3 * size_t(m[t]);
+    x += p[0];
+    y += p[1];
+    z += p[2];
+  }
+  const double s = 1.0 / cnt;
+  c.posc[3 * size_t(a)] = x * s;
+  c.posc[3 * size_t(a) + 1] = y * s;
+  c.posc[3 * size_t(a) + 2] = z * s;
+}
+
+__global__ void k_offsets(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const double* pc = c.posc + 3 * size_t(c.agg[i]);
+  for (int q = 0; q < 3; ++q) c.off[3 * size_t(i) + q] = c.pos[3 * size_t(i) + q] - pc[q];
+}
+
+// ---- prolongator ---------------------------------------------------------------------------------
+
+// candidates: the aggregates of the row's couplings (smoothed) or of the row itself, sorted,
+// unique, in cand[rowptr[i] ...]; Prow[i] = their count
+__global__ void k_p_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
+  int32_t* cd = c.cand + k0;
+  int n = 0;
+  if (c.omega != 0.0) {
+    for (int k = k0; k < k1; ++k) cd[n++] = c.agg[c.col[k]];
+  } else {
+    cd[n++] = c.agg[i];
+  }
+  sort_ints(cd, n);
+  c.Prow[i] = unique_sorted(cd, n);
+}
+
+__global__ void k_scan_rows(const CoarsenDev* __restrict__ J, int count, int which) {
+  if (int(blockIdx.x) >= count) return;
+  const CoarsenDev& c = J[blockIdx.x];
+  switch (which) {
+    case 0: block_scan(c.Prow, c.n, &c.counts[4]); break;
+    case 1: block_scan(c.Rrow, c.na, nullptr); break;
+    case 2: block_scan(c.cptr, c.n, &c.counts[7]); break;
+    case 3: block_scan(c.AProw, c.n, &c.counts[5]); break;
+    case 4: block_scan(c.cptr, c.na, &c.counts[7]); break;
+    case 5: block_scan(c.Crow, c.na, &c.counts[6]); break;
+  }
+}
+
+// P's blocks: (J == agg(i) ? P_rigid,i : 0) - w sum over couplings k of row i with agg(col k) = J of
+// D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order
+__global__ void k_p_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
+  const int32_t* cd = c.cand + k0;
+  const int nu = c.Prow[i + 1] - c.Prow[i];
+  const double* Di = c.Dinv + 36 * size_t(i);
+  const double* offi = c.off + 3 * size_t(i);
+  const int ai = c.agg[i];
+  for (int u = 0; u < nu; ++u) {
+    const int Jc = cd[u];
+    const size_t e = size_t(c.Prow[i] + u);
+    c.Pcol[e] = Jc;
+    double acc[36];
+    for (int q = 0; q < 36; ++q) acc[q] = (Jc == ai) ? rigid_at(offi, q / 6, q % 6) : 0.0;
+    if (c.omega != 0.0) {
+      for (int k = k0; k < k1; ++k) {
+        const int j = c.col[k];
+        if (c.agg[j] != Jc) continue;
+        const double* A = c.A + 36 * size_t(k);
+        const double* offj = c.off + 3 * size_t(j);
+        for (int col = 0; col < 6; ++col) {
+          double T[6];
+          for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+            for (int p = 0; p < 6; ++p) s += A[r * 6 + p] * rigid_at(offj, p, col);
+            T[r] = s;
+          }
+          for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+            for (int p = 0; p < 6; ++p) s += Di[r * 6 + p] * T[p];
+            acc[r * 6 + col] += -c.omega * s;
+          }
+        }
+      }
+    }
+    double* out = c.P + 36 * e;
+    for (int q = 0; q < 36; ++q) out[q] = acc[q];
+  }
+}
+
+// restriction lists: P's entries by coarse column, fine rows ascending
+__global__ void k_r_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  for (int e = c.Prow[i]; e < c.Prow[i + 1]; ++e) atomicAdd(&c.Rrow[c.Pcol[e]], 1);
+}
+
+__global__ void k_r_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  for (int e = c.Prow[i]; e < c.Prow[i + 1]; ++e) {
+    const int I = c.Pcol[e];
+    const int at = c.Rrow[I] + atomicAdd(&c.cursor[I], 1);
+    c.Rfine[at] = i;
+    c.Rent[at] = e;
+  }
+}
+
+// per coarse row: entries sorted by fine row (the pairs move together), R blocks = P blocks^T
+__global__ void k_r_sort(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
+  if (int(blockIdx.x) >= nctiles) return;
+  const Tile tl = ctiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int I = tl.row0 + threadIdx.x;
+  const int b = c.Rrow[I], n = c.Rrow[I + 1] - b;
+  int32_t* f = c.Rfine + b;
+  int32_t* en = c.Rent + b;
+  for (int s = 1; s < n; ++s) {
+    const int32_t fv = f[s], ev = en[s];
+    int t = s - 1;
+    while (t >= 0 && f[t] > fv) {
+      f[t + 1] = f[t];
+      en[t + 1] = en[t];
+      --t;
+    }
+    f[t + 1] = fv;
+    en[t + 1] = ev;
+  }
+  for (int s = 0; s < n; ++s) {
+    const double* P = c.P + 36 * size_t(en[s]);
+    double* R = c.R + 36 * size_t(b + s);
+    for (int r = 0; r < 6; ++r)
+      for (int q = 0; q < 6; ++q) R[r * 6 + q] = P[q * 6 + r];
+  }
+}
+
+// ---- Galerkin product ----------------------------------------------------------------------------
+
+// candidate counts of A P's row i: sum over couplings of the coupled row's P entries
+__global__ void k_ap_cand_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  int n = 0;
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+    const int j = c.col[k];
+    n += c.Prow[j + 1] - c.Prow[j];
+  }
+  c.cptr[i] = n;
+}
+
+__global__ void k_ap_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  int32_t* cd = c.cand + c.cptr[i];
+  int n = 0;
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+    const int j = c.col[k];
+    for (int e = c.Prow[j]; e < c.Prow[j + 1]; ++e) cd[n++] = c.Pcol[e];
+  }
+  sort_ints(cd, n);
+  c.AProw[i] = unique_sorted(cd, n);
+}
+
+// (A P)_iJ = sum over couplings k, P entries e of col k with column J of A_k P_e, in (k, e) order
+__global__ void k_ap_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int i = tl.row0 + threadIdx.x;
+  const int32_t* cd = c.cand + c.cptr[i];
+  const int nu = c.AProw[i + 1] - c.AProw[i];
+  for (int u = 0; u < nu; ++u) {
+    const int Jc = cd[u];
+    const size_t x = size_t(c.AProw[i] + u);
+    c.APcol[x] = Jc;
+    double acc[36];
+    for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      for (int e = c.Prow[j]; e < c.Prow[j + 1]; ++e) {
+        if (c.Pcol[e] != Jc) continue;
+        const double* A = c.A + 36 * size_t(k);
+        const double* P = c.P + 36 * size_t(e);
+        for (int r = 0; r < 6; ++r)
+          for (int col = 0; col < 6; ++col) {
+            double s = 0.0;
+            for (int q = 0; q < 6; ++q) s += A[r * 6 + q] * P[q * 6 + col];
+            acc[r * 6 + col] = acc[r * 6 + col] + s;
+          }
+      }
+    }
+    double* out = c.AP + 36 * x;
+    for (int q = 0; q < 36; ++q) out[q] = acc[q];
+  }
+}
+
+// candidate counts of the coarse row I: the A P rows of its restriction entries
+__global__ void k_c_cand_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
+  if (int(blockIdx.x) >= nctiles) return;
+  const Tile tl = ctiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int I = tl.row0 + threadIdx.x;
+  int n = 1;  // (the diagonal, always present)
+  for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
+    const int i = c.Rfine[r];
+    n += c.AProw[i + 1] - c.AProw[i];
+  }
+  c.cptr[I] = n;
+}
+
+// coarse row I's columns: diagonal first, then ascending (Bsr6Builder::finish)
+__global__ void k_c_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
+  if (int(blockIdx.x) >= nctiles) return;
+  const Tile tl = ctiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int I = tl.row0 + threadIdx.x;
+  int32_t* cd = c.cand + c.cptr[I];
+  int n = 0;
+  cd[n++] = I;
+  for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
+    const int i = c.Rfine[r];
+    for (int x = c.AProw[i]; x < c.AProw[i + 1]; ++x) cd[n++] = c.APcol[x];
+  }
+  sort_ints(cd, n);
+  int u = unique_sorted(cd, n);
+  // move the diagonal to the front
+  int at = 0;
+  while (cd[at] != I) ++at;
+  for (int t = at; t > 0; --t) cd[t] = cd[t - 1];
+  cd[0] = I;
+  c.Crow[I] = u;
+}
+
+// (P^T A P)_IJ = sum over restriction entries (i, e) of I, in order, of P_e^T (A P)_iJ; times the
+// coarse scale (unsmoothed levels)
+__global__ void k_c_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
+  if (int(blockIdx.x) >= nctiles) return;
+  const Tile tl = ctiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int I = tl.row0 + threadIdx.x;
+  const int32_t* cd = c.cand + c.cptr[I];
+  const int nu = c.Crow[I + 1] - c.Crow[I];
+  for (int u = 0; u < nu; ++u) {
+    const int Jc = cd[u];
+    const size_t y = size_t(c.Crow[I] + u);
+    c.Ccol[y] = Jc;
+    double acc[36];
+    for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+    for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
+      const int i = c.Rfine[r];
+      // the A P entry of row i in column Jc (columns ascending)
+      int lo = c.AProw[i], hi = c.AProw[i + 1];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (c.APcol[mid] < Jc) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo >= c.AProw[i + 1] || c.APcol[lo] != Jc) continue;
+      const double* P = c.P + 36 * size_t(c.Rent[r]);
+      const double* X = c.AP + 36 * size_t(lo);
+      for (int rr = 0; rr < 6; ++rr)
+        for (int col = 0; col < 6; ++col) {
+          double s = 0.0;
+          for (int q = 0; q < 6; ++q) s += P[q * 6 + rr] * X[q * 6 + col];
+          acc[rr * 6 + col] = acc[rr * 6 + col] + s;
+        }
+    }
+    double* out = c.C + 36 * y;
+    for (int q = 0; q < 36; ++q) out[q] = c.coarse_scale != 1.0 ? acc[q] * c.coarse_scale : acc[q];
+  }
+}
+
+// ---- power iteration (6 lanes a row, kTileRows-row tiles) ------------------------------------------
+
+__global__ void __launch_bounds__(kWG) k_power_init(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                    int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const PowerDev& p = J[tl.g];
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  p.v[0][e] = 1.0 + 0.001 * double(e % 7);
+}
+
+// v_{s+1} = D^-1 A (v_s / |v_s|), partial |v_{s+1}|^2
+__global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                    int ntiles, int step, double* __restrict__ partial) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const PowerDev& p = J[tl.g];
+  if (p.lam[1] <= 0.0) return;  // (stopped: a zero vector)
+  const double nv = step == 0 ? 1.0 : p.lam[1];
+  const double* in = p.v[step & 1];
+  double* out = p.v[(step + 1) & 1];
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  double w = 0.0;
+  if (on) {
+    for (int k = p.rowptr[row]; k < p.rowptr[row + 1]; ++k) {
+      const double* a = p.A + 36 * size_t(k) + 6 * q;
+      const double* x = in + 6 * size_t(p.col[k]);
+      double s = 0.0;
+      for (int c = 0; c < 6; ++c) s += a[c] * (x[c] / nv);
+      w += s;
+    }
+  }
+  double wv[6];
+  row_gather(w, wv);
+  double v = 0.0;
+  if (on) {
+    const double* D = p.Dinv + 36 * size_t(row) + 6 * q;
+    v = D[0] * wv[0] + D[1] * wv[1] + D[2] * wv[2] + D[3] * wv[3] + D[4] * wv[4] + D[5] * wv[5];
+    out[6 * size_t(row) + q] = v;
+  }
+  __shared__ double lds[4];
+  const double s = block_sum(v * v, lds);
+  if (threadIdx.x == 0) partial[2 * tl.slot] = s;
+}
+
+__global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict__ J, const int32_t* __restrict__ gtile,
+                                                      int count, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  const PowerDev& p = J[blockIdx.x];
+  if (p.lam[1] <= 0.0) return;
+  __shared__ double lds[4];
+  double a = 0.0;
+  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k];
+  const double s = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    const double nv = sqrt(s);
+    if (!(nv > 0)) {
+      p.lam[1] = 0.0;
+      return;
+    }
+    p.lam[0] = nv;
+    p.lam[1] = nv;
+  }
+}
+
+// ---- dense coarsest level: Cholesky (amg.cpp:535-578), then the explicit inverse --------------------
+
+__global__ void __launch_bounds__(kWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
+  if (int(blockIdx.x) >= count) return;
+  const DenseDev& d = J[blockIdx.x];
+  const int m = d.m;
+  double* L = d.L;
+  double* W = d.Minv;  // (the inverse factor first, then the inverse in place of L)
+  auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(m) + size_t(c)]; };
+  for (size_t t = threadIdx.x; t < size_t(m) * size_t(m); t += kWG) L[t] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < d.n; i += kWG)
+    for (int k = d.rowptr[i]; k < d.rowptr[i + 1]; ++k) {
+      const int j = d.col[k];
+      const double* b = d.A + 36 * size_t(k);
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) at(6 * i + r, 6 * j + c) += b[r * 6 + c];
+    }
+  __syncthreads();
+  __shared__ double sdmax, ss;
+  if (threadIdx.x == 0) {
+    double dmax = 0.0;
+    for (int i = 0; i < m; ++i) dmax = fmax(dmax, fabs(at(i, i)));
+    sdmax = dmax;
+  }
+  __syncthreads();
+  const double dmax = sdmax;
+  for (int j = 0; j < m; ++j) {
+    if (threadIdx.x == 0) {
+      double dd = at(j, j);
+      for (int k = 0; k < j; ++k) dd -= at(j, k) * at(j, k);
+      if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
+      const double s = sqrt(dd);
+      at(j, j) = s;
+      ss = s;
+    }
+    __syncthreads();
+    const double s = ss;
+    for (int i = j + 1 + threadIdx.x; i < m; i += kWG) {
+      double v = 0.5 * (at(i, j) + at(j, i));
+      for (int k = 0; k < j; ++k) v -= at(i, k) * at(j, k);
+      at(i, j) = v / s;
+    }
+    __syncthreads();
+  }
+  // W = L^-1 (lower triangular), one column a thread
+  for (int c = threadIdx.x; c < m; c += kWG) {
+    for (int i = 0; i < c; ++i) W[size_t(i) * m + c] = 0.0;
+    W[size_t(c) * m + c] = 1.0 / at(c, c);
+    for (int i = c + 1; i < m; ++i) {
+      double v = 0.0;
+      for (int k = c; k < i; ++k) v += at(i, k) * W[size_t(k) * m + c];
+      W[size_t(i) * m + c] = -v / at(i, i);
+    }
+  }
+  __syncthreads();
+  // M^-1 = W^T W into L's storage, then copied over W
+  for (size_t t = threadIdx.x; t < size_t(m) * size_t(m); t += kWG) {
+    const int i = int(t / m), j = int(t % m);
+    const int k0 = i > j ? i : j;
+    double v = 0.0;
+    for (int k = k0; k < m; ++k) v += W[size_t(k) * m + i] * W[size_t(k) * m + j];
+    L[t] = v;
+  }
+  __syncthreads();
+  for (size_t t = threadIdx.x; t < size_t(m) * size_t(m); t += kWG) W[t] = L[t];
+}
+
+// ---- conversions -----------------------------------------------------------------------------------
+
+__global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int64_t k = t; k < count; k += int64_t(gridDim.x) * blockDim.x) out[k] = float(in[k]);
+}
+
+// ---- bond checks -------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void mv6_rigid(const double r[3], const double* u, double* y) {
+  // y = rigid_block(r) u, each row summed left to right as blk6::mv6 does
+  for (int R = 0; R < 6; ++R) {
+    double s = rigid_at(r, R, 0) * u[0];
+    for (int c = 1; c < 6; ++c) s = s + rigid_at(r, R, c) * u[c];
+    y[R] = s;
+  }
+}
+
+__global__ void k_bond_checks(const CheckDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CheckDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int bi = tl.row0 + threadIdx.x;
+  const svxg_bond& b = c.bonds[bi];
+  const svxg_material& MA = mat_of(c.mats, c.nmats, b.ma);
+  const svxg_material& MB = mat_of(c.mats, c.nmats, b.mb);
+  double xa[6] = {0, 0, 0, 0, 0, 0}, xb[6] = {0, 0, 0, 0, 0, 0};
+  const svxg_node& na = c.nodes[b.a];
+  if (!na.fixed) {
+    const double r[3] = {b.p[0] - na.c[0], b.p[1] - na.c[1], b.p[2] - na.c[2]};
+    mv6_rigid(r, c.u + 6 * size_t(b.a), xa);
+  }
+  if (b.b >= 0) {
+    const svxg_node& nb = c.nodes[b.b];
+    if (!nb.fixed) {
+      const double r[3] = {b.p[0] - nb.c[0], b.p[1] - nb.c[1], b.p[2] - nb.c[2]};
+      mv6_rigid(r, c.u + 6 * size_t(b.b), xb);
+    }
+  }
+  const double dt[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
+  const double dr[3] = {xb[3] - xa[3], xb[4] - xa[4], xb[5] - xa[5]};
+  auto dot3 = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  const double la = fmax(b.la, 1e-4), lb = fmax(b.lb, 1e-4);
+  const double invE = la / MA.E + lb / MB.E;
+  const double invG = la / MA.G + lb / MB.G;
+  svxg_bond_load L;
+  L.N = b.area / invE * dot3(dt, b.n);
+  L.V1 = (5.0 / 6.0) * b.area / invG * dot3(dt, b.t1);
+  L.V2 = (5.0 / 6.0) * b.area / invG * dot3(dt, b.t2);
+  L.T = 0.8 * (b.s1 + b.s2) / invG * dot3(dr, b.n);
+  L.M1 = b.s2 / invE * dot3(dr, b.t1);
+  L.M2 = b.s1 / invE * dot3(dr, b.t2);
+  if (c.loads) c.loads[bi] = L;
+  // strengths (stress.cpp:14-21)
+  const double sc = b.strength / fmax(1e-6, c.fragility);
+  double ft, fb, fc, coh, mu;
+  if (b.sectioned) {
+    ft = sc * double(b.ft);
+    fb = sc * double(b.fb);
+    fc = sc * double(b.fc);
+    coh = sc * double(b.coh);
+    mu = double(b.mu);
+  } else {
+    ft = sc * fmin(MA.ft, MB.ft);
+    fb = sc * fmin(MA.fb, MB.fb);
+    fc = sc * fmin(MA.fc, MB.fc);
+    coh = sc * fmin(MA.cohesion, MB.cohesion);
+    mu = fmin(MA.friction, MB.friction);
+  }
+  // utilisation (stress.cpp:23-57)
+  const double kWeak = 1e-9;
+  ft = fmax(ft, kWeak);
+  fb = fmax(fb, kWeak);
+  fc = fmax(fc, kWeak);
+  const double A = fmax(b.area, 1e-12);
+  const double sN = L.N / A;
+  const double sb = (b.s2 > 0 ? fabs(L.M1) * b.c2 / b.s2 : 0.0) + (b.s1 > 0 ? fabs(L.M2) * b.c1 / b.s1 : 0.0);
+  const double phi_t = sN >= 0.0 ? sN / ft + sb / fb : fmax(0.0, sb + sN) / fb;
+  const double phi_c = fmax(0.0, sb - sN) / fc;
+  const double Jt = fmax(b.s1 + b.s2, 1e-18);
+  const double tau = 1.5 * sqrt(L.V1 * L.V1 + L.V2 * L.V2) / A + fabs(L.T) * b.rmax / Jt;
+  const double cap = fmax(coh + mu * fmax(0.0, -sN), kWeak);
+  const double phi_s = tau / cap;
+  double phi = phi_t;
+  uint8_t md = 1;
+  if (phi_c > phi) {
+    phi = phi_c;
+    md = 2;
+  }
+  if (phi_s > phi) {
+    phi = phi_s;
+    md = 3;
+  }
+  c.phi[bi] = phi;
+  c.mode[bi] = phi > 0 ? md : 0;
+}
+
+__global__ void k_scatter_u(const double* __restrict__ x, const int32_t* __restrict__ dof, double* __restrict__ u,
+                            int nnodes) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * nnodes) return;
+  const int i = t / 6, q = t % 6;
+  const int d = dof[i];
+  u[t] = d >= 0 ? x[6 * size_t(d) + q] : 0.0;
+}
+
+// ---- host launchers -----------------------------------------------------------------------------------
+
+void launch_bond_products(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_bond_products, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_gather_blocks(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_gather_blocks, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_diag_trace(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_diag_trace, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count) {
+  if (count > 0) hipLaunchKernelGGL(k_id_scale, dim3((count + 63) / 64), dim3(64), 0, s, jobs, count);
+}
+void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_identity_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_remove_bonds(hipStream_t s, const AssemblyDev* job, const int32_t* ids, int count) {
+  if (count > 0) hipLaunchKernelGGL(k_remove_bonds, dim3(1), dim3(64), 0, s, job, ids, count);
+}
+void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* rows, int count) {
+  if (count > 0) hipLaunchKernelGGL(k_retire_rows, dim3((36 * count + 255) / 256), dim3(256), 0, s, job, rows, count);
+}
+void launch_inv_diag(hipStream_t s, const DiagDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_inv_diag, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_strength(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_strength, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_strong(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_strong, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_mis_round(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_mis_t1, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_mis_t2, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_mis_update, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_aggregate(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int count) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_agg_pass1, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_agg_pass2, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_scan_pair, dim3(count), dim3(kWG), 0, s, jobs, count);
+}
+void launch_scan_pair(hipStream_t s, const CoarsenDev* jobs, int count) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_scan_pair, dim3(count), dim3(kWG), 0, s, jobs, count);
+}
+void launch_assign(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_assign, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_coarse_positions(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
+                             int nctiles, int count) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_member_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_scan_acnt, dim3(count), dim3(kWG), 0, s, jobs, count);
+  hipLaunchKernelGGL(k_member_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_coarse_pos, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+  hipLaunchKernelGGL(k_offsets, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_p_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_p_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_scan_rows(hipStream_t s, const CoarsenDev* jobs, int count, int which) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_scan_rows, dim3(count), dim3(kWG), 0, s, jobs, count, which);
+}
+void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_p_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
+                    int nctiles, int count) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_r_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_scan_rows, dim3(count), dim3(kWG), 0, s, jobs, count, 1);
+  hipLaunchKernelGGL(k_r_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_r_sort, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+}
+void launch_ap_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_ap_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_ap_cand_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_ap_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
+  if (nctiles <= 0) return;
+  hipLaunchKernelGGL(k_c_cand_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+}
+void launch_c_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
+  if (nctiles <= 0) return;
+  hipLaunchKernelGGL(k_c_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+}
+void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
+  if (nctiles <= 0) return;
+  hipLaunchKernelGGL(k_c_fill, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+}
+void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_power_init, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_power_step, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
+}
+void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_power_finish, dim3(count), dim3(kWG), 0, s, jobs, gtile, count, partial);
+}
+void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count) {
+  if (count > 0) hipLaunchKernelGGL(k_dense_factor, dim3(count), dim3(kWG), 0, s, jobs, count);
+}
+void launch_to_f32(hipStream_t s, const double* in, float* out, int64_t count) {
+  if (count <= 0) return;
+  const int64_t blocks = (count + 255) / 256;
+  hipLaunchKernelGGL(k_to_f32, dim3(unsigned(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, in, out, count);
+}
+void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_bond_checks, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_scatter_u(hipStream_t s, const double* x, const int32_t* dof, double* u, int nnodes) {
+  if (nnodes > 0) hipLaunchKernelGGL(k_scatter_u, dim3((6 * nnodes + 255) / 256), dim3(256), 0, s, x, dof, u, nnodes);
+}
+
+}  // namespace svxg

@@ -1,0 +1,879 @@
+// Solve-phase kernels of the svxg stress solve: block-sparse products, the Chebyshev-smoothed
+// V-cycle (smoothing, residual, restriction, prolongation, dense coarsest solve), block-Jacobi
+// preconditioning and the PCG vector updates with their fixed-order reductions.
+//
+// The reference's hot loops these replace (core/src/solve/amg.cpp, core/src/stress/stress.cpp):
+//   Bsr6::apply amg.cpp:174-183       -> spmv_row (the same per-row sum order: a row's six
+//                                        products summed, then added to the running sum)
+//   Amg::smooth_f amg.cpp:630-659     -> k_cheb_step (Chebyshev in D^-1 A, row-parallel)
+//   residual / restriction / prolong  -> k_residual, k_restrict, k_prolong (amg.cpp:683-714)
+//   solve_coarsest amg.cpp:580-592    -> k_dense_solve (the coarsest level's inverse, FP64)
+//   StressProblem::iterate            -> k_pcg_spmv, k_axpy_rr, k_update_p, k_finish_*
+//     stress.cpp:403-464                 (the dots: per-tile partials summed per graph in order)
+#include "svxg_kernels.hpp"
+
+namespace svxg {
+
+namespace {
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct V2;
+template <>
+struct V2<double> {
+  using t = dvec2;
+};
+template <>
+struct V2<float> {
+  using t = fvec2;
+};
+
+// six consecutive values (16-byte aligned for double, 8 for float)
+template <typename T>
+__device__ __forceinline__ void load6(const T* p, T v[6]) {
+  using P = typename V2<T>::t;
+  const P* q = reinterpret_cast<const P*>(p);
+  const P a = q[0], b = q[1], c = q[2];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+  v[4] = c.x;
+  v[5] = c.y;
+}
+
+// streamed matrix rows: loaded once per product, not worth caching
+template <typename T>
+__device__ __forceinline__ void load6_stream(const T* p, T v[6]) {
+  using P = typename V2<T>::t;
+  const P* q = reinterpret_cast<const P*>(p);
+  const P a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1), c = __builtin_nontemporal_load(q + 2);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+  v[4] = c.x;
+  v[5] = c.y;
+}
+
+// (A x)_q of block row `row`: sum over the row's blocks of (row q of the block) . x_col
+template <typename T>
+__device__ __forceinline__ T spmv_row(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                      const T* __restrict__ A, const T* __restrict__ x, int row, int q) {
+  T acc = T(0);
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  for (int k = k0; k < k1; ++k) {
+    const int j = col[k];
+    T a[6], v[6];
+    load6_stream(A + 36 * size_t(k) + 6 * q, a);
+    load6(x + 6 * size_t(j), v);
+    const T s = a[0] * v[0] + a[1] * v[1] + a[2] * v[2] + a[3] * v[3] + a[4] * v[4] + a[5] * v[5];
+    acc += s;
+  }
+  return acc;
+}
+
+// (M v)_q with M a 6x6 row-major block and v the row's values gathered from its lanes
+template <typename T>
+__device__ __forceinline__ T block_row(const T* __restrict__ M, const T v[6], int q) {
+  T m[6];
+  load6(M + 6 * q, m);
+  return m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
+}
+
+template <typename T>
+__device__ __forceinline__ const LevelArrays<T>& arrays(const LevelDev& L);
+template <>
+__device__ __forceinline__ const LevelArrays<double>& arrays<double>(const LevelDev& L) {
+  return L.f64;
+}
+template <>
+__device__ __forceinline__ const LevelArrays<float>& arrays<float>(const LevelDev& L) {
+  return L.f32;
+}
+
+// the graph takes part in this launch: its (inner, for the FP32 V-cycle of the mixed path) solve
+// is running
+__device__ __forceinline__ bool live(const GraphDev& g, int inner) {
+  const PcgState* s = g.st;
+  if (s->state != ST_RUNNING) return false;
+  return !inner || s->inner_state == ST_RUNNING;
+}
+
+// Chebyshev coefficients of step j of the smoother on [lmax / ratio, lmax]: the direction is
+// d_j = c1 d_{j-1} + c2 D^-1 r_j (c1 = 0 at step 0)
+__device__ __forceinline__ void cheb_coeffs(double lmax, int j, double& c1, double& c2) {
+  const double hi = lmax, lo = lmax / kChebRatio;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  if (j == 0) {
+    c1 = 0.0;
+    c2 = 1.0 / theta;
+    return;
+  }
+  double rho_prev = rho;
+  for (int s = 1; s <= j; ++s) {
+    rho_prev = rho;
+    rho = 1.0 / (2.0 * sigma - rho_prev);
+  }
+  c1 = rho * rho_prev;
+  c2 = 2.0 * rho / delta;
+}
+
+__device__ __forceinline__ void finish_partials(double v0, double v1, double* partial, int slot, bool two) {
+  __shared__ double lds[8];
+  const double s0 = block_sum(v0, lds);
+  double s1 = 0.0;
+  if (two) s1 = block_sum(v1, lds + 4);
+  if (threadIdx.x == 0) {
+    partial[2 * slot] = s0;
+    if (two) partial[2 * slot + 1] = s1;
+  }
+}
+
+}  // namespace
+
+// ---- V-cycle kernels ---------------------------------------------------------------------------
+
+// One Chebyshev step on `level`: r = b - A x_in (b when fresh: x_in = 0), d = c1 d + c2 D^-1 r,
+// x_out = x_in + d. With `dot`, the partial sums of b . x_out are written (the PCG's r . z when this
+// is level 0's last step).
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                   int ntiles, int level, int step, int in_buf, int out_buf, int fresh,
+                                                   int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const LevelDev& L = g.lv[level];
+  const LevelArrays<T>& a = arrays<T>(L);
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  const size_t e = 6 * size_t(row) + q;
+  T r = T(0), xin = T(0), bq = T(0);
+  if (on) {
+    bq = a.b[e];
+    if (fresh) {
+      r = bq;
+    } else {
+      xin = a.x[in_buf][e];
+      r = bq - spmv_row<T>(L.rowptr, L.col, a.A, a.x[in_buf], row, q);
+    }
+  }
+  T rv[6];
+  row_gather(r, rv);
+  double c1, c2;
+  cheb_coeffs(L.lmax, step, c1, c2);
+  T xo = T(0);
+  if (on) {
+    const T z = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
+    const T d = (step == 0) ? T(c2) * z : T(c1) * a.d[e] + T(c2) * z;
+    a.d[e] = d;
+    xo = xin + d;
+    a.x[out_buf][e] = xo;
+  }
+  if (partial) finish_partials(on ? double(bq) * double(xo) : 0.0, 0.0, partial, tl.slot, false);
+}
+
+// r = b - A x
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int level, int buf, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const LevelDev& L = g.lv[level];
+  const LevelArrays<T>& a = arrays<T>(L);
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const int row = tl.row0 + rl;
+  const size_t e = 6 * size_t(row) + q;
+  a.r[e] = a.b[e] - spmv_row<T>(L.rowptr, L.col, a.A, a.x[buf], row, q);
+}
+
+// b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int level, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const LevelDev& F = g.lv[level];
+  const LevelArrays<T>& af = arrays<T>(F);
+  const LevelArrays<T>& ac = arrays<T>(g.lv[level + 1]);
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const int I = tl.row0 + rl;
+  T acc = T(0);
+  for (int e = F.Rrow[I]; e < F.Rrow[I + 1]; ++e) {
+    const int i = F.Rfine[e];
+    T m[6], v[6];
+    load6(af.R + 36 * size_t(e) + 6 * q, m);
+    load6(af.r + 6 * size_t(i), v);
+    acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
+  }
+  ac.b[6 * size_t(I) + q] = acc;
+}
+
+// x += P x_c (x_c: the coarse level's result, x[1])
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                 int ntiles, int level, int buf, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const LevelDev& F = g.lv[level];
+  const LevelArrays<T>& af = arrays<T>(F);
+  const T* xc = arrays<T>(g.lv[level + 1]).x[1];
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const int i = tl.row0 + rl;
+  T acc = af.x[buf][6 * size_t(i) + q];
+  for (int e = F.Prow[i]; e < F.Prow[i + 1]; ++e) {
+    const int J = F.Pcol[e];
+    T m[6], v[6];
+    load6(af.P + 36 * size_t(e) + 6 * q, m);
+    load6(xc + 6 * size_t(J), v);
+    acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
+  }
+  af.x[buf][6 * size_t(i) + q] = acc;
+}
+
+// x[1] = M^-1 b at the coarsest level of each listed graph (FP64 products): one workgroup a graph,
+// a wavefront a row at a time, lanes across the row
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict__ G, const int32_t* __restrict__ graphs,
+                                                     int count, int level, int inner) {
+  if (int(blockIdx.x) >= count) return;
+  const GraphDev& g = G[graphs[blockIdx.x]];
+  if (!live(g, inner)) return;
+  const LevelDev& L = g.lv[level];
+  const LevelArrays<T>& a = arrays<T>(L);
+  const int m = L.dense_m;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int i = w; i < m; i += 4) {
+    const double* row = L.Minv + size_t(i) * size_t(m);
+    double s = 0.0;
+    for (int j = l; j < m; j += 64) s += row[j] * double(a.b[j]);
+    s = wave_sum(s);
+    if (l == 0) a.x[1][i] = T(s);
+  }
+}
+
+// z = D^-1 r (block-Jacobi preconditioning of a graph without a hierarchy)
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_jacobi(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                int ntiles, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const LevelArrays<T>& a = arrays<T>(g.lv[0]);
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  const size_t e = 6 * size_t(row) + q;
+  const T r = on ? a.b[e] : T(0);
+  T rv[6];
+  row_gather(r, rv);
+  if (on) a.x[1][e] = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
+}
+
+// ---- PCG kernels (level 0 tiles) -------------------------------------------------------------
+
+template <typename T>
+struct PcgVecs {
+  T *x, *r, *z, *p, *q;
+};
+template <typename T>
+__device__ __forceinline__ PcgVecs<T> pcg_vecs(const GraphDev& g);
+template <>
+__device__ __forceinline__ PcgVecs<double> pcg_vecs<double>(const GraphDev& g) {
+  return {g.x, g.r, g.z, g.p, g.q};
+}
+template <>
+__device__ __forceinline__ PcgVecs<float> pcg_vecs<float>(const GraphDev& g) {
+  return {g.ix, g.ir, g.iz, g.ip, g.iq};
+}
+
+// q = K p, partial p . q
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const LevelDev& L = g.lv[0];
+  const PcgVecs<T> v = pcg_vecs<T>(g);
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double s = 0.0;
+  if (on) {
+    const int row = tl.row0 + rl;
+    const size_t e = 6 * size_t(row) + q;
+    const T y = spmv_row<T>(L.rowptr, L.col, arrays<T>(L).A, v.p, row, q);
+    v.q[e] = y;
+    s = double(v.p[e]) * double(y);
+  }
+  finish_partials(s, 0.0, partial, tl.slot, false);
+}
+
+// x += alpha p, r -= alpha q, partial r . r
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_axpy_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                 int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const PcgVecs<T> v = pcg_vecs<T>(g);
+  const double alpha = g.st->alpha;
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double s = 0.0;
+  if (on) {
+    const size_t e = 6 * size_t(tl.row0 + rl) + q;
+    v.x[e] = T(double(v.x[e]) + alpha * double(v.p[e]));
+    const T r = T(double(v.r[e]) - alpha * double(v.q[e]));
+    v.r[e] = r;
+    s = double(r) * double(r);
+  }
+  finish_partials(s, 0.0, partial, tl.slot, false);
+}
+
+// p = z + beta p
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_update_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const PcgVecs<T> v = pcg_vecs<T>(g);
+  const double beta = g.st->beta;
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  v.p[e] = T(double(v.z[e]) + beta * double(v.p[e]));
+}
+
+// partial r . z (after the preconditioner)
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_dot_rz(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const PcgVecs<T> v = pcg_vecs<T>(g);
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double s = 0.0;
+  if (on) {
+    const size_t e = 6 * size_t(tl.row0 + rl) + q;
+    s = double(v.r[e]) * double(v.z[e]);
+  }
+  finish_partials(s, 0.0, partial, tl.slot, false);
+}
+
+// partial r . r
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_dot_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const PcgVecs<T> v = pcg_vecs<T>(g);
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double s = 0.0;
+  if (on) {
+    const T r = v.r[6 * size_t(tl.row0 + rl) + q];
+    s = double(r) * double(r);
+  }
+  finish_partials(s, 0.0, partial, tl.slot, false);
+}
+
+// y = A x over one matrix (no graph table): tests and the bandwidth probe
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_spmv_plain(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                    const T* __restrict__ A, const T* __restrict__ x, T* __restrict__ y,
+                                                    int n) {
+  const int ntiles = (n + kTileRows - 1) / kTileRows;
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  int rl, q;
+  lane_row(rl, q);
+  const int row = t * kTileRows + rl;
+  if (rl < 0 || row >= n) return;
+  y[6 * size_t(row) + q] = spmv_row<T>(rowptr, col, A, x, row, q);
+}
+
+// ---- start of a solve ------------------------------------------------------------------------
+
+// r = b - K x (FP64), partials b . b and r . r
+__global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                        int ntiles, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.st->state != ST_RUNNING) return;
+  const LevelDev& L = g.lv[0];
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double sb = 0.0, sr = 0.0;
+  if (on) {
+    const int row = tl.row0 + rl;
+    const size_t e = 6 * size_t(row) + q;
+    const double b = g.b[e];
+    const double r = b - spmv_row<double>(L.rowptr, L.col, L.f64.A, g.x, row, q);
+    g.r[e] = r;
+    sb = b * b;
+    sr = r * r;
+  }
+  finish_partials(sb, sr, partial, tl.slot, true);
+}
+
+// the solve's start, per graph: |b|, a zero or non-finite load (x = 0, r = 0)
+__global__ void __launch_bounds__(kWG) k_finish_begin(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                      int count, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  const GraphDev& g = G[blockIdx.x];
+  PcgState* s = g.st;
+  if (s->state != ST_RUNNING) return;
+  __shared__ double lds[8];
+  double a = 0.0;
+  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k];
+  const double bb = block_sum(a, lds);
+  if (threadIdx.x == 0) {
+    s->bn = sqrt(bb);
+    s->target = s->rtol * s->bn;
+    s->zero_load = (bb > 0.0) ? 0 : 1;
+  }
+}
+
+// x = 0, r = 0 where the load is zero or not finite
+__global__ void __launch_bounds__(kWG) k_zero_load(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                   int ntiles) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.st->state != ST_RUNNING || !g.st->zero_load) return;
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  g.x[e] = 0.0;
+  g.r[e] = 0.0;
+}
+
+// p = z, partials r . z and r . r (the start of the conjugate directions)
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                 int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, inner)) return;
+  const PcgVecs<T> v = pcg_vecs<T>(g);
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double s0 = 0.0, s1 = 0.0;
+  if (on) {
+    const size_t e = 6 * size_t(tl.row0 + rl) + q;
+    const T z = v.z[e], r = v.r[e];
+    v.p[e] = z;
+    s0 = double(r) * double(z);
+    s1 = double(r) * double(r);
+  }
+  finish_partials(s0, s1, partial, tl.slot, true);
+}
+
+// ---- per-graph finishes (one workgroup a graph; partials summed in tile order) ----------------
+
+__device__ __forceinline__ double graph_sum(const int32_t* gtile, const double* partial, int which) {
+  __shared__ double lds[4];
+  double a = 0.0;
+  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k + which];
+  return block_sum(a, lds);
+}
+
+__device__ __forceinline__ PcgState* finish_state(const GraphDev& g, int inner, bool& go) {
+  PcgState* s = g.st;
+  go = s->state == ST_RUNNING && (!inner || s->inner_state == ST_RUNNING);
+  return s;
+}
+
+// after p = z: rz, the first residual check (StressProblem::begin / the head of iterate)
+__global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                      int count, int inner, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  bool go;
+  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  if (!go) return;
+  const double rz = graph_sum(gtile, partial, 0);
+  const double rr = graph_sum(gtile, partial, 1);
+  if (threadIdx.x != 0) return;
+  s->rz = rz;
+  const double rn = sqrt(rr);
+  if (inner) {
+    s->inner_iters = 0;
+    if (!(rn > s->inner_target)) s->inner_state = ST_CONVERGED;
+    return;
+  }
+  s->rn = rn;
+  if (!isfinite(s->bn) || !isfinite(rn)) {
+    s->rel_res = INFINITY;
+    s->state = ST_BREAKDOWN;
+    return;
+  }
+  s->rel_res = s->bn > 0 ? rn / s->bn : 0.0;
+  if (rn <= s->target || !(s->bn > 0) || G[blockIdx.x].nfree == 0) s->state = ST_CONVERGED;
+}
+
+__global__ void __launch_bounds__(kWG) k_finish_alpha(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                      int count, int inner, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  bool go;
+  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  if (!go) return;
+  const double pq = graph_sum(gtile, partial, 0);
+  if (threadIdx.x != 0) return;
+  // (the iteration budget is checked at an iteration's head: a stopped run resumes where it was)
+  if (inner) {
+    if (s->inner_iters >= s->inner_maxit) {
+      s->inner_state = ST_STOPPED;
+      return;
+    }
+    ++s->inner_iters;
+    if (!(pq > 0)) {
+      s->inner_state = ST_BREAKDOWN;
+      return;
+    }
+  } else {
+    if (s->iters >= s->maxit) {
+      s->state = ST_STOPPED;
+      return;
+    }
+    ++s->iters;
+    if (!(pq > 0)) {
+      s->state = ST_BREAKDOWN;
+      return;
+    }
+  }
+  s->pq = pq;
+  s->alpha = s->rz / pq;
+}
+
+__global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                   int count, int inner, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  bool go;
+  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  if (!go) return;
+  const double rr = graph_sum(gtile, partial, 0);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(rr);
+  if (inner) {
+    if (rn <= s->inner_target) s->inner_state = ST_CONVERGED;
+    return;
+  }
+  s->rn = rn;
+  s->rel_res = rn / s->bn;
+  if (rn <= s->target) s->state = ST_CONVERGED;
+}
+
+__global__ void __launch_bounds__(kWG) k_finish_beta(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                     int count, int inner, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  bool go;
+  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  if (!go) return;
+  const double rz1 = graph_sum(gtile, partial, 0);
+  if (threadIdx.x != 0) return;
+  if (!(rz1 > 0)) {
+    if (inner) s->inner_state = ST_BREAKDOWN;
+    else s->state = ST_BREAKDOWN;
+    return;
+  }
+  s->beta = rz1 / s->rz;
+  s->rz = rz1;
+}
+
+// ---- mixed precision: the refinement steps --------------------------------------------------
+
+// r = b - K x in FP64, partial r . r (the refinement's true residual)
+__global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                        int ntiles, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.st->state != ST_RUNNING) return;
+  const LevelDev& L = g.lv[0];
+  int rl, q;
+  lane_row(rl, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  double sr = 0.0;
+  if (on) {
+    const int row = tl.row0 + rl;
+    const size_t e = 6 * size_t(row) + q;
+    const double r = g.b[e] - spmv_row<double>(L.rowptr, L.col, L.f64.A, g.x, row, q);
+    g.r[e] = r;
+    sr = r * r;
+  }
+  finish_partials(sr, 0.0, partial, tl.slot, false);
+}
+
+// outer check: converged, or an inner solve of K d = r to max(eta |r|, target / 2)
+__global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                      int count, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  PcgState* s = G[blockIdx.x].st;
+  if (s->state != ST_RUNNING) return;
+  const double rr = graph_sum(gtile, partial, 0);
+  if (threadIdx.x != 0) return;
+  const double rn = sqrt(rr);
+  s->rn = rn;
+  s->outer_rn = rn;
+  if (!isfinite(s->bn) || !isfinite(rn)) {
+    s->rel_res = INFINITY;
+    s->state = ST_BREAKDOWN;
+    return;
+  }
+  s->rel_res = s->bn > 0 ? rn / s->bn : 0.0;
+  if (rn <= s->target || !(s->bn > 0) || G[blockIdx.x].nfree == 0) {
+    s->state = ST_CONVERGED;
+    return;
+  }
+  if (s->iters >= s->maxit) {
+    s->state = ST_STOPPED;
+    return;
+  }
+  s->inner_target = fmax(s->eta * rn, 0.5 * s->target);
+  s->inner_state = ST_RUNNING;
+  s->inner_iters = 0;
+  s->inner_maxit = s->maxit - s->iters;
+  ++s->outer;
+}
+
+// inner start: r_i = (float) r, x_i = 0
+__global__ void __launch_bounds__(kWG) k_inner_init(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                    int ntiles) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, 1)) return;
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  g.ir[e] = float(g.r[e]);
+  g.ix[e] = 0.0f;
+}
+
+// x += (double) x_i after an inner solve; the inner iterations count towards the solve's
+__global__ void __launch_bounds__(kWG) k_inner_apply(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                     int ntiles) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.st->state != ST_RUNNING || g.st->inner_state == ST_IDLE) return;
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  g.x[e] += double(g.ix[e]);
+}
+
+__global__ void k_inner_close(const GraphDev* __restrict__ G, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  PcgState* s = G[k].st;
+  if (s->state != ST_RUNNING || s->inner_state == ST_IDLE) return;
+  s->iters += s->inner_iters;
+  if (s->inner_state == ST_BREAKDOWN && s->inner_iters == 0) s->state = ST_BREAKDOWN;
+  s->inner_state = ST_IDLE;
+}
+
+// ---- host launchers ----------------------------------------------------------------------------
+
+int grid_for(int ntiles) {
+  if (ntiles < 64) return ntiles;
+  return 8 * ((ntiles + 7) / 8);
+}
+
+template <typename T>
+void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
+                      int out_buf, int fresh, int inner, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_cheb_step<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, step, in_buf,
+                     out_buf, fresh, inner, partial);
+}
+template <typename T>
+void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_residual<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf, inner);
+}
+template <typename T>
+void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int inner) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, inner);
+}
+template <typename T>
+void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf, inner);
+}
+template <typename T>
+void launch_dense_solve(hipStream_t s, const GraphDev* G, const int32_t* graphs, int count, int level, int inner) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kWG), 0, s, G, graphs, count, level, inner);
+}
+template <typename T>
+void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner);
+}
+template <typename T>
+void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_dot_rz<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+}
+template <typename T>
+void launch_dot_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_dot_rr<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+}
+template <typename T>
+void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col, const T* A, const T* x, T* y, int n) {
+  const int ntiles = (n + kTileRows - 1) / kTileRows;
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_spmv_plain<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, rowptr, col, A, x, y, n);
+}
+template <typename T>
+void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+}
+template <typename T>
+void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_axpy_rr<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+}
+template <typename T>
+void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_update_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner);
+}
+template <typename T>
+void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_start_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+}
+
+void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_begin_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
+}
+void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, const double* partial) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_finish_begin, dim3(count), dim3(kWG), 0, s, G, gtile, count, partial);
+}
+void launch_zero_load(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_zero_load, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
+}
+void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int32_t* gtile, int count, int inner,
+                   const double* partial) {
+  if (count <= 0) return;
+  switch (kind) {
+    case FinishKind::Start:
+      hipLaunchKernelGGL(k_finish_start, dim3(count), dim3(kWG), 0, s, G, gtile, count, inner, partial);
+      break;
+    case FinishKind::Alpha:
+      hipLaunchKernelGGL(k_finish_alpha, dim3(count), dim3(kWG), 0, s, G, gtile, count, inner, partial);
+      break;
+    case FinishKind::Rn:
+      hipLaunchKernelGGL(k_finish_rn, dim3(count), dim3(kWG), 0, s, G, gtile, count, inner, partial);
+      break;
+    case FinishKind::Beta:
+      hipLaunchKernelGGL(k_finish_beta, dim3(count), dim3(kWG), 0, s, G, gtile, count, inner, partial);
+      break;
+    case FinishKind::Outer:
+      hipLaunchKernelGGL(k_finish_outer, dim3(count), dim3(kWG), 0, s, G, gtile, count, partial);
+      break;
+  }
+}
+void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_outer_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
+}
+void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_inner_init, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
+}
+void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_inner_apply, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
+}
+void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_inner_close, dim3((count + 255) / 256), dim3(256), 0, s, G, count);
+}
+
+#define SVXG_INST(T)                                                                                                \
+  template void launch_cheb_step<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int, int, int,   \
+                                    double*);                                                                       \
+  template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
+  template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int);                        \
+  template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
+  template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const int32_t*, int, int, int);                  \
+  template void launch_jacobi<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                               \
+  template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
+  template void launch_dot_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
+  template void launch_spmv_plain<T>(hipStream_t, const int32_t*, const int32_t*, const T*, const T*, T*, int);      \
+  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                   \
+  template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                     \
+  template void launch_update_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                             \
+  template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);
+SVXG_INST(double)
+SVXG_INST(float)
+
+}  // namespace svxg

@@ -1,0 +1,152 @@
+// Device-side data layout of the svxg stress solve (shared by the kernels and the host driver).
+//
+// Every problem ("graph") owns its device arrays; a launch works on a batch of graphs through a
+// table of per-graph descriptors (GraphDev) and a table of row tiles. A tile is up to kTileRows
+// block rows of ONE graph, numbered from that graph's first row, so every per-tile partial sum,
+// and the per-graph sums made from them in tile order, are the same whatever else is in the
+// batch: a graph's answer is bitwise the same alone and in a batch.
+//
+// Lanes: a workgroup of 256 lanes is four 64-lane wavefronts; each wavefront serves 10 block rows
+// with 6 lanes a row (lane q of a row owns DOF q of the row), lanes 60..63 idle. A row's 6 lanes
+// never straddle a wavefront, so a row's values are exchanged with wavefront shuffles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/svxg.h"
+
+namespace svxg {
+
+constexpr int kWG = 256;
+constexpr int kRowsPerWave = 10;
+constexpr int kTileRows = 40;  // 4 wavefronts x 10 rows
+constexpr int kMaxLevels = 16;
+
+struct Tile {
+  int32_t g;      // graph (index into the batch's GraphDev table)
+  int32_t row0;   // first block row (the graph's own numbering)
+  int32_t nrows;  // rows in this tile (<= kTileRows)
+  int32_t slot;   // partial-sum slot (index of the tile within its table)
+};
+
+// per-graph solve state machine
+enum : int32_t {
+  ST_RUNNING = 0,
+  ST_CONVERGED = 1,
+  ST_BREAKDOWN = 2,
+  ST_STOPPED = 3,   // maxit reached
+  ST_IDLE = 4,      // not in this solve
+};
+
+struct PcgState {
+  double rz, pq, alpha, beta, rn, bn, target, rtol;
+  double rel_res;
+  // mixed precision: outer residual norm and the inner solve's stopping norm
+  double outer_rn, inner_target, eta;
+  int32_t iters, maxit, state, outer;
+  int32_t inner_state, inner_iters, inner_maxit, zero_load;
+};
+
+template <typename T>
+struct LevelArrays {
+  const T* A;      // 36 per block, row-major 6x6
+  const T* Dinv;   // 36 per row
+  const T* P;      // 36 per prolongation entry (rows of this level, columns of the next)
+  const T* R;      // 36 per restriction entry: P^T blocks by coarse row
+  T* x[2];         // ping-pong iterates (x[1] holds a V-cycle's result)
+  T* b;            // right-hand side
+  T* r;            // residual (restriction input)
+  T* d;            // Chebyshev direction
+};
+
+struct LevelDev {
+  int32_t n;          // block rows
+  int32_t nc;         // rows of the next level (0 at the coarsest)
+  int32_t terminal;   // 1: the graph's coarsest level
+  int32_t dense_m;    // 6 n when a dense inverse exists at the coarsest, else 0
+  int64_t nnzb, np;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const int32_t* Prow;   // n + 1
+  const int32_t* Pcol;
+  const int32_t* Rrow;   // nc + 1 (by coarse row)
+  const int32_t* Rfine;  // the fine row of each restriction entry
+  const double* Minv;    // dense inverse of the coarsest level (dense_m x dense_m, row-major)
+  double lmax;           // lambda_max(D^-1 A) estimate (x 1.1)
+  int32_t cheb_degree;
+  int32_t pad_;
+  LevelArrays<double> f64;
+  LevelArrays<float> f32;
+};
+
+struct GraphDev {
+  int32_t nlev;
+  int32_t nfree;
+  int32_t jacobi;   // block-Jacobi preconditioner (one level, no hierarchy)
+  int32_t pad_;
+  PcgState* st;
+  // outer / FP64 PCG vectors, 6 nfree each
+  double *x, *r, *z, *p, *q, *b;
+  // inner FP32 PCG vectors (mixed precision)
+  float *ix, *ir, *iz, *ip, *iq;
+  LevelDev lv[kMaxLevels];
+};
+
+// ---- small device helpers --------------------------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// the lane's (row within the tile, DOF) or row = -1 for the 4 idle lanes of each wavefront
+__device__ __forceinline__ void lane_row(int& rl, int& q) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l >= 60) {
+    rl = -1;
+    q = 0;
+    return;
+  }
+  rl = w * kRowsPerWave + l / 6;
+  q = l % 6;
+}
+
+// sum over a wavefront in a fixed butterfly order (every lane gets the sum)
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// sum over the workgroup in a fixed order: wavefront butterflies, then the four wavefront sums in
+// order; valid in thread 0 (scratch: 4 doubles of LDS)
+__device__ __forceinline__ double block_sum(double v, double* lds4) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+  __syncthreads();
+  return s;
+}
+
+// the 6 values of this lane's row held by its 6 lanes (lane q holds v_q), read by every lane of
+// the row
+__device__ __forceinline__ void row_gather(double v, double out[6]) {
+  const int base = (lane_id() / 6) * 6;
+  for (int k = 0; k < 6; ++k) out[k] = __shfl(v, base + k, 64);
+}
+__device__ __forceinline__ void row_gather(float v, float out[6]) {
+  const int base = (lane_id() / 6) * 6;
+  for (int k = 0; k < 6; ++k) out[k] = __shfl(v, base + k, 64);
+}
+
+// the tile a workgroup serves: consecutive workgroups go round-robin over the 8 XCDs, so each
+// XCD is given a contiguous range of tiles (its L2 holds the x values its rows gather)
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+  const int b = blockIdx.x;
+  if (ntiles < 64) return b;
+  const int per = (ntiles + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  return t;
+}
+
+}  // namespace svxg

@@ -1,0 +1,228 @@
+// Host-side launchers of the svxg kernels (solve_kernels.hip, setup_kernels.hip, check_kernels.hip).
+#pragma once
+
+#include "svxg_dev.hpp"
+
+namespace svxg {
+
+// Chebyshev smoothing interval [lmax / kChebRatio, lmax]
+constexpr double kChebRatio = 30.0;
+
+enum class FinishKind { Start, Alpha, Rn, Beta, Outer };
+
+int grid_for(int ntiles);
+
+// ---- solve ---------------------------------------------------------------------------------------
+template <typename T>
+void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
+                      int out_buf, int fresh, int inner, double* partial);
+template <typename T>
+void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner);
+template <typename T>
+void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int inner);
+template <typename T>
+void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner);
+template <typename T>
+void launch_dense_solve(hipStream_t s, const GraphDev* G, const int32_t* graphs, int count, int level, int inner);
+template <typename T>
+void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
+template <typename T>
+void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
+template <typename T>
+void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
+template <typename T>
+void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
+template <typename T>
+void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
+template <typename T>
+void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
+template <typename T>
+void launch_dot_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
+
+void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial);
+void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, const double* partial);
+void launch_zero_load(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
+void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int32_t* gtile, int count, int inner,
+                   const double* partial);
+void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial);
+void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
+void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
+void launch_inner_close(hipStream_t s, const GraphDev* G, int count);
+// plain y = A x on one graph's level (FP64 or FP32 copy), tests and the roofline probe
+template <typename T>
+void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col, const T* A, const T* x, T* y, int n);
+
+// ---- setup ---------------------------------------------------------------------------------------
+
+// Assembly of one graph's K (stress.cpp:84-132): the bonds' four block products, their sums into
+// the pattern's blocks in bond order, the identity rows.
+struct AssemblyDev {
+  int32_t nbonds, nfree, nslots;
+  int32_t pad_;
+  const svxg_bond* bonds;     // the graph's bonds
+  const double* nodec;        // 3 per node
+  const svxg_material* mats;  // the material table
+  int32_t nmats;
+  int32_t pad2_;
+  const int32_t* bond_slot;   // 4 per bond: the (aa, bb, ab, ba) blocks, -1 absent
+  const int32_t* cptr;        // nslots + 1
+  const int32_t* cidx;        // 4 bond + product, in bond order
+  const uint8_t* row_gone;    // per free row: its node is retired
+  double* prod;               // 4 x 36 per bond (scratch)
+  const int32_t* rowptr;
+  double* K;                  // 36 per block
+  double* id_scale;           // 1 (out)
+  double* trace;              // per free row (scratch)
+};
+
+void launch_bond_products(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles);
+void launch_gather_blocks(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles);
+void launch_diag_trace(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles);
+void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count);
+void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles);
+// in-place removal of bonds from K (stress.cpp:260-284): the listed bonds' products, subtracted
+void launch_remove_bonds(hipStream_t s, const AssemblyDev* job, const int32_t* ids, int count);
+// rows of retired nodes become id_scale I (stress.cpp:299-308)
+void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* rows, int count);
+
+// Inverse diagonal blocks: mode 0 the multigrid's (amg.cpp:228-254: regularised by 1e-9 tr on a
+// singular block; *fail set when that too is singular), mode 1 block-Jacobi's
+// (stress.cpp:152-157: (1 / id_scale) I on a singular block)
+struct DiagDev {
+  int32_t n;
+  int32_t mode;
+  const int32_t* rowptr;
+  const double* A;
+  double* Dinv;
+  const double* id_scale;
+  int32_t* fail;
+};
+void launch_inv_diag(hipStream_t s, const DiagDev* jobs, const Tile* tiles, int ntiles);
+
+// One coarsening step of one graph's level (amg.cpp:320-533)
+struct CoarsenDev {
+  // the fine level
+  int32_t n;
+  int32_t pad0_;
+  int64_t nnzb;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* A;
+  const double* Dinv;
+  const double* pos;   // 3 per row
+  double theta;
+  double omega;        // prolongation smoothing weight (0: unsmoothed)
+  // scratch over blocks / rows
+  double* s;           // block strength (nnzb)
+  uint8_t* strong;     // nnzb
+  int32_t* state;      // n
+  uint64_t* t1;        // n
+  uint64_t* t2;        // n
+  int32_t* rootof;     // n
+  int32_t* rootof2;    // n
+  int32_t* scan_a;     // n + 1 (anchors)
+  int32_t* scan_b;     // n + 1 (lonely)
+  int32_t* cursor;     // n + 1 (fill cursors)
+  int32_t* counts;     // 8 ints: [0] undecided, [1] anchors, [2] lonely, [3] na, [4] np, [5] nap, [6] nac
+  // results
+  int32_t* agg;        // n
+  int32_t na;
+  int32_t pad1_;
+  int32_t* acnt;       // na + 1
+  int32_t* amem;       // n (members by aggregate)
+  double* posc;        // 3 na
+  double* off;         // 3 n
+  int32_t* cand;       // candidate scratch
+  int64_t ncand;
+  int32_t* cptr;       // n + 1 (candidate offsets)
+  int32_t* Prow;       // n + 1
+  int32_t* Pcol;
+  double* P;
+  int64_t np;
+  int32_t* Rrow;       // na + 1
+  int32_t* Rfine;      // np
+  int32_t* Rent;       // np (the P entry of each restriction entry)
+  double* R;           // 36 np
+  int32_t* AProw;      // n + 1
+  int32_t* APcol;
+  double* AP;
+  int64_t nap;
+  int32_t* Crow;       // na + 1
+  int32_t* Ccol;
+  double* C;
+  int64_t nac;
+  double coarse_scale; // applied to C (1: none)
+};
+
+// MIS-2 aggregation
+void launch_strength(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_strong(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_mis_round(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_aggregate(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int count);
+// per-graph exclusive scans: scan_a / scan_b (n + 1 each) -> totals in counts[1], counts[2]
+void launch_scan_pair(hipStream_t s, const CoarsenDev* jobs, int count);
+void launch_assign(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_coarse_positions(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles,
+                             const Tile* ctiles, int nctiles, int count);
+// prolongator
+void launch_p_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_scan_rows(hipStream_t s, const CoarsenDev* jobs, int count, int which);
+void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
+                    int nctiles, int count);
+// Galerkin product
+void launch_ap_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_c_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+
+// power iteration for lambda_max(D^-1 A) (amg.cpp:256-275): 15 steps, per-graph norms
+struct PowerDev {
+  int32_t n;
+  int32_t pad_;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* A;
+  const double* Dinv;
+  double* v[2];
+  double* lam;   // 2 doubles: [0] the estimate, [1] the last norm
+};
+void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles);
+void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial);
+void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
+
+// the coarsest level's dense inverse (amg.cpp:535-578, then inverted)
+struct DenseDev {
+  int32_t n;
+  int32_t m;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* A;
+  double* L;      // m x m scratch
+  double* Minv;   // m x m
+};
+void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count);
+
+// FP32 copies
+void launch_to_f32(hipStream_t s, const double* in, float* out, int64_t count);
+
+// ---- bond checks (stress.cpp:14-57, 477-502) ---------------------------------------------------
+struct CheckDev {
+  int32_t nbonds;
+  int32_t nmats;
+  const svxg_bond* bonds;
+  const svxg_node* nodes;
+  const svxg_material* mats;
+  const double* u;   // 6 per node (all nodes)
+  double fragility;
+  svxg_bond_load* loads;
+  double* phi;
+  uint8_t* mode;
+};
+void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, int ntiles);
+// u over all nodes (6 per node) from the free-DOF solution
+void launch_scatter_u(hipStream_t s, const double* x, const int32_t* dof, double* u, int nnodes);
+
+}  // namespace svxg

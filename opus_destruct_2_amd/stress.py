@@ -1,0 +1,310 @@
+"""Python mirror of the reference's StressProblem (core/include/svx/stress/stress.hpp:74-160) over
+the device library's C ABI (include/svxg.h, libsvxg.so).
+
+Same names, argument meaning and results as the reference: assemble(options) -> bool,
+solve(f, u, rtol, maxit, warm) -> (u, PcgResult), begin / iterate / current, bond_load,
+remove_bond, retire_nodes, plus the batched entry solve_batch(). Nothing here computes: every call
+goes to the device; a missing or unloadable libsvxg.so raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .abi import (BOND, BOND_LOAD, NODE, SVXG_FP64, SVXG_MIXED, SVXG_OK, SVXG_SINGULAR, PcgResult, ProblemInfo,
+                  StressOptions, reference_options)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvxg.so")
+_LIB = None
+
+
+class SvxgError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded device library (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise SvxgError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        sig = {
+            "svxg_ctx_create": ([ctypes.c_int, i64, ctypes.POINTER(vp)], ctypes.c_int),
+            "svxg_ctx_destroy": ([vp], ctypes.c_int),
+            "svxg_ctx_bytes_in_use": ([vp], i64),
+            "svxg_last_error": ([], ctypes.c_char_p),
+            "svxg_default_options": ([vp], None),
+            "svxg_problem_create": ([vp, vp, i32, vp, i32, vp, i32, ctypes.POINTER(vp)], ctypes.c_int),
+            "svxg_problem_destroy": ([vp], ctypes.c_int),
+            "svxg_problem_assemble": ([vp, vp], ctypes.c_int),
+            "svxg_problem_release": ([vp], ctypes.c_int),
+            "svxg_problem_invalidate": ([vp], ctypes.c_int),
+            "svxg_problem_solve": ([vp, vp, vp, f64, i32, i32, vp], ctypes.c_int),
+            "svxg_problem_begin": ([vp, vp, vp], ctypes.c_int),
+            "svxg_problem_iterate": ([vp, i32, f64, vp], ctypes.c_int),
+            "svxg_problem_current": ([vp, vp], ctypes.c_int),
+            "svxg_problem_remove_bonds": ([vp, vp, i32], ctypes.c_int),
+            "svxg_problem_retire_nodes": ([vp, vp, i32], ctypes.c_int),
+            "svxg_problem_set_bond_broken": ([vp, i32, i32], ctypes.c_int),
+            "svxg_problem_bond_loads": ([vp, vp, vp], ctypes.c_int),
+            "svxg_problem_bond_checks": ([vp, vp, f64, vp, vp], ctypes.c_int),
+            "svxg_add_force": ([vp, i32, vp, vp, vp], None),
+            "svxg_problem_info_get": ([vp, vp], ctypes.c_int),
+            "svxg_problem_get_bonds": ([vp, vp], ctypes.c_int),
+            "svxg_problem_get_nodes": ([vp, vp], ctypes.c_int),
+            "svxg_solve_batch": ([vp, vp, i32, vp, vp, f64, i32, i32, vp], ctypes.c_int),
+            "svxg_problem_get_matrix": ([vp, vp, vp, vp], ctypes.c_int),
+            "svxg_problem_apply_matrix": ([vp, vp, vp, i32], ctypes.c_int),
+            "svxg_problem_apply_preconditioner": ([vp, vp, vp], ctypes.c_int),
+            "svxg_problem_last_timing": ([vp, ctypes.POINTER(f64), ctypes.POINTER(f64)], ctypes.c_int),
+            "svxg_ctx_set_profiling": ([vp, i32], ctypes.c_int),
+            "svxg_ctx_kernel_stats": ([vp, i32, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)],
+                                      ctypes.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _check(status, what):
+    if status != SVXG_OK:
+        msg = lib().svxg_last_error().decode(errors="replace")
+        raise SvxgError(f"{what}: status {status}: {msg}")
+
+
+def default_options(**kw):
+    """StressOptions{} (stress.hpp:68-72) as the device library defaults them."""
+    o = StressOptions()
+    lib().svxg_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if k in ("rtol", "amg_min_nodes", "precision"):
+            setattr(o, k, v)
+        else:
+            setattr(o.amg, k, v)
+    return o
+
+
+class Context:
+    """One GPU and its HBM budget (bytes; 0 = 200 GB)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = ctypes.c_void_p()
+        _check(lib().svxg_ctx_create(device, hbm_budget, ctypes.byref(h)), "svxg_ctx_create")
+        self.h = h
+
+    def bytes_in_use(self):
+        return lib().svxg_ctx_bytes_in_use(self.h)
+
+    def set_profiling(self, on):
+        _check(lib().svxg_ctx_set_profiling(self.h, int(bool(on))), "set_profiling")
+
+    def kernel_stats(self, kind):
+        """(launches, device ms, algorithmic bytes) of a profiled kernel kind since the last call."""
+        n, ms, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().svxg_ctx_kernel_stats(self.h, int(kind), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
+               "kernel_stats")
+        return n.value, ms.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().svxg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEFAULT_CTX = None
+
+
+def default_context():
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+class StressProblem:
+    """A fragment graph's stress problem on the device (StressProblem, stress.hpp:74-160)."""
+
+    def __init__(self, nodes, bonds, mats, ctx=None):
+        self.ctx = ctx or default_context()
+        self._nodes = np.ascontiguousarray(nodes, NODE)
+        self._bonds = np.ascontiguousarray(bonds, BOND)
+        self.mats = np.ascontiguousarray(mats)
+        h = ctypes.c_void_p()
+        _check(lib().svxg_problem_create(self.ctx.h, self._nodes.ctypes.data, len(self._nodes), self._bonds.ctypes.data,
+                                         len(self._bonds), self.mats.ctypes.data, len(self.mats), ctypes.byref(h)),
+               "svxg_problem_create")
+        self.h = h
+        self.options = default_options()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().svxg_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n(self):
+        return len(self._nodes)
+
+    @property
+    def nodes(self):
+        out = self._nodes.copy()
+        _check(lib().svxg_problem_get_nodes(self.h, out.ctypes.data), "get_nodes")
+        return out
+
+    @property
+    def bonds(self):
+        out = self._bonds.copy()
+        _check(lib().svxg_problem_get_bonds(self.h, out.ctypes.data), "get_bonds")
+        return out
+
+    def assemble(self, options=None):
+        """StressProblem::assemble (stress.cpp:134-176): False where the reference returns false."""
+        if options is not None:
+            self.options = options
+        st = lib().svxg_problem_assemble(self.h, ctypes.byref(self.options))
+        if st == SVXG_SINGULAR:
+            return False
+        _check(st, "svxg_problem_assemble")
+        return True
+
+    def release(self):
+        _check(lib().svxg_problem_release(self.h), "release")
+
+    def invalidate(self):
+        _check(lib().svxg_problem_invalidate(self.h), "invalidate")
+
+    def solve(self, f, u=None, rtol=1e-8, maxit=-1, warm=False):
+        """StressProblem::solve (stress.cpp:348-360): (u over all nodes, PcgResult as a dict)."""
+        f = np.ascontiguousarray(f, np.float64)
+        u = np.zeros(6 * self.n) if u is None else np.array(u, np.float64)
+        r = PcgResult()
+        _check(lib().svxg_problem_solve(self.h, f.ctypes.data, u.ctypes.data, rtol, int(maxit), int(bool(warm)),
+                                        ctypes.byref(r)), "svxg_problem_solve")
+        return u, r.as_dict()
+
+    def begin(self, f, u0=None):
+        f = np.ascontiguousarray(f, np.float64)
+        u0 = None if u0 is None else np.ascontiguousarray(u0, np.float64)
+        _check(lib().svxg_problem_begin(self.h, f.ctypes.data, None if u0 is None else u0.ctypes.data), "begin")
+
+    def iterate(self, maxit, rtol):
+        r = PcgResult()
+        _check(lib().svxg_problem_iterate(self.h, int(maxit), float(rtol), ctypes.byref(r)), "iterate")
+        return r.as_dict()
+
+    def current(self):
+        u = np.zeros(6 * self.n)
+        _check(lib().svxg_problem_current(self.h, u.ctypes.data), "current")
+        return u
+
+    def remove_bond(self, b):
+        self.remove_bonds([b])
+
+    def remove_bonds(self, ids):
+        ids = np.ascontiguousarray(ids, np.int32)
+        _check(lib().svxg_problem_remove_bonds(self.h, ids.ctypes.data, len(ids)), "remove_bonds")
+
+    def retire_nodes(self, ids):
+        ids = np.ascontiguousarray(ids, np.int32)
+        _check(lib().svxg_problem_retire_nodes(self.h, ids.ctypes.data, len(ids)), "retire_nodes")
+
+    def set_bond_broken(self, b, broken):
+        _check(lib().svxg_problem_set_bond_broken(self.h, int(b), int(bool(broken))), "set_bond_broken")
+
+    def bond_loads(self, u=None):
+        """bond_load of every bond (stress.cpp:477-502); u None: the last solve's, resident."""
+        out = np.zeros(len(self._bonds), BOND_LOAD)
+        uu = None if u is None else np.ascontiguousarray(u, np.float64)
+        _check(lib().svxg_problem_bond_loads(self.h, None if uu is None else uu.ctypes.data, out.ctypes.data),
+               "bond_loads")
+        return out
+
+    def bond_load(self, b, u=None):
+        return self.bond_loads(u)[b]
+
+    def bond_checks(self, u=None, fragility=1.0):
+        """(phi, mode) of every bond: bond_utilization after bond_load (stress.cpp:14-57)."""
+        phi = np.zeros(len(self._bonds))
+        mode = np.zeros(len(self._bonds), np.uint8)
+        uu = None if u is None else np.ascontiguousarray(u, np.float64)
+        _check(lib().svxg_problem_bond_checks(self.h, None if uu is None else uu.ctypes.data, float(fragility),
+                                              phi.ctypes.data, mode.ctypes.data), "bond_checks")
+        return phi, mode
+
+    def info(self):
+        i = ProblemInfo()
+        _check(lib().svxg_problem_info_get(self.h, ctypes.byref(i)), "info")
+        return i.as_dict()
+
+    def work_per_iteration(self):
+        return self.info()["work_per_iteration"]
+
+    def matrix(self):
+        inf = self.info()
+        rowptr = np.zeros(inf["nfree"] + 1, np.int32)
+        col = np.zeros(inf["blocks"], np.int32)
+        val = np.zeros((inf["blocks"], 6, 6))
+        _check(lib().svxg_problem_get_matrix(self.h, rowptr.ctypes.data, col.ctypes.data, val.ctypes.data), "matrix")
+        return rowptr, col, val
+
+    def apply_matrix(self, x, fp32=False):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        _check(lib().svxg_problem_apply_matrix(self.h, x.ctypes.data, y.ctypes.data, int(bool(fp32))), "apply_matrix")
+        return y
+
+    def apply_preconditioner(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        _check(lib().svxg_problem_apply_preconditioner(self.h, r.ctypes.data, z.ctypes.data), "apply_preconditioner")
+        return z
+
+    def last_timing(self):
+        s, a = ctypes.c_double(), ctypes.c_double()
+        _check(lib().svxg_problem_last_timing(self.h, ctypes.byref(s), ctypes.byref(a)), "last_timing")
+        return {"solve_ms": s.value, "setup_ms": a.value}
+
+
+def add_force(f, node, c, F, at):
+    """StressProblem::add_force (stress.cpp:504-513) on a host load vector."""
+    c, F, at = (np.ascontiguousarray(v, np.float64) for v in (c, F, at))
+    lib().svxg_add_force(f.ctypes.data, int(node), c.ctypes.data, F.ctypes.data, at.ctypes.data)
+
+
+def solve_batch(problems, fs, us=None, rtol=1e-8, maxit=-1, warm=False, ctx=None):
+    """Solve many problems in one set of device launches; returns (list of u, list of results)."""
+    ctx = ctx or problems[0].ctx
+    n = len(problems)
+    fs = [np.ascontiguousarray(f, np.float64) for f in fs]
+    if us is None:
+        us = [np.zeros(6 * p.n) for p in problems]
+    else:
+        us = [np.array(u, np.float64) for u in us]
+    ph = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    fp = (ctypes.c_void_p * n)(*[f.ctypes.data for f in fs])
+    up = (ctypes.c_void_p * n)(*[u.ctypes.data for u in us])
+    res = (PcgResult * n)()
+    _check(lib().svxg_solve_batch(ctx.h, ph, n, fp, up, rtol, int(maxit), int(bool(warm)), res), "svxg_solve_batch")
+    return us, [r.as_dict() for r in res]
+
+
+__all__ = ["StressProblem", "Context", "default_context", "default_options", "reference_options", "solve_batch",
+           "add_force", "lib", "SvxgError", "SVXG_FP64", "SVXG_MIXED"]

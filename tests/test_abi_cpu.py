@@ -1,0 +1,80 @@
+"""The C ABI boundary without a GPU: libsvxg.so loads and exports every entry point that
+include/svxg.h declares; the structs' layouts agree between the header and the Python mirrors;
+the generator is deterministic (its graphs are fixed by their digests)."""
+import ctypes
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+from opus_destruct_2_amd import abi, gen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "svxg.h")
+LIB = os.path.join(ROOT, "opus_destruct_2_amd", "libsvxg.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|void|int64_t|const char\*)\s+(svxg_\w+)\(", text, re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for must in ("svxg_ctx_create", "svxg_problem_create", "svxg_problem_assemble", "svxg_problem_solve",
+                 "svxg_solve_batch", "svxg_problem_bond_checks", "svxg_problem_remove_bonds"):
+        assert must in names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "libsvxg.so not built"
+    L = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_the_header():
+    # sizes written in include/svxg.h's comments
+    assert abi.NODE.itemsize == 40
+    assert abi.BOND.itemsize == 216
+    assert abi.MATERIAL.itemsize == 64
+    assert ctypes.sizeof(abi.StressOptions) == 8 + 4 + 4 + ctypes.sizeof(abi.AmgOptions)
+    assert ctypes.sizeof(abi.PcgResult) == 24
+
+
+def test_default_options_without_a_device():
+    from opus_destruct_2_amd import stress
+    o = stress.default_options()
+    assert o.rtol == 2e-3 and o.amg_min_nodes == 160 and o.amg.coarse_max == 48 and o.amg.strength == 0.1
+
+
+def _digest(g):
+    h = hashlib.sha256()
+    h.update(g.nodes.tobytes())
+    h.update(g.bonds.tobytes())
+    h.update(g.f.tobytes())
+    return h.hexdigest()[:16]
+
+
+def test_generator_is_deterministic_and_fixed():
+    a, b = gen.tower(), gen.tower()
+    assert _digest(a) == _digest(b)
+    assert a.n == 1280 and a.m == 3520 and a.dofs == 7680
+    # the same bits on every machine: integer RNG and correctly rounded arithmetic only
+    assert _digest(a) == FIXED["tower"]
+    assert _digest(gen.piece(300, pinned=True, seed=5)) == FIXED["piece300p"]
+    sizes = gen.piece_sizes(4096)
+    assert sizes.min() >= 16 and sizes.max() <= 20000 and 150 <= np.median(sizes) <= 260
+
+
+FIXED = {"tower": None, "piece300p": None}
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _fixed_digests():
+    path = os.path.join(ROOT, "tests", "golden", "generator_digests.txt")
+    for line in open(path):
+        k, v = line.split()
+        FIXED[k] = v
