@@ -170,9 +170,11 @@ struct Batch {
   DevBuf gtile;
   DevBuf tj;
   int ntj = 0;
+  // per level: smoothing tiles (S = 1 and S = 10 sub-groups a row), prolongation tiles, restriction
+  // tiles (coarse rows, S = 1 / 10), stalled-coarsest tiles (S = 1 / 10), dense-coarsest graphs
   struct Lv {
-    DevBuf ts, tc, tstall, dense;
-    int nts = 0, ntc = 0, ntstall = 0, ndense = 0;
+    DevBuf ts[2], tp, tc[2], tstall[2], dense;
+    int nts[2] = {0, 0}, ntp = 0, ntc[2] = {0, 0}, ntstall[2] = {0, 0}, ndense = 0;
   };
   std::vector<Lv> lvl;
   DevBuf partial;
@@ -235,6 +237,9 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
   }
 }
 
+// rows averaging more than 16 entries are served a wavefront each (S = 10 sub-groups)
+static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entries > 16 * rows; }
+
 static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A, hipStream_t s) {
   B.probs.assign(probs, probs + count);
   B.G = count;
@@ -268,29 +273,36 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   B.lvl.clear();
   B.lvl.resize(size_t(B.maxlev));
   for (int l = 0; l < B.maxlev; ++l) {
-    std::vector<Tile> ts, tc, tstall;
+    std::vector<Tile> ts[2], tp, tc[2], tstall[2];
     std::vector<int32_t> dense;
     for (int g = 0; g < count; ++g) {
       svxg_problem* P = probs[g];
       if (P->jacobi_only || l >= int(P->lv.size())) continue;
       const HostLevel& H = P->lv[size_t(l)];
+      const int vs = long_rows(H.nnzb, H.n) ? 1 : 0;
       if (!H.terminal) {
-        add_tiles(ts, g, H.n, kTileRows);
-        add_tiles(tc, g, P->lv[size_t(l) + 1].n, kTileRows);
+        add_tiles(ts[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
+        add_tiles(tp, g, H.n, kTileRows);
+        const int vc = long_rows(H.np, H.nc) ? 1 : 0;
+        add_tiles(tc[vc], g, P->lv[size_t(l) + 1].n, vc ? tile_rows<10>() : tile_rows<1>());
       } else if (H.dense_m > 0) {
         dense.push_back(g);
       } else {
-        add_tiles(tstall, g, H.n, kTileRows);
+        add_tiles(tstall[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
       }
     }
     Batch::Lv& V = B.lvl[size_t(l)];
-    V.nts = int(ts.size());
-    V.ntc = int(tc.size());
-    V.ntstall = int(tstall.size());
+    for (int v = 0; v < 2; ++v) {
+      V.nts[v] = int(ts[v].size());
+      V.ntc[v] = int(tc[v].size());
+      V.ntstall[v] = int(tstall[v].size());
+      SVXG_TRY(upload(V.ts[v], A, ts[v], s));
+      SVXG_TRY(upload(V.tc[v], A, tc[v], s));
+      SVXG_TRY(upload(V.tstall[v], A, tstall[v], s));
+    }
+    V.ntp = int(tp.size());
     V.ndense = int(dense.size());
-    SVXG_TRY(upload(V.ts, A, ts, s));
-    SVXG_TRY(upload(V.tc, A, tc, s));
-    SVXG_TRY(upload(V.tstall, A, tstall, s));
+    SVXG_TRY(upload(V.tp, A, tp, s));
     SVXG_TRY(upload(V.dense, A, dense, s));
   }
   return SVXG_OK;
@@ -367,31 +379,38 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   if (B.ntj) launch_jacobi<T>(s, G, B.tj.as<Tile>(), B.ntj, inner);
   if (!B.any_amg) return;
   const int L = B.maxlev;
+  const int SV[2] = {1, 10};
   for (int l = 0; l < L; ++l) {
     Batch::Lv& V = B.lvl[size_t(l)];
-    if (V.nts) {
+    for (int v = 0; v < 2; ++v) {
+      if (!V.nts[v]) continue;
       for (int st = 0; st < degree; ++st) {
         ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
-        launch_cheb_step<T>(s, G, V.ts.as<Tile>(), V.nts, l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr);
+        launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr,
+                            SV[v]);
       }
-      launch_residual<T>(s, G, V.ts.as<Tile>(), V.nts, l, (degree - 1) & 1, inner);
-      launch_restrict<T>(s, G, V.tc.as<Tile>(), V.ntc, l, inner);
+      launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (degree - 1) & 1, inner, SV[v]);
     }
+    for (int v = 0; v < 2; ++v) launch_restrict<T>(s, G, V.tc[v].as<Tile>(), V.ntc[v], l, inner, SV[v]);
     if (V.ndense) launch_dense_solve<T>(s, G, V.dense.as<int32_t>(), V.ndense, l, inner);
-    if (V.ntstall)
-      for (int st = 0; st < kStallDegree; ++st)
-        launch_cheb_step<T>(s, G, V.tstall.as<Tile>(), V.ntstall, l, st, (st + 1) & 1, st & 1, st == 0, inner,
-                            nullptr);
+    for (int v = 0; v < 2; ++v)
+      if (V.ntstall[v])
+        for (int st = 0; st < kStallDegree; ++st)
+          launch_cheb_step<T>(s, G, V.tstall[v].as<Tile>(), V.ntstall[v], l, st, (st + 1) & 1, st & 1, st == 0, inner,
+                              nullptr, SV[v]);
   }
   for (int l = L - 2; l >= 0; --l) {
     Batch::Lv& V = B.lvl[size_t(l)];
-    if (!V.nts) continue;
+    if (!V.ntp) continue;
     const int cur = (degree - 1) & 1;
-    launch_prolong<T>(s, G, V.ts.as<Tile>(), V.nts, l, cur, inner);
+    launch_prolong<T>(s, G, V.tp.as<Tile>(), V.ntp, l, cur, inner);
     for (int j = 0; j < degree; ++j) {
       const int gs = degree + j;
-      ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
-      launch_cheb_step<T>(s, G, V.ts.as<Tile>(), V.nts, l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr);
+      for (int v = 0; v < 2; ++v) {
+        if (!V.nts[v]) continue;
+        ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
+        launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, SV[v]);
+      }
     }
   }
 }
@@ -452,15 +471,21 @@ static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
   const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
+  const int I32 = SERVE_INNER32, I64 = SERVE_INNER64, IN = SERVE_INNER;
   for (int outer = 0; outer < 100; ++outer) {
     launch_outer_residual(s, G, t0, B.nt0, part);
     launch_finish(s, FinishKind::Outer, G, gt, B.G, 0, part);
     SVXG_TRY(read_states(B, s));
     if (!any_running(B, false)) break;
+    bool any32 = false, any64 = false;
+    for (const PcgState& st : B.hstate)
+      if (st.state == ST_RUNNING) (st.inner64 ? any64 : any32) = true;
     launch_inner_init(s, G, t0, B.nt0);
-    vcycle<float>(B, s, 1, degree);
-    launch_start_p<float>(s, G, t0, B.nt0, 1, part);
-    launch_finish(s, FinishKind::Start, G, gt, B.G, 1, part);
+    vcycle<float>(B, s, IN, degree);
+    if (any64) launch_convert64(s, G, t0, B.nt0, 0);
+    if (any32) launch_start_p<float>(s, G, t0, B.nt0, I32, part);
+    if (any64) launch_start_p<double>(s, G, t0, B.nt0, I64, part);
+    launch_finish(s, FinishKind::Start, G, gt, B.G, IN, part);
     for (int it = 0;; ++it) {
       if (it % kPollEvery == 0) {
         SVXG_TRY(read_states(B, s));
@@ -468,15 +493,21 @@ static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
       }
       {
         ProfScope ps(B, 1, s);
-        launch_pcg_spmv<float>(s, G, t0, B.nt0, 1, part);
+        if (any32) launch_pcg_spmv<float>(s, G, t0, B.nt0, I32, part);
       }
-      launch_finish(s, FinishKind::Alpha, G, gt, B.G, 1, part);
-      launch_axpy_rr<float>(s, G, t0, B.nt0, 1, part);
-      launch_finish(s, FinishKind::Rn, G, gt, B.G, 1, part);
-      vcycle<float>(B, s, 1, degree);
-      launch_dot_rz<float>(s, G, t0, B.nt0, 1, part);
-      launch_finish(s, FinishKind::Beta, G, gt, B.G, 1, part);
-      launch_update_p<float>(s, G, t0, B.nt0, 1);
+      if (any64) launch_pcg_spmv<double>(s, G, t0, B.nt0, I64, part);
+      launch_finish(s, FinishKind::Alpha, G, gt, B.G, IN, part);
+      if (any32) launch_axpy_rr<float>(s, G, t0, B.nt0, I32, part);
+      if (any64) launch_axpy_rr<double>(s, G, t0, B.nt0, I64, part);
+      launch_finish(s, FinishKind::Rn, G, gt, B.G, IN, part);
+      if (any64) launch_convert64(s, G, t0, B.nt0, 1);
+      vcycle<float>(B, s, IN, degree);
+      if (any64) launch_convert64(s, G, t0, B.nt0, 0);
+      if (any32) launch_dot_rz<float>(s, G, t0, B.nt0, I32, part);
+      if (any64) launch_dot_rz<double>(s, G, t0, B.nt0, I64, part);
+      launch_finish(s, FinishKind::Beta, G, gt, B.G, IN, part);
+      if (any32) launch_update_p<float>(s, G, t0, B.nt0, I32);
+      if (any64) launch_update_p<double>(s, G, t0, B.nt0, I64);
     }
     launch_inner_apply(s, G, t0, B.nt0);
     launch_inner_close(s, G, B.G);

@@ -281,21 +281,28 @@ __global__ void k_diag_trace(const AssemblyDev* __restrict__ J, const Tile* __re
   a.trace[i] = d[0] + d[7] + d[14];
 }
 
-// id_scale = mean of tr(K_ii[0:3,0:3]) / 3 over rows with a positive trace, in row order
+// id_scale = mean of tr(K_ii[0:3,0:3]) / 3 over rows with a positive trace, summed in row order
+// (one wavefront a graph: 64 traces loaded at a time, added by lane 0 in order)
 __global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x;
   if (g >= count) return;
   const AssemblyDev& a = J[g];
+  const int l = threadIdx.x;
   double tr = 0.0;
   int cnt = 0;
-  for (int i = 0; i < a.nfree; ++i) {
-    const double t = a.trace[i];
-    if (t > 0) {
-      tr += t / 3.0;
-      ++cnt;
+  for (int base = 0; base < a.nfree; base += 64) {
+    const int i = base + l;
+    const double t = i < a.nfree ? a.trace[i] : 0.0;
+    const int m = min(64, a.nfree - base);
+    for (int k = 0; k < m; ++k) {
+      const double v = __shfl(t, k, 64);
+      if (v > 0) {
+        tr += v / 3.0;
+        ++cnt;
+      }
     }
   }
-  *a.id_scale = cnt ? tr / cnt : 1.0;
+  if (l == 0) *a.id_scale = cnt ? tr / cnt : 1.0;
 }
 
 __global__ void k_identity_rows(const AssemblyDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
@@ -629,50 +636,57 @@ __global__ void k_scan_rows(const CoarsenDev* __restrict__ J, int count, int whi
   }
 }
 
-// P's blocks: (J == agg(i) ? P_rigid,i : 0) - w sum over couplings k of row i with agg(col k) = J of
-// D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order
-__global__ void k_p_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+// P's columns from the row's sorted unique candidates, and each entry's row
+__global__ void k_p_cols(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
   if (int(threadIdx.x) >= tl.nrows) return;
   const int i = tl.row0 + threadIdx.x;
-  const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
-  const int32_t* cd = c.cand + k0;
-  const int nu = c.Prow[i + 1] - c.Prow[i];
+  const int32_t* cd = c.cand + c.rowptr[i];
+  for (int e = c.Prow[i]; e < c.Prow[i + 1]; ++e) {
+    c.Pcol[e] = cd[e - c.Prow[i]];
+    c.rid[e] = i;
+  }
+}
+
+// P's blocks, one entry a thread: (J == agg(i) ? P_rigid,i : 0) - w sum over couplings k of row i
+// with agg(col k) = J of D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order
+__global__ void k_p_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = etiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int e = tl.row0 + threadIdx.x;
+  const int i = c.rid[e];
+  const int Jc = c.Pcol[e];
   const double* Di = c.Dinv + 36 * size_t(i);
   const double* offi = c.off + 3 * size_t(i);
-  const int ai = c.agg[i];
-  for (int u = 0; u < nu; ++u) {
-    const int Jc = cd[u];
-    const size_t e = size_t(c.Prow[i] + u);
-    c.Pcol[e] = Jc;
-    double acc[36];
-    for (int q = 0; q < 36; ++q) acc[q] = (Jc == ai) ? rigid_at(offi, q / 6, q % 6) : 0.0;
-    if (c.omega != 0.0) {
-      for (int k = k0; k < k1; ++k) {
-        const int j = c.col[k];
-        if (c.agg[j] != Jc) continue;
-        const double* A = c.A + 36 * size_t(k);
-        const double* offj = c.off + 3 * size_t(j);
-        for (int col = 0; col < 6; ++col) {
-          double T[6];
-          for (int r = 0; r < 6; ++r) {
-            double s = 0.0;
-            for (int p = 0; p < 6; ++p) s += A[r * 6 + p] * rigid_at(offj, p, col);
-            T[r] = s;
-          }
-          for (int r = 0; r < 6; ++r) {
-            double s = 0.0;
-            for (int p = 0; p < 6; ++p) s += Di[r * 6 + p] * T[p];
-            acc[r * 6 + col] += -c.omega * s;
-          }
+  double acc[36];
+  for (int q = 0; q < 36; ++q) acc[q] = (Jc == c.agg[i]) ? rigid_at(offi, q / 6, q % 6) : 0.0;
+  if (c.omega != 0.0) {
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      if (c.agg[j] != Jc) continue;
+      const double* A = c.A + 36 * size_t(k);
+      const double* offj = c.off + 3 * size_t(j);
+      for (int col = 0; col < 6; ++col) {
+        double T[6];
+        for (int r = 0; r < 6; ++r) {
+          double s = 0.0;
+          for (int p = 0; p < 6; ++p) s += A[r * 6 + p] * rigid_at(offj, p, col);
+          T[r] = s;
+        }
+        for (int r = 0; r < 6; ++r) {
+          double s = 0.0;
+          for (int p = 0; p < 6; ++p) s += Di[r * 6 + p] * T[p];
+          acc[r * 6 + col] += -c.omega * s;
         }
       }
     }
-    double* out = c.P + 36 * e;
-    for (int q = 0; q < 36; ++q) out[q] = acc[q];
   }
+  double* out = c.P + 36 * size_t(e);
+  for (int q = 0; q < 36; ++q) out[q] = acc[q];
 }
 
 // restriction lists: P's entries by coarse column, fine rows ascending
@@ -745,7 +759,8 @@ __global__ void k_ap_cand_count(const CoarsenDev* __restrict__ J, const Tile* __
   c.cptr[i] = n;
 }
 
-__global__ void k_ap_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+// A P's candidate columns of row i (unsorted): the P columns of each coupled row
+__global__ void k_ap_cand_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
@@ -757,52 +772,16 @@ __global__ void k_ap_count(const CoarsenDev* __restrict__ J, const Tile* __restr
     const int j = c.col[k];
     for (int e = c.Prow[j]; e < c.Prow[j + 1]; ++e) cd[n++] = c.Pcol[e];
   }
-  sort_ints(cd, n);
-  c.AProw[i] = unique_sorted(cd, n);
 }
 
-// (A P)_iJ = sum over couplings k, P entries e of col k with column J of A_k P_e, in (k, e) order
-__global__ void k_ap_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
-  if (int(blockIdx.x) >= ntiles) return;
-  const Tile tl = tiles[blockIdx.x];
-  const CoarsenDev& c = J[tl.g];
-  if (int(threadIdx.x) >= tl.nrows) return;
-  const int i = tl.row0 + threadIdx.x;
-  const int32_t* cd = c.cand + c.cptr[i];
-  const int nu = c.AProw[i + 1] - c.AProw[i];
-  for (int u = 0; u < nu; ++u) {
-    const int Jc = cd[u];
-    const size_t x = size_t(c.AProw[i] + u);
-    c.APcol[x] = Jc;
-    double acc[36];
-    for (int q = 0; q < 36; ++q) acc[q] = 0.0;
-    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
-      const int j = c.col[k];
-      for (int e = c.Prow[j]; e < c.Prow[j + 1]; ++e) {
-        if (c.Pcol[e] != Jc) continue;
-        const double* A = c.A + 36 * size_t(k);
-        const double* P = c.P + 36 * size_t(e);
-        for (int r = 0; r < 6; ++r)
-          for (int col = 0; col < 6; ++col) {
-            double s = 0.0;
-            for (int q = 0; q < 6; ++q) s += A[r * 6 + q] * P[q * 6 + col];
-            acc[r * 6 + col] = acc[r * 6 + col] + s;
-          }
-      }
-    }
-    double* out = c.AP + 36 * x;
-    for (int q = 0; q < 36; ++q) out[q] = acc[q];
-  }
-}
-
-// candidate counts of the coarse row I: the A P rows of its restriction entries
+// candidate counts of the coarse row I: its diagonal and the A P rows of its restriction entries
 __global__ void k_c_cand_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
   if (int(blockIdx.x) >= nctiles) return;
   const Tile tl = ctiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
   if (int(threadIdx.x) >= tl.nrows) return;
   const int I = tl.row0 + threadIdx.x;
-  int n = 1;  // (the diagonal, always present)
+  int n = 1;
   for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
     const int i = c.Rfine[r];
     n += c.AProw[i + 1] - c.AProw[i];
@@ -810,8 +789,8 @@ __global__ void k_c_cand_count(const CoarsenDev* __restrict__ J, const Tile* __r
   c.cptr[I] = n;
 }
 
-// coarse row I's columns: diagonal first, then ascending (Bsr6Builder::finish)
-__global__ void k_c_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
+// the coarse row I's candidate columns (unsorted): I, then the A P columns of its restriction rows
+__global__ void k_c_cand_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
   if (int(blockIdx.x) >= nctiles) return;
   const Tile tl = ctiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
@@ -824,54 +803,185 @@ __global__ void k_c_count(const CoarsenDev* __restrict__ J, const Tile* __restri
     const int i = c.Rfine[r];
     for (int x = c.AProw[i]; x < c.AProw[i + 1]; ++x) cd[n++] = c.APcol[x];
   }
-  sort_ints(cd, n);
-  int u = unique_sorted(cd, n);
-  // move the diagonal to the front
-  int at = 0;
-  while (cd[at] != I) ++at;
-  for (int t = at; t > 0; --t) cd[t] = cd[t - 1];
-  cd[0] = I;
-  c.Crow[I] = u;
 }
 
-// (P^T A P)_IJ = sum over restriction entries (i, e) of I, in order, of P_e^T (A P)_iJ; times the
-// coarse scale (unsmoothed levels)
-__global__ void k_c_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
-  if (int(blockIdx.x) >= nctiles) return;
-  const Tile tl = ctiles[blockIdx.x];
+constexpr int kHashSlots = 16384;  // LDS hash set (64 KB)
+constexpr int kSortSlots = 4096;   // LDS sort buffer (16 KB): at most this many distinct columns a row
+
+// Each row's candidate segment (cand[cptr[r] ..]) made its sorted distinct columns in place, their
+// count in out[r] (diagonal first when `diag`: coarse rows). One workgroup per row of the tile: an
+// LDS hash set (order-independent, so the result is deterministic), then a bitonic sort.
+__global__ void __launch_bounds__(kWG) k_unique_rows(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                     int ntiles, int which) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  __shared__ int32_t tab[kHashSlots];
+  __shared__ int32_t keys[kSortSlots];
+  __shared__ int32_t wsum[4];
+  __shared__ int32_t total;
+  int32_t* out = which == 0 ? c.AProw : c.Crow;
+  for (int rr = 0; rr < tl.nrows; ++rr) {
+    const int r = tl.row0 + rr;
+    int32_t* cd = c.cand + c.cptr[r];
+    const int len = c.cptr[r + 1] - c.cptr[r];
+    int H = 64;
+    while (H < 2 * len && H < kHashSlots) H <<= 1;
+    for (int h = threadIdx.x; h < H; h += kWG) tab[h] = -1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < len; k += kWG) {
+      const int32_t key = cd[k];
+      uint32_t h = (uint32_t(key) * 2654435761U) & uint32_t(H - 1);
+      for (;;) {
+        const int32_t old = atomicCAS(&tab[h], -1, key);
+        if (old == -1 || old == key) break;
+        h = (h + 1) & uint32_t(H - 1);
+      }
+    }
+    __syncthreads();
+    // compact the occupied slots (slot order; sorted below)
+    const int per = (H + kWG - 1) / kWG;
+    int mine = 0;
+    for (int h = threadIdx.x * per; h < min(H, (threadIdx.x + 1) * per); ++h) mine += tab[h] >= 0;
+    int incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if ((threadIdx.x & 63) >= o) incl += t;
+    }
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < int(threadIdx.x >> 6); ++w) woff += wsum[w];
+    if (threadIdx.x == 0) total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int u = total;
+    if (u > kSortSlots) {
+      if (threadIdx.x == 0) {
+        atomicAdd(&c.counts[3], 1);
+        out[r] = 0;
+      }
+      __syncthreads();
+      continue;
+    }
+    int at = woff + incl - mine;
+    for (int h = threadIdx.x * per; h < min(H, (threadIdx.x + 1) * per); ++h)
+      if (tab[h] >= 0) keys[at++] = tab[h];
+    int P2 = 1;
+    while (P2 < u) P2 <<= 1;
+    for (int k = u + threadIdx.x; k < P2; k += kWG) keys[k] = 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= P2; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = threadIdx.x; t < P2 / 2; t += kWG) {
+          const int lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+          const bool up = ((lo & size) == 0);
+          const int32_t a0 = keys[lo], a1 = keys[hi];
+          if ((a0 > a1) == up) {
+            keys[lo] = a1;
+            keys[hi] = a0;
+          }
+        }
+        __syncthreads();
+      }
+    if (which == 1) {
+      // the diagonal first, then ascending (Bsr6Builder::finish)
+      if (threadIdx.x == 0) {
+        int d = 0;
+        while (d < u && keys[d] != r) ++d;
+        for (int t = d; t > 0; --t) keys[t] = keys[t - 1];
+        keys[0] = r;
+      }
+      __syncthreads();
+    }
+    for (int k = threadIdx.x; k < u; k += kWG) cd[k] = keys[k];
+    if (threadIdx.x == 0) out[r] = u;
+    __syncthreads();
+  }
+}
+
+// the pattern from the rows' sorted distinct candidates, and each entry's row
+__global__ void k_pattern_cols(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles,
+                               int which) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
   if (int(threadIdx.x) >= tl.nrows) return;
-  const int I = tl.row0 + threadIdx.x;
-  const int32_t* cd = c.cand + c.cptr[I];
-  const int nu = c.Crow[I + 1] - c.Crow[I];
-  for (int u = 0; u < nu; ++u) {
-    const int Jc = cd[u];
-    const size_t y = size_t(c.Crow[I] + u);
-    c.Ccol[y] = Jc;
-    double acc[36];
-    for (int q = 0; q < 36; ++q) acc[q] = 0.0;
-    for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
-      const int i = c.Rfine[r];
-      // the A P entry of row i in column Jc (columns ascending)
-      int lo = c.AProw[i], hi = c.AProw[i + 1];
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (c.APcol[mid] < Jc) lo = mid + 1;
-        else hi = mid;
-      }
-      if (lo >= c.AProw[i + 1] || c.APcol[lo] != Jc) continue;
-      const double* P = c.P + 36 * size_t(c.Rent[r]);
-      const double* X = c.AP + 36 * size_t(lo);
-      for (int rr = 0; rr < 6; ++rr)
-        for (int col = 0; col < 6; ++col) {
-          double s = 0.0;
-          for (int q = 0; q < 6; ++q) s += P[q * 6 + rr] * X[q * 6 + col];
-          acc[rr * 6 + col] = acc[rr * 6 + col] + s;
-        }
-    }
-    double* out = c.C + 36 * y;
-    for (int q = 0; q < 36; ++q) out[q] = c.coarse_scale != 1.0 ? acc[q] * c.coarse_scale : acc[q];
+  const int r = tl.row0 + threadIdx.x;
+  const int32_t* cd = c.cand + c.cptr[r];
+  const int32_t* ptr = which == 0 ? c.AProw : c.Crow;
+  int32_t* cols = which == 0 ? c.APcol : c.Ccol;
+  for (int x = ptr[r]; x < ptr[r + 1]; ++x) {
+    cols[x] = cd[x - ptr[r]];
+    c.rid[x] = r;
   }
+}
+
+// (A P)_iJ, one entry a thread: the sum over couplings k of row i (in order) of A_k P_e, e the entry
+// of column J in P's row col k (P rows ascending: a binary search)
+__global__ void k_ap_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = etiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int x = tl.row0 + threadIdx.x;
+  const int i = c.rid[x];
+  const int Jc = c.APcol[x];
+  double acc[36];
+  for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+    const int j = c.col[k];
+    int lo = c.Prow[j], hi = c.Prow[j + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (c.Pcol[mid] < Jc) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= c.Prow[j + 1] || c.Pcol[lo] != Jc) continue;
+    const double* A = c.A + 36 * size_t(k);
+    const double* P = c.P + 36 * size_t(lo);
+    for (int r = 0; r < 6; ++r)
+      for (int col = 0; col < 6; ++col) {
+        double s = 0.0;
+        for (int q = 0; q < 6; ++q) s += A[r * 6 + q] * P[q * 6 + col];
+        acc[r * 6 + col] = acc[r * 6 + col] + s;
+      }
+  }
+  double* out = c.AP + 36 * size_t(x);
+  for (int q = 0; q < 36; ++q) out[q] = acc[q];
+}
+
+// (P^T A P)_IJ, one entry a thread: the sum over restriction entries (i, e) of I, in order, of
+// P_e^T (A P)_iJ; times the coarse scale (unsmoothed levels)
+__global__ void k_c_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = etiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  if (int(threadIdx.x) >= tl.nrows) return;
+  const int y = tl.row0 + threadIdx.x;
+  const int I = c.rid[y];
+  const int Jc = c.Ccol[y];
+  double acc[36];
+  for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+  for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
+    const int i = c.Rfine[r];
+    int lo = c.AProw[i], hi = c.AProw[i + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (c.APcol[mid] < Jc) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= c.AProw[i + 1] || c.APcol[lo] != Jc) continue;
+    const double* P = c.P + 36 * size_t(c.Rent[r]);
+    const double* X = c.AP + 36 * size_t(lo);
+    for (int rr = 0; rr < 6; ++rr)
+      for (int col = 0; col < 6; ++col) {
+        double s = 0.0;
+        for (int q = 0; q < 6; ++q) s += P[q * 6 + rr] * X[q * 6 + col];
+        acc[rr * 6 + col] = acc[rr * 6 + col] + s;
+      }
+  }
+  double* out = c.C + 36 * size_t(y);
+  for (int q = 0; q < 36; ++q) out[q] = c.coarse_scale != 1.0 ? acc[q] * c.coarse_scale : acc[q];
 }
 
 // ---- power iteration (6 lanes a row, kTileRows-row tiles) ------------------------------------------
@@ -1132,7 +1242,7 @@ void launch_diag_trace(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles
   if (ntiles > 0) hipLaunchKernelGGL(k_diag_trace, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
 void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count) {
-  if (count > 0) hipLaunchKernelGGL(k_id_scale, dim3((count + 63) / 64), dim3(64), 0, s, jobs, count);
+  if (count > 0) hipLaunchKernelGGL(k_id_scale, dim3(count), dim3(64), 0, s, jobs, count);
 }
 void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_identity_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
@@ -1191,9 +1301,11 @@ void launch_scan_rows(hipStream_t s, const CoarsenDev* jobs, int count, int whic
   if (count <= 0) return;
   hipLaunchKernelGGL(k_scan_rows, dim3(count), dim3(kWG), 0, s, jobs, count, which);
 }
-void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_p_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+void launch_p_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_p_cols, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_p_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
 }
 void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
                     int nctiles, int count) {
@@ -1203,29 +1315,29 @@ void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, in
   hipLaunchKernelGGL(k_r_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
   hipLaunchKernelGGL(k_r_sort, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
-void launch_ap_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_ap_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
-}
 void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_ap_cand_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  if (ntiles > 0) hipLaunchKernelGGL(k_ap_cand_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
-void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_ap_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+void launch_ap_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_ap_cand_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
 void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
-  if (nctiles <= 0) return;
-  hipLaunchKernelGGL(k_c_cand_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+  if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
-void launch_c_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
-  if (nctiles <= 0) return;
-  hipLaunchKernelGGL(k_c_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+void launch_c_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
+  if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_fill, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
-void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
-  if (nctiles <= 0) return;
-  hipLaunchKernelGGL(k_c_fill, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+void launch_unique_rows(hipStream_t s, const CoarsenDev* jobs, const Tile* utiles, int ntiles, int which) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_unique_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, utiles, ntiles, which);
+}
+void launch_pattern_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int which) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_pattern_cols, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, which);
+}
+void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_ap_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+}
+void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
+  if (ntiles > 0) hipLaunchKernelGGL(k_c_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
 }
 void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;

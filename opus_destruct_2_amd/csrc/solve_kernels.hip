@@ -74,6 +74,23 @@ __device__ __forceinline__ T spmv_row(const int32_t* __restrict__ rowptr, const 
   return acc;
 }
 
+// this sub-group's share of (A x)_q of block row `row`: the row's blocks sub, sub + S, ...
+template <typename T, int S>
+__device__ __forceinline__ T spmv_row_part(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                           const T* __restrict__ A, const T* __restrict__ x, int row, int q, int sub) {
+  T acc = T(0);
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  for (int k = k0 + sub; k < k1; k += S) {
+    const int j = col[k];
+    T a[6], v[6];
+    load6_stream(A + 36 * size_t(k) + 6 * q, a);
+    load6(x + 6 * size_t(j), v);
+    const T s = a[0] * v[0] + a[1] * v[1] + a[2] * v[2] + a[3] * v[3] + a[4] * v[4] + a[5] * v[5];
+    acc += s;
+  }
+  return acc;
+}
+
 // (M v)_q with M a 6x6 row-major block and v the row's values gathered from its lanes
 template <typename T>
 __device__ __forceinline__ T block_row(const T* __restrict__ M, const T v[6], int q) {
@@ -98,7 +115,11 @@ __device__ __forceinline__ const LevelArrays<float>& arrays<float>(const LevelDe
 __device__ __forceinline__ bool live(const GraphDev& g, int inner) {
   const PcgState* s = g.st;
   if (s->state != ST_RUNNING) return false;
-  return !inner || s->inner_state == ST_RUNNING;
+  if (inner == SERVE_OUTER) return true;
+  if (s->inner_state != ST_RUNNING) return false;
+  if (inner == SERVE_INNER32) return !s->inner64;
+  if (inner == SERVE_INNER64) return s->inner64 != 0;
+  return true;
 }
 
 // Chebyshev coefficients of step j of the smoother on [lmax / ratio, lmax]: the direction is
@@ -138,9 +159,8 @@ __device__ __forceinline__ void finish_partials(double v0, double v1, double* pa
 // ---- V-cycle kernels ---------------------------------------------------------------------------
 
 // One Chebyshev step on `level`: r = b - A x_in (b when fresh: x_in = 0), d = c1 d + c2 D^-1 r,
-// x_out = x_in + d. With `dot`, the partial sums of b . x_out are written (the PCG's r . z when this
-// is level 0's last step).
-template <typename T>
+// x_out = x_in + d. S: six-lane sub-groups a row. With `partial`, the partial sums of b . x_out.
+template <typename T, int S>
 __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
                                                    int ntiles, int level, int step, int in_buf, int out_buf, int fresh,
                                                    int inner, double* __restrict__ partial) {
@@ -151,8 +171,8 @@ __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ 
   if (!live(g, inner)) return;
   const LevelDev& L = g.lv[level];
   const LevelArrays<T>& a = arrays<T>(L);
-  int rl, q;
-  lane_row(rl, q);
+  int rl, sub, q;
+  lane_row_s<S>(rl, sub, q);
   const bool on = rl >= 0 && rl < tl.nrows;
   const int row = tl.row0 + (on ? rl : 0);
   const size_t e = 6 * size_t(row) + q;
@@ -163,26 +183,27 @@ __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ 
       r = bq;
     } else {
       xin = a.x[in_buf][e];
-      r = bq - spmv_row<T>(L.rowptr, L.col, a.A, a.x[in_buf], row, q);
+      r = spmv_row_part<T, S>(L.rowptr, L.col, a.A, a.x[in_buf], row, q, sub);
     }
   }
+  if (!fresh) r = bq - row_reduce<S>(r);
   T rv[6];
   row_gather(r, rv);
   double c1, c2;
   cheb_coeffs(L.lmax, step, c1, c2);
   T xo = T(0);
-  if (on) {
+  if (on && sub == 0) {
     const T z = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
     const T d = (step == 0) ? T(c2) * z : T(c1) * a.d[e] + T(c2) * z;
     a.d[e] = d;
     xo = xin + d;
     a.x[out_buf][e] = xo;
   }
-  if (partial) finish_partials(on ? double(bq) * double(xo) : 0.0, 0.0, partial, tl.slot, false);
+  if (partial) finish_partials(on && sub == 0 ? double(bq) * double(xo) : 0.0, 0.0, partial, tl.slot, false);
 }
 
 // r = b - A x
-template <typename T>
+template <typename T, int S>
 __global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
                                                   int ntiles, int level, int buf, int inner) {
   const int t = xcd_tile(ntiles);
@@ -192,16 +213,19 @@ __global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G
   if (!live(g, inner)) return;
   const LevelDev& L = g.lv[level];
   const LevelArrays<T>& a = arrays<T>(L);
-  int rl, q;
-  lane_row(rl, q);
-  if (rl < 0 || rl >= tl.nrows) return;
-  const int row = tl.row0 + rl;
+  int rl, sub, q;
+  lane_row_s<S>(rl, sub, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  T v = on ? spmv_row_part<T, S>(L.rowptr, L.col, a.A, a.x[buf], row, q, sub) : T(0);
+  v = row_reduce<S>(v);
+  if (!on || sub != 0) return;
   const size_t e = 6 * size_t(row) + q;
-  a.r[e] = a.b[e] - spmv_row<T>(L.rowptr, L.col, a.A, a.x[buf], row, q);
+  a.r[e] = a.b[e] - v;
 }
 
 // b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`
-template <typename T>
+template <typename T, int S>
 __global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
                                                   int ntiles, int level, int inner) {
   const int t = xcd_tile(ntiles);
@@ -212,18 +236,21 @@ __global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G
   const LevelDev& F = g.lv[level];
   const LevelArrays<T>& af = arrays<T>(F);
   const LevelArrays<T>& ac = arrays<T>(g.lv[level + 1]);
-  int rl, q;
-  lane_row(rl, q);
-  if (rl < 0 || rl >= tl.nrows) return;
-  const int I = tl.row0 + rl;
+  int rl, sub, q;
+  lane_row_s<S>(rl, sub, q);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  const int I = tl.row0 + (on ? rl : 0);
   T acc = T(0);
-  for (int e = F.Rrow[I]; e < F.Rrow[I + 1]; ++e) {
-    const int i = F.Rfine[e];
-    T m[6], v[6];
-    load6(af.R + 36 * size_t(e) + 6 * q, m);
-    load6(af.r + 6 * size_t(i), v);
-    acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
-  }
+  if (on)
+    for (int e = F.Rrow[I] + sub; e < F.Rrow[I + 1]; e += S) {
+      const int i = F.Rfine[e];
+      T m[6], v[6];
+      load6(af.R + 36 * size_t(e) + 6 * q, m);
+      load6(af.r + 6 * size_t(i), v);
+      acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
+    }
+  acc = row_reduce<S>(acc);
+  if (!on || sub != 0) return;
   ac.b[6 * size_t(I) + q] = acc;
 }
 
@@ -552,7 +579,14 @@ __global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict
   const double rn = sqrt(rr);
   if (inner) {
     s->inner_iters = 0;
-    if (!(rn > s->inner_target)) s->inner_state = ST_CONVERGED;
+    if (!(rn > s->inner_target)) {
+      s->inner_state = ST_CONVERGED;
+      if (s->inner64) {
+        s->rn = rn;
+        s->rel_res = rn / s->bn;
+        s->state = ST_CONVERGED;
+      }
+    }
     return;
   }
   s->rn = rn;
@@ -609,7 +643,14 @@ __global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ 
   if (threadIdx.x != 0) return;
   const double rn = sqrt(rr);
   if (inner) {
-    if (rn <= s->inner_target) s->inner_state = ST_CONVERGED;
+    if (rn <= s->inner_target) {
+      s->inner_state = ST_CONVERGED;
+      if (s->inner64) {
+        s->rn = rn;
+        s->rel_res = rn / s->bn;
+        s->state = ST_CONVERGED;
+      }
+    }
     return;
   }
   s->rn = rn;
@@ -684,7 +725,12 @@ __global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict
     s->state = ST_STOPPED;
     return;
   }
-  s->inner_target = fmax(s->eta * rn, 0.5 * s->target);
+  // a step that missed its goal tenfold: FP32 K's error times K's condition number stalls this graph;
+  // its inner solves take FP64 K products from now on and stop, as the plain FP64 PCG does, on their
+  // recurrence residual at the target
+  if (s->outer > 0 && rn > 10.0 * s->inner_target) s->inner64 = 1;
+  s->outer_rn_prev = rn;
+  s->inner_target = s->inner64 ? s->target : fmax(s->eta * rn, 0.5 * s->target);
   s->inner_state = ST_RUNNING;
   s->inner_iters = 0;
   s->inner_maxit = s->maxit - s->iters;
@@ -698,13 +744,29 @@ __global__ void __launch_bounds__(kWG) k_inner_init(const GraphDev* __restrict__
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, 1)) return;
+  if (!live(g, SERVE_INNER)) return;
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
   const size_t e = 6 * size_t(tl.row0 + rl) + q;
   g.ir[e] = float(g.r[e]);
   g.ix[e] = 0.0f;
+}
+
+// the FP64-product inner solves' preconditioner input and output: r_i = (float) r, z = (double) z_i
+__global__ void __launch_bounds__(kWG) k_convert64(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                   int ntiles, int to32) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (!live(g, SERVE_INNER64)) return;
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  if (to32) g.ir[e] = float(g.r[e]);
+  else g.z[e] = double(g.iz[e]);
 }
 
 // x += (double) x_i after an inner solve; the inner iterations count towards the solve's
@@ -714,7 +776,8 @@ __global__ void __launch_bounds__(kWG) k_inner_apply(const GraphDev* __restrict_
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (g.st->state != ST_RUNNING || g.st->inner_state == ST_IDLE) return;
+  // (an FP64-product inner solve updated x itself)
+  if (g.st->state != ST_RUNNING || g.st->inner_state == ST_IDLE || g.st->inner64) return;
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
@@ -726,7 +789,7 @@ __global__ void k_inner_close(const GraphDev* __restrict__ G, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   PcgState* s = G[k].st;
-  if (s->state != ST_RUNNING || s->inner_state == ST_IDLE) return;
+  if (s->inner_state == ST_IDLE) return;
   s->iters += s->inner_iters;
   if (s->inner_state == ST_BREAKDOWN && s->inner_iters == 0) s->state = ST_BREAKDOWN;
   s->inner_state = ST_IDLE;
@@ -741,20 +804,32 @@ int grid_for(int ntiles) {
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
-                      int out_buf, int fresh, int inner, double* partial) {
+                      int out_buf, int fresh, int inner, double* partial, int S) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_cheb_step<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, step, in_buf,
-                     out_buf, fresh, inner, partial);
+  if (S == 10)
+    hipLaunchKernelGGL((k_cheb_step<T, 10>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, step,
+                       in_buf, out_buf, fresh, inner, partial);
+  else
+    hipLaunchKernelGGL((k_cheb_step<T, 1>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, step,
+                       in_buf, out_buf, fresh, inner, partial);
 }
 template <typename T>
-void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner) {
+void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
+                     int S) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_residual<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf, inner);
+  if (S == 10)
+    hipLaunchKernelGGL((k_residual<T, 10>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf,
+                       inner);
+  else
+    hipLaunchKernelGGL((k_residual<T, 1>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf, inner);
 }
 template <typename T>
-void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int inner) {
+void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int inner, int S) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, inner);
+  if (S == 10)
+    hipLaunchKernelGGL((k_restrict<T, 10>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, inner);
+  else
+    hipLaunchKernelGGL((k_restrict<T, 1>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, inner);
 }
 template <typename T>
 void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner) {
@@ -853,6 +928,10 @@ void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_inner_apply, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
 }
+void launch_convert64(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int to32) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_convert64, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, to32);
+}
 void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_inner_close, dim3((count + 255) / 256), dim3(256), 0, s, G, count);
@@ -860,9 +939,9 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
 
 #define SVXG_INST(T)                                                                                                \
   template void launch_cheb_step<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int, int, int,   \
-                                    double*);                                                                       \
-  template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
-  template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int);                        \
+                                    double*, int);                                                                  \
+  template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int);              \
+  template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
   template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
   template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const int32_t*, int, int, int);                  \
   template void launch_jacobi<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                               \

@@ -44,10 +44,17 @@ struct PcgState {
   double rz, pq, alpha, beta, rn, bn, target, rtol;
   double rel_res;
   // mixed precision: outer residual norm and the inner solve's stopping norm
-  double outer_rn, inner_target, eta;
+  double outer_rn, inner_target, eta, outer_rn_prev;
   int32_t iters, maxit, state, outer;
   int32_t inner_state, inner_iters, inner_maxit, zero_load;
+  // mixed precision: 1 once a refinement step stalled (the FP32 K's error times K's condition
+  // number near 1): this graph's inner solves then take FP64 K products, FP32 V-cycle
+  int32_t inner64, pad_;
 };
+
+// who a launch serves: 0 the FP64 PCG; 1 inner solves with FP32 K products; 2 inner solves with
+// FP64 K products; 3 every inner solve (the FP32 V-cycle, the per-graph finishes)
+enum : int32_t { SERVE_OUTER = 0, SERVE_INNER32 = 1, SERVE_INNER64 = 2, SERVE_INNER = 3 };
 
 template <typename T>
 struct LevelArrays {
@@ -108,6 +115,38 @@ __device__ __forceinline__ void lane_row(int& rl, int& q) {
   }
   rl = w * kRowsPerWave + l / 6;
   q = l % 6;
+}
+
+// Rows served by S six-lane sub-groups each (S in {1, 2, 5, 10}): a wavefront serves 10 / S rows;
+// sub-group s of a row takes the row's blocks s, s + S, s + 2S, ... (long coarse rows spread their
+// blocks over a wavefront instead of walking them with 6 lanes)
+template <int S>
+__device__ __forceinline__ void lane_row_s(int& rl, int& sub, int& q) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l >= 60) {
+    rl = -1;
+    sub = 0;
+    q = 0;
+    return;
+  }
+  rl = w * (kRowsPerWave / S) + l / (6 * S);
+  sub = (l / 6) % S;
+  q = l % 6;
+}
+template <int S>
+constexpr int tile_rows() {
+  return 4 * (kRowsPerWave / S);
+}
+
+// the sum over a row's S sub-groups of each DOF's partial, s ascending (every lane of the row gets it)
+template <int S, typename T>
+__device__ __forceinline__ T row_reduce(T v) {
+  if (S == 1) return v;
+  const int l = threadIdx.x & 63;
+  const int base = (l / (6 * S)) * (6 * S) + (l % 6);
+  T acc = __shfl(v, base, 64);
+  for (int s = 1; s < S; ++s) acc += __shfl(v, base + 6 * s, 64);
+  return acc;
 }
 
 // sum over a wavefront in a fixed butterfly order (every lane gets the sum)

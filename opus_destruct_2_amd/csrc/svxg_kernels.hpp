@@ -13,13 +13,15 @@ enum class FinishKind { Start, Alpha, Rn, Beta, Outer };
 int grid_for(int ntiles);
 
 // ---- solve ---------------------------------------------------------------------------------------
+// S: six-lane sub-groups a row (1: 10 rows a wavefront; 10: one row a wavefront, for long rows)
 template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
-                      int out_buf, int fresh, int inner, double* partial);
+                      int out_buf, int fresh, int inner, double* partial, int S);
 template <typename T>
-void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner);
+void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
+                     int S);
 template <typename T>
-void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int inner);
+void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int inner, int S);
 template <typename T>
 void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner);
 template <typename T>
@@ -48,6 +50,8 @@ void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, 
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_inner_close(hipStream_t s, const GraphDev* G, int count);
+// FP64-product inner solves: r_i = (float) r (to32 = 1) or z = (double) z_i (to32 = 0)
+void launch_convert64(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int to32);
 // plain y = A x on one graph's level (FP64 or FP32 copy), tests and the roofline probe
 template <typename T>
 void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col, const T* A, const T* x, T* y, int n);
@@ -123,7 +127,8 @@ struct CoarsenDev {
   int32_t* scan_a;     // n + 1 (anchors)
   int32_t* scan_b;     // n + 1 (lonely)
   int32_t* cursor;     // n + 1 (fill cursors)
-  int32_t* counts;     // 8 ints: [0] undecided, [1] anchors, [2] lonely, [3] na, [4] np, [5] nap, [6] nac
+  int32_t* counts;     // 8 ints: [0] undecided (cumulative), [1] anchors, [2] lonely, [3] rows over the
+                       // distinct-column limit, [4] np, [5] nap, [6] nac, [7] candidates
   // results
   int32_t* agg;        // n
   int32_t na;
@@ -151,6 +156,7 @@ struct CoarsenDev {
   int32_t* Ccol;
   double* C;
   int64_t nac;
+  int32_t* rid;        // the row of each P / A P / C entry (scratch)
   double coarse_scale; // applied to C (1: none)
 };
 
@@ -164,19 +170,24 @@ void launch_scan_pair(hipStream_t s, const CoarsenDev* jobs, int count);
 void launch_assign(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
 void launch_coarse_positions(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles,
                              const Tile* ctiles, int nctiles, int count);
-// prolongator
+// prolongator: candidates (sorted, distinct) per row, their count in Prow; columns and each entry's
+// row; the blocks one entry a thread (etiles: tiles over entries)
 void launch_p_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
 void launch_scan_rows(hipStream_t s, const CoarsenDev* jobs, int count, int which);
-void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
-void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
-void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+void launch_p_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles);
 void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
                     int nctiles, int count);
-// Galerkin product
-void launch_ap_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
-void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
-void launch_c_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
-void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+// Galerkin product: candidate counts and candidates per row, distinct sorted columns (one
+// workgroup a row: utiles), the pattern, the blocks one entry a thread. which: 0 A P, 1 P^T (A P)
+void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_ap_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+void launch_c_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
+void launch_unique_rows(hipStream_t s, const CoarsenDev* jobs, const Tile* utiles, int ntiles, int which);
+void launch_pattern_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int which);
+void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles);
+void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles);
 
 // power iteration for lambda_max(D^-1 A) (amg.cpp:256-275): 15 steps, per-graph norms
 struct PowerDev {
