@@ -111,6 +111,61 @@ def cpu_baseline(g, mats):
             "u": u, "phi": phi, "setup_s": setup_s, "solve_s": solve_s, "iters": r["iters"]}
 
 
+def run_batch(mats, ctx, count, steps, sample_plain):
+    """Batched uneven graphs: `count` pieces (log-normal sizes, median 200, 16..20000 nodes; even ones
+    on supports, odd ones pinned), one batch, mixed precision to 1e-8; against the plain FP64 path
+    solving graphs one at a time (a sample)."""
+    from opus_destruct_2_amd import gen, stress
+    from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
+    sizes = gen.piece_sizes(count, seed=3)
+    gs = [gen.piece(int(n), pinned=bool(k & 1), seed=5000 + k) for k, n in enumerate(sizes)]
+    dofs = sum(g.dofs for g in gs)
+    log(f"batched: {count} graphs, {sum(g.n for g in gs)} nodes, {dofs} DOFs (sizes {sizes.min()}..{sizes.max()}, "
+        f"median {int(np.median(sizes))})")
+    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0)
+    probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
+    t = time.perf_counter()
+    ok = stress.assemble_batch(probs)
+    setup_ms = (time.perf_counter() - t) * 1e3
+    assert all(ok)
+    fs = [g.f for g in gs]
+    us, rs = stress.solve_batch(probs, fs, rtol=1e-8)
+    dev, wall = [], []
+    for s_ in range(steps):
+        t = time.perf_counter()
+        us, rs = stress.solve_batch(probs, fs, rtol=1e-8)
+        wall.append((time.perf_counter() - t) * 1e3)
+        dev.append(probs[0].last_timing()["solve_ms"])
+        assert all(r["converged"] for r in rs)
+        log(f"  batch step {s_}: {dev[-1]:.2f} ms device ({wall[-1]:.2f} wall), iters max "
+            f"{max(r['iters'] for r in rs)} median {int(np.median([r['iters'] for r in rs]))}")
+    solve_ms = float(np.median(dev))
+    # the plain path: FP64, one graph at a time
+    pick = list(range(0, count, max(1, count // sample_plain)))[:sample_plain]
+    pdofs, pms = 0, 0.0
+    popt = stress.default_options(precision=SVXG_FP64, amg_min_nodes=0)
+    for k in pick:
+        g = gs[k]
+        p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=popt)
+        assert p.assemble()
+        p.solve(g.f, rtol=1e-8)
+        u, r = p.solve(g.f, rtol=1e-8)
+        assert r["converged"]
+        pdofs += g.dofs
+        pms += p.last_timing()["solve_ms"]
+        p.close()
+    plain = pdofs / (pms / 1e3)
+    batched = dofs / (solve_ms / 1e3)
+    for p in probs:
+        p.close()
+    return {"graphs": count, "dofs": dofs, "value": round(batched, 1), "unit": "DOF/s", "solve_ms": round(solve_ms, 3),
+            "wall_ms_pcie_inclusive": round(float(np.median(wall)), 3), "setup_ms": round(setup_ms, 1),
+            "precision": "mixed", "iters_max": max(r["iters"] for r in rs),
+            "plain_fp64_single_graph": {"value": round(plain, 1), "sample_graphs": len(pick), "sample_dofs": pdofs,
+                                        "solve_ms_sum": round(pms, 3)},
+            "batched_vs_plain": round(batched / plain, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +176,8 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=48)
+    ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0: skip)")
+    ap.add_argument("--plain-sample", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the stress solve here is measured on one MI355X's share; "
@@ -160,6 +217,8 @@ def main():
     scale = np.abs(ref_u).max()
     line["mixed_vs_fp64"] = {"u_rel_err": float(np.abs(mixed_res["u"] - ref_u).max() / scale),
                              "phi_max_abs_err": float(np.abs(mixed_res["phi"] - plain["phi"]).max())}
+    if args.batch > 0:
+        line["batched"] = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
     if not args.no_cpu:
         log("cpu baseline: the reference's solver on the same workload ...")
         cb = cpu_baseline(g, mats)

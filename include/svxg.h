@@ -146,6 +146,9 @@ int svxg_problem_destroy(svxg_problem* p);
    device, then block-Jacobi (nfree < amg_min_nodes) or the multigrid hierarchy, built on the
    device. Returns SVXG_SINGULAR where the reference's assemble returns false. */
 int svxg_problem_assemble(svxg_problem* p, const svxg_stress_options* opt);
+/* The options of the next assemble (StressProblem::opt_, stress.hpp:153), without assembling:
+   svxg_assemble_batch / svxg_solve_batch assemble unassembled problems with them. */
+int svxg_problem_set_options(svxg_problem* p, const svxg_stress_options* opt);
 /* StressProblem::release (stress.hpp:99; stress.cpp:326-339). */
 int svxg_problem_release(svxg_problem* p);
 /* StressProblem::invalidate (stress.hpp:85-88). */
@@ -208,6 +211,10 @@ int svxg_problem_get_nodes(const svxg_problem* p, svxg_node* out);
    every reduction runs in a fixed order over the problem's own rows). */
 int svxg_solve_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_t count, const double* const* f,
                      double* const* u, double rtol, int32_t maxit, int32_t warm, svxg_pcg_result* results);
+
+/* StressProblem::assemble of many problems in one set of device launches (their own options).
+   status[i]: SVXG_OK, or SVXG_SINGULAR where the reference's assemble returns false. */
+int svxg_assemble_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_t count, int32_t* status);
 
 /* ---- device-resident views (tests and benchmarks) ----------------------------------------- */
 

@@ -129,6 +129,96 @@ static int upload_into(const DevBuf& d, const std::vector<T>& h, hipStream_t s) 
   return SVXG_OK;
 }
 
+// Uploads gathered into the context's pinned staging buffer and issued as asynchronous copies,
+// with one synchronisation for the lot (a batch of problems makes many small arrays)
+class Stager {
+ public:
+  Stager(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
+  template <typename T>
+  int add(DevBuf& d, Arena* a, const std::vector<T>& h) {
+    SVXG_TRY(d.make(a, std::max<size_t>(1, h.size()) * sizeof(T)));
+    if (!h.empty()) jobs_.push_back({d.as<void>(), h.data(), h.size() * sizeof(T)});
+    return SVXG_OK;
+  }
+  int add_raw(void* dst, const void* src, size_t bytes) {
+    if (bytes) jobs_.push_back({dst, src, bytes});
+    return SVXG_OK;
+  }
+  int flush() {
+    size_t total = 0;
+    for (const Job& j : jobs_) total += (j.bytes + 255) & ~size_t(255);
+    if (total > c_->pinned_bytes) {
+      if (c_->pinned) (void)hipHostFree(c_->pinned);
+      c_->pinned = nullptr;
+      const size_t want = std::max(total, size_t(64) << 20);
+      SVXG_HIP(hipHostMalloc(&c_->pinned, want, hipHostMallocDefault));
+      c_->pinned_bytes = want;
+    }
+    char* p = static_cast<char*>(c_->pinned);
+    for (const Job& j : jobs_) {
+      std::memcpy(p, j.src, j.bytes);
+      SVXG_HIP(hipMemcpyAsync(j.dst, p, j.bytes, hipMemcpyHostToDevice, s_));
+      p += (j.bytes + 255) & ~size_t(255);
+    }
+    SVXG_HIP(hipStreamSynchronize(s_));
+    jobs_.clear();
+    return SVXG_OK;
+  }
+
+ private:
+  struct Job {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  svxg_ctx* c_;
+  hipStream_t s_;
+  std::vector<Job> jobs_;
+};
+
+// readbacks into the pinned staging buffer, one synchronisation, then to their destinations
+class Gatherer {
+ public:
+  Gatherer(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
+  void add(void* dst, const void* dev, size_t bytes) {
+    if (bytes) jobs_.push_back({dst, dev, bytes});
+  }
+  int flush() {
+    size_t total = 0;
+    for (const Job& j : jobs_) total += (j.bytes + 255) & ~size_t(255);
+    if (total > c_->pinned_bytes) {
+      if (c_->pinned) (void)hipHostFree(c_->pinned);
+      c_->pinned = nullptr;
+      const size_t want = std::max(total, size_t(64) << 20);
+      SVXG_HIP(hipHostMalloc(&c_->pinned, want, hipHostMallocDefault));
+      c_->pinned_bytes = want;
+    }
+    char* p = static_cast<char*>(c_->pinned);
+    for (const Job& j : jobs_) {
+      SVXG_HIP(hipMemcpyAsync(p, j.dev, j.bytes, hipMemcpyDeviceToHost, s_));
+      p += (j.bytes + 255) & ~size_t(255);
+    }
+    SVXG_HIP(hipStreamSynchronize(s_));
+    p = static_cast<char*>(c_->pinned);
+    for (const Job& j : jobs_) {
+      std::memcpy(j.dst, p, j.bytes);
+      p += (j.bytes + 255) & ~size_t(255);
+    }
+    jobs_.clear();
+    return SVXG_OK;
+  }
+
+ private:
+  struct Job {
+    void* dst;
+    const void* dev;
+    size_t bytes;
+  };
+  svxg_ctx* c_;
+  hipStream_t s_;
+  std::vector<Job> jobs_;
+};
+
 template <typename T>
 static int download(std::vector<T>& h, const DevBuf& d, size_t count, hipStream_t s) {
   h.resize(count);

@@ -125,6 +125,9 @@ struct svxg_ctx {
   std::mutex mu;  // batches and setup on the context stream, one at a time
   // kernel timing: event pairs around the launches of the profiled kernels (kinds 0..3: the outer
   // K product FP64 / FP32, a level-0 smoothing step FP64 / FP32)
+  // pinned host staging for batched uploads and readbacks (grown on demand)
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
   bool profiling = false;
   struct Prof {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;

@@ -987,7 +987,12 @@ __global__ void k_c_fill(const CoarsenDev* __restrict__ J, const Tile* __restric
 // ---- power iteration (6 lanes a row, kTileRows-row tiles) ------------------------------------------
 
 __global__ void __launch_bounds__(kWG) k_power_init(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
-                                                    int ntiles) {
+                                                    int ntiles, int count) {
+  if (blockIdx.x == 0)
+    for (int j = threadIdx.x; j < count; j += kWG) {
+      J[j].lam[0] = 1.0;
+      J[j].lam[1] = 1.0;
+    }
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const PowerDev& p = J[tl.g];
@@ -1339,9 +1344,9 @@ void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, i
 void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_c_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
 }
-void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles) {
+void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int count) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_power_init, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_power_init, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, count);
 }
 void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial) {
   if (ntiles <= 0) return;

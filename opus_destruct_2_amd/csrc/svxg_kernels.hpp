@@ -200,7 +200,7 @@ struct PowerDev {
   double* v[2];
   double* lam;   // 2 doubles: [0] the estimate, [1] the last norm
 };
-void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles);
+void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int count);
 void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial);
 void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
 

@@ -60,6 +60,8 @@ def lib():
             "svxg_problem_apply_matrix": ([vp, vp, vp, i32], ctypes.c_int),
             "svxg_problem_apply_preconditioner": ([vp, vp, vp], ctypes.c_int),
             "svxg_problem_last_timing": ([vp, ctypes.POINTER(f64), ctypes.POINTER(f64)], ctypes.c_int),
+            "svxg_problem_set_options": ([vp, vp], ctypes.c_int),
+            "svxg_assemble_batch": ([vp, vp, i32, vp], ctypes.c_int),
             "svxg_ctx_set_profiling": ([vp, i32], ctypes.c_int),
             "svxg_ctx_kernel_stats": ([vp, i32, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)],
                                       ctypes.c_int),
@@ -136,7 +138,7 @@ def default_context():
 class StressProblem:
     """A fragment graph's stress problem on the device (StressProblem, stress.hpp:74-160)."""
 
-    def __init__(self, nodes, bonds, mats, ctx=None):
+    def __init__(self, nodes, bonds, mats, ctx=None, options=None):
         self.ctx = ctx or default_context()
         self._nodes = np.ascontiguousarray(nodes, NODE)
         self._bonds = np.ascontiguousarray(bonds, BOND)
@@ -147,6 +149,13 @@ class StressProblem:
                "svxg_problem_create")
         self.h = h
         self.options = default_options()
+        if options is not None:
+            self.set_options(options)
+
+    def set_options(self, options):
+        """The options of the next assemble (batched assembly uses them)."""
+        self.options = options
+        _check(lib().svxg_problem_set_options(self.h, ctypes.byref(options)), "set_options")
 
     def close(self):
         if getattr(self, "h", None):
@@ -289,6 +298,16 @@ def add_force(f, node, c, F, at):
     lib().svxg_add_force(f.ctypes.data, int(node), c.ctypes.data, F.ctypes.data, at.ctypes.data)
 
 
+def assemble_batch(problems, ctx=None):
+    """Assemble many problems in one set of device launches; returns each one's success (bool)."""
+    ctx = ctx or problems[0].ctx
+    n = len(problems)
+    ph = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    st = (ctypes.c_int32 * n)()
+    _check(lib().svxg_assemble_batch(ctx.h, ph, n, st), "svxg_assemble_batch")
+    return [s == SVXG_OK for s in st]
+
+
 def solve_batch(problems, fs, us=None, rtol=1e-8, maxit=-1, warm=False, ctx=None):
     """Solve many problems in one set of device launches; returns (list of u, list of results)."""
     ctx = ctx or problems[0].ctx
@@ -307,4 +326,5 @@ def solve_batch(problems, fs, us=None, rtol=1e-8, maxit=-1, warm=False, ctx=None
 
 
 __all__ = ["StressProblem", "Context", "default_context", "default_options", "reference_options", "solve_batch",
+           "assemble_batch",
            "add_force", "lib", "SvxgError", "SVXG_FP64", "SVXG_MIXED"]

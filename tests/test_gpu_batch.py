@@ -1,0 +1,71 @@
+"""Batches of uneven graphs (the reference solves its pieces one by one, world_pieces.cpp:1452-1463):
+each graph's answer in a batch is bitwise its answer alone, and matches the oracle."""
+import numpy as np
+import pytest
+
+from oracle import CpuProblem, reference_options
+from opus_destruct_2_amd import gen, stress
+from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [16, 31, 48, 60, 100, 159, 161, 250, 400, 900, 2000, 3500]
+
+
+def graphs():
+    return [gen.piece(s, pinned=bool(k & 1), seed=100 + k) for k, s in enumerate(SIZES)]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("min_nodes", [0, 160], ids=["all_multigrid", "small_jacobi"])
+@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
+def test_batch_equals_alone_bitwise(precision, min_nodes, mats, ctx):
+    gs = graphs()
+    opts = stress.default_options(precision=precision, amg_min_nodes=min_nodes)
+    alone = []
+    for g in gs:
+        p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
+        assert p.assemble()
+        u, r = p.solve(g.f, rtol=1e-9, maxit=5000)
+        assert r["converged"], r
+        alone.append((u, r, p.info()))
+        p.close()
+    probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
+    us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-9, maxit=5000)
+    for k, g in enumerate(gs):
+        assert rs[k] == alone[k][1], (SIZES[k], rs[k], alone[k][1])
+        assert probs[k].info()["level_rows"] == alone[k][2]["level_rows"]
+        assert np.array_equal(bits(us[k]), bits(alone[k][0])), SIZES[k]
+
+
+@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
+def test_batch_matches_oracle(precision, mats, ctx):
+    gs = graphs()
+    tol = 1e-6 if precision == SVXG_FP64 else 1e-5
+    opts = stress.default_options(precision=precision, amg_min_nodes=0)
+    probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
+    assert all(stress.assemble_batch(probs))
+    us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-10, maxit=5000)
+    for k, g in enumerate(gs):
+        o = CpuProblem(g.nodes, g.bonds, mats)
+        assert o.assemble(reference_options(amg_min_nodes=0))
+        uo, ro = o.solve(g.f, rtol=1e-10, maxit=20000)
+        assert rs[k]["converged"] and ro["converged"]
+        scale = np.abs(uo).max()
+        assert np.abs(us[k] - uo).max() <= tol * scale, (SIZES[k], np.abs(us[k] - uo).max() / scale)
+        phi_o, _ = o.bond_checks(uo)
+        phi_d, _ = probs[k].bond_checks(None)
+        assert np.all(np.abs(phi_d - phi_o) <= tol * np.maximum(1.0, np.abs(phi_o)))
+
+
+def test_batch_warm_start_and_resident_loads(mats, ctx):
+    """A warm start at the answer converges at once; a null load reuses the resident one."""
+    gs = graphs()[:6]
+    probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx) for g in gs]
+    us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-9)
+    us2, rs2 = stress.solve_batch(probs, [g.f for g in gs], us=us, rtol=1e-9, warm=True)
+    for k in range(len(gs)):
+        assert rs2[k]["converged"] and rs2[k]["iters"] <= 1
