@@ -2,7 +2,7 @@
 
   libsvxg.so    the device library: HIP kernels + host driver + C ABI (include/svxg.h)
   libsvxgen.so  the deterministic synthetic-workload generator (CPU)
-  tests_cpp/    the C++ restatement of the reference's stress tests over the C ABI (optional)
+  tests/cpp/test_stress_facade  the reference's stress tests restated over include/svxg_stress.hpp
 
 Kernels are compiled with -ffp-contract=off (assembly, P and the bond checks reproduce the
 reference's operation order) and without packed FP32 instructions.
@@ -62,9 +62,20 @@ def build_generator():
     return lib
 
 
+def build_cpp_tests():
+    """The C++ restatement of the reference's stress tests over include/svxg_stress.hpp."""
+    out = os.path.join(ROOT, "tests", "cpp", "test_stress_facade")
+    _run(["g++", "-std=c++20", "-O2", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp",
+          os.path.join(ROOT, "tests", "cpp", "test_stress_facade.cpp"), "-L", HERE, "-lsvxg",
+          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../opus_destruct_2_amd"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_generator()
     build_device()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

@@ -282,7 +282,9 @@ __global__ void k_diag_trace(const AssemblyDev* __restrict__ J, const Tile* __re
 }
 
 // id_scale = mean of tr(K_ii[0:3,0:3]) / 3 over rows with a positive trace, summed in row order
-// (one wavefront a graph: 64 traces loaded at a time, added by lane 0 in order)
+// (stress.cpp:111-122). One wavefront a graph: 64 traces are loaded and divided by 3 at a time, then
+// added in row order from scalar reads of the lanes whose trace is positive (the sum stays the
+// reference's sequential one; only the adds and the lane reads are serial).
 __global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
   const int g = blockIdx.x;
   if (g >= count) return;
@@ -293,13 +295,17 @@ __global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
   for (int base = 0; base < a.nfree; base += 64) {
     const int i = base + l;
     const double t = i < a.nfree ? a.trace[i] : 0.0;
-    const int m = min(64, a.nfree - base);
-    for (int k = 0; k < m; ++k) {
-      const double v = __shfl(t, k, 64);
-      if (v > 0) {
-        tr += v / 3.0;
-        ++cnt;
-      }
+    const double t3 = t / 3.0;
+    uint64_t pos = __ballot(t > 0);
+    cnt += __popcll(pos);
+    const int2 w = *reinterpret_cast<const int2*>(&t3);
+    while (pos) {
+      const int k = __ffsll(static_cast<unsigned long long>(pos)) - 1;
+      pos &= pos - 1;
+      int2 v;
+      v.x = __builtin_amdgcn_readlane(w.x, k);
+      v.y = __builtin_amdgcn_readlane(w.y, k);
+      tr += *reinterpret_cast<const double*>(&v);
     }
   }
   if (l == 0) *a.id_scale = cnt ? tr / cnt : 1.0;
@@ -1003,34 +1009,37 @@ __global__ void __launch_bounds__(kWG) k_power_init(const PowerDev* __restrict__
   p.v[0][e] = 1.0 + 0.001 * double(e % 7);
 }
 
-// v_{s+1} = D^-1 A (v_s / |v_s|), partial |v_{s+1}|^2
+// v_{s+1} = D^-1 A (v_s / |v_s|), partial |v_{s+1}|^2 (S six-lane sub-groups a row: S = 10 serves
+// the long rows of coarse levels a wavefront each)
+template <int S>
 __global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
                                                     int ntiles, int step, double* __restrict__ partial) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const PowerDev& p = J[tl.g];
   if (p.lam[1] <= 0.0) return;  // (stopped: a zero vector)
-  const double nv = step == 0 ? 1.0 : p.lam[1];
+  const double inv = step == 0 ? 1.0 : 1.0 / p.lam[1];
   const double* in = p.v[step & 1];
   double* out = p.v[(step + 1) & 1];
-  int rl, q;
-  lane_row(rl, q);
+  int rl, sub, q;
+  lane_row_s<S>(rl, sub, q);
   const bool on = rl >= 0 && rl < tl.nrows;
   const int row = tl.row0 + (on ? rl : 0);
   double w = 0.0;
   if (on) {
-    for (int k = p.rowptr[row]; k < p.rowptr[row + 1]; ++k) {
+    for (int k = p.rowptr[row] + sub; k < p.rowptr[row + 1]; k += S) {
       const double* a = p.A + 36 * size_t(k) + 6 * q;
       const double* x = in + 6 * size_t(p.col[k]);
       double s = 0.0;
-      for (int c = 0; c < 6; ++c) s += a[c] * (x[c] / nv);
+      for (int c = 0; c < 6; ++c) s += a[c] * x[c];
       w += s;
     }
   }
+  w = row_reduce<S>(w) * inv;
   double wv[6];
   row_gather(w, wv);
   double v = 0.0;
-  if (on) {
+  if (on && sub == 0) {
     const double* D = p.Dinv + 36 * size_t(row) + 6 * q;
     v = D[0] * wv[0] + D[1] * wv[1] + D[2] * wv[2] + D[3] * wv[3] + D[4] * wv[4] + D[5] * wv[5];
     out[6 * size_t(row) + q] = v;
@@ -1348,9 +1357,13 @@ void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, i
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_power_init, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, count);
 }
-void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial) {
+void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial,
+                       bool long_rows) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_power_step, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
+  if (long_rows)
+    hipLaunchKernelGGL(k_power_step<10>, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
+  else
+    hipLaunchKernelGGL(k_power_step<1>, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
 }
 void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial) {
   if (count <= 0) return;

@@ -201,7 +201,8 @@ struct PowerDev {
   double* lam;   // 2 doubles: [0] the estimate, [1] the last norm
 };
 void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int count);
-void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial);
+void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial,
+                       bool long_rows);
 void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
 
 // the coarsest level's dense inverse (amg.cpp:535-578, then inverted)
