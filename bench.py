@@ -73,8 +73,8 @@ def run_path(g, mats, precision, steps, warmup, ctx):
 
 
 def roofline(res, precision):
-    names = {0: "k_pcg_spmv<double>", 1: "k_pcg_spmv<float>", 2: "k_cheb_step<double> (level 0)",
-             3: "k_cheb_step<float> (level 0)"}
+    names = {0: "k_pcg_spmv<double>", 1: "k_pcg_spmv<float>", 2: "k_cheb_step<double, 1, false> (level 0)",
+             3: "k_cheb_step<float, 1, false> (level 0)"}
     kinds = (1, 3) if precision == "mixed" else (0, 2)
     best = max(kinds, key=lambda k: res["kstats"][k][1])
     n, ms, byts = res["kstats"][best]
@@ -178,11 +178,23 @@ def main():
     ap.add_argument("--nz", type=int, default=48)
     ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=128)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the stress solve here is measured on one MI355X's share; "
               "--gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        import torch
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n}))
+        return
 
     from opus_destruct_2_amd import gen, stress
     from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED

@@ -268,6 +268,7 @@ struct Batch {
   };
   std::vector<Lv> lvl;
   DevBuf partial;
+  DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
   bool any_amg = false;
 };
 
@@ -329,6 +330,8 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
 
 // rows averaging more than 16 entries are served a wavefront each (S = 10 sub-groups)
 static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entries > 16 * rows; }
+
+static int prof_table(Batch& B, Arena* A, hipStream_t s);
 
 static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A, hipStream_t s) {
   B.probs.assign(probs, probs + count);
@@ -400,28 +403,41 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
 
 // ---- kernel timing ----------------------------------------------------------------------------------
 
-// algorithmic bytes of one launch over the batch: the outer K product (K's blocks and column
-// indices, row pointers, p gathered once, q written) and a level-0 Chebyshev step (the same matrix
-// traffic, plus b, D^-1, d read and written, x written)
-static double prof_bytes(const Batch& B, int kind) {
+// algorithmic bytes of one graph in one launch: the outer K product (K's blocks and column indices,
+// row pointers, p gathered once, q written) and a level-0 Chebyshev step (the same matrix traffic,
+// plus b, D^-1, d read and written, x written); kinds 0/2 FP64, 1/3 FP32
+static double prof_bytes(const svxg_problem* P, int kind) {
+  if (P->lv.empty()) return 0.0;
   const double w = (kind & 1) ? 4.0 : 8.0;
-  double s = 0.0;
-  for (const svxg_problem* P : B.probs) {
-    if (P->lv.empty()) continue;
-    const double nb = double(P->lv[0].nnzb), n = double(P->lv[0].n);
-    const double mat = nb * (36.0 * w + 4.0) + 4.0 * (n + 1.0);
-    s += kind < 2 ? mat + 2.0 * 6.0 * n * w : mat + 36.0 * n * w + 5.0 * 6.0 * n * w;
-  }
-  return s;
+  const double nb = double(P->lv[0].nnzb), n = double(P->lv[0].n);
+  const double mat = nb * (36.0 * w + 4.0) + 4.0 * (n + 1.0);
+  return kind < 2 ? mat + 2.0 * 6.0 * n * w : mat + 36.0 * n * w + 5.0 * 6.0 * n * w;
 }
 
+// the batch's per-graph byte table (made when the batch is built with timing on)
+static int prof_table(Batch& B, Arena* A, hipStream_t s) {
+  std::vector<double> gb(4 * size_t(B.G));
+  for (int k = 0; k < 4; ++k)
+    for (int g = 0; g < B.G; ++g) gb[size_t(k) * size_t(B.G) + size_t(g)] = prof_bytes(B.probs[size_t(g)], k);
+  svxg_ctx* c = B.probs[0]->ctx;
+  if (!c->prof_acc.as<void>()) {
+    SVXG_TRY(c->prof_acc.make(A, 4 * sizeof(double)));
+    SVXG_HIP(hipMemsetAsync(c->prof_acc.as<void>(), 0, 4 * sizeof(double), s));
+  }
+  return upload(B.prof_gb, A, gb, s);
+}
+
+// The launch between construction and destruction is timed with an event pair; right after it a
+// one-workgroup kernel adds the bytes of the graphs that launch served (live for `inner`) to the
+// context's device counter, so launches issued after a graph stopped add time and no bytes.
 struct ProfScope {
   svxg_ctx* c;
-  int kind;
+  const Batch& B;
+  int kind, inner;
   hipStream_t s;
   hipEvent_t e1 = nullptr;
-  ProfScope(const Batch& B, int k, hipStream_t st) : c(B.probs[0]->ctx), kind(k), s(st) {
-    if (!c->profiling || kind < 0) return;
+  ProfScope(const Batch& b, int k, hipStream_t st, int in) : c(b.probs[0]->ctx), B(b), kind(k), inner(in), s(st) {
+    if (!c->profiling || kind < 0 || !c->prof_acc.as<void>() || !B.prof_gb.as<void>()) return;
     auto& P = c->prof[kind];
     std::pair<hipEvent_t, hipEvent_t> ev;
     if (!P.spare.empty()) {
@@ -433,11 +449,13 @@ struct ProfScope {
     }
     (void)hipEventRecord(ev.first, s);
     P.pending.push_back(ev);
-    P.pending_bytes += prof_bytes(B, kind);
     e1 = ev.second;
   }
   ~ProfScope() {
-    if (e1) (void)hipEventRecord(e1, s);
+    if (!e1) return;
+    (void)hipEventRecord(e1, s);
+    launch_prof_account(s, B.dG.as<GraphDev>(), B.G, inner, B.prof_gb.as<double>() + size_t(kind) * size_t(B.G),
+                        c->prof_acc.as<double>() + kind);
   }
 };
 
@@ -453,8 +471,6 @@ static void prof_collect(svxg_ctx* c) {
       }
       P.spare.push_back(ev);
     }
-    P.bytes += P.pending_bytes;
-    P.pending_bytes = 0.0;
     P.pending.clear();
   }
 }
@@ -475,7 +491,7 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
     for (int v = 0; v < 2; ++v) {
       if (!V.nts[v]) continue;
       for (int st = 0; st < degree; ++st) {
-        ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
+        ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
         launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr,
                             SV[v]);
       }
@@ -498,7 +514,7 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
       const int gs = degree + j;
       for (int v = 0; v < 2; ++v) {
         if (!V.nts[v]) continue;
-        ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s);
+        ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
         launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, SV[v]);
       }
     }
@@ -519,7 +535,43 @@ static bool any_running(const Batch& B, bool inner) {
   return false;
 }
 
-constexpr int kPollEvery = 8;
+// The host looks at the graphs' states (a synchronisation) to learn when every solve has stopped;
+// between looks it queues a window of iterations. A graph that stopped early lets the rest of the
+// window run through as empty launches, so the window is sized from the running graphs' own
+// residual histories: the iterations the slowest of them needs at its observed rate, at most
+// kPollMax (kPollFirst before a rate is known).
+constexpr int kPollFirst = 4;
+constexpr int kPollMax = 16;
+
+struct PollPlan {
+  std::vector<double> rn;
+  std::vector<int32_t> it;
+  // iterations to queue before the next look (states just read)
+  int next(const Batch& B, bool inner) {
+    const size_t G = B.hstate.size();
+    if (rn.size() != G) {
+      rn.assign(G, 0.0);
+      it.assign(G, -1);
+    }
+    int want = 1;
+    for (size_t g = 0; g < G; ++g) {
+      const PcgState& st = B.hstate[g];
+      const bool run = st.state == ST_RUNNING && (!inner || st.inner_state == ST_RUNNING);
+      const double r = inner ? st.inner_rn : st.rn;
+      const double target = inner ? st.inner_target : st.target;
+      const int32_t k = inner ? st.inner_iters : st.iters;
+      int need = kPollFirst;
+      if (run && it[g] >= 0 && k > it[g] && r > 0 && rn[g] > r && target > 0) {
+        const double rate = std::pow(r / rn[g], 1.0 / double(k - it[g]));
+        need = rate < 1.0 ? int(std::ceil(std::log(target / r) / std::log(rate))) : kPollMax;
+      }
+      if (run) want = std::max(want, std::min(kPollMax, std::max(1, need)));
+      rn[g] = r;
+      it[g] = run ? k : -1;
+    }
+    return want;
+  }
+};
 
 static void drop_self_batch_ptrs(svxg_problem* P);
 
@@ -531,13 +583,15 @@ static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
   const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
-  for (int it = 0;; ++it) {
-    if (it % kPollEvery == 0) {
+  PollPlan plan;
+  for (int left = 0;; --left) {
+    if (left == 0) {
       SVXG_TRY(read_states(B, s));
       if (!any_running(B, false)) break;
+      left = plan.next(B, false);
     }
     {
-      ProfScope ps(B, 0, s);
+      ProfScope ps(B, 0, s, SERVE_OUTER);
       launch_pcg_spmv<double>(s, G, t0, B.nt0, 0, part);
     }
     launch_finish(s, FinishKind::Alpha, G, gt, B.G, 0, part);
@@ -576,13 +630,15 @@ static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
     if (any32) launch_start_p<float>(s, G, t0, B.nt0, I32, part);
     if (any64) launch_start_p<double>(s, G, t0, B.nt0, I64, part);
     launch_finish(s, FinishKind::Start, G, gt, B.G, IN, part);
-    for (int it = 0;; ++it) {
-      if (it % kPollEvery == 0) {
+    PollPlan plan;
+    for (int left = 0;; --left) {
+      if (left == 0) {
         SVXG_TRY(read_states(B, s));
         if (!any_running(B, true)) break;
+        left = plan.next(B, true);
       }
       {
-        ProfScope ps(B, 1, s);
+        ProfScope ps(B, 1, s, SERVE_INNER32);
         if (any32) launch_pcg_spmv<float>(s, G, t0, B.nt0, I32, part);
       }
       if (any64) launch_pcg_spmv<double>(s, G, t0, B.nt0, I64, part);

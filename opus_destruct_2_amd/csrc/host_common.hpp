@@ -132,8 +132,9 @@ struct svxg_ctx {
   struct Prof {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
     int64_t launches = 0;
-    double ms = 0.0, bytes = 0.0, pending_bytes = 0.0;
+    double ms = 0.0;
   } prof[4];
+  svxg::DevBuf prof_acc;  // 4 doubles on the device: the bytes of the live graphs of each kind's launches
 };
 
 struct svxg_problem {

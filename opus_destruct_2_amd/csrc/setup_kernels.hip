@@ -282,9 +282,9 @@ __global__ void k_diag_trace(const AssemblyDev* __restrict__ J, const Tile* __re
 }
 
 // id_scale = mean of tr(K_ii[0:3,0:3]) / 3 over rows with a positive trace, summed in row order
-// (stress.cpp:111-122). One wavefront a graph: 64 traces are loaded and divided by 3 at a time, then
-// added in row order from scalar reads of the lanes whose trace is positive (the sum stays the
-// reference's sequential one; only the adds and the lane reads are serial).
+// (stress.cpp:111-122). One wavefront a graph: 64 traces are loaded and divided by 3 at a time
+// (rows without a positive trace contribute +0.0, which leaves the running sum's bits unchanged: it
+// starts at +0.0 and only grows), then added in row order from scalar reads of the 64 lanes.
 __global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
   const int g = blockIdx.x;
   if (g >= count) return;
@@ -295,13 +295,12 @@ __global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
   for (int base = 0; base < a.nfree; base += 64) {
     const int i = base + l;
     const double t = i < a.nfree ? a.trace[i] : 0.0;
-    const double t3 = t / 3.0;
-    uint64_t pos = __ballot(t > 0);
-    cnt += __popcll(pos);
+    const bool pos = t > 0;
+    const double t3 = pos ? t / 3.0 : 0.0;
+    cnt += __popcll(__ballot(pos));
     const int2 w = *reinterpret_cast<const int2*>(&t3);
-    while (pos) {
-      const int k = __ffsll(static_cast<unsigned long long>(pos)) - 1;
-      pos &= pos - 1;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
       int2 v;
       v.x = __builtin_amdgcn_readlane(w.x, k);
       v.y = __builtin_amdgcn_readlane(w.y, k);

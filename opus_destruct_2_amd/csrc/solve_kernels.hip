@@ -43,12 +43,14 @@ __device__ __forceinline__ void load6(const T* p, T v[6]) {
   v[5] = c.y;
 }
 
-// streamed matrix rows: loaded once per product, not worth caching
+// matrix rows: a row's 6 lanes read a block's 144 / 288 contiguous bytes in three loads of 8 / 16
+// bytes each, so a cache line is touched by all three: plain loads keep it in L2 between them (a
+// non-temporal hint here made the line leave and come back, half again the matrix's bytes)
 template <typename T>
 __device__ __forceinline__ void load6_stream(const T* p, T v[6]) {
   using P = typename V2<T>::t;
   const P* q = reinterpret_cast<const P*>(p);
-  const P a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1), c = __builtin_nontemporal_load(q + 2);
+  const P a = q[0], b = q[1], c = q[2];
   v[0] = a.x;
   v[1] = a.y;
   v[2] = b.x;
@@ -156,13 +158,29 @@ __device__ __forceinline__ void finish_partials(double v0, double v1, double* pa
 
 }  // namespace
 
+// ---- kernel timing -----------------------------------------------------------------------------
+
+// *acc += the algorithmic bytes of the graphs a launch served (those live for `inner`), so that
+// launches left running after a graph stopped count their time and none of its bytes
+__global__ void __launch_bounds__(kWG) k_prof_account(const GraphDev* __restrict__ G, int count, int inner,
+                                                      const double* __restrict__ gbytes, double* __restrict__ acc) {
+  __shared__ double lds[4];
+  double v = 0.0;
+  for (int g = threadIdx.x; g < count; g += kWG)
+    if (live(G[g], inner)) v += gbytes[g];
+  const double s = block_sum(v, lds);
+  if (threadIdx.x == 0) *acc += s;
+}
+
 // ---- V-cycle kernels ---------------------------------------------------------------------------
 
 // One Chebyshev step on `level`: r = b - A x_in (b when fresh: x_in = 0), d = c1 d + c2 D^-1 r,
-// x_out = x_in + d. S: six-lane sub-groups a row. With `partial`, the partial sums of b . x_out.
-template <typename T, int S>
+// x_out = x_in + d. S: six-lane sub-groups a row. FRESH: the first step from x_in = 0 (no matrix
+// pass; its own instantiation so that timings of the two kinds of step stay apart). With
+// `partial`, the partial sums of b . x_out.
+template <typename T, int S, bool FRESH>
 __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                   int ntiles, int level, int step, int in_buf, int out_buf, int fresh,
+                                                   int ntiles, int level, int step, int in_buf, int out_buf,
                                                    int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
@@ -179,14 +197,14 @@ __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ 
   T r = T(0), xin = T(0), bq = T(0);
   if (on) {
     bq = a.b[e];
-    if (fresh) {
+    if (FRESH) {
       r = bq;
     } else {
       xin = a.x[in_buf][e];
       r = spmv_row_part<T, S>(L.rowptr, L.col, a.A, a.x[in_buf], row, q, sub);
     }
   }
-  if (!fresh) r = bq - row_reduce<S>(r);
+  if (!FRESH) r = bq - row_reduce<S>(r);
   T rv[6];
   row_gather(r, rv);
   double c1, c2;
@@ -579,6 +597,7 @@ __global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict
   const double rn = sqrt(rr);
   if (inner) {
     s->inner_iters = 0;
+    s->inner_rn = rn;
     if (!(rn > s->inner_target)) {
       s->inner_state = ST_CONVERGED;
       if (s->inner64) {
@@ -643,6 +662,7 @@ __global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ 
   if (threadIdx.x != 0) return;
   const double rn = sqrt(rr);
   if (inner) {
+    s->inner_rn = rn;
     if (rn <= s->inner_target) {
       s->inner_state = ST_CONVERGED;
       if (s->inner64) {
@@ -806,12 +826,19 @@ template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
                       int out_buf, int fresh, int inner, double* partial, int S) {
   if (ntiles <= 0) return;
-  if (S == 10)
-    hipLaunchKernelGGL((k_cheb_step<T, 10>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, step,
-                       in_buf, out_buf, fresh, inner, partial);
+  const dim3 grid(grid_for(ntiles)), wg(kWG);
+  if (S == 10 && fresh)
+    hipLaunchKernelGGL((k_cheb_step<T, 10, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
+                       partial);
+  else if (S == 10)
+    hipLaunchKernelGGL((k_cheb_step<T, 10, false>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf,
+                       inner, partial);
+  else if (fresh)
+    hipLaunchKernelGGL((k_cheb_step<T, 1, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
+                       partial);
   else
-    hipLaunchKernelGGL((k_cheb_step<T, 1>), dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, step,
-                       in_buf, out_buf, fresh, inner, partial);
+    hipLaunchKernelGGL((k_cheb_step<T, 1, false>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf,
+                       inner, partial);
 }
 template <typename T>
 void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
@@ -923,6 +950,9 @@ void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, 
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_inner_init, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
+}
+void launch_prof_account(hipStream_t s, const GraphDev* G, int count, int inner, const double* gbytes, double* acc) {
+  if (count > 0) hipLaunchKernelGGL(k_prof_account, dim3(1), dim3(kWG), 0, s, G, count, inner, gbytes, acc);
 }
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;

@@ -45,6 +45,7 @@ struct PcgState {
   double rel_res;
   // mixed precision: outer residual norm and the inner solve's stopping norm
   double outer_rn, inner_target, eta, outer_rn_prev;
+  double inner_rn;  // the inner solve's latest residual norm (for the host's poll schedule)
   int32_t iters, maxit, state, outer;
   int32_t inner_state, inner_iters, inner_maxit, zero_load;
   // mixed precision: 1 once a refinement step stalled (the FP32 K's error times K's condition

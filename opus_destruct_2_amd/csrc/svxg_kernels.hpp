@@ -48,6 +48,9 @@ void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int3
                    const double* partial);
 void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial);
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
+// (kernel timing) acc += the bytes (gbytes per graph) of the graphs live for `inner`, read before the
+// next launch changes their states
+void launch_prof_account(hipStream_t s, const GraphDev* G, int count, int inner, const double* gbytes, double* acc);
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_inner_close(hipStream_t s, const GraphDev* G, int count);
 // FP64-product inner solves: r_i = (float) r (to32 = 1) or z = (double) z_i (to32 = 0)
