@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one live launch of a kernel on a workload, from two rocprofv3 counter passes of this
+# script (FETCH_SIZE scaled by 2.0, the factor a plain 1 GiB copy reads it with; WRITE_SIZE as read)
+MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 215.6e6}
 METRIC = "PCG-solved DOFs/s at 1e-8 residual, batched vs plain; p50/p99 tick latency"
 
 
@@ -72,7 +75,7 @@ def run_path(g, mats, precision, steps, warmup, ctx):
     return out
 
 
-def roofline(res, precision):
+def roofline(res, precision, shape):
     names = {0: "k_pcg_spmv<double>", 1: "k_pcg_spmv<float>", 2: "k_cheb_step<double, 1, false> (level 0)",
              3: "k_cheb_step<float, 1, false> (level 0)"}
     kinds = (1, 3) if precision == "mixed" else (0, 2)
@@ -83,8 +86,9 @@ def roofline(res, precision):
     avg_s = ms / n / 1e3
     per_launch = byts / n
     achieved = per_launch / avg_s / 1e9
+    traffic = MEASURED_TRAFFIC.get((names[best],) + tuple(shape))
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": names[best],
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": names[best],
             "launches": n, "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": int(per_launch)}
 
 
@@ -209,7 +213,7 @@ def main():
     plain = other if head_prec == SVXG_MIXED else head
     step_s = (head["solve_ms"] + head["checks_ms"]) / 1e3
     value = g.dofs / step_s
-    rl = roofline(head, args.precision)
+    rl = roofline(head, args.precision, (args.nx, args.ny, args.nz))
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
