@@ -6,9 +6,15 @@
 // contributions - the reference's Bsr6Builder rows, amg.cpp:185-224) and drives the device: every
 // floating-point value of K, of the hierarchy and of the solve is computed by the kernels.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
 
 #include "host_common.hpp"
 
@@ -129,8 +135,42 @@ static int upload_into(const DevBuf& d, const std::vector<T>& h, hipStream_t s) 
   return SVXG_OK;
 }
 
-// Uploads gathered into the context's pinned staging buffer and issued as asynchronous copies,
-// with one synchronisation for the lot (a batch of problems makes many small arrays)
+// the context's pinned host staging and device staging, at least `bytes` each
+static int staging(svxg_ctx* c, size_t bytes) {
+  if (bytes > c->pinned_bytes) {
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+    const size_t want = std::max(bytes, size_t(64) << 20);
+    SVXG_HIP(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_bytes = want;
+  }
+  if (bytes > c->dstage.bytes()) SVXG_TRY(c->dstage.make(c->arena.get(), std::max(bytes, size_t(64) << 20)));
+  return SVXG_OK;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// f(k) for k in [0, n) on up to 8 host threads (the memcpy of large batched uploads)
+template <typename F>
+static void copy_in_parallel(size_t n, F&& f) {
+  const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t T = std::min<size_t>({8, hw, n / 64 + 1});
+  if (T <= 1) {
+    for (size_t k = 0; k < n; ++k) f(k);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (size_t t = 0; t < T; ++t)
+    pool.emplace_back([&] {
+      for (size_t k = next.fetch_add(1); k < n; k = next.fetch_add(1)) f(k);
+    });
+  for (std::thread& th : pool) th.join();
+}
+
+// Uploads of a batch (many small arrays): packed into the pinned staging buffer with a table of
+// their destinations, sent in one copy, spread by one kernel, one synchronisation for the lot
 class Stager {
  public:
   Stager(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
@@ -145,21 +185,25 @@ class Stager {
     return SVXG_OK;
   }
   int flush() {
-    size_t total = 0;
-    for (const Job& j : jobs_) total += (j.bytes + 255) & ~size_t(255);
-    if (total > c_->pinned_bytes) {
-      if (c_->pinned) (void)hipHostFree(c_->pinned);
-      c_->pinned = nullptr;
-      const size_t want = std::max(total, size_t(64) << 20);
-      SVXG_HIP(hipHostMalloc(&c_->pinned, want, hipHostMallocDefault));
-      c_->pinned_bytes = want;
+    if (jobs_.empty()) return SVXG_OK;
+    size_t data = 0;
+    for (const Job& j : jobs_) data += round256(j.bytes);
+    const size_t table = round256(jobs_.size() * sizeof(BulkSeg));
+    SVXG_TRY(staging(c_, data + table));
+    char* hp = static_cast<char*>(c_->pinned);
+    char* dp = c_->dstage.as<char>();
+    BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
+    std::vector<size_t> offs(jobs_.size());
+    size_t off = 0;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+      offs[k] = off;
+      seg[k] = BulkSeg{jobs_[k].dst, dp + off, int64_t(jobs_[k].bytes)};
+      off += round256(jobs_[k].bytes);
     }
-    char* p = static_cast<char*>(c_->pinned);
-    for (const Job& j : jobs_) {
-      std::memcpy(p, j.src, j.bytes);
-      SVXG_HIP(hipMemcpyAsync(j.dst, p, j.bytes, hipMemcpyHostToDevice, s_));
-      p += (j.bytes + 255) & ~size_t(255);
-    }
+    copy_in_parallel(jobs_.size(), [&](size_t k) { std::memcpy(hp + offs[k], jobs_[k].src, jobs_[k].bytes); });
+    SVXG_HIP(hipMemcpyAsync(dp, hp, data + jobs_.size() * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
+    launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(jobs_.size()));
+    SVXG_HIP(hipGetLastError());
     SVXG_HIP(hipStreamSynchronize(s_));
     jobs_.clear();
     return SVXG_OK;
@@ -176,7 +220,8 @@ class Stager {
   std::vector<Job> jobs_;
 };
 
-// readbacks into the pinned staging buffer, one synchronisation, then to their destinations
+// Readbacks of a batch: one kernel packs them into the device staging buffer, one copy brings
+// them over, one synchronisation, then they go to their destinations
 class Gatherer {
  public:
   Gatherer(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
@@ -184,25 +229,28 @@ class Gatherer {
     if (bytes) jobs_.push_back({dst, dev, bytes});
   }
   int flush() {
-    size_t total = 0;
-    for (const Job& j : jobs_) total += (j.bytes + 255) & ~size_t(255);
-    if (total > c_->pinned_bytes) {
-      if (c_->pinned) (void)hipHostFree(c_->pinned);
-      c_->pinned = nullptr;
-      const size_t want = std::max(total, size_t(64) << 20);
-      SVXG_HIP(hipHostMalloc(&c_->pinned, want, hipHostMallocDefault));
-      c_->pinned_bytes = want;
+    if (jobs_.empty()) return SVXG_OK;
+    size_t data = 0;
+    for (const Job& j : jobs_) data += round256(j.bytes);
+    const size_t table = round256(jobs_.size() * sizeof(BulkSeg));
+    SVXG_TRY(staging(c_, data + table));
+    char* hp = static_cast<char*>(c_->pinned);
+    char* dp = c_->dstage.as<char>();
+    BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
+    size_t off = 0;
+    for (size_t k = 0; k < jobs_.size(); ++k) {
+      seg[k] = BulkSeg{dp + off, jobs_[k].dev, int64_t(jobs_[k].bytes)};
+      off += round256(jobs_[k].bytes);
     }
-    char* p = static_cast<char*>(c_->pinned);
-    for (const Job& j : jobs_) {
-      SVXG_HIP(hipMemcpyAsync(p, j.dev, j.bytes, hipMemcpyDeviceToHost, s_));
-      p += (j.bytes + 255) & ~size_t(255);
-    }
+    SVXG_HIP(hipMemcpyAsync(dp + data, seg, jobs_.size() * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
+    launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(jobs_.size()));
+    SVXG_HIP(hipGetLastError());
+    SVXG_HIP(hipMemcpyAsync(hp, dp, data, hipMemcpyDeviceToHost, s_));
     SVXG_HIP(hipStreamSynchronize(s_));
-    p = static_cast<char*>(c_->pinned);
+    off = 0;
     for (const Job& j : jobs_) {
-      std::memcpy(j.dst, p, j.bytes);
-      p += (j.bytes + 255) & ~size_t(255);
+      std::memcpy(j.dst, hp + off, j.bytes);
+      off += round256(j.bytes);
     }
     jobs_.clear();
     return SVXG_OK;
@@ -217,6 +265,51 @@ class Gatherer {
   svxg_ctx* c_;
   hipStream_t s_;
   std::vector<Job> jobs_;
+};
+
+// Fills and FP64 -> FP32 copies of a batch setup: small ones gathered into one launch each, large
+// ones (>= 4 MB) issued on their own; flush() before any launch reads what they write
+class BulkOps {
+ public:
+  explicit BulkOps(svxg_ctx* c) : c_(c) {}
+  void zero(void* dst, size_t bytes) {
+    if (!bytes) return;
+    if (bytes >= kBigBytes) big_zero_.push_back(BulkSeg{dst, nullptr, int64_t(bytes)});
+    else zero_.push_back(BulkSeg{dst, nullptr, int64_t(bytes)});
+  }
+  void to_f32(const double* src, float* dst, int64_t n) {
+    if (n <= 0) return;
+    if (size_t(n) * 8 >= kBigBytes) big_f32_.push_back(BulkSeg{dst, src, n});
+    else f32_.push_back(BulkSeg{dst, src, n});
+  }
+  int flush(hipStream_t s) {
+    for (const BulkSeg& g : big_zero_) SVXG_HIP(hipMemsetAsync(g.dst, 0, size_t(g.count), s));
+    for (const BulkSeg& g : big_f32_)
+      launch_to_f32(s, static_cast<const double*>(g.src), static_cast<float*>(g.dst), g.count);
+    if (!zero_.empty() || !f32_.empty()) {
+      const size_t bytes = (zero_.size() + f32_.size()) * sizeof(BulkSeg);
+      SVXG_TRY(staging(c_, bytes));
+      std::memcpy(c_->pinned, zero_.data(), zero_.size() * sizeof(BulkSeg));
+      std::memcpy(static_cast<char*>(c_->pinned) + zero_.size() * sizeof(BulkSeg), f32_.data(),
+                  f32_.size() * sizeof(BulkSeg));
+      SVXG_HIP(hipMemcpyAsync(c_->dstage.as<void>(), c_->pinned, bytes, hipMemcpyHostToDevice, s));
+      const BulkSeg* d = c_->dstage.as<BulkSeg>();
+      launch_zero_many(s, d, int(zero_.size()));
+      launch_f32_many(s, d + zero_.size(), int(f32_.size()));
+    }
+    SVXG_HIP(hipGetLastError());
+    SVXG_HIP(hipStreamSynchronize(s));
+    zero_.clear();
+    f32_.clear();
+    big_zero_.clear();
+    big_f32_.clear();
+    return SVXG_OK;
+  }
+
+ private:
+  static constexpr size_t kBigBytes = size_t(4) << 20;
+  svxg_ctx* c_;
+  std::vector<BulkSeg> zero_, f32_, big_zero_, big_f32_;
 };
 
 template <typename T>

@@ -134,6 +134,7 @@ struct svxg_ctx {
     int64_t launches = 0;
     double ms = 0.0;
   } prof[4];
+  svxg::DevBuf dstage;    // device staging of batched uploads and readbacks (grown on demand)
   svxg::DevBuf prof_acc;  // 4 doubles on the device: the bytes of the live graphs of each kind's launches
 };
 

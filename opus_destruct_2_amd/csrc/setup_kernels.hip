@@ -1070,16 +1070,26 @@ __global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict
 
 // ---- dense coarsest level: Cholesky (amg.cpp:535-578), then the explicit inverse --------------------
 
-__global__ void __launch_bounds__(kWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
+// One workgroup of 1024 lanes a graph, every phase parallel over the matrix: the symmetrised matrix,
+// a right-looking Cholesky with the reference's pivot floor (d >= 1e-12 max|a_ii|, amg.cpp:535-578),
+// W = L^-1 by right-looking forward substitution on I, then M^-1 = W^T W.
+constexpr int kDenseWG = 1024;
+__global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
   if (int(blockIdx.x) >= count) return;
   const DenseDev& d = J[blockIdx.x];
   const int m = d.m;
+  const int tid = threadIdx.x;
   double* L = d.L;
-  double* W = d.Minv;  // (the inverse factor first, then the inverse in place of L)
+  double* W = d.Minv;  // (W = L^-1 first; M^-1 is formed in L's storage and copied over)
   auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(m) + size_t(c)]; };
-  for (size_t t = threadIdx.x; t < size_t(m) * size_t(m); t += kWG) L[t] = 0.0;
+  auto wat = [&](int r, int c) -> double& { return W[size_t(r) * size_t(m) + size_t(c)]; };
+  const size_t mm = size_t(m) * size_t(m);
+  for (size_t t = tid; t < mm; t += kDenseWG) {
+    L[t] = 0.0;
+    W[t] = (t / size_t(m) == t % size_t(m)) ? 1.0 : 0.0;
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < d.n; i += kWG)
+  for (int i = tid; i < d.n; i += kDenseWG)
     for (int k = d.rowptr[i]; k < d.rowptr[i + 1]; ++k) {
       const int j = d.col[k];
       const double* b = d.A + 36 * size_t(k);
@@ -1087,53 +1097,64 @@ __global__ void __launch_bounds__(kWG) k_dense_factor(const DenseDev* __restrict
         for (int c = 0; c < 6; ++c) at(6 * i + r, 6 * j + c) += b[r * 6 + c];
     }
   __syncthreads();
-  __shared__ double sdmax, ss;
-  if (threadIdx.x == 0) {
-    double dmax = 0.0;
-    for (int i = 0; i < m; ++i) dmax = fmax(dmax, fabs(at(i, i)));
-    sdmax = dmax;
+  for (size_t t = tid; t < mm; t += kDenseWG) {
+    const int i = int(t / size_t(m)), j = int(t % size_t(m));
+    if (i > j) at(i, j) = 0.5 * (at(i, j) + at(j, i));
   }
+  __shared__ double red[kDenseWG / 64];
+  __shared__ double spiv;
+  double dm = 0.0;
+  for (int i = tid; i < m; i += kDenseWG) dm = fmax(dm, fabs(at(i, i)));
+  for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_xor(dm, o, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = dm;
   __syncthreads();
-  const double dmax = sdmax;
+  double dmax = 0.0;
+  for (int w = 0; w < kDenseWG / 64; ++w) dmax = fmax(dmax, red[w]);
+  // Cholesky (lower triangle)
   for (int j = 0; j < m; ++j) {
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       double dd = at(j, j);
-      for (int k = 0; k < j; ++k) dd -= at(j, k) * at(j, k);
       if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
-      const double s = sqrt(dd);
-      at(j, j) = s;
-      ss = s;
+      const double sq = sqrt(dd);
+      at(j, j) = sq;
+      spiv = sq;
     }
     __syncthreads();
-    const double s = ss;
-    for (int i = j + 1 + threadIdx.x; i < m; i += kWG) {
-      double v = 0.5 * (at(i, j) + at(j, i));
-      for (int k = 0; k < j; ++k) v -= at(i, k) * at(j, k);
-      at(i, j) = v / s;
+    const double sq = spiv;
+    for (int i = j + 1 + tid; i < m; i += kDenseWG) at(i, j) /= sq;
+    __syncthreads();
+    const int w = m - j - 1;
+    for (int t = tid; t < w * w; t += kDenseWG) {
+      const int i = j + 1 + t / w, k = j + 1 + t % w;
+      if (k <= i) at(i, k) -= at(i, j) * at(k, j);
     }
     __syncthreads();
   }
-  // W = L^-1 (lower triangular), one column a thread
-  for (int c = threadIdx.x; c < m; c += kWG) {
-    for (int i = 0; i < c; ++i) W[size_t(i) * m + c] = 0.0;
-    W[size_t(c) * m + c] = 1.0 / at(c, c);
-    for (int i = c + 1; i < m; ++i) {
-      double v = 0.0;
-      for (int k = c; k < i; ++k) v += at(i, k) * W[size_t(k) * m + c];
-      W[size_t(i) * m + c] = -v / at(i, i);
+  // W = L^-1: row k divided by L_kk, then taken off the rows below
+  for (int k = 0; k < m; ++k) {
+    const double lk = at(k, k);
+    for (int c = tid; c <= k; c += kDenseWG) wat(k, c) /= lk;
+    __syncthreads();
+    const int rows = m - k - 1, cols = k + 1;
+    for (int t = tid; t < rows * cols; t += kDenseWG) {
+      const int i = k + 1 + t / cols, c = t % cols;
+      wat(i, c) -= at(i, k) * wat(k, c);
     }
+    __syncthreads();
   }
-  __syncthreads();
-  // M^-1 = W^T W into L's storage, then copied over W
-  for (size_t t = threadIdx.x; t < size_t(m) * size_t(m); t += kWG) {
-    const int i = int(t / m), j = int(t % m);
-    const int k0 = i > j ? i : j;
+  // M^-1 = W^T W (lower half, mirrored), into L's storage, then over W
+  for (size_t t = tid; t < mm; t += kDenseWG) {
+    const int i = int(t / size_t(m)), j = int(t % size_t(m));
+    if (j > i) continue;
     double v = 0.0;
-    for (int k = k0; k < m; ++k) v += W[size_t(k) * m + i] * W[size_t(k) * m + j];
-    L[t] = v;
+    for (int r = i; r < m; ++r) v += wat(r, i) * wat(r, j);
+    at(i, j) = v;
   }
   __syncthreads();
-  for (size_t t = threadIdx.x; t < size_t(m) * size_t(m); t += kWG) W[t] = L[t];
+  for (size_t t = tid; t < mm; t += kDenseWG) {
+    const int i = int(t / size_t(m)), j = int(t % size_t(m));
+    W[t] = j <= i ? at(i, j) : at(j, i);
+  }
 }
 
 // ---- conversions -----------------------------------------------------------------------------------
@@ -1141,6 +1162,41 @@ __global__ void __launch_bounds__(kWG) k_dense_factor(const DenseDev* __restrict
 __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   for (int64_t k = t; k < count; k += int64_t(gridDim.x) * blockDim.x) out[k] = float(in[k]);
+}
+
+// ---- bulk segments (one workgroup a segment at a time) ------------------------------------------------
+
+__global__ void __launch_bounds__(kWG) k_zero_many(const BulkSeg* __restrict__ segs, int n) {
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const BulkSeg g = segs[k];
+    uint4* d = static_cast<uint4*>(g.dst);
+    const int64_t v = g.count >> 4;
+    for (int64_t i = threadIdx.x; i < v; i += kWG) d[i] = make_uint4(0u, 0u, 0u, 0u);
+    uint8_t* b = static_cast<uint8_t*>(g.dst);
+    for (int64_t i = (v << 4) + threadIdx.x; i < g.count; i += kWG) b[i] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kWG) k_f32_many(const BulkSeg* __restrict__ segs, int n) {
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const BulkSeg g = segs[k];
+    const double* in = static_cast<const double*>(g.src);
+    float* out = static_cast<float*>(g.dst);
+    for (int64_t i = threadIdx.x; i < g.count; i += kWG) out[i] = float(in[i]);
+  }
+}
+
+__global__ void __launch_bounds__(kWG) k_copy_many(const BulkSeg* __restrict__ segs, int n) {
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const BulkSeg g = segs[k];
+    const int64_t v = g.count >> 4;
+    const uint4* s4 = static_cast<const uint4*>(g.src);
+    uint4* d4 = static_cast<uint4*>(g.dst);
+    for (int64_t i = threadIdx.x; i < v; i += kWG) d4[i] = s4[i];
+    const uint8_t* s1 = static_cast<const uint8_t*>(g.src);
+    uint8_t* d1 = static_cast<uint8_t*>(g.dst);
+    for (int64_t i = (v << 4) + threadIdx.x; i < g.count; i += kWG) d1[i] = s1[i];
+  }
 }
 
 // ---- bond checks -------------------------------------------------------------------------------------
@@ -1369,12 +1425,22 @@ void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gti
   hipLaunchKernelGGL(k_power_finish, dim3(count), dim3(kWG), 0, s, jobs, gtile, count, partial);
 }
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count) {
-  if (count > 0) hipLaunchKernelGGL(k_dense_factor, dim3(count), dim3(kWG), 0, s, jobs, count);
+  if (count > 0) hipLaunchKernelGGL(k_dense_factor, dim3(count), dim3(kDenseWG), 0, s, jobs, count);
 }
 void launch_to_f32(hipStream_t s, const double* in, float* out, int64_t count) {
   if (count <= 0) return;
   const int64_t blocks = (count + 255) / 256;
   hipLaunchKernelGGL(k_to_f32, dim3(unsigned(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, in, out, count);
+}
+static unsigned bulk_grid(int n) { return unsigned(n < 65536 ? n : 65536); }
+void launch_zero_many(hipStream_t s, const BulkSeg* segs, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
+}
+void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_f32_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
+}
+void launch_copy_many(hipStream_t s, const BulkSeg* segs, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_copy_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
 }
 void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_bond_checks, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);

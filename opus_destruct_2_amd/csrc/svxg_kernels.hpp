@@ -223,6 +223,19 @@ void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count);
 // FP32 copies
 void launch_to_f32(hipStream_t s, const double* in, float* out, int64_t count);
 
+// ---- bulk copies of a batch setup: many small segments, one launch -----------------------------------
+struct BulkSeg {
+  void* dst;
+  const void* src;  // device source (copies), or a double array (conversions), or null (fills)
+  int64_t count;    // bytes (fills, copies) or elements (conversions)
+};
+// each segment's count bytes at dst set to zero (dst 16-byte aligned)
+void launch_zero_many(hipStream_t s, const BulkSeg* segs, int n);
+// each segment's count doubles at src written as floats at dst
+void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n);
+// each segment's count bytes copied from src to dst (both 16-byte aligned)
+void launch_copy_many(hipStream_t s, const BulkSeg* segs, int n);
+
 // ---- bond checks (stress.cpp:14-57, 477-502) ---------------------------------------------------
 struct CheckDev {
   int32_t nbonds;
