@@ -649,6 +649,16 @@ struct PollPlan {
     int want = 1;
     for (size_t g = 0; g < G; ++g) {
       const PcgState& st = B.hstate[g];
+      if (inner && st.state == ST_RUNNING && st.phase == PH_WAIT64) {
+        it[g] = -1;  // (released at the next look)
+        continue;
+      }
+      if (inner && st.state == ST_RUNNING && st.inner_state != ST_RUNNING) {
+        // (between inner solves: a refinement residual, then a new inner solve)
+        want = std::max(want, kPollFirst);
+        it[g] = -1;
+        continue;
+      }
       const bool run = st.state == ST_RUNNING && (!inner || st.inner_state == ST_RUNNING);
       const double r = inner ? st.inner_rn : st.rn;
       const double target = inner ? st.inner_target : st.target;
@@ -702,54 +712,71 @@ static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
 // FP64 iterative refinement around an FP32 PCG (FP32 K products and V-cycle, FP64 dots): each
 // outer step recomputes r = b - K x in FP64, solves K d = r to max(eta |r|, target / 2) in FP32 and
 // adds d to x
-static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
+// One step of every graph's mixed-precision solve, whatever its phase: a refinement step's FP64
+// residual and check (PH_OUTER), an inner solve's start (PH_START) or iteration (PH_INNER), the
+// correction x += d (PH_APPLY). The launches come in an order in which a graph moves on at most one
+// phase per kernel and never meets a launch of its new phase in the same step before that phase's
+// own sequence begins. p32 / p64: launch the FP32- / FP64-product kernels.
+static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64) {
   const GraphDev* G = B.dG.as<GraphDev>();
   const Tile* t0 = B.t0.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
+  launch_outer_residual(s, G, t0, B.nt0, part);
+  launch_finish(s, FinishKind::Outer, G, gt, B.G, SERVE_REFINE, part);
+  launch_inner_init(s, G, t0, B.nt0);
+  {
+    ProfScope ps(B, 1, s, SERVE_P32 | SERVE_IT);
+    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT, part);
+  }
+  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT, part);
+  launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_ITER, part);
+  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT, part);
+  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT, part);
+  launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_ITER, part);
+  if (p64) launch_convert64(s, G, t0, B.nt0, 1);
+  vcycle<float>(B, s, SERVE_INNER, degree);
+  if (p64) launch_convert64(s, G, t0, B.nt0, 0);
+  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT, part);
+  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT, part);
+  launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);
+  if (p32) launch_update_p<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT);
+  if (p64) launch_update_p<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT);
+  if (p32) launch_start_p<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_ST, part);
+  if (p64) launch_start_p<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_ST, part);
+  launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
+  launch_inner_apply(s, G, t0, B.nt0);
+  launch_inner_close(s, G, B.G);
+}
+
+// which product precisions the next window launches (states just read): FP64 products once a
+// running graph takes them or waits for them (waiting graphs are released first), FP32 products
+// while a running graph has not switched
+static void mixed_window(Batch& B, hipStream_t s, bool& p32, bool& p64) {
+  p32 = p64 = false;
+  bool wait = false;
+  for (const PcgState& st : B.hstate) {
+    if (st.state != ST_RUNNING) continue;
+    (st.inner64 ? p64 : p32) = true;
+    wait = wait || st.phase == PH_WAIT64;
+  }
+  if (wait) launch_release_wait64(s, B.dG.as<GraphDev>(), B.G);
+}
+
+// the mixed-precision solve (after run_begin): FP64 iterative refinement around FP32 inner PCG
+// solves, each graph stepping through its own refinements
+static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
   const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
-  const int I32 = SERVE_INNER32, I64 = SERVE_INNER64, IN = SERVE_INNER;
-  for (int outer = 0; outer < 100; ++outer) {
-    launch_outer_residual(s, G, t0, B.nt0, part);
-    launch_finish(s, FinishKind::Outer, G, gt, B.G, 0, part);
-    SVXG_TRY(read_states(B, s));
-    if (!any_running(B, false)) break;
-    bool any32 = false, any64 = false;
-    for (const PcgState& st : B.hstate)
-      if (st.state == ST_RUNNING) (st.inner64 ? any64 : any32) = true;
-    launch_inner_init(s, G, t0, B.nt0);
-    vcycle<float>(B, s, IN, degree);
-    if (any64) launch_convert64(s, G, t0, B.nt0, 0);
-    if (any32) launch_start_p<float>(s, G, t0, B.nt0, I32, part);
-    if (any64) launch_start_p<double>(s, G, t0, B.nt0, I64, part);
-    launch_finish(s, FinishKind::Start, G, gt, B.G, IN, part);
-    PollPlan plan;
-    for (int left = 0;; --left) {
-      if (left == 0) {
-        SVXG_TRY(read_states(B, s));
-        if (!any_running(B, true)) break;
-        left = plan.next(B, true);
-      }
-      {
-        ProfScope ps(B, 1, s, SERVE_INNER32);
-        if (any32) launch_pcg_spmv<float>(s, G, t0, B.nt0, I32, part);
-      }
-      if (any64) launch_pcg_spmv<double>(s, G, t0, B.nt0, I64, part);
-      launch_finish(s, FinishKind::Alpha, G, gt, B.G, IN, part);
-      if (any32) launch_axpy_rr<float>(s, G, t0, B.nt0, I32, part);
-      if (any64) launch_axpy_rr<double>(s, G, t0, B.nt0, I64, part);
-      launch_finish(s, FinishKind::Rn, G, gt, B.G, IN, part);
-      if (any64) launch_convert64(s, G, t0, B.nt0, 1);
-      vcycle<float>(B, s, IN, degree);
-      if (any64) launch_convert64(s, G, t0, B.nt0, 0);
-      if (any32) launch_dot_rz<float>(s, G, t0, B.nt0, I32, part);
-      if (any64) launch_dot_rz<double>(s, G, t0, B.nt0, I64, part);
-      launch_finish(s, FinishKind::Beta, G, gt, B.G, IN, part);
-      if (any32) launch_update_p<float>(s, G, t0, B.nt0, I32);
-      if (any64) launch_update_p<double>(s, G, t0, B.nt0, I64);
+  PollPlan plan;
+  bool p32 = true, p64 = false;
+  for (int left = 0;; --left) {
+    if (left == 0) {
+      SVXG_TRY(read_states(B, s));
+      if (!any_running(B, false)) break;
+      left = plan.next(B, true);
+      mixed_window(B, s, p32, p64);
     }
-    launch_inner_apply(s, G, t0, B.nt0);
-    launch_inner_close(s, G, B.G);
+    mixed_step(B, s, degree, p32, p64);
   }
   SVXG_HIP(hipGetLastError());
   return SVXG_OK;

@@ -112,16 +112,17 @@ __device__ __forceinline__ const LevelArrays<float>& arrays<float>(const LevelDe
   return L.f32;
 }
 
-// the graph takes part in this launch: its (inner, for the FP32 V-cycle of the mixed path) solve
-// is running
-__device__ __forceinline__ bool live(const GraphDev& g, int inner) {
+// the graph takes part in a launch serving `mode`
+__device__ __forceinline__ bool live(const GraphDev& g, int mode) {
   const PcgState* s = g.st;
   if (s->state != ST_RUNNING) return false;
-  if (inner == SERVE_OUTER) return true;
+  if (mode == SERVE_OUTER) return s->phase == PH_ITER;
+  if (mode == SERVE_NEW) return s->phase == PH_NEW;
+  if (mode == SERVE_REFINE) return s->phase == PH_OUTER;
   if (s->inner_state != ST_RUNNING) return false;
-  if (inner == SERVE_INNER32) return !s->inner64;
-  if (inner == SERVE_INNER64) return s->inner64 != 0;
-  return true;
+  if (!(mode & (s->inner64 ? SERVE_P64 : SERVE_P32))) return false;
+  return (mode & SERVE_ST) ? s->phase == PH_START || ((mode & SERVE_IT) && s->phase == PH_INNER)
+                           : (mode & SERVE_IT) && s->phase == PH_INNER;
 }
 
 // Chebyshev coefficients of step j of the smoother on [lmax / ratio, lmax]: the direction is
@@ -492,7 +493,7 @@ __global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restri
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (g.st->state != ST_RUNNING) return;
+  if (!live(g, SERVE_NEW)) return;
   const LevelDev& L = g.lv[0];
   int rl, q;
   lane_row(rl, q);
@@ -510,13 +511,14 @@ __global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restri
   finish_partials(sb, sr, partial, tl.slot, true);
 }
 
-// the solve's start, per graph: |b|, a zero or non-finite load (x = 0, r = 0)
+// the solve's start, per graph: |b|, a zero or non-finite load (x = 0, r = 0); a mixed-precision
+// graph goes on to its first refinement step
 __global__ void __launch_bounds__(kWG) k_finish_begin(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, const double* __restrict__ partial) {
+                                                      int count, int mixed, const double* __restrict__ partial) {
   if (int(blockIdx.x) >= count) return;
   const GraphDev& g = G[blockIdx.x];
   PcgState* s = g.st;
-  if (s->state != ST_RUNNING) return;
+  if (!live(g, SERVE_NEW)) return;
   __shared__ double lds[8];
   double a = 0.0;
   for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k];
@@ -525,6 +527,7 @@ __global__ void __launch_bounds__(kWG) k_finish_begin(const GraphDev* __restrict
     s->bn = sqrt(bb);
     s->target = s->rtol * s->bn;
     s->zero_load = (bb > 0.0) ? 0 : 1;
+    if (mixed) s->phase = PH_OUTER;
   }
 }
 
@@ -535,7 +538,7 @@ __global__ void __launch_bounds__(kWG) k_zero_load(const GraphDev* __restrict__ 
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (g.st->state != ST_RUNNING || !g.st->zero_load) return;
+  if (g.st->state != ST_RUNNING || !g.st->zero_load || (g.st->phase != PH_NEW && g.st->phase != PH_OUTER)) return;
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
@@ -577,29 +580,32 @@ __device__ __forceinline__ double graph_sum(const int32_t* gtile, const double* 
   return block_sum(a, lds);
 }
 
-__device__ __forceinline__ PcgState* finish_state(const GraphDev& g, int inner, bool& go) {
-  PcgState* s = g.st;
-  go = s->state == ST_RUNNING && (!inner || s->inner_state == ST_RUNNING);
-  return s;
+__device__ __forceinline__ PcgState* finish_state(const GraphDev& g, int mode, bool& go) {
+  go = live(g, mode);
+  return g.st;
 }
+__device__ __forceinline__ bool inner_mode(int mode) { return (mode & SERVE_INNER) != 0; }
 
-// after p = z: rz, the first residual check (StressProblem::begin / the head of iterate)
+// after p = z: rz, the first residual check (StressProblem::begin / the head of iterate); the
+// graph goes on to iterate
 __global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, int inner, const double* __restrict__ partial) {
+                                                      int count, int mode, const double* __restrict__ partial) {
   if (int(blockIdx.x) >= count) return;
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  PcgState* s = finish_state(G[blockIdx.x], mode, go);
   if (!go) return;
   const double rz = graph_sum(gtile, partial, 0);
   const double rr = graph_sum(gtile, partial, 1);
   if (threadIdx.x != 0) return;
   s->rz = rz;
   const double rn = sqrt(rr);
-  if (inner) {
+  if (inner_mode(mode)) {
     s->inner_iters = 0;
     s->inner_rn = rn;
+    s->phase = PH_INNER;
     if (!(rn > s->inner_target)) {
       s->inner_state = ST_CONVERGED;
+      s->phase = PH_APPLY;
       if (s->inner64) {
         s->rn = rn;
         s->rel_res = rn / s->bn;
@@ -608,6 +614,7 @@ __global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict
     }
     return;
   }
+  s->phase = PH_ITER;
   s->rn = rn;
   if (!isfinite(s->bn) || !isfinite(rn)) {
     s->rel_res = INFINITY;
@@ -619,22 +626,24 @@ __global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict
 }
 
 __global__ void __launch_bounds__(kWG) k_finish_alpha(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, int inner, const double* __restrict__ partial) {
+                                                      int count, int mode, const double* __restrict__ partial) {
   if (int(blockIdx.x) >= count) return;
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  PcgState* s = finish_state(G[blockIdx.x], mode, go);
   if (!go) return;
   const double pq = graph_sum(gtile, partial, 0);
   if (threadIdx.x != 0) return;
   // (the iteration budget is checked at an iteration's head: a stopped run resumes where it was)
-  if (inner) {
+  if (inner_mode(mode)) {
     if (s->inner_iters >= s->inner_maxit) {
       s->inner_state = ST_STOPPED;
+      s->phase = PH_APPLY;
       return;
     }
     ++s->inner_iters;
     if (!(pq > 0)) {
       s->inner_state = ST_BREAKDOWN;
+      s->phase = PH_APPLY;
       return;
     }
   } else {
@@ -653,18 +662,19 @@ __global__ void __launch_bounds__(kWG) k_finish_alpha(const GraphDev* __restrict
 }
 
 __global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                   int count, int inner, const double* __restrict__ partial) {
+                                                   int count, int mode, const double* __restrict__ partial) {
   if (int(blockIdx.x) >= count) return;
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  PcgState* s = finish_state(G[blockIdx.x], mode, go);
   if (!go) return;
   const double rr = graph_sum(gtile, partial, 0);
   if (threadIdx.x != 0) return;
   const double rn = sqrt(rr);
-  if (inner) {
+  if (inner_mode(mode)) {
     s->inner_rn = rn;
     if (rn <= s->inner_target) {
       s->inner_state = ST_CONVERGED;
+      s->phase = PH_APPLY;
       if (s->inner64) {
         s->rn = rn;
         s->rel_res = rn / s->bn;
@@ -679,16 +689,20 @@ __global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ 
 }
 
 __global__ void __launch_bounds__(kWG) k_finish_beta(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                     int count, int inner, const double* __restrict__ partial) {
+                                                     int count, int mode, const double* __restrict__ partial) {
   if (int(blockIdx.x) >= count) return;
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], inner, go);
+  PcgState* s = finish_state(G[blockIdx.x], mode, go);
   if (!go) return;
   const double rz1 = graph_sum(gtile, partial, 0);
   if (threadIdx.x != 0) return;
   if (!(rz1 > 0)) {
-    if (inner) s->inner_state = ST_BREAKDOWN;
-    else s->state = ST_BREAKDOWN;
+    if (inner_mode(mode)) {
+      s->inner_state = ST_BREAKDOWN;
+      s->phase = PH_APPLY;
+    } else {
+      s->state = ST_BREAKDOWN;
+    }
     return;
   }
   s->beta = rz1 / s->rz;
@@ -704,7 +718,7 @@ __global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restri
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (g.st->state != ST_RUNNING) return;
+  if (!live(g, SERVE_REFINE)) return;
   const LevelDev& L = g.lv[0];
   int rl, q;
   lane_row(rl, q);
@@ -725,7 +739,7 @@ __global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict
                                                       int count, const double* __restrict__ partial) {
   if (int(blockIdx.x) >= count) return;
   PcgState* s = G[blockIdx.x].st;
-  if (s->state != ST_RUNNING) return;
+  if (!live(G[blockIdx.x], SERVE_REFINE)) return;
   const double rr = graph_sum(gtile, partial, 0);
   if (threadIdx.x != 0) return;
   const double rn = sqrt(rr);
@@ -747,14 +761,24 @@ __global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict
   }
   // a step that missed its goal tenfold: FP32 K's error times K's condition number stalls this graph;
   // its inner solves take FP64 K products from now on and stop, as the plain FP64 PCG does, on their
-  // recurrence residual at the target
-  if (s->outer > 0 && rn > 10.0 * s->inner_target) s->inner64 = 1;
+  // recurrence residual at the target (it waits until the host launches FP64 products)
+  const bool to64 = s->outer > 0 && rn > 10.0 * s->inner_target && !s->inner64;
+  if (to64) s->inner64 = 1;
   s->outer_rn_prev = rn;
   s->inner_target = s->inner64 ? s->target : fmax(s->eta * rn, 0.5 * s->target);
   s->inner_state = ST_RUNNING;
   s->inner_iters = 0;
   s->inner_maxit = s->maxit - s->iters;
+  s->phase = to64 ? PH_WAIT64 : PH_START;
   ++s->outer;
+}
+
+// graphs waiting for FP64-product inner solves start them (the host launches those from now on)
+__global__ void k_release_wait64(const GraphDev* __restrict__ G, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  PcgState* s = G[k].st;
+  if (s->state == ST_RUNNING && s->phase == PH_WAIT64) s->phase = PH_START;
 }
 
 // inner start: r_i = (float) r, x_i = 0
@@ -764,7 +788,7 @@ __global__ void __launch_bounds__(kWG) k_inner_init(const GraphDev* __restrict__
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, SERVE_INNER)) return;
+  if (!live(g, SERVE_START)) return;
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
@@ -780,7 +804,7 @@ __global__ void __launch_bounds__(kWG) k_convert64(const GraphDev* __restrict__ 
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, SERVE_INNER64)) return;
+  if (!live(g, to32 ? (SERVE_P64 | SERVE_IT) : SERVE_INNER64)) return;
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
@@ -797,7 +821,7 @@ __global__ void __launch_bounds__(kWG) k_inner_apply(const GraphDev* __restrict_
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   // (an FP64-product inner solve updated x itself)
-  if (g.st->state != ST_RUNNING || g.st->inner_state == ST_IDLE || g.st->inner64) return;
+  if (g.st->state != ST_RUNNING || g.st->phase != PH_APPLY || g.st->inner64) return;
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
@@ -809,10 +833,11 @@ __global__ void k_inner_close(const GraphDev* __restrict__ G, int count) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   PcgState* s = G[k].st;
-  if (s->inner_state == ST_IDLE) return;
+  if (s->phase != PH_APPLY) return;
   s->iters += s->inner_iters;
   if (s->inner_state == ST_BREAKDOWN && s->inner_iters == 0) s->state = ST_BREAKDOWN;
   s->inner_state = ST_IDLE;
+  s->phase = PH_OUTER;
 }
 
 // ---- host launchers ----------------------------------------------------------------------------
@@ -914,9 +939,13 @@ void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, 
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_begin_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
 }
-void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, const double* partial) {
+void launch_release_wait64(hipStream_t s, const GraphDev* G, int count) {
+  if (count > 0) hipLaunchKernelGGL(k_release_wait64, dim3((count + 255) / 256), dim3(256), 0, s, G, count);
+}
+void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, int mixed,
+                         const double* partial) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_finish_begin, dim3(count), dim3(kWG), 0, s, G, gtile, count, partial);
+  hipLaunchKernelGGL(k_finish_begin, dim3(count), dim3(kWG), 0, s, G, gtile, count, mixed, partial);
 }
 void launch_zero_load(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;

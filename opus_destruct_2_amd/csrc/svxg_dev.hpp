@@ -50,12 +50,40 @@ struct PcgState {
   int32_t inner_state, inner_iters, inner_maxit, zero_load;
   // mixed precision: 1 once a refinement step stalled (the FP32 K's error times K's condition
   // number near 1): this graph's inner solves then take FP64 K products, FP32 V-cycle
-  int32_t inner64, pad_;
+  int32_t inner64;
+  int32_t phase;  // PH_*: where the graph is in its solve
 };
 
-// who a launch serves: 0 the FP64 PCG; 1 inner solves with FP32 K products; 2 inner solves with
-// FP64 K products; 3 every inner solve (the FP32 V-cycle, the per-graph finishes)
-enum : int32_t { SERVE_OUTER = 0, SERVE_INNER32 = 1, SERVE_INNER64 = 2, SERVE_INNER = 3 };
+// Every graph of a batch runs its own solve: each launch acts on the graphs in the phase (and, for
+// inner solves, the product precision) it serves and leaves the rest alone, so graphs can enter a
+// running batch, and leave it, between any two steps, each doing the same arithmetic as alone.
+enum : int32_t {
+  PH_NEW = 0,     // admitted: the start of the solve is next (r = b - K x, |b|; FP64: z, p, rz)
+  PH_ITER = 1,    // FP64 PCG iterations
+  PH_OUTER = 2,   // mixed: a refinement step's FP64 residual and check are next
+  PH_START = 3,   // mixed: an inner solve (K d = r) starts
+  PH_INNER = 4,   // mixed: the inner solve iterates
+  PH_APPLY = 5,   // mixed: the inner solve ended; x += d and the step closes
+  PH_WAIT64 = 6,  // mixed: its next inner solve takes FP64 products, once the host launches them
+};
+
+// who a launch serves (the graph must be running): SERVE_OUTER the FP64 PCG iterations,
+// SERVE_NEW solve starts, SERVE_REFINE refinement residuals; otherwise inner solves, by bits: 1
+// FP32 K products, 2 FP64 K products, 4 starting, 8 iterating
+enum : int32_t {
+  SERVE_OUTER = 0,
+  SERVE_P32 = 1,
+  SERVE_P64 = 2,
+  SERVE_ST = 4,
+  SERVE_IT = 8,
+  SERVE_START = 1 | 2 | 4,
+  SERVE_ITER = 1 | 2 | 8,
+  SERVE_INNER32 = 1 | 4 | 8,
+  SERVE_INNER64 = 2 | 4 | 8,
+  SERVE_INNER = 1 | 2 | 4 | 8,
+  SERVE_REFINE = 16,
+  SERVE_NEW = 32,
+};
 
 template <typename T>
 struct LevelArrays {

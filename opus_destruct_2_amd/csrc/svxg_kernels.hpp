@@ -42,7 +42,11 @@ template <typename T>
 void launch_dot_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 
 void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial);
-void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, const double* partial);
+// (mixed: the graphs go on to their first refinement step)
+void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, int mixed,
+                         const double* partial);
+// graphs waiting for FP64-product inner solves may start them
+void launch_release_wait64(hipStream_t s, const GraphDev* G, int count);
 void launch_zero_load(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int32_t* gtile, int count, int inner,
                    const double* partial);
