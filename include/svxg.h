@@ -239,6 +239,70 @@ int svxg_problem_last_timing(const svxg_problem* p, double* solve_ms, double* se
 int svxg_ctx_set_profiling(svxg_ctx* ctx, int32_t on);
 int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 
+/* ---- serving: a per-tick request loop with continuous batching ------------------------------
+   Replaces the reference's per-tick structure loop (step_structures, world.cpp:986-1129: the
+   structures of a tick solved one after another within a block-operation budget) and its piece
+   checks (world_pieces.cpp:1452-1463). A server owns two worker threads and two streams of the
+   context: one assembles what arrives (bond removals, assembly and multigrid setup, loads) while
+   the other steps the running batch; requests join the running batch, and finished ones leave it,
+   at its next look (every few PCG steps), so work that arrives while others are on the GPU
+   overlaps with them. Every graph does the same arithmetic as alone: a result is bitwise the same
+   whatever else is in the batch. Admission is by the context's HBM budget: a request whose problem
+   must be assembled waits until its estimated bytes fit. A problem submitted to a server belongs
+   to it until its request completes: no other call may use it meanwhile. */
+typedef struct svxg_server svxg_server;
+
+typedef struct svxg_server_options {
+  int32_t precision;      /* SVXG_MIXED (default) or SVXG_FP64 */
+  int32_t max_active;     /* graphs in the running batch at most (4096) */
+  double rtol;            /* relative residual of every solve (1e-8) */
+  int32_t maxit;          /* iterations of a solve at most (5000) */
+  int32_t amg_min_nodes;  /* assembly option of the problems a server assembles (160) */
+  int32_t window;         /* PCG steps between looks at most (8) */
+  int32_t pad_;
+} svxg_server_options;
+
+typedef struct svxg_serve_done {
+  uint64_t ticket;
+  svxg_problem* problem;
+  svxg_pcg_result result;
+  int32_t status;         /* SVXG_OK, or what kept the request from solving (SVXG_SINGULAR, ...) */
+  int32_t assembled;      /* 1: this request assembled its problem (a first solve or a re-setup) */
+  double t_submit_ms;     /* on the server's clock (svxg_server_now_ms) */
+  double t_ready_ms;      /* setup and loads done */
+  double t_admit_ms;      /* joined the running batch */
+  double t_done_ms;       /* solution written */
+} svxg_serve_done;
+
+typedef struct svxg_server_stats {
+  int64_t submitted, completed, failed;
+  int64_t steps;          /* PCG steps of the running batch */
+  int64_t looks;          /* synchronisations of the running batch (admissions, completions) */
+  int64_t assembled;      /* problems assembled */
+  int64_t max_active;     /* largest running batch */
+  double step_ms;         /* host time spent stepping and looking */
+  double setup_ms;        /* host time spent in setups and loads */
+} svxg_server_stats;
+
+void svxg_server_default_options(svxg_server_options* out);
+int svxg_server_create(svxg_ctx* ctx, const svxg_server_options* opt, svxg_server** out);
+/* Waits for every submitted request to complete, then stops the workers. */
+int svxg_server_destroy(svxg_server* srv);
+/* One solve of problem p: its bonds `remove` (count nremove, may be 0) leave K first (in place, as
+   svxg_problem_remove_bonds; an unassembled problem is assembled without them), then K u = f is
+   solved to the server's rtol. f: 6 per node (all nodes), or null to keep the problem's resident
+   load. warm: start from the problem's resident solution (else from 0). u: where the solution
+   (6 per node, all nodes) is written when the request completes, or null (it stays resident).
+   f is copied; u must stay valid until completion. Returns SVXG_BAD_INPUT if p is in flight. */
+int svxg_server_submit(svxg_server* srv, svxg_problem* p, const double* f, double* u, int32_t warm,
+                       const int32_t* remove, int32_t nremove, uint64_t* ticket);
+/* Completed requests, oldest first (up to cap); waits up to wait_ms for at least one. */
+int svxg_server_poll(svxg_server* srv, svxg_serve_done* out, int32_t cap, int32_t* count, int32_t wait_ms);
+/* Destroys p (its device memory returns to the budget) once no request of it is in flight. */
+int svxg_server_release(svxg_server* srv, svxg_problem* p);
+double svxg_server_now_ms(const svxg_server* srv);
+int svxg_server_stats_get(svxg_server* srv, svxg_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

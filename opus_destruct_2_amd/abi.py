@@ -86,6 +86,27 @@ class ProblemInfo(ctypes.Structure):
         }
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("max_active", ctypes.c_int32), ("rtol", ctypes.c_double),
+                ("maxit", ctypes.c_int32), ("amg_min_nodes", ctypes.c_int32), ("window", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+class ServeDone(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("problem", ctypes.c_void_p), ("result", PcgResult),
+                ("status", ctypes.c_int32), ("assembled", ctypes.c_int32), ("t_submit_ms", ctypes.c_double),
+                ("t_ready_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("steps", ctypes.c_int64), ("looks", ctypes.c_int64), ("assembled", ctypes.c_int64),
+                ("max_active", ctypes.c_int64), ("step_ms", ctypes.c_double), ("setup_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def ptr(a):
     """The data pointer of a numpy array (None for None)."""
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)

@@ -48,7 +48,7 @@ int Arena::alloc(size_t bytes, void** out, size_t* held) {
   *out = nullptr;
   if (bytes >= kBig) {
     const size_t sz = (bytes + 4095) & ~size_t(4095);
-    if (in_use_ + int64_t(sz) > budget_) {
+    if (in_use_.load() + int64_t(sz) > budget_) {
       set_error("HBM budget exceeded");
       return SVXG_OOM_BUDGET;
     }
@@ -66,7 +66,7 @@ int Arena::alloc(size_t bytes, void** out, size_t* held) {
   }
   const int c = size_class(bytes);
   const size_t sz = size_t(256) << c;
-  if (in_use_ + int64_t(sz) > budget_) {
+  if (in_use_.load() + int64_t(sz) > budget_) {
     set_error("HBM budget exceeded");
     return SVXG_OOM_BUDGET;
   }
@@ -135,17 +135,29 @@ static int upload_into(const DevBuf& d, const std::vector<T>& h, hipStream_t s) 
   return SVXG_OK;
 }
 
-// the context's pinned host staging and device staging, at least `bytes` each
-static int staging(svxg_ctx* c, size_t bytes) {
-  if (bytes > c->pinned_bytes) {
-    if (c->pinned) (void)hipHostFree(c->pinned);
-    c->pinned = nullptr;
-    c->pinned_bytes = 0;
-    const size_t want = std::max(bytes, size_t(64) << 20);
-    SVXG_HIP(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
-    c->pinned_bytes = want;
+// the staging lane of stream s: pinned host and device staging, at least `bytes` each
+static int staging(svxg_ctx* c, hipStream_t s, size_t bytes, svxg_ctx::Lane** out) {
+  svxg_ctx::Lane* L = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c->lanes_mu);
+    for (auto& l : c->lanes)
+      if (l->s == s) L = l.get();
+    if (!L) {
+      c->lanes.push_back(std::make_unique<svxg_ctx::Lane>());
+      L = c->lanes.back().get();
+      L->s = s;
+    }
   }
-  if (bytes > c->dstage.bytes()) SVXG_TRY(c->dstage.make(c->arena.get(), std::max(bytes, size_t(64) << 20)));
+  if (bytes > L->pinned_bytes) {
+    if (L->pinned) (void)hipHostFree(L->pinned);
+    L->pinned = nullptr;
+    L->pinned_bytes = 0;
+    const size_t want = std::max(bytes, size_t(16) << 20);
+    SVXG_HIP(hipHostMalloc(&L->pinned, want, hipHostMallocDefault));
+    L->pinned_bytes = want;
+  }
+  if (bytes > L->dstage.bytes()) SVXG_TRY(L->dstage.make(c->arena.get(), std::max(bytes, size_t(16) << 20)));
+  *out = L;
   return SVXG_OK;
 }
 
@@ -189,9 +201,10 @@ class Stager {
     size_t data = 0;
     for (const Job& j : jobs_) data += round256(j.bytes);
     const size_t table = round256(jobs_.size() * sizeof(BulkSeg));
-    SVXG_TRY(staging(c_, data + table));
-    char* hp = static_cast<char*>(c_->pinned);
-    char* dp = c_->dstage.as<char>();
+    svxg_ctx::Lane* L = nullptr;
+    SVXG_TRY(staging(c_, s_, data + table, &L));
+    char* hp = static_cast<char*>(L->pinned);
+    char* dp = L->dstage.as<char>();
     BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
     std::vector<size_t> offs(jobs_.size());
     size_t off = 0;
@@ -233,9 +246,10 @@ class Gatherer {
     size_t data = 0;
     for (const Job& j : jobs_) data += round256(j.bytes);
     const size_t table = round256(jobs_.size() * sizeof(BulkSeg));
-    SVXG_TRY(staging(c_, data + table));
-    char* hp = static_cast<char*>(c_->pinned);
-    char* dp = c_->dstage.as<char>();
+    svxg_ctx::Lane* L = nullptr;
+    SVXG_TRY(staging(c_, s_, data + table, &L));
+    char* hp = static_cast<char*>(L->pinned);
+    char* dp = L->dstage.as<char>();
     BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
     size_t off = 0;
     for (size_t k = 0; k < jobs_.size(); ++k) {
@@ -288,12 +302,13 @@ class BulkOps {
       launch_to_f32(s, static_cast<const double*>(g.src), static_cast<float*>(g.dst), g.count);
     if (!zero_.empty() || !f32_.empty()) {
       const size_t bytes = (zero_.size() + f32_.size()) * sizeof(BulkSeg);
-      SVXG_TRY(staging(c_, bytes));
-      std::memcpy(c_->pinned, zero_.data(), zero_.size() * sizeof(BulkSeg));
-      std::memcpy(static_cast<char*>(c_->pinned) + zero_.size() * sizeof(BulkSeg), f32_.data(),
+      svxg_ctx::Lane* L = nullptr;
+      SVXG_TRY(staging(c_, s, bytes, &L));
+      std::memcpy(L->pinned, zero_.data(), zero_.size() * sizeof(BulkSeg));
+      std::memcpy(static_cast<char*>(L->pinned) + zero_.size() * sizeof(BulkSeg), f32_.data(),
                   f32_.size() * sizeof(BulkSeg));
-      SVXG_HIP(hipMemcpyAsync(c_->dstage.as<void>(), c_->pinned, bytes, hipMemcpyHostToDevice, s));
-      const BulkSeg* d = c_->dstage.as<BulkSeg>();
+      SVXG_HIP(hipMemcpyAsync(L->dstage.as<void>(), L->pinned, bytes, hipMemcpyHostToDevice, s));
+      const BulkSeg* d = L->dstage.as<BulkSeg>();
       launch_zero_many(s, d, int(zero_.size()));
       launch_f32_many(s, d + zero_.size(), int(f32_.size()));
     }
@@ -637,26 +652,25 @@ constexpr int kPollFirst = 4;
 constexpr int kPollMax = 16;
 
 struct PollPlan {
-  std::vector<double> rn;
-  std::vector<int32_t> it;
+  struct Seen {
+    double rn = 0.0;
+    int32_t it = -1;
+  };
+  std::unordered_map<const svxg_problem*, Seen> seen;  // (by problem: a batch may be rebuilt between looks)
   // iterations to queue before the next look (states just read)
   int next(const Batch& B, bool inner) {
-    const size_t G = B.hstate.size();
-    if (rn.size() != G) {
-      rn.assign(G, 0.0);
-      it.assign(G, -1);
-    }
     int want = 1;
-    for (size_t g = 0; g < G; ++g) {
+    for (size_t g = 0; g < B.hstate.size(); ++g) {
       const PcgState& st = B.hstate[g];
+      Seen& sn = seen[B.probs[g]];
       if (inner && st.state == ST_RUNNING && st.phase == PH_WAIT64) {
-        it[g] = -1;  // (released at the next look)
+        sn.it = -1;  // (released at the next look)
         continue;
       }
       if (inner && st.state == ST_RUNNING && st.inner_state != ST_RUNNING) {
         // (between inner solves: a refinement residual, then a new inner solve)
         want = std::max(want, kPollFirst);
-        it[g] = -1;
+        sn.it = -1;
         continue;
       }
       const bool run = st.state == ST_RUNNING && (!inner || st.inner_state == ST_RUNNING);
@@ -664,27 +678,43 @@ struct PollPlan {
       const double target = inner ? st.inner_target : st.target;
       const int32_t k = inner ? st.inner_iters : st.iters;
       int need = kPollFirst;
-      if (run && it[g] >= 0 && k > it[g] && r > 0 && rn[g] > r && target > 0) {
-        const double rate = std::pow(r / rn[g], 1.0 / double(k - it[g]));
+      if (run && sn.it >= 0 && k > sn.it && r > 0 && sn.rn > r && target > 0) {
+        const double rate = std::pow(r / sn.rn, 1.0 / double(k - sn.it));
         need = rate < 1.0 ? int(std::ceil(std::log(target / r) / std::log(rate))) : kPollMax;
       }
       if (run) want = std::max(want, std::min(kPollMax, std::max(1, need)));
-      rn[g] = r;
-      it[g] = run ? k : -1;
+      sn.rn = r;
+      sn.it = run ? k : -1;
     }
     return want;
   }
+  void forget(const svxg_problem* P) { seen.erase(P); }
 };
 
 static void drop_self_batch_ptrs(svxg_problem* P);
 
-// the FP64 PCG iterations of StressProblem::iterate (stress.cpp:403-464) over a batch (after
-// run_begin), each graph with its own scalars and stopping point
-static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
+// one FP64 PCG iteration of StressProblem::iterate (stress.cpp:403-464) of every iterating graph,
+// each with its own scalars and stopping point
+static void fp64_step(Batch& B, hipStream_t s, int degree) {
   const GraphDev* G = B.dG.as<GraphDev>();
   const Tile* t0 = B.t0.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
+  {
+    ProfScope ps(B, 0, s, SERVE_OUTER);
+    launch_pcg_spmv<double>(s, G, t0, B.nt0, SERVE_OUTER, part);
+  }
+  launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
+  launch_axpy_rr<double>(s, G, t0, B.nt0, SERVE_OUTER, part);
+  launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_OUTER, part);
+  vcycle<double>(B, s, SERVE_OUTER, degree);
+  launch_dot_rz<double>(s, G, t0, B.nt0, SERVE_OUTER, part);
+  launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_OUTER, part);
+  launch_update_p<double>(s, G, t0, B.nt0, SERVE_OUTER);
+}
+
+// the FP64 PCG over a batch (after run_begin)
+static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
   const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
   PollPlan plan;
   for (int left = 0;; --left) {
@@ -693,17 +723,7 @@ static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
       if (!any_running(B, false)) break;
       left = plan.next(B, false);
     }
-    {
-      ProfScope ps(B, 0, s, SERVE_OUTER);
-      launch_pcg_spmv<double>(s, G, t0, B.nt0, 0, part);
-    }
-    launch_finish(s, FinishKind::Alpha, G, gt, B.G, 0, part);
-    launch_axpy_rr<double>(s, G, t0, B.nt0, 0, part);
-    launch_finish(s, FinishKind::Rn, G, gt, B.G, 0, part);
-    vcycle<double>(B, s, 0, degree);
-    launch_dot_rz<double>(s, G, t0, B.nt0, 0, part);
-    launch_finish(s, FinishKind::Beta, G, gt, B.G, 0, part);
-    launch_update_p<double>(s, G, t0, B.nt0, 0);
+    fp64_step(B, s, degree);
   }
   SVXG_HIP(hipGetLastError());
   return SVXG_OK;
@@ -711,7 +731,8 @@ static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
 
 // FP64 iterative refinement around an FP32 PCG (FP32 K products and V-cycle, FP64 dots): each
 // outer step recomputes r = b - K x in FP64, solves K d = r to max(eta |r|, target / 2) in FP32 and
-// adds d to x
+// adds d to x.
+//
 // One step of every graph's mixed-precision solve, whatever its phase: a refinement step's FP64
 // residual and check (PH_OUTER), an inner solve's start (PH_START) or iteration (PH_INNER), the
 // correction x += d (PH_APPLY). The launches come in an order in which a graph moves on at most one
@@ -795,6 +816,7 @@ static int recheck_residual(Batch& B, hipStream_t s) {
 // the rest of the host side (plan, setup, ABI) is in host_setup.hip and host_abi.hip
 #include "host_setup.inc"
 #include "host_abi.inc"
+#include "host_serve.inc"
 
 namespace svxg {
 // (a kept batch-of-one holds device pointers: rebuilt when K's arrays change)

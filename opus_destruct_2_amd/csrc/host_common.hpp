@@ -2,6 +2,7 @@
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -39,7 +40,7 @@ class Arena {
   ~Arena();
   int alloc(size_t bytes, void** out, size_t* held);
   void release(void* p, size_t held);
-  int64_t in_use() const { return in_use_; }
+  int64_t in_use() const { return in_use_.load(std::memory_order_relaxed); }
   int64_t budget() const { return budget_; }
 
  private:
@@ -47,7 +48,7 @@ class Arena {
   static constexpr size_t kBig = size_t(64) << 20;
   int device_;
   int64_t budget_;
-  int64_t in_use_ = 0;
+  std::atomic<int64_t> in_use_{0};
   std::mutex mu_;
   std::vector<void*> slabs_, big_;
   char* bump_ = nullptr;
@@ -125,16 +126,21 @@ struct svxg_ctx {
   std::mutex mu;  // batches and setup on the context stream, one at a time
   // kernel timing: event pairs around the launches of the profiled kernels (kinds 0..3: the outer
   // K product FP64 / FP32, a level-0 smoothing step FP64 / FP32)
-  // pinned host staging for batched uploads and readbacks (grown on demand)
-  void* pinned = nullptr;
-  size_t pinned_bytes = 0;
+  // staging of batched uploads and readbacks, one lane per stream that uses it (grown on demand)
+  struct Lane {
+    hipStream_t s = nullptr;
+    void* pinned = nullptr;  // host, pinned
+    size_t pinned_bytes = 0;
+    svxg::DevBuf dstage;     // device
+  };
+  std::mutex lanes_mu;
+  std::vector<std::unique_ptr<Lane>> lanes;
   bool profiling = false;
   struct Prof {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, spare;
     int64_t launches = 0;
     double ms = 0.0;
   } prof[4];
-  svxg::DevBuf dstage;    // device staging of batched uploads and readbacks (grown on demand)
   svxg::DevBuf prof_acc;  // 4 doubles on the device: the bytes of the live graphs of each kind's launches
 };
 
@@ -157,6 +163,7 @@ struct svxg_problem {
   svxg::DevBuf x, r, z, p, q, b, ix, ir, iz, ip, iq;
   svxg::DevBuf u_all;  // 6 per node: the last solve's displacement (device-resident)
   bool have_solution = false;
+  bool have_load = false;  // b holds a load (a solve or a request gave one)
   // a solve spread over calls (begin / iterate)
   bool run_active = false;
   int32_t run_iters = 0;

@@ -325,12 +325,13 @@ __global__ void k_identity_rows(const AssemblyDev* __restrict__ J, const Tile* _
 }
 
 // bonds leave K in list order (one workgroup a graph): each one's products subtracted in place
-__global__ void __launch_bounds__(kWG) k_remove_bonds(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ ids,
-                                                      int count) {
-  const AssemblyDev& a = J[0];
+__global__ void __launch_bounds__(64) k_remove_bonds(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ ids,
+                                                    const int32_t* __restrict__ offs, int njobs) {
+  if (int(blockIdx.x) >= njobs) return;
+  const AssemblyDev& a = J[blockIdx.x];
   __shared__ double sD[36], sA[36], sB[36];
   const int e = threadIdx.x;
-  for (int t = 0; t < count; ++t) {
+  for (int t = offs[blockIdx.x]; t < offs[blockIdx.x + 1]; ++t) {
     const int bi = ids[t];
     const int32_t* sl = a.bond_slot + 4 * size_t(bi);
     if (e < 36) bond_blocks_lds(a.bonds[bi], a.nodec, a.mats, a.nmats, e, sD, sA, sB);
@@ -1299,6 +1300,16 @@ __global__ void k_scatter_u(const double* __restrict__ x, const int32_t* __restr
   u[t] = d >= 0 ? x[6 * size_t(d) + q] : 0.0;
 }
 
+__global__ void __launch_bounds__(kWG) k_scatter_many(const ScatterJob* __restrict__ jobs, int n) {
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const ScatterJob j = jobs[k];
+    for (int t = threadIdx.x; t < 6 * j.nnodes; t += kWG) {
+      const int d = j.dof[t / 6];
+      j.u[t] = d >= 0 ? j.x[6 * size_t(d) + t % 6] : 0.0;
+    }
+  }
+}
+
 // ---- host launchers -----------------------------------------------------------------------------------
 
 void launch_bond_products(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
@@ -1316,8 +1327,8 @@ void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count) {
 void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_identity_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
-void launch_remove_bonds(hipStream_t s, const AssemblyDev* job, const int32_t* ids, int count) {
-  if (count > 0) hipLaunchKernelGGL(k_remove_bonds, dim3(1), dim3(64), 0, s, job, ids, count);
+void launch_remove_bonds(hipStream_t s, const AssemblyDev* jobs, const int32_t* ids, const int32_t* offs, int njobs) {
+  if (njobs > 0) hipLaunchKernelGGL(k_remove_bonds, dim3(njobs), dim3(64), 0, s, jobs, ids, offs, njobs);
 }
 void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* rows, int count) {
   if (count > 0) hipLaunchKernelGGL(k_retire_rows, dim3((36 * count + 255) / 256), dim3(256), 0, s, job, rows, count);
@@ -1444,6 +1455,9 @@ void launch_copy_many(hipStream_t s, const BulkSeg* segs, int n) {
 }
 void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_bond_checks, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+}
+void launch_scatter_many(hipStream_t s, const ScatterJob* jobs, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, jobs, n);
 }
 void launch_scatter_u(hipStream_t s, const double* x, const int32_t* dof, double* u, int nnodes) {
   if (nnodes > 0) hipLaunchKernelGGL(k_scatter_u, dim3((6 * nnodes + 255) / 256), dim3(256), 0, s, x, dof, u, nnodes);

@@ -92,7 +92,8 @@ void launch_diag_trace(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles
 void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count);
 void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles);
 // in-place removal of bonds from K (stress.cpp:260-284): the listed bonds' products, subtracted
-void launch_remove_bonds(hipStream_t s, const AssemblyDev* job, const int32_t* ids, int count);
+// (one workgroup a job: job j's bonds are ids[offs[j] .. offs[j + 1]))
+void launch_remove_bonds(hipStream_t s, const AssemblyDev* jobs, const int32_t* ids, const int32_t* offs, int njobs);
 // rows of retired nodes become id_scale I (stress.cpp:299-308)
 void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* rows, int count);
 
@@ -256,5 +257,14 @@ struct CheckDev {
 void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, int ntiles);
 // u over all nodes (6 per node) from the free-DOF solution
 void launch_scatter_u(hipStream_t s, const double* x, const int32_t* dof, double* u, int nnodes);
+// u (all nodes) from x (free DOFs) for many problems, one workgroup a problem at a time
+struct ScatterJob {
+  const double* x;
+  const int32_t* dof;
+  double* u;
+  int32_t nnodes;
+  int32_t pad_;
+};
+void launch_scatter_many(hipStream_t s, const ScatterJob* jobs, int n);
 
 }  // namespace svxg

@@ -1,0 +1,98 @@
+"""The serving loop over the device library's C ABI (svxg_server_*, include/svxg.h).
+
+A Server takes solve requests for StressProblems at any time (submit), from any thread; its two
+native workers assemble what arrives and step one running batch into which requests are admitted,
+and from which finished ones leave, every few PCG steps (continuous batching). Completed requests
+come back through poll() / wait() with their results and timestamps on the server's clock.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from .abi import SVXG_MIXED, ServeDone, ServerOptions, ServerStats, ptr
+from .stress import SvxgError, _check, default_context, lib
+
+
+class Server:
+    def __init__(self, ctx=None, precision=SVXG_MIXED, max_active=4096, rtol=1e-8, maxit=5000,
+                 amg_min_nodes=160, window=8):
+        self.ctx = ctx or default_context()
+        o = ServerOptions()
+        lib().svxg_server_default_options(ctypes.byref(o))
+        o.precision, o.max_active, o.rtol, o.maxit = precision, max_active, rtol, maxit
+        o.amg_min_nodes, o.window = amg_min_nodes, window
+        h = ctypes.c_void_p()
+        _check(lib().svxg_server_create(self.ctx.h, ctypes.byref(o), ctypes.byref(h)), "svxg_server_create")
+        self.h = h
+        self._mu = threading.Lock()
+        self._inflight = {}  # ticket -> (problem, u array or None)
+        self._done = {}
+
+    def now_ms(self):
+        return lib().svxg_server_now_ms(self.h)
+
+    def submit(self, problem, f=None, warm=False, remove=None, want_u=True):
+        """Queue one solve of `problem` (its bonds `remove` leave K first). Returns the ticket."""
+        n = problem.n
+        fa = None if f is None else np.ascontiguousarray(f, np.float64).reshape(-1)
+        if fa is not None and fa.size != 6 * n:
+            raise ValueError("f: 6 values per node")
+        u = np.zeros(6 * n) if want_u else None
+        rm = None if remove is None or len(remove) == 0 else np.ascontiguousarray(remove, np.int32)
+        t = ctypes.c_uint64()
+        with self._mu:
+            _check(lib().svxg_server_submit(self.h, problem.h, ptr(fa), ptr(u), int(bool(warm)), ptr(rm),
+                                            0 if rm is None else int(rm.size), ctypes.byref(t)), "server submit")
+            self._inflight[t.value] = (problem, u)
+        return t.value
+
+    def poll(self, wait_ms=0, cap=4096):
+        """Completed requests since the last poll: {ticket: record}."""
+        buf = (ServeDone * cap)()
+        n = ctypes.c_int32()
+        _check(lib().svxg_server_poll(self.h, buf, cap, ctypes.byref(n), int(wait_ms)), "server poll")
+        out = {}
+        with self._mu:
+            for k in range(n.value):
+                d = buf[k]
+                problem, u = self._inflight.pop(d.ticket)
+                out[d.ticket] = {
+                    "problem": problem, "u": u, "status": d.status, "assembled": bool(d.assembled),
+                    "result": d.result.as_dict(), "t_submit_ms": d.t_submit_ms, "t_ready_ms": d.t_ready_ms,
+                    "t_admit_ms": d.t_admit_ms, "t_done_ms": d.t_done_ms,
+                }
+        return out
+
+    def wait(self, tickets, timeout_ms=120000):
+        """The records of these tickets (others completed meanwhile are kept for later calls)."""
+        want = set(tickets)
+        got = {t: self._done.pop(t) for t in list(want) if t in self._done}
+        t0 = self.now_ms()
+        while len(got) < len(want):
+            for t, rec in self.poll(wait_ms=50).items():
+                (got if t in want else self._done)[t] = rec
+            if self.now_ms() - t0 > timeout_ms:
+                raise SvxgError(f"server: {len(want) - len(got)} requests still in flight after {timeout_ms} ms")
+        return got
+
+    def release(self, problem):
+        """Destroy `problem` once its request (if any) completes."""
+        _check(lib().svxg_server_release(self.h, problem.h), "server release")
+        problem.h = None
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().svxg_server_stats_get(self.h, ctypes.byref(s)), "server stats")
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().svxg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

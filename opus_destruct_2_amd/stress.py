@@ -65,6 +65,14 @@ def lib():
             "svxg_ctx_set_profiling": ([vp, i32], ctypes.c_int),
             "svxg_ctx_kernel_stats": ([vp, i32, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)],
                                       ctypes.c_int),
+            "svxg_server_default_options": ([vp], None),
+            "svxg_server_create": ([vp, vp, ctypes.POINTER(vp)], ctypes.c_int),
+            "svxg_server_destroy": ([vp], ctypes.c_int),
+            "svxg_server_submit": ([vp, vp, vp, vp, i32, vp, i32, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+            "svxg_server_poll": ([vp, vp, i32, ctypes.POINTER(i32), i32], ctypes.c_int),
+            "svxg_server_release": ([vp, vp], ctypes.c_int),
+            "svxg_server_now_ms": ([vp], f64),
+            "svxg_server_stats_get": ([vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

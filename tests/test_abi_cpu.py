@@ -5,6 +5,7 @@ import ctypes
 import hashlib
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -18,7 +19,7 @@ LIB = os.path.join(ROOT, "opus_destruct_2_amd", "libsvxg.so")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|void|int64_t|const char\*)\s+(svxg_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|int64_t|double|const char\*)\s+(svxg_\w+)\(", text, re.M)))
 
 
 def test_header_declares_the_entry_points():
@@ -42,6 +43,29 @@ def test_struct_sizes_match_the_header():
     assert abi.MATERIAL.itemsize == 64
     assert ctypes.sizeof(abi.StressOptions) == 8 + 4 + 4 + ctypes.sizeof(abi.AmgOptions)
     assert ctypes.sizeof(abi.PcgResult) == 24
+
+
+def test_struct_layouts_match_a_c_compiler(tmp_path):
+    """Every struct of include/svxg.h: size and field offsets as gcc lays them out = the ctypes mirror."""
+    mirrors = {"svxg_amg_options": abi.AmgOptions, "svxg_stress_options": abi.StressOptions,
+               "svxg_pcg_result": abi.PcgResult, "svxg_problem_info": abi.ProblemInfo,
+               "svxg_server_options": abi.ServerOptions, "svxg_serve_done": abi.ServeDone,
+               "svxg_server_stats": abi.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svxg.h"', "int main(void) {"]
+    for c, py in mirrors.items():
+        lines.append(f'  printf("{c} %zu\\n", sizeof({c}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{c}.{f} %zu\\n", offsetof({c}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for c, py in mirrors.items():
+        assert int(got[c]) == ctypes.sizeof(py), c
+        for f, _ in py._fields_:
+            assert int(got[f"{c}.{f}"]) == getattr(py, f).offset, (c, f)
 
 
 def test_default_options_without_a_device():

@@ -1,0 +1,141 @@
+"""The serving loop (svxg_server_*): requests that arrive while others run are admitted into the
+running batch; every result is bitwise the one the same request gets alone, and matches the oracle.
+Incremental requests (bond removals + warm start) equal the same steps through the problem API."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import CpuProblem, reference_options
+from opus_destruct_2_amd import gen, serve, stress
+from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED, SVXG_OK
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [16, 40, 90, 159, 161, 300, 700, 1500, 3000]
+
+
+def graphs():
+    return [gen.piece(s, pinned=bool(k % 3 == 0), seed=300 + k) for k, s in enumerate(SIZES)]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+def problems(gs, mats, ctx):
+    return [stress.StressProblem(g.nodes, g.bonds, mats, ctx) for g in gs]
+
+
+@pytest.mark.parametrize("precision", [SVXG_MIXED, SVXG_FP64], ids=["mixed", "fp64"])
+def test_alone_and_in_the_mix_bitwise(precision, mats, ctx):
+    gs = graphs()
+    # alone: one request at a time
+    srv = serve.Server(ctx, precision=precision, rtol=1e-9)
+    alone = []
+    for p, g in zip(problems(gs, mats, ctx), gs):
+        t = srv.submit(p, g.f)
+        rec = srv.wait([t])[t]
+        assert rec["status"] == SVXG_OK and rec["result"]["converged"], rec
+        alone.append(rec)
+    srv.close()
+    # the mix: staggered arrivals into a running batch (small windows: many admissions mid-solve)
+    srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=2)
+    ps = problems(gs, mats, ctx)
+    order = [8, 0, 5, 2, 7, 1, 3, 6, 4]
+    tickets = {}
+    for k, i in enumerate(order):
+        tickets[srv.submit(ps[i], gs[i].f)] = i
+        if k % 3 == 2:
+            time.sleep(0.002)
+    got = srv.wait(list(tickets))
+    st = srv.stats()
+    srv.close()
+    for t, i in tickets.items():
+        rec = got[t]
+        assert rec["result"] == alone[i]["result"], (SIZES[i], rec["result"], alone[i]["result"])
+        assert np.array_equal(bits(rec["u"]), bits(alone[i]["u"])), SIZES[i]
+        assert rec["t_submit_ms"] <= rec["t_ready_ms"] <= rec["t_admit_ms"] <= rec["t_done_ms"]
+    assert st["completed"] == len(gs) and st["failed"] == 0
+
+
+def test_served_solutions_match_the_oracle(mats, ctx):
+    gs = graphs()[:7]
+    srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-10, amg_min_nodes=0)
+    ps = problems(gs, mats, ctx)
+    tickets = {srv.submit(p, g.f): k for k, (p, g) in enumerate(zip(ps, gs))}
+    got = srv.wait(list(tickets))
+    srv.close()
+    for t, k in tickets.items():
+        o = CpuProblem(gs[k].nodes, gs[k].bonds, mats)
+        assert o.assemble(reference_options(amg_min_nodes=0))
+        uo, ro = o.solve(gs[k].f, rtol=1e-10, maxit=20000)
+        assert ro["converged"]
+        scale = np.abs(uo).max()
+        assert np.abs(got[t]["u"] - uo).max() <= 1e-5 * scale, SIZES[k]
+
+
+@pytest.mark.parametrize("precision", [SVXG_MIXED, SVXG_FP64], ids=["mixed", "fp64"])
+def test_removals_and_warm_starts_equal_the_problem_api(precision, mats, ctx):
+    """A served request with bond removals and a warm start = remove_bonds + solve(warm) alone."""
+    gs = graphs()[3:8]
+    rng = np.random.default_rng(5)
+    cuts = [sorted(rng.choice(len(g.bonds), size=max(1, len(g.bonds) // 50), replace=False).tolist()) for g in gs]
+    opts = stress.default_options(precision=precision)
+    # the problem API
+    ref = []
+    for g, cut in zip(gs, cuts):
+        p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
+        assert p.assemble(opts)
+        u0, r0 = p.solve(g.f, rtol=1e-9, maxit=5000)
+        p.remove_bonds(cut)
+        u1, r1 = p.solve(g.f, u=u0, rtol=1e-9, maxit=5000, warm=True)
+        ref.append((u0, r0, u1, r1))
+        p.close()
+    # served: a first request, then removals + warm start with the resident load
+    srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=3)
+    ps = problems(gs, mats, ctx)
+    first = {srv.submit(p, g.f): k for k, (p, g) in enumerate(zip(ps, gs))}
+    got = srv.wait(list(first))
+    for t, k in first.items():
+        assert np.array_equal(bits(got[t]["u"]), bits(ref[k][0])), k
+    second = {srv.submit(p, None, warm=True, remove=cut): k for k, (p, cut) in enumerate(zip(ps, cuts))}
+    got = srv.wait(list(second))
+    srv.close()
+    for t, k in second.items():
+        assert got[t]["result"]["iters"] == ref[k][3]["iters"], (k, got[t]["result"], ref[k][3])
+        assert np.array_equal(bits(got[t]["u"]), bits(ref[k][2])), k
+
+
+def test_admission_waits_for_the_hbm_budget(mats):
+    """A budget that holds a few problems at a time: later requests wait for releases, all complete."""
+    gs = [gen.piece(400, pinned=False, seed=900 + k) for k in range(12)]
+    small = stress.Context(0, 160 << 20)
+    srv = serve.Server(small, precision=SVXG_MIXED, rtol=1e-8)
+    ps = problems(gs, mats, small)
+    pending = {srv.submit(p, g.f): p for p, g in zip(ps, gs)}
+    done = 0
+    t0 = time.time()
+    while pending and time.time() - t0 < 120:
+        for t, rec in srv.poll(wait_ms=50).items():
+            assert rec["status"] == SVXG_OK and rec["result"]["converged"], rec
+            srv.release(pending.pop(t))
+            done += 1
+    st = srv.stats()
+    srv.close()
+    small.close()
+    assert done == len(gs) and not pending
+    assert st["failed"] == 0
+
+
+def test_release_in_flight_and_busy_rejection(mats, ctx):
+    g = gen.piece(2000, pinned=True, seed=77)
+    srv = serve.Server(ctx, precision=SVXG_MIXED)
+    p = stress.StressProblem(g.nodes, g.bonds, mats, ctx)
+    t = srv.submit(p, g.f)
+    with pytest.raises(stress.SvxgError):
+        srv.submit(p, g.f)  # (in flight)
+    srv.release(p)  # destroyed when its request completes
+    rec = srv.wait([t])[t]
+    assert rec["result"]["converged"]
+    srv.close()
