@@ -170,6 +170,146 @@ def run_batch(mats, ctx, count, steps, sample_plain):
             "batched_vs_plain": round(batched / plain, 2)}
 
 
+class _Piece:
+    """A bare problem handle for the serving driver (created straight through the C ABI)."""
+    __slots__ = ("h", "n", "g", "broken")
+
+    def __init__(self, g, mats, ctx):
+        import ctypes
+        from opus_destruct_2_amd import stress
+        h = ctypes.c_void_p()
+        st = stress.lib().svxg_problem_create(ctx.h, g.nodes.ctypes.data, g.n, g.bonds.ctypes.data, len(g.bonds),
+                                              mats.ctypes.data, len(mats), ctypes.byref(h))
+        assert st == 0
+        self.h, self.n, self.g = h, g.n, g
+
+
+def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11):
+    """Serving stream: 60 Hz ticks; each tick, Poisson(lam) new structures (sizes as the batched
+    leg's, half on supports, half pinned) arrive and are solved cold (setup + solve), and 10 % of the
+    resident structures that are idle lose 1-5 % of their bonds and are solved again, warm, on their
+    resident load; a structure leaves `life` ticks after it arrived (its device memory returns to
+    the budget). Cuts never leave a mechanism (each free node keeps a path to a support or its pin:
+    a detached part would become a piece of its own). Mixed precision to 1e-8 through svxg_server (continuous batching, admission by the
+    context's HBM budget). A tick's latency: from its first submission to its last completion."""
+    from opus_destruct_2_amd import gen, serve
+    from opus_destruct_2_amd.abi import SVXG_MIXED
+    rng = np.random.default_rng(seed)
+    sizes = gen.piece_sizes(pool_size, seed=seed)
+    pool = [gen.piece(int(n), pinned=bool(k & 1), seed=9000 + k) for k, n in enumerate(sizes)]
+    out = {"tick_ms": 1000.0 / 60.0, "life_ticks": life, "measured_ticks": ticks, "warmup_ticks": warm_ticks,
+           "pool_graphs": pool_size, "per_lambda": []}
+    period = 1.0 / 60.0
+    for lam in lams:
+        srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-8, maxit=5000, amg_min_nodes=160, window=4)
+        residents = []  # [piece, birth tick, cut offset]
+        busy = {}  # ticket -> (tick, piece index or None, dofs)
+        in_flight = set()
+        tick_first, tick_last, tick_dofs, tick_req = {}, {}, {}, {}
+        iters, lat_req, unconverged = [], [], 0
+        stage = {"setup": [], "queue": [], "solve": []}
+        done_dofs, t_meas0, t_meas1 = 0, None, None
+        t0 = time.perf_counter()
+
+        def collect(wait_ms=0):
+            nonlocal done_dofs, unconverged
+            for t, rec in srv.poll(wait_ms=wait_ms).items():
+                k, piece, dofs = busy.pop(t)
+                in_flight.discard(id(piece))
+                assert rec["status"] == 0, rec
+                unconverged += 0 if rec["result"]["converged"] else 1
+                tick_last[k] = max(tick_last.get(k, 0.0), rec["t_done_ms"])
+                if k >= warm_ticks:
+                    done_dofs += dofs
+                    iters.append(rec["result"]["iters"])
+                    lat_req.append(rec["t_done_ms"] - rec["t_submit_ms"])
+                    stage["setup"].append(rec["t_ready_ms"] - rec["t_submit_ms"])
+                    stage["queue"].append(rec["t_admit_ms"] - rec["t_ready_ms"])
+                    stage["solve"].append(rec["t_done_ms"] - rec["t_admit_ms"])
+
+        for k in range(warm_ticks + ticks):
+            target = t0 + k * period
+            while True:
+                collect()
+                left = target - time.perf_counter()
+                if left <= 0:
+                    break
+                time.sleep(min(left, 0.001))
+            if k == warm_ticks:
+                t_meas0 = srv.now_ms()
+            # departures
+            keep = []
+            for r in residents:
+                if k - r[1] >= life and id(r[0]) not in in_flight:
+                    srv.release(r[0])
+                else:
+                    keep.append(r)
+            residents = keep
+            # damage: 10 % of the idle residents lose 1-5 % of their bonds
+            idle = [r for r in residents if id(r[0]) not in in_flight]
+            hit = [idle[i] for i in rng.choice(len(idle), size=len(idle) // 10, replace=False)] if idle else []
+            probs, fs, warm, rem = [], [], [], []
+            for r in hit:
+                piece = r[0]
+                m = len(piece.g.bonds)
+                c = max(1, int(m * rng.uniform(0.01, 0.05)))
+                ids = gen.safe_cuts(piece.g, piece.broken, c, int(rng.integers(1 << 62)))
+                piece.broken[ids] = 1
+                probs.append(piece)
+                fs.append(None)
+                warm.append(True)
+                rem.append(ids)
+            # arrivals
+            for _ in range(rng.poisson(lam)):
+                gi = int(rng.integers(pool_size))
+                piece = _Piece(pool[gi], mats, ctx)
+                piece.broken = np.zeros(len(pool[gi].bonds), np.uint8)
+                residents.append([piece, k, 0])
+                probs.append(piece)
+                fs.append(pool[gi].f)
+                warm.append(False)
+                rem.append(None)
+            if not probs:
+                continue
+            t_sub = srv.now_ms()
+            tk = srv.submit_many(probs, fs, warm, rem, want_u=False)
+            tick_first[k] = t_sub
+            tick_req[k] = len(probs)
+            tick_dofs[k] = sum(6 * p.n for p in probs)
+            for t, piece in zip(tk, probs):
+                busy[t] = (k, piece, 6 * piece.n)
+                in_flight.add(id(piece))
+        t_drain = time.perf_counter()
+        while busy and time.perf_counter() - t_drain < 120.0:
+            collect(wait_ms=20)
+        if busy:
+            raise RuntimeError(f"serving lambda={lam}: {len(busy)} requests still in flight after 120 s")
+        st = srv.stats()
+        for r in residents:
+            srv.release(r[0])
+        srv.close()
+        lat = np.array([tick_last[k] - tick_first[k] for k in range(warm_ticks, warm_ticks + ticks) if k in tick_first])
+        wall_s = (max(tick_last.values()) - t_meas0) / 1e3
+        rec = {"lambda": lam, "p50_tick_ms": round(float(np.percentile(lat, 50)), 2),
+               "p99_tick_ms": round(float(np.percentile(lat, 99)), 2), "max_tick_ms": round(float(lat.max()), 2),
+               "ticks_over_16.7ms": int((lat > 1000.0 / 60.0).sum()),
+               "requests_per_tick": round(float(np.mean([tick_req[k] for k in tick_req if k >= warm_ticks])), 1),
+               "dofs_per_tick": int(np.mean([tick_dofs[k] for k in tick_dofs if k >= warm_ticks])),
+               "value": round(done_dofs / wall_s, 1), "unit": "DOF/s",
+               "p50_request_ms": round(float(np.percentile(lat_req, 50)), 2),
+               "p99_request_ms": round(float(np.percentile(lat_req, 99)), 2),
+               "stage_p50_ms": {k_: round(float(np.median(v)), 2) for k_, v in stage.items()},
+               "stage_p99_ms": {k_: round(float(np.percentile(v, 99)), 2) for k_, v in stage.items()},
+               "iters_median": int(np.median(iters)), "iters_max": int(max(iters)), "unconverged": unconverged,
+               "server": {k_: (round(v, 1) if isinstance(v, float) else v) for k_, v in st.items()}}
+        log(f"  serving lambda={lam}: p50 {rec['p50_tick_ms']} ms, p99 {rec['p99_tick_ms']} ms per tick, "
+            f"{rec['requests_per_tick']} requests / tick, {rec['value'] / 1e6:.1f}M DOF/s, "
+            f"{rec['ticks_over_16.7ms']} of {len(lat)} ticks over 16.7 ms; iters median {rec['iters_median']} "
+            f"max {rec['iters_max']}; stages p50 {rec['stage_p50_ms']} p99 {rec['stage_p99_ms']}; server {rec['server']}")
+        out["per_lambda"].append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +322,11 @@ def main():
     ap.add_argument("--nz", type=int, default=48)
     ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=128)
+    ap.add_argument("--serve-lambdas", default="50,200,800", help="arrivals per tick of the serving leg ('' skips)")
+    ap.add_argument("--serve-ticks", type=int, default=90)
+    ap.add_argument("--serve-warmup", type=int, default=30)
+    ap.add_argument("--serve-life", type=int, default=20)
+    ap.add_argument("--serve-pool", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -235,6 +380,11 @@ def main():
                              "phi_max_abs_err": float(np.abs(mixed_res["phi"] - plain["phi"]).max())}
     if args.batch > 0:
         line["batched"] = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
+    if args.serve_lambdas:
+        lams = [int(x) for x in args.serve_lambdas.split(",") if x]
+        log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
+        line["serving"] = run_serving(mats, ctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
+                                      args.serve_pool)
     if not args.no_cpu:
         log("cpu baseline: the reference's solver on the same workload ...")
         cb = cpu_baseline(g, mats)

@@ -280,8 +280,12 @@ typedef struct svxg_server_stats {
   int64_t looks;          /* synchronisations of the running batch (admissions, completions) */
   int64_t assembled;      /* problems assembled */
   int64_t max_active;     /* largest running batch */
-  double step_ms;         /* host time spent stepping and looking */
-  double setup_ms;        /* host time spent in setups and loads */
+  double step_ms;         /* engine time stepping and looking (rebuilds included) */
+  double rebuild_ms;      /* engine time rebuilding the running batch and starting new solves */
+  double setup_ms;        /* setup-worker time: removals, assemblies, loads */
+  double assemble_ms;     /* of which assemblies */
+  int64_t setup_groups;   /* groups of requests the setup worker took at once */
+  int64_t rebuilds;
 } svxg_server_stats;
 
 void svxg_server_default_options(svxg_server_options* out);
@@ -296,6 +300,12 @@ int svxg_server_destroy(svxg_server* srv);
    f is copied; u must stay valid until completion. Returns SVXG_BAD_INPUT if p is in flight. */
 int svxg_server_submit(svxg_server* srv, svxg_problem* p, const double* f, double* u, int32_t warm,
                        const int32_t* remove, int32_t nremove, uint64_t* ticket);
+/* count requests at once (arrays of count; f[i], u[i], remove[i] may be null; warm, nremove may be
+   null for 0): as many svxg_server_submit calls, all or none (SVXG_BAD_INPUT names the first bad one
+   in svxg_last_error). */
+int svxg_server_submit_many(svxg_server* srv, int32_t count, svxg_problem* const* problems, const double* const* f,
+                            double* const* u, const int32_t* warm, const int32_t* const* remove, const int32_t* nremove,
+                            uint64_t* tickets);
 /* Completed requests, oldest first (up to cap); waits up to wait_ms for at least one. */
 int svxg_server_poll(svxg_server* srv, svxg_serve_done* out, int32_t cap, int32_t* count, int32_t wait_ms);
 /* Destroys p (its device memory returns to the budget) once no request of it is in flight. */

@@ -101,7 +101,9 @@ class ServeDone(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
                 ("steps", ctypes.c_int64), ("looks", ctypes.c_int64), ("assembled", ctypes.c_int64),
-                ("max_active", ctypes.c_int64), ("step_ms", ctypes.c_double), ("setup_ms", ctypes.c_double)]
+                ("max_active", ctypes.c_int64), ("step_ms", ctypes.c_double), ("rebuild_ms", ctypes.c_double),
+                ("setup_ms", ctypes.c_double), ("assemble_ms", ctypes.c_double), ("setup_groups", ctypes.c_int64),
+                ("rebuilds", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

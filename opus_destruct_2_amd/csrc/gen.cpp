@@ -336,6 +336,65 @@ void svxgen_piece_sizes(int32_t count, double median, double sigma, int32_t lo, 
   }
 }
 
+// Up to `count` intact bonds whose removal, all together, leaves every free node of the graph
+// connected to a support bond or a pinned node (no mechanism): a random spanning forest of the
+// intact bonds is grown from the supported and pinned nodes, and the cuts are drawn from the intact
+// bonds outside it. broken[k] != 0 marks bond k already broken. Returns the number written to out.
+int32_t svxgen_safe_cuts(int32_t n, const svxg_node* nodes, int32_t m, const svxg_bond* bonds, const uint8_t* broken,
+                         int32_t count, uint64_t seed, int32_t* out) {
+  Rng g{seed * 0x2545F4914F6CDD1Dull + 41};
+  std::vector<int32_t> deg(static_cast<size_t>(n) + 1, 0);
+  for (int32_t k = 0; k < m; ++k)
+    if (!broken[k] && bonds[k].b >= 0) {
+      ++deg[size_t(bonds[k].a) + 1];
+      ++deg[size_t(bonds[k].b) + 1];
+    }
+  for (int32_t i = 0; i < n; ++i) deg[size_t(i) + 1] += deg[size_t(i)];
+  std::vector<int32_t> adj(static_cast<size_t>(deg[size_t(n)])), put(deg.begin(), deg.end() - 1);
+  for (int32_t k = 0; k < m; ++k)
+    if (!broken[k] && bonds[k].b >= 0) {
+      adj[size_t(put[size_t(bonds[k].a)]++)] = k;
+      adj[size_t(put[size_t(bonds[k].b)]++)] = k;
+    }
+  std::vector<uint8_t> tree(static_cast<size_t>(m), 0), seen(static_cast<size_t>(n), 0);
+  std::vector<int32_t> front;
+  for (int32_t i = 0; i < n; ++i)
+    if (nodes[i].fixed) {
+      seen[size_t(i)] = 1;
+      front.push_back(i);
+    }
+  for (int32_t k = 0; k < m; ++k)
+    if (!broken[k] && bonds[k].b < 0 && !seen[size_t(bonds[k].a)]) {
+      seen[size_t(bonds[k].a)] = 1;
+      tree[size_t(k)] = 1;
+      front.push_back(bonds[k].a);
+    }
+  while (!front.empty()) {
+    const size_t pick = size_t(g.below(static_cast<int>(front.size())));
+    const int32_t i = front[pick];
+    front[pick] = front.back();
+    front.pop_back();
+    for (int32_t e = deg[size_t(i)]; e < deg[size_t(i) + 1]; ++e) {
+      const int32_t k = adj[size_t(e)];
+      const int32_t j = bonds[k].a == i ? bonds[k].b : bonds[k].a;
+      if (seen[size_t(j)]) continue;
+      seen[size_t(j)] = 1;
+      tree[size_t(k)] = 1;
+      front.push_back(j);
+    }
+  }
+  std::vector<int32_t> cand;
+  for (int32_t k = 0; k < m; ++k)
+    if (!broken[k] && !tree[size_t(k)]) cand.push_back(k);
+  const int32_t c = std::min<int32_t>(count, int32_t(cand.size()));
+  for (int32_t t = 0; t < c; ++t) {
+    const size_t j = size_t(t) + size_t(g.below(static_cast<int>(cand.size()) - t));
+    std::swap(cand[size_t(t)], cand[j]);
+    out[t] = cand[size_t(t)];
+  }
+  return c;
+}
+
 // A building of nx x ny x nz cells on supports (the city chunk), one material a storey.
 svxgen_graph* svxgen_building(int32_t nx, int32_t ny, int32_t nz, double L, uint64_t seed) {
   return svxgen_tower(nx, ny, nz, L, seed);

@@ -40,6 +40,8 @@ def lib():
         L.svxgen_copy.argtypes = [vp, vp, vp, vp]
         L.svxgen_free.argtypes = [vp]
         L.svxgen_uniform.argtypes = [u64, ctypes.c_int64, vp]
+        L.svxgen_safe_cuts.argtypes = [i32, vp, i32, vp, vp, i32, u64, vp]
+        L.svxgen_safe_cuts.restype = i32
         _LIB = L
     return _LIB
 
@@ -103,6 +105,16 @@ def piece(count, pinned=False, L=0.5, seed=1):
 
 def building(nx, ny, nz, L=0.5, seed=1):
     return _take(lib().svxgen_building(nx, ny, nz, L, seed), f"building{nx}x{ny}x{nz}")
+
+
+def safe_cuts(g, broken, count, seed):
+    """Up to `count` intact bonds of g whose removal leaves no free node without a path to a support
+    or pin (broken: uint8 per bond, 1 = already broken)."""
+    broken = np.ascontiguousarray(broken, np.uint8)
+    out = np.zeros(max(0, int(count)), np.int32)
+    k = lib().svxgen_safe_cuts(g.n, g.nodes.ctypes.data, len(g.bonds), g.bonds.ctypes.data, broken.ctypes.data,
+                               int(count), seed, out.ctypes.data)
+    return out[:k]
 
 
 def piece_sizes(count, median=200.0, sigma=0.75, lo=16, hi=20000, seed=1):

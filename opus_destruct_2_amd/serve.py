@@ -28,6 +28,7 @@ class Server:
         self._mu = threading.Lock()
         self._inflight = {}  # ticket -> (problem, u array or None)
         self._done = {}
+        self._buf = (ServeDone * 4096)()
 
     def now_ms(self):
         return lib().svxg_server_now_ms(self.h)
@@ -47,9 +48,34 @@ class Server:
             self._inflight[t.value] = (problem, u)
         return t.value
 
+    def submit_many(self, problems, fs=None, warm=None, removes=None, want_u=False):
+        """Queue len(problems) solves in one call (fs[i] None: the resident load). Returns tickets."""
+        n = len(problems)
+        if n == 0:
+            return []
+        fa = [None if fs is None or fs[i] is None else np.ascontiguousarray(fs[i], np.float64) for i in range(n)]
+        us = [np.zeros(6 * p.n) if want_u else None for p in problems]
+        rms = [None if removes is None or removes[i] is None or len(removes[i]) == 0
+               else np.ascontiguousarray(removes[i], np.int32) for i in range(n)]
+        vp = ctypes.c_void_p
+        ph = (vp * n)(*[p.h.value for p in problems])
+        fp = (vp * n)(*[None if a is None else a.ctypes.data for a in fa])
+        up = (vp * n)(*[None if a is None else a.ctypes.data for a in us])
+        wa = np.array([0] * n if warm is None else [int(bool(w)) for w in warm], np.int32)
+        rp = (vp * n)(*[None if a is None else a.ctypes.data for a in rms])
+        nr = np.array([0 if a is None else a.size for a in rms], np.int32)
+        tk = np.zeros(n, np.uint64)
+        with self._mu:
+            _check(lib().svxg_server_submit_many(self.h, n, ph, fp, up, ptr(wa), rp, ptr(nr), ptr(tk)),
+                   "server submit_many")
+            for i in range(n):
+                self._inflight[int(tk[i])] = (problems[i], us[i])
+        return [int(t) for t in tk]
+
     def poll(self, wait_ms=0, cap=4096):
-        """Completed requests since the last poll: {ticket: record}."""
-        buf = (ServeDone * cap)()
+        """Completed requests since the last poll (up to cap, at most 4096): {ticket: record}."""
+        cap = min(int(cap), 4096)
+        buf = self._buf
         n = ctypes.c_int32()
         _check(lib().svxg_server_poll(self.h, buf, cap, ctypes.byref(n), int(wait_ms)), "server poll")
         out = {}

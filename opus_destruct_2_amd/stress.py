@@ -70,6 +70,7 @@ def lib():
             "svxg_server_destroy": ([vp], ctypes.c_int),
             "svxg_server_submit": ([vp, vp, vp, vp, i32, vp, i32, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
             "svxg_server_poll": ([vp, vp, i32, ctypes.POINTER(i32), i32], ctypes.c_int),
+            "svxg_server_submit_many": ([vp, i32, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "svxg_server_release": ([vp, vp], ctypes.c_int),
             "svxg_server_now_ms": ([vp], f64),
             "svxg_server_stats_get": ([vp, vp], ctypes.c_int),

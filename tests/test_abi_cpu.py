@@ -102,3 +102,32 @@ def _fixed_digests():
     for line in open(path):
         k, v = line.split()
         FIXED[k] = v
+
+
+def test_safe_cuts_never_leave_a_mechanism():
+    """Every free node keeps a path to a support bond or a pinned node, cut after cut."""
+    for pinned in (False, True):
+        g = gen.piece(400, pinned=pinned, seed=21)
+        broken = np.zeros(len(g.bonds), np.uint8)
+        for r in range(30):
+            cut = gen.safe_cuts(g, broken, 8, seed=r)
+            assert len(set(cut.tolist())) == len(cut) and not broken[cut].any()
+            broken[cut] = 1
+        assert broken.sum() > len(g.bonds) // 4
+        parent = list(range(g.n + 1))  # node g.n: the ground
+
+        def find(i):
+            while parent[i] != i:
+                parent[i] = parent[parent[i]]
+                i = parent[i]
+            return i
+
+        for i in range(g.n):
+            if g.nodes["fixed"][i]:
+                parent[find(i)] = find(g.n)
+        for k in range(len(g.bonds)):
+            if broken[k]:
+                continue
+            a, b = int(g.bonds["a"][k]), int(g.bonds["b"][k])
+            parent[find(a)] = find(g.n if b < 0 else b)
+        assert all(find(i) == find(g.n) for i in range(g.n))
