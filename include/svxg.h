@@ -183,6 +183,11 @@ int svxg_problem_bond_checks(svxg_problem* p, const double* u, double fragility,
 
 /* StressProblem::add_force (stress.hpp:134; stress.cpp:504-513), on a host load vector. */
 void svxg_add_force(double* f, int32_t node, const double c[3], const double F[3], const double at[3]);
+/* StressProblem::extend_warm_start (stress.hpp:115; stress.cpp:213-250) over a graph's nodes and
+   bonds: u (6 per node) of nodes first_new.. that have none yet, the mean of their bonded older
+   neighbours' motion carried rigidly to their centres, in up to 4 passes. */
+void svxg_extend_warm_start(const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m, double* u,
+                            int32_t first_new);
 
 /* Problem facts: free nodes, K blocks, levels, per-level sizes and blocks (up to 16 each),
    StressProblem::work_per_iteration / assembly_work (stress.hpp:95,119), device bytes. */

@@ -3,7 +3,8 @@
 // The same public surface as the reference's svx::StressProblem
 // (core/include/svx/stress/stress.hpp:74-160): public `nodes`, `bonds`, `mats`; assemble(),
 // assembled(), invalidate(), preconditioner_current(), block_jacobi(), assembly_work(), release(),
-// remove_bond(), retire_nodes(), free_nodes(), matrix_blocks(), work_per_iteration(), solve(),
+// remove_bond(), retire_nodes(), reassemble(), extend_warm_start(), appended(), free_nodes(),
+// matrix_blocks(), memory_bytes(), solver_bytes(), work_per_iteration(), solve(),
 // begin() / iterate() / running() / stop() / current(), bond_load(), add_force(); and the free
 // functions bond_strength() / bond_utilization() (stress.hpp:53-66). Callers mutate nodes / bonds and
 // then assemble(), as World does; the device copy follows on the next assemble().
@@ -178,9 +179,27 @@ class StressProblem {
       pull_flags();
     }
   }
+  // K again from nodes and bonds as they are (stress.cpp:178-211). Here the whole problem is set up
+  // again, appended nodes included in the new multigrid, so nothing is left for appended() to count.
+  bool reassemble() { return assemble(opt_); }
+  // stress.cpp:213-250: u of the nodes from first_new on, carried from their older neighbours
+  void extend_warm_start(std::vector<f64>& u, i32 first_new) const {
+    if (u.size() != 6 * nodes.size()) return;
+    std::vector<svxg_node> n(nodes.size());
+    std::vector<svxg_bond> b(bonds.size());
+    for (size_t i = 0; i < nodes.size(); ++i) n[i] = to_c(nodes[i]);
+    for (size_t i = 0; i < bonds.size(); ++i) b[i] = to_c(bonds[i]);
+    svxg_extend_warm_start(n.data(), i32(n.size()), b.data(), i32(b.size()), u.data(), first_new);
+  }
+  i32 appended() const { return 0; }
   i32 free_nodes() const { return info().nfree; }
   i64 matrix_blocks() const { return info().blocks; }
   i64 work_per_iteration() const { return info().work_per_iteration; }
+  // (device bytes: the operator, hierarchy and solve vectors that release() frees)
+  i64 solver_bytes() const { return info().device_bytes; }
+  i64 memory_bytes() const {
+    return i64(nodes.capacity() * sizeof(SNode) + bonds.capacity() * sizeof(SBond)) + solver_bytes();
+  }
 
   PcgResult solve(const std::vector<f64>& f, std::vector<f64>& u, f64 rtol, int maxit, bool warm) {
     PcgResult r{};

@@ -52,6 +52,7 @@ def lib():
             "svxg_problem_bond_loads": ([vp, vp, vp], ctypes.c_int),
             "svxg_problem_bond_checks": ([vp, vp, f64, vp, vp], ctypes.c_int),
             "svxg_add_force": ([vp, i32, vp, vp, vp], None),
+            "svxg_extend_warm_start": ([vp, i32, vp, i32, vp, i32], None),
             "svxg_problem_info_get": ([vp, vp], ctypes.c_int),
             "svxg_problem_get_bonds": ([vp, vp], ctypes.c_int),
             "svxg_problem_get_nodes": ([vp, vp], ctypes.c_int),
@@ -315,6 +316,17 @@ def assemble_batch(problems, ctx=None):
     st = (ctypes.c_int32 * n)()
     _check(lib().svxg_assemble_batch(ctx.h, ph, n, st), "svxg_assemble_batch")
     return [s == SVXG_OK for s in st]
+
+
+def extend_warm_start(nodes, bonds, u, first_new):
+    """StressProblem::extend_warm_start (stress.hpp:115): u (6 per node) of the nodes from first_new
+    on, carried rigidly from their bonded older neighbours (a copy is returned)."""
+    nodes = np.ascontiguousarray(nodes, NODE)
+    bonds = np.ascontiguousarray(bonds, BOND)
+    out = np.array(u, np.float64)
+    lib().svxg_extend_warm_start(nodes.ctypes.data, len(nodes), bonds.ctypes.data, len(bonds), out.ctypes.data,
+                                 int(first_new))
+    return out
 
 
 def solve_batch(problems, fs, us=None, rtol=1e-8, maxit=-1, warm=False, ctx=None):

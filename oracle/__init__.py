@@ -44,6 +44,7 @@ def _bind(path, pre):
         "get_bonds": ([vp, vp], None),
         "set_threads": ([i32], None),
     }
+    sig["extend_warm_start"] = ([vp, i32, vp, i32, vp, i32], None)
     if pre == "orc_":
         sig.update({"get_matrix": ([vp, vp, vp, vp], None), "apply_matrix": ([vp, vp, vp], None),
                     "apply_preconditioner": ([vp, vp, vp], None), "get_nodes": ([vp, vp], None)})
@@ -65,6 +66,17 @@ def library(kind="oracle"):
             raise FileNotFoundError(path)
         _LIBS[kind] = (_bind(path, pre), pre)
     return _LIBS[kind]
+
+
+def extend_warm_start(nodes, bonds, u, first_new, kind="oracle"):
+    """StressProblem::extend_warm_start (stress.cpp:213-250) on a copy of u."""
+    L, pre = library(kind)
+    nodes = np.ascontiguousarray(nodes, NODE)
+    bonds = np.ascontiguousarray(bonds, BOND)
+    out = np.array(u, np.float64)
+    getattr(L, pre + "extend_warm_start")(nodes.ctypes.data, len(nodes), bonds.ctypes.data, len(bonds),
+                                         out.ctypes.data, int(first_new))
+    return out
 
 
 def ref_available():

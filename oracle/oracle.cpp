@@ -1117,6 +1117,43 @@ void orc_retire_nodes(orc_problem* h, const int32_t* ids, int32_t count) { h->re
 void orc_set_bond_broken(orc_problem* h, int32_t b, int32_t broken) { h->bonds[size_t(b)].broken = broken ? 1 : 0; }
 void orc_bond_load(orc_problem* h, int32_t b, const double* u, svxg_bond_load* out) { *out = h->bond_load(b, u); }
 
+// StressProblem::extend_warm_start (stress.cpp:213-250): passes over the intact internal bonds; a
+// node without a motion yet takes the mean, over its bonded neighbours that have one, of the
+// neighbour's translation plus rotation x (its centre - the neighbour's), and rotation as it is
+void orc_extend_warm_start(const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m, double* u,
+                           int32_t first_new) {
+  if (first_new >= n) return;
+  std::vector<f64> acc(6 * size_t(n), 0.0), w(size_t(n), 0.0);
+  std::vector<uint8_t> known(size_t(n), 0);
+  for (i32 i = 0; i < first_new; ++i) known[size_t(i)] = nodes[i].gone ? 0 : 1;
+  for (int pass = 0; pass < 4; ++pass) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    std::fill(w.begin(), w.end(), 0.0);
+    for (i32 k = 0; k < m; ++k) {
+      const svxg_bond& b = bonds[k];
+      if (b.broken || b.b < 0) continue;
+      i32 from = -1, to = -1;
+      if (known[size_t(b.a)] && !known[size_t(b.b)] && b.b >= first_new) from = b.a, to = b.b;
+      else if (known[size_t(b.b)] && !known[size_t(b.a)] && b.a >= first_new) from = b.b, to = b.a;
+      if (from < 0) continue;
+      const V3 r = vsub(v3(nodes[to].c), v3(nodes[from].c));
+      const V3 th{u[6 * size_t(from) + 3], u[6 * size_t(from) + 4], u[6 * size_t(from) + 5]};
+      const V3 t = vcross(th, r);
+      for (int q = 0; q < 3; ++q) acc[6 * size_t(to) + size_t(q)] += u[6 * size_t(from) + size_t(q)] + vget(t, q);
+      for (int q = 3; q < 6; ++q) acc[6 * size_t(to) + size_t(q)] += u[6 * size_t(from) + size_t(q)];
+      w[size_t(to)] += 1.0;
+    }
+    bool any = false;
+    for (i32 i = first_new; i < n; ++i) {
+      if (known[size_t(i)] || w[size_t(i)] == 0.0) continue;
+      for (int q = 0; q < 6; ++q) u[6 * size_t(i) + size_t(q)] = acc[6 * size_t(i) + size_t(q)] / w[size_t(i)];
+      known[size_t(i)] = 1;
+      any = true;
+    }
+    if (!any) break;
+  }
+}
+
 void orc_bond_strength(const svxg_bond* b, double fragility, const svxg_material* mats, int32_t nmats, double out[5]) {
   strengths(*b, fragility, mats, nmats, out);
 }

@@ -160,6 +160,15 @@ void ref_retire_nodes(ref_problem* h, const int32_t* ids, int32_t count) {
 
 void ref_set_bond_broken(ref_problem* h, int32_t b, int32_t broken) { h->P.bonds[size_t(b)].broken = broken != 0; }
 
+void ref_extend_warm_start(const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m, double* u,
+                           int32_t first_new) {
+  ref_problem* h = ref_create(nodes, n, bonds, m, nullptr, 0);
+  std::vector<f64> uv(u, u + 6 * size_t(n));
+  h->P.extend_warm_start(uv, first_new);
+  std::copy(uv.begin(), uv.end(), u);
+  ref_destroy(h);
+}
+
 void ref_bond_load(ref_problem* h, int32_t b, const double* u, svxg_bond_load* out) {
   const std::vector<f64> uv(u, u + 6 * h->P.nodes.size());
   const BondLoad L = h->P.bond_load(b, uv);

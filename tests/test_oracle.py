@@ -224,3 +224,34 @@ def test_oracle_matches_golden(path, mats):
 
 def test_golden_fixtures_present():
     assert len(glob.glob(os.path.join(GOLDEN, "*.npz"))) >= 4
+
+
+def test_extend_warm_start_parity_and_rigid_motion():
+    """extend_warm_start: the library's, the restatement's and the reference's agree bit for bit;
+    a rigid motion of the older nodes is carried exactly to the appended ones they reach."""
+    from opus_destruct_2_amd import gen, stress
+    import oracle
+    g = gen.piece(300, pinned=False, seed=8)
+    c = g.nodes["c"]
+    t, w = np.array([1e-3, -2e-3, 5e-4]), np.array([2e-4, -1e-4, 3e-4])
+    u = np.zeros((g.n, 6))
+    u[:, :3] = t + np.cross(w, c)
+    u[:, 3:] = w
+    u = u.reshape(-1)
+    first = 180
+    u0 = u.copy()
+    u0[6 * first:] = 0.0
+    bonds = g.bonds.copy()
+    bonds["broken"][::7] = 1  # (broken bonds carry nothing)
+    got = {"product": stress.extend_warm_start(g.nodes, bonds, u0, first),
+           "oracle": oracle.extend_warm_start(g.nodes, bonds, u0, first)}
+    if oracle.ref_available():
+        got["reference"] = oracle.extend_warm_start(g.nodes, bonds, u0, first, kind="ref")
+    for k, v in got.items():
+        assert np.array_equal(v.view(np.uint64), got["product"].view(np.uint64)), k
+    ue = got["product"].reshape(-1, 6)
+    reached = np.abs(ue[first:]).sum(axis=1) > 0
+    assert reached.sum() > 0.5 * (g.n - first)
+    ur = u.reshape(-1, 6)[first:]
+    assert np.allclose(ue[first:][reached], ur[reached], rtol=1e-12, atol=1e-15)
+    assert np.array_equal(ue[:first], u.reshape(-1, 6)[:first])
