@@ -186,7 +186,7 @@ class _Piece:
 
 def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11):
     """Serving stream: 60 Hz ticks; each tick, Poisson(lam) new structures (sizes as the batched
-    leg's, half on supports, half pinned) arrive and are solved cold (setup + solve), and 10 % of the
+    leg's, half on supports, half pinned; multigrid at every size, as pieces are) arrive and are solved cold (setup + solve), and 10 % of the
     resident structures that are idle lose 1-5 % of their bonds and are solved again, warm, on their
     resident load; a structure leaves `life` ticks after it arrived (its device memory returns to
     the budget). Cuts never leave a mechanism (each free node keeps a path to a support or its pin:
@@ -201,7 +201,7 @@ def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11):
            "pool_graphs": pool_size, "per_lambda": []}
     period = 1.0 / 60.0
     for lam in lams:
-        srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-8, maxit=5000, amg_min_nodes=160, window=4)
+        srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-8, maxit=5000, amg_min_nodes=0, window=4)
         residents = []  # [piece, birth tick, cut offset]
         busy = {}  # ticket -> (tick, piece index or None, dofs)
         in_flight = set()

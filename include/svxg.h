@@ -264,7 +264,14 @@ typedef struct svxg_server_options {
   int32_t maxit;          /* iterations of a solve at most (5000) */
   int32_t amg_min_nodes;  /* assembly option of the problems a server assembles (160) */
   int32_t window;         /* PCG steps between looks at most (8) */
-  int32_t pad_;
+  /* the reference's rebuild policy for stale preconditioners (world.cpp:1068-1110): a solve on a
+     stale multigrid or a block-Jacobi one stops after restart_iters (120) iterations, gets its
+     preconditioner built again (multigrid) and goes on from where it is (at most twice); one that
+     converged after more than rebuild_iters (60) on one rebuilds it for its next request. 0
+     turns either off. */
+  int32_t restart_iters;
+  int32_t rebuild_iters;
+  int32_t setup_workers;  /* setup workers (2), each with its own stream: groups of arrivals set up side by side */
 } svxg_server_options;
 
 typedef struct svxg_serve_done {
@@ -291,6 +298,7 @@ typedef struct svxg_server_stats {
   double assemble_ms;     /* of which assemblies */
   int64_t setup_groups;   /* groups of requests the setup worker took at once */
   int64_t rebuilds;
+  int64_t restarts;       /* solves restarted on a rebuilt preconditioner (world.cpp:1068-1110) */
 } svxg_server_stats;
 
 void svxg_server_default_options(svxg_server_options* out);

@@ -89,7 +89,7 @@ class ProblemInfo(ctypes.Structure):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_active", ctypes.c_int32), ("rtol", ctypes.c_double),
                 ("maxit", ctypes.c_int32), ("amg_min_nodes", ctypes.c_int32), ("window", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("restart_iters", ctypes.c_int32), ("rebuild_iters", ctypes.c_int32), ("setup_workers", ctypes.c_int32)]
 
 
 class ServeDone(ctypes.Structure):
@@ -103,7 +103,7 @@ class ServerStats(ctypes.Structure):
                 ("steps", ctypes.c_int64), ("looks", ctypes.c_int64), ("assembled", ctypes.c_int64),
                 ("max_active", ctypes.c_int64), ("step_ms", ctypes.c_double), ("rebuild_ms", ctypes.c_double),
                 ("setup_ms", ctypes.c_double), ("assemble_ms", ctypes.c_double), ("setup_groups", ctypes.c_int64),
-                ("rebuilds", ctypes.c_int64)]
+                ("rebuilds", ctypes.c_int64), ("restarts", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1071,9 +1071,11 @@ __global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict
 
 // ---- dense coarsest level: Cholesky (amg.cpp:535-578), then the explicit inverse --------------------
 
-// One workgroup of 1024 lanes a graph, every phase parallel over the matrix: the symmetrised matrix,
-// a right-looking Cholesky with the reference's pivot floor (d >= 1e-12 max|a_ii|, amg.cpp:535-578),
-// W = L^-1 by right-looking forward substitution on I, then M^-1 = W^T W.
+// One workgroup of 1024 lanes a graph, every phase parallel over the matrix, one barrier a
+// column: the symmetrised matrix; a right-looking Cholesky with the reference's pivot floor
+// (d >= 1e-12 max|a_ii|, amg.cpp:535-578) in which column j's scaling by its pivot is folded into
+// the trailing update (a_ik -= a_ij a_kj / d_j) and written one step later; W = L^-1 by
+// right-looking forward substitution on I (row k's division likewise deferred); M^-1 = W^T W.
 constexpr int kDenseWG = 1024;
 __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
   if (int(blockIdx.x) >= count) return;
@@ -1103,7 +1105,7 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
     if (i > j) at(i, j) = 0.5 * (at(i, j) + at(j, i));
   }
   __shared__ double red[kDenseWG / 64];
-  __shared__ double spiv;
+  __shared__ double piv[2];  // the pivots d_j of the step before and of this one
   double dm = 0.0;
   for (int i = tid; i < m; i += kDenseWG) dm = fmax(dm, fabs(at(i, i)));
   for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_xor(dm, o, 64));
@@ -1111,38 +1113,46 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
   __syncthreads();
   double dmax = 0.0;
   for (int w = 0; w < kDenseWG / 64; ++w) dmax = fmax(dmax, red[w]);
-  // Cholesky (lower triangle)
-  for (int j = 0; j < m; ++j) {
-    if (tid == 0) {
-      double dd = at(j, j);
+  // Cholesky: step j finishes column j - 1 (scaled by its pivot's root) and updates the trailing
+  // lower triangle with column j's unscaled entries
+  for (int j = 0; j <= m; ++j) {
+    double dd = 1.0;
+    if (j < m) {
+      dd = at(j, j);
       if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
-      const double sq = sqrt(dd);
-      at(j, j) = sq;
-      spiv = sq;
     }
-    __syncthreads();
-    const double sq = spiv;
-    for (int i = j + 1 + tid; i < m; i += kDenseWG) at(i, j) /= sq;
-    __syncthreads();
+    if (j > 0) {
+      const double sp = sqrt(piv[(j - 1) & 1]);
+      for (int i = j + tid; i < m; i += kDenseWG) at(i, j - 1) /= sp;
+      if (tid == 0) at(j - 1, j - 1) = sp;
+    }
+    if (j == m) break;
+    if (tid == 0) piv[j & 1] = dd;
     const int w = m - j - 1;
+    const double inv = 1.0 / dd;
     for (int t = tid; t < w * w; t += kDenseWG) {
       const int i = j + 1 + t / w, k = j + 1 + t % w;
-      if (k <= i) at(i, k) -= at(i, j) * at(k, j);
+      if (k <= i) at(i, k) -= at(i, j) * at(k, j) * inv;
     }
     __syncthreads();
   }
-  // W = L^-1: row k divided by L_kk, then taken off the rows below
-  for (int k = 0; k < m; ++k) {
-    const double lk = at(k, k);
-    for (int c = tid; c <= k; c += kDenseWG) wat(k, c) /= lk;
-    __syncthreads();
+  __syncthreads();
+  // W = L^-1: step k takes row k (not yet divided by L_kk) off the rows below, and divides row k - 1
+  for (int k = 0; k <= m; ++k) {
+    if (k > 0) {
+      const double lk = at(k - 1, k - 1);
+      for (int c = tid; c < k; c += kDenseWG) wat(k - 1, c) /= lk;
+    }
+    if (k == m) break;
+    const double inv = 1.0 / at(k, k);
     const int rows = m - k - 1, cols = k + 1;
     for (int t = tid; t < rows * cols; t += kDenseWG) {
       const int i = k + 1 + t / cols, c = t % cols;
-      wat(i, c) -= at(i, k) * wat(k, c);
+      wat(i, c) -= at(i, k) * wat(k, c) * inv;
     }
     __syncthreads();
   }
+  __syncthreads();
   // M^-1 = W^T W (lower half, mirrored), into L's storage, then over W
   for (size_t t = tid; t < mm; t += kDenseWG) {
     const int i = int(t / size_t(m)), j = int(t % size_t(m));

@@ -16,12 +16,13 @@ from .stress import SvxgError, _check, default_context, lib
 
 class Server:
     def __init__(self, ctx=None, precision=SVXG_MIXED, max_active=4096, rtol=1e-8, maxit=5000,
-                 amg_min_nodes=160, window=8):
+                 amg_min_nodes=160, window=8, restart_iters=120, rebuild_iters=60, setup_workers=2):
         self.ctx = ctx or default_context()
         o = ServerOptions()
         lib().svxg_server_default_options(ctypes.byref(o))
         o.precision, o.max_active, o.rtol, o.maxit = precision, max_active, rtol, maxit
         o.amg_min_nodes, o.window = amg_min_nodes, window
+        o.restart_iters, o.rebuild_iters, o.setup_workers = restart_iters, rebuild_iters, setup_workers
         h = ctypes.c_void_p()
         _check(lib().svxg_server_create(self.ctx.h, ctypes.byref(o), ctypes.byref(h)), "svxg_server_create")
         self.h = h

@@ -92,8 +92,9 @@ def test_removals_and_warm_starts_equal_the_problem_api(precision, mats, ctx):
         u1, r1 = p.solve(g.f, u=u0, rtol=1e-9, maxit=5000, warm=True)
         ref.append((u0, r0, u1, r1))
         p.close()
-    # served: a first request, then removals + warm start with the resident load
-    srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=3)
+    # served: a first request, then removals + warm start with the resident load (the rebuild policy
+    # off: the problem API has none)
+    srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=3, restart_iters=0, rebuild_iters=0)
     ps = problems(gs, mats, ctx)
     first = {srv.submit(p, g.f): k for k, (p, g) in enumerate(zip(ps, gs))}
     got = srv.wait(list(first))
@@ -105,6 +106,29 @@ def test_removals_and_warm_starts_equal_the_problem_api(precision, mats, ctx):
     for t, k in second.items():
         assert got[t]["result"]["iters"] == ref[k][3]["iters"], (k, got[t]["result"], ref[k][3])
         assert np.array_equal(bits(got[t]["u"]), bits(ref[k][2])), k
+
+
+def test_stale_and_block_jacobi_solves_restart_on_a_rebuilt_multigrid(mats, ctx):
+    """The rebuild policy: a block-Jacobi solve stopped at restart_iters goes on, on a multigrid set
+    up in its place, from where it was, and converges to the oracle's answer; the same alone."""
+    g = gen.piece(150, pinned=False, seed=4242)  # (under 160 nodes: block-Jacobi)
+    o = CpuProblem(g.nodes, g.bonds, mats)
+    assert o.assemble(reference_options())
+    uo, ro = o.solve(g.f, rtol=1e-10, maxit=20000)
+    res = []
+    for window in (2, 7):
+        srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-10, window=window, restart_iters=10)
+        p = stress.StressProblem(g.nodes, g.bonds, mats, ctx)
+        t = srv.submit(p, g.f)
+        rec = srv.wait([t])[t]
+        st = srv.stats()
+        srv.close()
+        assert rec["result"]["converged"] and st["restarts"] == 1, (rec, st)
+        assert p.info()["levels"] > 1  # (a multigrid now)
+        assert np.abs(rec["u"] - uo).max() <= 1e-5 * np.abs(uo).max()
+        res.append(rec)
+    assert np.array_equal(bits(res[0]["u"]), bits(res[1]["u"]))
+    assert res[0]["result"] == res[1]["result"]
 
 
 def test_admission_waits_for_the_hbm_budget(mats):
