@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--serve-warmup", type=int, default=30)
     ap.add_argument("--serve-life", type=int, default=20)
     ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--resident-rows", type=int, default=-1,
+                    help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -351,6 +353,8 @@ def main():
     g = gen.block(args.nx, args.ny, args.nz, 0.5, 0.15, seed=1)
     log(f"workload {g.name}: {g.n} nodes, {g.m} bonds, {g.dofs} DOFs")
     ctx = stress.default_context()
+    if args.resident_rows >= 0:
+        ctx.set_resident_rows(args.resident_rows)
     head_prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
     other_prec = SVXG_FP64 if head_prec == SVXG_MIXED else SVXG_MIXED
     head = run_path(g, mats, head_prec, args.steps, args.warmup, ctx)

@@ -242,6 +242,11 @@ int svxg_problem_last_timing(const svxg_problem* p, double* solve_ms, double* se
    (an A product fused with the smoother update), FP64 / FP32. Stats: launches, summed device
    milliseconds and summed algorithmic bytes since the last call (which resets them). */
 int svxg_ctx_set_profiling(svxg_ctx* ctx, int32_t on);
+/* Graphs with up to `rows` free nodes (0 by default: none) run their solve steps in one
+   workgroup each (one launch a window of steps), the rest in tiled launches. The arithmetic, and
+   so every bit of a result, is the same either way; this only moves time. No counterpart in the
+   reference. */
+int svxg_ctx_set_resident_rows(svxg_ctx* ctx, int32_t rows);
 int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------

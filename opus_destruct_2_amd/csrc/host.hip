@@ -363,11 +363,15 @@ struct Batch {
   std::vector<GraphDev> hG;
   DevBuf dG, dstate;
   std::vector<PcgState> hstate;
-  DevBuf t0;
+  DevBuf t0;       // level-0 tiles of every graph (their slots: the partial sums' layout)
   int nt0 = 0;
+  DevBuf t0run;    // those of the tiled graphs (what the tiled launches run over)
+  int nt0run = 0;
   DevBuf gtile;
   DevBuf tj;
   int ntj = 0;
+  DevBuf res;      // resident graphs (one workgroup each: k_resident_steps)
+  int nres = 0;
   // per level: smoothing tiles (S = 1 and S = 10 sub-groups a row), prolongation tiles, restriction
   // tiles (coarse rows, S = 1 / 10), stalled-coarsest tiles (S = 1 / 10), dense-coarsest graphs
   struct Lv {
@@ -415,6 +419,8 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.Minv = H.Minv.as<double>();
     L.lmax = H.lmax;
     L.cheb_degree = P->opt.amg.cheb_degree;
+    L.sub = H.nnzb > 16 * int64_t(H.n) ? 10 : 1;
+    L.rsub = H.nc > 0 && H.np > 16 * int64_t(H.nc) ? 10 : 1;
     L.f64.A = H.A64.as<double>();
     L.f64.Dinv = H.D64.as<double>();
     L.f64.P = H.P64.as<double>();
@@ -441,7 +447,10 @@ static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entrie
 
 static int prof_table(Batch& B, Arena* A, hipStream_t s);
 
-static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A, hipStream_t s) {
+static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A, hipStream_t s,
+                       int resident_rows = -1) {
+  svxg_ctx* c = count ? probs[0]->ctx : nullptr;
+  if (resident_rows < 0) resident_rows = c ? c->resident_rows : 0;
   B.probs.assign(probs, probs + count);
   B.G = count;
   SVXG_TRY(B.dstate.make(A, sizeof(PcgState) * size_t(std::max(1, count))));
@@ -449,64 +458,85 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   B.hG.resize(size_t(count));
   B.maxlev = 1;
   B.any_amg = false;
+  std::vector<int32_t> res;
+  std::vector<char> tiled(static_cast<size_t>(count), 1);
   for (int g = 0; g < count; ++g) {
-    fill_desc(probs[g], B.hG[size_t(g)], B.dstate.as<PcgState>() + g);
-    if (!probs[g]->jacobi_only) {
-      B.maxlev = std::max(B.maxlev, int(probs[g]->lv.size()));
+    svxg_problem* P = probs[g];
+    fill_desc(P, B.hG[size_t(g)], B.dstate.as<PcgState>() + g);
+    if (resident_rows > 0 && P->nfree <= resident_rows) {
+      B.hG[size_t(g)].resident = 1;
+      tiled[size_t(g)] = 0;
+      res.push_back(g);
+      continue;
+    }
+    if (!P->jacobi_only) {
+      B.maxlev = std::max(B.maxlev, int(P->lv.size()));
       B.any_amg = true;
     }
   }
-  SVXG_TRY(upload(B.dG, A, B.hG, s));
-  std::vector<Tile> t0, tj;
+  B.nres = int(res.size());
+  std::vector<Tile> t0, t0run, tj;
   std::vector<int32_t> gt(static_cast<size_t>(count) + 1, 0);
   for (int g = 0; g < count; ++g) {
     gt[size_t(g)] = int32_t(t0.size());
+    const size_t first = t0.size();
     add_tiles(t0, g, probs[g]->nfree, kTileRows);
+    if (!tiled[size_t(g)]) continue;
+    t0run.insert(t0run.end(), t0.begin() + std::ptrdiff_t(first), t0.end());
     if (probs[g]->jacobi_only) add_tiles(tj, g, probs[g]->nfree, kTileRows);
   }
   gt[size_t(count)] = int32_t(t0.size());
   B.nt0 = int(t0.size());
+  B.nt0run = int(t0run.size());
   B.ntj = int(tj.size());
-  SVXG_TRY(upload(B.t0, A, t0, s));
-  SVXG_TRY(upload(B.tj, A, tj, s));
-  SVXG_TRY(upload(B.gtile, A, gt, s));
+  Stager up(c, s);
+  SVXG_TRY(up.add(B.dG, A, B.hG));
+  SVXG_TRY(up.add(B.t0, A, t0));
+  SVXG_TRY(up.add(B.t0run, A, t0run));
+  SVXG_TRY(up.add(B.tj, A, tj));
+  SVXG_TRY(up.add(B.gtile, A, gt));
+  SVXG_TRY(up.add(B.res, A, res));
   SVXG_TRY(B.partial.make(A, sizeof(double) * 2 * size_t(std::max(1, B.nt0))));
   B.lvl.clear();
   B.lvl.resize(size_t(B.maxlev));
-  for (int l = 0; l < B.maxlev; ++l) {
+  struct LvTiles {
     std::vector<Tile> ts[2], tp, tc[2], tstall[2];
     std::vector<int32_t> dense;
+  };
+  std::vector<LvTiles> lt(static_cast<size_t>(B.maxlev));
+  for (int l = 0; l < B.maxlev; ++l) {
+    LvTiles& T = lt[size_t(l)];
     for (int g = 0; g < count; ++g) {
       svxg_problem* P = probs[g];
-      if (P->jacobi_only || l >= int(P->lv.size())) continue;
+      if (!tiled[size_t(g)] || P->jacobi_only || l >= int(P->lv.size())) continue;
       const HostLevel& H = P->lv[size_t(l)];
       const int vs = long_rows(H.nnzb, H.n) ? 1 : 0;
       if (!H.terminal) {
-        add_tiles(ts[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
-        add_tiles(tp, g, H.n, kTileRows);
+        add_tiles(T.ts[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
+        add_tiles(T.tp, g, H.n, kTileRows);
         const int vc = long_rows(H.np, H.nc) ? 1 : 0;
-        add_tiles(tc[vc], g, P->lv[size_t(l) + 1].n, vc ? tile_rows<10>() : tile_rows<1>());
+        add_tiles(T.tc[vc], g, P->lv[size_t(l) + 1].n, vc ? tile_rows<10>() : tile_rows<1>());
       } else if (H.dense_m > 0) {
-        dense.push_back(g);
+        T.dense.push_back(g);
       } else {
-        add_tiles(tstall[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
+        add_tiles(T.tstall[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
       }
     }
     Batch::Lv& V = B.lvl[size_t(l)];
     for (int v = 0; v < 2; ++v) {
-      V.nts[v] = int(ts[v].size());
-      V.ntc[v] = int(tc[v].size());
-      V.ntstall[v] = int(tstall[v].size());
-      SVXG_TRY(upload(V.ts[v], A, ts[v], s));
-      SVXG_TRY(upload(V.tc[v], A, tc[v], s));
-      SVXG_TRY(upload(V.tstall[v], A, tstall[v], s));
+      V.nts[v] = int(T.ts[v].size());
+      V.ntc[v] = int(T.tc[v].size());
+      V.ntstall[v] = int(T.tstall[v].size());
+      SVXG_TRY(up.add(V.ts[v], A, T.ts[v]));
+      SVXG_TRY(up.add(V.tc[v], A, T.tc[v]));
+      SVXG_TRY(up.add(V.tstall[v], A, T.tstall[v]));
     }
-    V.ntp = int(tp.size());
-    V.ndense = int(dense.size());
-    SVXG_TRY(upload(V.tp, A, tp, s));
-    SVXG_TRY(upload(V.dense, A, dense, s));
+    V.ntp = int(T.tp.size());
+    V.ndense = int(T.dense.size());
+    SVXG_TRY(up.add(V.tp, A, T.tp));
+    SVXG_TRY(up.add(V.dense, A, T.dense));
   }
-  return SVXG_OK;
+  return up.flush();
 }
 
 // ---- kernel timing ----------------------------------------------------------------------------------
@@ -585,7 +615,6 @@ static void prof_collect(svxg_ctx* c) {
 
 // ---- the V-cycle -----------------------------------------------------------------------------------
 
-constexpr int kStallDegree = 8;  // Chebyshev steps of a coarsest level left without a dense solve
 
 template <typename T>
 static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
@@ -697,33 +726,43 @@ static void drop_self_batch_ptrs(svxg_problem* P);
 // each with its own scalars and stopping point
 static void fp64_step(Batch& B, hipStream_t s, int degree) {
   const GraphDev* G = B.dG.as<GraphDev>();
-  const Tile* t0 = B.t0.as<Tile>();
+  const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
   {
     ProfScope ps(B, 0, s, SERVE_OUTER);
-    launch_pcg_spmv<double>(s, G, t0, B.nt0, SERVE_OUTER, part);
+    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
   }
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
-  launch_axpy_rr<double>(s, G, t0, B.nt0, SERVE_OUTER, part);
+  launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
   launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_OUTER, part);
   vcycle<double>(B, s, SERVE_OUTER, degree);
-  launch_dot_rz<double>(s, G, t0, B.nt0, SERVE_OUTER, part);
+  launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
   launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_OUTER, part);
-  launch_update_p<double>(s, G, t0, B.nt0, SERVE_OUTER);
+  launch_update_p<double>(s, G, t0, B.nt0run, SERVE_OUTER);
+}
+
+// a window of w steps: the resident graphs' in one launch, the tiled graphs' step by step
+static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64);
+static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, bool p64) {
+  const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
+  if (B.nres)
+    launch_resident_steps(s, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.res.as<int32_t>(), B.nres, w, degree,
+                          mixed ? 1 : 0, p32 ? 1 : 0, p64 ? 1 : 0, B.partial.as<double>());
+  if (!B.nt0run) return;
+  for (int k = 0; k < w; ++k) {
+    if (mixed) mixed_step(B, s, degree, p32, p64);
+    else fp64_step(B, s, degree);
+  }
 }
 
 // the FP64 PCG over a batch (after run_begin)
 static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
-  const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
   PollPlan plan;
-  for (int left = 0;; --left) {
-    if (left == 0) {
-      SVXG_TRY(read_states(B, s));
-      if (!any_running(B, false)) break;
-      left = plan.next(B, false);
-    }
-    fp64_step(B, s, degree);
+  for (;;) {
+    SVXG_TRY(read_states(B, s));
+    if (!any_running(B, false)) break;
+    run_window(B, s, plan.next(B, false), false, false, false);
   }
   SVXG_HIP(hipGetLastError());
   return SVXG_OK;
@@ -740,33 +779,33 @@ static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
 // own sequence begins. p32 / p64: launch the FP32- / FP64-product kernels.
 static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64) {
   const GraphDev* G = B.dG.as<GraphDev>();
-  const Tile* t0 = B.t0.as<Tile>();
+  const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
-  launch_outer_residual(s, G, t0, B.nt0, part);
+  launch_outer_residual(s, G, t0, B.nt0run, part);
   launch_finish(s, FinishKind::Outer, G, gt, B.G, SERVE_REFINE, part);
-  launch_inner_init(s, G, t0, B.nt0);
+  launch_inner_init(s, G, t0, B.nt0run);
   {
     ProfScope ps(B, 1, s, SERVE_P32 | SERVE_IT);
-    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT, part);
+    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
   }
-  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT, part);
+  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_ITER, part);
-  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT, part);
-  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT, part);
+  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
+  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_ITER, part);
-  if (p64) launch_convert64(s, G, t0, B.nt0, 1);
+  if (p64) launch_convert64(s, G, t0, B.nt0run, 1);
   vcycle<float>(B, s, SERVE_INNER, degree);
-  if (p64) launch_convert64(s, G, t0, B.nt0, 0);
-  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT, part);
-  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT, part);
+  if (p64) launch_convert64(s, G, t0, B.nt0run, 0);
+  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
+  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);
-  if (p32) launch_update_p<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_IT);
-  if (p64) launch_update_p<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_IT);
-  if (p32) launch_start_p<float>(s, G, t0, B.nt0, SERVE_P32 | SERVE_ST, part);
-  if (p64) launch_start_p<double>(s, G, t0, B.nt0, SERVE_P64 | SERVE_ST, part);
+  if (p32) launch_update_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT);
+  if (p64) launch_update_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT);
+  if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part);
+  if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
   launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
-  launch_inner_apply(s, G, t0, B.nt0);
+  launch_inner_apply(s, G, t0, B.nt0run);
   launch_inner_close(s, G, B.G);
 }
 
@@ -787,17 +826,14 @@ static void mixed_window(Batch& B, hipStream_t s, bool& p32, bool& p64) {
 // the mixed-precision solve (after run_begin): FP64 iterative refinement around FP32 inner PCG
 // solves, each graph stepping through its own refinements
 static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
-  const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
   PollPlan plan;
   bool p32 = true, p64 = false;
-  for (int left = 0;; --left) {
-    if (left == 0) {
-      SVXG_TRY(read_states(B, s));
-      if (!any_running(B, false)) break;
-      left = plan.next(B, true);
-      mixed_window(B, s, p32, p64);
-    }
-    mixed_step(B, s, degree, p32, p64);
+  for (;;) {
+    SVXG_TRY(read_states(B, s));
+    if (!any_running(B, false)) break;
+    const int w = plan.next(B, true);
+    mixed_window(B, s, p32, p64);
+    run_window(B, s, w, true, p32, p64);
   }
   SVXG_HIP(hipGetLastError());
   return SVXG_OK;
@@ -805,8 +841,10 @@ static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
 
 // the head of StressProblem::iterate on a resumed run: |r| against the (new) target
 static int recheck_residual(Batch& B, hipStream_t s) {
-  launch_dot_rr<double>(s, B.dG.as<GraphDev>(), B.t0.as<Tile>(), B.nt0, 0, B.partial.as<double>());
+  launch_dot_rr<double>(s, B.dG.as<GraphDev>(), B.t0run.as<Tile>(), B.nt0run, 0, B.partial.as<double>());
   launch_finish(s, FinishKind::Rn, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.G, 0, B.partial.as<double>());
+  launch_resident_recheck(s, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.res.as<int32_t>(), B.nres,
+                          B.partial.as<double>());
   SVXG_HIP(hipGetLastError());
   return SVXG_OK;
 }

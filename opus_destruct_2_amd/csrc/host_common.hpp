@@ -121,6 +121,7 @@ struct HostLevel {
 
 struct svxg_ctx {
   int device = 0;
+  int resident_rows = 0;  // graphs up to this many free rows run their steps in one workgroup (0: none)
   hipStream_t stream = nullptr;
   std::unique_ptr<svxg::Arena> arena;
   std::mutex mu;  // batches and setup on the context stream, one at a time

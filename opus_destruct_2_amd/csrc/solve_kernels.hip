@@ -157,6 +157,14 @@ __device__ __forceinline__ void finish_partials(double v0, double v1, double* pa
   }
 }
 
+// the sum of a graph's per-tile partials (its tiles' slots k0 .. k1), in the same order wherever run
+__device__ __forceinline__ double graph_sum(int k0, int k1, const double* partial, int which) {
+  __shared__ double lds[4];
+  double a = 0.0;
+  for (int k = k0 + int(threadIdx.x); k < k1; k += kWG) a += partial[2 * k + which];
+  return block_sum(a, lds);
+}
+
 }  // namespace
 
 // ---- kernel timing -----------------------------------------------------------------------------
@@ -168,7 +176,7 @@ __global__ void __launch_bounds__(kWG) k_prof_account(const GraphDev* __restrict
   __shared__ double lds[4];
   double v = 0.0;
   for (int g = threadIdx.x; g < count; g += kWG)
-    if (live(G[g], inner)) v += gbytes[g];
+    if (!G[g].resident && live(G[g], inner)) v += gbytes[g];
   const double s = block_sum(v, lds);
   if (threadIdx.x == 0) *acc += s;
 }
@@ -180,14 +188,7 @@ __global__ void __launch_bounds__(kWG) k_prof_account(const GraphDev* __restrict
 // pass; its own instantiation so that timings of the two kinds of step stay apart). With
 // `partial`, the partial sums of b . x_out.
 template <typename T, int S, bool FRESH>
-__global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                   int ntiles, int level, int step, int in_buf, int out_buf,
-                                                   int inner, double* __restrict__ partial) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+__device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl, int level, int step, int in_buf, int out_buf, int inner, double* __restrict__ partial) {
   const LevelDev& L = g.lv[level];
   const LevelArrays<T>& a = arrays<T>(L);
   int rl, sub, q;
@@ -219,17 +220,23 @@ __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ 
     a.x[out_buf][e] = xo;
   }
   if (partial) finish_partials(on && sub == 0 ? double(bq) * double(xo) : 0.0, 0.0, partial, tl.slot, false);
-}
 
-// r = b - A x
-template <typename T, int S>
-__global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int level, int buf, int inner) {
+}
+template <typename T, int S, bool FRESH>
+__global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                   int ntiles, int level, int step, int in_buf, int out_buf,
+                                                   int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  cheb_step_tile<T, S, FRESH>(g, tl, level, step, in_buf, out_buf, inner, partial);
+}
+
+// r = b - A x
+template <typename T, int S>
+__device__ __forceinline__ void residual_tile(const GraphDev& g, const Tile& tl, int level, int buf, int inner) {
   const LevelDev& L = g.lv[level];
   const LevelArrays<T>& a = arrays<T>(L);
   int rl, sub, q;
@@ -241,17 +248,22 @@ __global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G
   if (!on || sub != 0) return;
   const size_t e = 6 * size_t(row) + q;
   a.r[e] = a.b[e] - v;
-}
 
-// b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`
+}
 template <typename T, int S>
-__global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int level, int inner) {
+__global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int level, int buf, int inner) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  residual_tile<T, S>(g, tl, level, buf, inner);
+}
+
+// b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`
+template <typename T, int S>
+__device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl, int level, int inner) {
   const LevelDev& F = g.lv[level];
   const LevelArrays<T>& af = arrays<T>(F);
   const LevelArrays<T>& ac = arrays<T>(g.lv[level + 1]);
@@ -271,17 +283,22 @@ __global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G
   acc = row_reduce<S>(acc);
   if (!on || sub != 0) return;
   ac.b[6 * size_t(I) + q] = acc;
-}
 
-// x += P x_c (x_c: the coarse level's result, x[1])
-template <typename T>
-__global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int level, int buf, int inner) {
+}
+template <typename T, int S>
+__global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int level, int inner) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  restrict_tile<T, S>(g, tl, level, inner);
+}
+
+// x += P x_c (x_c: the coarse level's result, x[1])
+template <typename T>
+__device__ __forceinline__ void prolong_tile(const GraphDev& g, const Tile& tl, int level, int buf, int inner) {
   const LevelDev& F = g.lv[level];
   const LevelArrays<T>& af = arrays<T>(F);
   const T* xc = arrays<T>(g.lv[level + 1]).x[1];
@@ -298,16 +315,23 @@ __global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G,
     acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
   }
   af.x[buf][6 * size_t(i) + q] = acc;
+
+}
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                 int ntiles, int level, int buf, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, inner)) return;
+  prolong_tile<T>(g, tl, level, buf, inner);
 }
 
 // x[1] = M^-1 b at the coarsest level of each listed graph (FP64 products): one workgroup a graph,
 // a wavefront a row at a time, lanes across the row
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict__ G, const int32_t* __restrict__ graphs,
-                                                     int count, int level, int inner) {
-  if (int(blockIdx.x) >= count) return;
-  const GraphDev& g = G[graphs[blockIdx.x]];
-  if (!live(g, inner)) return;
+__device__ __forceinline__ void dense_solve_g(const GraphDev& g, int level) {
   const LevelDev& L = g.lv[level];
   const LevelArrays<T>& a = arrays<T>(L);
   const int m = L.dense_m;
@@ -320,16 +344,18 @@ __global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict_
     if (l == 0) a.x[1][i] = T(s);
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict__ G, const int32_t* __restrict__ graphs,
+                                                     int count, int level, int inner) {
+  if (int(blockIdx.x) >= count) return;
+  const GraphDev& g = G[graphs[blockIdx.x]];
+  if (g.resident || !live(g, inner)) return;
+  dense_solve_g<T>(g, level);
+}
 
 // z = D^-1 r (block-Jacobi preconditioning of a graph without a hierarchy)
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_jacobi(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                int ntiles, int inner) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+__device__ __forceinline__ void jacobi_tile(const GraphDev& g, const Tile& tl, int inner) {
   const LevelArrays<T>& a = arrays<T>(g.lv[0]);
   int rl, q;
   lane_row(rl, q);
@@ -340,6 +366,17 @@ __global__ void __launch_bounds__(kWG) k_jacobi(const GraphDev* __restrict__ G, 
   T rv[6];
   row_gather(r, rv);
   if (on) a.x[1][e] = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
+
+}
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_jacobi(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                int ntiles, int inner) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, inner)) return;
+  jacobi_tile<T>(g, tl, inner);
 }
 
 // ---- PCG kernels (level 0 tiles) -------------------------------------------------------------
@@ -361,13 +398,7 @@ __device__ __forceinline__ PcgVecs<float> pcg_vecs<float>(const GraphDev& g) {
 
 // q = K p, partial p . q
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int inner, double* __restrict__ partial) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+__device__ __forceinline__ void pcg_spmv_tile(const GraphDev& g, const Tile& tl, int inner, double* __restrict__ partial) {
   const LevelDev& L = g.lv[0];
   const PcgVecs<T> v = pcg_vecs<T>(g);
   int rl, q;
@@ -382,17 +413,22 @@ __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G
     s = double(v.p[e]) * double(y);
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
-}
 
-// x += alpha p, r -= alpha q, partial r . r
+}
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_axpy_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int inner, double* __restrict__ partial) {
+__global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  pcg_spmv_tile<T>(g, tl, inner, partial);
+}
+
+// x += alpha p, r -= alpha q, partial r . r
+template <typename T>
+__device__ __forceinline__ void axpy_rr_tile(const GraphDev& g, const Tile& tl, int inner, double* __restrict__ partial) {
   const PcgVecs<T> v = pcg_vecs<T>(g);
   const double alpha = g.st->alpha;
   int rl, q;
@@ -407,17 +443,22 @@ __global__ void __launch_bounds__(kWG) k_axpy_rr(const GraphDev* __restrict__ G,
     s = double(r) * double(r);
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
-}
 
-// p = z + beta p
+}
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_update_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int inner) {
+__global__ void __launch_bounds__(kWG) k_axpy_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                 int ntiles, int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  axpy_rr_tile<T>(g, tl, inner, partial);
+}
+
+// p = z + beta p
+template <typename T>
+__device__ __forceinline__ void update_p_tile(const GraphDev& g, const Tile& tl, int inner) {
   const PcgVecs<T> v = pcg_vecs<T>(g);
   const double beta = g.st->beta;
   int rl, q;
@@ -425,17 +466,22 @@ __global__ void __launch_bounds__(kWG) k_update_p(const GraphDev* __restrict__ G
   if (rl < 0 || rl >= tl.nrows) return;
   const size_t e = 6 * size_t(tl.row0 + rl) + q;
   v.p[e] = T(double(v.z[e]) + beta * double(v.p[e]));
-}
 
-// partial r . z (after the preconditioner)
+}
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_dot_rz(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                int ntiles, int inner, double* __restrict__ partial) {
+__global__ void __launch_bounds__(kWG) k_update_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                  int ntiles, int inner) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  update_p_tile<T>(g, tl, inner);
+}
+
+// partial r . z (after the preconditioner)
+template <typename T>
+__device__ __forceinline__ void dot_rz_tile(const GraphDev& g, const Tile& tl, int inner, double* __restrict__ partial) {
   const PcgVecs<T> v = pcg_vecs<T>(g);
   int rl, q;
   lane_row(rl, q);
@@ -446,17 +492,22 @@ __global__ void __launch_bounds__(kWG) k_dot_rz(const GraphDev* __restrict__ G, 
     s = double(v.r[e]) * double(v.z[e]);
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
-}
 
-// partial r . r
+}
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_dot_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+__global__ void __launch_bounds__(kWG) k_dot_rz(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
                                                 int ntiles, int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || !live(g, inner)) return;
+  dot_rz_tile<T>(g, tl, inner, partial);
+}
+
+// partial r . r
+template <typename T>
+__device__ __forceinline__ void dot_rr_tile(const GraphDev& g, const Tile& tl, int inner, double* __restrict__ partial) {
   const PcgVecs<T> v = pcg_vecs<T>(g);
   int rl, q;
   lane_row(rl, q);
@@ -467,6 +518,17 @@ __global__ void __launch_bounds__(kWG) k_dot_rr(const GraphDev* __restrict__ G, 
     s = double(r) * double(r);
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
+
+}
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_dot_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, inner)) return;
+  dot_rr_tile<T>(g, tl, inner, partial);
 }
 
 // y = A x over one matrix (no graph table): tests and the bandwidth probe
@@ -487,13 +549,7 @@ __global__ void __launch_bounds__(kWG) k_spmv_plain(const int32_t* __restrict__ 
 // ---- start of a solve ------------------------------------------------------------------------
 
 // r = b - K x (FP64), partials b . b and r . r
-__global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                        int ntiles, double* __restrict__ partial) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (!live(g, SERVE_NEW)) return;
+__device__ __forceinline__ void begin_residual_tile(const GraphDev& g, const Tile& tl, double* __restrict__ partial) {
   const LevelDev& L = g.lv[0];
   int rl, q;
   lane_row(rl, q);
@@ -509,20 +565,24 @@ __global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restri
     sr = r * r;
   }
   finish_partials(sb, sr, partial, tl.slot, true);
+
+}
+__global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                        int ntiles, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, SERVE_NEW)) return;
+  begin_residual_tile(g, tl, partial);
 }
 
 // the solve's start, per graph: |b|, a zero or non-finite load (x = 0, r = 0); a mixed-precision
 // graph goes on to its first refinement step
-__global__ void __launch_bounds__(kWG) k_finish_begin(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, int mixed, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
-  const GraphDev& g = G[blockIdx.x];
+__device__ __forceinline__ void finish_begin_g(const GraphDev& g, int k0, int k1, int mixed, const double* __restrict__ partial) {
   PcgState* s = g.st;
   if (!live(g, SERVE_NEW)) return;
-  __shared__ double lds[8];
-  double a = 0.0;
-  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k];
-  const double bb = block_sum(a, lds);
+  const double bb = graph_sum(k0, k1, partial, 0);
   if (threadIdx.x == 0) {
     s->bn = sqrt(bb);
     s->target = s->rtol * s->bn;
@@ -530,32 +590,35 @@ __global__ void __launch_bounds__(kWG) k_finish_begin(const GraphDev* __restrict
     if (mixed) s->phase = PH_OUTER;
   }
 }
+__global__ void __launch_bounds__(kWG) k_finish_begin(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                      int count, int mixed, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count || G[blockIdx.x].resident) return;
+  finish_begin_g(G[blockIdx.x], gtile[blockIdx.x], gtile[blockIdx.x + 1], mixed, partial);
+}
 
 // x = 0, r = 0 where the load is zero or not finite
-__global__ void __launch_bounds__(kWG) k_zero_load(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                   int ntiles) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (g.st->state != ST_RUNNING || !g.st->zero_load || (g.st->phase != PH_NEW && g.st->phase != PH_OUTER)) return;
+__device__ __forceinline__ void zero_load_tile(const GraphDev& g, const Tile& tl) {
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
   const size_t e = 6 * size_t(tl.row0 + rl) + q;
   g.x[e] = 0.0;
   g.r[e] = 0.0;
-}
 
-// p = z, partials r . z and r . r (the start of the conjugate directions)
-template <typename T>
-__global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int inner, double* __restrict__ partial) {
+}
+__global__ void __launch_bounds__(kWG) k_zero_load(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                   int ntiles) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, inner)) return;
+  if (g.resident || g.st->state != ST_RUNNING || !g.st->zero_load || (g.st->phase != PH_NEW && g.st->phase != PH_OUTER)) return;
+  zero_load_tile(g, tl);
+}
+
+// p = z, partials r . z and r . r (the start of the conjugate directions)
+template <typename T>
+__device__ __forceinline__ void start_p_tile(const GraphDev& g, const Tile& tl, int inner, double* __restrict__ partial) {
   const PcgVecs<T> v = pcg_vecs<T>(g);
   int rl, q;
   lane_row(rl, q);
@@ -569,16 +632,21 @@ __global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G,
     s1 = double(r) * double(r);
   }
   finish_partials(s0, s1, partial, tl.slot, true);
+
+}
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                 int ntiles, int inner, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, inner)) return;
+  start_p_tile<T>(g, tl, inner, partial);
 }
 
 // ---- per-graph finishes (one workgroup a graph; partials summed in tile order) ----------------
 
-__device__ __forceinline__ double graph_sum(const int32_t* gtile, const double* partial, int which) {
-  __shared__ double lds[4];
-  double a = 0.0;
-  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k + which];
-  return block_sum(a, lds);
-}
 
 __device__ __forceinline__ PcgState* finish_state(const GraphDev& g, int mode, bool& go) {
   go = live(g, mode);
@@ -588,14 +656,12 @@ __device__ __forceinline__ bool inner_mode(int mode) { return (mode & SERVE_INNE
 
 // after p = z: rz, the first residual check (StressProblem::begin / the head of iterate); the
 // graph goes on to iterate
-__global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, int mode, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
+__device__ __forceinline__ void finish_start_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial) {
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], mode, go);
+  PcgState* s = finish_state(g, mode, go);
   if (!go) return;
-  const double rz = graph_sum(gtile, partial, 0);
-  const double rr = graph_sum(gtile, partial, 1);
+  const double rz = graph_sum(k0, k1, partial, 0);
+  const double rr = graph_sum(k0, k1, partial, 1);
   if (threadIdx.x != 0) return;
   s->rz = rz;
   const double rn = sqrt(rr);
@@ -622,16 +688,19 @@ __global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict
     return;
   }
   s->rel_res = s->bn > 0 ? rn / s->bn : 0.0;
-  if (rn <= s->target || !(s->bn > 0) || G[blockIdx.x].nfree == 0) s->state = ST_CONVERGED;
+  if (rn <= s->target || !(s->bn > 0) || g.nfree == 0) s->state = ST_CONVERGED;
+}
+__global__ void __launch_bounds__(kWG) k_finish_start(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                         int count, int mode, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count || G[blockIdx.x].resident) return;
+  finish_start_g(G[blockIdx.x], gtile[blockIdx.x], gtile[blockIdx.x + 1], mode, partial);
 }
 
-__global__ void __launch_bounds__(kWG) k_finish_alpha(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, int mode, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
+__device__ __forceinline__ void finish_alpha_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial) {
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], mode, go);
+  PcgState* s = finish_state(g, mode, go);
   if (!go) return;
-  const double pq = graph_sum(gtile, partial, 0);
+  const double pq = graph_sum(k0, k1, partial, 0);
   if (threadIdx.x != 0) return;
   // (the iteration budget is checked at an iteration's head: a stopped run resumes where it was)
   if (inner_mode(mode)) {
@@ -660,14 +729,17 @@ __global__ void __launch_bounds__(kWG) k_finish_alpha(const GraphDev* __restrict
   s->pq = pq;
   s->alpha = s->rz / pq;
 }
+__global__ void __launch_bounds__(kWG) k_finish_alpha(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                         int count, int mode, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count || G[blockIdx.x].resident) return;
+  finish_alpha_g(G[blockIdx.x], gtile[blockIdx.x], gtile[blockIdx.x + 1], mode, partial);
+}
 
-__global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                   int count, int mode, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
+__device__ __forceinline__ void finish_rn_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial) {
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], mode, go);
+  PcgState* s = finish_state(g, mode, go);
   if (!go) return;
-  const double rr = graph_sum(gtile, partial, 0);
+  const double rr = graph_sum(k0, k1, partial, 0);
   if (threadIdx.x != 0) return;
   const double rn = sqrt(rr);
   if (inner_mode(mode)) {
@@ -687,14 +759,17 @@ __global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ 
   s->rel_res = rn / s->bn;
   if (rn <= s->target) s->state = ST_CONVERGED;
 }
+__global__ void __launch_bounds__(kWG) k_finish_rn(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                         int count, int mode, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count || G[blockIdx.x].resident) return;
+  finish_rn_g(G[blockIdx.x], gtile[blockIdx.x], gtile[blockIdx.x + 1], mode, partial);
+}
 
-__global__ void __launch_bounds__(kWG) k_finish_beta(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                     int count, int mode, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
+__device__ __forceinline__ void finish_beta_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial) {
   bool go;
-  PcgState* s = finish_state(G[blockIdx.x], mode, go);
+  PcgState* s = finish_state(g, mode, go);
   if (!go) return;
-  const double rz1 = graph_sum(gtile, partial, 0);
+  const double rz1 = graph_sum(k0, k1, partial, 0);
   if (threadIdx.x != 0) return;
   if (!(rz1 > 0)) {
     if (inner_mode(mode)) {
@@ -708,17 +783,16 @@ __global__ void __launch_bounds__(kWG) k_finish_beta(const GraphDev* __restrict_
   s->beta = rz1 / s->rz;
   s->rz = rz1;
 }
+__global__ void __launch_bounds__(kWG) k_finish_beta(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                         int count, int mode, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count || G[blockIdx.x].resident) return;
+  finish_beta_g(G[blockIdx.x], gtile[blockIdx.x], gtile[blockIdx.x + 1], mode, partial);
+}
 
 // ---- mixed precision: the refinement steps --------------------------------------------------
 
 // r = b - K x in FP64, partial r . r (the refinement's true residual)
-__global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                        int ntiles, double* __restrict__ partial) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (!live(g, SERVE_REFINE)) return;
+__device__ __forceinline__ void outer_residual_tile(const GraphDev& g, const Tile& tl, double* __restrict__ partial) {
   const LevelDev& L = g.lv[0];
   int rl, q;
   lane_row(rl, q);
@@ -732,15 +806,23 @@ __global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restri
     sr = r * r;
   }
   finish_partials(sr, 0.0, partial, tl.slot, false);
+
+}
+__global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                        int ntiles, double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, SERVE_REFINE)) return;
+  outer_residual_tile(g, tl, partial);
 }
 
 // outer check: converged, or an inner solve of K d = r to max(eta |r|, target / 2)
-__global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
-                                                      int count, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
-  PcgState* s = G[blockIdx.x].st;
-  if (!live(G[blockIdx.x], SERVE_REFINE)) return;
-  const double rr = graph_sum(gtile, partial, 0);
+__device__ __forceinline__ void finish_outer_g(const GraphDev& g, int k0, int k1, const double* __restrict__ partial) {
+  PcgState* s = g.st;
+  if (!live(g, SERVE_REFINE)) return;
+  const double rr = graph_sum(k0, k1, partial, 0);
   if (threadIdx.x != 0) return;
   const double rn = sqrt(rr);
   s->rn = rn;
@@ -751,7 +833,7 @@ __global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict
     return;
   }
   s->rel_res = s->bn > 0 ? rn / s->bn : 0.0;
-  if (rn <= s->target || !(s->bn > 0) || G[blockIdx.x].nfree == 0) {
+  if (rn <= s->target || !(s->bn > 0) || g.nfree == 0) {
     s->state = ST_CONVERGED;
     return;
   }
@@ -772,6 +854,11 @@ __global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict
   s->phase = to64 ? PH_WAIT64 : PH_START;
   ++s->outer;
 }
+__global__ void __launch_bounds__(kWG) k_finish_outer(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                      int count, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count || G[blockIdx.x].resident) return;
+  finish_outer_g(G[blockIdx.x], gtile[blockIdx.x], gtile[blockIdx.x + 1], partial);
+}
 
 // graphs waiting for FP64-product inner solves start them (the host launches those from now on)
 __global__ void k_release_wait64(const GraphDev* __restrict__ G, int count) {
@@ -782,62 +869,256 @@ __global__ void k_release_wait64(const GraphDev* __restrict__ G, int count) {
 }
 
 // inner start: r_i = (float) r, x_i = 0
-__global__ void __launch_bounds__(kWG) k_inner_init(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                    int ntiles) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (!live(g, SERVE_START)) return;
+__device__ __forceinline__ void inner_init_tile(const GraphDev& g, const Tile& tl) {
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
   const size_t e = 6 * size_t(tl.row0 + rl) + q;
   g.ir[e] = float(g.r[e]);
   g.ix[e] = 0.0f;
-}
 
-// the FP64-product inner solves' preconditioner input and output: r_i = (float) r, z = (double) z_i
-__global__ void __launch_bounds__(kWG) k_convert64(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                   int ntiles, int to32) {
+}
+__global__ void __launch_bounds__(kWG) k_inner_init(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                    int ntiles) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  if (!live(g, to32 ? (SERVE_P64 | SERVE_IT) : SERVE_INNER64)) return;
+  if (g.resident || !live(g, SERVE_START)) return;
+  inner_init_tile(g, tl);
+}
+
+// the FP64-product inner solves' preconditioner input and output: r_i = (float) r, z = (double) z_i
+__device__ __forceinline__ void convert64_tile(const GraphDev& g, const Tile& tl, int to32) {
   int rl, q;
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
   const size_t e = 6 * size_t(tl.row0 + rl) + q;
   if (to32) g.ir[e] = float(g.r[e]);
   else g.z[e] = double(g.iz[e]);
+
+}
+__global__ void __launch_bounds__(kWG) k_convert64(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                   int ntiles, int to32) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, to32 ? (SERVE_P64 | SERVE_IT) : SERVE_INNER64)) return;
+  convert64_tile(g, tl, to32);
 }
 
 // x += (double) x_i after an inner solve; the inner iterations count towards the solve's
+__device__ __forceinline__ void inner_apply_tile(const GraphDev& g, const Tile& tl) {
+  // (an FP64-product inner solve updated x itself)
+  int rl, q;
+  lane_row(rl, q);
+  if (rl < 0 || rl >= tl.nrows) return;
+  const size_t e = 6 * size_t(tl.row0 + rl) + q;
+  g.x[e] += double(g.ix[e]);
+
+}
 __global__ void __launch_bounds__(kWG) k_inner_apply(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
                                                      int ntiles) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
-  // (an FP64-product inner solve updated x itself)
-  if (g.st->state != ST_RUNNING || g.st->phase != PH_APPLY || g.st->inner64) return;
-  int rl, q;
-  lane_row(rl, q);
-  if (rl < 0 || rl >= tl.nrows) return;
-  const size_t e = 6 * size_t(tl.row0 + rl) + q;
-  g.x[e] += double(g.ix[e]);
+  if (g.resident || g.st->state != ST_RUNNING || g.st->phase != PH_APPLY || g.st->inner64) return;
+  inner_apply_tile(g, tl);
 }
 
-__global__ void k_inner_close(const GraphDev* __restrict__ G, int count) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= count) return;
-  PcgState* s = G[k].st;
+// the inner iterations count towards the solve's; the graph goes on to its next refinement step
+__device__ __forceinline__ void inner_close_g(const GraphDev& g) {
+  PcgState* s = g.st;
   if (s->phase != PH_APPLY) return;
   s->iters += s->inner_iters;
   if (s->inner_state == ST_BREAKDOWN && s->inner_iters == 0) s->state = ST_BREAKDOWN;
   s->inner_state = ST_IDLE;
   s->phase = PH_OUTER;
+}
+__global__ void k_inner_close(const GraphDev* __restrict__ G, int count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count || G[k].resident) return;
+  inner_close_g(G[k]);
+}
+
+// ---- resident graphs: whole steps in one workgroup ------------------------------------------
+
+// tile k of n rows, `per` rows a tile, partial slots from slot0
+__device__ __forceinline__ Tile tile_of(int gi, int k, int n, int per, int slot0) {
+  const int r0 = k * per;
+  return Tile{gi, r0, min(per, n - r0), slot0 + k};
+}
+
+template <typename T, int S>
+__device__ void smooth_g(const GraphDev& g, int gi, int l, int st, int in_buf, int out_buf, bool fresh, int mode) {
+  const LevelDev& L = g.lv[l];
+  const int per = tile_rows<S>(), nt = (L.n + per - 1) / per;
+  for (int k = 0; k < nt; ++k) {
+    const Tile tl = tile_of(gi, k, L.n, per, 0);
+    if (fresh) cheb_step_tile<T, S, true>(g, tl, l, st, in_buf, out_buf, mode, nullptr);
+    else cheb_step_tile<T, S, false>(g, tl, l, st, in_buf, out_buf, mode, nullptr);
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void smooth_any(const GraphDev& g, int gi, int l, int st, int in_buf, int out_buf, bool fresh, int mode) {
+  if (g.lv[l].sub == 10) smooth_g<T, 10>(g, gi, l, st, in_buf, out_buf, fresh, mode);
+  else smooth_g<T, 1>(g, gi, l, st, in_buf, out_buf, fresh, mode);
+}
+
+template <typename T, int S>
+__device__ void residual_g(const GraphDev& g, int gi, int l, int buf, int mode) {
+  const LevelDev& L = g.lv[l];
+  const int per = tile_rows<S>(), nt = (L.n + per - 1) / per;
+  for (int k = 0; k < nt; ++k) residual_tile<T, S>(g, tile_of(gi, k, L.n, per, 0), l, buf, mode);
+  __syncthreads();
+}
+
+template <typename T, int S>
+__device__ void restrict_g(const GraphDev& g, int gi, int l, int mode) {
+  const int nc = g.lv[l].nc;
+  const int per = tile_rows<S>(), nt = (nc + per - 1) / per;
+  for (int k = 0; k < nt; ++k) restrict_tile<T, S>(g, tile_of(gi, k, nc, per, 0), l, mode);
+  __syncthreads();
+}
+
+// the V-cycle of one graph (vcycle<T> of the host driver, restricted to this graph's levels)
+template <typename T>
+__device__ void vcycle_g(const GraphDev& g, int gi, int mode, int degree) {
+  if (g.jacobi) {
+    const int nt = (g.nfree + kTileRows - 1) / kTileRows;
+    for (int k = 0; k < nt; ++k) jacobi_tile<T>(g, tile_of(gi, k, g.nfree, kTileRows, 0), mode);
+    __syncthreads();
+    return;
+  }
+  for (int l = 0; l < g.nlev; ++l) {
+    const LevelDev& L = g.lv[l];
+    if (!L.terminal) {
+      for (int st = 0; st < degree; ++st) smooth_any<T>(g, gi, l, st, (st + 1) & 1, st & 1, st == 0, mode);
+      if (L.sub == 10) residual_g<T, 10>(g, gi, l, (degree - 1) & 1, mode);
+      else residual_g<T, 1>(g, gi, l, (degree - 1) & 1, mode);
+      if (L.rsub == 10) restrict_g<T, 10>(g, gi, l, mode);
+      else restrict_g<T, 1>(g, gi, l, mode);
+    } else if (L.dense_m > 0) {
+      dense_solve_g<T>(g, l);
+      __syncthreads();
+    } else {
+      for (int st = 0; st < kStallDegree; ++st) smooth_any<T>(g, gi, l, st, (st + 1) & 1, st & 1, st == 0, mode);
+    }
+  }
+  for (int l = g.nlev - 2; l >= 0; --l) {
+    const LevelDev& L = g.lv[l];
+    const int nt = (L.n + kTileRows - 1) / kTileRows;
+    for (int k = 0; k < nt; ++k) prolong_tile<T>(g, tile_of(gi, k, L.n, kTileRows, 0), l, (degree - 1) & 1, mode);
+    __syncthreads();
+    for (int j = 0; j < degree; ++j) {
+      const int gs = degree + j;
+      smooth_any<T>(g, gi, l, j, (gs + 1) & 1, gs & 1, false, mode);
+    }
+  }
+}
+
+// the graph's level-0 tiles (kTileRows rows; partial slots k0 ..)
+#define SVXG_T0_LOOP(call)                                                          \
+  do {                                                                               \
+    for (int k = 0; k < nt0; ++k) {                                                  \
+      const Tile tl = tile_of(gi, k, g.nfree, kTileRows, k0);                       \
+      call;                                                                          \
+    }                                                                                \
+    __syncthreads();                                                                 \
+  } while (0)
+
+__global__ void __launch_bounds__(kWG) k_resident_steps(const GraphDev* __restrict__ G, const int32_t* __restrict__ gtile,
+                                                        const int32_t* __restrict__ graphs, int count, int nsteps,
+                                                        int degree, int mixed, int p32, int p64,
+                                                        double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  const int gi = graphs[blockIdx.x];
+  const GraphDev& g = G[gi];
+  const int k0 = gtile[gi], k1 = gtile[gi + 1], nt0 = k1 - k0;
+  for (int step = 0; step < nsteps; ++step) {
+    if (g.st->state != ST_RUNNING) return;
+    // a solve's start (run_begin)
+    if (live(g, SERVE_NEW)) {
+      SVXG_T0_LOOP(begin_residual_tile(g, tl, partial));
+      finish_begin_g(g, k0, k1, mixed, partial);
+      __syncthreads();
+      SVXG_T0_LOOP(if (g.st->state == ST_RUNNING && g.st->zero_load &&
+                       (g.st->phase == PH_NEW || g.st->phase == PH_OUTER)) zero_load_tile(g, tl));
+      if (!mixed) {
+        vcycle_g<double>(g, gi, SERVE_NEW, degree);
+        SVXG_T0_LOOP(if (live(g, SERVE_NEW)) start_p_tile<double>(g, tl, SERVE_NEW, partial));
+        finish_start_g(g, k0, k1, SERVE_NEW, partial);
+        __syncthreads();
+      }
+    }
+    if (!mixed) {
+      // fp64_step
+      if (!live(g, SERVE_OUTER)) continue;
+      SVXG_T0_LOOP(pcg_spmv_tile<double>(g, tl, SERVE_OUTER, partial));
+      finish_alpha_g(g, k0, k1, SERVE_OUTER, partial);
+      __syncthreads();
+      if (live(g, SERVE_OUTER)) SVXG_T0_LOOP(axpy_rr_tile<double>(g, tl, SERVE_OUTER, partial));
+      finish_rn_g(g, k0, k1, SERVE_OUTER, partial);
+      __syncthreads();
+      if (live(g, SERVE_OUTER)) vcycle_g<double>(g, gi, SERVE_OUTER, degree);
+      if (live(g, SERVE_OUTER)) SVXG_T0_LOOP(dot_rz_tile<double>(g, tl, SERVE_OUTER, partial));
+      finish_beta_g(g, k0, k1, SERVE_OUTER, partial);
+      __syncthreads();
+      if (live(g, SERVE_OUTER)) SVXG_T0_LOOP(update_p_tile<double>(g, tl, SERVE_OUTER));
+      continue;
+    }
+    // mixed_step
+    if (live(g, SERVE_REFINE)) {
+      SVXG_T0_LOOP(outer_residual_tile(g, tl, partial));
+      finish_outer_g(g, k0, k1, partial);
+      __syncthreads();
+    }
+    if (live(g, SERVE_START)) SVXG_T0_LOOP(inner_init_tile(g, tl));
+    if (p32 && live(g, SERVE_P32 | SERVE_IT)) SVXG_T0_LOOP(pcg_spmv_tile<float>(g, tl, SERVE_P32 | SERVE_IT, partial));
+    if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(pcg_spmv_tile<double>(g, tl, SERVE_P64 | SERVE_IT, partial));
+    finish_alpha_g(g, k0, k1, SERVE_ITER, partial);
+    __syncthreads();
+    if (p32 && live(g, SERVE_P32 | SERVE_IT)) SVXG_T0_LOOP(axpy_rr_tile<float>(g, tl, SERVE_P32 | SERVE_IT, partial));
+    if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(axpy_rr_tile<double>(g, tl, SERVE_P64 | SERVE_IT, partial));
+    finish_rn_g(g, k0, k1, SERVE_ITER, partial);
+    __syncthreads();
+    if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(convert64_tile(g, tl, 1));
+    if (live(g, SERVE_INNER)) vcycle_g<float>(g, gi, SERVE_INNER, degree);
+    if (p64 && live(g, SERVE_INNER64)) SVXG_T0_LOOP(convert64_tile(g, tl, 0));
+    if (p32 && live(g, SERVE_P32 | SERVE_IT)) SVXG_T0_LOOP(dot_rz_tile<float>(g, tl, SERVE_P32 | SERVE_IT, partial));
+    if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(dot_rz_tile<double>(g, tl, SERVE_P64 | SERVE_IT, partial));
+    finish_beta_g(g, k0, k1, SERVE_ITER, partial);
+    __syncthreads();
+    if (p32 && live(g, SERVE_P32 | SERVE_IT)) SVXG_T0_LOOP(update_p_tile<float>(g, tl, SERVE_P32 | SERVE_IT));
+    if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(update_p_tile<double>(g, tl, SERVE_P64 | SERVE_IT));
+    if (p32 && live(g, SERVE_P32 | SERVE_ST)) SVXG_T0_LOOP(start_p_tile<float>(g, tl, SERVE_P32 | SERVE_ST, partial));
+    if (p64 && live(g, SERVE_P64 | SERVE_ST)) SVXG_T0_LOOP(start_p_tile<double>(g, tl, SERVE_P64 | SERVE_ST, partial));
+    finish_start_g(g, k0, k1, SERVE_START, partial);
+    __syncthreads();
+    if (g.st->state == ST_RUNNING && g.st->phase == PH_APPLY && !g.st->inner64) SVXG_T0_LOOP(inner_apply_tile(g, tl));
+    if (threadIdx.x == 0) inner_close_g(g);
+    __syncthreads();
+  }
+}
+#undef SVXG_T0_LOOP
+
+// the current residual of each listed resident graph (the head of a resumed FP64 iterate)
+__global__ void __launch_bounds__(kWG) k_resident_recheck(const GraphDev* __restrict__ G,
+                                                          const int32_t* __restrict__ gtile,
+                                                          const int32_t* __restrict__ graphs, int count,
+                                                          double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  const int gi = graphs[blockIdx.x];
+  const GraphDev& g = G[gi];
+  if (!live(g, SERVE_OUTER)) return;
+  const int k0 = gtile[gi], k1 = gtile[gi + 1];
+  for (int k = 0; k < k1 - k0; ++k) dot_rr_tile<double>(g, tile_of(gi, k, g.nfree, kTileRows, k0), SERVE_OUTER, partial);
+  __syncthreads();
+  finish_rn_g(g, k0, k1, SERVE_OUTER, partial);
 }
 
 // ---- host launchers ----------------------------------------------------------------------------
@@ -990,6 +1271,16 @@ void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int
 void launch_convert64(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int to32) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_convert64, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, to32);
+}
+void launch_resident_steps(hipStream_t s, const GraphDev* G, const int32_t* gtile, const int32_t* graphs, int count,
+                           int nsteps, int degree, int mixed, int p32, int p64, double* partial) {
+  if (count <= 0 || nsteps <= 0) return;
+  hipLaunchKernelGGL(k_resident_steps, dim3(count), dim3(kWG), 0, s, G, gtile, graphs, count, nsteps, degree, mixed,
+                     p32, p64, partial);
+}
+void launch_resident_recheck(hipStream_t s, const GraphDev* G, const int32_t* gtile, const int32_t* graphs, int count,
+                             double* partial) {
+  if (count > 0) hipLaunchKernelGGL(k_resident_recheck, dim3(count), dim3(kWG), 0, s, G, gtile, graphs, count, partial);
 }
 void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   if (count <= 0) return;

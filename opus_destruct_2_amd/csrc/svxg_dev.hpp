@@ -112,6 +112,8 @@ struct LevelDev {
   const double* Minv;    // dense inverse of the coarsest level (dense_m x dense_m, row-major)
   double lmax;           // lambda_max(D^-1 A) estimate (x 1.1)
   int32_t cheb_degree;
+  int32_t sub;           // six-lane sub-groups a row in the tiled launches of this level (1 or 10)
+  int32_t rsub;          // the same for its restriction (coarse rows of the next level)
   int32_t pad_;
   LevelArrays<double> f64;
   LevelArrays<float> f32;
@@ -121,7 +123,7 @@ struct GraphDev {
   int32_t nlev;
   int32_t nfree;
   int32_t jacobi;   // block-Jacobi preconditioner (one level, no hierarchy)
-  int32_t pad_;
+  int32_t resident; // small graph: its steps run in one workgroup (k_resident_steps), not in the tiled launches
   PcgState* st;
   // outer / FP64 PCG vectors, 6 nfree each
   double *x, *r, *z, *p, *q, *b;

@@ -7,6 +7,7 @@ namespace svxg {
 
 // Chebyshev smoothing interval [lmax / kChebRatio, lmax]
 constexpr double kChebRatio = 30.0;
+constexpr int kStallDegree = 8;  // Chebyshev steps of a coarsest level left without a dense solve
 
 enum class FinishKind { Start, Alpha, Rn, Beta, Outer };
 
@@ -57,6 +58,13 @@ void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int 
 void launch_prof_account(hipStream_t s, const GraphDev* G, int count, int inner, const double* gbytes, double* acc);
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_inner_close(hipStream_t s, const GraphDev* G, int count);
+// nsteps whole steps (a solve start first, for graphs just admitted) of each listed resident graph,
+// one workgroup a graph: the same device functions, on the same tiles, in the same order as the
+// tiled launches (mixed: mixed_step / fp64_step of the host driver), so the same bits
+void launch_resident_steps(hipStream_t s, const GraphDev* G, const int32_t* gtile, const int32_t* graphs, int count,
+                           int nsteps, int degree, int mixed, int p32, int p64, double* partial);
+void launch_resident_recheck(hipStream_t s, const GraphDev* G, const int32_t* gtile, const int32_t* graphs, int count,
+                             double* partial);
 // FP64-product inner solves: r_i = (float) r (to32 = 1) or z = (double) z_i (to32 = 0)
 void launch_convert64(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int to32);
 // plain y = A x on one graph's level (FP64 or FP32 copy), tests and the roofline probe

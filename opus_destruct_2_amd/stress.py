@@ -64,6 +64,7 @@ def lib():
             "svxg_problem_set_options": ([vp, vp], ctypes.c_int),
             "svxg_assemble_batch": ([vp, vp, i32, vp], ctypes.c_int),
             "svxg_ctx_set_profiling": ([vp, i32], ctypes.c_int),
+            "svxg_ctx_set_resident_rows": ([vp, i32], ctypes.c_int),
             "svxg_ctx_kernel_stats": ([vp, i32, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)],
                                       ctypes.c_int),
             "svxg_server_default_options": ([vp], None),
@@ -112,6 +113,10 @@ class Context:
 
     def bytes_in_use(self):
         return lib().svxg_ctx_bytes_in_use(self.h)
+
+    def set_resident_rows(self, rows):
+        """Graphs up to `rows` free nodes step in one workgroup each (0: none); results are unchanged."""
+        _check(lib().svxg_ctx_set_resident_rows(self.h, int(rows)), "set_resident_rows")
 
     def set_profiling(self, on):
         _check(lib().svxg_ctx_set_profiling(self.h, int(bool(on))), "set_profiling")

@@ -69,3 +69,27 @@ def test_batch_warm_start_and_resident_loads(mats, ctx):
     us2, rs2 = stress.solve_batch(probs, [g.f for g in gs], us=us, rtol=1e-9, warm=True)
     for k in range(len(gs)):
         assert rs2[k]["converged"] and rs2[k]["iters"] <= 1
+
+
+@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
+def test_resident_steps_equal_tiled_steps_bitwise(precision, mats):
+    """Graphs stepped in one workgroup each give the bits the tiled launches give (the same device
+    functions on the same tiles in the same order)."""
+    gs = graphs()[:10]
+    out = []
+    for rows in (0, 4096):
+        c = stress.Context(0)
+        c.set_resident_rows(rows)
+        opts = stress.default_options(precision=precision, amg_min_nodes=0 if rows else 160)
+        opts.amg_min_nodes = 0
+        probs = [stress.StressProblem(g.nodes, g.bonds, mats, c, options=opts) for g in gs]
+        us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-9, maxit=5000)
+        us2, rs2 = stress.solve_batch(probs, [g.f for g in gs], us=us, rtol=1e-11, maxit=5000, warm=True)
+        out.append((us, rs, us2, rs2))
+        for p in probs:
+            p.close()
+        c.close()
+    (a, ra, a2, ra2), (b, rb, b2, rb2) = out
+    for k in range(len(gs)):
+        assert ra[k] == rb[k] and ra2[k] == rb2[k], (SIZES[k], ra[k], rb[k])
+        assert np.array_equal(bits(a[k]), bits(b[k])) and np.array_equal(bits(a2[k]), bits(b2[k])), SIZES[k]
