@@ -36,11 +36,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+AMG_KW = {}  # (device multigrid options of every leg, from the command line)
+
+
 def run_path(g, mats, precision, steps, warmup, ctx):
     from opus_destruct_2_amd import stress
     p = stress.StressProblem(g.nodes, g.bonds, mats, ctx)
     t = time.perf_counter()
-    assert p.assemble(stress.default_options(precision=precision))
+    assert p.assemble(stress.default_options(precision=precision, **AMG_KW))
     setup_ms = (time.perf_counter() - t) * 1e3
     info = p.info()
     log(f"  precision={precision} setup {setup_ms:.1f} ms levels {info['level_rows']} blocks {info['level_blocks']}")
@@ -126,7 +129,7 @@ def run_batch(mats, ctx, count, steps, sample_plain):
     dofs = sum(g.dofs for g in gs)
     log(f"batched: {count} graphs, {sum(g.n for g in gs)} nodes, {dofs} DOFs (sizes {sizes.min()}..{sizes.max()}, "
         f"median {int(np.median(sizes))})")
-    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0)
+    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0, **AMG_KW)
     probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
     t = time.perf_counter()
     ok = stress.assemble_batch(probs)
@@ -147,7 +150,7 @@ def run_batch(mats, ctx, count, steps, sample_plain):
     # the plain path: FP64, one graph at a time
     pick = list(range(0, count, max(1, count // sample_plain)))[:sample_plain]
     pdofs, pms = 0, 0.0
-    popt = stress.default_options(precision=SVXG_FP64, amg_min_nodes=0)
+    popt = stress.default_options(precision=SVXG_FP64, amg_min_nodes=0, **AMG_KW)
     for k in pick:
         g = gs[k]
         p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=popt)
@@ -327,6 +330,9 @@ def main():
     ap.add_argument("--serve-warmup", type=int, default=30)
     ap.add_argument("--serve-life", type=int, default=20)
     ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--smooth-levels", type=int, default=None,
+                    help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
+    ap.add_argument("--cheb-degree", type=int, default=None)
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -352,6 +358,10 @@ def main():
     mats = gen.materials()
     g = gen.block(args.nx, args.ny, args.nz, 0.5, 0.15, seed=1)
     log(f"workload {g.name}: {g.n} nodes, {g.m} bonds, {g.dofs} DOFs")
+    if args.smooth_levels is not None:
+        AMG_KW["smooth_levels"] = args.smooth_levels
+    if args.cheb_degree is not None:
+        AMG_KW["cheb_degree"] = args.cheb_degree
     ctx = stress.default_context()
     if args.resident_rows >= 0:
         ctx.set_resident_rows(args.resident_rows)

@@ -92,7 +92,9 @@ typedef struct svxg_amg_options {
   int32_t cheb_degree;       /* device smoother degree (default 2) */
   int32_t coarsen_dense;     /* 1 */
   int32_t reaggregate;       /* 1 */
-  int32_t pad_;
+  int32_t smooth_levels;     /* device: levels whose prolongator is smoothed, from the finest (0: all,
+                                as the reference); coarser ones are plain aggregation (scaled by
+                                coarse_scale), which keeps coarse operators sparse */
 } svxg_amg_options;
 
 /* StressOptions (stress.hpp:68-72). */
