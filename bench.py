@@ -313,6 +313,101 @@ def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11):
     return out
 
 
+def run_city(mats, ctx, count, rounds, rate, seed=21):
+    """City chunk: `count` resident buildings (towers on supports, 5k..150k nodes, log-normal sizes
+    with median 20k, one material a storey) solved together, one batch a round, mixed precision to
+    1e-8 from a cold start under gravity and a wind load that turns every round; the multigrids stay
+    resident. Collapse events arrive at `rate` per second of the timed rounds' wall time: each cuts
+    one building's ground-storey columns under half its footprint (bonds out in place) and re-sets
+    up that building's multigrid at the start of the next round, inside its timed region. After the
+    last round a 5 % sample (every 20th building) is solved on the FP64 path and compared."""
+    from opus_destruct_2_amd import gen, stress
+    from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
+    t = time.perf_counter()
+    shapes = gen.city_shapes(count, seed=seed)
+    gs = [gen.building(*s, seed=seed * 1000 + k) for k, s in enumerate(shapes)]
+    gen_s = time.perf_counter() - t
+    nodes, dofs = sum(g.n for g in gs), sum(g.dofs for g in gs)
+    sizes = np.array([g.n for g in gs])
+    log(f"city: {count} buildings, {nodes} nodes, {dofs} DOFs (sizes {sizes.min()}..{sizes.max()}, median "
+        f"{int(np.median(sizes))}), generated in {gen_s:.1f} s")
+    opts = stress.default_options(precision=SVXG_MIXED, **AMG_KW)
+    probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
+    t = time.perf_counter()
+    assert all(stress.assemble_batch(probs))
+    setup_ms = (time.perf_counter() - t) * 1e3
+    hbm = ctx.bytes_in_use()
+    log(f"  setup of all buildings: {setup_ms:.0f} ms, {hbm / 2**30:.1f} GiB resident")
+    rng = np.random.default_rng(seed)
+    order = [int(k) for k in rng.permutation(count)]
+    angle = lambda r: 2.0 * np.pi * r / 8.0  # noqa: E731
+    t = time.perf_counter()
+    us, rs = stress.solve_batch(probs, [gen.wind_load(g, angle(0)) for g in gs], rtol=1e-8)
+    assert all(r["converged"] for r in rs)
+    due = rate * (time.perf_counter() - t)  # collapses that arrived during the untimed first round
+    log(f"  first round (untimed): {probs[0].last_timing()['solve_ms']:.1f} ms device, iters max "
+        f"{max(r['iters'] for r in rs)}")
+    hit_all, per_round, t_sum = [], [], 0.0
+    for r in range(1, rounds + 1):
+        fs = [gen.wind_load(g, angle(r)) for g in gs]
+        t0 = time.perf_counter()
+        nev = min(int(due), count - len(hit_all))
+        due -= nev
+        hit = order[len(hit_all):len(hit_all) + nev]
+        hit_all += hit
+        for k in hit:
+            probs[k].remove_bonds(gen.ground_storey_cuts(gs[k]))
+        if hit:
+            assert all(stress.assemble_batch([probs[k] for k in hit]))
+        t1 = time.perf_counter()
+        us, rs = stress.solve_batch(probs, fs, rtol=1e-8)
+        t2 = time.perf_counter()
+        assert all(r_["converged"] for r_ in rs)
+        due += rate * (t2 - t0)
+        t_sum += t2 - t0
+        its = [r_["iters"] for r_ in rs]
+        rec = {"collapses": nev, "resetup_ms": round((t1 - t0) * 1e3, 1), "solve_ms_device":
+               round(probs[0].last_timing()["solve_ms"], 1), "wall_ms": round((t2 - t0) * 1e3, 1),
+               "iters_max": max(its), "iters_median": int(np.median(its)),
+               "iters_max_collapsed": max([its[k] for k in hit], default=0)}
+        per_round.append(rec)
+        log(f"  round {r}: {rec}")
+    # the 5 % sample on the FP64 path, same bonds and the last round's loads
+    pick = list(range(0, count, 20))
+    p64 = []
+    for k in pick:
+        p64.append(stress.StressProblem(gs[k].nodes, probs[k].bonds, mats, ctx,
+                                        options=stress.default_options(precision=SVXG_FP64, **AMG_KW)))
+    assert all(stress.assemble_batch(p64))
+    u64, r64 = stress.solve_batch(p64, [fs[k] for k in pick], rtol=1e-8)
+    assert all(r_["converged"] for r_ in r64)
+    u_err, phi_err, failed_m, failed_d, disagree = 0.0, 0.0, 0, 0, 0
+    for j, k in enumerate(pick):
+        scale = np.abs(u64[j]).max()
+        u_err = max(u_err, float(np.abs(us[k] - u64[j]).max() / scale))
+        phi, _ = probs[k].bond_checks(us[k])
+        phi64, _ = p64[j].bond_checks(u64[j])
+        phi_err = max(phi_err, float(np.abs(phi - phi64).max() / max(1.0, np.abs(phi64).max())))
+        a, b = phi >= 1.0, phi64 >= 1.0
+        near = (np.abs(phi - 1.0) <= 1e-4) | (np.abs(phi64 - 1.0) <= 1e-4)
+        failed_m += int(a.sum())
+        failed_d += int(b.sum())
+        disagree += int(((a != b) & ~near).sum())
+        p64[j].close()
+    for p in probs:
+        p.close()
+    solve_dev = float(np.median([x["solve_ms_device"] for x in per_round]))
+    return {"buildings": count, "nodes": nodes, "dofs": dofs, "sizes": [int(sizes.min()), int(np.median(sizes)),
+                                                                         int(sizes.max())],
+            "precision": "mixed", "rtol": 1e-8, "rounds": rounds, "collapse_rate_per_s": rate,
+            "collapses": len(hit_all), "setup_ms": round(setup_ms, 1), "resident_gib": round(hbm / 2**30, 2),
+            "value": round(dofs * rounds / t_sum, 1), "unit": "DOF/s (wall, re-setups and PCIe copies included)",
+            "device_solve_dof_per_s": round(dofs / (solve_dev / 1e3), 1), "per_round": per_round,
+            "fp64_sample": {"buildings": len(pick), "u_rel_err_max": u_err, "phi_rel_err_max": phi_err,
+                            "failed_bonds_mixed": failed_m, "failed_bonds_fp64": failed_d,
+                            "failed_disagreeing_outside_1e-4": disagree}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,6 +425,9 @@ def main():
     ap.add_argument("--serve-warmup", type=int, default=30)
     ap.add_argument("--serve-life", type=int, default=20)
     ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--city", type=int, default=400, help="buildings in the city-chunk leg (0: skip)")
+    ap.add_argument("--city-rounds", type=int, default=3)
+    ap.add_argument("--city-rate", type=float, default=20.0, help="collapse events per second")
     ap.add_argument("--smooth-levels", type=int, default=None,
                     help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
     ap.add_argument("--cheb-degree", type=int, default=None)
@@ -399,6 +497,8 @@ def main():
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
         line["serving"] = run_serving(mats, ctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
                                       args.serve_pool)
+    if args.city > 0:
+        line["city"] = run_city(mats, ctx, args.city, args.city_rounds, args.city_rate)
     if not args.no_cpu:
         log("cpu baseline: the reference's solver on the same workload ...")
         cb = cpu_baseline(g, mats)

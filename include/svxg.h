@@ -146,10 +146,13 @@ int svxg_problem_destroy(svxg_problem* p);
 
 /* StressProblem::assemble (stress.hpp:83; stress.cpp:134-176): K from the intact bonds on the
    device, then block-Jacobi (nfree < amg_min_nodes) or the multigrid hierarchy, built on the
-   device. Returns SVXG_SINGULAR where the reference's assemble returns false. */
+   device. Returns SVXG_SINGULAR where the reference's assemble returns false. A problem set up
+   for SVXG_MIXED keeps its hierarchy in FP32 only (its FP64 steps read K and the coarsest
+   level's FP64 inverse); one set up for SVXG_FP64 keeps both copies. */
 int svxg_problem_assemble(svxg_problem* p, const svxg_stress_options* opt);
 /* The options of the next assemble (StressProblem::opt_, stress.hpp:153), without assembling:
-   svxg_assemble_batch / svxg_solve_batch assemble unassembled problems with them. */
+   svxg_assemble_batch / svxg_solve_batch assemble unassembled problems with them. Switching a
+   problem set up for SVXG_MIXED to SVXG_FP64 makes it unassembled (it needs an FP64 hierarchy). */
 int svxg_problem_set_options(svxg_problem* p, const svxg_stress_options* opt);
 /* StressProblem::release (stress.hpp:99; stress.cpp:326-339). */
 int svxg_problem_release(svxg_problem* p);

@@ -27,13 +27,13 @@ void set_error(const std::string& msg) { g_error = msg; }
 
 // ---- arena ---------------------------------------------------------------------------------------
 
+// size classes, four a power of two from 256 B (256, 320, 384, 448, 512, 640, ...): a block is
+// under a fifth larger than the request that takes it
+static size_t class_size(int c) { return (size_t(4 + (c & 3)) << (c >> 2)) * 64; }
+
 static int size_class(size_t bytes) {
-  size_t s = 256;
   int c = 0;
-  while (s < bytes) {
-    s <<= 1;
-    ++c;
-  }
+  while (class_size(c) < bytes) ++c;
   return c;
 }
 
@@ -64,8 +64,14 @@ int Arena::alloc(size_t bytes, void** out, size_t* held) {
     *held = sz;
     return SVXG_OK;
   }
-  const int c = size_class(bytes);
-  const size_t sz = size_t(256) << c;
+  int c = size_class(bytes);
+  // a free block of this class or of the next two (at most 56 % larger) before new memory
+  for (int k = 0; k < 3 && size_t(c + k) < free_.size(); ++k)
+    if (!free_[size_t(c + k)].empty()) {
+      c += k;
+      break;
+    }
+  const size_t sz = class_size(c);
   if (in_use_.load() + int64_t(sz) > budget_) {
     set_error("HBM budget exceeded");
     return SVXG_OOM_BUDGET;
@@ -181,8 +187,36 @@ static void copy_in_parallel(size_t n, F&& f) {
   for (std::thread& th : pool) th.join();
 }
 
+// Parts of at most kStageCap bytes each: large batches pass through the staging buffers in
+// several round trips instead of growing them to the size of the whole batch
+static constexpr size_t kStageCap = size_t(256) << 20;
+
+struct StagePart {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+
+static std::vector<StagePart> stage_parts(const std::vector<StagePart>& jobs) {
+  std::vector<StagePart> out;
+  out.reserve(jobs.size());
+  for (const StagePart& j : jobs)
+    for (size_t o = 0; o < j.bytes; o += kStageCap)
+      out.push_back({static_cast<char*>(j.dst) + o, static_cast<const char*>(j.src) + o,
+                     std::min(kStageCap, j.bytes - o)});
+  return out;
+}
+
+// the end of the round trip that starts at part k0: parts up to kStageCap bytes in all (one at least)
+static size_t stage_round(const std::vector<StagePart>& parts, size_t k0, size_t* data) {
+  size_t k1 = k0, d = 0;
+  while (k1 < parts.size() && (k1 == k0 || d + round256(parts[k1].bytes) <= kStageCap)) d += round256(parts[k1++].bytes);
+  *data = d;
+  return k1;
+}
+
 // Uploads of a batch (many small arrays): packed into the pinned staging buffer with a table of
-// their destinations, sent in one copy, spread by one kernel, one synchronisation for the lot
+// their destinations, sent in one copy, spread by one kernel, one synchronisation a round trip
 class Stager {
  public:
   Stager(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
@@ -198,43 +232,46 @@ class Stager {
   }
   int flush() {
     if (jobs_.empty()) return SVXG_OK;
-    size_t data = 0;
-    for (const Job& j : jobs_) data += round256(j.bytes);
-    const size_t table = round256(jobs_.size() * sizeof(BulkSeg));
-    svxg_ctx::Lane* L = nullptr;
-    SVXG_TRY(staging(c_, s_, data + table, &L));
-    char* hp = static_cast<char*>(L->pinned);
-    char* dp = L->dstage.as<char>();
-    BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
-    std::vector<size_t> offs(jobs_.size());
-    size_t off = 0;
-    for (size_t k = 0; k < jobs_.size(); ++k) {
-      offs[k] = off;
-      seg[k] = BulkSeg{jobs_[k].dst, dp + off, int64_t(jobs_[k].bytes)};
-      off += round256(jobs_[k].bytes);
-    }
-    copy_in_parallel(jobs_.size(), [&](size_t k) { std::memcpy(hp + offs[k], jobs_[k].src, jobs_[k].bytes); });
-    SVXG_HIP(hipMemcpyAsync(dp, hp, data + jobs_.size() * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
-    launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(jobs_.size()));
-    SVXG_HIP(hipGetLastError());
-    SVXG_HIP(hipStreamSynchronize(s_));
+    const std::vector<StagePart> parts = stage_parts(jobs_);
     jobs_.clear();
+    for (size_t k0 = 0, k1 = 0; k0 < parts.size(); k0 = k1) {
+      size_t data = 0;
+      k1 = stage_round(parts, k0, &data);
+      SVXG_TRY(send(parts, k0, k1, data));
+    }
     return SVXG_OK;
   }
 
  private:
-  struct Job {
-    void* dst;
-    const void* src;
-    size_t bytes;
-  };
+  int send(const std::vector<StagePart>& parts, size_t k0, size_t k1, size_t data) {
+    const size_t n = k1 - k0;
+    svxg_ctx::Lane* L = nullptr;
+    SVXG_TRY(staging(c_, s_, data + round256(n * sizeof(BulkSeg)), &L));
+    char* hp = static_cast<char*>(L->pinned);
+    char* dp = L->dstage.as<char>();
+    BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
+    std::vector<size_t> offs(n);
+    size_t off = 0;
+    for (size_t k = 0; k < n; ++k) {
+      offs[k] = off;
+      seg[k] = BulkSeg{parts[k0 + k].dst, dp + off, int64_t(parts[k0 + k].bytes)};
+      off += round256(parts[k0 + k].bytes);
+    }
+    copy_in_parallel(n, [&](size_t k) { std::memcpy(hp + offs[k], parts[k0 + k].src, parts[k0 + k].bytes); });
+    SVXG_HIP(hipMemcpyAsync(dp, hp, data + n * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
+    launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(n));
+    SVXG_HIP(hipGetLastError());
+    SVXG_HIP(hipStreamSynchronize(s_));
+    return SVXG_OK;
+  }
   svxg_ctx* c_;
   hipStream_t s_;
-  std::vector<Job> jobs_;
+  std::vector<StagePart> jobs_;
 };
 
 // Readbacks of a batch: one kernel packs them into the device staging buffer, one copy brings
-// them over, one synchronisation, then they go to their destinations
+// them over, one synchronisation, then they go to their destinations (a round trip per
+// kStageCap bytes)
 class Gatherer {
  public:
   Gatherer(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
@@ -243,42 +280,44 @@ class Gatherer {
   }
   int flush() {
     if (jobs_.empty()) return SVXG_OK;
-    size_t data = 0;
-    for (const Job& j : jobs_) data += round256(j.bytes);
-    const size_t table = round256(jobs_.size() * sizeof(BulkSeg));
-    svxg_ctx::Lane* L = nullptr;
-    SVXG_TRY(staging(c_, s_, data + table, &L));
-    char* hp = static_cast<char*>(L->pinned);
-    char* dp = L->dstage.as<char>();
-    BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
-    size_t off = 0;
-    for (size_t k = 0; k < jobs_.size(); ++k) {
-      seg[k] = BulkSeg{dp + off, jobs_[k].dev, int64_t(jobs_[k].bytes)};
-      off += round256(jobs_[k].bytes);
-    }
-    SVXG_HIP(hipMemcpyAsync(dp + data, seg, jobs_.size() * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
-    launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(jobs_.size()));
-    SVXG_HIP(hipGetLastError());
-    SVXG_HIP(hipMemcpyAsync(hp, dp, data, hipMemcpyDeviceToHost, s_));
-    SVXG_HIP(hipStreamSynchronize(s_));
-    off = 0;
-    for (const Job& j : jobs_) {
-      std::memcpy(j.dst, hp + off, j.bytes);
-      off += round256(j.bytes);
-    }
+    const std::vector<StagePart> parts = stage_parts(jobs_);
     jobs_.clear();
+    for (size_t k0 = 0, k1 = 0; k0 < parts.size(); k0 = k1) {
+      size_t data = 0;
+      k1 = stage_round(parts, k0, &data);
+      SVXG_TRY(fetch(parts, k0, k1, data));
+    }
     return SVXG_OK;
   }
 
  private:
-  struct Job {
-    void* dst;
-    const void* dev;
-    size_t bytes;
-  };
+  int fetch(const std::vector<StagePart>& parts, size_t k0, size_t k1, size_t data) {
+    const size_t n = k1 - k0;
+    svxg_ctx::Lane* L = nullptr;
+    SVXG_TRY(staging(c_, s_, data + round256(n * sizeof(BulkSeg)), &L));
+    char* hp = static_cast<char*>(L->pinned);
+    char* dp = L->dstage.as<char>();
+    BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
+    size_t off = 0;
+    for (size_t k = 0; k < n; ++k) {
+      seg[k] = BulkSeg{dp + off, parts[k0 + k].src, int64_t(parts[k0 + k].bytes)};
+      off += round256(parts[k0 + k].bytes);
+    }
+    SVXG_HIP(hipMemcpyAsync(dp + data, seg, n * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
+    launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(n));
+    SVXG_HIP(hipGetLastError());
+    SVXG_HIP(hipMemcpyAsync(hp, dp, data, hipMemcpyDeviceToHost, s_));
+    SVXG_HIP(hipStreamSynchronize(s_));
+    off = 0;
+    for (size_t k = k0; k < k1; ++k) {
+      std::memcpy(parts[k].dst, hp + off, parts[k].bytes);
+      off += round256(parts[k].bytes);
+    }
+    return SVXG_OK;
+  }
   svxg_ctx* c_;
   hipStream_t s_;
-  std::vector<Job> jobs_;
+  std::vector<StagePart> jobs_;
 };
 
 // Fills and FP64 -> FP32 copies of a batch setup: small ones gathered into one launch each, large
@@ -346,7 +385,7 @@ int64_t svxg_problem::device_bytes() const {
     bytes += int64_t(d->held());
   for (const HostLevel& L : lv)
     for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32,
-                            &L.Rrow, &L.Rfine, &L.Rent, &L.R64, &L.R32, &L.Minv, &L.x64[0], &L.x64[1], &L.b64,
+                            &L.Rrow, &L.Rfine, &L.Rent, &L.Minv, &L.x64[0], &L.x64[1], &L.b64,
                             &L.r64, &L.d64, &L.x32[0], &L.x32[1], &L.b32, &L.r32, &L.d32, &L.pos, &L.lam})
       bytes += int64_t(d->held());
   return bytes;
@@ -416,6 +455,7 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.Pcol = H.Pcol.as<int32_t>();
     L.Rrow = H.Rrow.as<int32_t>();
     L.Rfine = H.Rfine.as<int32_t>();
+    L.Rent = H.Rent.as<int32_t>();
     L.Minv = H.Minv.as<double>();
     L.lmax = H.lmax;
     L.cheb_degree = P->opt.amg.cheb_degree;
@@ -424,7 +464,6 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.f64.A = H.A64.as<double>();
     L.f64.Dinv = H.D64.as<double>();
     L.f64.P = H.P64.as<double>();
-    L.f64.R = H.R64.as<double>();
     L.f64.x[0] = H.x64[0].as<double>();
     L.f64.x[1] = l == 0 ? P->z.as<double>() : H.x64[1].as<double>();
     L.f64.b = l == 0 ? P->r.as<double>() : H.b64.as<double>();
@@ -433,7 +472,6 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.f32.A = H.A32.as<float>();
     L.f32.Dinv = H.D32.as<float>();
     L.f32.P = H.P32.as<float>();
-    L.f32.R = H.R32.as<float>();
     L.f32.x[0] = H.x32[0].as<float>();
     L.f32.x[1] = l == 0 ? P->iz.as<float>() : H.x32[1].as<float>();
     L.f32.b = l == 0 ? P->ir.as<float>() : H.b32.as<float>();
@@ -451,6 +489,13 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
                        int resident_rows = -1) {
   svxg_ctx* c = count ? probs[0]->ctx : nullptr;
   if (resident_rows < 0) resident_rows = c ? c->resident_rows : 0;
+  for (int g = 0; g < count; ++g) {
+    const svxg_problem* P = probs[g];
+    if (P->opt.precision == SVXG_FP64 && !P->fp64_levels && !P->jacobi_only) {
+      set_error("a problem set up for mixed precision keeps no FP64 hierarchy: assemble it again for FP64");
+      return SVXG_NOT_ASSEMBLED;
+    }
+  }
   B.probs.assign(probs, probs + count);
   B.G = count;
   SVXG_TRY(B.dstate.make(A, sizeof(PcgState) * size_t(std::max(1, count))));

@@ -31,9 +31,9 @@ void set_error(const std::string& msg);
     if (st_ != SVXG_OK) return st_; \
   } while (0)
 
-// Device memory for one context: blocks of power-of-two classes (256 B .. 64 MB) carved from
-// 512 MB slabs and recycled through per-class free lists; larger requests get their own
-// allocation. Every byte is admitted against the context's HBM budget.
+// Device memory for one context: blocks of size classes (four a power of two, 256 B .. 64 MB)
+// carved from 512 MB slabs and recycled through per-class free lists; larger requests get their
+// own allocation. Every byte is admitted against the context's HBM budget.
 class Arena {
  public:
   Arena(int device, int64_t budget) : device_(device), budget_(budget) {}
@@ -53,7 +53,7 @@ class Arena {
   std::vector<void*> slabs_, big_;
   char* bump_ = nullptr;
   size_t bump_left_ = 0;
-  std::array<std::vector<void*>, 32> free_;
+  std::array<std::vector<void*>, 80> free_;  // (class 72: 64 MB)
 };
 
 // a device allocation owned by its holder (returned to the arena on destruction)
@@ -110,7 +110,7 @@ struct HostLevel {
   int32_t dense_m = 0;
   double lmax = 1.0;
   DevBuf rowptr, col, A64, A32, D64, D32;
-  DevBuf Prow, Pcol, P64, P32, Rrow, Rfine, Rent, R64, R32, Minv;
+  DevBuf Prow, Pcol, P64, P32, Rrow, Rfine, Rent, Minv;
   DevBuf x64[2], b64, r64, d64, x32[2], b32, r32, d32;
   DevBuf pos, lam;
   std::vector<int32_t> h_rowptr;  // (level 0: K's pattern on the host too)
@@ -153,6 +153,7 @@ struct svxg_problem {
   std::vector<svxg_material> mats;
   svxg_stress_options opt{};
   bool assembled = false, pc_current = false, jacobi_only = false, have_opt = false;
+  bool fp64_levels = false;  // the hierarchy keeps its FP64 arrays (set up for the FP64 path)
   int64_t assembly_work = 0, work_per_apply = 0;
   int32_t nfree = 0;
   std::vector<int32_t> dof;

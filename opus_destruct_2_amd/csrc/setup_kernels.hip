@@ -719,7 +719,7 @@ __global__ void k_r_fill(const CoarsenDev* __restrict__ J, const Tile* __restric
   }
 }
 
-// per coarse row: entries sorted by fine row (the pairs move together), R blocks = P blocks^T
+// per coarse row: entries sorted by fine row (the pairs move together)
 __global__ void k_r_sort(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
   if (int(blockIdx.x) >= nctiles) return;
   const Tile tl = ctiles[blockIdx.x];
@@ -739,12 +739,6 @@ __global__ void k_r_sort(const CoarsenDev* __restrict__ J, const Tile* __restric
     }
     f[t + 1] = fv;
     en[t + 1] = ev;
-  }
-  for (int s = 0; s < n; ++s) {
-    const double* P = c.P + 36 * size_t(en[s]);
-    double* R = c.R + 36 * size_t(b + s);
-    for (int r = 0; r < 6; ++r)
-      for (int q = 0; q < 6; ++q) R[r * 6 + q] = P[q * 6 + r];
   }
 }
 

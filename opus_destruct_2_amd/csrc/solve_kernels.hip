@@ -261,7 +261,16 @@ __global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G
   residual_tile<T, S>(g, tl, level, buf, inner);
 }
 
-// b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`
+// column q of a row-major 6x6 block (row q of its transpose): the six lanes of a row read the
+// whole block, six consecutive values a load
+template <typename T>
+__device__ __forceinline__ void load6_col(const T* __restrict__ blk, int q, T* m) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) m[k] = blk[6 * k + q];
+}
+
+// b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`, each the
+// transpose of its prolongation block (read in place through Rent)
 template <typename T, int S>
 __device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl, int level, int inner) {
   const LevelDev& F = g.lv[level];
@@ -276,7 +285,7 @@ __device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl,
     for (int e = F.Rrow[I] + sub; e < F.Rrow[I + 1]; e += S) {
       const int i = F.Rfine[e];
       T m[6], v[6];
-      load6(af.R + 36 * size_t(e) + 6 * q, m);
+      load6_col(af.P + 36 * size_t(F.Rent[e]), q, m);
       load6(af.r + 6 * size_t(i), v);
       acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
     }

@@ -90,7 +90,6 @@ struct LevelArrays {
   const T* A;      // 36 per block, row-major 6x6
   const T* Dinv;   // 36 per row
   const T* P;      // 36 per prolongation entry (rows of this level, columns of the next)
-  const T* R;      // 36 per restriction entry: P^T blocks by coarse row
   T* x[2];         // ping-pong iterates (x[1] holds a V-cycle's result)
   T* b;            // right-hand side
   T* r;            // residual (restriction input)
@@ -109,6 +108,7 @@ struct LevelDev {
   const int32_t* Pcol;
   const int32_t* Rrow;   // nc + 1 (by coarse row)
   const int32_t* Rfine;  // the fine row of each restriction entry
+  const int32_t* Rent;   // its prolongation entry (the restriction's block is that P block transposed)
   const double* Minv;    // dense inverse of the coarsest level (dense_m x dense_m, row-major)
   double lmax;           // lambda_max(D^-1 A) estimate (x 1.1)
   int32_t cheb_degree;

@@ -163,7 +163,6 @@ struct CoarsenDev {
   int32_t* Rrow;       // na + 1
   int32_t* Rfine;      // np
   int32_t* Rent;       // np (the P entry of each restriction entry)
-  double* R;           // 36 np
   int32_t* AProw;      // n + 1
   int32_t* APcol;
   double* AP;

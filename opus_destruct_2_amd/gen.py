@@ -5,6 +5,8 @@ Configurations (sizes as generated; realised counts are reported by the callers)
   block   64 x 64 x 48 cubes, 15 % voids, only ground-connected cells, materials per column
   pieces  connected random lattice subsets, log-normal sizes (median 200, clamped 16..20000),
           half on supports, half free bodies pinned at the node nearest their centre of mass
+  city    buildings (towers on supports) of 5k..150k nodes, log-normal sizes (median 20k), and
+          ground-storey collapses that cut the lowest storey's columns under half the footprint
 """
 import ctypes
 import os
@@ -105,6 +107,52 @@ def piece(count, pinned=False, L=0.5, seed=1):
 
 def building(nx, ny, nz, L=0.5, seed=1):
     return _take(lib().svxgen_building(nx, ny, nz, L, seed), f"building{nx}x{ny}x{nz}")
+
+
+def city_shapes(count, median=20000.0, sigma=0.8, lo=5000, hi=150000, seed=1):
+    """(nx, ny, nz) of `count` buildings of a city chunk: node counts log-normal (median, sigma)
+    clamped to [lo, hi], a footprint of s x s or s x (s+1) cells and a height about r*s storeys,
+    r in [1, 3) (taller than wide). Integer arithmetic only past the draws."""
+    sizes = piece_sizes(count, median, sigma, lo, hi, seed)
+    u = uniform(seed * 7 + 3, 2 * count)
+    out = []
+    for k, n in enumerate(sizes):
+        r = 1.0 + 2.0 * float(u[2 * k])
+        s = 1
+        while (s + 1) ** 3 * r <= n:
+            s += 1
+        nx, ny = s, s + (1 if u[2 * k + 1] >= 0.5 else 0)
+        out.append((nx, ny, max(3, (int(n) + nx * ny // 2) // (nx * ny))))
+    return out
+
+
+def ground_storey_cuts(g, frac=0.5):
+    """A ground-storey collapse: the bonds between the two lowest storeys under the part of the
+    footprint with x below `frac` of its width (the support bonds stay). The rest of the footprint
+    still carries the storeys above, so no free node loses its path to a support."""
+    c = g.nodes["c"]
+    a, b = g.bonds["a"], g.bonds["b"]
+    _, lay = np.unique(c[:, 2], return_inverse=True)
+    x = c[:, 0]
+    xcut = x.min() + frac * (x.max() - x.min())
+    inner = b >= 0
+    bb = np.where(inner, b, 0)
+    lo, hi = np.minimum(lay[a], lay[bb]), np.maximum(lay[a], lay[bb])
+    pick = inner & (lo == 0) & (hi == 1) & (x[a] < xcut) & (x[bb] < xcut)
+    return np.nonzero(pick)[0].astype(np.int32)
+
+
+def wind_load(g, angle, scale=0.1):
+    """g's gravity load plus a horizontal load in direction `angle` (radians) on every node,
+    scale * its weight * its height over the building's height (0 at the lowest storey)."""
+    c = g.nodes["c"]
+    z = c[:, 2] - c[:, 2].min()
+    h = max(float(z.max()), 1e-12)
+    w = scale * 9.81 * g.nodes["mass"] * (z / h)
+    f = g.f.copy()
+    f[0::6] += w * np.cos(angle)
+    f[1::6] += w * np.sin(angle)
+    return f
 
 
 def safe_cuts(g, broken, count, seed):

@@ -104,6 +104,27 @@ def _fixed_digests():
         FIXED[k] = v
 
 
+def _grounded(g, broken):
+    """Whether every node of g reaches a support bond or a pinned node over intact bonds."""
+    parent = list(range(g.n + 1))  # node g.n: the ground
+
+    def find(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+
+    for i in range(g.n):
+        if g.nodes["fixed"][i]:
+            parent[find(i)] = find(g.n)
+    for k in range(len(g.bonds)):
+        if broken[k]:
+            continue
+        a, b = int(g.bonds["a"][k]), int(g.bonds["b"][k])
+        parent[find(a)] = find(g.n if b < 0 else b)
+    return all(find(i) == find(g.n) for i in range(g.n))
+
+
 def test_safe_cuts_never_leave_a_mechanism():
     """Every free node keeps a path to a support bond or a pinned node, cut after cut."""
     for pinned in (False, True):
@@ -114,20 +135,25 @@ def test_safe_cuts_never_leave_a_mechanism():
             assert len(set(cut.tolist())) == len(cut) and not broken[cut].any()
             broken[cut] = 1
         assert broken.sum() > len(g.bonds) // 4
-        parent = list(range(g.n + 1))  # node g.n: the ground
+        assert _grounded(g, broken)
 
-        def find(i):
-            while parent[i] != i:
-                parent[i] = parent[parent[i]]
-                i = parent[i]
-            return i
 
-        for i in range(g.n):
-            if g.nodes["fixed"][i]:
-                parent[find(i)] = find(g.n)
-        for k in range(len(g.bonds)):
-            if broken[k]:
-                continue
-            a, b = int(g.bonds["a"][k]), int(g.bonds["b"][k])
-            parent[find(a)] = find(g.n if b < 0 else b)
-        assert all(find(i) == find(g.n) for i in range(g.n))
+def test_city_shapes_and_ground_storey_collapse():
+    """City buildings: sizes in range and about the requested total; a ground-storey collapse cuts
+    only bonds between the two lowest storeys under half the footprint and leaves no mechanism."""
+    sh = gen.city_shapes(400, seed=21)
+    n = np.array([a * b * c for a, b, c in sh])
+    assert 4500 <= n.min() and n.max() <= 155000 and 10e6 < n.sum() < 14e6
+    assert all(c >= a for a, b, c in sh)  # (taller than wide)
+    nx, ny, nz = 6, 7, 9
+    g = gen.building(nx, ny, nz, seed=3)
+    cut = gen.ground_storey_cuts(g)
+    assert len(cut) == (nx // 2) * ny
+    z = g.nodes["c"][:, 2]
+    zs = np.unique(z)
+    for k in cut:
+        a, b = int(g.bonds["a"][k]), int(g.bonds["b"][k])
+        assert b >= 0 and sorted((z[a], z[b])) == [zs[0], zs[1]]
+    broken = np.zeros(len(g.bonds), np.uint8)
+    broken[cut] = 1
+    assert _grounded(g, broken)
