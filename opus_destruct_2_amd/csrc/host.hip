@@ -546,7 +546,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   B.lvl.resize(size_t(B.maxlev));
   struct LvTiles {
     std::vector<Tile> ts[2], tp, tc[2], tstall[2];
-    std::vector<int32_t> dense;
+    std::vector<Tile> dense;  // (64-row chunks of the dense inverses)
   };
   std::vector<LvTiles> lt(static_cast<size_t>(B.maxlev));
   for (int l = 0; l < B.maxlev; ++l) {
@@ -562,7 +562,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
         const int vc = long_rows(H.np, H.nc) ? 1 : 0;
         add_tiles(T.tc[vc], g, P->lv[size_t(l) + 1].n, vc ? tile_rows<10>() : tile_rows<1>());
       } else if (H.dense_m > 0) {
-        T.dense.push_back(g);
+        for (int r = 0; r < H.dense_m; r += 64) T.dense.push_back(Tile{g, r, std::min(64, H.dense_m - r), 0});
       } else {
         add_tiles(T.tstall[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
       }
@@ -680,7 +680,7 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
       launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (degree - 1) & 1, inner, SV[v]);
     }
     for (int v = 0; v < 2; ++v) launch_restrict<T>(s, G, V.tc[v].as<Tile>(), V.ntc[v], l, inner, SV[v]);
-    if (V.ndense) launch_dense_solve<T>(s, G, V.dense.as<int32_t>(), V.ndense, l, inner);
+    if (V.ndense) launch_dense_solve<T>(s, G, V.dense.as<Tile>(), V.ndense, l, inner);
     for (int v = 0; v < 2; ++v)
       if (V.ntstall[v])
         for (int st = 0; st < kStallDegree; ++st)

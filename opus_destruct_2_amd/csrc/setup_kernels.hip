@@ -1063,27 +1063,69 @@ __global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict
   }
 }
 
-// ---- dense coarsest level: Cholesky (amg.cpp:535-578), then the explicit inverse --------------------
+// ---- dense coarsest level: blocked Cholesky and explicit inverse on the matrix cores ---------------
+//
+// One workgroup of 16 waves a graph. The coarsest matrix (m = 6 n_c rows, symmetrised, as
+// amg.cpp:535-578) is padded with an identity to mp = 16 ceil(m / 16) rows and held as 16 x 16
+// tiles in global scratch (L2-resident at the usual m <= 288). A right-looking blocked Cholesky
+// L L^T, per block column: one wave factors the diagonal tile in LDS with the reference's pivot
+// floor (d >= 1e-12 max|a_ii|) and forms the tile's triangular inverse; the panel below is
+// multiplied by that inverse's transpose; the trailing lower triangle takes the panel's rank-16
+// update. Then W = L^-1 (row block by row block) and M^-1 = W^T W, written to the m x m inverse.
+// Every 16 x 16 x 16 tile product is four v_mfma_f64_16x16x4f64.
 
-// One workgroup of 1024 lanes a graph, every phase parallel over the matrix, one barrier a
-// column: the symmetrised matrix; a right-looking Cholesky with the reference's pivot floor
-// (d >= 1e-12 max|a_ii|, amg.cpp:535-578) in which column j's scaling by its pivot is folded into
-// the trailing update (a_ik -= a_ij a_kj / d_j) and written one step later; W = L^-1 by
-// right-looking forward substitution on I (row k's division likewise deferred); M^-1 = W^T W.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int kDenseWG = 1024;
+constexpr int kDenseWaves = kDenseWG / 64;
+
+// lane layout of v_mfma_f64_16x16x4f64 (lane l = 16 g + i): A[i][k] at k = g, B[k][j] at k = g,
+// j = i; the accumulator's register r holds D[g + 4 r][i]
+__device__ __forceinline__ dbl4 tile_load(const double* T, int ld) {
+  const int i = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
+  dbl4 c;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) c[r] = T[size_t(g + 4 * r) * size_t(ld) + size_t(i)];
+  return c;
+}
+__device__ __forceinline__ void tile_store(double* T, int ld, dbl4 c) {
+  const int i = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) T[size_t(g + 4 * r) * size_t(ld) + size_t(i)] = c[r];
+}
+// c + sign op(X) op(Y) over a 16-wide inner dimension; X, Y row-major tiles with leading dimensions
+// ldx, ldy; tX: X read transposed, tY: Y read transposed
+template <bool tX, bool tY>
+__device__ __forceinline__ dbl4 tile_mma(dbl4 c, const double* X, int ldx, const double* Y, int ldy, double sign) {
+  const int i = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int k = 4 * kk + g;
+    const double a = tX ? X[size_t(k) * size_t(ldx) + size_t(i)] : X[size_t(i) * size_t(ldx) + size_t(k)];
+    const double b = tY ? Y[size_t(i) * size_t(ldy) + size_t(k)] : Y[size_t(k) * size_t(ldy) + size_t(i)];
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(sign * a, b, c, 0, 0, 0);
+  }
+  return c;
+}
+// LDS writes of this wave visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
   if (int(blockIdx.x) >= count) return;
   const DenseDev& d = J[blockIdx.x];
-  const int m = d.m;
-  const int tid = threadIdx.x;
-  double* L = d.L;
-  double* W = d.Minv;  // (W = L^-1 first; M^-1 is formed in L's storage and copied over)
-  auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(m) + size_t(c)]; };
-  auto wat = [&](int r, int c) -> double& { return W[size_t(r) * size_t(m) + size_t(c)]; };
-  const size_t mm = size_t(m) * size_t(m);
+  const int m = d.m, mp = d.mp, NB = mp / 16;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  double* L = d.L;  // mp x mp: the matrix, then its factor (lower tiles)
+  double* W = d.W;  // mp x mp: L^-1 (lower tiles)
+  auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(mp) + size_t(c)]; };
+  const size_t mm = size_t(mp) * size_t(mp);
   for (size_t t = tid; t < mm; t += kDenseWG) {
-    L[t] = 0.0;
-    W[t] = (t / size_t(m) == t % size_t(m)) ? 1.0 : 0.0;
+    const int r = int(t / size_t(mp)), c = int(t % size_t(mp));
+    L[t] = (r == c && r >= m) ? 1.0 : 0.0;
+    W[t] = 0.0;
   }
   __syncthreads();
   for (int i = tid; i < d.n; i += kDenseWG)
@@ -1094,71 +1136,119 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
         for (int c = 0; c < 6; ++c) at(6 * i + r, 6 * j + c) += b[r * 6 + c];
     }
   __syncthreads();
-  for (size_t t = tid; t < mm; t += kDenseWG) {
+  __shared__ double red[kDenseWaves];
+  double dm = 0.0;
+  for (size_t t = tid; t < size_t(m) * size_t(m); t += kDenseWG) {
     const int i = int(t / size_t(m)), j = int(t % size_t(m));
     if (i > j) at(i, j) = 0.5 * (at(i, j) + at(j, i));
+    if (i == j) dm = fmax(dm, fabs(at(i, i)));
   }
-  __shared__ double red[kDenseWG / 64];
-  __shared__ double piv[2];  // the pivots d_j of the step before and of this one
-  double dm = 0.0;
-  for (int i = tid; i < m; i += kDenseWG) dm = fmax(dm, fabs(at(i, i)));
   for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_xor(dm, o, 64));
-  if ((tid & 63) == 0) red[tid >> 6] = dm;
+  if (lane == 0) red[wave] = dm;
   __syncthreads();
   double dmax = 0.0;
-  for (int w = 0; w < kDenseWG / 64; ++w) dmax = fmax(dmax, red[w]);
-  // Cholesky: step j finishes column j - 1 (scaled by its pivot's root) and updates the trailing
-  // lower triangle with column j's unscaled entries
-  for (int j = 0; j <= m; ++j) {
-    double dd = 1.0;
-    if (j < m) {
-      dd = at(j, j);
-      if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
+  for (int w = 0; w < kDenseWaves; ++w) dmax = fmax(dmax, red[w]);
+  __shared__ double Dk[16][17];
+  __shared__ double Ik[16][17];
+  __shared__ double Sw[kDenseWaves][16][17];
+  for (int kb = 0; kb < NB; ++kb) {
+    const int k0 = 16 * kb;
+    if (wave == 0) {
+      // the diagonal tile: unblocked Cholesky in LDS, then its triangular inverse
+      for (int e = lane; e < 256; e += 64) {
+        const int r = e >> 4, c = e & 15;
+        Dk[r][c] = c <= r ? at(k0 + r, k0 + c) : 0.0;
+      }
+      wave_lds_sync();
+      for (int j = 0; j < 16; ++j) {
+        double dd = Dk[j][j];
+        if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
+        const double lj = sqrt(dd);
+        wave_lds_sync();
+        if (lane == j) Dk[j][j] = lj;
+        else if (lane > j && lane < 16) Dk[lane][j] /= lj;
+        wave_lds_sync();
+        for (int e = lane; e < 256; e += 64) {
+          const int r = e >> 4, c = e & 15;
+          if (c > j && c <= r) Dk[r][c] -= Dk[r][j] * Dk[c][j];
+        }
+        wave_lds_sync();
+      }
+      if (lane < 16) {  // column `lane` of the inverse by forward substitution
+        const int c = lane;
+        for (int r = 0; r < c; ++r) Ik[r][c] = 0.0;
+        Ik[c][c] = 1.0 / Dk[c][c];
+        for (int r = c + 1; r < 16; ++r) {
+          double s = 0.0;
+          for (int q = c; q < r; ++q) s += Dk[r][q] * Ik[q][c];
+          Ik[r][c] = -s / Dk[r][r];
+        }
+      }
+      wave_lds_sync();
+      for (int e = lane; e < 256; e += 64) {
+        const int r = e >> 4, c = e & 15;
+        at(k0 + r, k0 + c) = Dk[r][c];
+        W[size_t(k0 + r) * size_t(mp) + size_t(k0 + c)] = Ik[r][c];
+      }
     }
-    if (j > 0) {
-      const double sp = sqrt(piv[(j - 1) & 1]);
-      for (int i = j + tid; i < m; i += kDenseWG) at(i, j - 1) /= sp;
-      if (tid == 0) at(j - 1, j - 1) = sp;
+    __syncthreads();
+    // the panel: L_ib,kb = A_ib,kb (L_kb,kb)^-T
+    for (int ib = kb + 1 + wave; ib < NB; ib += kDenseWaves) {
+      double* T = &at(16 * ib, k0);
+      dbl4 c = {0.0, 0.0, 0.0, 0.0};
+      c = tile_mma<false, true>(c, T, mp, &Ik[0][0], 17, 1.0);
+      tile_store(T, mp, c);
     }
-    if (j == m) break;
-    if (tid == 0) piv[j & 1] = dd;
-    const int w = m - j - 1;
-    const double inv = 1.0 / dd;
-    for (int t = tid; t < w * w; t += kDenseWG) {
-      const int i = j + 1 + t / w, k = j + 1 + t % w;
-      if (k <= i) at(i, k) -= at(i, j) * at(k, j) * inv;
+    __syncthreads();
+    // the trailing lower triangle: A_ib,jb -= L_ib,kb L_jb,kb^T, kb < jb <= ib
+    const int n = NB - kb - 1;
+    const int tiles = n * (n + 1) / 2;
+    for (int t = wave; t < tiles; t += kDenseWaves) {
+      int a = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+      while (a * (a + 1) / 2 > t) --a;
+      while ((a + 1) * (a + 2) / 2 <= t) ++a;
+      const int ib = kb + 1 + a, jb = kb + 1 + (t - a * (a + 1) / 2);
+      double* C = &at(16 * ib, 16 * jb);
+      dbl4 c = tile_load(C, mp);
+      c = tile_mma<false, true>(c, &at(16 * ib, k0), mp, &at(16 * jb, k0), mp, -1.0);
+      tile_store(C, mp, c);
     }
     __syncthreads();
   }
-  __syncthreads();
-  // W = L^-1: step k takes row k (not yet divided by L_kk) off the rows below, and divides row k - 1
-  for (int k = 0; k <= m; ++k) {
-    if (k > 0) {
-      const double lk = at(k - 1, k - 1);
-      for (int c = tid; c < k; c += kDenseWG) wat(k - 1, c) /= lk;
-    }
-    if (k == m) break;
-    const double inv = 1.0 / at(k, k);
-    const int rows = m - k - 1, cols = k + 1;
-    for (int t = tid; t < rows * cols; t += kDenseWG) {
-      const int i = k + 1 + t / cols, c = t % cols;
-      wat(i, c) -= at(i, k) * wat(k, c) * inv;
+  // W = L^-1 below the diagonal: W_ib,jb = -W_ib,ib sum_{jb <= kb < ib} L_ib,kb W_kb,jb
+  for (int ib = 1; ib < NB; ++ib) {
+    for (int jb = wave; jb < ib; jb += kDenseWaves) {
+      dbl4 c = {0.0, 0.0, 0.0, 0.0};
+      for (int kb = jb; kb < ib; ++kb)
+        c = tile_mma<false, false>(c, &at(16 * ib, 16 * kb), mp, &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)],
+                                   mp, 1.0);
+      tile_store(&Sw[wave][0][0], 17, c);
+      wave_lds_sync();
+      dbl4 z = {0.0, 0.0, 0.0, 0.0};
+      z = tile_mma<false, false>(z, &W[size_t(16 * ib) * size_t(mp) + size_t(16 * ib)], mp, &Sw[wave][0][0], 17, -1.0);
+      wave_lds_sync();
+      tile_store(&W[size_t(16 * ib) * size_t(mp) + size_t(16 * jb)], mp, z);
     }
     __syncthreads();
   }
-  __syncthreads();
-  // M^-1 = W^T W (lower half, mirrored), into L's storage, then over W
-  for (size_t t = tid; t < mm; t += kDenseWG) {
-    const int i = int(t / size_t(m)), j = int(t % size_t(m));
-    if (j > i) continue;
-    double v = 0.0;
-    for (int r = i; r < m; ++r) v += wat(r, i) * wat(r, j);
-    at(i, j) = v;
-  }
-  __syncthreads();
-  for (size_t t = tid; t < mm; t += kDenseWG) {
-    const int i = int(t / size_t(m)), j = int(t % size_t(m));
-    W[t] = j <= i ? at(i, j) : at(j, i);
+  // M^-1 = W^T W: tile (ib, jb), ib >= jb, sums over kb >= ib; lower part written and mirrored
+  const int tiles = NB * (NB + 1) / 2;
+  for (int t = wave; t < tiles; t += kDenseWaves) {
+    int a = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (a * (a + 1) / 2 > t) --a;
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    const int ib = a, jb = t - a * (a + 1) / 2;
+    dbl4 c = {0.0, 0.0, 0.0, 0.0};
+    for (int kb = ib; kb < NB; ++kb)
+      c = tile_mma<true, false>(c, &W[size_t(16 * kb) * size_t(mp) + size_t(16 * ib)], mp,
+                                &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)], mp, 1.0);
+    const int i = lane & 15, g = lane >> 4;
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * ib + g + 4 * r, col = 16 * jb + i;
+      if (row >= m || col >= m || col > row) continue;
+      d.Minv[size_t(row) * size_t(m) + size_t(col)] = c[r];
+      d.Minv[size_t(col) * size_t(m) + size_t(row)] = c[r];
+    }
   }
 }
 

@@ -339,27 +339,52 @@ __global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G,
 
 // x[1] = M^-1 b at the coarsest level of each listed graph (FP64 products): one workgroup a graph,
 // a wavefront a row at a time, lanes across the row
+// x = M^-1 b at a dense coarsest level, rows [r0, r0 + 64): M^-1 is symmetric, so row i is read as
+// column i and the 64 lanes' loads of one j are one contiguous run; the workgroup's four waves
+// split the j range, their partial sums meet in LDS in a fixed order
+constexpr int kDenseRows = 64;
 template <typename T>
-__device__ __forceinline__ void dense_solve_g(const GraphDev& g, int level) {
-  const LevelDev& L = g.lv[level];
-  const LevelArrays<T>& a = arrays<T>(L);
+__device__ __forceinline__ void dense_rows(const LevelDev& L, const LevelArrays<T>& a, int r0, double* sb,
+                                           double* part) {
   const int m = L.dense_m;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  for (int i = w; i < m; i += 4) {
-    const double* row = L.Minv + size_t(i) * size_t(m);
-    double s = 0.0;
-    for (int j = l; j < m; j += 64) s += row[j] * double(a.b[j]);
-    s = wave_sum(s);
-    if (l == 0) a.x[1][i] = T(s);
+  for (int j = threadIdx.x; j < m; j += kWG) sb[j] = double(a.b[j]);
+  __syncthreads();
+  constexpr int nw = kWG / 64;
+  const int w = threadIdx.x >> 6, i = r0 + (threadIdx.x & 63);
+  const int span = (m + nw - 1) / nw, j0 = w * span, j1 = min(m, j0 + span);
+  double s = 0.0;
+  if (i < m) {
+    const double* col = L.Minv + i;
+    for (int j = j0; j < j1; ++j) s += col[size_t(j) * size_t(m)] * sb[j];
   }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (w == 0 && i < m) {
+    double t = part[threadIdx.x];
+    for (int q = 1; q < nw; ++q) t += part[q * 64 + threadIdx.x];
+    a.x[1][i] = T(t);
+  }
+  __syncthreads();
 }
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict__ G, const int32_t* __restrict__ graphs,
+__device__ __forceinline__ void dense_solve_g(const GraphDev& g, int level) {
+  __shared__ double sb[kMaxDenseM];
+  __shared__ double part[kWG];
+  const LevelDev& L = g.lv[level];
+  for (int r0 = 0; r0 < L.dense_m; r0 += kDenseRows) dense_rows<T>(L, arrays<T>(L), r0, sb, part);
+}
+// one workgroup a chunk of 64 rows (tile: graph, first row, rows)
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict__ G, const Tile* __restrict__ chunks,
                                                      int count, int level, int inner) {
   if (int(blockIdx.x) >= count) return;
-  const GraphDev& g = G[graphs[blockIdx.x]];
+  const Tile tl = chunks[blockIdx.x];
+  const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
-  dense_solve_g<T>(g, level);
+  __shared__ double sb[kMaxDenseM];
+  __shared__ double part[kWG];
+  const LevelDev& L = g.lv[level];
+  dense_rows<T>(L, arrays<T>(L), tl.row0, sb, part);
 }
 
 // z = D^-1 r (block-Jacobi preconditioning of a graph without a hierarchy)
@@ -1179,9 +1204,9 @@ void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int nti
   hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf, inner);
 }
 template <typename T>
-void launch_dense_solve(hipStream_t s, const GraphDev* G, const int32_t* graphs, int count, int level, int inner) {
+void launch_dense_solve(hipStream_t s, const GraphDev* G, const Tile* chunks, int count, int level, int inner) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kWG), 0, s, G, graphs, count, level, inner);
+  hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kWG), 0, s, G, chunks, count, level, inner);
 }
 template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {
@@ -1302,7 +1327,7 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int);              \
   template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
   template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
-  template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const int32_t*, int, int, int);                  \
+  template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int);                  \
   template void launch_jacobi<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                               \
   template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_dot_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \

@@ -22,6 +22,9 @@ namespace svxg {
 constexpr int kWG = 256;
 constexpr int kRowsPerWave = 10;
 constexpr int kTileRows = 40;  // 4 wavefronts x 10 rows
+// the largest dense coarsest level (rows of its inverse): 6 x 192 nodes, the reference's limit at
+// its default coarse_max (amg.cpp:535-541)
+constexpr int kMaxDenseM = 1152;
 constexpr int kMaxLevels = 16;
 
 struct Tile {

@@ -26,7 +26,7 @@ void launch_restrict(hipStream_t s, const GraphDev* G, const Tile* tiles, int nt
 template <typename T>
 void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner);
 template <typename T>
-void launch_dense_solve(hipStream_t s, const GraphDev* G, const int32_t* graphs, int count, int level, int inner);
+void launch_dense_solve(hipStream_t s, const GraphDev* G, const Tile* chunks, int count, int level, int inner);
 template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
 template <typename T>
@@ -224,10 +224,13 @@ void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gti
 struct DenseDev {
   int32_t n;
   int32_t m;
+  int32_t mp;     // m rounded up to a multiple of 16 (the padded scratch)
+  int32_t pad_;
   const int32_t* rowptr;
   const int32_t* col;
   const double* A;
-  double* L;      // m x m scratch
+  double* L;      // mp x mp scratch (the factor)
+  double* W;      // mp x mp scratch (its inverse)
   double* Minv;   // m x m
 };
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count);
