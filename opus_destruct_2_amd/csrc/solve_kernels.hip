@@ -29,12 +29,19 @@ struct V2<float> {
   using t = fvec2;
 };
 
+// a load through a global-address-space pointer (every array here is device memory; through a
+// generic pointer the compiler emits flat loads)
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // six consecutive values (16-byte aligned for double, 8 for float)
 template <typename T>
 __device__ __forceinline__ void load6(const T* p, T v[6]) {
   using P = typename V2<T>::t;
   const P* q = reinterpret_cast<const P*>(p);
-  const P a = q[0], b = q[1], c = q[2];
+  const P a = ldg(q), b = ldg(q + 1), c = ldg(q + 2);
   v[0] = a.x;
   v[1] = a.y;
   v[2] = b.x;
@@ -48,49 +55,47 @@ __device__ __forceinline__ void load6(const T* p, T v[6]) {
 // non-temporal hint here made the line leave and come back, half again the matrix's bytes)
 template <typename T>
 __device__ __forceinline__ void load6_stream(const T* p, T v[6]) {
-  using P = typename V2<T>::t;
-  const P* q = reinterpret_cast<const P*>(p);
-  const P a = q[0], b = q[1], c = q[2];
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
-  v[4] = c.x;
-  v[5] = c.y;
+  load6(p, v);
+}
+
+// this lane's share of (A x)_q of block row `row`: its blocks k0 + sub, k0 + sub + S, ... in order,
+// each block's row q . x_col summed left to right, the blocks' sums added in turn. Blocks go in
+// groups of four whose column indices, then matrix rows and x values, are all loaded before any is
+// used (two memory round trips a group instead of two a block; the sums keep their order)
+template <typename T, int S>
+__device__ __forceinline__ T spmv_row_part(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                           const T* __restrict__ A, const T* __restrict__ x, int row, int q, int sub) {
+  constexpr int U = 4;
+  T acc = T(0);
+  const int k0 = ldg(rowptr + row), k1 = ldg(rowptr + row + 1);
+  for (int k = k0 + sub; k < k1; k += U * S) {
+    // (past the row's end a group re-reads its last block, unused: the loads stay unconditional)
+    int kk[U], j[U];
+    T a[U][6], v[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) kk[u] = min(k + u * S, k1 - 1);
+#pragma unroll
+    for (int u = 0; u < U; ++u) j[u] = ldg(col + kk[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) load6_stream(A + 36 * size_t(kk[u]) + 6 * q, a[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) load6(x + 6 * size_t(j[u]), v[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k + u * S < k1) {
+        const T s = a[u][0] * v[u][0] + a[u][1] * v[u][1] + a[u][2] * v[u][2] + a[u][3] * v[u][3] +
+                    a[u][4] * v[u][4] + a[u][5] * v[u][5];
+        acc += s;
+      }
+  }
+  return acc;
 }
 
 // (A x)_q of block row `row`: sum over the row's blocks of (row q of the block) . x_col
 template <typename T>
 __device__ __forceinline__ T spmv_row(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                       const T* __restrict__ A, const T* __restrict__ x, int row, int q) {
-  T acc = T(0);
-  const int k0 = rowptr[row], k1 = rowptr[row + 1];
-  for (int k = k0; k < k1; ++k) {
-    const int j = col[k];
-    T a[6], v[6];
-    load6_stream(A + 36 * size_t(k) + 6 * q, a);
-    load6(x + 6 * size_t(j), v);
-    const T s = a[0] * v[0] + a[1] * v[1] + a[2] * v[2] + a[3] * v[3] + a[4] * v[4] + a[5] * v[5];
-    acc += s;
-  }
-  return acc;
-}
-
-// this sub-group's share of (A x)_q of block row `row`: the row's blocks sub, sub + S, ...
-template <typename T, int S>
-__device__ __forceinline__ T spmv_row_part(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                           const T* __restrict__ A, const T* __restrict__ x, int row, int q, int sub) {
-  T acc = T(0);
-  const int k0 = rowptr[row], k1 = rowptr[row + 1];
-  for (int k = k0 + sub; k < k1; k += S) {
-    const int j = col[k];
-    T a[6], v[6];
-    load6_stream(A + 36 * size_t(k) + 6 * q, a);
-    load6(x + 6 * size_t(j), v);
-    const T s = a[0] * v[0] + a[1] * v[1] + a[2] * v[2] + a[3] * v[3] + a[4] * v[4] + a[5] * v[5];
-    acc += s;
-  }
-  return acc;
+  return spmv_row_part<T, 1>(rowptr, col, A, x, row, q, 0);
 }
 
 // (M v)_q with M a 6x6 row-major block and v the row's values gathered from its lanes
@@ -266,7 +271,7 @@ __global__ void __launch_bounds__(kWG) k_residual(const GraphDev* __restrict__ G
 template <typename T>
 __device__ __forceinline__ void load6_col(const T* __restrict__ blk, int q, T* m) {
 #pragma unroll
-  for (int k = 0; k < 6; ++k) m[k] = blk[6 * k + q];
+  for (int k = 0; k < 6; ++k) m[k] = ldg(blk + 6 * k + q);
 }
 
 // b_c = P^T r: coarse rows (tiles of level + 1), restriction entries of `level`, each the
@@ -281,14 +286,31 @@ __device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl,
   const bool on = rl >= 0 && rl < tl.nrows;
   const int I = tl.row0 + (on ? rl : 0);
   T acc = T(0);
-  if (on)
-    for (int e = F.Rrow[I] + sub; e < F.Rrow[I + 1]; e += S) {
-      const int i = F.Rfine[e];
-      T m[6], v[6];
-      load6_col(af.P + 36 * size_t(F.Rent[e]), q, m);
-      load6(af.r + 6 * size_t(i), v);
-      acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
+  if (on) {
+    // (entries in groups of four, loads first: see spmv_row_part)
+    constexpr int U = 4;
+    const int e0 = ldg(F.Rrow + I), e1 = ldg(F.Rrow + I + 1);
+    for (int e = e0 + sub; e < e1; e += U * S) {
+      int i[U], p[U];
+      T m[U][6], v[U][6];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int ee = min(e + u * S, e1 - 1);
+        i[u] = ldg(F.Rfine + ee);
+        p[u] = ldg(F.Rent + ee);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        load6_col(af.P + 36 * size_t(p[u]), q, m[u]);
+        load6(af.r + 6 * size_t(i[u]), v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (e + u * S < e1)
+          acc += m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2] + m[u][3] * v[u][3] +
+                 m[u][4] * v[u][4] + m[u][5] * v[u][5];
     }
+  }
   acc = row_reduce<S>(acc);
   if (!on || sub != 0) return;
   ac.b[6 * size_t(I) + q] = acc;
@@ -316,12 +338,27 @@ __device__ __forceinline__ void prolong_tile(const GraphDev& g, const Tile& tl, 
   if (rl < 0 || rl >= tl.nrows) return;
   const int i = tl.row0 + rl;
   T acc = af.x[buf][6 * size_t(i) + q];
-  for (int e = F.Prow[i]; e < F.Prow[i + 1]; ++e) {
-    const int J = F.Pcol[e];
-    T m[6], v[6];
-    load6(af.P + 36 * size_t(e) + 6 * q, m);
-    load6(xc + 6 * size_t(J), v);
-    acc += m[0] * v[0] + m[1] * v[1] + m[2] * v[2] + m[3] * v[3] + m[4] * v[4] + m[5] * v[5];
+  // (entries in groups of four, loads first: see spmv_row_part)
+  constexpr int U = 4;
+  const int e0 = ldg(F.Prow + i), e1 = ldg(F.Prow + i + 1);
+  for (int e = e0; e < e1; e += U) {
+    int ee[U], J[U];
+    T m[U][6], v[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ee[u] = min(e + u, e1 - 1);
+      J[u] = ldg(F.Pcol + ee[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      load6(af.P + 36 * size_t(ee[u]) + 6 * q, m[u]);
+      load6(xc + 6 * size_t(J[u]), v[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e + u < e1)
+        acc += m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2] + m[u][3] * v[u][3] + m[u][4] * v[u][4] +
+               m[u][5] * v[u][5];
   }
   af.x[buf][6 * size_t(i) + q] = acc;
 
