@@ -39,11 +39,11 @@ def log(*a):
 AMG_KW = {}  # (device multigrid options of every leg, from the command line)
 
 
-def run_path(g, mats, precision, steps, warmup, ctx):
+def run_path(g, mats, precision, steps, warmup, ctx, **kw):
     from opus_destruct_2_amd import stress
     p = stress.StressProblem(g.nodes, g.bonds, mats, ctx)
     t = time.perf_counter()
-    assert p.assemble(stress.default_options(precision=precision, **AMG_KW))
+    assert p.assemble(stress.default_options(precision=precision, **{**AMG_KW, **kw}))
     setup_ms = (time.perf_counter() - t) * 1e3
     info = p.info()
     log(f"  precision={precision} setup {setup_ms:.1f} ms levels {info['level_rows']} blocks {info['level_blocks']}")
@@ -431,6 +431,8 @@ def main():
     ap.add_argument("--smooth-levels", type=int, default=None,
                     help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
     ap.add_argument("--cheb-degree", type=int, default=None)
+    ap.add_argument("--mixed-fp64-rows", type=int, default=None,
+                    help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -460,6 +462,8 @@ def main():
         AMG_KW["smooth_levels"] = args.smooth_levels
     if args.cheb_degree is not None:
         AMG_KW["cheb_degree"] = args.cheb_degree
+    if args.mixed_fp64_rows is not None:
+        AMG_KW["mixed_fp64_rows"] = args.mixed_fp64_rows
     ctx = stress.default_context()
     if args.resident_rows >= 0:
         ctx.set_resident_rows(args.resident_rows)
@@ -490,6 +494,12 @@ def main():
     scale = np.abs(ref_u).max()
     line["mixed_vs_fp64"] = {"u_rel_err": float(np.abs(mixed_res["u"] - ref_u).max() / scale),
                              "phi_max_abs_err": float(np.abs(mixed_res["phi"] - plain["phi"]).max())}
+    # the other mixed split: FP32 K products too, inside FP64 iterative refinement
+    ir = run_path(g, mats, SVXG_MIXED, max(1, args.steps // 2), 1, ctx, mixed_fp64_rows=0)
+    line["mixed_fp32_products_ir"] = {
+        "value": round(g.dofs / ((ir["solve_ms"] + ir["checks_ms"]) / 1e3), 1), "solve_ms": round(ir["solve_ms"], 3),
+        "iters": ir["iters"], "refinements": ir["refinements"],
+        "u_rel_err_vs_fp64": float(np.abs(ir["u"] - ref_u).max() / scale)}
     if args.batch > 0:
         line["batched"] = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
     if args.serve_lambdas:

@@ -103,6 +103,11 @@ typedef struct svxg_stress_options {
   int32_t amg_min_nodes;     /* 160: smaller graphs are block-Jacobi preconditioned */
   int32_t precision;         /* SVXG_FP64 or SVXG_MIXED */
   svxg_amg_options amg;
+  int32_t mixed_fp64_rows;   /* SVXG_MIXED, device: graphs with fewer free nodes (default 2^30: all)
+                                run one PCG with FP64 K products and the FP32 V-cycle (the
+                                reference's own precision split, no refinement restarts); larger
+                                ones take FP32 K products inside FP64 iterative refinement (0: all) */
+  int32_t pad_;
 } svxg_stress_options;
 
 /* PcgResult (amg.hpp:120-125). */

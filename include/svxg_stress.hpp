@@ -63,6 +63,7 @@ struct StressOptions {
   svxg_amg_options amg{0.1, 0.6, 48, 1, 16, 1, 2, 1, 1, 0};
   i32 amg_min_nodes = 160;
   i32 precision = SVXG_FP64;
+  i32 mixed_fp64_rows = 1 << 30;  // (device: see svxg_stress_options)
 };
 
 using PcgResult = svxg_pcg_result;
@@ -147,7 +148,7 @@ class StressProblem {
     if (svxg_problem_create(default_context(), n.data(), i32(n.size()), b.data(), i32(b.size()), m.data(),
                             i32(m.size()), &p_) != SVXG_OK)
       return false;
-    svxg_stress_options o{opt.rtol, opt.amg_min_nodes, opt.precision, opt.amg};
+    svxg_stress_options o{opt.rtol, opt.amg_min_nodes, opt.precision, opt.amg, opt.mixed_fp64_rows, 0};
     const int st = svxg_problem_assemble(p_, &o);
     pull_flags();
     return st == SVXG_OK;

@@ -50,7 +50,7 @@ class AmgOptions(ctypes.Structure):
 class StressOptions(ctypes.Structure):
     """StressOptions (core/include/svx/stress/stress.hpp:68-72) plus the solve's precision."""
     _fields_ = [("rtol", ctypes.c_double), ("amg_min_nodes", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("amg", AmgOptions)]
+                ("amg", AmgOptions), ("mixed_fp64_rows", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class PcgResult(ctypes.Structure):

@@ -41,7 +41,7 @@ def test_struct_sizes_match_the_header():
     assert abi.NODE.itemsize == 40
     assert abi.BOND.itemsize == 216
     assert abi.MATERIAL.itemsize == 64
-    assert ctypes.sizeof(abi.StressOptions) == 8 + 4 + 4 + ctypes.sizeof(abi.AmgOptions)
+    assert ctypes.sizeof(abi.StressOptions) == 8 + 4 + 4 + ctypes.sizeof(abi.AmgOptions) + 8
     assert ctypes.sizeof(abi.PcgResult) == 24
 
 

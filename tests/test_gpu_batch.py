@@ -10,6 +10,9 @@ from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
 pytestmark = pytest.mark.gpu
 
 SIZES = [16, 31, 48, 60, 100, 159, 161, 250, 400, 900, 2000, 3500]
+# plain FP64; mixed (FP64 K products, FP32 V-cycle); mixed with FP32 K products in iterative refinement
+PRECISIONS = [(SVXG_FP64, {}), (SVXG_MIXED, {}), (SVXG_MIXED, {"mixed_fp64_rows": 0})]
+PREC_IDS = ["fp64", "mixed", "mixed_ir"]
 
 
 def graphs():
@@ -21,10 +24,10 @@ def bits(a):
 
 
 @pytest.mark.parametrize("min_nodes", [0, 160], ids=["all_multigrid", "small_jacobi"])
-@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
-def test_batch_equals_alone_bitwise(precision, min_nodes, mats, ctx):
+@pytest.mark.parametrize("prec", PRECISIONS, ids=PREC_IDS)
+def test_batch_equals_alone_bitwise(prec, min_nodes, mats, ctx):
     gs = graphs()
-    opts = stress.default_options(precision=precision, amg_min_nodes=min_nodes)
+    opts = stress.default_options(precision=prec[0], amg_min_nodes=min_nodes, **prec[1])
     alone = []
     for g in gs:
         p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
@@ -41,11 +44,12 @@ def test_batch_equals_alone_bitwise(precision, min_nodes, mats, ctx):
         assert np.array_equal(bits(us[k]), bits(alone[k][0])), SIZES[k]
 
 
-@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
-def test_batch_matches_oracle(precision, mats, ctx):
+@pytest.mark.parametrize("prec", PRECISIONS, ids=PREC_IDS)
+def test_batch_matches_oracle(prec, mats, ctx):
     gs = graphs()
+    precision = prec[0]
     tol = 1e-6 if precision == SVXG_FP64 else 1e-5
-    opts = stress.default_options(precision=precision, amg_min_nodes=0)
+    opts = stress.default_options(precision=precision, amg_min_nodes=0, **prec[1])
     probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
     assert all(stress.assemble_batch(probs))
     us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-10, maxit=5000)
@@ -71,8 +75,8 @@ def test_batch_warm_start_and_resident_loads(mats, ctx):
         assert rs2[k]["converged"] and rs2[k]["iters"] <= 1
 
 
-@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
-def test_resident_steps_equal_tiled_steps_bitwise(precision, mats):
+@pytest.mark.parametrize("prec", PRECISIONS, ids=PREC_IDS)
+def test_resident_steps_equal_tiled_steps_bitwise(prec, mats):
     """Graphs stepped in one workgroup each give the bits the tiled launches give (the same device
     functions on the same tiles in the same order)."""
     gs = graphs()[:10]
@@ -80,7 +84,7 @@ def test_resident_steps_equal_tiled_steps_bitwise(precision, mats):
     for rows in (0, 4096):
         c = stress.Context(0)
         c.set_resident_rows(rows)
-        opts = stress.default_options(precision=precision, amg_min_nodes=0 if rows else 160)
+        opts = stress.default_options(precision=prec[0], amg_min_nodes=0 if rows else 160, **prec[1])
         opts.amg_min_nodes = 0
         probs = [stress.StressProblem(g.nodes, g.bonds, mats, c, options=opts) for g in gs]
         us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-9, maxit=5000)

@@ -45,6 +45,10 @@ def cases():
 
 CASES = cases()
 IDS = [c[0] for c in CASES]
+# the solve's precision: plain FP64; mixed (FP64 K products, FP32 V-cycle); mixed with FP32 K
+# products too, inside FP64 iterative refinement
+PRECISIONS = [(SVXG_FP64, {}), (SVXG_MIXED, {}), (SVXG_MIXED, {"mixed_fp64_rows": 0})]
+PREC_IDS = ["fp64", "mixed", "mixed_ir"]
 
 
 def failed_sets_agree(phi, phi_ref, band=1e-4):
@@ -84,16 +88,17 @@ def test_block_product_bit_identical(case, mats, ctx):
 
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)
-@pytest.mark.parametrize("precision", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
-def test_solution_matches_oracle(case, precision, mats, ctx):
+@pytest.mark.parametrize("prec", PRECISIONS, ids=PREC_IDS)
+def test_solution_matches_oracle(case, prec, mats, ctx):
     name, nodes, bonds, f, kw = case
+    precision, pkw = prec
     tol = 1e-6 if precision == SVXG_FP64 else 1e-5
     o = CpuProblem(nodes, bonds, mats)
     assert o.assemble(reference_options(**kw))
     uo, ro = o.solve(f, rtol=1e-10, maxit=20000)
     assert ro["converged"]
     d = stress.StressProblem(nodes, bonds, mats, ctx)
-    assert d.assemble(stress.default_options(precision=precision, **kw))
+    assert d.assemble(stress.default_options(precision=precision, **kw, **pkw))
     ud, rd = d.solve(f, rtol=1e-10, maxit=20000)
     assert rd["converged"], rd
     assert rd["rel_res"] <= 1e-10

@@ -23,17 +23,19 @@ def bits(a):
     return np.ascontiguousarray(a, np.float64).view(np.uint64)
 
 
-def problems(gs, mats, ctx):
-    return [stress.StressProblem(g.nodes, g.bonds, mats, ctx) for g in gs]
+def problems(gs, mats, ctx, **kw):
+    opts = stress.default_options(**kw) if kw else None
+    return [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
 
 
-@pytest.mark.parametrize("precision", [SVXG_MIXED, SVXG_FP64], ids=["mixed", "fp64"])
-def test_alone_and_in_the_mix_bitwise(precision, mats, ctx):
+@pytest.mark.parametrize("precision,kw", [(SVXG_MIXED, {}), (SVXG_MIXED, {"mixed_fp64_rows": 0}), (SVXG_FP64, {})],
+                         ids=["mixed", "mixed_ir", "fp64"])
+def test_alone_and_in_the_mix_bitwise(precision, kw, mats, ctx):
     gs = graphs()
     # alone: one request at a time
     srv = serve.Server(ctx, precision=precision, rtol=1e-9)
     alone = []
-    for p, g in zip(problems(gs, mats, ctx), gs):
+    for p, g in zip(problems(gs, mats, ctx, **kw), gs):
         t = srv.submit(p, g.f)
         rec = srv.wait([t])[t]
         assert rec["status"] == SVXG_OK and rec["result"]["converged"], rec
@@ -41,7 +43,7 @@ def test_alone_and_in_the_mix_bitwise(precision, mats, ctx):
     srv.close()
     # the mix: staggered arrivals into a running batch (small windows: many admissions mid-solve)
     srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=2)
-    ps = problems(gs, mats, ctx)
+    ps = problems(gs, mats, ctx, **kw)
     order = [8, 0, 5, 2, 7, 1, 3, 6, 4]
     tickets = {}
     for k, i in enumerate(order):
