@@ -187,7 +187,7 @@ class _Piece:
         self.h, self.n, self.g = h, g.n, g
 
 
-def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11):
+def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11, setup_workers=None):
     """Serving stream: 60 Hz ticks; each tick, Poisson(lam) new structures (sizes as the batched
     leg's, half on supports, half pinned; multigrid at every size, as pieces are) arrive and are solved cold (setup + solve), and 10 % of the
     resident structures that are idle lose 1-5 % of their bonds and are solved again, warm, on their
@@ -204,7 +204,8 @@ def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11):
            "pool_graphs": pool_size, "per_lambda": []}
     period = 1.0 / 60.0
     for lam in lams:
-        srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-8, maxit=5000, amg_min_nodes=0, window=4)
+        skw = {} if setup_workers is None else {"setup_workers": setup_workers}
+        srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-8, maxit=5000, amg_min_nodes=0, window=4, **skw)
         residents = []  # [piece, birth tick, cut offset]
         busy = {}  # ticket -> (tick, piece index or None, dofs)
         in_flight = set()
@@ -425,6 +426,7 @@ def main():
     ap.add_argument("--serve-warmup", type=int, default=30)
     ap.add_argument("--serve-life", type=int, default=20)
     ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--serve-setup-workers", type=int, default=None)
     ap.add_argument("--city", type=int, default=400, help="buildings in the city-chunk leg (0: skip)")
     ap.add_argument("--city-rounds", type=int, default=3)
     ap.add_argument("--city-rate", type=float, default=20.0, help="collapse events per second")
@@ -506,7 +508,7 @@ def main():
         lams = [int(x) for x in args.serve_lambdas.split(",") if x]
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
         line["serving"] = run_serving(mats, ctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
-                                      args.serve_pool)
+                                      args.serve_pool, setup_workers=args.serve_setup_workers)
     if args.city > 0:
         line["city"] = run_city(mats, ctx, args.city, args.city_rounds, args.city_rate)
     if not args.no_cpu:
