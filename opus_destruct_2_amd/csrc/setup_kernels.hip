@@ -202,24 +202,33 @@ __device__ __forceinline__ uint64_t mis_tuple(int state, int i) {
 
 // exclusive scan of a[0..len) in place, a[len] = total, one workgroup
 __device__ void block_scan(int32_t* a, int len, int32_t* total_out) {
-  __shared__ int32_t buf[kWG];
+  // four consecutive values a thread, a wavefront prefix sum of the threads' sums by shuffles, the
+  // wavefronts' totals through LDS: three barriers a chunk of 4 kWG values
+  __shared__ int32_t wsum[kWG / 64];
   __shared__ int32_t carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (int base = 0; base < len; base += kWG) {
-    const int i = base + int(threadIdx.x);
-    const int32_t v = i < len ? a[i] : 0;
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < kWG; o <<= 1) {
-      const int32_t t = threadIdx.x >= unsigned(o) ? buf[threadIdx.x - o] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += t;
-      __syncthreads();
+  const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6;
+  for (int base = 0; base < len; base += 4 * kWG) {
+    const int i0 = base + 4 * int(threadIdx.x);
+    int32_t v[4];
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < len ? a[i0 + k] : 0;
+    const int32_t mine = v[0] + v[1] + v[2] + v[3];
+    int32_t incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
     }
-    if (i < len) a[i] = carry + buf[threadIdx.x] - v;
+    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    if (threadIdx.x == kWG - 1) carry += buf[kWG - 1];
+    int32_t run = carry + incl - mine;
+    for (int q = 0; q < w; ++q) run += wsum[q];
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < len) a[i0 + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == kWG - 1) carry = run;
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -658,7 +667,7 @@ __global__ void k_p_cols(const CoarsenDev* __restrict__ J, const Tile* __restric
 
 // P's blocks, one entry a thread: (J == agg(i) ? P_rigid,i : 0) - w sum over couplings k of row i
 // with agg(col k) = J of D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order
-__global__ void k_p_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
+__global__ void k_p_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = etiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
@@ -695,6 +704,42 @@ __global__ void k_p_fill(const CoarsenDev* __restrict__ J, const Tile* __restric
   for (int q = 0; q < 36; ++q) out[q] = acc[q];
 }
 
+// P's blocks: (J == agg(i) ? P_rigid,i : 0) - w sum over couplings k of row i with agg(col k) = J
+// of D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order. 36 lanes an entry
+// (lane = element (r, c)), seven entries a workgroup (entry tiles of kFillEntries)
+__global__ void __launch_bounds__(kWG) k_p_fill_lanes(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles,
+                                                int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = etiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  const int el = int(threadIdx.x) / 36, q = int(threadIdx.x) % 36;
+  if (el >= tl.nrows) return;
+  const int e = tl.row0 + el;
+  const int i = c.rid[e];
+  const int Jc = c.Pcol[e];
+  const int r = q / 6, col = q % 6;
+  const double* Di = c.Dinv + 36 * size_t(i);
+  double acc = (Jc == c.agg[i]) ? rigid_at(c.off + 3 * size_t(i), r, col) : 0.0;
+  if (c.omega != 0.0) {
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+      const int j = c.col[k];
+      if (c.agg[j] != Jc) continue;
+      const double* A = c.A + 36 * size_t(k);
+      const double* offj = c.off + 3 * size_t(j);
+      double T[6];
+      for (int rr = 0; rr < 6; ++rr) {
+        double s2 = 0.0;
+        for (int p = 0; p < 6; ++p) s2 += A[rr * 6 + p] * rigid_at(offj, p, col);
+        T[rr] = s2;
+      }
+      double s2 = 0.0;
+      for (int p = 0; p < 6; ++p) s2 += Di[r * 6 + p] * T[p];
+      acc += -c.omega * s2;
+    }
+  }
+  c.P[36 * size_t(e) + size_t(q)] = acc;
+}
+
 // restriction lists: P's entries by coarse column, fine rows ascending
 __global__ void k_r_count(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
@@ -719,26 +764,53 @@ __global__ void k_r_fill(const CoarsenDev* __restrict__ J, const Tile* __restric
   }
 }
 
-// per coarse row: entries sorted by fine row (the pairs move together)
-__global__ void k_r_sort(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
-  if (int(blockIdx.x) >= nctiles) return;
-  const Tile tl = ctiles[blockIdx.x];
+// per coarse row: entries sorted by fine row (the pairs move together). A wavefront a row (four
+// rows a workgroup, 64 workgroups a coarse-row tile): each entry's rank is the count of smaller
+// fine rows in the row (they are distinct), the pairs land at their ranks through LDS; a row longer
+// than the LDS holds is sorted by one lane
+constexpr int kRSortCap = 1024;
+__global__ void __launch_bounds__(kWG) k_r_sort(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles,
+                                                int nctiles) {
+  const int tile = int(blockIdx.x) >> 6, part = int(blockIdx.x) & 63;
+  if (tile >= nctiles) return;
+  const Tile tl = ctiles[tile];
   const CoarsenDev& c = J[tl.g];
-  if (int(threadIdx.x) >= tl.nrows) return;
-  const int I = tl.row0 + threadIdx.x;
+  const int w = int(threadIdx.x) >> 6, l = int(threadIdx.x) & 63;
+  const int rl = 4 * part + w;
+  if (rl >= tl.nrows) return;
+  const int I = tl.row0 + rl;
   const int b = c.Rrow[I], n = c.Rrow[I + 1] - b;
   int32_t* f = c.Rfine + b;
   int32_t* en = c.Rent + b;
-  for (int s = 1; s < n; ++s) {
-    const int32_t fv = f[s], ev = en[s];
-    int t = s - 1;
-    while (t >= 0 && f[t] > fv) {
-      f[t + 1] = f[t];
-      en[t + 1] = en[t];
-      --t;
+  __shared__ int32_t sf[4][kRSortCap], se[4][kRSortCap];
+  if (n <= kRSortCap) {
+    for (int t = l; t < n; t += 64) {
+      sf[w][t] = f[t];
+      se[w][t] = en[t];
     }
-    f[t + 1] = fv;
-    en[t + 1] = ev;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int t = l; t < n; t += 64) {
+      const int32_t key = sf[w][t];
+      int rank = 0;
+      for (int u = 0; u < n; ++u) rank += sf[w][u] < key ? 1 : 0;
+      f[rank] = key;
+      en[rank] = se[w][t];
+    }
+    return;
+  }
+  if (l != 0) return;
+  for (int t = 1; t < n; ++t) {
+    const int32_t fv = f[t], ev = en[t];
+    int u = t - 1;
+    while (u >= 0 && f[u] > fv) {
+      f[u + 1] = f[u];
+      en[u + 1] = en[u];
+      --u;
+    }
+    f[u + 1] = fv;
+    en[u + 1] = ev;
   }
 }
 
@@ -790,18 +862,38 @@ __global__ void k_c_cand_count(const CoarsenDev* __restrict__ J, const Tile* __r
 }
 
 // the coarse row I's candidate columns (unsorted): I, then the A P columns of its restriction rows
-__global__ void k_c_cand_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles, int nctiles) {
-  if (int(blockIdx.x) >= nctiles) return;
-  const Tile tl = ctiles[blockIdx.x];
+// in order. A wavefront a row (four rows a workgroup, 64 workgroups a coarse-row tile): each lane
+// copies one restriction row's columns to its offset (a wavefront prefix sum of the row lengths)
+__global__ void __launch_bounds__(kWG) k_c_cand_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ ctiles,
+                                                     int nctiles) {
+  const int tile = int(blockIdx.x) >> 6, part = int(blockIdx.x) & 63;
+  if (tile >= nctiles) return;
+  const Tile tl = ctiles[tile];
   const CoarsenDev& c = J[tl.g];
-  if (int(threadIdx.x) >= tl.nrows) return;
-  const int I = tl.row0 + threadIdx.x;
+  const int w = int(threadIdx.x) >> 6, l = int(threadIdx.x) & 63;
+  const int rl = 4 * part + w;
+  if (rl >= tl.nrows) return;
+  const int I = tl.row0 + rl;
   int32_t* cd = c.cand + c.cptr[I];
-  int n = 0;
-  cd[n++] = I;
-  for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
-    const int i = c.Rfine[r];
-    for (int x = c.AProw[i]; x < c.AProw[i + 1]; ++x) cd[n++] = c.APcol[x];
+  if (l == 0) cd[0] = I;
+  int base = 1;
+  const int r1 = c.Rrow[I + 1];
+  for (int r0 = c.Rrow[I]; r0 < r1; r0 += 64) {
+    const int r = r0 + l;
+    int x0 = 0, len = 0;
+    if (r < r1) {
+      const int i = c.Rfine[r];
+      x0 = c.AProw[i];
+      len = c.AProw[i + 1] - x0;
+    }
+    int incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (l >= o) incl += t;
+    }
+    int32_t* dst = cd + base + incl - len;
+    for (int x = 0; x < len; ++x) dst[x] = c.APcol[x0 + x];
+    base += __shfl(incl, 63, 64);
   }
 }
 
@@ -918,7 +1010,7 @@ __global__ void k_pattern_cols(const CoarsenDev* __restrict__ J, const Tile* __r
 
 // (A P)_iJ, one entry a thread: the sum over couplings k of row i (in order) of A_k P_e, e the entry
 // of column J in P's row col k (P rows ascending: a binary search)
-__global__ void k_ap_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
+__global__ void k_ap_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = etiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
@@ -950,9 +1042,42 @@ __global__ void k_ap_fill(const CoarsenDev* __restrict__ J, const Tile* __restri
   for (int q = 0; q < 36; ++q) out[q] = acc[q];
 }
 
+// (A P)_iJ: the sum over couplings k of row i (in order) of A_k P_e, e the entry of column J in P's
+// row col k (P rows ascending: a binary search). 36 lanes an entry (lane = element), seven entries a
+// workgroup
+__global__ void __launch_bounds__(kWG) k_ap_fill_lanes(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles,
+                                                 int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = etiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  const int el = int(threadIdx.x) / 36, q = int(threadIdx.x) % 36;
+  if (el >= tl.nrows) return;
+  const int x = tl.row0 + el;
+  const int i = c.rid[x];
+  const int Jc = c.APcol[x];
+  const int r = q / 6, col = q % 6;
+  double acc = 0.0;
+  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
+    const int j = c.col[k];
+    int lo = c.Prow[j], hi = c.Prow[j + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (c.Pcol[mid] < Jc) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= c.Prow[j + 1] || c.Pcol[lo] != Jc) continue;
+    const double* A = c.A + 36 * size_t(k) + 6 * r;
+    const double* P = c.P + 36 * size_t(lo) + col;
+    double s2 = 0.0;
+    for (int t = 0; t < 6; ++t) s2 += A[t] * P[6 * t];
+    acc = acc + s2;
+  }
+  c.AP[36 * size_t(x) + size_t(q)] = acc;
+}
+
 // (P^T A P)_IJ, one entry a thread: the sum over restriction entries (i, e) of I, in order, of
 // P_e^T (A P)_iJ; times the coarse scale (unsmoothed levels)
-__global__ void k_c_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
+__global__ void k_c_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = etiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
@@ -982,6 +1107,38 @@ __global__ void k_c_fill(const CoarsenDev* __restrict__ J, const Tile* __restric
   }
   double* out = c.C + 36 * size_t(y);
   for (int q = 0; q < 36; ++q) out[q] = c.coarse_scale != 1.0 ? acc[q] * c.coarse_scale : acc[q];
+}
+
+// (P^T A P)_IJ: the sum over restriction entries (i, e) of I, in order, of P_e^T (A P)_iJ; times
+// the coarse scale (unsmoothed levels). 36 lanes an entry (lane = element), seven entries a workgroup
+__global__ void __launch_bounds__(kWG) k_c_fill_lanes(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles,
+                                                int ntiles) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = etiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  const int el = int(threadIdx.x) / 36, q = int(threadIdx.x) % 36;
+  if (el >= tl.nrows) return;
+  const int y = tl.row0 + el;
+  const int I = c.rid[y];
+  const int Jc = c.Ccol[y];
+  const int rr = q / 6, col = q % 6;
+  double acc = 0.0;
+  for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
+    const int i = c.Rfine[r];
+    int lo = c.AProw[i], hi = c.AProw[i + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (c.APcol[mid] < Jc) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= c.AProw[i + 1] || c.APcol[lo] != Jc) continue;
+    const double* P = c.P + 36 * size_t(c.Rent[r]) + rr;
+    const double* X = c.AP + 36 * size_t(lo) + col;
+    double s2 = 0.0;
+    for (int t = 0; t < 6; ++t) s2 += P[6 * t] * X[6 * t];
+    acc = acc + s2;
+  }
+  c.C[36 * size_t(y) + size_t(q)] = c.coarse_scale != 1.0 ? acc * c.coarse_scale : acc;
 }
 
 // ---- power iteration (6 lanes a row, kTileRows-row tiles) ------------------------------------------
@@ -1478,8 +1635,10 @@ void launch_scan_rows(hipStream_t s, const CoarsenDev* jobs, int count, int whic
 void launch_p_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_p_cols, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
-void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_p_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes) {
+  if (ntiles <= 0) return;
+  if (lanes) hipLaunchKernelGGL(k_p_fill_lanes, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+  else hipLaunchKernelGGL(k_p_fill_thread, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
 }
 void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
                     int nctiles, int count) {
@@ -1487,7 +1646,7 @@ void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, in
   hipLaunchKernelGGL(k_r_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
   hipLaunchKernelGGL(k_scan_rows, dim3(count), dim3(kWG), 0, s, jobs, count, 1);
   hipLaunchKernelGGL(k_r_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
-  hipLaunchKernelGGL(k_r_sort, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+  hipLaunchKernelGGL(k_r_sort, dim3(64 * nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
 void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_ap_cand_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
@@ -1499,7 +1658,7 @@ void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctil
   if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
 void launch_c_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
-  if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_fill, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
+  if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_fill, dim3(64 * nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
 void launch_unique_rows(hipStream_t s, const CoarsenDev* jobs, const Tile* utiles, int ntiles, int which) {
   if (ntiles > 0) hipLaunchKernelGGL(k_unique_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, utiles, ntiles, which);
@@ -1507,11 +1666,15 @@ void launch_unique_rows(hipStream_t s, const CoarsenDev* jobs, const Tile* utile
 void launch_pattern_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int which) {
   if (ntiles > 0) hipLaunchKernelGGL(k_pattern_cols, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, which);
 }
-void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_ap_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes) {
+  if (ntiles <= 0) return;
+  if (lanes) hipLaunchKernelGGL(k_ap_fill_lanes, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+  else hipLaunchKernelGGL(k_ap_fill_thread, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
 }
-void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_c_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes) {
+  if (ntiles <= 0) return;
+  if (lanes) hipLaunchKernelGGL(k_c_fill_lanes, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
+  else hipLaunchKernelGGL(k_c_fill_thread, dim3(ntiles), dim3(kWG), 0, s, jobs, etiles, ntiles);
 }
 void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int count) {
   if (ntiles <= 0) return;

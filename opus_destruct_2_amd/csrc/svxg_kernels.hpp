@@ -190,7 +190,7 @@ void launch_coarse_positions(hipStream_t s, const CoarsenDev* jobs, const Tile* 
 void launch_p_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
 void launch_scan_rows(hipStream_t s, const CoarsenDev* jobs, int count, int which);
 void launch_p_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
-void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles);
+void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes);
 void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
                     int nctiles, int count);
 // Galerkin product: candidate counts and candidates per row, distinct sorted columns (one
@@ -201,8 +201,8 @@ void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctil
 void launch_c_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
 void launch_unique_rows(hipStream_t s, const CoarsenDev* jobs, const Tile* utiles, int ntiles, int which);
 void launch_pattern_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int which);
-void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles);
-void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles);
+void launch_ap_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes);
+void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes);
 
 // power iteration for lambda_max(D^-1 A) (amg.cpp:256-275): 15 steps, per-graph norms
 struct PowerDev {
@@ -221,6 +221,12 @@ void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, i
 void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
 
 // the coarsest level's dense inverse (amg.cpp:535-578, then inverted)
+// the P, A P and P^T A P fills run 36 lanes an entry (one a block element, seven entries a
+// workgroup) when a setup has fewer than kFillLanesMax entries to fill (latency-bound), else one
+// thread an entry (throughput-bound); both give the same bits
+constexpr int kFillEntries = kWG / 36;
+constexpr int64_t kFillLanesMax = 200000;
+
 struct DenseDev {
   int32_t n;
   int32_t m;
