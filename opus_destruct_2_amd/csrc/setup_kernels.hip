@@ -901,8 +901,79 @@ constexpr int kHashSlots = 16384;  // LDS hash set (64 KB)
 constexpr int kSortSlots = 4096;   // LDS sort buffer (16 KB): at most this many distinct columns a row
 
 // Each row's candidate segment (cand[cptr[r] ..]) made its sorted distinct columns in place, their
-// count in out[r] (diagonal first when `diag`: coarse rows). One workgroup per row of the tile: an
-// LDS hash set (order-independent, so the result is deterministic), then a bitonic sort.
+// count in out[r] (the diagonal first when which = 1: coarse rows, as Bsr6Builder::finish). A
+// wavefront a row (tiles of four rows): an LDS hash set (order-independent, so the result is
+// deterministic), the occupied slots compacted by a wavefront prefix sum, each column placed at its
+// rank. A row whose distinct columns overflow the wavefront's share of LDS is left with out[r] = -1
+// for the workgroup pass (k_unique_rows) that follows.
+constexpr int kUHash = 2048;  // hash slots a wavefront
+constexpr int kUCap = 1024;   // distinct columns a wavefront places
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+__global__ void __launch_bounds__(kWG) k_unique_rows_wave(const CoarsenDev* __restrict__ J,
+                                                          const Tile* __restrict__ tiles, int ntiles, int which) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const CoarsenDev& c = J[tl.g];
+  const int w = int(threadIdx.x) >> 6, l = int(threadIdx.x) & 63;
+  if (w >= tl.nrows) return;
+  __shared__ int32_t tab[4][kUHash];
+  __shared__ int32_t keys[4][kUCap];
+  const int r = tl.row0 + w;
+  int32_t* out = which == 0 ? c.AProw : c.Crow;
+  int32_t* cd = c.cand + c.cptr[r];
+  const int len = c.cptr[r + 1] - c.cptr[r];
+  int H = 64;
+  while (H < 2 * len && H < kUHash) H <<= 1;
+  for (int h = l; h < H; h += 64) tab[w][h] = -1;
+  wave_lds_fence();
+  bool full = false;
+  for (int k = l; k < len; k += 64) {
+    const int32_t key = cd[k];
+    uint32_t h = (uint32_t(key) * 2654435761U) & uint32_t(H - 1);
+    for (int probes = 0;; ++probes) {
+      if (probes >= H) {
+        full = true;
+        break;
+      }
+      const int32_t old = atomicCAS(&tab[w][h], -1, key);
+      if (old == -1 || old == key) break;
+      h = (h + 1) & uint32_t(H - 1);
+    }
+  }
+  wave_lds_fence();
+  const int per = H / 64;
+  int mine = 0;
+  for (int h = l * per; h < (l + 1) * per; ++h) mine += tab[w][h] >= 0;
+  int incl = mine;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (l >= o) incl += t;
+  }
+  const int u = __shfl(incl, 63, 64);
+  if (__any(full) || u > kUCap) {
+    if (l == 0) out[r] = -1;
+    return;
+  }
+  int at = incl - mine;
+  for (int h = l * per; h < (l + 1) * per; ++h)
+    if (tab[w][h] >= 0) keys[w][at++] = tab[w][h];
+  wave_lds_fence();
+  for (int t = l; t < u; t += 64) {
+    const int32_t key = keys[w][t];
+    int rank = 0;
+    for (int q = 0; q < u; ++q) rank += keys[w][q] < key ? 1 : 0;
+    if (which == 1) rank = key == r ? 0 : rank + (key < r ? 1 : 0);
+    cd[rank] = key;
+  }
+  if (l == 0) out[r] = u;
+}
+
+// The rows k_unique_rows_wave left (out[r] = -1): one workgroup a row, an LDS hash set, then a
+// bitonic sort (at most kSortSlots distinct columns)
 __global__ void __launch_bounds__(kWG) k_unique_rows(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles,
                                                      int ntiles, int which) {
   if (int(blockIdx.x) >= ntiles) return;
@@ -915,6 +986,7 @@ __global__ void __launch_bounds__(kWG) k_unique_rows(const CoarsenDev* __restric
   int32_t* out = which == 0 ? c.AProw : c.Crow;
   for (int rr = 0; rr < tl.nrows; ++rr) {
     const int r = tl.row0 + rr;
+    if (out[r] != -1) continue;
     int32_t* cd = c.cand + c.cptr[r];
     const int len = c.cptr[r + 1] - c.cptr[r];
     int H = 64;
@@ -1661,7 +1733,9 @@ void launch_c_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* ctile
   if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_fill, dim3(64 * nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
 }
 void launch_unique_rows(hipStream_t s, const CoarsenDev* jobs, const Tile* utiles, int ntiles, int which) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_unique_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, utiles, ntiles, which);
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_unique_rows_wave, dim3(ntiles), dim3(kWG), 0, s, jobs, utiles, ntiles, which);
+  hipLaunchKernelGGL(k_unique_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, utiles, ntiles, which);
 }
 void launch_pattern_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int which) {
   if (ntiles > 0) hipLaunchKernelGGL(k_pattern_cols, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, which);

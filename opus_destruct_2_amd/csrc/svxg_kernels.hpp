@@ -193,8 +193,9 @@ void launch_p_cols(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int
 void launch_p_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, int ntiles, bool lanes);
 void launch_r_build(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, const Tile* ctiles,
                     int nctiles, int count);
-// Galerkin product: candidate counts and candidates per row, distinct sorted columns (one
-// workgroup a row: utiles), the pattern, the blocks one entry a thread. which: 0 A P, 1 P^T (A P)
+// Galerkin product: candidate counts and candidates per row, distinct sorted columns (a wavefront
+// a row, utiles of four rows; a workgroup for the rows too long for that), the pattern, the blocks.
+// which: 0 A P, 1 P^T (A P)
 void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
 void launch_ap_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
 void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles);
