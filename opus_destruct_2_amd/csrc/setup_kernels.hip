@@ -435,6 +435,7 @@ __global__ void k_mis_t1(const CoarsenDev* __restrict__ J, const Tile* __restric
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
+  if (tl.row0 == 0 && threadIdx.x == 0) c.counts[0] = 0;  // (this round's undecided rows, counted below)
   if (int(threadIdx.x) >= tl.nrows) return;
   const int i = tl.row0 + threadIdx.x;
   uint64_t m = mis_tuple(c.state[i], i);
