@@ -1310,11 +1310,15 @@ constexpr int kDenseWaves = kDenseWG / 64;
 
 // lane layout of v_mfma_f64_16x16x4f64 (lane l = 16 g + i): A[i][k] at k = g, B[k][j] at k = g,
 // j = i; the accumulator's register r holds D[g + 4 r][i]
+// a load through a global-address-space pointer (the scratch is device memory; through a generic
+// pointer the compiler emits flat loads)
+__device__ __forceinline__ double gld(const double* p) { return *(const __attribute__((address_space(1))) double*)p; }
+
 __device__ __forceinline__ dbl4 tile_load(const double* T, int ld) {
   const int i = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
   dbl4 c;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) c[r] = T[size_t(g + 4 * r) * size_t(ld) + size_t(i)];
+  for (int r = 0; r < 4; ++r) c[r] = gld(T + size_t(g + 4 * r) * size_t(ld) + size_t(i));
   return c;
 }
 __device__ __forceinline__ void tile_store(double* T, int ld, dbl4 c) {
@@ -1327,13 +1331,17 @@ __device__ __forceinline__ void tile_store(double* T, int ld, dbl4 c) {
 template <bool tX, bool tY>
 __device__ __forceinline__ dbl4 tile_mma(dbl4 c, const double* X, int ldx, const double* Y, int ldy, double sign) {
   const int i = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
+  // (all eight operands loaded before the first product: one memory round trip a tile product)
+  double a[4], b[4];
 #pragma unroll
   for (int kk = 0; kk < 4; ++kk) {
     const int k = 4 * kk + g;
-    const double a = tX ? X[size_t(k) * size_t(ldx) + size_t(i)] : X[size_t(i) * size_t(ldx) + size_t(k)];
-    const double b = tY ? Y[size_t(i) * size_t(ldy) + size_t(k)] : Y[size_t(k) * size_t(ldy) + size_t(i)];
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(sign * a, b, c, 0, 0, 0);
+    a[kk] = tX ? X[size_t(k) * size_t(ldx) + size_t(i)] : X[size_t(i) * size_t(ldx) + size_t(k)];
+    b[kk] = tY ? Y[size_t(i) * size_t(ldy) + size_t(k)] : Y[size_t(k) * size_t(ldy) + size_t(i)];
+    // (X, Y: global scratch or LDS, so the generic pointer stays)
   }
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) c = __builtin_amdgcn_mfma_f64_16x16x4f64(sign * a[kk], b[kk], c, 0, 0, 0);
   return c;
 }
 // LDS writes of this wave visible to its other lanes
@@ -1351,27 +1359,27 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
   double* L = d.L;  // mp x mp: the matrix, then its factor (lower tiles)
   double* W = d.W;  // mp x mp: L^-1 (lower tiles)
   auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(mp) + size_t(c)]; };
-  const size_t mm = size_t(mp) * size_t(mp);
-  for (size_t t = tid; t < mm; t += kDenseWG) {
-    const int r = int(t / size_t(mp)), c = int(t % size_t(mp));
-    L[t] = (r == c && r >= m) ? 1.0 : 0.0;
-    W[t] = 0.0;
-  }
-  __syncthreads();
-  for (int i = tid; i < d.n; i += kDenseWG)
-    for (int k = d.rowptr[i]; k < d.rowptr[i + 1]; ++k) {
-      const int j = d.col[k];
-      const double* b = d.A + 36 * size_t(k);
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) at(6 * i + r, 6 * j + c) += b[r * 6 + c];
+  // (rows a wave, columns its lanes)
+  for (int r = wave; r < mp; r += kDenseWaves)
+    for (int c = lane; c < mp; c += 64) {
+      at(r, c) = (r == c && r >= m) ? 1.0 : 0.0;
+      W[size_t(r) * size_t(mp) + size_t(c)] = 0.0;
     }
+  __syncthreads();
+  // the blocks, an element a thread (a block row's blocks have distinct columns: no two write one place)
+  for (int i = wave; i < d.n; i += kDenseWaves) {
+    const int k0 = d.rowptr[i], k1 = d.rowptr[i + 1];
+    for (int t = lane; t < 36 * (k1 - k0); t += 64) {
+      const int k = k0 + t / 36, e = t % 36;
+      at(6 * i + e / 6, 6 * d.col[k] + e % 6) = d.A[36 * size_t(k) + size_t(e)];
+    }
+  }
   __syncthreads();
   __shared__ double red[kDenseWaves];
   double dm = 0.0;
-  for (size_t t = tid; t < size_t(m) * size_t(m); t += kDenseWG) {
-    const int i = int(t / size_t(m)), j = int(t % size_t(m));
-    if (i > j) at(i, j) = 0.5 * (at(i, j) + at(j, i));
-    if (i == j) dm = fmax(dm, fabs(at(i, i)));
+  for (int i = wave; i < m; i += kDenseWaves) {
+    for (int j = lane; j < i; j += 64) at(i, j) = 0.5 * (at(i, j) + at(j, i));
+    if (lane == 0) dm = fmax(dm, fabs(at(i, i)));
   }
   for (int o = 32; o > 0; o >>= 1) dm = fmax(dm, __shfl_xor(dm, o, 64));
   if (lane == 0) red[wave] = dm;
