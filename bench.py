@@ -3,8 +3,11 @@
 Workload (the metric's single-GPU configuration): one irregular block of 64 x 64 x 48 fragments with
 15 % voids (about 167k nodes, 1.0M DOFs, stiffness ratio up to 20:1), solved to a relative residual
 of 1e-8 and bond-checked. A step is one solve from a cold start plus the bond checks on the
-device-resident solution. The headline runs the mixed path (FP32 K products and V-cycle inside FP64
-iterative refinement); the plain FP64 path is measured beside it.
+device-resident solution. The headline runs the mixed path (FP64 PCG and K products around the FP32
+V-cycle); the plain FP64 path (FP64 V-cycle) and the mixed split with FP32 K products inside FP64
+iterative refinement are measured beside it. Further legs, each reported in the same line: the
+batched uneven graphs (4096 pieces), the serving stream (60 Hz ticks) and the city chunk (400
+resident buildings with ground-storey collapses).
 
 value: solved DOFs / (device time of the solve + wall time of the checks), inputs resident.
 roofline: the dominant kernel's algorithmic bytes per launch / its average launch time, measured
@@ -13,6 +16,7 @@ cpu_baseline: the reference's own solver (compiled from its sources into oracle/
 workload, one assemble + solve, all host threads (up to 16).
 
   python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu] [--gpus 1]
+  python bench.py --only batch|serve|city ...   (one leg alone, for profiling)
 """
 import argparse
 import json
@@ -437,6 +441,8 @@ def main():
                     help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
+    ap.add_argument("--only", choices=["batch", "serve", "city"], default=None,
+                    help="profiling: run only this leg and print its record (not the benchmark line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -469,6 +475,17 @@ def main():
     ctx = stress.default_context()
     if args.resident_rows >= 0:
         ctx.set_resident_rows(args.resident_rows)
+    if args.only is not None:
+        if args.only == "batch":
+            rec = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
+        elif args.only == "serve":
+            rec = run_serving(mats, ctx, [int(x) for x in args.serve_lambdas.split(",") if x], args.serve_ticks,
+                              args.serve_warmup, args.serve_life, args.serve_pool,
+                              setup_workers=args.serve_setup_workers)
+        else:
+            rec = run_city(mats, ctx, args.city, args.city_rounds, args.city_rate)
+        print(json.dumps({"only": args.only, "record": rec}), flush=True)
+        return
     head_prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
     other_prec = SVXG_FP64 if head_prec == SVXG_MIXED else SVXG_MIXED
     head = run_path(g, mats, head_prec, args.steps, args.warmup, ctx)

@@ -197,13 +197,16 @@ struct StagePart {
   size_t bytes;
 };
 
+// (and the copy kernel takes a workgroup a segment: segments of at most kStagePiece bytes)
+static constexpr size_t kStagePiece = size_t(256) << 10;
+
 static std::vector<StagePart> stage_parts(const std::vector<StagePart>& jobs) {
   std::vector<StagePart> out;
   out.reserve(jobs.size());
   for (const StagePart& j : jobs)
-    for (size_t o = 0; o < j.bytes; o += kStageCap)
+    for (size_t o = 0; o < j.bytes; o += kStagePiece)
       out.push_back({static_cast<char*>(j.dst) + o, static_cast<const char*>(j.src) + o,
-                     std::min(kStageCap, j.bytes - o)});
+                     std::min(kStagePiece, j.bytes - o)});
   return out;
 }
 
@@ -327,13 +330,21 @@ class BulkOps {
   explicit BulkOps(svxg_ctx* c) : c_(c) {}
   void zero(void* dst, size_t bytes) {
     if (!bytes) return;
-    if (bytes >= kBigBytes) big_zero_.push_back(BulkSeg{dst, nullptr, int64_t(bytes)});
-    else zero_.push_back(BulkSeg{dst, nullptr, int64_t(bytes)});
+    if (bytes >= kBigBytes) {
+      big_zero_.push_back(BulkSeg{dst, nullptr, int64_t(bytes)});
+      return;
+    }
+    for (size_t o = 0; o < bytes; o += kPieceBytes)
+      zero_.push_back(BulkSeg{static_cast<char*>(dst) + o, nullptr, int64_t(std::min(kPieceBytes, bytes - o))});
   }
   void to_f32(const double* src, float* dst, int64_t n) {
     if (n <= 0) return;
-    if (size_t(n) * 8 >= kBigBytes) big_f32_.push_back(BulkSeg{dst, src, n});
-    else f32_.push_back(BulkSeg{dst, src, n});
+    if (size_t(n) * 8 >= kBigBytes) {
+      big_f32_.push_back(BulkSeg{dst, src, n});
+      return;
+    }
+    const int64_t per = int64_t(kPieceBytes / 8);
+    for (int64_t o = 0; o < n; o += per) f32_.push_back(BulkSeg{dst + o, src + o, std::min(per, n - o)});
   }
   int flush(hipStream_t s) {
     for (const BulkSeg& g : big_zero_) SVXG_HIP(hipMemsetAsync(g.dst, 0, size_t(g.count), s));
@@ -362,6 +373,7 @@ class BulkOps {
 
  private:
   static constexpr size_t kBigBytes = size_t(4) << 20;
+  static constexpr size_t kPieceBytes = size_t(256) << 10;  // (a workgroup a piece)
   svxg_ctx* c_;
   std::vector<BulkSeg> zero_, f32_, big_zero_, big_f32_;
 };
