@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one live launch of a kernel on a workload, from two rocprofv3 counter passes of this
 # script (FETCH_SIZE scaled by 2.0, the factor a plain 1 GiB copy reads it with; WRITE_SIZE as read)
-MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 215.6e6}
+MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 202.7e6}
 METRIC = "PCG-solved DOFs/s at 1e-8 residual, batched vs plain; p50/p99 tick latency"
 
 
