@@ -117,17 +117,20 @@ __device__ __forceinline__ const LevelArrays<float>& arrays<float>(const LevelDe
   return L.f32;
 }
 
-// the graph takes part in a launch serving `mode`
+// the graph takes part in a launch serving `mode` (state, phase, inner state and product precision
+// read in one 16-byte load)
 __device__ __forceinline__ bool live(const GraphDev& g, int mode) {
-  const PcgState* s = g.st;
-  if (s->state != ST_RUNNING) return false;
-  if (mode == SERVE_OUTER) return s->phase == PH_ITER;
-  if (mode == SERVE_NEW) return s->phase == PH_NEW;
-  if (mode == SERVE_REFINE) return s->phase == PH_OUTER;
-  if (s->inner_state != ST_RUNNING) return false;
-  if (!(mode & (s->inner64 ? SERVE_P64 : SERVE_P32))) return false;
-  return (mode & SERVE_ST) ? s->phase == PH_START || ((mode & SERVE_IT) && s->phase == PH_INNER)
-                           : (mode & SERVE_IT) && s->phase == PH_INNER;
+  typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+  const i32x4 f = ldg(reinterpret_cast<const i32x4*>(&g.st->state));
+  const int32_t state = f.x, phase = f.y, inner_state = f.z, inner64 = f.w;
+  if (state != ST_RUNNING) return false;
+  if (mode == SERVE_OUTER) return phase == PH_ITER;
+  if (mode == SERVE_NEW) return phase == PH_NEW;
+  if (mode == SERVE_REFINE) return phase == PH_OUTER;
+  if (inner_state != ST_RUNNING) return false;
+  if (!(mode & (inner64 ? SERVE_P64 : SERVE_P32))) return false;
+  return (mode & SERVE_ST) ? phase == PH_START || ((mode & SERVE_IT) && phase == PH_INNER)
+                           : (mode & SERVE_IT) && phase == PH_INNER;
 }
 
 // Chebyshev coefficients of step j of the smoother on [lmax / ratio, lmax]: the direction is
@@ -391,8 +394,17 @@ __device__ __forceinline__ void dense_rows(const LevelDev& L, const LevelArrays<
   const int span = (m + nw - 1) / nw, j0 = w * span, j1 = min(m, j0 + span);
   double s = 0.0;
   if (i < m) {
+    // (eight loads in flight, the sum in order)
     const double* col = L.Minv + i;
-    for (int j = j0; j < j1; ++j) s += col[size_t(j) * size_t(m)] * sb[j];
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = ldg(col + size_t(j + u) * size_t(m));
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u] * sb[j + u];
+    }
+    for (; j < j1; ++j) s += ldg(col + size_t(j) * size_t(m)) * sb[j];
   }
   part[threadIdx.x] = s;
   __syncthreads();

@@ -49,12 +49,15 @@ struct PcgState {
   // mixed precision: outer residual norm and the inner solve's stopping norm
   double outer_rn, inner_target, eta, outer_rn_prev;
   double inner_rn;  // the inner solve's latest residual norm (for the host's poll schedule)
-  int32_t iters, maxit, state, outer;
-  int32_t inner_state, inner_iters, inner_maxit, zero_load;
-  // mixed precision: 1 once a refinement step stalled (the FP32 K's error times K's condition
-  // number near 1): this graph's inner solves then take FP64 K products, FP32 V-cycle
-  int32_t inner64;
+  // (the four fields every launch tests, together in 16 bytes: one load)
+  alignas(16) int32_t state;
   int32_t phase;  // PH_*: where the graph is in its solve
+  int32_t inner_state;
+  // mixed precision: 1 when this graph's inner solves take FP64 K products (from the start, or
+  // once a refinement step stalled: the FP32 K's error times K's condition number near 1)
+  int32_t inner64;
+  int32_t iters, maxit, outer;
+  int32_t inner_iters, inner_maxit, zero_load;
 };
 
 // Every graph of a batch runs its own solve: each launch acts on the graphs in the phase (and, for
