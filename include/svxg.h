@@ -257,6 +257,12 @@ int svxg_ctx_set_profiling(svxg_ctx* ctx, int32_t on);
    so every bit of a result, is the same either way; this only moves time. No counterpart in the
    reference. */
 int svxg_ctx_set_resident_rows(svxg_ctx* ctx, int32_t rows);
+/* Tiled solve steps recorded once per batch and precision mix as a HIP graph and replayed, or
+   launched kernel by kernel: on = 0 never recorded, 1 (the default) recorded from a batch's third
+   window of steps for batches of 8192 or more level-0 tiles (below that, measured, the launches are
+   faster), 2 always. Same kernels, same arguments, same bits; the graph saves the host's launches.
+   (Profiled solves launch kernel by kernel.) No counterpart in the reference. */
+int svxg_ctx_set_step_graphs(svxg_ctx* ctx, int32_t on);
 int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------

@@ -433,6 +433,24 @@ struct Batch {
   DevBuf partial;
   DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
   bool any_amg = false;
+  // one tiled step recorded as a HIP graph, by (mixed, FP32 products, FP64 products): the launches'
+  // arguments are this batch's tables, fixed for its life; what a step does to each graph is read
+  // by the kernels from the graph's state, so every replay is the launches made again
+  hipGraphExec_t step_graph[8] = {};
+  bool graphs_failed = false;
+  int windows = 0;  // windows run (a batch records its graph from its third window: short solves
+                    // and short-lived serving batches never pay for a recording)
+  // (measured on MI355X: a replayed step is no faster on the device than the same launches and the
+  // recording costs a step's time, so only batches with at least kGraphMinTiles level-0 tiles, whose
+  // host launch time shows in the wall time, record one)
+  static constexpr int kGraphMinTiles = 8192;
+  Batch() = default;
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+  ~Batch() {
+    for (hipGraphExec_t& g : step_graph)
+      if (g) (void)hipGraphExecDestroy(g);
+  }
 };
 
 static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
@@ -807,6 +825,30 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
     launch_resident_steps(s, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.res.as<int32_t>(), B.nres, w, degree,
                           mixed ? 1 : 0, p32 ? 1 : 0, p64 ? 1 : 0, B.partial.as<double>());
   if (!B.nt0run) return;
+  const svxg_ctx* c = B.probs[0]->ctx;
+  const bool record = c->step_graphs == 2 || (c->step_graphs == 1 && B.nt0run >= Batch::kGraphMinTiles && ++B.windows > 2);
+  if (record && !c->profiling && !B.graphs_failed) {
+    hipGraphExec_t& ex = B.step_graph[(mixed ? 4 : 0) + (p32 ? 2 : 0) + (p64 ? 1 : 0)];
+    if (!ex) {
+      // (recorded on first use; a failed recording leaves this batch to launch kernel by kernel)
+      hipGraph_t gr = nullptr;
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        if (mixed) mixed_step(B, s, degree, p32, p64);
+        else fp64_step(B, s, degree);
+        if (hipStreamEndCapture(s, &gr) != hipSuccess) gr = nullptr;
+      }
+      if (!gr || hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+      if (gr) (void)hipGraphDestroy(gr);
+      if (!ex) {
+        B.graphs_failed = true;
+        (void)hipGetLastError();
+      }
+    }
+    if (ex) {
+      for (int k = 0; k < w; ++k) (void)hipGraphLaunch(ex, s);
+      return;
+    }
+  }
   for (int k = 0; k < w; ++k) {
     if (mixed) mixed_step(B, s, degree, p32, p64);
     else fp64_step(B, s, degree);

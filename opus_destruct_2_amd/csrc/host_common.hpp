@@ -122,6 +122,7 @@ struct HostLevel {
 struct svxg_ctx {
   int device = 0;
   int resident_rows = 0;  // graphs up to this many free rows run their steps in one workgroup (0: none)
+  int step_graphs = 1;  // tiled solve steps recorded per batch (hipGraph): 0 never, 1 large batches, 2 always
   hipStream_t stream = nullptr;
   std::unique_ptr<svxg::Arena> arena;
   std::mutex mu;  // batches and setup on the context stream, one at a time

@@ -65,6 +65,7 @@ def lib():
             "svxg_assemble_batch": ([vp, vp, i32, vp], ctypes.c_int),
             "svxg_ctx_set_profiling": ([vp, i32], ctypes.c_int),
             "svxg_ctx_set_resident_rows": ([vp, i32], ctypes.c_int),
+            "svxg_ctx_set_step_graphs": ([vp, i32], ctypes.c_int),
             "svxg_ctx_kernel_stats": ([vp, i32, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)],
                                       ctypes.c_int),
             "svxg_server_default_options": ([vp], None),
@@ -117,6 +118,11 @@ class Context:
     def set_resident_rows(self, rows):
         """Graphs up to `rows` free nodes step in one workgroup each (0: none); results are unchanged."""
         _check(lib().svxg_ctx_set_resident_rows(self.h, int(rows)), "set_resident_rows")
+
+    def set_step_graphs(self, mode):
+        """Tiled solve steps recorded as a HIP graph and replayed: 0 never, 1 large batches
+        (default), 2 always. Results are unchanged."""
+        _check(lib().svxg_ctx_set_step_graphs(self.h, int(mode)), "set_step_graphs")
 
     def set_profiling(self, on):
         _check(lib().svxg_ctx_set_profiling(self.h, int(bool(on))), "set_profiling")

@@ -97,3 +97,34 @@ def test_resident_steps_equal_tiled_steps_bitwise(prec, mats):
     for k in range(len(gs)):
         assert ra[k] == rb[k] and ra2[k] == rb2[k], (SIZES[k], ra[k], rb[k])
         assert np.array_equal(bits(a[k]), bits(b[k])) and np.array_equal(bits(a2[k]), bits(b2[k])), SIZES[k]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS, ids=PREC_IDS)
+def test_step_graphs_equal_kernel_launches_bitwise(prec, mats):
+    """Solve steps replayed from a recorded HIP graph give the bits of the same kernels launched one
+    by one: batched, and a single problem solved again and again (its batch, and so its graph,
+    kept between calls, with other device work in between)."""
+    gs = graphs()
+    out = []
+    for mode in (2, 0):  # (recorded from the first step, whatever the batch's size; never)
+        c = stress.Context(0)
+        c.set_step_graphs(mode)
+        opts = stress.default_options(precision=prec[0], amg_min_nodes=0, **prec[1])
+        probs = [stress.StressProblem(g.nodes, g.bonds, mats, c, options=opts) for g in gs]
+        us, rs = stress.solve_batch(probs, [g.f for g in gs], rtol=1e-9, maxit=5000)
+        p = probs[-1]
+        seq = []
+        for k in range(3):
+            f = gs[-1].f * (1.0 + 0.5 * k)
+            seq.append(p.solve(f, rtol=1e-9, maxit=5000))
+            p.apply_matrix(gen.uniform(k, 6 * p.info()["nfree"]))
+        out.append((us, rs, seq))
+        for q in probs:
+            q.close()
+        c.close()
+    (a, ra, sa), (b, rb, sb) = out
+    for k in range(len(gs)):
+        assert ra[k] == rb[k], (SIZES[k], ra[k], rb[k])
+        assert np.array_equal(bits(a[k]), bits(b[k])), SIZES[k]
+    for (ua, qa), (ub, qb) in zip(sa, sb):
+        assert qa == qb and np.array_equal(bits(ua), bits(ub))
