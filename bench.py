@@ -545,10 +545,17 @@ def main():
     if args.serve_lambdas:
         lams = [int(x) for x in args.serve_lambdas.split(",") if x]
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
-        line["serving"] = run_serving(mats, ctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
+        # (a context of its own, its arena empty, as a server's would be: run on the context the batch
+        # leg had filled and emptied, its ticks took 40 % longer, the new problems' arrays then coming
+        # from blocks scattered over the earlier problems' slabs)
+        sctx = stress.Context(0)
+        line["serving"] = run_serving(mats, sctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
                                       args.serve_pool, setup_workers=args.serve_setup_workers)
+        sctx.close()
     if args.city > 0:
-        line["city"] = run_city(mats, ctx, args.city, args.city_rounds, args.city_rate)
+        cctx = stress.Context(0)  # (a context of its own, freed after the leg)
+        line["city"] = run_city(mats, cctx, args.city, args.city_rounds, args.city_rate)
+        cctx.close()
     if not args.no_cpu:
         log("cpu baseline: the reference's solver on the same workload ...")
         cb = cpu_baseline(g, mats)
