@@ -545,9 +545,8 @@ def main():
     if args.serve_lambdas:
         lams = [int(x) for x in args.serve_lambdas.split(",") if x]
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
-        # (a context of its own, its arena empty, as a server's would be: run on the context the batch
-        # leg had filled and emptied, its ticks took 40 % longer, the new problems' arrays then coming
-        # from blocks scattered over the earlier problems' slabs)
+        # (a context of its own, as a server's would be: run on the context the batch leg had filled
+        # and emptied, the setup workers took 45 % longer, for a reason not yet found)
         sctx = stress.Context(0)
         line["serving"] = run_serving(mats, sctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
                                       args.serve_pool, setup_workers=args.serve_setup_workers)

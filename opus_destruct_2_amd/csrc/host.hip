@@ -39,8 +39,34 @@ static int size_class(size_t bytes) {
 
 Arena::~Arena() {
   (void)hipSetDevice(device_);
-  for (void* p : slabs_) (void)hipFree(p);
+  for (const Slab& sl : slabs_) (void)hipFree(sl.base);
   for (void* p : big_) (void)hipFree(p);
+}
+
+uint32_t Arena::slab_of(const void* p) const {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  auto it = std::upper_bound(by_addr_.begin(), by_addr_.end(), std::make_pair(a, UINT32_MAX));
+  return std::prev(it)->second;
+}
+
+// every block of slab s is free: its free-list entries go stale and the slab is carved afresh
+void Arena::reset_slab(uint32_t s) {
+  Slab& sl = slabs_[s];
+  stale_ += sl.freed;
+  sl.freed = 0;
+  ++sl.gen;
+  if (s == cur_)
+    bump_off_ = 0;
+  else
+    empty_.push_back(s);
+  if (stale_ > 65536) purge_stale();
+}
+
+void Arena::purge_stale() {
+  for (auto& fl : free_)
+    fl.erase(std::remove_if(fl.begin(), fl.end(), [&](const FreeBlock& f) { return f.gen != slabs_[f.slab].gen; }),
+             fl.end());
+  stale_ = 0;
 }
 
 int Arena::alloc(size_t bytes, void** out, size_t* held) {
@@ -64,37 +90,57 @@ int Arena::alloc(size_t bytes, void** out, size_t* held) {
     *held = sz;
     return SVXG_OK;
   }
-  int c = size_class(bytes);
+  const int c0 = size_class(bytes);
   // a free block of this class or of the next two (at most 56 % larger) before new memory
-  for (int k = 0; k < 3 && size_t(c + k) < free_.size(); ++k)
-    if (!free_[size_t(c + k)].empty()) {
-      c += k;
-      break;
+  int c = c0;
+  bool found = false;
+  for (int k = 0; k < 3 && size_t(c0 + k) < free_.size() && !found; ++k) {
+    auto& fl = free_[size_t(c0 + k)];
+    while (!fl.empty() && fl.back().gen != slabs_[fl.back().slab].gen) {
+      fl.pop_back();
+      --stale_;
     }
+    if (!fl.empty()) {
+      c = c0 + k;
+      found = true;
+    }
+  }
   const size_t sz = class_size(c);
   if (in_use_.load() + int64_t(sz) > budget_) {
     set_error("HBM budget exceeded");
     return SVXG_OOM_BUDGET;
   }
-  if (!free_[size_t(c)].empty()) {
-    *out = free_[size_t(c)].back();
+  uint32_t s;
+  if (found) {
+    const FreeBlock f = free_[size_t(c)].back();
     free_[size_t(c)].pop_back();
+    s = f.slab;
+    --slabs_[s].freed;
+    *out = f.p;
   } else {
-    if (bump_left_ < sz) {
-      void* p = nullptr;
-      hipError_t e = hipMalloc(&p, kSlab);
-      if (e != hipSuccess) {
-        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
-        return SVXG_DEVICE_ERROR;
+    if (cur_ == UINT32_MAX || kSlab - bump_off_ < sz) {
+      if (!empty_.empty()) {
+        cur_ = empty_.back();
+        empty_.pop_back();
+      } else {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, kSlab);
+        if (e != hipSuccess) {
+          set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+          return SVXG_DEVICE_ERROR;
+        }
+        cur_ = uint32_t(slabs_.size());
+        slabs_.push_back(Slab{static_cast<char*>(p), 0, 0, 0});
+        const auto key = std::make_pair(reinterpret_cast<uintptr_t>(p), cur_);
+        by_addr_.insert(std::upper_bound(by_addr_.begin(), by_addr_.end(), key), key);
       }
-      slabs_.push_back(p);
-      bump_ = static_cast<char*>(p);
-      bump_left_ = kSlab;
+      bump_off_ = 0;
     }
-    *out = bump_;
-    bump_ += sz;
-    bump_left_ -= sz;
+    s = cur_;
+    *out = slabs_[s].base + bump_off_;
+    bump_off_ += sz;
   }
+  ++slabs_[s].live;
   in_use_ += int64_t(sz);
   *held = sz;
   return SVXG_OK;
@@ -113,7 +159,14 @@ void Arena::release(void* p, size_t held) {
       }
     return;
   }
-  free_[size_t(size_class(held))].push_back(p);
+  const uint32_t s = slab_of(p);
+  Slab& sl = slabs_[s];
+  if (--sl.live == 0) {
+    reset_slab(s);
+    return;
+  }
+  free_[size_t(size_class(held))].push_back(FreeBlock{static_cast<char*>(p), s, sl.gen});
+  ++sl.freed;
 }
 
 // ---- tiles -----------------------------------------------------------------------------------------

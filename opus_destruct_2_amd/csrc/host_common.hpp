@@ -33,7 +33,10 @@ void set_error(const std::string& msg);
 
 // Device memory for one context: blocks of size classes (four a power of two, 256 B .. 64 MB)
 // carved from 512 MB slabs and recycled through per-class free lists; larger requests get their
-// own allocation. Every byte is admitted against the context's HBM budget.
+// own allocation. A slab whose blocks are all free again is reset whole (its free-list entries go
+// stale by generation) and carved afresh, so memory freed by a large batch returns as contiguous
+// slabs rather than as blocks scattered over all of them. Every byte is admitted against the
+// context's HBM budget.
 class Arena {
  public:
   Arena(int device, int64_t budget) : device_(device), budget_(budget) {}
@@ -46,14 +49,31 @@ class Arena {
  private:
   static constexpr size_t kSlab = size_t(512) << 20;
   static constexpr size_t kBig = size_t(64) << 20;
+  struct Slab {
+    char* base;
+    int64_t live;   // blocks handed out and not yet released
+    int64_t freed;  // valid entries of this slab in the free lists
+    uint32_t gen;   // bumped when the slab is reset; older free-list entries are stale
+  };
+  struct FreeBlock {
+    char* p;
+    uint32_t slab, gen;
+  };
+  uint32_t slab_of(const void* p) const;
+  void reset_slab(uint32_t s);
+  void purge_stale();
   int device_;
   int64_t budget_;
   std::atomic<int64_t> in_use_{0};
   std::mutex mu_;
-  std::vector<void*> slabs_, big_;
-  char* bump_ = nullptr;
-  size_t bump_left_ = 0;
-  std::array<std::vector<void*>, 80> free_;  // (class 72: 64 MB)
+  std::vector<Slab> slabs_;
+  std::vector<std::pair<uintptr_t, uint32_t>> by_addr_;  // slab bases, sorted
+  std::vector<uint32_t> empty_;                           // reset slabs not being carved
+  std::vector<void*> big_;
+  uint32_t cur_ = UINT32_MAX;  // the slab being carved
+  size_t bump_off_ = 0;
+  int64_t stale_ = 0;
+  std::array<std::vector<FreeBlock>, 80> free_;  // (class 72: 64 MB)
 };
 
 // a device allocation owned by its holder (returned to the arena on destruction)
