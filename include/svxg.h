@@ -265,6 +265,126 @@ int svxg_ctx_set_resident_rows(svxg_ctx* ctx, int32_t rows);
 int svxg_ctx_set_step_graphs(svxg_ctx* ctx, int32_t on);
 int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 
+/* ---- the judge and break rounds ------------------------------------------------------------
+   The World's judgement of a solved structure (World::Impl::judge, world.cpp:1162-1277) and of a
+   piece's check (world_pieces.cpp:647-671): every intact bond's loads from the solution, the DIF
+   lerp of a sudden change (lerp_load, world_internal.hpp:452-461), bond_utilization, and the
+   round's picks - the bonds over strength sorted by phi (descending, ties by bond index), those at
+   or above max(1, break_band x max phi) and at least the rule's quota, at most max_breaks - with
+   the plastic hinges among them (hinge_of, world.cpp:1286-1334). svxg_break_rounds_batch runs the
+   cascade: solve, judge, break the picks in K, detach what lost its supports, solve again, with K,
+   the multigrid and the solution resident on the device throughout. */
+enum {
+  SVXG_JUDGE_STRUCTURE = 0, /* quota max(1, (over + 3) / 4) (world.cpp:1245); broken bonds skipped */
+  SVXG_JUDGE_PIECE = 1      /* quota ceil(quota_fraction x over) (world_pieces.cpp:665); bonds that are
+                               broken or whose node a is retired skipped (world_pieces.cpp:653) */
+};
+
+/* WorldConfig's judge settings (world.hpp:114-183) and Material::ductile (material.hpp:66). */
+typedef struct svxg_judge_options {
+  double fragility;        /* 1.0: divides every bond strength */
+  double dif;              /* 1.5: a shock's loads are L0 + dif (L - L0), L0 the baseline */
+  double break_band;       /* 0.85 */
+  double quota_fraction;   /* 0.25 (impact_round_fraction; SVXG_JUDGE_PIECE), clamped to [0, 1] */
+  double hinge_shape;      /* 1.3: a hinge holds hinge_shape x the section's elastic moment */
+  uint64_t ductile[2];     /* bit k of word k / 64: material k is ductile (hinges form in it) */
+  int32_t rule;            /* SVXG_JUDGE_STRUCTURE or SVXG_JUDGE_PIECE */
+  int32_t max_breaks;      /* 256: picks of a round at most (1..4096) */
+  int32_t max_rounds;      /* 400: a structure past this many rounds is judged, nothing breaks */
+  int32_t plastic_hinges;  /* 1: hinge cuts of the picks */
+  int32_t restart_iters;   /* cascade: a solve on a stale multigrid that has not converged after
+                              this many iterations (120) gets it rebuilt and goes on (world.cpp:1068-1093) */
+  int32_t rebuild_iters;   /* cascade: one that converged after more (60) rebuilds it for the next round */
+} svxg_judge_options;      /* 80 bytes */
+void svxg_default_judge_options(svxg_judge_options* out);
+
+/* A structure's cascade state (Structure::shock, transient, rounds). */
+typedef struct svxg_judge_state {
+  int32_t shock;      /* the loads changed suddenly: bonds with a baseline are judged on the DIF lerp */
+  int32_t transient;  /* an impact load case: judged, the baseline is left as it was */
+  int32_t rounds;     /* break rounds of the running cascade so far */
+  int32_t pad_;
+} svxg_judge_state;
+
+/* HingeCut (world_impl.hpp:372-378): the pivot p, the axis the sides turn about, the bond's
+   normal n, the plastic moment mp (N m) and the force that tears it (N). */
+typedef struct svxg_hinge_cut {
+  int32_t a, b;
+  double p[3], axis[3], n[3];
+  double mp, pull;
+} svxg_hinge_cut;          /* 96 bytes */
+
+typedef struct svxg_break {
+  int32_t bond;
+  int32_t round;      /* the cascade round it broke in (svxg_judge_state.rounds then) */
+  double phi;
+  uint8_t mode;       /* FailMode: 1 Tension, 2 Crush, 3 Shear */
+  uint8_t hinge;      /* 1: cut is a plastic hinge */
+  uint8_t pad_[6];
+  svxg_hinge_cut cut;
+} svxg_break;              /* 120 bytes */
+
+typedef struct svxg_judge_result {
+  double max_phi;     /* over the judged bonds (WorldStats::max_utilization) */
+  int32_t over;       /* judged bonds with phi >= 1 */
+  int32_t picked;     /* bonds this round breaks (0: none over, or rounds >= max_rounds) */
+  int32_t hinges;     /* picks with a hinge cut */
+  int32_t pad_;
+} svxg_judge_result;       /* 24 bytes */
+
+/* The judged loads a shock is measured from (World::Impl::judged_, world.cpp:1176-1190), per bond
+   (svxg_bond_load), has[b] = 0 where a bond has none. loads == NULL clears them (none has one). */
+int svxg_problem_set_judge_baseline(svxg_problem* p, const svxg_bond_load* loads, const uint8_t* has);
+int svxg_problem_get_judge_baseline(svxg_problem* p, svxg_bond_load* loads, uint8_t* has);
+
+/* One judgement of `count` problems (World::Impl::judge without its side effects): the loads
+   from u[i] (host, 6 per node) or, where u or u[i] is null, from the problem's device-resident
+   solution; the baseline becomes these loads unless state[i].transient. picks[i] (room for
+   max_breaks) receives the round's picks in order, round = state[i].rounds. phi[i] (optional, one
+   per bond): each judged bond's utilisation, 0 for the others. Nothing breaks: the caller removes
+   the picks (svxg_problem_remove_bonds), as the World does. */
+int svxg_judge_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_t count, const double* const* u,
+                     const svxg_judge_options* opt, const svxg_judge_state* state, svxg_judge_result* res,
+                     svxg_break* const* picks, double* const* phi);
+
+enum {
+  SVXG_CASCADE_SETTLED = 0, /* a judgement found no bond over strength */
+  SVXG_CASCADE_ROUNDS = 1,  /* state.rounds reached max_rounds */
+  SVXG_CASCADE_UNSOLVED = 2,/* a solve did not converge (after a rebuild) or broke down */
+  SVXG_CASCADE_GONE = 3,    /* every node retired */
+  SVXG_CASCADE_RESEED = 4,  /* more than max(64, active / 2) nodes retired: the World extracts the
+                               structure again (world.cpp:1488-1493) */
+  SVXG_CASCADE_LIMIT = 5    /* the call's round limit */
+};
+
+typedef struct svxg_cascade_result {
+  svxg_pcg_result last;     /* the last solve */
+  double max_phi;           /* of the last judgement */
+  int32_t end;              /* SVXG_CASCADE_* */
+  int32_t rounds;           /* rounds that broke bonds in this call */
+  int32_t breaks;           /* bonds broken by the judge (records written: min(breaks, capacity)) */
+  int32_t retired;          /* nodes retired by detaching (ids written: min(retired, n)) */
+  int32_t solves, iters;    /* solves and PCG iterations */
+  int32_t rebuilds;         /* multigrid rebuilds (stale after breaks) */
+  int32_t pad_;
+} svxg_cascade_result;     /* 64 bytes */
+
+/* The break-round cascade of `count` structures (World::Impl::step_structures' solve, judge and
+   detach_unsupported, world.cpp:986-1129, 1162-1277, 1371-1444, one round after another). Each
+   round: solve K u = f warm from the last u (maxit iterations, rtol; a stale multigrid is rebuilt
+   as opt.restart_iters / rebuild_iters say), judge (as svxg_judge_batch, with state[i], which the
+   call advances: shock after a round that broke, rounds counted), break the picks in K, then
+   retire the nodes no longer connected to a support bond (graph_components,
+   world_internal.hpp:660-706; their loads and motion become 0). Ends per structure as
+   svxg_cascade_result.end says, or after max_call_rounds rounds (0: no limit). f[i], u[i]: host,
+   6 per node; u[i] is the warm start and receives the last solution. breaks[i] (capacity[i]
+   records) and retired[i] (room for every node; may be null) receive what broke and what left.
+   Every problem must be assembled with one precision; the judge runs in FP64. */
+int svxg_break_rounds_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_t count, const double* const* f,
+                            double* const* u, double rtol, int32_t maxit, const svxg_judge_options* opt,
+                            int32_t max_call_rounds, svxg_judge_state* state, svxg_cascade_result* res,
+                            svxg_break* const* breaks, const int32_t* capacity, int32_t* const* retired);
+
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------
    Replaces the reference's per-tick structure loop (step_structures, world.cpp:986-1129: the
    structures of a tick solved one after another within a block-operation budget) and its piece

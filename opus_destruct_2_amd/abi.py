@@ -109,6 +109,75 @@ class ServerStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+# ---- the judge and break rounds (World::Impl::judge, world.cpp:1162-1277) ------------------------
+SVXG_JUDGE_STRUCTURE, SVXG_JUDGE_PIECE = 0, 1
+(SVXG_CASCADE_SETTLED, SVXG_CASCADE_ROUNDS, SVXG_CASCADE_UNSOLVED, SVXG_CASCADE_GONE, SVXG_CASCADE_RESEED,
+ SVXG_CASCADE_LIMIT) = range(6)
+CASCADE_ENDS = ("settled", "rounds", "unsolved", "gone", "reseed", "limit")
+
+HINGE_CUT = np.dtype([
+    ("a", "<i4"), ("b", "<i4"), ("p", "<f8", (3,)), ("axis", "<f8", (3,)), ("n", "<f8", (3,)),
+    ("mp", "<f8"), ("pull", "<f8"),
+], align=True)
+BREAK = np.dtype([
+    ("bond", "<i4"), ("round", "<i4"), ("phi", "<f8"), ("mode", "u1"), ("hinge", "u1"), ("pad_", "u1", (6,)),
+    ("cut", HINGE_CUT),
+], align=True)
+assert HINGE_CUT.itemsize == 96 and BREAK.itemsize == 120
+
+
+class JudgeOptions(ctypes.Structure):
+    """WorldConfig's judge settings (core/include/svx/world/world.hpp:114-183)."""
+    _fields_ = [("fragility", ctypes.c_double), ("dif", ctypes.c_double), ("break_band", ctypes.c_double),
+                ("quota_fraction", ctypes.c_double), ("hinge_shape", ctypes.c_double),
+                ("ductile", ctypes.c_uint64 * 2), ("rule", ctypes.c_int32), ("max_breaks", ctypes.c_int32),
+                ("max_rounds", ctypes.c_int32), ("plastic_hinges", ctypes.c_int32),
+                ("restart_iters", ctypes.c_int32), ("rebuild_iters", ctypes.c_int32)]
+
+
+class JudgeState(ctypes.Structure):
+    """A structure's cascade state (Structure::shock, transient, rounds)."""
+    _fields_ = [("shock", ctypes.c_int32), ("transient", ctypes.c_int32), ("rounds", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"shock": bool(self.shock), "transient": bool(self.transient), "rounds": self.rounds}
+
+
+class JudgeResult(ctypes.Structure):
+    _fields_ = [("max_phi", ctypes.c_double), ("over", ctypes.c_int32), ("picked", ctypes.c_int32),
+                ("hinges", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"max_phi": self.max_phi, "over": self.over, "picked": self.picked, "hinges": self.hinges}
+
+
+class CascadeResult(ctypes.Structure):
+    _fields_ = [("last", PcgResult), ("max_phi", ctypes.c_double), ("end", ctypes.c_int32),
+                ("rounds", ctypes.c_int32), ("breaks", ctypes.c_int32), ("retired", ctypes.c_int32),
+                ("solves", ctypes.c_int32), ("iters", ctypes.c_int32), ("rebuilds", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"last": self.last.as_dict(), "max_phi": self.max_phi, "end": CASCADE_ENDS[self.end],
+                "rounds": self.rounds, "breaks": self.breaks, "retired": self.retired, "solves": self.solves,
+                "iters": self.iters, "rebuilds": self.rebuilds}
+
+
+assert ctypes.sizeof(JudgeOptions) == 80 and ctypes.sizeof(JudgeResult) == 24
+assert ctypes.sizeof(CascadeResult) == 64 and ctypes.sizeof(JudgeState) == 16
+
+
+def ductile_mask(ids):
+    """svxg_judge_options.ductile from a list of material ids (Material::ductile, material.hpp:66)."""
+    m = [0, 0]
+    for i in ids:
+        if not 0 <= int(i) < 128:
+            raise ValueError(f"material id {i} out of range")
+        m[int(i) >> 6] |= 1 << (int(i) & 63)
+    return m
+
+
 def ptr(a):
     """The data pointer of a numpy array (None for None)."""
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)

@@ -20,32 +20,13 @@
 //   coarsest        dense Cholesky with the reference's symmetrisation and pivot floor
 //                   (amg.cpp:535-578), then its explicit inverse for a parallel apply
 // Bond checks: bond_load + bond_utilization (stress.cpp:14-57, 477-502), same operation order.
-#include "svxg_kernels.hpp"
+#include "bond_math.hpp"
 
 namespace svxg {
 
 namespace {
 
 constexpr int kBondsPerWG = 7;  // 36 lanes a bond / a block
-
-__device__ __forceinline__ const svxg_material& mat_of(const svxg_material* m, int nm, int id) {
-  return (id >= 0 && id < nm) ? m[id] : m[0];
-}
-
-// element (r, c) of rigid_block(r): [[I, -[r]x], [0, I]] (amg.cpp:143-153)
-__device__ __forceinline__ double rigid_at(const double r[3], int R, int C) {
-  if (R == C) return 1.0;
-  const int e = R * 6 + C;
-  switch (e) {
-    case 4: return r[2];
-    case 5: return -r[1];
-    case 9: return -r[2];
-    case 11: return r[0];
-    case 15: return r[1];
-    case 16: return -r[0];
-    default: return 0.0;
-  }
-}
 
 // the bond's local stiffness in the global frame, element (R, C) (stress.cpp:59-79)
 __device__ double bond_D(const svxg_bond& b, const svxg_material& A, const svxg_material& B, int R, int C) {
@@ -87,17 +68,17 @@ __device__ __forceinline__ double atbd_elem(const double* L, const double* M, co
 // a bond's D, Ba, Bb into LDS (lane e of the bond's 36 computes element e of each)
 __device__ void bond_blocks_lds(const svxg_bond& b, const double* nodec, const svxg_material* mats, int nmats, int e,
                                 double* sD, double* sA, double* sB) {
-  const svxg_material& MA = mat_of(mats, nmats, b.ma);
-  const svxg_material& MB = mat_of(mats, nmats, b.mb);
+  const svxg_material& MA = bond_mat(mats, nmats, b.ma);
+  const svxg_material& MB = bond_mat(mats, nmats, b.mb);
   const int R = e / 6, C = e % 6;
   sD[e] = bond_D(b, MA, MB, R, C);
   const double* ca = nodec + 3 * size_t(b.a);
   const double ra[3] = {b.p[0] - ca[0], b.p[1] - ca[1], b.p[2] - ca[2]};
-  sA[e] = rigid_at(ra, R, C);
+  sA[e] = rigid_elem(ra, R, C);
   if (b.b >= 0) {
     const double* cb = nodec + 3 * size_t(b.b);
     const double rb[3] = {b.p[0] - cb[0], b.p[1] - cb[1], b.p[2] - cb[2]};
-    sB[e] = rigid_at(rb, R, C);
+    sB[e] = rigid_elem(rb, R, C);
   } else {
     sB[e] = 0.0;
   }
@@ -679,7 +660,7 @@ __global__ void k_p_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __
   const double* Di = c.Dinv + 36 * size_t(i);
   const double* offi = c.off + 3 * size_t(i);
   double acc[36];
-  for (int q = 0; q < 36; ++q) acc[q] = (Jc == c.agg[i]) ? rigid_at(offi, q / 6, q % 6) : 0.0;
+  for (int q = 0; q < 36; ++q) acc[q] = (Jc == c.agg[i]) ? rigid_elem(offi, q / 6, q % 6) : 0.0;
   if (c.omega != 0.0) {
     for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
       const int j = c.col[k];
@@ -690,7 +671,7 @@ __global__ void k_p_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __
         double T[6];
         for (int r = 0; r < 6; ++r) {
           double s = 0.0;
-          for (int p = 0; p < 6; ++p) s += A[r * 6 + p] * rigid_at(offj, p, col);
+          for (int p = 0; p < 6; ++p) s += A[r * 6 + p] * rigid_elem(offj, p, col);
           T[r] = s;
         }
         for (int r = 0; r < 6; ++r) {
@@ -720,7 +701,7 @@ __global__ void __launch_bounds__(kWG) k_p_fill_lanes(const CoarsenDev* __restri
   const int Jc = c.Pcol[e];
   const int r = q / 6, col = q % 6;
   const double* Di = c.Dinv + 36 * size_t(i);
-  double acc = (Jc == c.agg[i]) ? rigid_at(c.off + 3 * size_t(i), r, col) : 0.0;
+  double acc = (Jc == c.agg[i]) ? rigid_elem(c.off + 3 * size_t(i), r, col) : 0.0;
   if (c.omega != 0.0) {
     for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
       const int j = c.col[k];
@@ -730,7 +711,7 @@ __global__ void __launch_bounds__(kWG) k_p_fill_lanes(const CoarsenDev* __restri
       double T[6];
       for (int rr = 0; rr < 6; ++rr) {
         double s2 = 0.0;
-        for (int p = 0; p < 6; ++p) s2 += A[rr * 6 + p] * rigid_at(offj, p, col);
+        for (int p = 0; p < 6; ++p) s2 += A[rr * 6 + p] * rigid_elem(offj, p, col);
         T[rr] = s2;
       }
       double s2 = 0.0;
@@ -1534,15 +1515,6 @@ __global__ void __launch_bounds__(kWG) k_copy_many(const BulkSeg* __restrict__ s
 
 // ---- bond checks -------------------------------------------------------------------------------------
 
-__device__ __forceinline__ void mv6_rigid(const double r[3], const double* u, double* y) {
-  // y = rigid_block(r) u, each row summed left to right as blk6::mv6 does
-  for (int R = 0; R < 6; ++R) {
-    double s = rigid_at(r, R, 0) * u[0];
-    for (int c = 1; c < 6; ++c) s = s + rigid_at(r, R, c) * u[c];
-    y[R] = s;
-  }
-}
-
 __global__ void k_bond_checks(const CheckDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
@@ -1550,77 +1522,14 @@ __global__ void k_bond_checks(const CheckDev* __restrict__ J, const Tile* __rest
   if (int(threadIdx.x) >= tl.nrows) return;
   const int bi = tl.row0 + threadIdx.x;
   const svxg_bond& b = c.bonds[bi];
-  const svxg_material& MA = mat_of(c.mats, c.nmats, b.ma);
-  const svxg_material& MB = mat_of(c.mats, c.nmats, b.mb);
-  double xa[6] = {0, 0, 0, 0, 0, 0}, xb[6] = {0, 0, 0, 0, 0, 0};
-  const svxg_node& na = c.nodes[b.a];
-  if (!na.fixed) {
-    const double r[3] = {b.p[0] - na.c[0], b.p[1] - na.c[1], b.p[2] - na.c[2]};
-    mv6_rigid(r, c.u + 6 * size_t(b.a), xa);
-  }
-  if (b.b >= 0) {
-    const svxg_node& nb = c.nodes[b.b];
-    if (!nb.fixed) {
-      const double r[3] = {b.p[0] - nb.c[0], b.p[1] - nb.c[1], b.p[2] - nb.c[2]};
-      mv6_rigid(r, c.u + 6 * size_t(b.b), xb);
-    }
-  }
-  const double dt[3] = {xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]};
-  const double dr[3] = {xb[3] - xa[3], xb[4] - xa[4], xb[5] - xa[5]};
-  auto dot3 = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
-  const double la = fmax(b.la, 1e-4), lb = fmax(b.lb, 1e-4);
-  const double invE = la / MA.E + lb / MB.E;
-  const double invG = la / MA.G + lb / MB.G;
-  svxg_bond_load L;
-  L.N = b.area / invE * dot3(dt, b.n);
-  L.V1 = (5.0 / 6.0) * b.area / invG * dot3(dt, b.t1);
-  L.V2 = (5.0 / 6.0) * b.area / invG * dot3(dt, b.t2);
-  L.T = 0.8 * (b.s1 + b.s2) / invG * dot3(dr, b.n);
-  L.M1 = b.s2 / invE * dot3(dr, b.t1);
-  L.M2 = b.s1 / invE * dot3(dr, b.t2);
+  const svxg_material& MA = bond_mat(c.mats, c.nmats, b.ma);
+  const svxg_material& MB = bond_mat(c.mats, c.nmats, b.mb);
+  const svxg_bond_load L = bond_load_dev(b, c.nodes, MA, MB, c.u);
   if (c.loads) c.loads[bi] = L;
-  // strengths (stress.cpp:14-21)
-  const double sc = b.strength / fmax(1e-6, c.fragility);
-  double ft, fb, fc, coh, mu;
-  if (b.sectioned) {
-    ft = sc * double(b.ft);
-    fb = sc * double(b.fb);
-    fc = sc * double(b.fc);
-    coh = sc * double(b.coh);
-    mu = double(b.mu);
-  } else {
-    ft = sc * fmin(MA.ft, MB.ft);
-    fb = sc * fmin(MA.fb, MB.fb);
-    fc = sc * fmin(MA.fc, MB.fc);
-    coh = sc * fmin(MA.cohesion, MB.cohesion);
-    mu = fmin(MA.friction, MB.friction);
-  }
-  // utilisation (stress.cpp:23-57)
-  const double kWeak = 1e-9;
-  ft = fmax(ft, kWeak);
-  fb = fmax(fb, kWeak);
-  fc = fmax(fc, kWeak);
-  const double A = fmax(b.area, 1e-12);
-  const double sN = L.N / A;
-  const double sb = (b.s2 > 0 ? fabs(L.M1) * b.c2 / b.s2 : 0.0) + (b.s1 > 0 ? fabs(L.M2) * b.c1 / b.s1 : 0.0);
-  const double phi_t = sN >= 0.0 ? sN / ft + sb / fb : fmax(0.0, sb + sN) / fb;
-  const double phi_c = fmax(0.0, sb - sN) / fc;
-  const double Jt = fmax(b.s1 + b.s2, 1e-18);
-  const double tau = 1.5 * sqrt(L.V1 * L.V1 + L.V2 * L.V2) / A + fabs(L.T) * b.rmax / Jt;
-  const double cap = fmax(coh + mu * fmax(0.0, -sN), kWeak);
-  const double phi_s = tau / cap;
-  double phi = phi_t;
-  uint8_t md = 1;
-  if (phi_c > phi) {
-    phi = phi_c;
-    md = 2;
-  }
-  if (phi_s > phi) {
-    phi = phi_s;
-    md = 3;
-  }
-  c.phi[bi] = phi;
-  c.mode[bi] = phi > 0 ? md : 0;
+  uint8_t md;
+  const double phi = bond_phi_dev(b, bond_strength_dev(b, MA, MB, c.fragility), L, &md);
+  if (c.phi) c.phi[bi] = phi;
+  if (c.mode) c.mode[bi] = md;
 }
 
 __global__ void k_scatter_u(const double* __restrict__ x, const int32_t* __restrict__ dof, double* __restrict__ u,

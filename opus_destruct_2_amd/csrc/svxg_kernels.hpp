@@ -284,4 +284,57 @@ struct ScatterJob {
 };
 void launch_scatter_many(hipStream_t s, const ScatterJob* jobs, int n);
 
+// ---- the judge (world.cpp:1162-1277, world_pieces.cpp:647-671) ----------------------------------
+// a bond over strength: its phi's bits (phi >= 1, so the bits order as the values), index, mode
+struct OverEntry {
+  uint64_t key;
+  int32_t bond;
+  int32_t mode;
+};
+// a problem's judgement: filled by the judge pass (max_bits, over) and the selection (picked, hinges)
+struct JudgeOut {
+  unsigned long long max_bits;  // the largest judged phi's bits (phi > 0; 0 if none)
+  int32_t over;
+  int32_t picked;
+  int32_t hinges;
+  int32_t pad_;
+};
+constexpr int kMaxPicks = 4096;  // a round's picks at most (the selection sorts them in LDS)
+struct JudgeDev {
+  int32_t nbonds, nmats;
+  int32_t shock, transient;
+  int32_t rule;        // 0 structure quota, 1 piece quota
+  int32_t max_breaks;  // 1..kMaxPicks
+  int32_t round;       // written into the picks
+  int32_t skip;        // judged, nothing picked (rounds >= max_rounds)
+  int32_t hinges;      // plastic-hinge cuts of the picks
+  int32_t pad_;
+  double fragility, dif, band, quota_fraction, hinge_shape;
+  uint64_t ductile[2];
+  const svxg_bond* bonds;
+  const svxg_node* nodes;
+  const svxg_material* mats;
+  const double* u;          // 6 per node (all nodes)
+  const uint8_t* dead;      // per bond: 1 = not judged
+  svxg_bond_load* base;     // baseline loads (null: no bond has one)
+  uint8_t* has_base;
+  double* phi;              // optional: each judged bond's phi, 0 for the others
+  OverEntry* over;          // room for nbonds
+  JudgeOut* out;
+  svxg_break* picks;        // room for max_breaks
+};
+// phi of every judged bond, the baseline update and the list of bonds over strength
+void launch_judge(hipStream_t s, const JudgeDev* jobs, const Tile* tiles, int ntiles);
+// each problem's picks (one workgroup a problem): sorted, banded, capped, with hinge cuts
+void launch_judge_select(hipStream_t s, const JudgeDev* jobs, int count);
+// rows (6 values each) of x and b set to 0, a workgroup a job
+struct ZeroRowsJob {
+  double* x;
+  double* b;
+  const int32_t* rows;
+  int32_t n;
+  int32_t pad_;
+};
+void launch_zero_rows(hipStream_t s, const ZeroRowsJob* jobs, int count);
+
 }  // namespace svxg

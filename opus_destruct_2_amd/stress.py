@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 from .abi import (BOND, BOND_LOAD, NODE, SVXG_FP64, SVXG_MIXED, SVXG_OK, SVXG_SINGULAR, PcgResult, ProblemInfo,
-                  StressOptions, reference_options)
+                  StressOptions, reference_options,
+                  BREAK, CascadeResult, JudgeOptions, JudgeResult, JudgeState, ductile_mask)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsvxg.so")
@@ -77,6 +78,11 @@ def lib():
             "svxg_server_release": ([vp, vp], ctypes.c_int),
             "svxg_server_now_ms": ([vp], f64),
             "svxg_server_stats_get": ([vp, vp], ctypes.c_int),
+            "svxg_default_judge_options": ([vp], None),
+            "svxg_problem_set_judge_baseline": ([vp, vp, vp], ctypes.c_int),
+            "svxg_problem_get_judge_baseline": ([vp, vp, vp], ctypes.c_int),
+            "svxg_judge_batch": ([vp, vp, i32, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+            "svxg_break_rounds_batch": ([vp, vp, i32, vp, vp, f64, i32, vp, i32, vp, vp, vp, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -307,6 +313,23 @@ class StressProblem:
         _check(lib().svxg_problem_apply_preconditioner(self.h, r.ctypes.data, z.ctypes.data), "apply_preconditioner")
         return z
 
+    def set_judge_baseline(self, loads=None, has=None):
+        """The judged loads a shock is measured from (World::Impl::judged_, world.cpp:1176-1190):
+        loads (BOND_LOAD per bond) and has (per bond, 0: none); None clears them."""
+        if loads is None:
+            _check(lib().svxg_problem_set_judge_baseline(self.h, None, None), "set_judge_baseline")
+            return
+        loads = np.ascontiguousarray(loads, BOND_LOAD)
+        has = np.ascontiguousarray(has, np.uint8)
+        _check(lib().svxg_problem_set_judge_baseline(self.h, loads.ctypes.data, has.ctypes.data), "set_judge_baseline")
+
+    def judge_baseline(self):
+        m = len(self._bonds)
+        loads = np.zeros(m, BOND_LOAD)
+        has = np.zeros(m, np.uint8)
+        _check(lib().svxg_problem_get_judge_baseline(self.h, loads.ctypes.data, has.ctypes.data), "judge_baseline")
+        return loads, has
+
     def last_timing(self):
         s, a = ctypes.c_double(), ctypes.c_double()
         _check(lib().svxg_problem_last_timing(self.h, ctypes.byref(s), ctypes.byref(a)), "last_timing")
@@ -357,6 +380,92 @@ def solve_batch(problems, fs, us=None, rtol=1e-8, maxit=-1, warm=False, ctx=None
     return us, [r.as_dict() for r in res]
 
 
+def default_judge_options(**kw):
+    """svxg_judge_options as the library defaults them (WorldConfig, world.hpp:114-183); ductile
+    takes a list of material ids."""
+    o = JudgeOptions()
+    lib().svxg_default_judge_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "ductile":
+            o.ductile[0], o.ductile[1] = ductile_mask(v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def _states(states, n):
+    st = (JudgeState * n)()
+    for k in range(n):
+        d = (states[k] if states is not None else None) or {}
+        st[k].shock = int(bool(d.get("shock", False)))
+        st[k].transient = int(bool(d.get("transient", False)))
+        st[k].rounds = int(d.get("rounds", 0))
+    return st
+
+
+def judge_batch(problems, states=None, opts=None, us=None, want_phi=False, ctx=None):
+    """World::Impl::judge without side effects for many problems (svxg_judge_batch): per problem
+    a dict of max_phi, over, picked, hinges, picks (BREAK records in order) and phi (if asked)."""
+    ctx = ctx or problems[0].ctx
+    n = len(problems)
+    opts = opts or default_judge_options()
+    ph = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    ua = None
+    if us is not None:
+        us = [None if u is None else np.ascontiguousarray(u, np.float64) for u in us]
+        ua = (ctypes.c_void_p * n)(*[None if u is None else u.ctypes.data for u in us])
+    st = _states(states, n)
+    res = (JudgeResult * n)()
+    picks = [np.zeros(opts.max_breaks, BREAK) for _ in problems]
+    pa = (ctypes.c_void_p * n)(*[p.ctypes.data for p in picks])
+    phis = [np.zeros(len(p._bonds)) for p in problems] if want_phi else None
+    phia = (ctypes.c_void_p * n)(*[f.ctypes.data for f in phis]) if want_phi else None
+    _check(lib().svxg_judge_batch(ctx.h, ph, n, ua, ctypes.byref(opts), st, res, pa, phia), "svxg_judge_batch")
+    out = []
+    for k in range(n):
+        d = res[k].as_dict()
+        d["picks"] = picks[k][:d["picked"]].copy()
+        if want_phi:
+            d["phi"] = phis[k]
+        out.append(d)
+    return out
+
+
+def break_rounds_batch(problems, fs, us=None, opts=None, states=None, rtol=1e-8, maxit=5000, max_call_rounds=0,
+                       capacity=None, ctx=None):
+    """The break-round cascade of many structures (svxg_break_rounds_batch): returns (us, results),
+    each result a dict with the cascade's counters, 'breaks' (BREAK records), 'retired' (node ids)
+    and 'state' (the advanced JudgeState)."""
+    ctx = ctx or problems[0].ctx
+    n = len(problems)
+    opts = opts or default_judge_options()
+    fs = [np.ascontiguousarray(f, np.float64) for f in fs]
+    us = [np.zeros(6 * p.n) if us is None or us[k] is None else np.array(us[k], np.float64)
+          for k, p in enumerate(problems)]
+    ph = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    fp = (ctypes.c_void_p * n)(*[f.ctypes.data for f in fs])
+    up = (ctypes.c_void_p * n)(*[u.ctypes.data for u in us])
+    st = _states(states, n)
+    res = (CascadeResult * n)()
+    cap = [int(capacity) if capacity is not None else len(p._bonds) for p in problems]
+    brs = [np.zeros(max(1, c), BREAK) for c in cap]
+    ba = (ctypes.c_void_p * n)(*[b.ctypes.data for b in brs])
+    ca = (ctypes.c_int32 * n)(*cap)
+    ret = [np.zeros(max(1, p.n), np.int32) for p in problems]
+    ra = (ctypes.c_void_p * n)(*[r.ctypes.data for r in ret])
+    _check(lib().svxg_break_rounds_batch(ctx.h, ph, n, fp, up, rtol, int(maxit), ctypes.byref(opts),
+                                         int(max_call_rounds), st, res, ba, ca, ra), "svxg_break_rounds_batch")
+    out = []
+    for k in range(n):
+        d = res[k].as_dict()
+        d["breaks"] = brs[k][:min(d["breaks"], cap[k])].copy()
+        d["retired"] = ret[k][:min(d["retired"], problems[k].n)].copy()
+        d["state"] = st[k].as_dict()
+        out.append(d)
+    return us, out
+
+
 __all__ = ["StressProblem", "Context", "default_context", "default_options", "reference_options", "solve_batch",
            "assemble_batch",
-           "add_force", "lib", "SvxgError", "SVXG_FP64", "SVXG_MIXED"]
+           "add_force", "lib", "SvxgError", "SVXG_FP64", "SVXG_MIXED", "default_judge_options", "judge_batch",
+           "break_rounds_batch"]

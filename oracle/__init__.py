@@ -10,7 +10,8 @@ import subprocess
 
 import numpy as np
 
-from opus_destruct_2_amd.abi import BOND, BOND_LOAD, NODE, PcgResult, ProblemInfo, StressOptions, reference_options
+from opus_destruct_2_amd.abi import (BOND, BOND_LOAD, BREAK, CASCADE_ENDS, HINGE_CUT, NODE, SVXG_JUDGE_PIECE, JudgeResult,
+                                     PcgResult, ProblemInfo, StressOptions, reference_options)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
@@ -47,7 +48,11 @@ def _bind(path, pre):
     sig["extend_warm_start"] = ([vp, i32, vp, i32, vp, i32], None)
     if pre == "orc_":
         sig.update({"get_matrix": ([vp, vp, vp, vp], None), "apply_matrix": ([vp, vp, vp], None),
-                    "apply_preconditioner": ([vp, vp, vp], None), "get_nodes": ([vp, vp], None)})
+                    "apply_preconditioner": ([vp, vp, vp], None), "get_nodes": ([vp, vp], None),
+                    "judge": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+                    "hinge_of": ([vp, vp, vp, vp, vp, i32, vp, f64, f64, vp], i32),
+                    "lerp_load": ([vp, vp, f64, vp], None),
+                    "graph_components": ([i32, vp, i32, vp, vp], i32)})
     for name, (args, res) in sig.items():
         fn = getattr(L, pre + name)
         fn.argtypes = args
@@ -198,11 +203,159 @@ class CpuProblem:
         self._f("apply_matrix")(self.h, x.ctypes.data, y.ctypes.data)
         return y
 
+    def get_nodes(self):
+        out = self.nodes.copy()
+        self._f("get_nodes")(self.h, out.ctypes.data)
+        return out
+
     def apply_preconditioner(self, r):
         r = np.ascontiguousarray(r, np.float64)
         z = np.zeros_like(r)
         self._f("apply_preconditioner")(self.h, r.ctypes.data, z.ctypes.data)
         return z
+
+
+class _JudgeState(ctypes.Structure):
+    _fields_ = [("shock", ctypes.c_int32), ("transient", ctypes.c_int32), ("rounds", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+def _judge(problem, u, opts, state, base, has, want_phi):
+    """orc_judge: World::Impl::judge (world.cpp:1162-1277) without its side effects."""
+    if problem.kind != "oracle":
+        raise ValueError("the judge is restated in the oracle only")
+    u = np.ascontiguousarray(u, np.float64)
+    st = _JudgeState(int(bool(state.get("shock", False))), int(bool(state.get("transient", False))),
+                     int(state.get("rounds", 0)), 0)
+    res = JudgeResult()
+    picks = np.zeros(max(1, opts.max_breaks), BREAK)
+    phi = np.zeros(len(problem.bonds)) if want_phi else None
+    problem._f("judge")(problem.h, u.ctypes.data, ctypes.byref(opts), ctypes.byref(st), base.ctypes.data,
+                        has.ctypes.data, None if phi is None else phi.ctypes.data, ctypes.byref(res), picks.ctypes.data)
+    d = res.as_dict()
+    d["picks"] = picks[:d["picked"]].copy()
+    if want_phi:
+        d["phi"] = phi
+    return d
+
+
+class CpuJudge:
+    """The judge's state beside a CpuProblem: the baseline loads (World::Impl::judged_) of its bonds."""
+
+    def __init__(self, problem):
+        self.p = problem
+        self.base = np.zeros(len(problem.bonds), BOND_LOAD)
+        self.has = np.zeros(len(problem.bonds), np.uint8)
+
+    def judge(self, u, opts, state, want_phi=False):
+        return _judge(self.p, u, opts, state, self.base, self.has, want_phi)
+
+
+def hinge_of(bond, load, rot_a, rot_b, mats, ductile, fragility=1.0, hinge_shape=1.3):
+    """hinge_of (core/src/world/world.cpp:1286-1334): the HingeCut record, or None. ductile: the
+    two-word mask of svxg_judge_options."""
+    L, _ = library("oracle")
+    b = np.ascontiguousarray(np.atleast_1d(bond), BOND)
+    ld = np.zeros(1, BOND_LOAD)
+    for k in ("N", "V1", "V2", "T", "M1", "M2"):
+        ld[k] = load.get(k, 0.0) if isinstance(load, dict) else load[k]
+    ra = np.ascontiguousarray(rot_a, np.float64)
+    rb = np.ascontiguousarray(rot_b, np.float64)
+    mats = np.ascontiguousarray(mats)
+    dm = (ctypes.c_uint64 * 2)(*ductile)
+    out = np.zeros(1, HINGE_CUT)
+    ok = L.orc_hinge_of(b.ctypes.data, ld.ctypes.data, ra.ctypes.data, rb.ctypes.data, mats.ctypes.data, len(mats), dm,
+                        fragility, hinge_shape, out.ctypes.data)
+    return out[0] if ok else None
+
+
+def lerp_load(prev, now, k):
+    L, _ = library("oracle")
+    p = np.ascontiguousarray(np.atleast_1d(prev), BOND_LOAD)
+    q = np.ascontiguousarray(np.atleast_1d(now), BOND_LOAD)
+    out = np.zeros(1, BOND_LOAD)
+    L.orc_lerp_load(p.ctypes.data, q.ctypes.data, float(k), out.ctypes.data)
+    return out[0]
+
+
+def graph_components(n, bonds, seed):
+    """graph_components (core/src/world/world_internal.hpp:660-706): (count, comp)."""
+    L, _ = library("oracle")
+    bonds = np.ascontiguousarray(bonds, BOND)
+    seed = np.ascontiguousarray(seed, np.uint8)
+    comp = np.zeros(max(1, n), np.int32)
+    nc = L.orc_graph_components(int(n), bonds.ctypes.data, len(bonds), seed.ctypes.data, comp.ctypes.data)
+    return nc, comp[:n]
+
+
+def cascade(problem, judge, f, u, opts, state, rtol=1e-10, maxit=20000, max_call_rounds=0, assemble_opt=None):
+    """The break-round cascade of one structure on the CPU (World::Impl::step_structures' solve,
+    judge and detach_unsupported, world.cpp:986-1129, 1162-1277, 1371-1444, round after round),
+    the checker of svxg_break_rounds_batch. Each round solves on a freshly assembled K (a stale
+    preconditioner changes the iterations, not the answer), judges, breaks the picks, and retires
+    what lost its supports (their loads and motion 0). Returns (u, result dict)."""
+    f = np.array(f, np.float64)
+    u = np.array(u, np.float64)
+    state = dict(state)
+    out = {"end": None, "rounds": 0, "breaks": [], "retired": [], "solves": 0, "iters": 0, "max_phi": 0.0}
+    calls = 0
+    n = len(problem.nodes)
+    while True:
+        if max_call_rounds and calls >= max_call_rounds:
+            out["end"] = "limit"
+            break
+        calls += 1
+        if not problem.assemble(assemble_opt):
+            out["end"] = "unsolved"
+            break
+        u, r = problem.solve(f, u, rtol=rtol, maxit=maxit, warm=True)
+        out["solves"] += 1
+        out["iters"] += r["iters"]
+        out["last"] = r
+        if not r["converged"]:
+            out["end"] = "unsolved"
+            break
+        j = judge.judge(u, opts, state)
+        out["max_phi"] = j["max_phi"]
+        if j["over"] == 0 or state.get("rounds", 0) >= opts.max_rounds:
+            out["end"] = "settled" if j["over"] == 0 else "rounds"
+            if j["over"] == 0:
+                state["rounds"] = 0
+            state["shock"] = False
+            state["transient"] = False
+            break
+        for rec in j["picks"]:
+            problem.remove_bond(int(rec["bond"]))
+            out["breaks"].append(rec)
+        state["shock"] = True
+        state["transient"] = False
+        state["rounds"] = state.get("rounds", 0) + 1
+        out["rounds"] += 1
+        bonds = problem.get_bonds()
+        seed = np.zeros(n, np.uint8)
+        sup = (bonds["broken"] == 0) & (bonds["b"] < 0)
+        seed[bonds["a"][sup]] = 1
+        _, comp = graph_components(n, bonds, seed)
+        nodes = problem.get_nodes()
+        loose = np.flatnonzero((comp > 0) & (nodes["gone"] == 0)).astype(np.int32)
+        if len(loose):
+            problem.retire_nodes(loose)
+            out["retired"].extend(int(i) for i in loose)
+            for i in loose:
+                f[6 * i:6 * i + 6] = 0.0
+                u[6 * i:6 * i + 6] = 0.0
+            gone = int(problem.get_nodes()["gone"].astype(bool).sum())
+            act = n - gone
+            if act <= 0:
+                out["end"] = "gone"
+                break
+            if gone > max(64, act // 2):
+                out["end"] = "reseed"
+                break
+    out["breaks"] = np.array(out["breaks"], BREAK) if out["breaks"] else np.zeros(0, BREAK)
+    out["retired"] = np.array(out["retired"], np.int32)
+    out["state"] = state
+    return u, out
 
 
 def set_ref_threads(n):
@@ -233,5 +386,6 @@ def bond_strength(bond, fragility, mats, kind="oracle"):
     return out
 
 
-__all__ = ["CpuProblem", "build", "library", "ref_available", "bond_utilization", "bond_strength",
-           "reference_options", "StressOptions", "set_ref_threads"]
+__all__ = ["CpuProblem", "CpuJudge", "build", "library", "ref_available", "bond_utilization", "bond_strength",
+           "reference_options", "StressOptions", "set_ref_threads", "hinge_of", "lerp_load", "graph_components",
+           "cascade"]

@@ -1189,6 +1189,179 @@ double orc_bond_utilization(const svxg_bond* bp, const svxg_bond_load* Lp, doubl
   return phi;
 }
 
+void orc_lerp_load(const svxg_bond_load* prev, const svxg_bond_load* now, double k, svxg_bond_load* out) {
+  const f64* p = &prev->N;
+  const f64* q = &now->N;
+  f64* r = &out->N;
+  for (int c = 0; c < 6; ++c) r[c] = p[c] + k * (q[c] - p[c]);
+}
+
+int32_t orc_hinge_of(const svxg_bond* bp, const svxg_bond_load* Lp, const double rot_a[3], const double rot_b[3],
+                     const svxg_material* mats, int32_t nmats, const uint64_t ductile[2], double fragility,
+                     double hinge_shape, svxg_hinge_cut* out) {
+  const svxg_bond& b = *bp;
+  const svxg_bond_load& L = *Lp;
+  auto is_ductile = [&](int m) { return m >= 0 && m < 128 && ((ductile[m >> 6] >> (m & 63)) & 1ull); };
+  if (!is_ductile(b.ma) || !is_ductile(b.mb)) return 0;
+  f64 S[5];
+  strengths(b, fragility, mats, nmats, S);
+  const f64 area = std::max(b.area, 1e-12);
+  const f64 sN = L.N / area;
+  const f64 w1 = b.s2 > 0 ? b.c2 / b.s2 : 0.0;
+  const f64 w2 = b.s1 > 0 ? b.c1 / b.s1 : 0.0;
+  const f64 sb = std::abs(L.M1) * w1 + std::abs(L.M2) * w2;
+  const f64 fb = std::max(S[1], 1e-9), ft = std::max(S[0], 1e-9), fc = std::max(S[2], 1e-9);
+  const f64 pb = sb / fb;
+  const f64 pn = sN >= 0.0 ? sN / ft : -sN / fc;
+  const f64 J = std::max(b.s1 + b.s2, 1e-18);
+  const f64 tau = 1.5 * std::sqrt(L.V1 * L.V1 + L.V2 * L.V2) / area + std::abs(L.T) * b.rmax / J;
+  const f64 ps = tau / std::max(S[3] + S[4] * std::max(0.0, -sN), 1e-9);
+  // bending governs by half again over pull and shear, or it is no hinge
+  if (!(pb > 0.0) || pb < 1.5 * pn || pb < 1.5 * ps) return 0;
+  const V3 n = v3(b.n), t1 = v3(b.t1), t2 = v3(b.t2);
+  const V3 ra = v3(rot_a);
+  const V3 rb = b.b < 0 ? V3{0.0, 0.0, 0.0} : v3(rot_b);
+  V3 w = vsub(rb, ra);
+  const f64 wn = vdot(w, n);
+  w = vsub(w, V3{n.x * wn, n.y * wn, n.z * wn});
+  const V3 mt{t1.x * L.M1 + t2.x * L.M2, t1.y * L.M1 + t2.y * L.M2, t1.z * L.M1 + t2.z * L.M2};
+  svxg_hinge_cut hc{};
+  hc.a = b.a;
+  hc.b = b.b;
+  for (int c = 0; c < 3; ++c) {
+    hc.n[c] = b.n[c];
+    hc.p[c] = b.p[c];
+  }
+  V3 axis;
+  if (vdot(w, w) > 1e-30) {
+    const f64 inv = 1.0 / std::sqrt(vdot(w, w));
+    axis = V3{w.x * inv, w.y * inv, w.z * inv};
+    V3 y = vcross(axis, n);
+    const f64 yl = std::sqrt(vdot(y, y));
+    if (yl > 1e-9) {
+      const f64 iy = 1.0 / yl;
+      y = V3{y.x * iy, y.y * iy, y.z * iy};
+      const f64 edge = std::abs(vdot(y, t1)) * b.c1 + std::abs(vdot(y, t2)) * b.c2;
+      for (int c = 0; c < 3; ++c) hc.p[c] = b.p[c] - vget(y, c) * (0.9 * edge);
+    }
+  } else if (vdot(mt, mt) > 1e-30) {
+    const f64 inv = 1.0 / std::sqrt(vdot(mt, mt));
+    axis = V3{mt.x * inv, mt.y * inv, mt.z * inv};
+  } else {
+    return 0;
+  }
+  hc.axis[0] = axis.x;
+  hc.axis[1] = axis.y;
+  hc.axis[2] = axis.z;
+  const f64 a1 = std::abs(vdot(axis, t1)), a2 = std::abs(vdot(axis, t2));
+  hc.mp = std::max(0.0, hinge_shape) * fb / std::max(a1 * w1 + a2 * w2, 1e-12);
+  hc.pull = ft * area;
+  *out = hc;
+  return 1;
+}
+
+int32_t orc_judge(orc_problem* h, const double* u, const svxg_judge_options* opt, const svxg_judge_state* st,
+                  svxg_bond_load* base, uint8_t* has, double* phi_out, svxg_judge_result* res, svxg_break* picks) {
+  const i32 m = static_cast<i32>(h->bonds.size());
+  const i32 nm = static_cast<i32>(h->mats.size());
+  const bool piece = opt->rule == SVXG_JUDGE_PIECE;
+  std::vector<std::pair<f64, i32>> over;
+  std::vector<uint8_t> modes(size_t(m), 0);
+  f64 maxphi = 0.0;
+  for (i32 k = 0; k < m; ++k) {
+    const svxg_bond& b = h->bonds[size_t(k)];
+    if (b.broken || (piece && h->nodes[size_t(b.a)].gone)) {
+      if (phi_out) phi_out[k] = 0.0;
+      continue;
+    }
+    const svxg_bond_load L = h->bond_load(k, u);
+    svxg_bond_load Le = L;
+    if (st->shock && opt->dif != 1.0 && base && has[k]) orc_lerp_load(&base[k], &L, opt->dif, &Le);
+    const f64 phi = orc_bond_utilization(&b, &Le, opt->fragility, h->mats.data(), nm, &modes[size_t(k)]);
+    if (!st->transient && base) {
+      base[k] = L;
+      has[k] = 1;
+    }
+    if (phi_out) phi_out[k] = phi;
+    maxphi = std::max(maxphi, phi);
+    if (phi >= 1.0) over.push_back({phi, k});
+  }
+  *res = svxg_judge_result{};
+  res->max_phi = maxphi;
+  res->over = static_cast<i32>(over.size());
+  if (over.empty() || st->rounds >= opt->max_rounds) return 0;
+  std::sort(over.begin(), over.end(), [](const std::pair<f64, i32>& x, const std::pair<f64, i32>& y) {
+    return x.first != y.first ? x.first > y.first : x.second < y.second;
+  });
+  const f64 thr = std::max(1.0, opt->break_band * over.front().first);
+  const size_t n_over = over.size();
+  const size_t quota = piece ? static_cast<size_t>(std::ceil(std::clamp(opt->quota_fraction, 0.0, 1.0) * f64(n_over)))
+                             : std::max<size_t>(1, (n_over + 3) / 4);
+  i32 count = 0;
+  for (size_t q = 0; q < n_over; ++q) {
+    if ((over[q].first < thr && q >= quota) || count >= opt->max_breaks) break;
+    const i32 k = over[q].second;
+    const svxg_bond& b = h->bonds[size_t(k)];
+    svxg_break r{};
+    r.bond = k;
+    r.round = st->rounds;
+    r.phi = over[q].first;
+    r.mode = modes[size_t(k)];
+    if (opt->plastic_hinges && (!piece || b.b >= 0)) {
+      const double zero[3] = {0.0, 0.0, 0.0};
+      const double* ra = u + 6 * size_t(b.a) + 3;
+      const double* rb = b.b >= 0 ? u + 6 * size_t(b.b) + 3 : zero;
+      const svxg_bond_load L = h->bond_load(k, u);
+      if (orc_hinge_of(&b, &L, ra, rb, h->mats.data(), nm, opt->ductile, opt->fragility, opt->hinge_shape, &r.cut)) {
+        r.hinge = 1;
+        ++res->hinges;
+      }
+    }
+    picks[count++] = r;
+  }
+  res->picked = count;
+  return count;
+}
+
+int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp) {
+  // adjacency of the intact bonds between nodes, then floods: the seeds first (component 0), then
+  // every node not reached yet, in index order
+  std::vector<std::vector<i32>> adj(static_cast<size_t>(n));
+  for (i32 k = 0; k < m; ++k) {
+    const svxg_bond& b = bonds[k];
+    if (b.broken || b.b < 0) continue;
+    adj[size_t(b.a)].push_back(b.b);
+    adj[size_t(b.b)].push_back(b.a);
+  }
+  std::fill(comp, comp + n, -1);
+  std::vector<i32> todo;
+  auto flood = [&](i32 c) {
+    while (!todo.empty()) {
+      const i32 k = todo.back();
+      todo.pop_back();
+      for (i32 j : adj[size_t(k)])
+        if (comp[j] < 0) {
+          comp[j] = c;
+          todo.push_back(j);
+        }
+    }
+  };
+  for (i32 i = 0; i < n; ++i)
+    if (seed[i] && comp[i] < 0) {
+      comp[i] = 0;
+      todo.push_back(i);
+    }
+  flood(0);
+  i32 nc = 1;
+  for (i32 i = 0; i < n; ++i) {
+    if (comp[i] >= 0) continue;
+    comp[i] = nc;
+    todo.push_back(i);
+    flood(nc++);
+  }
+  return nc;
+}
+
 void orc_info(orc_problem* h, svxg_problem_info* out) {
   *out = svxg_problem_info{};
   out->nodes = static_cast<int32_t>(h->nodes.size());

@@ -45,6 +45,21 @@ void orc_apply_preconditioner(orc_problem* h, const double* r, double* z);
 void orc_get_bonds(orc_problem* h, svxg_bond* out);
 void orc_get_nodes(orc_problem* h, svxg_node* out);
 void orc_set_threads(int32_t n);
+/* World::Impl::judge without its side effects (world.cpp:1162-1277; the piece rule of
+   world_pieces.cpp:647-671): the judged bonds' loads from u (6 per node), the DIF lerp against
+   base/has (updated unless st->transient), phi (phi_out: per bond, 0 where not judged), the round's
+   picks (room for opt->max_breaks) with their hinge cuts. Returns the number of picks. */
+int32_t orc_judge(orc_problem* h, const double* u, const svxg_judge_options* opt, const svxg_judge_state* st,
+                  svxg_bond_load* base, uint8_t* has, double* phi_out, svxg_judge_result* res, svxg_break* picks);
+/* hinge_of (world.cpp:1286-1334); ductile as svxg_judge_options.ductile. Returns 1 with *out set. */
+int32_t orc_hinge_of(const svxg_bond* b, const svxg_bond_load* L, const double rot_a[3], const double rot_b[3],
+                     const svxg_material* mats, int32_t nmats, const uint64_t ductile[2], double fragility,
+                     double hinge_shape, svxg_hinge_cut* out);
+/* lerp_load (world_internal.hpp:452-461) */
+void orc_lerp_load(const svxg_bond_load* prev, const svxg_bond_load* now, double k, svxg_bond_load* out);
+/* graph_components (world_internal.hpp:660-706): comp[n]; returns the number of components */
+int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp);
+
 void orc_default_options(svxg_stress_options* out);
 
 #ifdef __cplusplus
