@@ -6,8 +6,9 @@ of 1e-8 and bond-checked. A step is one solve from a cold start plus the bond ch
 device-resident solution. The headline runs the mixed path (FP64 PCG and K products around the FP32
 V-cycle); the plain FP64 path (FP64 V-cycle) and the mixed split with FP32 K products inside FP64
 iterative refinement are measured beside it. Further legs, each reported in the same line: the
-batched uneven graphs (4096 pieces), the serving stream (60 Hz ticks) and the city chunk (400
-resident buildings with ground-storey collapses).
+batched uneven graphs (4096 pieces), the serving stream (60 Hz ticks), the city chunk (400
+resident buildings with ground-storey collapses) and the block's break-round cascade (strengths
+scaled so that its largest utilisation is 3, solved, judged and broken until it settles).
 
 value: solved DOFs / (device time of the solve + wall time of the checks), inputs resident.
 roofline: the dominant kernel's algorithmic bytes per launch / its average launch time, measured
@@ -16,7 +17,7 @@ cpu_baseline: the reference's own solver (compiled from its sources into oracle/
 workload, one assemble + solve, all host threads (up to 16).
 
   python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu] [--gpus 1]
-  python bench.py --only batch|serve|city ...   (one leg alone, for profiling)
+  python bench.py --only batch|serve|city|cascade ...   (one leg alone, for profiling)
 """
 import argparse
 import json
@@ -186,6 +187,56 @@ def run_batch(mats, ctx, count, steps, sample_plain):
             "plain_fp64_single_graph": {"value": round(plain, 1), "sample_graphs": len(pick), "sample_dofs": pdofs,
                                         "solve_ms_sum": round(pms, 3)},
             "batched_vs_plain": round(batched / plain, 2)}
+
+
+def run_cascade(mats, ctx, g, target, cpu=None):
+    """The break-round cascade of the single large structure (the block of the headline): its
+    strengths scaled so that the largest phi under gravity is `target`, then solve, judge, break and
+    detach round after round until nothing is over strength (svxg_break_rounds_batch, mixed
+    precision to 1e-8), K, the multigrid and the solution staying on the device. The judge alone is
+    timed on the resident solution; cpu: the reference's solve of the same block (setup and solve
+    seconds) for a per-round CPU figure (its setup, as the device's rebuilds, is counted once a round)."""
+    import oracle
+    from opus_destruct_2_amd import stress
+    from opus_destruct_2_amd.abi import SVXG_MIXED
+    opts = stress.default_options(precision=SVXG_MIXED, **AMG_KW)
+    p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
+    assert p.assemble()
+    u, r = p.solve(g.f, rtol=1e-8)
+    phi, _ = p.bond_checks()
+    jo = stress.default_judge_options(fragility=target / float(phi.max()))
+    judge_ms = []
+    for _ in range(5):
+        t = time.perf_counter()
+        j = stress.judge_batch([p], [{}], jo)[0]
+        judge_ms.append((time.perf_counter() - t) * 1e3)
+    p.close()
+    p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
+    assert p.assemble()
+    t = time.perf_counter()
+    us, res = stress.break_rounds_batch([p], [g.f], None, jo, rtol=1e-8)
+    wall = (time.perf_counter() - t) * 1e3
+    c = res[0]
+    p.close()
+    rec = {"target_phi": target, "end": c["end"], "rounds": c["rounds"], "breaks": int(len(c["breaks"])),
+           "retired": int(len(c["retired"])), "solves": c["solves"], "iters": c["iters"], "rebuilds": c["rebuilds"],
+           "wall_ms": round(wall, 1), "ms_per_round": round(wall / max(1, c["rounds"] + 1), 2),
+           "first_judgement": {"over": j["over"], "picked": j["picked"],
+                               "wall_ms_median": round(float(np.median(judge_ms)), 3)}}
+    if cpu is not None:
+        # the oracle's judge of the reference's solution, timed: with the reference's setup and
+        # solve it is what one round costs on the host cores
+        q = oracle.CpuProblem(g.nodes, g.bonds, mats)
+        t = time.perf_counter()
+        oracle.CpuJudge(q).judge(cpu["u"], jo, {})
+        js = time.perf_counter() - t
+        per = cpu["setup_s"] + cpu["solve_s"] + js
+        rec["cpu_round_s"] = {"setup": round(cpu["setup_s"], 3), "solve": round(cpu["solve_s"], 3),
+                              "judge": round(js, 3), "total": round(per, 3), "cores": cpu["cores"],
+                              "kind": cpu["kind"]}
+        rec["vs_cpu_per_round"] = round(per * 1e3 / rec["ms_per_round"], 1)
+    log(f"  cascade: {rec}")
+    return rec
 
 
 class _Piece:
@@ -445,6 +496,8 @@ def main():
     ap.add_argument("--city", type=int, default=400, help="buildings in the city-chunk leg (0: skip)")
     ap.add_argument("--city-rounds", type=int, default=3)
     ap.add_argument("--city-rate", type=float, default=20.0, help="collapse events per second")
+    ap.add_argument("--cascade", type=float, default=3.0,
+                    help="largest phi of the cascade leg's block under gravity (0: skip the leg)")
     ap.add_argument("--smooth-levels", type=int, default=None,
                     help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
     ap.add_argument("--cheb-degree", type=int, default=None)
@@ -452,7 +505,7 @@ def main():
                     help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
-    ap.add_argument("--only", choices=["batch", "serve", "city"], default=None,
+    ap.add_argument("--only", choices=["batch", "serve", "city", "cascade"], default=None,
                     help="profiling: run only this leg and print its record (not the benchmark line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
@@ -501,6 +554,8 @@ def main():
             rec = run_serving(mats, ctx, [int(x) for x in args.serve_lambdas.split(",") if x], args.serve_ticks,
                               args.serve_warmup, args.serve_life, args.serve_pool,
                               setup_workers=args.serve_setup_workers)
+        elif args.only == "cascade":
+            rec = run_cascade(mats, ctx, g, args.cascade)
         else:
             rec = run_city(mats, ctx, args.city, args.city_rounds, args.city_rate)
         print(json.dumps({"only": args.only, "record": rec}), flush=True)
@@ -555,6 +610,7 @@ def main():
         cctx = stress.Context(0)  # (a context of its own, freed after the leg)
         line["city"] = run_city(mats, cctx, args.city, args.city_rounds, args.city_rate)
         cctx.close()
+    cb = None
     if not args.no_cpu:
         log("cpu baseline: the reference's solver on the same workload ...")
         cb = cpu_baseline(g, mats)
@@ -565,6 +621,12 @@ def main():
             "fp64_phi_abs": float(np.abs(plain["phi"] - cb["phi"]).max()),
             "failed_bonds_gpu": int((head["phi"] >= 1).sum()), "failed_bonds_cpu": int((cb["phi"] >= 1).sum()),
         }
+    if args.cascade > 0:
+        log(f"cascade: the block's break rounds, largest phi {args.cascade} under gravity")
+        kctx = stress.Context(0)
+        line["cascade"] = run_cascade(mats, kctx, g, args.cascade, cb)
+        kctx.close()
+    if cb is not None:
         for k in ("u", "phi", "setup_s", "solve_s", "iters"):
             cb.pop(k)
         line["cpu_baseline"] = cb

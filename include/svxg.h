@@ -375,7 +375,8 @@ typedef struct svxg_cascade_result {
    as opt.restart_iters / rebuild_iters say), judge (as svxg_judge_batch, with state[i], which the
    call advances: shock after a round that broke, rounds counted), break the picks in K, then
    retire the nodes no longer connected to a support bond (graph_components,
-   world_internal.hpp:660-706; their loads and motion become 0). Ends per structure as
+   world_internal.hpp:660-706; their loads and motion become 0, as those of nodes retired before
+   the call, world.cpp:1472-1480). Ends per structure as
    svxg_cascade_result.end says, or after max_call_rounds rounds (0: no limit). f[i], u[i]: host,
    6 per node; u[i] is the warm start and receives the last solution. breaks[i] (capacity[i]
    records) and retired[i] (room for every node; may be null) receive what broke and what left.
