@@ -33,7 +33,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one live launch of a kernel on a workload, from two rocprofv3 counter passes of this
 # script (FETCH_SIZE scaled by 2.0, the factor a plain 1 GiB copy reads it with; WRITE_SIZE as read)
-MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 202.7e6}
+MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0, FP16 block copies)", 64, 64, 48): 129.5e6,
+                    ("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 202.7e6}
 METRIC = "PCG-solved DOFs/s at 1e-8 residual, batched vs plain; p50/p99 tick latency"
 
 
@@ -85,7 +86,7 @@ def run_path(g, mats, precision, steps, warmup, ctx, **kw):
 
 def roofline(res, precision, shape):
     names = {0: "k_pcg_spmv<double>", 1: "k_pcg_spmv<float>", 2: "k_cheb_step<double, 1, false> (level 0)",
-             3: "k_cheb_step<float, 1, false> (level 0)"}
+             3: "k_cheb_step<float, 1, false> (level 0" + (", FP16 block copies)" if res["info"]["vcycle_fp16"] else ")")}
     best = max(range(4), key=lambda k: res["kstats"][k][1])  # (the kind with the most device time)
     n, ms, byts = res["kstats"][best]
     if n == 0 or ms <= 0:

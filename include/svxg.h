@@ -107,7 +107,11 @@ typedef struct svxg_stress_options {
                                 run one PCG with FP64 K products and the FP32 V-cycle (the
                                 reference's own precision split, no refinement restarts); larger
                                 ones take FP32 K products inside FP64 iterative refinement (0: all) */
-  int32_t pad_;
+  int32_t vcycle_fp16_rows;  /* SVXG_MIXED, device: graphs with at least this many free nodes
+                                (default 16384; 0: all, 2^30: none) read their V-cycle's matrices
+                                (smoothing and residual passes) as FP16 blocks, each 6x6 block over
+                                its largest magnitude (a symmetric matrix stays symmetric),
+                                computing in FP32; smaller graphs, latency-bound, read FP32 copies */
 } svxg_stress_options;
 
 /* PcgResult (amg.hpp:120-125). */
@@ -203,7 +207,8 @@ void svxg_extend_warm_start(const svxg_node* nodes, int32_t n, const svxg_bond* 
    StressProblem::work_per_iteration / assembly_work (stress.hpp:95,119), device bytes. */
 typedef struct svxg_problem_info {
   int32_t nodes, bonds, nfree, levels;
-  int32_t block_jacobi, assembled, preconditioner_current, pad_;
+  int32_t block_jacobi, assembled, preconditioner_current;
+  int32_t vcycle_fp16;       /* the V-cycle reads FP16 block copies (svxg_stress_options.vcycle_fp16_rows) */
   int64_t blocks;
   int64_t work_per_iteration, assembly_work;
   int64_t device_bytes;

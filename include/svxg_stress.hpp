@@ -64,6 +64,7 @@ struct StressOptions {
   i32 amg_min_nodes = 160;
   i32 precision = SVXG_FP64;
   i32 mixed_fp64_rows = 1 << 30;  // (device: see svxg_stress_options)
+  i32 vcycle_fp16_rows = 16384;   // (device: see svxg_stress_options)
 };
 
 using PcgResult = svxg_pcg_result;
@@ -148,7 +149,7 @@ class StressProblem {
     if (svxg_problem_create(default_context(), n.data(), i32(n.size()), b.data(), i32(b.size()), m.data(),
                             i32(m.size()), &p_) != SVXG_OK)
       return false;
-    svxg_stress_options o{opt.rtol, opt.amg_min_nodes, opt.precision, opt.amg, opt.mixed_fp64_rows, 0};
+    svxg_stress_options o{opt.rtol, opt.amg_min_nodes, opt.precision, opt.amg, opt.mixed_fp64_rows, opt.vcycle_fp16_rows};
     const int st = svxg_problem_assemble(p_, &o);
     pull_flags();
     return st == SVXG_OK;

@@ -50,7 +50,7 @@ class AmgOptions(ctypes.Structure):
 class StressOptions(ctypes.Structure):
     """StressOptions (core/include/svx/stress/stress.hpp:68-72) plus the solve's precision."""
     _fields_ = [("rtol", ctypes.c_double), ("amg_min_nodes", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("amg", AmgOptions), ("mixed_fp64_rows", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("amg", AmgOptions), ("mixed_fp64_rows", ctypes.c_int32), ("vcycle_fp16_rows", ctypes.c_int32)]
 
 
 class PcgResult(ctypes.Structure):
@@ -67,7 +67,7 @@ class ProblemInfo(ctypes.Structure):
     _fields_ = [
         ("nodes", ctypes.c_int32), ("bonds", ctypes.c_int32), ("nfree", ctypes.c_int32), ("levels", ctypes.c_int32),
         ("block_jacobi", ctypes.c_int32), ("assembled", ctypes.c_int32), ("preconditioner_current", ctypes.c_int32),
-        ("pad_", ctypes.c_int32),
+        ("vcycle_fp16", ctypes.c_int32),
         ("blocks", ctypes.c_int64), ("work_per_iteration", ctypes.c_int64), ("assembly_work", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
         ("level_rows", ctypes.c_int32 * 16), ("level_blocks", ctypes.c_int64 * 16), ("level_pblocks", ctypes.c_int64 * 16),
@@ -78,7 +78,8 @@ class ProblemInfo(ctypes.Structure):
         return {
             "nodes": self.nodes, "bonds": self.bonds, "nfree": self.nfree, "levels": L,
             "block_jacobi": bool(self.block_jacobi), "assembled": bool(self.assembled),
-            "preconditioner_current": bool(self.preconditioner_current), "blocks": self.blocks,
+            "preconditioner_current": bool(self.preconditioner_current), "vcycle_fp16": bool(self.vcycle_fp16),
+            "blocks": self.blocks,
             "work_per_iteration": self.work_per_iteration, "assembly_work": self.assembly_work,
             "device_bytes": self.device_bytes,
             "level_rows": list(self.level_rows[:L]), "level_blocks": list(self.level_blocks[:L]),

@@ -449,7 +449,7 @@ int64_t svxg_problem::device_bytes() const {
                           &d_id_scale, &x, &r, &z, &p, &q, &b, &ix, &ir, &iz, &ip, &iq, &u_all})
     bytes += int64_t(d->held());
   for (const HostLevel& L : lv)
-    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32,
+    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.A16, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32,
                             &L.Rrow, &L.Rfine, &L.Rent, &L.Minv, &L.x64[0], &L.x64[1], &L.b64,
                             &L.r64, &L.d64, &L.x32[0], &L.x32[1], &L.b32, &L.r32, &L.d32, &L.pos, &L.lam})
       bytes += int64_t(d->held());
@@ -553,6 +553,7 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.f64.r = H.r64.as<double>();
     L.f64.d = H.d64.as<double>();
     L.f32.A = H.A32.as<float>();
+    L.A16 = H.A16.as<uint8_t>();
     L.f32.Dinv = H.D32.as<float>();
     L.f32.P = H.P32.as<float>();
     L.f32.x[0] = H.x32[0].as<float>();
@@ -676,7 +677,9 @@ static double prof_bytes(const svxg_problem* P, int kind) {
   if (P->lv.empty()) return 0.0;
   const double w = (kind & 1) ? 4.0 : 8.0;
   const double nb = double(P->lv[0].nnzb), n = double(P->lv[0].n);
-  const double mat = nb * (36.0 * w + 4.0) + 4.0 * (n + 1.0);
+  // (the smoothing step of a level with FP16 copies reads kHalfBlock bytes a block)
+  const double blk = (kind == 3 && P->lv[0].A16) ? double(kHalfBlock) : 36.0 * w;
+  const double mat = nb * (blk + 4.0) + 4.0 * (n + 1.0);
   return kind < 2 ? mat + 2.0 * 6.0 * n * w : mat + 36.0 * n * w + 5.0 * 6.0 * n * w;
 }
 

@@ -129,7 +129,7 @@ struct HostLevel {
   bool terminal = false;
   int32_t dense_m = 0;
   double lmax = 1.0;
-  DevBuf rowptr, col, A64, A32, D64, D32;
+  DevBuf rowptr, col, A64, A32, A16, D64, D32;
   DevBuf Prow, Pcol, P64, P32, Rrow, Rfine, Rent, Minv;
   DevBuf x64[2], b64, r64, d64, x32[2], b32, r32, d32;
   DevBuf pos, lam;

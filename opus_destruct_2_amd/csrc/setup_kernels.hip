@@ -20,6 +20,8 @@
 //   coarsest        dense Cholesky with the reference's symmetrisation and pivot floor
 //                   (amg.cpp:535-578), then its explicit inverse for a parallel apply
 // Bond checks: bond_load + bond_utilization (stress.cpp:14-57, 477-502), same operation order.
+#include <hip/hip_fp16.h>
+
 #include "bond_math.hpp"
 
 namespace svxg {
@@ -1491,6 +1493,24 @@ __global__ void __launch_bounds__(kWG) k_zero_many(const BulkSeg* __restrict__ s
   }
 }
 
+// a block's entries over its largest magnitude, rounded to FP16 (A_ij and A_ji = A_ij^T hold the
+// same values, so a symmetric matrix's copy stays symmetric)
+__global__ void __launch_bounds__(kWG) k_half_blocks(const float* __restrict__ A, uint8_t* __restrict__ H, int64_t nnzb) {
+  const int64_t k = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (k >= nnzb) return;
+  const float* a = A + 36 * k;
+  float m = 0.0f;
+  for (int i = 0; i < 36; ++i) m = fmaxf(m, fabsf(a[i]));
+  const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+  uint8_t* h = H + kHalfBlock * k;
+  for (int i = 0; i < 36; i += 2) {
+    const __half2 v = __floats2half2_rn(a[i] * inv, a[i + 1] * inv);
+    *reinterpret_cast<__half2*>(h + 2 * i) = v;
+  }
+  reinterpret_cast<float*>(h + 72)[0] = m;
+  reinterpret_cast<float*>(h + 72)[1] = 0.0f;
+}
+
 __global__ void __launch_bounds__(kWG) k_f32_many(const BulkSeg* __restrict__ segs, int n) {
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const BulkSeg g = segs[k];
@@ -1696,6 +1716,10 @@ static unsigned bulk_grid(int n) { return unsigned(n < 65536 ? n : 65536); }
 void launch_zero_many(hipStream_t s, const BulkSeg* segs, int n) {
   if (n > 0) hipLaunchKernelGGL(k_zero_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
 }
+void launch_half_blocks(hipStream_t s, const float* A, uint8_t* H, int64_t nnzb) {
+  if (nnzb > 0) hipLaunchKernelGGL(k_half_blocks, dim3(unsigned((nnzb + kWG - 1) / kWG)), dim3(kWG), 0, s, A, H, nnzb);
+}
+
 void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n) {
   if (n > 0) hipLaunchKernelGGL(k_f32_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
 }

@@ -10,6 +10,10 @@
 //   solve_coarsest amg.cpp:580-592    -> k_dense_solve (the coarsest level's inverse, FP64)
 //   StressProblem::iterate            -> k_pcg_spmv, k_axpy_rr, k_update_p, k_finish_*
 //     stress.cpp:403-464                 (the dots: per-tile partials summed per graph in order)
+#include <hip/hip_fp16.h>
+
+#include <type_traits>
+
 #include "svxg_kernels.hpp"
 
 namespace svxg {
@@ -89,6 +93,63 @@ __device__ __forceinline__ T spmv_row_part(const int32_t* __restrict__ rowptr, c
       }
   }
   return acc;
+}
+
+// spmv_row_part over the FP16 block copies: row q of each block (six halves at byte 12 q) times x,
+// scaled by the block's largest magnitude, in FP32
+template <int S>
+__device__ __forceinline__ float spmv_row_part_h(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                 const uint8_t* __restrict__ A16, const float* __restrict__ x, int row,
+                                                 int q, int sub) {
+  constexpr int U = 4;
+  float acc = 0.0f;
+  const int k0 = ldg(rowptr + row), k1 = ldg(rowptr + row + 1);
+  for (int k = k0 + sub; k < k1; k += U * S) {
+    int kk[U], j[U];
+    uint32_t w[U][3];
+    float sc[U], v[U][6];
+#pragma unroll
+    for (int u = 0; u < U; ++u) kk[u] = min(k + u * S, k1 - 1);
+#pragma unroll
+    for (int u = 0; u < U; ++u) j[u] = ldg(col + kk[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint8_t* blk = A16 + size_t(kHalfBlock) * size_t(kk[u]);
+      const uint32_t* rp = reinterpret_cast<const uint32_t*>(blk + 12 * q);
+      w[u][0] = ldg(rp);
+      w[u][1] = ldg(rp + 1);
+      w[u][2] = ldg(rp + 2);
+      sc[u] = ldg(reinterpret_cast<const float*>(blk + 72));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) load6(x + 6 * size_t(j[u]), v[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k + u * S < k1) {
+        float a[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          __half2 h;
+          __builtin_memcpy(&h, &w[u][c], 4);
+          const float2 f = __half22float2(h);
+          a[2 * c] = f.x;
+          a[2 * c + 1] = f.y;
+        }
+        const float s = a[0] * v[u][0] + a[1] * v[u][1] + a[2] * v[u][2] + a[3] * v[u][3] + a[4] * v[u][4] +
+                        a[5] * v[u][5];
+        acc += sc[u] * s;
+      }
+  }
+  return acc;
+}
+
+template <typename T, int S>
+__device__ __forceinline__ T spmv_row_part_v(const LevelDev& L, const T* __restrict__ A, const T* __restrict__ x,
+                                             int row, int q, int sub) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.A16) return spmv_row_part_h<S>(L.rowptr, L.col, L.A16, x, row, q, sub);
+  }
+  return spmv_row_part<T, S>(L.rowptr, L.col, A, x, row, q, sub);
 }
 
 // (A x)_q of block row `row`: sum over the row's blocks of (row q of the block) . x_col
@@ -211,7 +272,7 @@ __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl
       r = bq;
     } else {
       xin = a.x[in_buf][e];
-      r = spmv_row_part<T, S>(L.rowptr, L.col, a.A, a.x[in_buf], row, q, sub);
+      r = spmv_row_part_v<T, S>(L, a.A, a.x[in_buf], row, q, sub);
     }
   }
   if (!FRESH) r = bq - row_reduce<S>(r);
@@ -251,7 +312,7 @@ __device__ __forceinline__ void residual_tile(const GraphDev& g, const Tile& tl,
   lane_row_s<S>(rl, sub, q);
   const bool on = rl >= 0 && rl < tl.nrows;
   const int row = tl.row0 + (on ? rl : 0);
-  T v = on ? spmv_row_part<T, S>(L.rowptr, L.col, a.A, a.x[buf], row, q, sub) : T(0);
+  T v = on ? spmv_row_part_v<T, S>(L, a.A, a.x[buf], row, q, sub) : T(0);
   v = row_reduce<S>(v);
   if (!on || sub != 0) return;
   const size_t e = 6 * size_t(row) + q;

@@ -91,6 +91,11 @@ enum : int32_t {
   SERVE_NEW = 32,
 };
 
+// an FP16 copy of a 6x6 block: the 36 entries over the block's largest magnitude (row-major), that
+// magnitude as a float, 4 bytes of padding. (Scaling each 3x3 quarter on its own was measured: no
+// fewer iterations, more bytes.)
+constexpr int kHalfBlock = 80;
+
 template <typename T>
 struct LevelArrays {
   const T* A;      // 36 per block, row-major 6x6
@@ -121,6 +126,7 @@ struct LevelDev {
   int32_t sub;           // six-lane sub-groups a row in the tiled launches of this level (1 or 10)
   int32_t rsub;          // the same for its restriction (coarse rows of the next level)
   int32_t pad_;
+  const uint8_t* A16;    // A in FP16 blocks scaled by their largest entry (kHalfBlock bytes a block), or null
   LevelArrays<double> f64;
   LevelArrays<float> f32;
 };

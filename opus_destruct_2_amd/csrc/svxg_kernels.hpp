@@ -255,6 +255,8 @@ struct BulkSeg {
 void launch_zero_many(hipStream_t s, const BulkSeg* segs, int n);
 // each segment's count doubles at src written as floats at dst
 void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n);
+// FP16 block copies (kHalfBlock bytes each) of nnzb FP32 6x6 blocks
+void launch_half_blocks(hipStream_t s, const float* A, uint8_t* H, int64_t nnzb);
 // each segment's count bytes copied from src to dst (both 16-byte aligned)
 void launch_copy_many(hipStream_t s, const BulkSeg* segs, int n);
 

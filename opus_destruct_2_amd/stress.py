@@ -103,7 +103,7 @@ def default_options(**kw):
     o = StressOptions()
     lib().svxg_default_options(ctypes.byref(o))
     for k, v in kw.items():
-        if k in ("rtol", "amg_min_nodes", "precision", "mixed_fp64_rows"):
+        if k in ("rtol", "amg_min_nodes", "precision", "mixed_fp64_rows", "vcycle_fp16_rows"):
             setattr(o, k, v)
         else:
             setattr(o.amg, k, v)

@@ -50,11 +50,18 @@ def test_struct_layouts_match_a_c_compiler(tmp_path):
     mirrors = {"svxg_amg_options": abi.AmgOptions, "svxg_stress_options": abi.StressOptions,
                "svxg_pcg_result": abi.PcgResult, "svxg_problem_info": abi.ProblemInfo,
                "svxg_server_options": abi.ServerOptions, "svxg_serve_done": abi.ServeDone,
-               "svxg_server_stats": abi.ServerStats}
+               "svxg_server_stats": abi.ServerStats, "svxg_judge_options": abi.JudgeOptions,
+               "svxg_judge_state": abi.JudgeState, "svxg_judge_result": abi.JudgeResult,
+               "svxg_cascade_result": abi.CascadeResult}
+    dtypes = {"svxg_hinge_cut": abi.HINGE_CUT, "svxg_break": abi.BREAK, "svxg_bond_load": abi.BOND_LOAD}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svxg.h"', "int main(void) {"]
     for c, py in mirrors.items():
         lines.append(f'  printf("{c} %zu\\n", sizeof({c}));')
         for f, _ in py._fields_:
+            lines.append(f'  printf("{c}.{f} %zu\\n", offsetof({c}, {f}));')
+    for c, dt in dtypes.items():
+        lines.append(f'  printf("{c} %zu\\n", sizeof({c}));')
+        for f in dt.names:
             lines.append(f'  printf("{c}.{f} %zu\\n", offsetof({c}, {f}));')
     lines.append("  return 0;\n}")
     src = tmp_path / "layout.c"
@@ -66,12 +73,20 @@ def test_struct_layouts_match_a_c_compiler(tmp_path):
         assert int(got[c]) == ctypes.sizeof(py), c
         for f, _ in py._fields_:
             assert int(got[f"{c}.{f}"]) == getattr(py, f).offset, (c, f)
+    for c, dt in dtypes.items():
+        assert int(got[c]) == dt.itemsize, c
+        for f in dt.names:
+            assert int(got[f"{c}.{f}"]) == dt.fields[f][1], (c, f)
 
 
 def test_default_options_without_a_device():
     from opus_destruct_2_amd import stress
     o = stress.default_options()
     assert o.rtol == 2e-3 and o.amg_min_nodes == 160 and o.amg.coarse_max == 48 and o.amg.strength == 0.1
+    assert o.mixed_fp64_rows == 1 << 30 and o.vcycle_fp16_rows == 16384
+    j = stress.default_judge_options()
+    assert (j.fragility, j.dif, j.break_band, j.quota_fraction, j.hinge_shape) == (1.0, 1.5, 0.85, 0.25, 1.3)
+    assert (j.max_breaks, j.max_rounds, j.plastic_hinges, j.rule) == (256, 400, 1, 0)
 
 
 def _digest(g):

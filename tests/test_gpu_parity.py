@@ -47,8 +47,9 @@ CASES = cases()
 IDS = [c[0] for c in CASES]
 # the solve's precision: plain FP64; mixed (FP64 K products, FP32 V-cycle); mixed with FP32 K
 # products too, inside FP64 iterative refinement
-PRECISIONS = [(SVXG_FP64, {}), (SVXG_MIXED, {}), (SVXG_MIXED, {"mixed_fp64_rows": 0})]
-PREC_IDS = ["fp64", "mixed", "mixed_ir"]
+PRECISIONS = [(SVXG_FP64, {}), (SVXG_MIXED, {}), (SVXG_MIXED, {"mixed_fp64_rows": 0}),
+              (SVXG_MIXED, {"vcycle_fp16_rows": 0})]
+PREC_IDS = ["fp64", "mixed", "mixed_ir", "mixed_vc16"]
 
 
 def failed_sets_agree(phi, phi_ref, band=1e-4):
@@ -137,3 +138,31 @@ def test_resident_solution_checks(mats, ctx):
     a = d.bond_checks(None)
     b = d.bond_checks(u)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_vcycle_fp16_copies(mats, ctx):
+    """The V-cycle's FP16 block copies (the mixed default for large graphs) against its FP32 copies: the same
+    iterations within a few, the same answer within the mixed bar, a symmetric preconditioner; a
+    change of the option needs a new setup."""
+    g = gen.block(16, 16, 12, 0.5, 0.15, seed=3)
+    us, its = [], []
+    for fp16 in (1, 0):
+        opts = stress.default_options(precision=SVXG_MIXED, vcycle_fp16_rows=0 if fp16 else 1 << 30)
+        p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
+        assert p.assemble(opts)
+        u, r = p.solve(g.f, rtol=1e-10, maxit=2000)
+        assert r["converged"]
+        us.append(u)
+        its.append(r["iters"])
+        n6 = 6 * p.info()["nfree"]
+        rng = np.random.default_rng(5)
+        r1, r2 = rng.standard_normal(n6), rng.standard_normal(n6)
+        z1, z2 = p.apply_preconditioner(r1), p.apply_preconditioner(r2)
+        assert abs(r2 @ z1 - r1 @ z2) <= 1e-5 * np.linalg.norm(r2) * np.linalg.norm(z1)
+        if fp16:
+            p.set_options(stress.default_options(precision=SVXG_MIXED, vcycle_fp16_rows=1 << 30))
+            assert not p.info()["assembled"]
+        p.close()
+    assert abs(its[0] - its[1]) <= 3
+    scale = np.abs(us[1]).max()
+    assert np.abs(us[0] - us[1]).max() <= 1e-6 * scale
