@@ -7,8 +7,9 @@ device-resident solution. The headline runs the mixed path (FP64 PCG and K produ
 V-cycle); the plain FP64 path (FP64 V-cycle) and the mixed split with FP32 K products inside FP64
 iterative refinement are measured beside it. Further legs, each reported in the same line: the
 batched uneven graphs (4096 pieces), the serving stream (60 Hz ticks), the city chunk (400
-resident buildings with ground-storey collapses) and the block's break-round cascade (strengths
-scaled so that its largest utilisation is 3, solved, judged and broken until it settles).
+resident buildings with ground-storey collapses), the block's break-round cascade (strengths
+scaled so that its largest utilisation is 3, solved, judged and broken until it settles) and a
+structure's graph built from voxels (a 128 x 128 x 96 Voronoi block: bond sections on the device).
 
 value: solved DOFs / (device time of the solve + wall time of the checks), inputs resident.
 roofline: the dominant kernel's algorithmic bytes per launch / its average launch time, measured
@@ -17,7 +18,7 @@ cpu_baseline: the reference's own solver (compiled from its sources into oracle/
 workload, one assemble + solve, all host threads (up to 16).
 
   python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu] [--gpus 1]
-  python bench.py --only batch|serve|city|cascade ...   (one leg alone, for profiling)
+  python bench.py --only batch|serve|city|cascade|graph ...   (one leg alone, for profiling)
 """
 import argparse
 import json
@@ -237,6 +238,52 @@ def run_cascade(mats, ctx, g, target, cpu=None):
                               "kind": cpu["kind"]}
         rec["vs_cpu_per_round"] = round(per * 1e3 / rec["ms_per_round"], 1)
     log(f"  cascade: {rec}")
+    return rec
+
+
+def run_graph(mats, ctx, shape, cpu=True):
+    """The graph of a voxel structure: a Voronoi block of `shape` voxels (fragments of about 2^3
+    voxels, 15 % of them void, on an anchored ground layer) extracted on the host into nodes and face
+    lists (opus_destruct_2_amd/voxels.py), the bonds' sections computed on the device
+    (svxg_bond_sections, wall time with the copies), against the oracle's restatement on one host
+    thread; then the graph assembled and solved (mixed, 1e-8)."""
+    import oracle
+    from opus_destruct_2_amd import stress, voxels
+    from opus_destruct_2_amd.abi import SVXG_MIXED
+    h = 0.25
+    t = time.perf_counter()
+    vox, frag, lo = voxels.voronoi_block(*shape, cell=2, voids=0.15, seed=1)
+    gen_s = time.perf_counter() - t
+    t = time.perf_counter()
+    nodes, heads, faces = voxels.extract(vox, frag, lo, h, mats)
+    extract_s = time.perf_counter() - t
+    walls = []
+    for _ in range(3):
+        t = time.perf_counter()
+        bonds = voxels.bond_sections(nodes, heads, faces, vox, lo, h, mats, ctx=ctx)
+        walls.append((time.perf_counter() - t) * 1e3)
+    rec = {"voxels": int(vox.size), "nodes": int(len(nodes)), "bonds": int(len(heads)), "faces": int(len(faces)),
+           "supports": int((heads["b"] < 0).sum()), "host_generate_s": round(gen_s, 2),
+           "host_extract_s": round(extract_s, 2), "sections_wall_ms": round(float(np.median(walls)), 2),
+           "bonds_per_s_wall": round(len(heads) / (float(np.median(walls)) / 1e3), 1)}
+    if cpu:
+        t = time.perf_counter()
+        ob = oracle.bond_sections(nodes, heads, faces, vox, lo, h, mats)
+        cs = time.perf_counter() - t
+        rec["cpu_sections_s"] = round(cs, 3)
+        rec["cpu_cores"] = 1
+        rec["bitwise_equal_cpu"] = bool(ob.tobytes() == bonds.tobytes())
+    p = stress.StressProblem(nodes, bonds, mats, ctx, options=stress.default_options(precision=SVXG_MIXED, **AMG_KW))
+    t = time.perf_counter()
+    assert p.assemble()
+    rec["setup_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+    u, r = p.solve(voxels.gravity(nodes), rtol=1e-8)
+    u, r = p.solve(voxels.gravity(nodes), rtol=1e-8)
+    rec["solve_ms"] = round(p.last_timing()["solve_ms"], 3)
+    rec["iters"] = r["iters"]
+    rec["dofs"] = int(6 * p.info()["nfree"])
+    p.close()
+    log(f"  graph from voxels: {rec}")
     return rec
 
 
@@ -499,6 +546,8 @@ def main():
     ap.add_argument("--city-rate", type=float, default=20.0, help="collapse events per second")
     ap.add_argument("--cascade", type=float, default=3.0,
                     help="largest phi of the cascade leg's block under gravity (0: skip the leg)")
+    ap.add_argument("--graph", default="128x128x96",
+                    help="voxels of the graph-from-voxels leg ('' skips it)")
     ap.add_argument("--smooth-levels", type=int, default=None,
                     help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
     ap.add_argument("--cheb-degree", type=int, default=None)
@@ -506,7 +555,7 @@ def main():
                     help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
-    ap.add_argument("--only", choices=["batch", "serve", "city", "cascade"], default=None,
+    ap.add_argument("--only", choices=["batch", "serve", "city", "cascade", "graph"], default=None,
                     help="profiling: run only this leg and print its record (not the benchmark line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
@@ -557,6 +606,8 @@ def main():
                               setup_workers=args.serve_setup_workers)
         elif args.only == "cascade":
             rec = run_cascade(mats, ctx, g, args.cascade)
+        elif args.only == "graph":
+            rec = run_graph(mats, ctx, tuple(int(x) for x in args.graph.split("x")), cpu=not args.no_cpu)
         else:
             rec = run_city(mats, ctx, args.city, args.city_rounds, args.city_rate)
         print(json.dumps({"only": args.only, "record": rec}), flush=True)
@@ -627,6 +678,12 @@ def main():
         kctx = stress.Context(0)
         line["cascade"] = run_cascade(mats, kctx, g, args.cascade, cb)
         kctx.close()
+    if args.graph:
+        log(f"graph from voxels: a {args.graph} Voronoi block")
+        gctx = stress.Context(0)
+        line["graph_from_voxels"] = run_graph(mats, gctx, tuple(int(x) for x in args.graph.split("x")),
+                                              cpu=not args.no_cpu)
+        gctx.close()
     if cb is not None:
         for k in ("u", "phi", "setup_s", "solve_s", "iters"):
             cb.pop(k)

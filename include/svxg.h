@@ -391,6 +391,55 @@ int svxg_break_rounds_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_
                             int32_t max_call_rounds, svxg_judge_state* state, svxg_cascade_result* res,
                             svxg_break* const* breaks, const int32_t* capacity, int32_t* const* retired);
 
+/* ---- bond sections from voxel faces ----------------------------------------------------------
+   The geometry and strengths of a structure's bonds from the voxel faces between its fragments
+   (SecAcc::finish_lattice, world_internal.hpp:129-249, and section_strengths, :374-400, called by
+   World::Impl::append_nodes, world.cpp:835-850): per bond, the interface normal (the faces' mean
+   normal from a to b; a support's face axis), the section's centroid, principal axes, area, second
+   moments, extreme fibres and torsion radius, the spring lengths la / lb from the two centres, and
+   the section strengths (each face as strong as the weaker of its two voxels' materials, times the
+   condition of the more damaged; the faces' mean). One lattice, unrotated (the reference's
+   finish_lattice case); junction samples between lattices are not covered. */
+
+/* Vox (core/include/svx/world/grid.hpp:28-36): 0 air; else material id + 1 in the low 7 bits,
+   0x80 for an anchored (ground) voxel. A box of voxels lo .. lo + dim - 1, x fastest; voxels
+   outside it are air. */
+typedef struct svxg_voxel_box {
+  int32_t lo[3];
+  int32_t dim[3];
+  double h;                  /* voxel size (m) */
+  const uint8_t* vox;        /* dim[0] * dim[1] * dim[2] values */
+  const uint8_t* damage;     /* the same count, 0 intact .. 255 no strength left; or null */
+} svxg_voxel_box;
+
+/* One bond's accumulated faces (SecHead / SecAcc, world_internal.hpp:105-125): nodes a, b (b < 0:
+   a support, its face axis and a's side sign), faces [face0, face0 + nfaces) of the face list, the
+   two sides' materials and strength multipliers (class_mult; a support's strength_b 1e9). */
+typedef struct svxg_section_head {
+  int32_t a, b;
+  int32_t axis, sign;
+  int32_t face0, nfaces;
+  uint8_t ma, mb;
+  uint8_t pad_[6];
+  double strength_a, strength_b;
+} svxg_section_head;       /* 48 bytes */
+
+/* A face: its lower voxel p (the face joins p and p + e_axis) and whether a is on that lower side
+   (+1: the face's normal from a to b is +e_axis). */
+typedef struct svxg_section_face {
+  int32_t p[3];
+  int8_t axis;
+  int8_t sign;
+  uint8_t pad_[2];
+} svxg_section_face;       /* 16 bytes */
+
+/* out[k] (m bonds): bond k of heads[k], as append_nodes makes it: a, b, ma, mb, faces, the frame,
+   area, p, s1, s2, c1, c2, rmax, la, lb, strength = min(strength_a, strength_b), the section
+   strengths (sectioned = 1) and tag = k; broken / in_k 0. nodes: the graph's node centres. */
+int svxg_bond_sections(svxg_ctx* ctx, const svxg_node* nodes, int32_t n, const svxg_section_head* heads, int32_t m,
+                       const svxg_section_face* faces, int32_t nfaces, const svxg_voxel_box* box,
+                       const svxg_material* mats, int32_t nmats, svxg_bond* out);
+
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------
    Replaces the reference's per-tick structure loop (step_structures, world.cpp:986-1129: the
    structures of a tick solved one after another within a block-operation budget) and its piece

@@ -179,6 +179,21 @@ def ductile_mask(ids):
     return m
 
 
+# ---- bond sections from voxel faces (world_internal.hpp:105-125, 129-249) ------------------------
+SECTION_HEAD = np.dtype([
+    ("a", "<i4"), ("b", "<i4"), ("axis", "<i4"), ("sign", "<i4"), ("face0", "<i4"), ("nfaces", "<i4"),
+    ("ma", "u1"), ("mb", "u1"), ("pad_", "u1", (6,)), ("strength_a", "<f8"), ("strength_b", "<f8"),
+], align=True)
+SECTION_FACE = np.dtype([("p", "<i4", (3,)), ("axis", "i1"), ("sign", "i1"), ("pad_", "u1", (2,))], align=True)
+assert SECTION_HEAD.itemsize == 48 and SECTION_FACE.itemsize == 16
+
+
+class VoxelBox(ctypes.Structure):
+    """svxg_voxel_box: Vox codes (0 air, material id + 1, 0x80 anchored), x fastest."""
+    _fields_ = [("lo", ctypes.c_int32 * 3), ("dim", ctypes.c_int32 * 3), ("h", ctypes.c_double),
+                ("vox", ctypes.c_void_p), ("damage", ctypes.c_void_p)]
+
+
 def ptr(a):
     """The data pointer of a numpy array (None for None)."""
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)

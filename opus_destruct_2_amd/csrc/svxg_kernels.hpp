@@ -339,4 +339,19 @@ struct ZeroRowsJob {
 };
 void launch_zero_rows(hipStream_t s, const ZeroRowsJob* jobs, int count);
 
+// ---- bond sections from voxel faces (world_internal.hpp:129-249, 374-400) --------------------------
+struct SectionDev {
+  const svxg_node* nodes;
+  const svxg_section_head* heads;
+  const svxg_section_face* faces;
+  const uint8_t* vox;
+  const uint8_t* damage;  // or null
+  const svxg_material* mats;
+  svxg_bond* out;
+  int32_t lo[3], dim[3];
+  int32_t nmats, m;
+  double h;
+};
+void launch_bond_sections(hipStream_t s, const SectionDev* dev, int m);
+
 }  // namespace svxg

@@ -83,6 +83,7 @@ def lib():
             "svxg_problem_get_judge_baseline": ([vp, vp, vp], ctypes.c_int),
             "svxg_judge_batch": ([vp, vp, i32, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "svxg_break_rounds_batch": ([vp, vp, i32, vp, vp, f64, i32, vp, i32, vp, vp, vp, vp, vp], ctypes.c_int),
+            "svxg_bond_sections": ([vp, vp, i32, vp, i32, vp, i32, vp, vp, i32, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

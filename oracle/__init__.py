@@ -52,7 +52,8 @@ def _bind(path, pre):
                     "judge": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
                     "hinge_of": ([vp, vp, vp, vp, vp, i32, vp, f64, f64, vp], i32),
                     "lerp_load": ([vp, vp, f64, vp], None),
-                    "graph_components": ([i32, vp, i32, vp, vp], i32)})
+                    "graph_components": ([i32, vp, i32, vp, vp], i32),
+                    "bond_sections": ([vp, i32, vp, i32, vp, i32, vp, vp, i32, vp], None)})
     for name, (args, res) in sig.items():
         fn = getattr(L, pre + name)
         fn.argtypes = args
@@ -288,6 +289,25 @@ def graph_components(n, bonds, seed):
     return nc, comp[:n]
 
 
+def bond_sections(nodes, heads, faces, vox, lo, h, mats, damage=None):
+    """orc_bond_sections: SecAcc::finish_lattice + section_strengths (world_internal.hpp:129-249,
+    374-400) of every head, on the CPU."""
+    from opus_destruct_2_amd.voxels import voxel_box
+    from opus_destruct_2_amd.abi import SECTION_FACE, SECTION_HEAD
+    L, _ = library("oracle")
+    vox = np.ascontiguousarray(vox, np.uint8)
+    damage = None if damage is None else np.ascontiguousarray(damage, np.uint8)
+    nodes = np.ascontiguousarray(nodes, NODE)
+    heads = np.ascontiguousarray(heads, SECTION_HEAD)
+    faces = np.ascontiguousarray(faces, SECTION_FACE)
+    mats = np.ascontiguousarray(mats)
+    box = voxel_box(vox, lo, h, damage)
+    out = np.zeros(len(heads), BOND)
+    L.orc_bond_sections(nodes.ctypes.data, len(nodes), heads.ctypes.data, len(heads), faces.ctypes.data, len(faces),
+                        ctypes.byref(box), mats.ctypes.data, len(mats), out.ctypes.data)
+    return out
+
+
 def cascade(problem, judge, f, u, opts, state, rtol=1e-10, maxit=20000, max_call_rounds=0, assemble_opt=None):
     """The break-round cascade of one structure on the CPU (World::Impl::step_structures' solve,
     judge and detach_unsupported, world.cpp:986-1129, 1162-1277, 1371-1444, round after round),
@@ -388,4 +408,4 @@ def bond_strength(bond, fragility, mats, kind="oracle"):
 
 __all__ = ["CpuProblem", "CpuJudge", "build", "library", "ref_available", "bond_utilization", "bond_strength",
            "reference_options", "StressOptions", "set_ref_threads", "hinge_of", "lerp_load", "graph_components",
-           "cascade"]
+           "cascade", "bond_sections"]

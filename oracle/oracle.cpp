@@ -1323,6 +1323,166 @@ int32_t orc_judge(orc_problem* h, const double* u, const svxg_judge_options* opt
   return count;
 }
 
+void orc_bond_sections(const svxg_node* nodes, int32_t, const svxg_section_head* heads, int32_t m,
+                       const svxg_section_face* faces, int32_t, const svxg_voxel_box* box,
+                       const svxg_material* mats, int32_t nmats, svxg_bond* out) {
+  const f64 h = box->h, A1 = h * h;
+  auto at = [&](const int32_t p[3], uint8_t* v, uint8_t* dmg) {
+    i64 idx = 0, mul = 1;
+    for (int c = 0; c < 3; ++c) {
+      const i32 r = p[c] - box->lo[c];
+      if (r < 0 || r >= box->dim[c]) {
+        *v = 0;
+        *dmg = 0;
+        return;
+      }
+      idx += mul * r;
+      mul *= box->dim[c];
+    }
+    *v = box->vox[idx];
+    *dmg = box->damage ? box->damage[idx] : 0;
+  };
+  auto material = [&](uint8_t v) -> const svxg_material& {
+    const int id = (v & 0x7F) - 1;
+    return mats[(id >= 0 && id < nmats) ? id : 0];
+  };
+  auto comp = [](V3& a, int i) -> f64& { return i == 0 ? a.x : (i == 1 ? a.y : a.z); };
+  auto unit = [](const V3& a) {
+    const f64 l = std::sqrt(vdot(a, a));
+    return l > 0 ? V3{a.x * (1.0 / l), a.y * (1.0 / l), a.z * (1.0 / l)} : V3{0, 0, 0};
+  };
+  for (i32 k = 0; k < m; ++k) {
+    const svxg_section_head& hd = heads[k];
+    const svxg_section_face* F = faces + hd.face0;
+    const i32 nf = hd.nfaces;
+    const V3 ca = v3(nodes[hd.a].c);
+    const bool pair = hd.b >= 0;
+    const V3 cb = pair ? v3(nodes[hd.b].c) : V3{0, 0, 0};
+    // the interface normal: the faces' mean normal from a to b; a support's face axis
+    V3 n{0, 0, 0};
+    if (pair) {
+      for (i32 f = 0; f < nf; ++f) comp(n, F[f].axis) += F[f].sign;
+      if (vdot(n, n) < 1e-12) n = vsub(cb, ca);
+      if (vdot(n, n) < 1e-12) n = V3{0, 0, 1};
+      n = unit(n);
+    } else {
+      comp(n, hd.axis) = hd.sign;
+    }
+    // frame_of (world_internal.hpp:64-69)
+    const f64 ax = std::abs(n.x), ay = std::abs(n.y), az = std::abs(n.z);
+    const V3 e = (ax <= ay && ax <= az) ? V3{1, 0, 0} : (ay <= az ? V3{0, 1, 0} : V3{0, 0, 1});
+    V3 t1 = unit(vcross(n, e));
+    V3 t2 = vcross(n, t1);
+    std::vector<V3> xs(static_cast<size_t>(nf));
+    std::vector<f64> ws(static_cast<size_t>(nf));
+    f64 W = 0.0;
+    V3 pc{0, 0, 0};
+    for (i32 f = 0; f < nf; ++f) {
+      V3 x{h * F[f].p[0], h * F[f].p[1], h * F[f].p[2]};
+      comp(x, F[f].axis) += 0.5 * h;
+      xs[size_t(f)] = x;
+      V3 nn = n;
+      ws[size_t(f)] = A1 * std::max(0.05, std::abs(comp(nn, F[f].axis)));
+      W += ws[size_t(f)];
+      pc = V3{pc.x + x.x * ws[size_t(f)], pc.y + x.y * ws[size_t(f)], pc.z + x.z * ws[size_t(f)]};
+    }
+    pc = V3{pc.x * (1.0 / W), pc.y * (1.0 / W), pc.z * (1.0 / W)};
+    // principal axes of the section (the eigenvector of the larger eigenvalue, sqrt only)
+    f64 s11 = 0.0, s22 = 0.0, s12 = 0.0;
+    for (i32 f = 0; f < nf; ++f) {
+      const V3 d = vsub(xs[size_t(f)], pc);
+      const f64 y1 = vdot(d, t1), y2 = vdot(d, t2);
+      s11 += ws[size_t(f)] * y1 * y1;
+      s22 += ws[size_t(f)] * y2 * y2;
+      s12 += ws[size_t(f)] * y1 * y2;
+    }
+    if (std::abs(s12) > 1e-9 * (s11 + s22)) {
+      const f64 half = 0.5 * (s11 - s22);
+      const f64 l1 = 0.5 * (s11 + s22) + std::sqrt(half * half + s12 * s12);
+      f64 e1 = s12, e2 = l1 - s11;
+      if (std::abs(l1 - s22) > std::abs(e2)) {
+        e1 = l1 - s22;
+        e2 = s12;
+      }
+      const f64 en = std::sqrt(e1 * e1 + e2 * e2);
+      if (en > 0.0) {
+        e1 /= en;
+        e2 /= en;
+        const V3 u1{t1.x * e1 + t2.x * e2, t1.y * e1 + t2.y * e2, t1.z * e1 + t2.z * e2};
+        const V3 u2{t2.x * e1 - t1.x * e2, t2.y * e1 - t1.y * e2, t2.z * e1 - t1.z * e2};
+        t1 = u1;
+        t2 = u2;
+      }
+    }
+    svxg_bond B{};
+    B.a = hd.a;
+    B.b = hd.b;
+    B.ma = hd.ma;
+    B.mb = hd.mb;
+    B.faces = nf;
+    B.tag = k;
+    for (int c = 0; c < 3; ++c) {
+      B.n[c] = vget(n, c);
+      B.t1[c] = vget(t1, c);
+      B.t2[c] = vget(t2, c);
+      B.p[c] = vget(pc, c);
+    }
+    B.area = std::max(W, 0.25 * nf * A1);
+    f64 s1 = 0.0, s2 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (i32 f = 0; f < nf; ++f) {
+      const V3 d = vsub(xs[size_t(f)], pc);
+      const f64 y1 = vdot(d, t1), y2 = vdot(d, t2);
+      s1 += ws[size_t(f)] * y1 * y1;
+      s2 += ws[size_t(f)] * y2 * y2;
+      m1 = std::max(m1, std::abs(y1));
+      m2 = std::max(m2, std::abs(y2));
+    }
+    B.s1 = s1 + B.area * A1 / 12.0;
+    B.s2 = s2 + B.area * A1 / 12.0;
+    B.c1 = m1 + 0.5 * h;
+    B.c2 = m2 + 0.5 * h;
+    B.rmax = std::sqrt(B.c1 * B.c1 + B.c2 * B.c2);
+    if (pair) {
+      const V3 d = vsub(cb, ca);
+      const f64 L = std::max({std::abs(vdot(d, n)), 0.5 * std::sqrt(vdot(d, d)), 0.5 * h});
+      B.la = std::clamp(vdot(vsub(pc, ca), n), 0.1 * L, 0.9 * L);
+      B.lb = L - B.la;
+    } else {
+      B.la = std::max(0.25 * h, vdot(vsub(pc, ca), n));
+      B.lb = 0.5 * h;
+    }
+    B.strength = std::min(hd.strength_a, hd.strength_b);
+    // section_strengths: per face the weaker material of its two voxels (an air side takes the
+    // other's), times the condition of the more damaged; the faces' mean
+    f64 ft = 0, fb = 0, fc = 0, coh = 0, mu = 0;
+    for (i32 f = 0; f < nf; ++f) {
+      uint8_t va, da, vb, db;
+      at(F[f].p, &va, &da);
+      int32_t q[3] = {F[f].p[0], F[f].p[1], F[f].p[2]};
+      q[F[f].axis] += 1;
+      at(q, &vb, &db);
+      const svxg_material& Ma = material(va ? va : vb);
+      const svxg_material& Mb = material(vb ? vb : va);
+      const f64 cond = std::max(1e-6, 1.0 - static_cast<f64>(std::max(da, db)) / 255.0);
+      ft += cond * std::min(Ma.ft, Mb.ft);
+      fb += cond * std::min(Ma.fb, Mb.fb);
+      fc += cond * std::min(Ma.fc, Mb.fc);
+      coh += cond * std::min(Ma.cohesion, Mb.cohesion);
+      mu += std::min(Ma.friction, Mb.friction);
+    }
+    if (nf > 0) {
+      const f64 w = 1.0 / static_cast<f64>(nf);
+      B.sectioned = 1;
+      B.ft = static_cast<f32>(ft * w);
+      B.fb = static_cast<f32>(fb * w);
+      B.fc = static_cast<f32>(fc * w);
+      B.coh = static_cast<f32>(coh * w);
+      B.mu = static_cast<f32>(mu * w);
+    }
+    out[k] = B;
+  }
+}
+
 int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp) {
   // adjacency of the intact bonds between nodes, then floods: the seeds first (component 0), then
   // every node not reached yet, in index order

@@ -57,6 +57,12 @@ int32_t orc_hinge_of(const svxg_bond* b, const svxg_bond_load* L, const double r
                      double hinge_shape, svxg_hinge_cut* out);
 /* lerp_load (world_internal.hpp:452-461) */
 void orc_lerp_load(const svxg_bond_load* prev, const svxg_bond_load* now, double k, svxg_bond_load* out);
+/* SecAcc::finish_lattice + section_strengths (world_internal.hpp:129-249, 374-400) of every head, as
+   World::Impl::append_nodes makes the bonds (world.cpp:835-850): the restatement of
+   svxg_bond_sections. */
+void orc_bond_sections(const svxg_node* nodes, int32_t n, const svxg_section_head* heads, int32_t m,
+                       const svxg_section_face* faces, int32_t nfaces, const svxg_voxel_box* box,
+                       const svxg_material* mats, int32_t nmats, svxg_bond* out);
 /* graph_components (world_internal.hpp:660-706): comp[n]; returns the number of components */
 int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp);
 
