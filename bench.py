@@ -551,6 +551,7 @@ def main():
     ap.add_argument("--smooth-levels", type=int, default=None,
                     help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
     ap.add_argument("--cheb-degree", type=int, default=None)
+    ap.add_argument("--coarse-cheb-degree", type=int, default=None)
     ap.add_argument("--mixed-fp64-rows", type=int, default=None,
                     help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
@@ -592,6 +593,8 @@ def main():
         AMG_KW["smooth_levels"] = args.smooth_levels
     if args.cheb_degree is not None:
         AMG_KW["cheb_degree"] = args.cheb_degree
+    if args.coarse_cheb_degree is not None:
+        AMG_KW["coarse_cheb_degree"] = args.coarse_cheb_degree
     if args.mixed_fp64_rows is not None:
         AMG_KW["mixed_fp64_rows"] = args.mixed_fp64_rows
     ctx = stress.default_context()

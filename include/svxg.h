@@ -95,6 +95,9 @@ typedef struct svxg_amg_options {
   int32_t smooth_levels;     /* device: levels whose prolongator is smoothed, from the finest (0: all,
                                 as the reference); coarser ones are plain aggregation (scaled by
                                 coarse_scale), which keeps coarse operators sparse */
+  int32_t coarse_cheb_degree;/* device: the smoother's degree below level 0 (default 1; 0: cheb_degree).
+                                A batch runs its first problem's degrees */
+  int32_t pad_;
 } svxg_amg_options;
 
 /* StressOptions (stress.hpp:68-72). */

@@ -747,6 +747,14 @@ static void prof_collect(svxg_ctx* c) {
 // ---- the V-cycle -----------------------------------------------------------------------------------
 
 
+// the smoother's degrees of a problem's options (level_degree's packing)
+static int batch_degree(const svxg_problem* P) {
+  const int d0 = std::clamp(P->opt.amg.cheb_degree, 1, 255);
+  const int dc = std::clamp(P->opt.amg.coarse_cheb_degree, 0, 255);
+  return d0 | (dc << 8);
+}
+
+// the V-cycle of a batch; `degree` packs the smoother's degrees (level_degree)
 template <typename T>
 static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   const GraphDev* G = B.dG.as<GraphDev>();
@@ -756,14 +764,15 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   const int SV[2] = {1, 10};
   for (int l = 0; l < L; ++l) {
     Batch::Lv& V = B.lvl[size_t(l)];
+    const int dg = level_degree(degree, l);
     for (int v = 0; v < 2; ++v) {
       if (!V.nts[v]) continue;
-      for (int st = 0; st < degree; ++st) {
+      for (int st = 0; st < dg; ++st) {
         ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
         launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr,
                             SV[v]);
       }
-      launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (degree - 1) & 1, inner, SV[v]);
+      launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (dg - 1) & 1, inner, SV[v]);
     }
     for (int v = 0; v < 2; ++v) launch_restrict<T>(s, G, V.tc[v].as<Tile>(), V.ntc[v], l, inner, SV[v]);
     if (V.ndense) launch_dense_solve<T>(s, G, V.dense.as<Tile>(), V.ndense, l, inner);
@@ -776,10 +785,11 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   for (int l = L - 2; l >= 0; --l) {
     Batch::Lv& V = B.lvl[size_t(l)];
     if (!V.ntp) continue;
-    const int cur = (degree - 1) & 1;
+    const int dg = level_degree(degree, l);
+    const int cur = (dg - 1) & 1;
     launch_prolong<T>(s, G, V.tp.as<Tile>(), V.ntp, l, cur, inner);
-    for (int j = 0; j < degree; ++j) {
-      const int gs = degree + j;
+    for (int j = 0; j < dg; ++j) {
+      const int gs = dg + j;
       for (int v = 0; v < 2; ++v) {
         if (!V.nts[v]) continue;
         ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
@@ -876,7 +886,7 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
 // a window of w steps: the resident graphs' in one launch, the tiled graphs' step by step
 static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64);
 static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, bool p64) {
-  const int degree = std::max(1, B.probs[0]->opt.amg.cheb_degree);
+  const int degree = batch_degree(B.probs[0]);
   if (B.nres)
     launch_resident_steps(s, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.res.as<int32_t>(), B.nres, w, degree,
                           mixed ? 1 : 0, p32 ? 1 : 0, p64 ? 1 : 0, B.partial.as<double>());

@@ -1129,7 +1129,8 @@ __device__ void restrict_g(const GraphDev& g, int gi, int l, int mode) {
   __syncthreads();
 }
 
-// the V-cycle of one graph (vcycle<T> of the host driver, restricted to this graph's levels)
+// the V-cycle of one graph (vcycle<T> of the host driver, restricted to this graph's levels;
+// `degree` packs the smoother's degrees, level_degree)
 template <typename T>
 __device__ void vcycle_g(const GraphDev& g, int gi, int mode, int degree) {
   if (g.jacobi) {
@@ -1140,10 +1141,11 @@ __device__ void vcycle_g(const GraphDev& g, int gi, int mode, int degree) {
   }
   for (int l = 0; l < g.nlev; ++l) {
     const LevelDev& L = g.lv[l];
+    const int dg = level_degree(degree, l);
     if (!L.terminal) {
-      for (int st = 0; st < degree; ++st) smooth_any<T>(g, gi, l, st, (st + 1) & 1, st & 1, st == 0, mode);
-      if (L.sub == 10) residual_g<T, 10>(g, gi, l, (degree - 1) & 1, mode);
-      else residual_g<T, 1>(g, gi, l, (degree - 1) & 1, mode);
+      for (int st = 0; st < dg; ++st) smooth_any<T>(g, gi, l, st, (st + 1) & 1, st & 1, st == 0, mode);
+      if (L.sub == 10) residual_g<T, 10>(g, gi, l, (dg - 1) & 1, mode);
+      else residual_g<T, 1>(g, gi, l, (dg - 1) & 1, mode);
       if (L.rsub == 10) restrict_g<T, 10>(g, gi, l, mode);
       else restrict_g<T, 1>(g, gi, l, mode);
     } else if (L.dense_m > 0) {
@@ -1155,11 +1157,12 @@ __device__ void vcycle_g(const GraphDev& g, int gi, int mode, int degree) {
   }
   for (int l = g.nlev - 2; l >= 0; --l) {
     const LevelDev& L = g.lv[l];
+    const int dg = level_degree(degree, l);
     const int nt = (L.n + kTileRows - 1) / kTileRows;
-    for (int k = 0; k < nt; ++k) prolong_tile<T>(g, tile_of(gi, k, L.n, kTileRows, 0), l, (degree - 1) & 1, mode);
+    for (int k = 0; k < nt; ++k) prolong_tile<T>(g, tile_of(gi, k, L.n, kTileRows, 0), l, (dg - 1) & 1, mode);
     __syncthreads();
-    for (int j = 0; j < degree; ++j) {
-      const int gs = degree + j;
+    for (int j = 0; j < dg; ++j) {
+      const int gs = dg + j;
       smooth_any<T>(g, gi, l, j, (gs + 1) & 1, gs & 1, false, mode);
     }
   }

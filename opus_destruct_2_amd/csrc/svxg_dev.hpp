@@ -91,6 +91,13 @@ enum : int32_t {
   SERVE_NEW = 32,
 };
 
+// the smoother's degrees of a batch in one int: level 0's in the low byte, the coarser levels' in the
+// next (0: the same as level 0's)
+__host__ __device__ inline int level_degree(int packed, int level) {
+  const int d0 = packed & 255, dc = (packed >> 8) & 255;
+  return (level > 0 && dc > 0) ? dc : d0;
+}
+
 // an FP16 copy of a 6x6 block: the 36 entries over the block's largest magnitude (row-major), that
 // magnitude as a float, 4 bytes of padding. (Scaling each 3x3 quarter on its own was measured: no
 // fewer iterations, more bytes.)
