@@ -246,7 +246,8 @@ def run_graph(mats, ctx, shape, cpu=True):
     voxels, 15 % of them void, on an anchored ground layer) extracted on the host into nodes and face
     lists (opus_destruct_2_amd/voxels.py), the bonds' sections computed on the device
     (svxg_bond_sections, wall time with the copies), against the oracle's restatement on one host
-    thread; then the graph assembled and solved (mixed, 1e-8)."""
+    thread; the whole graph on the device (svxg_lattice_graph: two calls, the counts then the
+    records, wall time with the copies); then the graph assembled and solved (mixed, 1e-8)."""
     import oracle
     from opus_destruct_2_amd import stress, voxels
     from opus_destruct_2_amd.abi import SVXG_MIXED
@@ -266,6 +267,15 @@ def run_graph(mats, ctx, shape, cpu=True):
            "supports": int((heads["b"] < 0).sum()), "host_generate_s": round(gen_s, 2),
            "host_extract_s": round(extract_s, 2), "sections_wall_ms": round(float(np.median(walls)), 2),
            "bonds_per_s_wall": round(len(heads) / (float(np.median(walls)) / 1e3), 1)}
+    # the whole graph on the device (faces, bonds, nodes, sections): svxg_lattice_graph
+    lw = []
+    for _ in range(3):
+        t = time.perf_counter()
+        dn, dh, df, db = voxels.lattice_graph(vox, frag, lo, h, mats, ctx=ctx)
+        lw.append((time.perf_counter() - t) * 1e3)
+    rec["device_graph_wall_ms"] = round(float(np.median(lw)), 2)
+    rec["device_graph_equal_host"] = bool(dn.tobytes() == nodes.tobytes() and dh.tobytes() == heads.tobytes()
+                                          and df.tobytes() == faces.tobytes() and db.tobytes() == bonds.tobytes())
     if cpu:
         t = time.perf_counter()
         ob = oracle.bond_sections(nodes, heads, faces, vox, lo, h, mats)

@@ -443,6 +443,21 @@ int svxg_bond_sections(svxg_ctx* ctx, const svxg_node* nodes, int32_t n, const s
                        const svxg_section_face* faces, int32_t nfaces, const svxg_voxel_box* box,
                        const svxg_material* mats, int32_t nmats, svxg_bond* out);
 
+/* A structure's graph from one voxel lattice, all on the device (the walk of World::Impl::extract
+   and append_nodes, world.cpp:425-865, for one unrotated lattice and a node a fragment): frag holds
+   each voxel's fragment (0 .. nfrag-1 for solid unanchored voxels, -1 elsewhere), box the Vox codes
+   (and damage). nodes[f] (nfrag): the fragment's centre of mass and mass (the materials' densities,
+   voxel centres at h times their integer coordinates, fragments.cpp:315). The bonds: one a pair of
+   touching fragments and one a (fragment, axis, side) resting on an anchored voxel, ordered by
+   (node a, pairs before supports, node b / axis and side), their faces in the box's scan order
+   (x fastest, then the axis); heads / faces: the svxg_bond_sections inputs, bonds: its outputs.
+   *nbonds, *nfaces: in, the room of heads and bonds / of faces; out, the counts. With heads, faces
+   and bonds all null only the counts (and nodes) are made; SVXG_BAD_INPUT when the room is short
+   (the counts are set). */
+int svxg_lattice_graph(svxg_ctx* ctx, const svxg_voxel_box* box, const int32_t* frag, int32_t nfrag,
+                       const svxg_material* mats, int32_t nmats, svxg_node* nodes, int32_t* nbonds, int32_t* nfaces,
+                       svxg_section_head* heads, svxg_section_face* faces, svxg_bond* bonds);
+
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------
    Replaces the reference's per-tick structure loop (step_structures, world.cpp:986-1129: the
    structures of a tick solved one after another within a block-operation budget) and its piece

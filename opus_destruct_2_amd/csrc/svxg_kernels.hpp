@@ -354,4 +354,30 @@ struct SectionDev {
 };
 void launch_bond_sections(hipStream_t s, const SectionDev* dev, int m);
 
+// ---- one lattice's graph (world.cpp:425-865 for one lattice, a node a fragment) -----------------------
+struct LatticeDev {
+  const uint8_t* vox;
+  const int32_t* frag;
+  const svxg_material* mats;
+  int32_t lo[3], dim[3];
+  int64_t nvox;
+  int32_t nfrag, nmats;
+  double h;
+};
+void launch_face_flags(hipStream_t s, const LatticeDev* L, int32_t* flag, int64_t nslots);
+void launch_face_write(hipStream_t s, const LatticeDev* L, const int32_t* flag, const int32_t* at, uint64_t* key,
+                       int32_t* idx, svxg_section_face* face, uint8_t* mb, int64_t nslots);
+void launch_gather_faces(hipStream_t s, const int32_t* idx, const svxg_section_face* in, const uint8_t* mb_in,
+                         svxg_section_face* out, uint8_t* mb_out, const uint64_t* key, int32_t* first, int n);
+void launch_heads(hipStream_t s, const uint64_t* key, const int32_t* first, const int32_t* hid, const uint8_t* mb,
+                  const uint8_t* nmat, int nfaces, svxg_section_head* heads, int nheads);
+void launch_voxel_keys(hipStream_t s, const LatticeDev* L, uint32_t* key, int32_t* idx, int32_t* count, int64_t nvox);
+void launch_node_sums(hipStream_t s, const LatticeDev* L, const int32_t* vidx, const int32_t* start, int64_t skip,
+                      svxg_node* nodes, uint8_t* nmat, int nfrag);
+hipError_t scan_exclusive_i32(void* tmp, size_t& bytes, const int32_t* in, int32_t* out, int64_t n, hipStream_t s);
+hipError_t sort_pairs_u64(void* tmp, size_t& bytes, const uint64_t* kin, uint64_t* kout, const int32_t* vin,
+                          int32_t* vout, int64_t n, int bits, hipStream_t s);
+hipError_t sort_pairs_u32(void* tmp, size_t& bytes, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
+                          int32_t* vout, int64_t n, int bits, hipStream_t s);
+
 }  // namespace svxg

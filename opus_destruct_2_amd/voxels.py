@@ -78,7 +78,7 @@ def extract(vox, frag, lo, h, mats):
     zz, yy, xx = np.nonzero(solid)
     f = frag[zz, yy, xx]
     mid = (vox[zz, yy, xx] & 0x7F).astype(np.int64) - 1
-    m = rho[mid] * h ** 3
+    m = rho[mid] * (h * h * h)
     # (a voxel's centre is h times its integer coordinates, core/src/frag/fragments.cpp:315)
     cen = np.stack([xx + lo[0], yy + lo[1], zz + lo[2]], -1) * h
     mass = np.bincount(f, weights=m, minlength=nf)
@@ -176,6 +176,31 @@ def bond_sections(nodes, heads, faces, vox, lo, h, mats, damage=None, ctx=None):
                                                   len(heads), faces.ctypes.data, len(faces), ctypes.byref(box),
                                                   mats.ctypes.data, len(mats), out.ctypes.data), "svxg_bond_sections")
     return out
+
+
+def lattice_graph(vox, frag, lo, h, mats, damage=None, ctx=None):
+    """The whole graph on the device (svxg_lattice_graph): (nodes, heads, faces, bonds), the same
+    records as extract() followed by bond_sections(). Two calls: the counts, then the records."""
+    from . import stress
+    ctx = ctx or stress.default_context()
+    vox = np.ascontiguousarray(vox, np.uint8)
+    frag = np.ascontiguousarray(frag, np.int32)
+    damage = None if damage is None else np.ascontiguousarray(damage, np.uint8)
+    mats = np.ascontiguousarray(mats)
+    box = voxel_box(vox, lo, h, damage)
+    nfrag = int(frag.max()) + 1
+    nb, nf = ctypes.c_int32(0), ctypes.c_int32(0)
+    L = stress.lib()
+    stress._check(L.svxg_lattice_graph(ctx.h, ctypes.byref(box), frag.ctypes.data, nfrag, mats.ctypes.data, len(mats),
+                                       None, ctypes.byref(nb), ctypes.byref(nf), None, None, None), "svxg_lattice_graph")
+    nodes = np.zeros(nfrag, NODE)
+    heads = np.zeros(max(1, nb.value), SECTION_HEAD)
+    faces = np.zeros(max(1, nf.value), SECTION_FACE)
+    bonds = np.zeros(max(1, nb.value), BOND)
+    stress._check(L.svxg_lattice_graph(ctx.h, ctypes.byref(box), frag.ctypes.data, nfrag, mats.ctypes.data, len(mats),
+                                       nodes.ctypes.data, ctypes.byref(nb), ctypes.byref(nf), heads.ctypes.data,
+                                       faces.ctypes.data, bonds.ctypes.data), "svxg_lattice_graph")
+    return nodes, heads[:nb.value], faces[:nf.value], bonds[:nb.value]
 
 
 def gravity(nodes):

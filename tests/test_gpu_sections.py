@@ -51,3 +51,26 @@ def test_sections_edge_cases(mats, ctx):
     bad[0]["nfaces"] = len(faces) + 1
     with pytest.raises(stress.SvxgError):
         voxels.bond_sections(nodes, bad, faces, vox, lo, 0.25, mats, ctx=ctx)
+
+
+@pytest.mark.parametrize("shape,cell,voids,seed,mixed", [((16, 14, 12), 2, 0.1, 4, False),
+                                                         ((30, 24, 20), 3, 0.2, 5, False),
+                                                         ((9, 9, 40), 2, 0.0, 6, False),
+                                                         ((18, 18, 15), 3, 0.1, 7, True)])
+def test_lattice_graph_on_device_is_the_host_extraction(shape, cell, voids, seed, mixed, mats, ctx):
+    """svxg_lattice_graph (faces found, sorted and grouped, fragments summed, sections, all on the
+    device) gives the records of the host extraction followed by the oracle's sections, bit for bit.
+    (mixed: every voxel a random material, so fragments hold many; the commonest one names the node)"""
+    vox, frag, lo = voxels.voronoi_block(*shape, cell=cell, voids=voids, seed=seed)
+    rng = np.random.default_rng(seed)
+    if mixed:
+        solid = frag >= 0
+        vox[solid] = rng.integers(0, 12, int(solid.sum())) + 1
+    damage = np.where(rng.random(vox.shape) < 0.05, rng.integers(1, 256, vox.shape), 0).astype(np.uint8)
+    nodes, heads, faces = voxels.extract(vox, frag, lo, 0.25, mats)
+    bonds = oracle.bond_sections(nodes, heads, faces, vox, lo, 0.25, mats, damage=damage)
+    dn, dh, df, db = voxels.lattice_graph(vox, frag, lo, 0.25, mats, damage=damage, ctx=ctx)
+    assert dn.tobytes() == nodes.tobytes()
+    assert dh.tobytes() == heads.tobytes()
+    assert df.tobytes() == faces.tobytes()
+    assert db.tobytes() == bonds.tobytes()
