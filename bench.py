@@ -562,11 +562,16 @@ def main():
                     help="levels with a smoothed prolongator, from the finest (0: all; default: the library's)")
     ap.add_argument("--cheb-degree", type=int, default=None)
     ap.add_argument("--coarse-cheb-degree", type=int, default=None)
+    ap.add_argument("--amg", default="", metavar="KEY=VALUE,...",
+                    help="further multigrid options of every leg (svxg_amg_options / svxg_stress_options field names)")
     ap.add_argument("--mixed-fp64-rows", type=int, default=None,
                     help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
-    ap.add_argument("--only", choices=["batch", "serve", "city", "cascade", "graph"], default=None,
+    ap.add_argument("--step-graphs", type=int, default=None, choices=[0, 1, 2],
+                    help="--only legs: record tiled solve steps as HIP graphs (0 never, 1 large batches, 2 always; "
+                         "default: the library's)")
+    ap.add_argument("--only", choices=["head", "batch", "serve", "city", "cascade", "graph"], default=None,
                     help="profiling: run only this leg and print its record (not the benchmark line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
@@ -607,11 +612,21 @@ def main():
         AMG_KW["coarse_cheb_degree"] = args.coarse_cheb_degree
     if args.mixed_fp64_rows is not None:
         AMG_KW["mixed_fp64_rows"] = args.mixed_fp64_rows
+    for kv in filter(None, args.amg.split(",")):
+        k, v = kv.split("=")
+        AMG_KW[k.strip()] = float(v) if "." in v or "e" in v else int(v)
     ctx = stress.default_context()
     if args.resident_rows >= 0:
         ctx.set_resident_rows(args.resident_rows)
     if args.only is not None:
-        if args.only == "batch":
+        if args.step_graphs is not None:
+            ctx.set_step_graphs(args.step_graphs)
+        if args.only == "head":  # (the headline solves alone, with their live roofline)
+            prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
+            head = run_path(g, mats, prec, args.steps, args.warmup, ctx)
+            rec = {"solve_ms": round(head["solve_ms"], 3), "iters": head["iters"],
+                   "roofline": roofline(head, args.precision, (args.nx, args.ny, args.nz))}
+        elif args.only == "batch":
             rec = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
         elif args.only == "serve":
             rec = run_serving(mats, ctx, [int(x) for x in args.serve_lambdas.split(",") if x], args.serve_ticks,

@@ -97,7 +97,11 @@ typedef struct svxg_amg_options {
                                 coarse_scale), which keeps coarse operators sparse */
   int32_t coarse_cheb_degree;/* device: the smoother's degree below level 0 (default 1; 0: cheb_degree).
                                 A batch runs its first problem's degrees */
-  int32_t pad_;
+  int32_t lean_rows;         /* device: graphs with at least this many free nodes (default 16384; 0: none)
+                                take a lean hierarchy: unless smooth_levels says otherwise only the
+                                finest prolongator is smoothed, and coarsening stops at the first level
+                                the dense solve takes (at most 4 coarse_max nodes). Their coarse
+                                operators stay sparse and the cycle short, at the same iterations */
 } svxg_amg_options;
 
 /* StressOptions (stress.hpp:68-72). */

@@ -60,7 +60,7 @@ enum class FailMode : u8 { None = 0, Tension, Crush, Shear };
 
 struct StressOptions {
   f64 rtol = 2e-3;
-  svxg_amg_options amg{0.1, 0.6, 48, 1, 16, 1, 2, 1, 1, 0, 1, 0};
+  svxg_amg_options amg{0.1, 0.6, 48, 1, 16, 1, 2, 1, 1, 0, 1, 16384};
   i32 amg_min_nodes = 160;
   i32 precision = SVXG_FP64;
   i32 mixed_fp64_rows = 1 << 30;  // (device: see svxg_stress_options)
