@@ -449,7 +449,7 @@ int64_t svxg_problem::device_bytes() const {
                           &d_id_scale, &x, &r, &z, &p, &q, &b, &ix, &ir, &iz, &ip, &iq, &u_all})
     bytes += int64_t(d->held());
   for (const HostLevel& L : lv)
-    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.A16, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32,
+    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.A16, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32, &L.P16, &L.R16,
                             &L.Rrow, &L.Rfine, &L.Rent, &L.Minv, &L.x64[0], &L.x64[1], &L.b64,
                             &L.r64, &L.d64, &L.x32[0], &L.x32[1], &L.b32, &L.r32, &L.d32, &L.pos, &L.lam})
       bytes += int64_t(d->held());
@@ -506,6 +506,18 @@ struct Batch {
   }
 };
 
+// rows averaging more than 16 entries are served a wavefront each (S = 10 sub-groups)
+static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entries > 16 * rows; }
+
+// the sub-groups a row of a level's smoothing and residual launches: a wavefront (10) for long rows,
+// and for small levels with more than 8 blocks a row, whose launches are too few to hide the
+// latency of a lane's walk along its row; else six lanes (1)
+constexpr int kSmallLevelRows = 4096;
+static int smoother_sub(const HostLevel& H) {
+  const bool small = H.n > 0 && H.n < kSmallLevelRows && H.nnzb > 8 * int64_t(H.n);
+  return long_rows(H.nnzb, H.n) || small ? 10 : 1;
+}
+
 static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
   std::memset(&g, 0, sizeof g);
   g.nlev = P->jacobi_only ? 1 : int32_t(P->lv.size());
@@ -542,7 +554,7 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.Minv = H.Minv.as<double>();
     L.lmax = H.lmax;
     L.cheb_degree = P->opt.amg.cheb_degree;
-    L.sub = H.nnzb > 16 * int64_t(H.n) ? 10 : 1;
+    L.sub = smoother_sub(H);
     L.rsub = H.nc > 0 && H.np > 16 * int64_t(H.nc) ? 10 : 1;
     L.f64.A = H.A64.as<double>();
     L.f64.Dinv = H.D64.as<double>();
@@ -554,6 +566,8 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.f64.d = H.d64.as<double>();
     L.f32.A = H.A32.as<float>();
     L.A16 = H.A16.as<uint8_t>();
+    L.P16 = H.P16.as<uint8_t>();
+    L.R16 = H.R16.as<uint8_t>();
     L.f32.Dinv = H.D32.as<float>();
     L.f32.P = H.P32.as<float>();
     L.f32.x[0] = H.x32[0].as<float>();
@@ -563,9 +577,6 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.f32.d = H.d32.as<float>();
   }
 }
-
-// rows averaging more than 16 entries are served a wavefront each (S = 10 sub-groups)
-static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entries > 16 * rows; }
 
 static int prof_table(Batch& B, Arena* A, hipStream_t s);
 
@@ -639,7 +650,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
       svxg_problem* P = probs[g];
       if (!tiled[size_t(g)] || P->jacobi_only || l >= int(P->lv.size())) continue;
       const HostLevel& H = P->lv[size_t(l)];
-      const int vs = long_rows(H.nnzb, H.n) ? 1 : 0;
+      const int vs = smoother_sub(H) == 10 ? 1 : 0;
       if (!H.terminal) {
         add_tiles(T.ts[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
         add_tiles(T.tp, g, H.n, kTileRows);

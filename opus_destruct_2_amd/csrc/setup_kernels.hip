@@ -1511,6 +1511,26 @@ __global__ void __launch_bounds__(kWG) k_half_blocks(const float* __restrict__ A
   reinterpret_cast<float*>(h + 72)[1] = 0.0f;
 }
 
+// the same for the transposes of the blocks P[ent[e]] (a restriction entry's block), in entry order
+__global__ void __launch_bounds__(kWG) k_half_blocks_t(const float* __restrict__ P, const int32_t* __restrict__ ent,
+                                                       uint8_t* __restrict__ H, int64_t n) {
+  const int64_t e = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (e >= n) return;
+  const float* a = P + 36 * size_t(ent[e]);
+  float m = 0.0f;
+  for (int i = 0; i < 36; ++i) m = fmaxf(m, fabsf(a[i]));
+  const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+  uint8_t* h = H + kHalfBlock * e;
+  for (int i = 0; i < 36; i += 2) {
+    // (the transpose's entries (r, c), (r, c + 1) are P's (c, r), (c + 1, r))
+    const int r = i / 6, c = i % 6;
+    const __half2 v = __floats2half2_rn(a[6 * c + r] * inv, a[6 * (c + 1) + r] * inv);
+    *reinterpret_cast<__half2*>(h + 2 * i) = v;
+  }
+  reinterpret_cast<float*>(h + 72)[0] = m;
+  reinterpret_cast<float*>(h + 72)[1] = 0.0f;
+}
+
 __global__ void __launch_bounds__(kWG) k_f32_many(const BulkSeg* __restrict__ segs, int n) {
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const BulkSeg g = segs[k];
@@ -1718,6 +1738,10 @@ void launch_zero_many(hipStream_t s, const BulkSeg* segs, int n) {
 }
 void launch_half_blocks(hipStream_t s, const float* A, uint8_t* H, int64_t nnzb) {
   if (nnzb > 0) hipLaunchKernelGGL(k_half_blocks, dim3(unsigned((nnzb + kWG - 1) / kWG)), dim3(kWG), 0, s, A, H, nnzb);
+}
+
+void launch_half_blocks_t(hipStream_t s, const float* P, const int32_t* ent, uint8_t* H, int64_t n) {
+  if (n > 0) hipLaunchKernelGGL(k_half_blocks_t, dim3(unsigned((n + kWG - 1) / kWG)), dim3(kWG), 0, s, P, ent, H, n);
 }
 
 void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n) {

@@ -143,6 +143,28 @@ __device__ __forceinline__ float spmv_row_part_h(const int32_t* __restrict__ row
   return acc;
 }
 
+// row q of an FP16 block (three words at byte 12 q) and its scale (byte 72)
+__device__ __forceinline__ void half_row_load(const uint8_t* __restrict__ blk, int q, uint32_t w[3], float* sc) {
+  const uint32_t* rp = reinterpret_cast<const uint32_t*>(blk + 12 * q);
+  w[0] = ldg(rp);
+  w[1] = ldg(rp + 1);
+  w[2] = ldg(rp + 2);
+  *sc = ldg(reinterpret_cast<const float*>(blk + 72));
+}
+// sc (row . v), the row's six products summed left to right in FP32
+__device__ __forceinline__ float half_row_dot(const uint32_t w[3], float sc, const float v[6]) {
+  float a[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    __half2 h;
+    __builtin_memcpy(&h, &w[c], 4);
+    const float2 f = __half22float2(h);
+    a[2 * c] = f.x;
+    a[2 * c + 1] = f.y;
+  }
+  return sc * (a[0] * v[0] + a[1] * v[1] + a[2] * v[2] + a[3] * v[3] + a[4] * v[4] + a[5] * v[5]);
+}
+
 template <typename T, int S>
 __device__ __forceinline__ T spmv_row_part_v(const LevelDev& L, const T* __restrict__ A, const T* __restrict__ x,
                                              int row, int q, int sub) {
@@ -350,7 +372,31 @@ __device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl,
   const bool on = rl >= 0 && rl < tl.nrows;
   const int I = tl.row0 + (on ? rl : 0);
   T acc = T(0);
-  if (on) {
+  bool half = false;
+  if constexpr (std::is_same<T, float>::value) half = F.R16 != nullptr;
+  if (on && half) {
+    // (the FP16 transposed blocks in restriction order: row q of each, no Rent indirection)
+    constexpr int U = 4;
+    const int e0 = ldg(F.Rrow + I), e1 = ldg(F.Rrow + I + 1);
+    for (int e = e0 + sub; e < e1; e += U * S) {
+      int i[U], ee[U];
+      uint32_t w[U][3];
+      float sc[U], v[U][6];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ee[u] = min(e + u * S, e1 - 1);
+        i[u] = ldg(F.Rfine + ee[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        half_row_load(F.R16 + size_t(kHalfBlock) * size_t(ee[u]), q, w[u], &sc[u]);
+        load6(reinterpret_cast<const float*>(af.r) + 6 * size_t(i[u]), v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (e + u * S < e1) acc += T(half_row_dot(w[u], sc[u], v[u]));
+    }
+  } else if (on) {
     // (entries in groups of four, loads first: see spmv_row_part)
     constexpr int U = 4;
     const int e0 = ldg(F.Rrow + I), e1 = ldg(F.Rrow + I + 1);
@@ -405,6 +451,30 @@ __device__ __forceinline__ void prolong_tile(const GraphDev& g, const Tile& tl, 
   // (entries in groups of four, loads first: see spmv_row_part)
   constexpr int U = 4;
   const int e0 = ldg(F.Prow + i), e1 = ldg(F.Prow + i + 1);
+  bool half = false;
+  if constexpr (std::is_same<T, float>::value) half = F.P16 != nullptr;
+  if (half) {
+    for (int e = e0; e < e1; e += U) {
+      int ee[U], J[U];
+      uint32_t w[U][3];
+      float sc[U], v[U][6];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ee[u] = min(e + u, e1 - 1);
+        J[u] = ldg(F.Pcol + ee[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        half_row_load(F.P16 + size_t(kHalfBlock) * size_t(ee[u]), q, w[u], &sc[u]);
+        load6(reinterpret_cast<const float*>(xc) + 6 * size_t(J[u]), v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (e + u < e1) acc += T(half_row_dot(w[u], sc[u], v[u]));
+    }
+    af.x[buf][6 * size_t(i) + q] = acc;
+    return;
+  }
   for (int e = e0; e < e1; e += U) {
     int ee[U], J[U];
     T m[U][6], v[U][6];
@@ -441,37 +511,43 @@ __global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G,
 // x[1] = M^-1 b at the coarsest level of each listed graph (FP64 products): one workgroup a graph,
 // a wavefront a row at a time, lanes across the row
 // x = M^-1 b at a dense coarsest level, rows [r0, r0 + 64): M^-1 is symmetric, so row i is read as
-// column i and the 64 lanes' loads of one j are one contiguous run; the workgroup's four waves
-// split the j range, their partial sums meet in LDS in a fixed order
+// column i and the 64 lanes' loads of one j are one contiguous run; the j range is cut into
+// kDenseSpans spans, which the workgroup's NT / 64 waves share out, and the spans' partial sums meet
+// in LDS in span order (so any NT gives the same bits)
 constexpr int kDenseRows = 64;
-template <typename T>
+constexpr int kDenseSpans = 16;
+constexpr int kDenseSolveWG = 1024;  // (a wave a span: a 864-row inverse is 54 loads a lane, not 216)
+template <typename T, int NT>
 __device__ __forceinline__ void dense_rows(const LevelDev& L, const LevelArrays<T>& a, int r0, double* sb,
                                            double* part) {
   const int m = L.dense_m;
-  for (int j = threadIdx.x; j < m; j += kWG) sb[j] = double(a.b[j]);
+  for (int j = threadIdx.x; j < m; j += NT) sb[j] = double(a.b[j]);
   __syncthreads();
-  constexpr int nw = kWG / 64;
-  const int w = threadIdx.x >> 6, i = r0 + (threadIdx.x & 63);
-  const int span = (m + nw - 1) / nw, j0 = w * span, j1 = min(m, j0 + span);
-  double s = 0.0;
-  if (i < m) {
-    // (eight loads in flight, the sum in order)
-    const double* col = L.Minv + i;
-    int j = j0;
-    for (; j + 8 <= j1; j += 8) {
-      double v[8];
+  constexpr int nw = NT / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i = r0 + lane;
+  const int span = (m + kDenseSpans - 1) / kDenseSpans;
+  for (int sp = w; sp < kDenseSpans; sp += nw) {
+    const int j0 = sp * span, j1 = min(m, j0 + span);
+    double s = 0.0;
+    if (i < m) {
+      // (eight loads in flight, the sum in order)
+      const double* col = L.Minv + i;
+      int j = j0;
+      for (; j + 8 <= j1; j += 8) {
+        double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = ldg(col + size_t(j + u) * size_t(m));
+        for (int u = 0; u < 8; ++u) v[u] = ldg(col + size_t(j + u) * size_t(m));
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s += v[u] * sb[j + u];
+        for (int u = 0; u < 8; ++u) s += v[u] * sb[j + u];
+      }
+      for (; j < j1; ++j) s += ldg(col + size_t(j) * size_t(m)) * sb[j];
     }
-    for (; j < j1; ++j) s += ldg(col + size_t(j) * size_t(m)) * sb[j];
+    part[sp * 64 + lane] = s;
   }
-  part[threadIdx.x] = s;
   __syncthreads();
   if (w == 0 && i < m) {
-    double t = part[threadIdx.x];
-    for (int q = 1; q < nw; ++q) t += part[q * 64 + threadIdx.x];
+    double t = part[lane];
+    for (int q = 1; q < kDenseSpans; ++q) t += part[q * 64 + lane];
     a.x[1][i] = T(t);
   }
   __syncthreads();
@@ -479,22 +555,23 @@ __device__ __forceinline__ void dense_rows(const LevelDev& L, const LevelArrays<
 template <typename T>
 __device__ __forceinline__ void dense_solve_g(const GraphDev& g, int level) {
   __shared__ double sb[kMaxDenseM];
-  __shared__ double part[kWG];
+  __shared__ double part[kDenseSpans * 64];
   const LevelDev& L = g.lv[level];
-  for (int r0 = 0; r0 < L.dense_m; r0 += kDenseRows) dense_rows<T>(L, arrays<T>(L), r0, sb, part);
+  for (int r0 = 0; r0 < L.dense_m; r0 += kDenseRows) dense_rows<T, kWG>(L, arrays<T>(L), r0, sb, part);
 }
 // one workgroup a chunk of 64 rows (tile: graph, first row, rows)
 template <typename T>
-__global__ void __launch_bounds__(kWG) k_dense_solve(const GraphDev* __restrict__ G, const Tile* __restrict__ chunks,
-                                                     int count, int level, int inner) {
+__global__ void __launch_bounds__(kDenseSolveWG) k_dense_solve(const GraphDev* __restrict__ G,
+                                                               const Tile* __restrict__ chunks, int count, int level,
+                                                               int inner) {
   if (int(blockIdx.x) >= count) return;
   const Tile tl = chunks[blockIdx.x];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   __shared__ double sb[kMaxDenseM];
-  __shared__ double part[kWG];
+  __shared__ double part[kDenseSpans * 64];
   const LevelDev& L = g.lv[level];
-  dense_rows<T>(L, arrays<T>(L), tl.row0, sb, part);
+  dense_rows<T, kDenseSolveWG>(L, arrays<T>(L), tl.row0, sb, part);
 }
 
 // z = D^-1 r (block-Jacobi preconditioning of a graph without a hierarchy)
@@ -1319,7 +1396,7 @@ void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int nti
 template <typename T>
 void launch_dense_solve(hipStream_t s, const GraphDev* G, const Tile* chunks, int count, int level, int inner) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kWG), 0, s, G, chunks, count, level, inner);
+  hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kDenseSolveWG), 0, s, G, chunks, count, level, inner);
 }
 template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {

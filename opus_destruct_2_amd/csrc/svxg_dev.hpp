@@ -134,6 +134,8 @@ struct LevelDev {
   int32_t rsub;          // the same for its restriction (coarse rows of the next level)
   int32_t pad_;
   const uint8_t* A16;    // A in FP16 blocks scaled by their largest entry (kHalfBlock bytes a block), or null
+  const uint8_t* P16;    // P the same way, by prolongation entry (row-major blocks), or null
+  const uint8_t* R16;    // P's blocks transposed, by restriction entry (the order of Rfine / Rent), or null
   LevelArrays<double> f64;
   LevelArrays<float> f32;
 };

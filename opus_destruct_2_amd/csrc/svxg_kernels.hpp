@@ -257,6 +257,8 @@ void launch_zero_many(hipStream_t s, const BulkSeg* segs, int n);
 void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n);
 // FP16 block copies (kHalfBlock bytes each) of nnzb FP32 6x6 blocks
 void launch_half_blocks(hipStream_t s, const float* A, uint8_t* H, int64_t nnzb);
+// the FP16 copy of the transposes of the blocks P[ent[e]], e < n, in the order of e
+void launch_half_blocks_t(hipStream_t s, const float* P, const int32_t* ent, uint8_t* H, int64_t n);
 // each segment's count bytes copied from src to dst (both 16-byte aligned)
 void launch_copy_many(hipStream_t s, const BulkSeg* segs, int n);
 
