@@ -449,7 +449,7 @@ int64_t svxg_problem::device_bytes() const {
                           &d_id_scale, &x, &r, &z, &p, &q, &b, &ix, &ir, &iz, &ip, &iq, &u_all})
     bytes += int64_t(d->held());
   for (const HostLevel& L : lv)
-    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.A16, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32, &L.P16, &L.R16,
+    for (const DevBuf* d : {&L.rowptr, &L.col, &L.A64, &L.A32, &L.A16, &L.D64, &L.D32, &L.Prow, &L.Pcol, &L.P64, &L.P32, &L.P16, &L.R16, &L.D16,
                             &L.Rrow, &L.Rfine, &L.Rent, &L.Minv, &L.x64[0], &L.x64[1], &L.b64,
                             &L.r64, &L.d64, &L.x32[0], &L.x32[1], &L.b32, &L.r32, &L.d32, &L.pos, &L.lam})
       bytes += int64_t(d->held());
@@ -567,6 +567,7 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.f32.A = H.A32.as<float>();
     L.A16 = H.A16.as<uint8_t>();
     L.P16 = H.P16.as<uint8_t>();
+    L.D16 = H.D16.as<uint8_t>();
     L.R16 = H.R16.as<uint8_t>();
     L.f32.Dinv = H.D32.as<float>();
     L.f32.P = H.P32.as<float>();
@@ -690,8 +691,9 @@ static double prof_bytes(const svxg_problem* P, int kind) {
   const double nb = double(P->lv[0].nnzb), n = double(P->lv[0].n);
   // (the smoothing step of a level with FP16 copies reads kHalfBlock bytes a block)
   const double blk = (kind == 3 && P->lv[0].A16) ? double(kHalfBlock) : 36.0 * w;
+  const double dblk = (kind == 3 && P->lv[0].D16) ? double(kHalfBlock) : 36.0 * w;
   const double mat = nb * (blk + 4.0) + 4.0 * (n + 1.0);
-  return kind < 2 ? mat + 2.0 * 6.0 * n * w : mat + 36.0 * n * w + 5.0 * 6.0 * n * w;
+  return kind < 2 ? mat + 2.0 * 6.0 * n * w : mat + dblk * n + 5.0 * 6.0 * n * w;
 }
 
 // the batch's per-graph byte table (made when the batch is built with timing on)
@@ -970,9 +972,7 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64) 
   if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
   if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_ITER, part);
-  if (p64) launch_convert64(s, G, t0, B.nt0run, 1);
   vcycle<float>(B, s, SERVE_INNER, degree);
-  if (p64) launch_convert64(s, G, t0, B.nt0run, 0);
   if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
   if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);

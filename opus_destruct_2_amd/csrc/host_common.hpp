@@ -131,6 +131,7 @@ struct HostLevel {
   double lmax = 1.0;
   DevBuf rowptr, col, A64, A32, A16, D64, D32;
   DevBuf Prow, Pcol, P64, P32, Rrow, Rfine, Rent, Minv;
+  DevBuf D16;       // FP16 copy of the inverse diagonal blocks
   DevBuf P16, R16;  // FP16 copies of P by prolongation entry and of its transpose by restriction entry
   DevBuf x64[2], b64, r64, d64, x32[2], b32, r32, d32;
   DevBuf pos, lam;

@@ -304,7 +304,17 @@ __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl
   cheb_coeffs(L.lmax, step, c1, c2);
   T xo = T(0);
   if (on && sub == 0) {
-    const T z = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
+    T z;
+    bool half = false;
+    if constexpr (std::is_same<T, float>::value) half = L.D16 != nullptr;
+    if (half) {
+      uint32_t w[3];
+      float sc;
+      half_row_load(L.D16 + size_t(kHalfBlock) * size_t(row), q, w, &sc);
+      z = T(half_row_dot(w, sc, reinterpret_cast<const float*>(rv)));
+    } else {
+      z = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
+    }
     const T d = (step == 0) ? T(c2) * z : T(c1) * a.d[e] + T(c2) * z;
     a.d[e] = d;
     xo = xin + d;
@@ -647,6 +657,18 @@ __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G
   pcg_spmv_tile<T>(g, tl, inner, partial);
 }
 
+// FP64-product inner solves (mixed precision): the FP32 V-cycle's input r_i = (float) r is written
+// with r, and its output z_i is read as (double) z_i wherever z is, so that no conversion passes run
+template <typename T>
+__device__ __forceinline__ bool inner64_mode(int inner) {
+  return std::is_same<T, double>::value && (inner & SERVE_P64) != 0;
+}
+template <typename T>
+__device__ __forceinline__ T z_at(const GraphDev& g, const PcgVecs<T>& v, size_t e, int inner) {
+  if (inner64_mode<T>(inner)) return T(g.iz[e]);
+  return v.z[e];
+}
+
 // x += alpha p, r -= alpha q, partial r . r
 template <typename T>
 __device__ __forceinline__ void axpy_rr_tile(const GraphDev& g, const Tile& tl, int inner, double* __restrict__ partial) {
@@ -661,6 +683,7 @@ __device__ __forceinline__ void axpy_rr_tile(const GraphDev& g, const Tile& tl, 
     v.x[e] = T(double(v.x[e]) + alpha * double(v.p[e]));
     const T r = T(double(v.r[e]) - alpha * double(v.q[e]));
     v.r[e] = r;
+    if (inner64_mode<T>(inner)) g.ir[e] = float(r);
     s = double(r) * double(r);
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
@@ -686,7 +709,7 @@ __device__ __forceinline__ void update_p_tile(const GraphDev& g, const Tile& tl,
   lane_row(rl, q);
   if (rl < 0 || rl >= tl.nrows) return;
   const size_t e = 6 * size_t(tl.row0 + rl) + q;
-  v.p[e] = T(double(v.z[e]) + beta * double(v.p[e]));
+  v.p[e] = T(double(z_at<T>(g, v, e, inner)) + beta * double(v.p[e]));
 
 }
 template <typename T>
@@ -710,7 +733,7 @@ __device__ __forceinline__ void dot_rz_tile(const GraphDev& g, const Tile& tl, i
   double s = 0.0;
   if (on) {
     const size_t e = 6 * size_t(tl.row0 + rl) + q;
-    s = double(v.r[e]) * double(v.z[e]);
+    s = double(v.r[e]) * double(z_at<T>(g, v, e, inner));
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
 
@@ -847,7 +870,7 @@ __device__ __forceinline__ void start_p_tile(const GraphDev& g, const Tile& tl, 
   double s0 = 0.0, s1 = 0.0;
   if (on) {
     const size_t e = 6 * size_t(tl.row0 + rl) + q;
-    const T z = v.z[e], r = v.r[e];
+    const T z = z_at<T>(g, v, e, inner), r = v.r[e];
     v.p[e] = z;
     s0 = double(r) * double(z);
     s1 = double(r) * double(r);
@@ -1109,26 +1132,6 @@ __global__ void __launch_bounds__(kWG) k_inner_init(const GraphDev* __restrict__
   inner_init_tile(g, tl);
 }
 
-// the FP64-product inner solves' preconditioner input and output: r_i = (float) r, z = (double) z_i
-__device__ __forceinline__ void convert64_tile(const GraphDev& g, const Tile& tl, int to32) {
-  int rl, q;
-  lane_row(rl, q);
-  if (rl < 0 || rl >= tl.nrows) return;
-  const size_t e = 6 * size_t(tl.row0 + rl) + q;
-  if (to32) g.ir[e] = float(g.r[e]);
-  else g.z[e] = double(g.iz[e]);
-
-}
-__global__ void __launch_bounds__(kWG) k_convert64(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                   int ntiles, int to32) {
-  const int t = xcd_tile(ntiles);
-  if (t >= ntiles) return;
-  const Tile tl = tiles[t];
-  const GraphDev& g = G[tl.g];
-  if (g.resident || !live(g, to32 ? (SERVE_P64 | SERVE_IT) : SERVE_INNER64)) return;
-  convert64_tile(g, tl, to32);
-}
-
 // x += (double) x_i after an inner solve; the inner iterations count towards the solve's
 __device__ __forceinline__ void inner_apply_tile(const GraphDev& g, const Tile& tl) {
   // (an FP64-product inner solve updated x itself)
@@ -1310,9 +1313,7 @@ __global__ void __launch_bounds__(kWG) k_resident_steps(const GraphDev* __restri
     if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(axpy_rr_tile<double>(g, tl, SERVE_P64 | SERVE_IT, partial));
     finish_rn_g(g, k0, k1, SERVE_ITER, partial);
     __syncthreads();
-    if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(convert64_tile(g, tl, 1));
     if (live(g, SERVE_INNER)) vcycle_g<float>(g, gi, SERVE_INNER, degree);
-    if (p64 && live(g, SERVE_INNER64)) SVXG_T0_LOOP(convert64_tile(g, tl, 0));
     if (p32 && live(g, SERVE_P32 | SERVE_IT)) SVXG_T0_LOOP(dot_rz_tile<float>(g, tl, SERVE_P32 | SERVE_IT, partial));
     if (p64 && live(g, SERVE_P64 | SERVE_IT)) SVXG_T0_LOOP(dot_rz_tile<double>(g, tl, SERVE_P64 | SERVE_IT, partial));
     finish_beta_g(g, k0, k1, SERVE_ITER, partial);
@@ -1491,10 +1492,6 @@ void launch_prof_account(hipStream_t s, const GraphDev* G, int count, int inner,
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_inner_apply, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
-}
-void launch_convert64(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int to32) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_convert64, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, to32);
 }
 void launch_resident_steps(hipStream_t s, const GraphDev* G, const int32_t* gtile, const int32_t* graphs, int count,
                            int nsteps, int degree, int mixed, int p32, int p64, double* partial) {

@@ -136,6 +136,7 @@ struct LevelDev {
   const uint8_t* A16;    // A in FP16 blocks scaled by their largest entry (kHalfBlock bytes a block), or null
   const uint8_t* P16;    // P the same way, by prolongation entry (row-major blocks), or null
   const uint8_t* R16;    // P's blocks transposed, by restriction entry (the order of Rfine / Rent), or null
+  const uint8_t* D16;    // the smoother's inverse diagonal blocks the same way, or null
   LevelArrays<double> f64;
   LevelArrays<float> f32;
 };

@@ -66,7 +66,6 @@ void launch_resident_steps(hipStream_t s, const GraphDev* G, const int32_t* gtil
 void launch_resident_recheck(hipStream_t s, const GraphDev* G, const int32_t* gtile, const int32_t* graphs, int count,
                              double* partial);
 // FP64-product inner solves: r_i = (float) r (to32 = 1) or z = (double) z_i (to32 = 0)
-void launch_convert64(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int to32);
 // plain y = A x on one graph's level (FP64 or FP32 copy), tests and the roofline probe
 template <typename T>
 void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col, const T* A, const T* x, T* y, int n);
