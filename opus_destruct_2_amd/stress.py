@@ -279,10 +279,18 @@ class StressProblem:
     def bond_load(self, b, u=None):
         return self.bond_loads(u)[b]
 
-    def bond_checks(self, u=None, fragility=1.0):
-        """(phi, mode) of every bond: bond_utilization after bond_load (stress.cpp:14-57)."""
-        phi = np.zeros(len(self._bonds))
-        mode = np.zeros(len(self._bonds), np.uint8)
+    def bond_checks(self, u=None, fragility=1.0, out=None):
+        """(phi, mode) of every bond: bond_utilization after bond_load (stress.cpp:14-57). out: a
+        (phi float64, mode uint8) pair of contiguous arrays of the bonds' length to write into (a
+        caller checking every tick keeps its arrays)."""
+        if out is None:
+            phi = np.zeros(len(self._bonds))
+            mode = np.zeros(len(self._bonds), np.uint8)
+        else:
+            phi, mode = out
+            if (phi.dtype != np.float64 or mode.dtype != np.uint8 or phi.shape != (len(self._bonds),)
+                    or mode.shape != (len(self._bonds),) or not phi.flags.c_contiguous or not mode.flags.c_contiguous):
+                raise ValueError("bond_checks: out must be contiguous float64 and uint8 arrays of the bonds' length")
         uu = None if u is None else np.ascontiguousarray(u, np.float64)
         _check(lib().svxg_problem_bond_checks(self.h, None if uu is None else uu.ctypes.data, float(fragility),
                                               phi.ctypes.data, mode.ctypes.data), "bond_checks")

@@ -61,8 +61,15 @@ def test_batch_matches_oracle(prec, mats, ctx):
         scale = np.abs(uo).max()
         assert np.abs(us[k] - uo).max() <= tol * scale, (SIZES[k], np.abs(us[k] - uo).max() / scale)
         phi_o, _ = o.bond_checks(uo)
-        phi_d, _ = probs[k].bond_checks(None)
+        phi_d, mode_d = probs[k].bond_checks(None)
         assert np.all(np.abs(phi_d - phi_o) <= tol * np.maximum(1.0, np.abs(phi_o)))
+        # (the caller's arrays, written over: the same bits)
+        out = (np.full(len(phi_d), np.nan), np.full(len(phi_d), 7, np.uint8))
+        phi_w, mode_w = probs[k].bond_checks(None, out=out)
+        assert phi_w is out[0] and np.array_equal(phi_w.view(np.int64), phi_d.view(np.int64))
+        assert np.array_equal(mode_w, mode_d)
+        with pytest.raises(ValueError):
+            probs[k].bond_checks(None, out=(np.zeros(len(phi_d), np.float32), mode_w))
 
 
 def test_batch_warm_start_and_resident_loads(mats, ctx):
