@@ -486,10 +486,10 @@ struct Batch {
   DevBuf partial;
   DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
   bool any_amg = false;
-  // one tiled step recorded as a HIP graph, by (mixed, FP32 products, FP64 products): the launches'
+  // one tiled step recorded as a HIP graph, by (inner-only, mixed, FP32 products, FP64 products): the launches'
   // arguments are this batch's tables, fixed for its life; what a step does to each graph is read
   // by the kernels from the graph's state, so every replay is the launches made again
-  hipGraphExec_t step_graph[8] = {};
+  hipGraphExec_t step_graph[16] = {};
   bool graphs_failed = false;
   int windows = 0;  // windows run (a batch records its graph from its third window: short solves
                     // and short-lived serving batches never pay for a recording)
@@ -897,8 +897,8 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
 }
 
 // a window of w steps: the resident graphs' in one launch, the tiled graphs' step by step
-static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64);
-static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, bool p64) {
+static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, bool lean);
+static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, bool p64, bool lean = false) {
   const int degree = batch_degree(B.probs[0]);
   if (B.nres)
     launch_resident_steps(s, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.res.as<int32_t>(), B.nres, w, degree,
@@ -907,12 +907,12 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
   const svxg_ctx* c = B.probs[0]->ctx;
   const bool record = c->step_graphs == 2 || (c->step_graphs == 1 && B.nt0run >= Batch::kGraphMinTiles && ++B.windows > 2);
   if (record && !c->profiling && !B.graphs_failed) {
-    hipGraphExec_t& ex = B.step_graph[(mixed ? 4 : 0) + (p32 ? 2 : 0) + (p64 ? 1 : 0)];
+    hipGraphExec_t& ex = B.step_graph[(lean ? 8 : 0) + (mixed ? 4 : 0) + (p32 ? 2 : 0) + (p64 ? 1 : 0)];
     if (!ex) {
       // (recorded on first use; a failed recording leaves this batch to launch kernel by kernel)
       hipGraph_t gr = nullptr;
       if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-        if (mixed) mixed_step(B, s, degree, p32, p64);
+        if (mixed) mixed_step(B, s, degree, p32, p64, lean);
         else fp64_step(B, s, degree);
         if (hipStreamEndCapture(s, &gr) != hipSuccess) gr = nullptr;
       }
@@ -929,7 +929,7 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
     }
   }
   for (int k = 0; k < w; ++k) {
-    if (mixed) mixed_step(B, s, degree, p32, p64);
+    if (mixed) mixed_step(B, s, degree, p32, p64, lean);
     else fp64_step(B, s, degree);
   }
 }
@@ -954,15 +954,20 @@ static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
 // residual and check (PH_OUTER), an inner solve's start (PH_START) or iteration (PH_INNER), the
 // correction x += d (PH_APPLY). The launches come in an order in which a graph moves on at most one
 // phase per kernel and never meets a launch of its new phase in the same step before that phase's
-// own sequence begins. p32 / p64: launch the FP32- / FP64-product kernels.
-static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64) {
+// own sequence begins. p32 / p64: launch the FP32- / FP64-product kernels. lean: the inner
+// iteration's launches only (no refinement residual, no inner start): for windows that begin with
+// every running graph iterating an inner solve; a graph whose inner solve ends within the window
+// applies its correction and then waits, untouched, for the next window's full steps.
+static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, bool lean) {
   const GraphDev* G = B.dG.as<GraphDev>();
   const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
-  launch_outer_residual(s, G, t0, B.nt0run, part);
-  launch_finish(s, FinishKind::Outer, G, gt, B.G, SERVE_REFINE, part);
-  launch_inner_init(s, G, t0, B.nt0run);
+  if (!lean) {
+    launch_outer_residual(s, G, t0, B.nt0run, part);
+    launch_finish(s, FinishKind::Outer, G, gt, B.G, SERVE_REFINE, part);
+    launch_inner_init(s, G, t0, B.nt0run);
+  }
   {
     ProfScope ps(B, 1, s, SERVE_P32 | SERVE_IT);
     if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
@@ -978,9 +983,11 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64) 
   launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);
   if (p32) launch_update_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT);
   if (p64) launch_update_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT);
-  if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part);
-  if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
-  launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
+  if (!lean) {
+    if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part);
+    if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
+    launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
+  }
   launch_inner_apply(s, G, t0, B.nt0run);
   launch_inner_close(s, G, B.G);
 }
@@ -1009,7 +1016,10 @@ static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
     if (!any_running(B, false)) break;
     const int w = plan.next(B, true);
     mixed_window(B, s, p32, p64);
-    run_window(B, s, w, true, p32, p64);
+    bool lean = true;  // (every running graph iterates an inner solve)
+    for (const PcgState& st : B.hstate)
+      if (st.state == ST_RUNNING && st.phase != PH_INNER) lean = false;
+    run_window(B, s, w, true, p32, p64, lean);
   }
   SVXG_HIP(hipGetLastError());
   return SVXG_OK;
