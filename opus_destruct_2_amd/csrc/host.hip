@@ -988,7 +988,7 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
     if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
     launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
   }
-  launch_inner_apply(s, G, t0, B.nt0run);
+  if (p32) launch_inner_apply(s, G, t0, B.nt0run);  // (FP64-product inner solves update x themselves)
   launch_inner_close(s, G, B.G);
 }
 
