@@ -709,9 +709,10 @@ static int prof_table(Batch& B, Arena* A, hipStream_t s) {
   return upload(B.prof_gb, A, gb, s);
 }
 
-// The launch between construction and destruction is timed with an event pair; right after it a
-// one-workgroup kernel adds the bytes of the graphs that launch served (live for `inner`) to the
-// context's device counter, so launches issued after a graph stopped add time and no bytes.
+// The launch between construction and destruction is timed with an event pair; it is given pa(),
+// with which the workgroups of each served graph's first tile add that graph's bytes (live for
+// `inner`) to the context's device counter, so launches issued after a graph stopped add time and
+// no bytes.
 struct ProfScope {
   svxg_ctx* c;
   const Batch& B;
@@ -733,11 +734,12 @@ struct ProfScope {
     P.pending.push_back(ev);
     e1 = ev.second;
   }
+  ProfAcc pa() const {
+    if (!e1) return {};
+    return {B.prof_gb.as<double>() + size_t(kind) * size_t(B.G), c->prof_acc.as<double>() + kind};
+  }
   ~ProfScope() {
-    if (!e1) return;
-    (void)hipEventRecord(e1, s);
-    launch_prof_account(s, B.dG.as<GraphDev>(), B.G, inner, B.prof_gb.as<double>() + size_t(kind) * size_t(B.G),
-                        c->prof_acc.as<double>() + kind);
+    if (e1) (void)hipEventRecord(e1, s);
   }
 };
 
@@ -783,7 +785,7 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
       for (int st = 0; st < dg; ++st) {
         ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
         launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr,
-                            SV[v]);
+                            SV[v], ps.pa());
       }
       launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (dg - 1) & 1, inner, SV[v]);
     }
@@ -806,7 +808,8 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
       for (int v = 0; v < 2; ++v) {
         if (!V.nts[v]) continue;
         ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
-        launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, SV[v]);
+        launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, SV[v],
+                            ps.pa());
       }
     }
   }
@@ -885,7 +888,7 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
   double* part = B.partial.as<double>();
   {
     ProfScope ps(B, 0, s, SERVE_OUTER);
-    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
+    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa());
   }
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
   launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
@@ -969,8 +972,8 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
     launch_inner_init(s, G, t0, B.nt0run);
   }
   {
-    ProfScope ps(B, 1, s, SERVE_P32 | SERVE_IT);
-    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
+    ProfScope ps(B, p32 ? 1 : -1, s, SERVE_P32 | SERVE_IT);
+    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa());
   }
   if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_ITER, part);

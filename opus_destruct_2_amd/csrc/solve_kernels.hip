@@ -258,20 +258,6 @@ __device__ __forceinline__ double graph_sum(int k0, int k1, const double* partia
 
 }  // namespace
 
-// ---- kernel timing -----------------------------------------------------------------------------
-
-// *acc += the algorithmic bytes of the graphs a launch served (those live for `inner`), so that
-// launches left running after a graph stopped count their time and none of its bytes
-__global__ void __launch_bounds__(kWG) k_prof_account(const GraphDev* __restrict__ G, int count, int inner,
-                                                      const double* __restrict__ gbytes, double* __restrict__ acc) {
-  __shared__ double lds[4];
-  double v = 0.0;
-  for (int g = threadIdx.x; g < count; g += kWG)
-    if (!G[g].resident && live(G[g], inner)) v += gbytes[g];
-  const double s = block_sum(v, lds);
-  if (threadIdx.x == 0) *acc += s;
-}
-
 // ---- V-cycle kernels ---------------------------------------------------------------------------
 
 // One Chebyshev step on `level`: r = b - A x_in (b when fresh: x_in = 0), d = c1 d + c2 D^-1 r,
@@ -326,12 +312,13 @@ __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl
 template <typename T, int S, bool FRESH>
 __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
                                                    int ntiles, int level, int step, int in_buf, int out_buf,
-                                                   int inner, double* __restrict__ partial) {
+                                                   int inner, double* __restrict__ partial, ProfAcc pa) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
+  if (pa.acc && tl.row0 == 0 && threadIdx.x == 0) atomicAdd(pa.acc, pa.gb[tl.g]);
   cheb_step_tile<T, S, FRESH>(g, tl, level, step, in_buf, out_buf, inner, partial);
 }
 
@@ -648,12 +635,13 @@ __device__ __forceinline__ void pcg_spmv_tile(const GraphDev& g, const Tile& tl,
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int inner, double* __restrict__ partial) {
+                                                  int ntiles, int inner, double* __restrict__ partial, ProfAcc pa) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
+  if (pa.acc && tl.row0 == 0 && threadIdx.x == 0) atomicAdd(pa.acc, pa.gb[tl.g]);
   pcg_spmv_tile<T>(g, tl, inner, partial);
 }
 
@@ -1355,21 +1343,21 @@ int grid_for(int ntiles) {
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
-                      int out_buf, int fresh, int inner, double* partial, int S) {
+                      int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa) {
   if (ntiles <= 0) return;
   const dim3 grid(grid_for(ntiles)), wg(kWG);
   if (S == 10 && fresh)
     hipLaunchKernelGGL((k_cheb_step<T, 10, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
-                       partial);
+                       partial, pa);
   else if (S == 10)
     hipLaunchKernelGGL((k_cheb_step<T, 10, false>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf,
-                       inner, partial);
+                       inner, partial, pa);
   else if (fresh)
     hipLaunchKernelGGL((k_cheb_step<T, 1, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
-                       partial);
+                       partial, pa);
   else
     hipLaunchKernelGGL((k_cheb_step<T, 1, false>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf,
-                       inner, partial);
+                       inner, partial, pa);
 }
 template <typename T>
 void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
@@ -1421,9 +1409,10 @@ void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col,
     hipLaunchKernelGGL(k_spmv_plain<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, rowptr, col, A, x, y, n);
 }
 template <typename T>
-void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
+                     ProfAcc pa) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa);
 }
 template <typename T>
 void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
@@ -1486,9 +1475,6 @@ void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int 
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_inner_init, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
 }
-void launch_prof_account(hipStream_t s, const GraphDev* G, int count, int inner, const double* gbytes, double* acc) {
-  if (count > 0) hipLaunchKernelGGL(k_prof_account, dim3(1), dim3(kWG), 0, s, G, count, inner, gbytes, acc);
-}
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_inner_apply, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles);
@@ -1510,7 +1496,7 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
 
 #define SVXG_INST(T)                                                                                                \
   template void launch_cheb_step<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int, int, int,   \
-                                    double*, int);                                                                  \
+                                    double*, int, ProfAcc);                                                         \
   template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int);              \
   template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
   template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
@@ -1519,7 +1505,7 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_dot_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_spmv_plain<T>(hipStream_t, const int32_t*, const int32_t*, const T*, const T*, T*, int);      \
-  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                   \
+  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc);          \
   template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                     \
   template void launch_update_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                             \
   template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);

@@ -14,10 +14,16 @@ enum class FinishKind { Start, Alpha, Rn, Beta, Outer };
 int grid_for(int ntiles);
 
 // ---- solve ---------------------------------------------------------------------------------------
+// timing mode: the workgroup of each served graph's first tile adds that graph's algorithmic bytes
+// (gb[g]) to *acc, so that launches left running after a graph stopped count time and no bytes
+struct ProfAcc {
+  const double* gb = nullptr;
+  double* acc = nullptr;
+};
 // S: six-lane sub-groups a row (1: 10 rows a wavefront; 10: one row a wavefront, for long rows)
 template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
-                      int out_buf, int fresh, int inner, double* partial, int S);
+                      int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa = {});
 template <typename T>
 void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
                      int S);
@@ -30,7 +36,8 @@ void launch_dense_solve(hipStream_t s, const GraphDev* G, const Tile* chunks, in
 template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
 template <typename T>
-void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
+void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
+                     ProfAcc pa = {});
 template <typename T>
 void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 template <typename T>
@@ -55,7 +62,6 @@ void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, 
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 // (kernel timing) acc += the bytes (gbytes per graph) of the graphs live for `inner`, read before the
 // next launch changes their states
-void launch_prof_account(hipStream_t s, const GraphDev* G, int count, int inner, const double* gbytes, double* acc);
 void launch_inner_apply(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_inner_close(hipStream_t s, const GraphDev* G, int count);
 // nsteps whole steps (a solve start first, for graphs just admitted) of each listed resident graph,
