@@ -46,10 +46,12 @@ def cases():
 CASES = cases()
 IDS = [c[0] for c in CASES]
 # the solve's precision: plain FP64; mixed (FP64 K products, FP32 V-cycle); mixed with FP32 K
-# products too, inside FP64 iterative refinement
+# products too, inside FP64 iterative refinement; mixed with the FP16 V-cycle copies, and with them
+# and the lean hierarchy (the settings of graphs of 16384+ free nodes, forced on these); FP64 lean
 PRECISIONS = [(SVXG_FP64, {}), (SVXG_MIXED, {}), (SVXG_MIXED, {"mixed_fp64_rows": 0}),
-              (SVXG_MIXED, {"vcycle_fp16_rows": 0})]
-PREC_IDS = ["fp64", "mixed", "mixed_ir", "mixed_vc16"]
+              (SVXG_MIXED, {"vcycle_fp16_rows": 0}), (SVXG_MIXED, {"vcycle_fp16_rows": 0, "lean_rows": 1}),
+              (SVXG_FP64, {"lean_rows": 1})]
+PREC_IDS = ["fp64", "mixed", "mixed_ir", "mixed_vc16", "mixed_vc16_lean", "fp64_lean"]
 
 
 def failed_sets_agree(phi, phi_ref, band=1e-4):
