@@ -15,7 +15,9 @@ value: solved DOFs / (device time of the solve + wall time of the checks), input
 roofline: the dominant kernel's algorithmic bytes per launch / its average launch time, measured
 live with HIP events on the library's stream during the timed steps.
 cpu_baseline: the reference's own solver (compiled from its sources into oracle/_ref) on the same
-workload, one assemble + solve, all host threads (up to 16).
+workload: the median of --cpu-runs runs of assemble + cold solve on every host thread of this
+process's share (setup and solve timed apart), with a one-thread figure beside it; cpu_baselines
+holds the same for the tower (C1) and the batched leg's plain sample (the piece path).
 
   python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu] [--gpus 1]
   python bench.py --only batch|serve|city|cascade|graph ...   (one leg alone, for profiling)
@@ -37,6 +39,9 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0, FP16 block copies)", 64, 64, 48): 117.7e6,
                     ("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 202.7e6}
 METRIC = "PCG-solved DOFs/s at 1e-8 residual, batched vs plain; p50/p99 tick latency"
+PRECISION_MIXED = ("FP64 PCG (FP64 K products, dots and vector updates) around an FP32-arithmetic V-cycle with an "
+                   "FP64 dense coarsest solve")
+PRECISION_FP64 = "FP64 PCG around an FP64 V-cycle (every product, smoother and vector in FP64)"
 
 
 def log(*a):
@@ -114,40 +119,123 @@ def copy_bandwidth(mib=1024):
     return json.loads(r.stdout.strip().splitlines()[-1]).get("copy_gbs")
 
 
-def cpu_baseline(g, mats):
-    """The reference's own solve (oracle/_ref: its sources compiled here) on the same workload."""
+def cpu_share():
+    """(threads, facts): the host threads a CPU baseline uses (the CPUs this process may run on, capped
+    by the cgroup's CPU quota and OMP_NUM_THREADS where either is set, as on a machine whose GPU owns a
+    share of the host), with the CPU model, nproc and that share."""
+    nproc = os.cpu_count() or 1
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            share = min(share, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return share, {"cpu_model": model, "nproc": nproc, "threads_all": share}
+
+
+def cpu_solve(graphs, mats, threads, runs, opts=None, keep=False):
+    """The reference's own solve (oracle/_ref, compiled from its sources; the oracle's restatement,
+    one thread, where _ref is absent) of `graphs` one after another, to 1e-8: `runs` runs of assemble
+    + cold solve; medians of the runs' setup and solve seconds (summed over the graphs)."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
     kind = "reference" if oracle.ref_available() else "port"
     if kind == "reference":
         oracle.set_ref_threads(threads)
     else:
         threads = 1
-    p = oracle.CpuProblem(g.nodes, g.bonds, mats, "ref" if kind == "reference" else "oracle")
-    t = time.perf_counter()
-    assert p.assemble(oracle.reference_options())
-    setup_s = time.perf_counter() - t
-    t = time.perf_counter()
-    u, r = p.solve(g.f, rtol=1e-8, maxit=5000)
-    solve_s = time.perf_counter() - t
-    phi, mode = p.bond_checks(u)
-    return {"value": round(g.dofs / solve_s, 1), "unit": "DOF/s", "cores": threads, "kind": kind,
-            "sample": f"one solve of the same block to 1e-8 ({r['iters']} iterations, {solve_s:.2f} s; "
-                      f"assemble + multigrid build {setup_s:.2f} s)",
-            "u": u, "phi": phi, "setup_s": setup_s, "solve_s": solve_s, "iters": r["iters"]}
+    setups, solves, iters, out = [], [], [], []
+    for run in range(runs):
+        su = so = 0.0
+        its = []
+        for g in graphs:
+            p = oracle.CpuProblem(g.nodes, g.bonds, mats, "ref" if kind == "reference" else "oracle")
+            t = time.perf_counter()
+            assert p.assemble(opts or oracle.reference_options())
+            t1 = time.perf_counter()
+            u, r = p.solve(g.f, rtol=1e-8, maxit=5000)
+            t2 = time.perf_counter()
+            assert r["converged"], r
+            su += t1 - t
+            so += t2 - t1
+            its.append(r["iters"])
+            if keep and run == runs - 1:
+                phi, _ = p.bond_checks(u)
+                out.append((u, phi))
+        setups.append(su)
+        solves.append(so)
+        iters = its
+    dofs = sum(g.dofs for g in graphs)
+    solve_s = float(np.median(solves))
+    return {"kind": kind, "threads": threads, "runs": runs, "setup_s": round(float(np.median(setups)), 4),
+            "solve_s": round(solve_s, 4), "dof_per_s": round(dofs / solve_s, 1),
+            "iters": int(max(iters)), "solutions": out}
 
 
-def run_batch(mats, ctx, count, steps, sample_plain):
-    """Batched uneven graphs: `count` pieces (log-normal sizes, median 200, 16..20000 nodes; even ones
-    on supports, odd ones pinned), one batch, mixed precision to 1e-8; against the plain FP64 path
-    solving graphs one at a time (a sample)."""
+def compare(u, phi, u_ref, phi_ref, band=1e-4):
+    """Errors of one solution against a reference one: u relative to max |u_ref|, phi relative to
+    max(1, |phi_ref|), and the failed bonds {phi >= 1}, which must agree except within `band` of 1."""
+    a, b = phi >= 1.0, phi_ref >= 1.0
+    near = (np.abs(phi - 1.0) <= band) | (np.abs(phi_ref - 1.0) <= band)
+    bad = int(((a != b) & ~near).sum())
+    return {"u_rel": float(np.abs(u - u_ref).max() / max(np.abs(u_ref).max(), 1e-300)),
+            "phi_rel": float((np.abs(phi - phi_ref) / np.maximum(1.0, np.abs(phi_ref))).max()),
+            "failed": int(a.sum()), "failed_ref": int(b.sum()), "failed_sets_agree": bad == 0,
+            "failed_disagreeing_outside_1e-4": bad}
+
+
+def merge_compare(recs):
+    """compare() records of a sample merged: the largest errors, the failed counts summed."""
+    if not recs:
+        return None
+    return {"u_rel": max(r["u_rel"] for r in recs), "phi_rel": max(r["phi_rel"] for r in recs),
+            "failed": sum(r["failed"] for r in recs), "failed_ref": sum(r["failed_ref"] for r in recs),
+            "failed_sets_agree": all(r["failed_sets_agree"] for r in recs),
+            "failed_disagreeing_outside_1e-4": sum(r["failed_disagreeing_outside_1e-4"] for r in recs),
+            "graphs": len(recs)}
+
+
+def cpu_baseline(name, graphs, mats, runs_all, runs_one, opts=None, keep=False):
+    """BASELINE's CPU protocol on one workload: the reference's solve with every host thread of this
+    process's share and with one thread, medians of the runs, setup and solve apart."""
+    threads, facts = cpu_share()
+    t = time.perf_counter()
+    allc = cpu_solve(graphs, mats, threads, runs_all, opts, keep=keep)
+    one = cpu_solve(graphs, mats, 1, runs_one, opts) if runs_one > 0 else None
+    sols = allc.pop("solutions")
+    if one is not None:
+        one.pop("solutions")
+    rec = {"workload": name, "graphs": len(graphs), "dofs": int(sum(g.dofs for g in graphs)), **facts,
+           "all_threads": allc, "one_thread": one, "wall_s": round(time.perf_counter() - t, 1)}
+    log(f"  cpu baseline {name}: {rec}")
+    return rec, sols
+
+
+def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)):
+    """Batched uneven graphs: `count` pieces (log-normal sizes, median 200, spread sigma, 16..20000
+    nodes; even ones on supports, odd ones pinned), one batch, mixed precision to 1e-8; against the
+    plain FP64 path solving graphs one at a time (a sample). On the sample: the batch's answers
+    against the plain path's and, with cpu_runs, against the reference's CPU solve of the same pieces
+    (the piece path: multigrid at every size), which is also timed."""
+    import oracle
     from opus_destruct_2_amd import gen, stress
     from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
-    sizes = gen.piece_sizes(count, seed=3)
+    sizes = gen.piece_sizes(count, sigma=sigma, seed=3)
     gs = [gen.piece(int(n), pinned=bool(k & 1), seed=5000 + k) for k, n in enumerate(sizes)]
     dofs = sum(g.dofs for g in gs)
     log(f"batched: {count} graphs, {sum(g.n for g in gs)} nodes, {dofs} DOFs (sizes {sizes.min()}..{sizes.max()}, "
-        f"median {int(np.median(sizes))})")
+        f"median {int(np.median(sizes))}, sigma {sigma})")
     opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0, **AMG_KW)
     probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
     t = time.perf_counter()
@@ -166,10 +254,13 @@ def run_batch(mats, ctx, count, steps, sample_plain):
         log(f"  batch step {s_}: {dev[-1]:.2f} ms device ({wall[-1]:.2f} wall), iters max "
             f"{max(r['iters'] for r in rs)} median {int(np.median([r['iters'] for r in rs]))}")
     solve_ms = float(np.median(dev))
-    # the plain path: FP64, one graph at a time
+    fp16 = sum(1 for p in probs if p.info()["vcycle_fp16"])
+    lean = sum(1 for p in probs if p.info()["lean"])
+    # the plain path: FP64, one graph at a time; the batch's answers on the same graphs against its
     pick = list(range(0, count, max(1, count // sample_plain)))[:sample_plain]
     pdofs, pms = 0, 0.0
     popt = stress.default_options(precision=SVXG_FP64, amg_min_nodes=0, **AMG_KW)
+    plain_sol, vs_plain = [], []
     for k in pick:
         g = gs[k]
         p = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=popt)
@@ -179,17 +270,34 @@ def run_batch(mats, ctx, count, steps, sample_plain):
         assert r["converged"]
         pdofs += g.dofs
         pms += p.last_timing()["solve_ms"]
+        phi, _ = p.bond_checks(u)
+        phib, _ = probs[k].bond_checks(us[k])
+        plain_sol.append((u, phi, phib))
+        vs_plain.append(compare(us[k], phib, u, phi))
         p.close()
     plain = pdofs / (pms / 1e3)
     batched = dofs / (solve_ms / 1e3)
     for p in probs:
         p.close()
-    return {"graphs": count, "dofs": dofs, "value": round(batched, 1), "unit": "DOF/s", "solve_ms": round(solve_ms, 3),
-            "wall_ms_pcie_inclusive": round(float(np.median(wall)), 3), "setup_ms": round(setup_ms, 1),
-            "precision": "mixed", "iters_max": max(r["iters"] for r in rs),
-            "plain_fp64_single_graph": {"value": round(plain, 1), "sample_graphs": len(pick), "sample_dofs": pdofs,
-                                        "solve_ms_sum": round(pms, 3)},
-            "batched_vs_plain": round(batched / plain, 2)}
+    rec = {"graphs": count, "dofs": dofs, "sizes": [int(sizes.min()), int(np.median(sizes)), int(sizes.max())],
+           "sigma": sigma, "graphs_fp16_vcycle": fp16, "graphs_lean": lean,
+           "value": round(batched, 1), "unit": "DOF/s", "solve_ms": round(solve_ms, 3),
+           "wall_ms_pcie_inclusive": round(float(np.median(wall)), 3), "setup_ms": round(setup_ms, 1),
+           "precision": "mixed", "iters_max": max(r["iters"] for r in rs),
+           "plain_fp64_single_graph": {"value": round(plain, 1), "sample_graphs": len(pick), "sample_dofs": pdofs,
+                                       "solve_ms_sum": round(pms, 3)},
+           "batched_vs_plain": round(batched / plain, 2),
+           "batched_mixed_vs_plain_fp64": merge_compare(vs_plain)}
+    cpu = None
+    if cpu_runs[0] > 0:
+        sample = [gs[k] for k in pick]
+        cpu, sols = cpu_baseline("batched pieces: the plain sample", sample, mats, cpu_runs[0], cpu_runs[1],
+                                 oracle.reference_options(amg_min_nodes=0), keep=True)
+        rec["error_vs_cpu_reference"] = {
+            "plain_fp64": merge_compare([compare(pu, pp, cu, cp) for (pu, pp, _), (cu, cp) in zip(plain_sol, sols)]),
+            "batched_mixed": merge_compare([compare(us[k], pb, cu, cp)
+                                            for k, (_, _, pb), (cu, cp) in zip(pick, plain_sol, sols)])}
+    return rec, cpu
 
 
 def run_cascade(mats, ctx, g, target, cpu=None):
@@ -541,11 +649,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: runs with every host thread (median)")
+    ap.add_argument("--cpu-runs-one-thread", type=int, default=1,
+                    help="CPU baseline of the large block with one thread: runs (median; the smaller workloads "
+                         "always take --cpu-runs)")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=48)
     ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=128)
+    ap.add_argument("--batch-sigma", type=float, default=0.75, help="log-normal spread of the batched leg's sizes")
     ap.add_argument("--serve-lambdas", default="50,200,800", help="arrivals per tick of the serving leg ('' skips)")
     ap.add_argument("--serve-ticks", type=int, default=90)
     ap.add_argument("--serve-warmup", type=int, default=30)
@@ -572,7 +685,7 @@ def main():
     ap.add_argument("--step-graphs", type=int, default=None, choices=[0, 1, 2],
                     help="--only legs: record tiled solve steps as HIP graphs (0 never, 1 large batches, 2 always; "
                          "default: the library's)")
-    ap.add_argument("--only", choices=["head", "batch", "serve", "city", "cascade", "graph"], default=None,
+    ap.add_argument("--only", choices=["head", "batch", "serve", "city", "cascade", "graph", "cpu"], default=None,
                     help="profiling: run only this leg and print its record (not the benchmark line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (calibrates HBM byte counters) and exit")
@@ -628,7 +741,9 @@ def main():
             rec = {"solve_ms": round(head["solve_ms"], 3), "iters": head["iters"],
                    "roofline": roofline(head, args.precision, (args.nx, args.ny, args.nz))}
         elif args.only == "batch":
-            rec = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
+            rec, _ = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample, args.batch_sigma)
+        elif args.only == "cpu":  # (the CPU protocol on the block: every thread and one thread)
+            rec, _ = cpu_baseline(g.name, [g], mats, args.cpu_runs, args.cpu_runs_one_thread)
         elif args.only == "serve":
             rec = run_serving(mats, ctx, [int(x) for x in args.serve_lambdas.split(",") if x], args.serve_ticks,
                               args.serve_warmup, args.serve_life, args.serve_pool,
@@ -646,6 +761,7 @@ def main():
     head = run_path(g, mats, head_prec, args.steps, args.warmup, ctx)
     other = run_path(g, mats, other_prec, max(1, args.steps // 2), 1, ctx)
     plain = other if head_prec == SVXG_MIXED else head
+    mixed_res = head if head_prec == SVXG_MIXED else other
     step_s = (head["solve_ms"] + head["checks_ms"]) / 1e3
     value = g.dofs / step_s
     rl = roofline(head, args.precision, (args.nx, args.ny, args.nz))
@@ -657,32 +773,46 @@ def main():
         "vs_baseline": None, "dtype": "f32+f64" if head_prec == SVXG_MIXED else "f64", "data": "synthetic",
         "config": {"workload": f"single large structure: block {args.nx}x{args.ny}x{args.nz}, 15% voids, "
                                f"{g.n} nodes, {g.dofs} DOFs, AMG-PCG to 1e-8 + bond checks",
-                   "precision": args.precision, "iters": head["iters"], "refinements": head["refinements"],
+                   "precision": args.precision,
+                   "precision_detail": (PRECISION_MIXED if head_prec == SVXG_MIXED else PRECISION_FP64)
+                   + ("; V-cycle matrices, prolongators and inverse diagonal blocks stored as FP16 blocks "
+                      "(computed in FP32)" if head["info"]["vcycle_fp16"] else ""),
+                   "iters": head["iters"], "refinements": head["refinements"],
                    "setup_ms": round(head["setup_ms"], 1), "solve_ms": round(head["solve_ms"], 3),
                    "checks_ms": round(head["checks_ms"], 3), "wall_ms_pcie_inclusive": round(head["wall_ms"], 3)},
         "plain_fp64": {"value": round(g.dofs / ((plain["solve_ms"] + plain["checks_ms"]) / 1e3), 1),
+                       "precision_detail": PRECISION_FP64,
                        "solve_ms": round(plain["solve_ms"], 3), "iters": plain["iters"],
                        "setup_ms": round(plain["setup_ms"], 1)},
         "roofline": rl,
     }
-    ref_u = plain["u"]
-    mixed_res = head if head_prec == SVXG_MIXED else other
-    scale = np.abs(ref_u).max()
-    line["mixed_vs_fp64"] = {"u_rel_err": float(np.abs(mixed_res["u"] - ref_u).max() / scale),
-                             "phi_max_abs_err": float(np.abs(mixed_res["phi"] - plain["phi"]).max())}
+    line["mixed_vs_fp64"] = compare(mixed_res["u"], mixed_res["phi"], plain["u"], plain["phi"])
     # the other mixed split: FP32 K products too, inside FP64 iterative refinement
     ir = run_path(g, mats, SVXG_MIXED, max(1, args.steps // 2), 1, ctx, mixed_fp64_rows=0)
     line["mixed_fp32_products_ir"] = {
         "value": round(g.dofs / ((ir["solve_ms"] + ir["checks_ms"]) / 1e3), 1), "solve_ms": round(ir["solve_ms"], 3),
         "iters": ir["iters"], "refinements": ir["refinements"],
-        "u_rel_err_vs_fp64": float(np.abs(ir["u"] - ref_u).max() / scale)}
+        "vs_fp64": compare(ir["u"], ir["phi"], plain["u"], plain["phi"])}
+    # C1: the plain method on the small tower (8 x 8 x 20, FP64), and the mixed path beside it
+    tw = gen.tower()
+    c1 = run_path(tw, mats, SVXG_FP64, args.steps, args.warmup, ctx)
+    c1m = run_path(tw, mats, SVXG_MIXED, args.steps, args.warmup, ctx)
+    line["tower"] = {"workload": f"tower 8x8x20: {tw.n} nodes, {tw.dofs} DOFs", "dofs": tw.dofs,
+                     "fp64": {"value": round(tw.dofs / ((c1["solve_ms"] + c1["checks_ms"]) / 1e3), 1),
+                              "solve_ms": round(c1["solve_ms"], 3), "checks_ms": round(c1["checks_ms"], 3),
+                              "iters": c1["iters"], "setup_ms": round(c1["setup_ms"], 1)},
+                     "mixed": {"value": round(tw.dofs / ((c1m["solve_ms"] + c1m["checks_ms"]) / 1e3), 1),
+                               "solve_ms": round(c1m["solve_ms"], 3), "iters": c1m["iters"],
+                               "setup_ms": round(c1m["setup_ms"], 1)}}
+    cpu_all = {}
     if args.batch > 0:
-        line["batched"] = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample)
+        line["batched"], cpu_all["batched_sample"] = run_batch(
+            mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample, args.batch_sigma,
+            (0, 0) if args.no_cpu else (args.cpu_runs, args.cpu_runs))
     if args.serve_lambdas:
         lams = [int(x) for x in args.serve_lambdas.split(",") if x]
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
-        # (a context of its own, as a server's would be: run on the context the batch leg had filled
-        # and emptied, the setup workers took 45 % longer, for a reason not yet found)
+        # (a context of its own, as a server's would be)
         sctx = stress.Context(0)
         line["serving"] = run_serving(mats, sctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
                                       args.serve_pool, setup_workers=args.serve_setup_workers)
@@ -693,15 +823,27 @@ def main():
         cctx.close()
     cb = None
     if not args.no_cpu:
-        log("cpu baseline: the reference's solver on the same workload ...")
-        cb = cpu_baseline(g, mats)
-        su = np.abs(cb["u"]).max()
-        line["error_vs_cpu_reference"] = {
-            "fp64_u_rel": float(np.abs(plain["u"] - cb["u"]).max() / su),
-            "mixed_u_rel": float(np.abs(mixed_res["u"] - cb["u"]).max() / su),
-            "fp64_phi_abs": float(np.abs(plain["phi"] - cb["phi"]).max()),
-            "failed_bonds_gpu": int((head["phi"] >= 1).sum()), "failed_bonds_cpu": int((cb["phi"] >= 1).sum()),
-        }
+        log("cpu baseline: the reference's solver on the same workloads ...")
+        c1cpu, s1 = cpu_baseline(tw.name, [tw], mats, args.cpu_runs, args.cpu_runs, keep=True)
+        cpu_all["tower"] = c1cpu
+        line["tower"]["error_vs_cpu_reference"] = {
+            "fp64": compare(c1["u"], c1["phi"], *s1[0]), "mixed": compare(c1m["u"], c1m["phi"], *s1[0])}
+        line["tower"]["fp64_vs_cpu_all_threads"] = round(
+            (c1cpu["all_threads"]["solve_s"]) / (c1["solve_ms"] / 1e3), 1)
+        c2cpu, s2 = cpu_baseline(g.name, [g], mats, args.cpu_runs, args.cpu_runs_one_thread, keep=True)
+        cpu_all["block"] = c2cpu
+        cu, cphi = s2[0]
+        line["error_vs_cpu_reference"] = {"fp64": compare(plain["u"], plain["phi"], cu, cphi),
+                                          "mixed": compare(mixed_res["u"], mixed_res["phi"], cu, cphi),
+                                          "mixed_fp32_products_ir": compare(ir["u"], ir["phi"], cu, cphi)}
+        allc = c2cpu["all_threads"]
+        cb = {"value": allc["dof_per_s"], "unit": "DOF/s", "cores": allc["threads"], "kind": allc["kind"],
+              "sample": f"the same block solved to 1e-8 ({allc['iters']} iterations), median of {allc['runs']} runs "
+                        f"of assemble + cold solve on {allc['threads']} threads ({c2cpu['cpu_model']}, nproc "
+                        f"{c2cpu['nproc']}): setup {allc['setup_s']:.2f} s, solve {allc['solve_s']:.2f} s"}
+        if c2cpu["one_thread"]:
+            cb["one_thread"] = {k: c2cpu["one_thread"][k] for k in ("dof_per_s", "setup_s", "solve_s", "runs")}
+        cb["setup_s"], cb["solve_s"], cb["u"] = allc["setup_s"], allc["solve_s"], cu
     if args.cascade > 0:
         log(f"cascade: the block's break rounds, largest phi {args.cascade} under gravity")
         kctx = stress.Context(0)
@@ -714,11 +856,10 @@ def main():
                                               cpu=not args.no_cpu)
         gctx.close()
     if cb is not None:
-        for k in ("u", "phi", "setup_s", "solve_s", "iters"):
+        for k in ("setup_s", "solve_s", "u"):
             cb.pop(k)
-        line["cpu_baseline"] = cb
-    else:
-        line["cpu_baseline"] = None
+    line["cpu_baseline"] = cb
+    line["cpu_baselines"] = cpu_all or None
     print(json.dumps(line), flush=True)
 
 

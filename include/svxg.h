@@ -34,10 +34,15 @@ enum {
 
 /* ---- precision of a solve ---------------------------------------------------------------- */
 enum {
-  /* FP64 PCG, FP64 K products, FP64 V-cycle: the plain path */
+  /* the plain path: FP64 PCG (FP64 K products, dots and vector updates) around an FP64 V-cycle
+     (FP64 smoother, residuals, transfers and dense coarsest solve) */
   SVXG_FP64 = 0,
-  /* FP64 iterative refinement (FP64 residual recomputation, FP64 dots, FP64 coarse solve)
-     around an inner PCG whose K products and V-cycle run in FP32 */
+  /* mixed precision, by default (svxg_stress_options.mixed_fp64_rows): FP64 PCG (FP64 K products,
+     dots and updates) around a V-cycle computed in FP32 (its matrices, prolongators and inverse
+     diagonal blocks stored in FP32, or as FP16 blocks scaled by their largest entry for graphs of
+     vcycle_fp16_rows free nodes and more; the dense coarsest solve in FP64), checked by an FP64
+     residual. Graphs below mixed_fp64_rows free nodes instead run FP64 iterative refinement (FP64
+     residual recomputation and norms) around an inner PCG whose K products also run in FP32. */
   SVXG_MIXED = 1
 };
 
@@ -222,6 +227,9 @@ typedef struct svxg_problem_info {
   int32_t level_rows[16];
   int64_t level_blocks[16];
   int64_t level_pblocks[16];
+  int32_t lean;              /* the lean hierarchy (svxg_amg_options.lean_rows): finest prolongator smoothed
+                                only, coarsening ended at the first level the dense solve takes */
+  int32_t dense_rows;        /* rows of the coarsest level's dense inverse (6 x its nodes), 0 if none */
 } svxg_problem_info;
 int svxg_problem_info_get(const svxg_problem* p, svxg_problem_info* out);
 
@@ -273,7 +281,9 @@ int svxg_ctx_set_resident_rows(svxg_ctx* ctx, int32_t rows);
    launched kernel by kernel: on = 0 never recorded, 1 (the default) recorded from a batch's third
    window of steps for batches of 8192 or more level-0 tiles (below that, measured, the launches are
    faster), 2 always. Same kernels, same arguments, same bits; the graph saves the host's launches.
-   (Profiled solves launch kernel by kernel.) No counterpart in the reference. */
+   (Profiled solves launch kernel by kernel, and a context created in a process with a rocprofiler
+   tool attached - ROCP_TOOL_LIBRARIES set, as rocprofv3 does - starts at 0.) No counterpart in
+   the reference. */
 int svxg_ctx_set_step_graphs(svxg_ctx* ctx, int32_t on);
 int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 

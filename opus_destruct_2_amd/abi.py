@@ -72,6 +72,7 @@ class ProblemInfo(ctypes.Structure):
         ("blocks", ctypes.c_int64), ("work_per_iteration", ctypes.c_int64), ("assembly_work", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
         ("level_rows", ctypes.c_int32 * 16), ("level_blocks", ctypes.c_int64 * 16), ("level_pblocks", ctypes.c_int64 * 16),
+        ("lean", ctypes.c_int32), ("dense_rows", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -85,6 +86,7 @@ class ProblemInfo(ctypes.Structure):
             "device_bytes": self.device_bytes,
             "level_rows": list(self.level_rows[:L]), "level_blocks": list(self.level_blocks[:L]),
             "level_pblocks": list(self.level_pblocks[:L]),
+            "lean": bool(self.lean), "dense_rows": self.dense_rows,
         }
 
 

@@ -40,6 +40,9 @@ def cases():
     out.append(("piece400_pinned", g.nodes, g.bonds, g.f, dict(amg_min_nodes=0)))
     g = gen.piece(700, pinned=False, seed=9)
     out.append(("piece700", g.nodes, g.bonds, g.f, dict(amg_min_nodes=0)))
+    # (the single-large-structure generator at a small size: 15 % voids, materials mixed per column)
+    g = gen.block(12, 12, 10, 0.5, 0.15, seed=5)
+    out.append(("block12x12x10_voids", g.nodes, g.bonds, g.f, dict()))
     return out
 
 
