@@ -181,6 +181,111 @@ __device__ __forceinline__ T spmv_row(const int32_t* __restrict__ rowptr, const 
   return spmv_row_part<T, 1>(rowptr, col, A, x, row, q, 0);
 }
 
+// ---- products of a tile's rows, a block a lane ---------------------------------------------------
+//
+// The rows of a tile own a contiguous range of blocks (a BSR level stores a row's blocks together,
+// rows in order). A workgroup takes that range kWG blocks a pass: lane k loads block k's column
+// index, the whole block in 16-byte pieces and the six x values of its column, and writes the
+// block's six row sums (row q . x, left to right; an FP16 block's scaled by its largest magnitude)
+// to LDS; then lane q of each row adds its row's sums in block order. Each block and each gathered
+// x is loaded once (row-walking lanes load every x six times), the loads of a pass are all in flight
+// together, and the sums are those of spmv_row_part's in the same order: the same bits.
+
+// block kb's six row sums: y_q = sum_c A_qc x_c (FP16 blocks: scale * sum_c a_qc x_c, in FP32)
+template <typename T, bool HALF>
+__device__ __forceinline__ void lpb_block(const void* __restrict__ A, const int32_t* __restrict__ col,
+                                          const T* __restrict__ x, int kb, T y[6]) {
+  const int j = ldg(col + kb);
+  if constexpr (HALF) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* bp = reinterpret_cast<const u32x4*>(static_cast<const uint8_t*>(A) + size_t(kHalfBlock) * size_t(kb));
+    u32x4 w[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) w[c] = ldg(bp + c);
+    float v[6];
+    load6(reinterpret_cast<const float*>(x) + 6 * size_t(j), v);
+    uint32_t ww[20];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      ww[4 * c] = w[c].x;
+      ww[4 * c + 1] = w[c].y;
+      ww[4 * c + 2] = w[c].z;
+      ww[4 * c + 3] = w[c].w;
+    }
+    float sc;
+    __builtin_memcpy(&sc, &ww[18], 4);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      float a[6];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        __half2 h;
+        __builtin_memcpy(&h, &ww[3 * q + c], 4);
+        const float2 f = __half22float2(h);
+        a[2 * c] = f.x;
+        a[2 * c + 1] = f.y;
+      }
+      const float s = a[0] * v[0] + a[1] * v[1] + a[2] * v[2] + a[3] * v[3] + a[4] * v[4] + a[5] * v[5];
+      y[q] = T(sc * s);
+    }
+  } else {
+    using P = typename V2<T>::t;
+    const P* bp = reinterpret_cast<const P*>(static_cast<const T*>(A) + 36 * size_t(kb));
+    P m[18];
+#pragma unroll
+    for (int c = 0; c < 18; ++c) m[c] = ldg(bp + c);
+    T v[6];
+    load6(x + 6 * size_t(j), v);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const P a0 = m[3 * q], a1 = m[3 * q + 1], a2 = m[3 * q + 2];
+      y[q] = a0.x * v[0] + a0.y * v[1] + a1.x * v[2] + a1.y * v[3] + a2.x * v[4] + a2.y * v[5];
+    }
+  }
+}
+
+// (A x)_q of row `row` (lanes with on = false get 0) over the tile's rows [row0, row0 + nrows); every
+// lane of the workgroup must call it (it synchronises the workgroup)
+template <typename T, bool HALF>
+__device__ __forceinline__ T lpb_rows(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                      const void* __restrict__ A, const T* __restrict__ x, int row0, int nrows,
+                                      bool on, int row, int q) {
+  __shared__ T yb[6 * kWG];
+  const int k0 = ldg(rowptr + row0), k1 = ldg(rowptr + row0 + nrows);
+  int rb0 = 0, rb1 = 0;
+  if (on) {
+    rb0 = ldg(rowptr + row);
+    rb1 = ldg(rowptr + row + 1);
+  }
+  T acc = T(0);
+  for (int c = k0; c < k1; c += kWG) {
+    const int kb = c + int(threadIdx.x);
+    if (kb < k1) {
+      T y[6];
+      lpb_block<T, HALF>(A, col, x, kb, y);
+#pragma unroll
+      for (int u = 0; u < 6; ++u) yb[6 * threadIdx.x + u] = y[u];
+    }
+    __syncthreads();
+    if (on) {
+      const int e0 = max(rb0, c), e1 = min(rb1, c + kWG);
+      for (int k = e0; k < e1; ++k) acc += yb[6 * (k - c) + q];
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
+// lpb_rows over a level's matrix in the precision T (FP32: its FP16 copy where one exists)
+template <typename T>
+__device__ __forceinline__ T lpb_rows_v(const LevelDev& L, const T* __restrict__ A, const T* __restrict__ x, int row0,
+                                        int nrows, bool on, int row, int q) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (L.A16) return lpb_rows<float, true>(L.rowptr, L.col, L.A16, x, row0, nrows, on, row, q);
+  }
+  return lpb_rows<T, false>(L.rowptr, L.col, A, x, row0, nrows, on, row, q);
+}
+
 // (M v)_q with M a 6x6 row-major block and v the row's values gathered from its lanes
 template <typename T>
 __device__ __forceinline__ T block_row(const T* __restrict__ M, const T v[6], int q) {
@@ -273,17 +378,30 @@ __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl
   const bool on = rl >= 0 && rl < tl.nrows;
   const int row = tl.row0 + (on ? rl : 0);
   const size_t e = 6 * size_t(row) + q;
-  T r = T(0), xin = T(0), bq = T(0);
+  bool half = false;
+  if constexpr (std::is_same<T, float>::value) half = L.D16 != nullptr;
+  // (the loads that do not wait for the product go first: b, x_in, the direction and the row of
+  // the inverse diagonal block)
+  T xin = T(0), bq = T(0), dold = T(0);
+  uint32_t dw[3] = {0u, 0u, 0u};
+  float dsc = 0.0f;
+  T dm[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   if (on) {
     bq = a.b[e];
-    if (FRESH) {
-      r = bq;
-    } else {
-      xin = a.x[in_buf][e];
-      r = spmv_row_part_v<T, S>(L, a.A, a.x[in_buf], row, q, sub);
+    if (!FRESH) xin = a.x[in_buf][e];
+    if (sub == 0) {
+      if (half) half_row_load(L.D16 + size_t(kHalfBlock) * size_t(row), q, dw, &dsc);
+      else load6(a.Dinv + 36 * size_t(row) + 6 * q, dm);
+      if (step > 0) dold = a.d[e];
     }
   }
-  if (!FRESH) r = bq - row_reduce<S>(r);
+  T r = bq;
+  if constexpr (!FRESH) {
+    T ax = T(0);
+    if constexpr (S == 1) ax = lpb_rows_v<T>(L, a.A, a.x[in_buf], tl.row0, tl.nrows, on, row, q);
+    else if (on) ax = spmv_row_part_v<T, S>(L, a.A, a.x[in_buf], row, q, sub);
+    r = bq - row_reduce<S>(ax);
+  }
   T rv[6];
   row_gather(r, rv);
   double c1, c2;
@@ -291,17 +409,9 @@ __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl
   T xo = T(0);
   if (on && sub == 0) {
     T z;
-    bool half = false;
-    if constexpr (std::is_same<T, float>::value) half = L.D16 != nullptr;
-    if (half) {
-      uint32_t w[3];
-      float sc;
-      half_row_load(L.D16 + size_t(kHalfBlock) * size_t(row), q, w, &sc);
-      z = T(half_row_dot(w, sc, reinterpret_cast<const float*>(rv)));
-    } else {
-      z = block_row<T>(a.Dinv + 36 * size_t(row), rv, q);
-    }
-    const T d = (step == 0) ? T(c2) * z : T(c1) * a.d[e] + T(c2) * z;
+    if (half) z = T(half_row_dot(dw, dsc, reinterpret_cast<const float*>(rv)));
+    else z = dm[0] * rv[0] + dm[1] * rv[1] + dm[2] * rv[2] + dm[3] * rv[3] + dm[4] * rv[4] + dm[5] * rv[5];
+    const T d = (step == 0) ? T(c2) * z : T(c1) * dold + T(c2) * z;
     a.d[e] = d;
     xo = xin + d;
     a.x[out_buf][e] = xo;
@@ -331,11 +441,14 @@ __device__ __forceinline__ void residual_tile(const GraphDev& g, const Tile& tl,
   lane_row_s<S>(rl, sub, q);
   const bool on = rl >= 0 && rl < tl.nrows;
   const int row = tl.row0 + (on ? rl : 0);
-  T v = on ? spmv_row_part_v<T, S>(L, a.A, a.x[buf], row, q, sub) : T(0);
+  const size_t e = 6 * size_t(row) + q;
+  const T bq = on && sub == 0 ? a.b[e] : T(0);
+  T v = T(0);
+  if constexpr (S == 1) v = lpb_rows_v<T>(L, a.A, a.x[buf], tl.row0, tl.nrows, on, row, q);
+  else if (on) v = spmv_row_part_v<T, S>(L, a.A, a.x[buf], row, q, sub);
   v = row_reduce<S>(v);
   if (!on || sub != 0) return;
-  const size_t e = 6 * size_t(row) + q;
-  a.r[e] = a.b[e] - v;
+  a.r[e] = bq - v;
 
 }
 template <typename T, int S>
@@ -622,13 +735,14 @@ __device__ __forceinline__ void pcg_spmv_tile(const GraphDev& g, const Tile& tl,
   int rl, q;
   lane_row(rl, q);
   const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  const size_t e = 6 * size_t(row) + q;
+  const T pe = on ? v.p[e] : T(0);
+  const T y = lpb_rows<T, false>(L.rowptr, L.col, arrays<T>(L).A, v.p, tl.row0, tl.nrows, on, row, q);
   double s = 0.0;
   if (on) {
-    const int row = tl.row0 + rl;
-    const size_t e = 6 * size_t(row) + q;
-    const T y = spmv_row<T>(L.rowptr, L.col, arrays<T>(L).A, v.p, row, q);
     v.q[e] = y;
-    s = double(v.p[e]) * double(y);
+    s = double(pe) * double(y);
   }
   finish_partials(s, 0.0, partial, tl.slot, false);
 
@@ -773,9 +887,11 @@ __global__ void __launch_bounds__(kWG) k_spmv_plain(const int32_t* __restrict__ 
   if (t >= ntiles) return;
   int rl, q;
   lane_row(rl, q);
-  const int row = t * kTileRows + rl;
-  if (rl < 0 || row >= n) return;
-  y[6 * size_t(row) + q] = spmv_row<T>(rowptr, col, A, x, row, q);
+  const int row0 = t * kTileRows, nrows = min(kTileRows, n - row0);
+  const bool on = rl >= 0 && rl < nrows;
+  const int row = row0 + (on ? rl : 0);
+  const T v = lpb_rows<T, false>(rowptr, col, A, x, row0, nrows, on, row, q);
+  if (on) y[6 * size_t(row) + q] = v;
 }
 
 // ---- start of a solve ------------------------------------------------------------------------
@@ -786,12 +902,13 @@ __device__ __forceinline__ void begin_residual_tile(const GraphDev& g, const Til
   int rl, q;
   lane_row(rl, q);
   const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  const size_t e = 6 * size_t(row) + q;
+  const double b = on ? g.b[e] : 0.0;
+  const double kx = lpb_rows<double, false>(L.rowptr, L.col, L.f64.A, g.x, tl.row0, tl.nrows, on, row, q);
   double sb = 0.0, sr = 0.0;
   if (on) {
-    const int row = tl.row0 + rl;
-    const size_t e = 6 * size_t(row) + q;
-    const double b = g.b[e];
-    const double r = b - spmv_row<double>(L.rowptr, L.col, L.f64.A, g.x, row, q);
+    const double r = b - kx;
     g.r[e] = r;
     sb = b * b;
     sr = r * r;
@@ -1029,11 +1146,13 @@ __device__ __forceinline__ void outer_residual_tile(const GraphDev& g, const Til
   int rl, q;
   lane_row(rl, q);
   const bool on = rl >= 0 && rl < tl.nrows;
+  const int row = tl.row0 + (on ? rl : 0);
+  const size_t e = 6 * size_t(row) + q;
+  const double b = on ? g.b[e] : 0.0;
+  const double kx = lpb_rows<double, false>(L.rowptr, L.col, L.f64.A, g.x, tl.row0, tl.nrows, on, row, q);
   double sr = 0.0;
   if (on) {
-    const int row = tl.row0 + rl;
-    const size_t e = 6 * size_t(row) + q;
-    const double r = g.b[e] - spmv_row<double>(L.rowptr, L.col, L.f64.A, g.x, row, q);
+    const double r = b - kx;
     g.r[e] = r;
     sr = r * r;
   }

@@ -39,6 +39,7 @@ def _bind(path, pre):
         "retire_nodes": ([vp, vp, i32], None),
         "set_bond_broken": ([vp, i32, i32], None),
         "bond_load": ([vp, i32, vp, vp], None),
+        "bond_checks": ([vp, vp, f64, vp, vp, vp], None),
         "bond_utilization": ([vp, vp, f64, vp, i32, vp], f64),
         "bond_strength": ([vp, f64, vp, i32, vp], None),
         "info": ([vp, vp], None),
@@ -158,25 +159,18 @@ class CpuProblem:
         return out[0]
 
     def bond_loads(self, u):
+        """bond_load of every bond (stress.cpp:477-502), one bond after another in the library."""
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros(len(self.bonds), BOND_LOAD)
-        fn = self._f("bond_load")
-        for b in range(len(self.bonds)):
-            fn(self.h, b, u.ctypes.data, out[b:b + 1].ctypes.data)
+        self._f("bond_checks")(self.h, u.ctypes.data, 1.0, out.ctypes.data, None, None)
         return out
 
     def bond_checks(self, u, fragility=1.0):
-        """phi and FailMode of every bond (bond_utilization after bond_load)."""
-        loads = self.bond_loads(u)
-        bonds = self.get_bonds()
-        phi = np.zeros(len(bonds))
-        mode = np.zeros(len(bonds), np.uint8)
-        fn = self._f("bond_utilization")
-        md = ctypes.c_uint8()
-        for b in range(len(bonds)):
-            phi[b] = fn(bonds[b:b + 1].ctypes.data, loads[b:b + 1].ctypes.data, fragility, self.mats.ctypes.data,
-                        len(self.mats), ctypes.byref(md))
-            mode[b] = md.value
+        """phi and FailMode of every bond (bond_utilization after bond_load, stress.cpp:14-57)."""
+        u = np.ascontiguousarray(u, np.float64)
+        phi = np.zeros(len(self.bonds))
+        mode = np.zeros(len(self.bonds), np.uint8)
+        self._f("bond_checks")(self.h, u.ctypes.data, float(fragility), None, phi.ctypes.data, mode.ctypes.data)
         return phi, mode
 
     def info(self):

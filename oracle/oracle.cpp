@@ -1189,6 +1189,17 @@ double orc_bond_utilization(const svxg_bond* bp, const svxg_bond_load* Lp, doubl
   return phi;
 }
 
+// every bond's loads and checks from one u (orc_bond_load + orc_bond_utilization of each bond)
+void orc_bond_checks(orc_problem* h, const double* u, double fragility, svxg_bond_load* loads, double* phi,
+                     uint8_t* mode) {
+  for (size_t b = 0; b < h->bonds.size(); ++b) {
+    const svxg_bond_load L = h->bond_load(static_cast<i32>(b), u);
+    if (loads) loads[b] = L;
+    if (phi) phi[b] = orc_bond_utilization(&h->bonds[b], &L, fragility, h->mats.data(), static_cast<i32>(h->mats.size()),
+                                           mode ? &mode[b] : nullptr);
+  }
+}
+
 void orc_lerp_load(const svxg_bond_load* prev, const svxg_bond_load* now, double k, svxg_bond_load* out) {
   const f64* p = &prev->N;
   const f64* q = &now->N;

@@ -33,6 +33,9 @@ void orc_remove_bond(orc_problem* h, int32_t b);
 void orc_retire_nodes(orc_problem* h, const int32_t* ids, int32_t count);
 void orc_set_bond_broken(orc_problem* h, int32_t b, int32_t broken);
 void orc_bond_load(orc_problem* h, int32_t b, const double* u, svxg_bond_load* out);
+/* every bond's loads (loads, may be null) and, with phi, its utilisation and FailMode (mode may be null) */
+void orc_bond_checks(orc_problem* h, const double* u, double fragility, svxg_bond_load* loads, double* phi,
+                     uint8_t* mode);
 void orc_extend_warm_start(const svxg_node* nodes, int32_t n, const svxg_bond* bonds, int32_t m, double* u,
                            int32_t first_new);
 double orc_bond_utilization(const svxg_bond* b, const svxg_bond_load* L, double fragility, const svxg_material* mats,

@@ -203,6 +203,21 @@ double ref_bond_utilization(const svxg_bond* b, const svxg_bond_load* L, double 
   return phi;
 }
 
+// every bond's loads and checks from one u (bond_load + bond_utilization of each bond in turn)
+void ref_bond_checks(ref_problem* h, const double* u, double fragility, svxg_bond_load* loads, double* phi,
+                     uint8_t* mode) {
+  const std::vector<f64> uv(u, u + 6 * h->P.nodes.size());
+  for (size_t b = 0; b < h->P.bonds.size(); ++b) {
+    const BondLoad L = h->P.bond_load(static_cast<i32>(b), uv);
+    if (loads) loads[b] = svxg_bond_load{L.N, L.V1, L.V2, L.T, L.M1, L.M2};
+    if (phi) {
+      FailMode fm = FailMode::None;
+      phi[b] = bond_utilization(h->P.bonds[b], L, fragility, h->table, &fm);
+      if (mode) mode[b] = static_cast<uint8_t>(fm);
+    }
+  }
+}
+
 void ref_info(ref_problem* h, svxg_problem_info* out) {
   *out = svxg_problem_info{};
   out->nodes = static_cast<int32_t>(h->P.nodes.size());
