@@ -481,9 +481,13 @@ struct Batch {
   struct Lv {
     DevBuf ts[2], tp, tc[2], tstall[2], dense;
     int nts[2] = {0, 0}, ntp = 0, ntc[2] = {0, 0}, ntstall[2] = {0, 0}, ndense = 0;
+    DevBuf dpart, dcnt;  // (few dense chunks: the solve a wavefront a (chunk, span), its partials and counters)
   };
+  // dense coarsest solves of at most this many 64-row chunks a level are spread a wavefront a span
+  static constexpr int kDenseSplitChunks = 64;
   std::vector<Lv> lvl;
   DevBuf partial;
+  DevBuf fcnt;     // per graph: tiles of the running fused launch done so far (0 between launches)
   DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
   bool any_amg = false;
   // one tiled step recorded as a HIP graph, by (inner-only, mixed, FP32 products, FP64 products): the launches'
@@ -627,6 +631,8 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     if (probs[g]->jacobi_only) add_tiles(tj, g, probs[g]->nfree, kTileRows);
   }
   gt[size_t(count)] = int32_t(t0.size());
+  // (the fused launches' and the split dense solves' counters start at 0)
+  const std::vector<int32_t> zc(size_t(std::max(Batch::kDenseSplitChunks, count)), 0);
   B.nt0 = int(t0.size());
   B.nt0run = int(t0run.size());
   B.ntj = int(tj.size());
@@ -638,6 +644,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   SVXG_TRY(up.add(B.gtile, A, gt));
   SVXG_TRY(up.add(B.res, A, res));
   SVXG_TRY(B.partial.make(A, sizeof(double) * 2 * size_t(std::max(1, B.nt0))));
+  SVXG_TRY(up.add(B.fcnt, A, zc));
   B.lvl.clear();
   B.lvl.resize(size_t(B.maxlev));
   struct LvTiles {
@@ -676,6 +683,11 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     V.ndense = int(T.dense.size());
     SVXG_TRY(up.add(V.tp, A, T.tp));
     SVXG_TRY(up.add(V.dense, A, T.dense));
+    if (V.ndense > 0 && V.ndense <= Batch::kDenseSplitChunks) {
+      SVXG_TRY(V.dpart.make(A, sizeof(double) * size_t(V.ndense) * kDenseSpans * 64));
+      SVXG_TRY(V.dcnt.make(A, sizeof(int32_t) * size_t(V.ndense)));
+      SVXG_TRY(up.add_raw(V.dcnt.as<void>(), zc.data(), sizeof(int32_t) * size_t(V.ndense)));
+    }
   }
   return up.flush();
 }
@@ -790,7 +802,8 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
       launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (dg - 1) & 1, inner, SV[v]);
     }
     for (int v = 0; v < 2; ++v) launch_restrict<T>(s, G, V.tc[v].as<Tile>(), V.ntc[v], l, inner, SV[v]);
-    if (V.ndense) launch_dense_solve<T>(s, G, V.dense.as<Tile>(), V.ndense, l, inner);
+    if (V.ndense)
+      launch_dense_solve<T>(s, G, V.dense.as<Tile>(), V.ndense, l, inner, V.dpart.as<double>(), V.dcnt.as<int32_t>());
     for (int v = 0; v < 2; ++v)
       if (V.ntstall[v])
         for (int st = 0; st < kStallDegree; ++st)
@@ -886,16 +899,16 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
   const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
+  // (the per-graph scalars - alpha, |r| and the stopping test, beta - are taken by the last workgroup
+  // of each graph's tiles in the vector kernels themselves: fused launches)
+  const Fuse fu{B.fcnt.as<int32_t>(), gt};
   {
     ProfScope ps(B, 0, s, SERVE_OUTER);
-    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa());
+    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa(), fu);
   }
-  launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
-  launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
-  launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_OUTER, part);
+  launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, fu);
   vcycle<double>(B, s, SERVE_OUTER, degree);
-  launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
-  launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_OUTER, part);
+  launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, fu);
   launch_update_p<double>(s, G, t0, B.nt0run, SERVE_OUTER);
 }
 
@@ -966,30 +979,27 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
   const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
+  // (per-graph scalars in the vector kernels' last workgroups: fused launches, as in fp64_step)
+  const Fuse fu{B.fcnt.as<int32_t>(), gt};
   if (!lean) {
-    launch_outer_residual(s, G, t0, B.nt0run, part);
-    launch_finish(s, FinishKind::Outer, G, gt, B.G, SERVE_REFINE, part);
+    launch_outer_residual(s, G, t0, B.nt0run, part, fu);
     launch_inner_init(s, G, t0, B.nt0run);
   }
   {
     ProfScope ps(B, p32 ? 1 : -1, s, SERVE_P32 | SERVE_IT);
-    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa());
+    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa(), fu);
   }
-  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
-  launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_ITER, part);
-  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
-  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
-  launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_ITER, part);
+  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part, {}, fu);
+  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, fu);
+  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part, fu);
   vcycle<float>(B, s, SERVE_INNER, degree);
-  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
-  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
-  launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);
+  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, fu);
+  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part, fu);
   if (p32) launch_update_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT);
   if (p64) launch_update_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT);
   if (!lean) {
-    if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part);
-    if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
-    launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
+    if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part, fu);
+    if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part, fu);
   }
   if (p32) launch_inner_apply(s, G, t0, B.nt0run);  // (FP64-product inner solves update x themselves)
   launch_inner_close(s, G, B.G);

@@ -191,12 +191,14 @@ __device__ __forceinline__ T spmv_row(const int32_t* __restrict__ rowptr, const 
 // x is loaded once (row-walking lanes load every x six times), the loads of a pass are all in flight
 // together, and the sums are those of spmv_row_part's in the same order: the same bits.
 
-// block kb's six row sums: y_q = sum_c A_qc x_c (FP16 blocks: scale * sum_c a_qc x_c, in FP32)
-template <typename T, bool HALF>
+// block kb's row sums y_q = sum_c A_qc x_c for its rows q in [q0, q0 + NQ) (FP16 blocks: all six,
+// scale * sum_c a_qc x_c in FP32)
+template <typename T, bool HALF, int NQ>
 __device__ __forceinline__ void lpb_block(const void* __restrict__ A, const int32_t* __restrict__ col,
-                                          const T* __restrict__ x, int kb, T y[6]) {
+                                          const T* __restrict__ x, int kb, int q0, T y[NQ]) {
   const int j = ldg(col + kb);
   if constexpr (HALF) {
+    static_assert(NQ == 6, "an FP16 block is taken whole");
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const u32x4* bp = reinterpret_cast<const u32x4*>(static_cast<const uint8_t*>(A) + size_t(kHalfBlock) * size_t(kb));
     u32x4 w[5];
@@ -230,14 +232,14 @@ __device__ __forceinline__ void lpb_block(const void* __restrict__ A, const int3
     }
   } else {
     using P = typename V2<T>::t;
-    const P* bp = reinterpret_cast<const P*>(static_cast<const T*>(A) + 36 * size_t(kb));
-    P m[18];
+    const P* bp = reinterpret_cast<const P*>(static_cast<const T*>(A) + 36 * size_t(kb) + 6 * q0);
+    P m[3 * NQ];
 #pragma unroll
-    for (int c = 0; c < 18; ++c) m[c] = ldg(bp + c);
+    for (int c = 0; c < 3 * NQ; ++c) m[c] = ldg(bp + c);
     T v[6];
     load6(x + 6 * size_t(j), v);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const P a0 = m[3 * q], a1 = m[3 * q + 1], a2 = m[3 * q + 2];
       y[q] = a0.x * v[0] + a0.y * v[1] + a1.x * v[2] + a1.y * v[3] + a2.x * v[4] + a2.y * v[5];
     }
@@ -245,30 +247,34 @@ __device__ __forceinline__ void lpb_block(const void* __restrict__ A, const int3
 }
 
 // (A x)_q of row `row` (lanes with on = false get 0) over the tile's rows [row0, row0 + nrows); every
-// lane of the workgroup must call it (it synchronises the workgroup)
+// lane of the workgroup must call it (it synchronises the workgroup). H lanes share a block, each
+// taking 6 / H of its rows (FP64 blocks: two lanes of 144 bytes each keep more blocks in flight)
 template <typename T, bool HALF>
 __device__ __forceinline__ T lpb_rows(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                       const void* __restrict__ A, const T* __restrict__ x, int row0, int nrows,
                                       bool on, int row, int q) {
-  __shared__ T yb[6 * kWG];
+  constexpr int H = (!HALF && sizeof(T) == 8) ? 2 : 1;
+  constexpr int NQ = 6 / H, PER = kWG / H;
+  __shared__ T yb[6 * PER];
   const int k0 = ldg(rowptr + row0), k1 = ldg(rowptr + row0 + nrows);
   int rb0 = 0, rb1 = 0;
   if (on) {
     rb0 = ldg(rowptr + row);
     rb1 = ldg(rowptr + row + 1);
   }
+  const int lb = int(threadIdx.x) / H, q0 = (int(threadIdx.x) % H) * NQ;
   T acc = T(0);
-  for (int c = k0; c < k1; c += kWG) {
-    const int kb = c + int(threadIdx.x);
+  for (int c = k0; c < k1; c += PER) {
+    const int kb = c + lb;
     if (kb < k1) {
-      T y[6];
-      lpb_block<T, HALF>(A, col, x, kb, y);
+      T y[NQ];
+      lpb_block<T, HALF, NQ>(A, col, x, kb, q0, y);
 #pragma unroll
-      for (int u = 0; u < 6; ++u) yb[6 * threadIdx.x + u] = y[u];
+      for (int u = 0; u < NQ; ++u) yb[6 * lb + q0 + u] = y[u];
     }
     __syncthreads();
     if (on) {
-      const int e0 = max(rb0, c), e1 = min(rb1, c + kWG);
+      const int e0 = max(rb0, c), e1 = min(rb1, c + PER);
       for (int k = e0; k < e1; ++k) acc += yb[6 * (k - c) + q];
     }
     __syncthreads();
@@ -354,14 +360,51 @@ __device__ __forceinline__ void finish_partials(double v0, double v1, double* pa
 }
 
 // the sum of a graph's per-tile partials (its tiles' slots k0 .. k1), in the same order wherever run
+// (each lane's strided sum in slot order, its loads four at a time; then the workgroup's fixed order)
 __device__ __forceinline__ double graph_sum(int k0, int k1, const double* partial, int which) {
   __shared__ double lds[4];
   double a = 0.0;
-  for (int k = k0 + int(threadIdx.x); k < k1; k += kWG) a += partial[2 * k + which];
+  int k = k0 + int(threadIdx.x);
+  for (; k + 3 * kWG < k1; k += 4 * kWG) {
+    const double v0 = partial[2 * k + which], v1 = partial[2 * (k + kWG) + which];
+    const double v2 = partial[2 * (k + 2 * kWG) + which], v3 = partial[2 * (k + 3 * kWG) + which];
+    a += v0;
+    a += v1;
+    a += v2;
+    a += v3;
+  }
+  for (; k < k1; k += kWG) a += partial[2 * k + which];
   return block_sum(a, lds);
 }
 
+// True in every lane of the workgroup that finishes the last of graph g's level-0 tiles in this
+// launch (counted in cnt[g], which that workgroup sets back to 0 for the next launch): it then runs
+// the graph's per-graph step (the sum of the partials, the scalars) in place of a launch of its own.
+// The partial of this workgroup was written by its lane 0, which fences it before counting.
+__device__ __forceinline__ bool last_tile_done(int32_t* __restrict__ cnt, int g, const int32_t* __restrict__ gtile) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int n = gtile[g + 1] - gtile[g];
+    const int old = atomicAdd(cnt + g, 1);
+    last = old == n - 1 ? 1 : 0;
+    if (last) {
+      cnt[g] = 0;
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  return last != 0;
+}
+
 }  // namespace
+
+__device__ __forceinline__ void finish_begin_g(const GraphDev& g, int k0, int k1, int mixed, const double* __restrict__ partial);
+__device__ __forceinline__ void finish_start_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
+__device__ __forceinline__ void finish_alpha_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
+__device__ __forceinline__ void finish_rn_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
+__device__ __forceinline__ void finish_beta_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
+__device__ __forceinline__ void finish_outer_g(const GraphDev& g, int k0, int k1, const double* __restrict__ partial);
 
 // ---- V-cycle kernels ---------------------------------------------------------------------------
 
@@ -625,7 +668,6 @@ __global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G,
 // kDenseSpans spans, which the workgroup's NT / 64 waves share out, and the spans' partial sums meet
 // in LDS in span order (so any NT gives the same bits)
 constexpr int kDenseRows = 64;
-constexpr int kDenseSpans = 16;
 constexpr int kDenseSolveWG = 1024;  // (a wave a span: a 864-row inverse is 54 loads a lane, not 216)
 template <typename T, int NT>
 __device__ __forceinline__ void dense_rows(const LevelDev& L, const LevelArrays<T>& a, int r0, double* sb,
@@ -682,6 +724,61 @@ __global__ void __launch_bounds__(kDenseSolveWG) k_dense_solve(const GraphDev* _
   __shared__ double part[kDenseSpans * 64];
   const LevelDev& L = g.lv[level];
   dense_rows<T, kDenseSolveWG>(L, arrays<T>(L), tl.row0, sb, part);
+}
+
+// The same product spread over more of the machine: one wavefront a (64-row chunk, span) pair, its
+// span's partial sums to `part` (kDenseSpans x 64 doubles a chunk); the wavefront that finishes a
+// chunk's last span adds the chunk's spans in span order, as dense_rows does: the same bits. For
+// batches of few dense chunks (a single large graph: 14 chunks of a 864-row inverse would occupy 14
+// of the 256 compute units).
+template <typename T>
+__global__ void __launch_bounds__(64) k_dense_solve_span(const GraphDev* __restrict__ G, const Tile* __restrict__ chunks,
+                                                         int count, int level, int inner, double* __restrict__ part,
+                                                         int32_t* __restrict__ cnt) {
+  const int c = int(blockIdx.x) / kDenseSpans, sp = int(blockIdx.x) % kDenseSpans;
+  if (c >= count) return;
+  const Tile tl = chunks[c];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, inner)) return;
+  const LevelDev& L = g.lv[level];
+  const LevelArrays<T>& a = arrays<T>(L);
+  const int m = L.dense_m;
+  const int span = (m + kDenseSpans - 1) / kDenseSpans;
+  const int j0 = sp * span, j1 = min(m, j0 + span);
+  __shared__ double sb[(kMaxDenseM + kDenseSpans - 1) / kDenseSpans];
+  for (int j = j0 + int(threadIdx.x); j < j1; j += 64) sb[j - j0] = double(a.b[j]);
+  __syncthreads();
+  const int lane = threadIdx.x, i = tl.row0 + lane;
+  double s = 0.0;
+  if (i < m) {
+    const double* col = L.Minv + i;
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = ldg(col + size_t(j + u) * size_t(m));
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u] * sb[j + u - j0];
+    }
+    for (; j < j1; ++j) s += ldg(col + size_t(j) * size_t(m)) * sb[j - j0];
+  }
+  double* pc = part + size_t(c) * kDenseSpans * 64;
+  pc[sp * 64 + lane] = s;
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int old = atomicAdd(cnt + c, 1);
+    last = old == kDenseSpans - 1 ? 1 : 0;
+    if (last) {
+      cnt[c] = 0;
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  if (!last || i >= m) return;
+  double t = pc[lane];
+  for (int q = 1; q < kDenseSpans; ++q) t += pc[q * 64 + lane];
+  a.x[1][i] = T(t);
 }
 
 // z = D^-1 r (block-Jacobi preconditioning of a graph without a hierarchy)
@@ -749,7 +846,8 @@ __device__ __forceinline__ void pcg_spmv_tile(const GraphDev& g, const Tile& tl,
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int inner, double* __restrict__ partial, ProfAcc pa) {
+                                                  int ntiles, int inner, double* __restrict__ partial, ProfAcc pa,
+                                                  Fuse fu) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
@@ -757,6 +855,8 @@ __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G
   if (g.resident || !live(g, inner)) return;
   if (pa.acc && tl.row0 == 0 && threadIdx.x == 0) atomicAdd(pa.acc, pa.gb[tl.g]);
   pcg_spmv_tile<T>(g, tl, inner, partial);
+  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
+    finish_alpha_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // FP64-product inner solves (mixed precision): the FP32 V-cycle's input r_i = (float) r is written
@@ -793,13 +893,15 @@ __device__ __forceinline__ void axpy_rr_tile(const GraphDev& g, const Tile& tl, 
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_axpy_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int inner, double* __restrict__ partial) {
+                                                 int ntiles, int inner, double* __restrict__ partial, Fuse fu) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   axpy_rr_tile<T>(g, tl, inner, partial);
+  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
+    finish_rn_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // p = z + beta p
@@ -842,13 +944,15 @@ __device__ __forceinline__ void dot_rz_tile(const GraphDev& g, const Tile& tl, i
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_dot_rz(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                int ntiles, int inner, double* __restrict__ partial) {
+                                                int ntiles, int inner, double* __restrict__ partial, Fuse fu) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   dot_rz_tile<T>(g, tl, inner, partial);
+  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
+    finish_beta_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // partial r . r
@@ -917,13 +1021,16 @@ __device__ __forceinline__ void begin_residual_tile(const GraphDev& g, const Til
 
 }
 __global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                        int ntiles, double* __restrict__ partial) {
+                                                        int ntiles, double* __restrict__ partial, int mixed,
+                                                        Fuse fu) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, SERVE_NEW)) return;
   begin_residual_tile(g, tl, partial);
+  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
+    finish_begin_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], mixed, partial);
 }
 
 // the solve's start, per graph: |b|, a zero or non-finite load (x = 0, r = 0); a mixed-precision
@@ -985,13 +1092,15 @@ __device__ __forceinline__ void start_p_tile(const GraphDev& g, const Tile& tl, 
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int inner, double* __restrict__ partial) {
+                                                 int ntiles, int inner, double* __restrict__ partial, Fuse fu) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   start_p_tile<T>(g, tl, inner, partial);
+  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
+    finish_start_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // ---- per-graph finishes (one workgroup a graph; partials summed in tile order) ----------------
@@ -1160,13 +1269,15 @@ __device__ __forceinline__ void outer_residual_tile(const GraphDev& g, const Til
 
 }
 __global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                        int ntiles, double* __restrict__ partial) {
+                                                        int ntiles, double* __restrict__ partial, Fuse fu) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, SERVE_REFINE)) return;
   outer_residual_tile(g, tl, partial);
+  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
+    finish_outer_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], partial);
 }
 
 // outer check: converged, or an inner solve of K d = r to max(eta |r|, target / 2)
@@ -1502,9 +1613,14 @@ void launch_prolong(hipStream_t s, const GraphDev* G, const Tile* tiles, int nti
   hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, level, buf, inner);
 }
 template <typename T>
-void launch_dense_solve(hipStream_t s, const GraphDev* G, const Tile* chunks, int count, int level, int inner) {
+void launch_dense_solve(hipStream_t s, const GraphDev* G, const Tile* chunks, int count, int level, int inner,
+                        double* part, int32_t* cnt) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kDenseSolveWG), 0, s, G, chunks, count, level, inner);
+  if (part && cnt)
+    hipLaunchKernelGGL(k_dense_solve_span<T>, dim3(count * kDenseSpans), dim3(64), 0, s, G, chunks, count, level, inner,
+                       part, cnt);
+  else
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3(count), dim3(kDenseSolveWG), 0, s, G, chunks, count, level, inner);
 }
 template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {
@@ -1512,9 +1628,10 @@ void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntil
   hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner);
 }
 template <typename T>
-void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
+                   Fuse fu) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_dot_rz<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+  hipLaunchKernelGGL(k_dot_rz<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, fu);
 }
 template <typename T>
 void launch_dot_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
@@ -1529,14 +1646,16 @@ void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col,
 }
 template <typename T>
 void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                     ProfAcc pa) {
+                     ProfAcc pa, Fuse fu) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa);
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa,
+                     fu);
 }
 template <typename T>
-void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
+                    Fuse fu) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_axpy_rr<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+  hipLaunchKernelGGL(k_axpy_rr<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, fu);
 }
 template <typename T>
 void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {
@@ -1544,14 +1663,16 @@ void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int nt
   hipLaunchKernelGGL(k_update_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner);
 }
 template <typename T>
-void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
+void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
+                    Fuse fu) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_start_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
+  hipLaunchKernelGGL(k_start_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, fu);
 }
 
-void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial) {
+void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial, int mixed,
+                           Fuse fu) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_begin_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
+  hipLaunchKernelGGL(k_begin_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial, mixed, fu);
 }
 void launch_release_wait64(hipStream_t s, const GraphDev* G, int count) {
   if (count > 0) hipLaunchKernelGGL(k_release_wait64, dim3((count + 255) / 256), dim3(256), 0, s, G, count);
@@ -1586,9 +1707,9 @@ void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int3
       break;
   }
 }
-void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial) {
+void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial, Fuse fu) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_outer_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
+  hipLaunchKernelGGL(k_outer_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial, fu);
 }
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;
@@ -1619,15 +1740,15 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int);              \
   template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
   template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
-  template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int);                  \
+  template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, double*, int32_t*); \
   template void launch_jacobi<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                               \
-  template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
+  template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, Fuse);                \
   template void launch_dot_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_spmv_plain<T>(hipStream_t, const int32_t*, const int32_t*, const T*, const T*, T*, int);      \
-  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc);          \
-  template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                     \
+  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc, Fuse);    \
+  template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, Fuse);               \
   template void launch_update_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                             \
-  template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);
+  template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, Fuse);
 SVXG_INST(double)
 SVXG_INST(float)
 
