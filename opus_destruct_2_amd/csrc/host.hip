@@ -487,7 +487,6 @@ struct Batch {
   static constexpr int kDenseSplitChunks = 64;
   std::vector<Lv> lvl;
   DevBuf partial;
-  DevBuf fcnt;     // per graph: tiles of the running fused launch done so far (0 between launches)
   DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
   bool any_amg = false;
   // one tiled step recorded as a HIP graph, by (inner-only, mixed, FP32 products, FP64 products): the launches'
@@ -510,16 +509,19 @@ struct Batch {
   }
 };
 
-// rows averaging more than 16 entries are served a wavefront each (S = 10 sub-groups)
+// rows averaging more than 16 entries are served a wavefront each (S = 10 sub-groups): the
+// restriction of levels without FP16 copies (it reads P's blocks transposed in place)
 static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entries > 16 * rows; }
 
-// the sub-groups a row of a level's smoothing and residual launches: a wavefront (10) for long rows,
-// and for small levels with more than 8 blocks a row, whose launches are too few to hide the
-// latency of a lane's walk along its row; else six lanes (1)
-constexpr int kSmallLevelRows = 4096;
-static int smoother_sub(const HostLevel& H) {
-  const bool small = H.n > 0 && H.n < kSmallLevelRows && H.nnzb > 8 * int64_t(H.n);
-  return long_rows(H.nnzb, H.n) || small ? 10 : 1;
+// Products a block a lane (smoothing, residuals, prolongation, the FP16 restriction) take a tile's
+// rows' blocks kLpbBlocks a pass; a level's tiles hold as many rows (at most kTileRows) as fill about
+// one pass at its mean entries a row, so that long coarse rows still make many workgroups. Every row's
+// sums are the same whatever tile holds it, so this sets only the time.
+constexpr int kLpbBlocks = 256;
+static int lpb_tile_rows(int64_t entries, int64_t rows) {
+  if (rows <= 0) return kTileRows;
+  const int64_t per = std::max<int64_t>(1, (entries + rows - 1) / rows);
+  return int(std::clamp<int64_t>(kLpbBlocks / per, 1, kTileRows));
 }
 
 static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
@@ -558,8 +560,8 @@ static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
     L.Minv = H.Minv.as<double>();
     L.lmax = H.lmax;
     L.cheb_degree = P->opt.amg.cheb_degree;
-    L.sub = smoother_sub(H);
-    L.rsub = H.nc > 0 && H.np > 16 * int64_t(H.nc) ? 10 : 1;
+    L.sub = 1;
+    L.rsub = H.R16.as<void>() || !long_rows(H.np, H.nc) ? 1 : 10;
     L.f64.A = H.A64.as<double>();
     L.f64.Dinv = H.D64.as<double>();
     L.f64.P = H.P64.as<double>();
@@ -631,8 +633,8 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     if (probs[g]->jacobi_only) add_tiles(tj, g, probs[g]->nfree, kTileRows);
   }
   gt[size_t(count)] = int32_t(t0.size());
-  // (the fused launches' and the split dense solves' counters start at 0)
-  const std::vector<int32_t> zc(size_t(std::max(Batch::kDenseSplitChunks, count)), 0);
+  // (the split dense solves' counters start at 0)
+  const std::vector<int32_t> zc(size_t(Batch::kDenseSplitChunks), 0);
   B.nt0 = int(t0.size());
   B.nt0run = int(t0run.size());
   B.ntj = int(tj.size());
@@ -644,7 +646,6 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   SVXG_TRY(up.add(B.gtile, A, gt));
   SVXG_TRY(up.add(B.res, A, res));
   SVXG_TRY(B.partial.make(A, sizeof(double) * 2 * size_t(std::max(1, B.nt0))));
-  SVXG_TRY(up.add(B.fcnt, A, zc));
   B.lvl.clear();
   B.lvl.resize(size_t(B.maxlev));
   struct LvTiles {
@@ -658,16 +659,19 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
       svxg_problem* P = probs[g];
       if (!tiled[size_t(g)] || P->jacobi_only || l >= int(P->lv.size())) continue;
       const HostLevel& H = P->lv[size_t(l)];
-      const int vs = smoother_sub(H) == 10 ? 1 : 0;
+      const int srows = lpb_tile_rows(H.nnzb, H.n);
       if (!H.terminal) {
-        add_tiles(T.ts[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
-        add_tiles(T.tp, g, H.n, kTileRows);
-        const int vc = long_rows(H.np, H.nc) ? 1 : 0;
-        add_tiles(T.tc[vc], g, P->lv[size_t(l) + 1].n, vc ? tile_rows<10>() : tile_rows<1>());
+        add_tiles(T.ts[0], g, H.n, srows);
+        add_tiles(T.tp, g, H.n, lpb_tile_rows(H.np, H.n));
+        const int nc = P->lv[size_t(l) + 1].n;
+        if (H.R16.as<void>() || !long_rows(H.np, H.nc))
+          add_tiles(T.tc[0], g, nc, H.R16.as<void>() ? lpb_tile_rows(H.np, nc) : tile_rows<1>());
+        else
+          add_tiles(T.tc[1], g, nc, tile_rows<10>());
       } else if (H.dense_m > 0) {
         for (int r = 0; r < H.dense_m; r += 64) T.dense.push_back(Tile{g, r, std::min(64, H.dense_m - r), 0});
       } else {
-        add_tiles(T.tstall[vs], g, H.n, vs ? tile_rows<10>() : tile_rows<1>());
+        add_tiles(T.tstall[0], g, H.n, srows);
       }
     }
     Batch::Lv& V = B.lvl[size_t(l)];
@@ -899,16 +903,16 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
   const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
-  // (the per-graph scalars - alpha, |r| and the stopping test, beta - are taken by the last workgroup
-  // of each graph's tiles in the vector kernels themselves: fused launches)
-  const Fuse fu{B.fcnt.as<int32_t>(), gt};
   {
     ProfScope ps(B, 0, s, SERVE_OUTER);
-    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa(), fu);
+    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa());
   }
-  launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, fu);
+  launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
+  launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
+  launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_OUTER, part);
   vcycle<double>(B, s, SERVE_OUTER, degree);
-  launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, fu);
+  launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
+  launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_OUTER, part);
   launch_update_p<double>(s, G, t0, B.nt0run, SERVE_OUTER);
 }
 
@@ -979,27 +983,30 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
   const Tile* t0 = B.t0run.as<Tile>();
   const int32_t* gt = B.gtile.as<int32_t>();
   double* part = B.partial.as<double>();
-  // (per-graph scalars in the vector kernels' last workgroups: fused launches, as in fp64_step)
-  const Fuse fu{B.fcnt.as<int32_t>(), gt};
   if (!lean) {
-    launch_outer_residual(s, G, t0, B.nt0run, part, fu);
+    launch_outer_residual(s, G, t0, B.nt0run, part);
+    launch_finish(s, FinishKind::Outer, G, gt, B.G, SERVE_REFINE, part);
     launch_inner_init(s, G, t0, B.nt0run);
   }
   {
     ProfScope ps(B, p32 ? 1 : -1, s, SERVE_P32 | SERVE_IT);
-    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa(), fu);
+    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa());
   }
-  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part, {}, fu);
-  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, fu);
-  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part, fu);
+  if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
+  launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_ITER, part);
+  if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
+  if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
+  launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_ITER, part);
   vcycle<float>(B, s, SERVE_INNER, degree);
-  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, fu);
-  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part, fu);
+  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
+  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
+  launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);
   if (p32) launch_update_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT);
   if (p64) launch_update_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT);
   if (!lean) {
-    if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part, fu);
-    if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part, fu);
+    if (p32) launch_start_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_ST, part);
+    if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
+    launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
   }
   if (p32) launch_inner_apply(s, G, t0, B.nt0run);  // (FP64-product inner solves update x themselves)
   launch_inner_close(s, G, B.G);

@@ -246,13 +246,14 @@ __device__ __forceinline__ void lpb_block(const void* __restrict__ A, const int3
   }
 }
 
-// (A x)_q of row `row` (lanes with on = false get 0) over the tile's rows [row0, row0 + nrows); every
-// lane of the workgroup must call it (it synchronises the workgroup). H lanes share a block, each
+// acc0 + (A x)_q of row `row` (lanes with on = false get 0) over the tile's rows [row0, row0 + nrows),
+// the row's block sums added to acc0 in block order; every lane of the workgroup must call it (it
+// synchronises the workgroup). H lanes share a block, each
 // taking 6 / H of its rows (FP64 blocks: two lanes of 144 bytes each keep more blocks in flight)
 template <typename T, bool HALF>
 __device__ __forceinline__ T lpb_rows(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                       const void* __restrict__ A, const T* __restrict__ x, int row0, int nrows,
-                                      bool on, int row, int q) {
+                                      bool on, int row, int q, T acc0 = T(0)) {
   constexpr int H = (!HALF && sizeof(T) == 8) ? 2 : 1;
   constexpr int NQ = 6 / H, PER = kWG / H;
   __shared__ T yb[6 * PER];
@@ -263,7 +264,7 @@ __device__ __forceinline__ T lpb_rows(const int32_t* __restrict__ rowptr, const 
     rb1 = ldg(rowptr + row + 1);
   }
   const int lb = int(threadIdx.x) / H, q0 = (int(threadIdx.x) % H) * NQ;
-  T acc = T(0);
+  T acc = on ? acc0 : T(0);
   for (int c = k0; c < k1; c += PER) {
     const int kb = c + lb;
     if (kb < k1) {
@@ -360,51 +361,24 @@ __device__ __forceinline__ void finish_partials(double v0, double v1, double* pa
 }
 
 // the sum of a graph's per-tile partials (its tiles' slots k0 .. k1), in the same order wherever run
-// (each lane's strided sum in slot order, its loads four at a time; then the workgroup's fixed order)
+// (each lane's strided sum in slot order, its loads eight at a time; then the workgroup's fixed order)
 __device__ __forceinline__ double graph_sum(int k0, int k1, const double* partial, int which) {
   __shared__ double lds[4];
   double a = 0.0;
   int k = k0 + int(threadIdx.x);
-  for (; k + 3 * kWG < k1; k += 4 * kWG) {
-    const double v0 = partial[2 * k + which], v1 = partial[2 * (k + kWG) + which];
-    const double v2 = partial[2 * (k + 2 * kWG) + which], v3 = partial[2 * (k + 3 * kWG) + which];
-    a += v0;
-    a += v1;
-    a += v2;
-    a += v3;
+  for (; k + 7 * kWG < k1; k += 8 * kWG) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partial[2 * (k + u * kWG) + which];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a += v[u];
   }
   for (; k < k1; k += kWG) a += partial[2 * k + which];
   return block_sum(a, lds);
 }
 
-// True in every lane of the workgroup that finishes the last of graph g's level-0 tiles in this
-// launch (counted in cnt[g], which that workgroup sets back to 0 for the next launch): it then runs
-// the graph's per-graph step (the sum of the partials, the scalars) in place of a launch of its own.
-// The partial of this workgroup was written by its lane 0, which fences it before counting.
-__device__ __forceinline__ bool last_tile_done(int32_t* __restrict__ cnt, int g, const int32_t* __restrict__ gtile) {
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const int n = gtile[g + 1] - gtile[g];
-    const int old = atomicAdd(cnt + g, 1);
-    last = old == n - 1 ? 1 : 0;
-    if (last) {
-      cnt[g] = 0;
-      __threadfence();
-    }
-  }
-  __syncthreads();
-  return last != 0;
-}
-
 }  // namespace
 
-__device__ __forceinline__ void finish_begin_g(const GraphDev& g, int k0, int k1, int mixed, const double* __restrict__ partial);
-__device__ __forceinline__ void finish_start_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
-__device__ __forceinline__ void finish_alpha_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
-__device__ __forceinline__ void finish_rn_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
-__device__ __forceinline__ void finish_beta_g(const GraphDev& g, int k0, int k1, int mode, const double* __restrict__ partial);
-__device__ __forceinline__ void finish_outer_g(const GraphDev& g, int k0, int k1, const double* __restrict__ partial);
 
 // ---- V-cycle kernels ---------------------------------------------------------------------------
 
@@ -527,6 +501,15 @@ __device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl,
   T acc = T(0);
   bool half = false;
   if constexpr (std::is_same<T, float>::value) half = F.R16 != nullptr;
+  if constexpr (S == 1) {
+    // (the FP16 transposed blocks in restriction order are a matrix by coarse row: a block a lane)
+    if (half) {
+      acc = T(lpb_rows<float, true>(F.Rrow, F.Rfine, F.R16, reinterpret_cast<const float*>(af.r), tl.row0, tl.nrows,
+                                    on, I, q));
+      if (on) ac.b[6 * size_t(I) + q] = acc;
+      return;
+    }
+  }
   if (on && half) {
     // (the FP16 transposed blocks in restriction order: row q of each, no Rent indirection)
     constexpr int U = 4;
@@ -590,7 +573,8 @@ __global__ void __launch_bounds__(kWG) k_restrict(const GraphDev* __restrict__ G
   restrict_tile<T, S>(g, tl, level, inner);
 }
 
-// x += P x_c (x_c: the coarse level's result, x[1])
+// x += P x_c (x_c: the coarse level's result, x[1]): the prolongator's rows a block a lane
+// (lpb_rows from x), each row's entries added to x in entry order
 template <typename T>
 __device__ __forceinline__ void prolong_tile(const GraphDev& g, const Tile& tl, int level, int buf, int inner) {
   const LevelDev& F = g.lv[level];
@@ -598,57 +582,19 @@ __device__ __forceinline__ void prolong_tile(const GraphDev& g, const Tile& tl, 
   const T* xc = arrays<T>(g.lv[level + 1]).x[1];
   int rl, q;
   lane_row(rl, q);
-  if (rl < 0 || rl >= tl.nrows) return;
-  const int i = tl.row0 + rl;
-  T acc = af.x[buf][6 * size_t(i) + q];
-  // (entries in groups of four, loads first: see spmv_row_part)
-  constexpr int U = 4;
-  const int e0 = ldg(F.Prow + i), e1 = ldg(F.Prow + i + 1);
+  const bool on = rl >= 0 && rl < tl.nrows;
+  const int i = tl.row0 + (on ? rl : 0);
+  const size_t e = 6 * size_t(i) + q;
+  const T x0 = on ? af.x[buf][e] : T(0);
   bool half = false;
   if constexpr (std::is_same<T, float>::value) half = F.P16 != nullptr;
-  if (half) {
-    for (int e = e0; e < e1; e += U) {
-      int ee[U], J[U];
-      uint32_t w[U][3];
-      float sc[U], v[U][6];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        ee[u] = min(e + u, e1 - 1);
-        J[u] = ldg(F.Pcol + ee[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        half_row_load(F.P16 + size_t(kHalfBlock) * size_t(ee[u]), q, w[u], &sc[u]);
-        load6(reinterpret_cast<const float*>(xc) + 6 * size_t(J[u]), v[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (e + u < e1) acc += T(half_row_dot(w[u], sc[u], v[u]));
-    }
-    af.x[buf][6 * size_t(i) + q] = acc;
-    return;
-  }
-  for (int e = e0; e < e1; e += U) {
-    int ee[U], J[U];
-    T m[U][6], v[U][6];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      ee[u] = min(e + u, e1 - 1);
-      J[u] = ldg(F.Pcol + ee[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      load6(af.P + 36 * size_t(ee[u]) + 6 * q, m[u]);
-      load6(xc + 6 * size_t(J[u]), v[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (e + u < e1)
-        acc += m[u][0] * v[u][0] + m[u][1] * v[u][1] + m[u][2] * v[u][2] + m[u][3] * v[u][3] + m[u][4] * v[u][4] +
-               m[u][5] * v[u][5];
-  }
-  af.x[buf][6 * size_t(i) + q] = acc;
-
+  T acc;
+  if (half)
+    acc = T(lpb_rows<float, true>(F.Prow, F.Pcol, F.P16, reinterpret_cast<const float*>(xc), tl.row0, tl.nrows, on, i,
+                                  q, float(x0)));
+  else
+    acc = lpb_rows<T, false>(F.Prow, F.Pcol, af.P, xc, tl.row0, tl.nrows, on, i, q, x0);
+  if (on) af.x[buf][e] = acc;
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_prolong(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
@@ -751,16 +697,16 @@ __global__ void __launch_bounds__(64) k_dense_solve_span(const GraphDev* __restr
   const int lane = threadIdx.x, i = tl.row0 + lane;
   double s = 0.0;
   if (i < m) {
+    // (32 loads in flight, the sum in order)
     const double* col = L.Minv + i;
-    int j = j0;
-    for (; j + 8 <= j1; j += 8) {
-      double v[8];
+    for (int jb = j0; jb < j1; jb += 32) {
+      double v[32];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = ldg(col + size_t(j + u) * size_t(m));
+      for (int u = 0; u < 32; ++u) v[u] = jb + u < j1 ? ldg(col + size_t(jb + u) * size_t(m)) : 0.0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) s += v[u] * sb[j + u - j0];
+      for (int u = 0; u < 32; ++u)
+        if (jb + u < j1) s += v[u] * sb[jb + u - j0];
     }
-    for (; j < j1; ++j) s += ldg(col + size_t(j) * size_t(m)) * sb[j - j0];
   }
   double* pc = part + size_t(c) * kDenseSpans * 64;
   pc[sp * 64 + lane] = s;
@@ -846,8 +792,7 @@ __device__ __forceinline__ void pcg_spmv_tile(const GraphDev& g, const Tile& tl,
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                  int ntiles, int inner, double* __restrict__ partial, ProfAcc pa,
-                                                  Fuse fu) {
+                                                  int ntiles, int inner, double* __restrict__ partial, ProfAcc pa) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
@@ -855,8 +800,6 @@ __global__ void __launch_bounds__(kWG) k_pcg_spmv(const GraphDev* __restrict__ G
   if (g.resident || !live(g, inner)) return;
   if (pa.acc && tl.row0 == 0 && threadIdx.x == 0) atomicAdd(pa.acc, pa.gb[tl.g]);
   pcg_spmv_tile<T>(g, tl, inner, partial);
-  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
-    finish_alpha_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // FP64-product inner solves (mixed precision): the FP32 V-cycle's input r_i = (float) r is written
@@ -893,15 +836,13 @@ __device__ __forceinline__ void axpy_rr_tile(const GraphDev& g, const Tile& tl, 
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_axpy_rr(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int inner, double* __restrict__ partial, Fuse fu) {
+                                                 int ntiles, int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   axpy_rr_tile<T>(g, tl, inner, partial);
-  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
-    finish_rn_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // p = z + beta p
@@ -944,15 +885,13 @@ __device__ __forceinline__ void dot_rz_tile(const GraphDev& g, const Tile& tl, i
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_dot_rz(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                int ntiles, int inner, double* __restrict__ partial, Fuse fu) {
+                                                int ntiles, int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   dot_rz_tile<T>(g, tl, inner, partial);
-  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
-    finish_beta_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // partial r . r
@@ -1021,16 +960,13 @@ __device__ __forceinline__ void begin_residual_tile(const GraphDev& g, const Til
 
 }
 __global__ void __launch_bounds__(kWG) k_begin_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                        int ntiles, double* __restrict__ partial, int mixed,
-                                                        Fuse fu) {
+                                                        int ntiles, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, SERVE_NEW)) return;
   begin_residual_tile(g, tl, partial);
-  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
-    finish_begin_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], mixed, partial);
 }
 
 // the solve's start, per graph: |b|, a zero or non-finite load (x = 0, r = 0); a mixed-precision
@@ -1092,15 +1028,13 @@ __device__ __forceinline__ void start_p_tile(const GraphDev& g, const Tile& tl, 
 }
 template <typename T>
 __global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                 int ntiles, int inner, double* __restrict__ partial, Fuse fu) {
+                                                 int ntiles, int inner, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, inner)) return;
   start_p_tile<T>(g, tl, inner, partial);
-  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
-    finish_start_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], inner, partial);
 }
 
 // ---- per-graph finishes (one workgroup a graph; partials summed in tile order) ----------------
@@ -1269,15 +1203,13 @@ __device__ __forceinline__ void outer_residual_tile(const GraphDev& g, const Til
 
 }
 __global__ void __launch_bounds__(kWG) k_outer_residual(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
-                                                        int ntiles, double* __restrict__ partial, Fuse fu) {
+                                                        int ntiles, double* __restrict__ partial) {
   const int t = xcd_tile(ntiles);
   if (t >= ntiles) return;
   const Tile tl = tiles[t];
   const GraphDev& g = G[tl.g];
   if (g.resident || !live(g, SERVE_REFINE)) return;
   outer_residual_tile(g, tl, partial);
-  if (fu.cnt && last_tile_done(fu.cnt, tl.g, fu.gtile))
-    finish_outer_g(g, fu.gtile[tl.g], fu.gtile[tl.g + 1], partial);
 }
 
 // outer check: converged, or an inner solve of K d = r to max(eta |r|, target / 2)
@@ -1628,10 +1560,9 @@ void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntil
   hipLaunchKernelGGL(k_jacobi<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner);
 }
 template <typename T>
-void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                   Fuse fu) {
+void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_dot_rz<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, fu);
+  hipLaunchKernelGGL(k_dot_rz<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
 }
 template <typename T>
 void launch_dot_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
@@ -1646,16 +1577,14 @@ void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col,
 }
 template <typename T>
 void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                     ProfAcc pa, Fuse fu) {
+                     ProfAcc pa) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa,
-                     fu);
+  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa);
 }
 template <typename T>
-void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                    Fuse fu) {
+void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_axpy_rr<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, fu);
+  hipLaunchKernelGGL(k_axpy_rr<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
 }
 template <typename T>
 void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner) {
@@ -1663,16 +1592,14 @@ void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int nt
   hipLaunchKernelGGL(k_update_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner);
 }
 template <typename T>
-void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                    Fuse fu) {
+void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_start_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, fu);
+  hipLaunchKernelGGL(k_start_p<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial);
 }
 
-void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial, int mixed,
-                           Fuse fu) {
+void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_begin_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial, mixed, fu);
+  hipLaunchKernelGGL(k_begin_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
 }
 void launch_release_wait64(hipStream_t s, const GraphDev* G, int count) {
   if (count > 0) hipLaunchKernelGGL(k_release_wait64, dim3((count + 255) / 256), dim3(256), 0, s, G, count);
@@ -1707,9 +1634,9 @@ void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int3
       break;
   }
 }
-void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial, Fuse fu) {
+void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_outer_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial, fu);
+  hipLaunchKernelGGL(k_outer_residual, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, partial);
 }
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;
@@ -1742,13 +1669,13 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
   template void launch_dense_solve<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, double*, int32_t*); \
   template void launch_jacobi<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                               \
-  template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, Fuse);                \
+  template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_dot_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_spmv_plain<T>(hipStream_t, const int32_t*, const int32_t*, const T*, const T*, T*, int);      \
-  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc, Fuse);    \
-  template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, Fuse);               \
+  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc);          \
+  template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                     \
   template void launch_update_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                             \
-  template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, Fuse);
+  template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);
 SVXG_INST(double)
 SVXG_INST(float)
 

@@ -20,13 +20,6 @@ struct ProfAcc {
   const double* gb = nullptr;
   double* acc = nullptr;
 };
-// A fused launch of a PCG vector kernel: the workgroup that finishes the last of a graph's level-0
-// tiles (counted per graph in cnt, which it sets back to 0) runs the graph's per-graph step - the
-// finish kernel that would follow - itself. cnt null: not fused. gtile: the graphs' first tiles.
-struct Fuse {
-  int32_t* cnt = nullptr;
-  const int32_t* gtile = nullptr;
-};
 // S: six-lane sub-groups a row (1: 10 rows a wavefront; 10: one row a wavefront, for long rows)
 template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
@@ -48,23 +41,19 @@ template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
 template <typename T>
 void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                     ProfAcc pa = {}, Fuse fu = {});
+                     ProfAcc pa = {});
 template <typename T>
-void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                    Fuse fu = {});
+void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 template <typename T>
 void launch_update_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
 template <typename T>
-void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                    Fuse fu = {});
+void launch_start_p(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 template <typename T>
-void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                   Fuse fu = {});
+void launch_dot_rz(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 template <typename T>
 void launch_dot_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 
-void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial,
-                           int mixed = 0, Fuse fu = {});
+void launch_begin_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial);
 // (mixed: the graphs go on to their first refinement step)
 void launch_finish_begin(hipStream_t s, const GraphDev* G, const int32_t* gtile, int count, int mixed,
                          const double* partial);
@@ -73,8 +62,7 @@ void launch_release_wait64(hipStream_t s, const GraphDev* G, int count);
 void launch_zero_load(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 void launch_finish(hipStream_t s, FinishKind kind, const GraphDev* G, const int32_t* gtile, int count, int inner,
                    const double* partial);
-void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial,
-                           Fuse fu = {});
+void launch_outer_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, double* partial);
 void launch_inner_init(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles);
 // (kernel timing) acc += the bytes (gbytes per graph) of the graphs live for `inner`, read before the
 // next launch changes their states
