@@ -206,6 +206,16 @@ int svxg_problem_bond_loads(svxg_problem* p, const double* u, svxg_bond_load* ou
 /* bond_utilization of every bond (stress.hpp:57,66; stress.cpp:14-57) after its bond_load:
    phi (1 = at strength) and the FailMode (0 None, 1 Tension, 2 Crush, 3 Shear). u as above. */
 int svxg_problem_bond_checks(svxg_problem* p, const double* u, double fragility, double* phi, uint8_t* mode);
+/* svxg_problem_bond_checks over device arrays, allocating no device memory. No counterpart in the
+   reference: for callers that cannot allocate (code under HIP graph capture, tight serving loops).
+   u: device, 6 doubles a node over all nodes, read in place (null: the resident solution of the last
+   solve, which this call leaves as it is); phi: device, one double a bond; mode: device, one byte a
+   bond, or null. The kernels' scratch (the job descriptor and the bond tile table) is a fixed array
+   the library holds on each GPU until the process ends, for at most 16,777,216 bonds (SVXG_BAD_INPUT
+   beyond that); the problem must be assembled (its graph's device copy is used). Calls on one GPU
+   share that array: calls that overlap (through two contexts on one device) race. Returns when phi
+   and mode are written. */
+int svxg_problem_bond_checks_noalloc(svxg_problem* p, const double* u, double fragility, double* phi, uint8_t* mode);
 
 /* StressProblem::add_force (stress.hpp:134; stress.cpp:504-513), on a host load vector. */
 void svxg_add_force(double* f, int32_t node, const double c[3], const double F[3], const double at[3]);

@@ -18,7 +18,48 @@
 
 #include "host_common.hpp"
 
+// The allocation-free bond checks' scratch (svxg_problem_bond_checks_noalloc): the job descriptor
+// and the bond tile table of k_bond_checks. Each GPU that loads the library holds its own copy until
+// the process ends; host code reaches it only through the address hipGetSymbolAddress gives for the
+// current device (scratch_address), never through this name.
 namespace svxg {
+constexpr int64_t kNoAllocMaxBonds = int64_t(1) << 24;
+constexpr size_t kNoAllocTileBase = 256;  // (the descriptor first, then the tiles)
+constexpr size_t kNoAllocBytes = kNoAllocTileBase + size_t((kNoAllocMaxBonds + kWG - 1) / kWG) * sizeof(Tile);
+static_assert(sizeof(CheckDev) <= kNoAllocTileBase, "the check descriptor fits the scratch head");
+}  // namespace svxg
+static __device__ __align__(256) uint8_t scratch[svxg::kNoAllocBytes];
+
+namespace svxg {
+
+// this device's scratch address (device current), once per device
+static int scratch_address(int device, uint8_t** out) {
+  static std::mutex mu;
+  static std::vector<uint8_t*> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (size_t(device) >= addr.size()) addr.resize(size_t(device) + 1, nullptr);
+  if (!addr[size_t(device)]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *out = nullptr;
+      return e != hipSuccess ? int(e) : -1;
+    }
+    addr[size_t(device)] = static_cast<uint8_t*>(p);
+  }
+  *out = addr[size_t(device)];
+  return 0;
+}
+
+// the descriptor (from the argument) and the tile table of one problem's bond checks, in the scratch
+__global__ void k_noalloc_checks_prep(CheckDev c, uint8_t* __restrict__ base, int ntiles) {
+  const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t == 0) *reinterpret_cast<CheckDev*>(base) = c;
+  if (t < ntiles) {
+    Tile* tiles = reinterpret_cast<Tile*>(base + kNoAllocTileBase);
+    tiles[t] = Tile{0, t * kWG, min(kWG, c.nbonds - t * kWG), t};
+  }
+}
 
 namespace {
 thread_local std::string g_error;

@@ -166,6 +166,9 @@ struct svxg_ctx {
     double ms = 0.0;
   } prof[4];
   svxg::DevBuf prof_acc;  // 4 doubles on the device: the bytes of the live graphs of each kind's launches
+  // the device address of this device's copy of the library's fixed scratch array (the
+  // allocation-free entry point's), taken at creation with the device current
+  uint8_t* scratch = nullptr;
 };
 
 struct svxg_problem {

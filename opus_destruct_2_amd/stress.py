@@ -52,6 +52,7 @@ def lib():
             "svxg_problem_set_bond_broken": ([vp, i32, i32], ctypes.c_int),
             "svxg_problem_bond_loads": ([vp, vp, vp], ctypes.c_int),
             "svxg_problem_bond_checks": ([vp, vp, f64, vp, vp], ctypes.c_int),
+            "svxg_problem_bond_checks_noalloc": ([vp, vp, f64, vp, vp], ctypes.c_int),
             "svxg_add_force": ([vp, i32, vp, vp, vp], None),
             "svxg_extend_warm_start": ([vp, i32, vp, i32, vp, i32], None),
             "svxg_problem_info_get": ([vp, vp], ctypes.c_int),
