@@ -1,7 +1,7 @@
 // svxg_stress.hpp — a C++20 StressProblem over the device library's C ABI (svxg.h).
 //
 // The same public surface as the reference's svx::StressProblem
-// (core/include/svx/stress/stress.hpp:74-160): public `nodes`, `bonds`, `mats`; assemble(),
+// (core/include/svx/stress/stress.hpp:74-160): public `nodes`, `bonds`, `mats` and materials(); assemble(),
 // assembled(), invalidate(), preconditioner_current(), block_jacobi(), assembly_work(), release(),
 // remove_bond(), retire_nodes(), reassemble(), extend_warm_start(), appended(), free_nodes(),
 // matrix_blocks(), memory_bytes(), solver_bytes(), work_per_iteration(), solve(),
@@ -9,12 +9,22 @@
 // functions bond_strength() / bond_utilization() (stress.hpp:53-66). Callers mutate nodes / bonds and
 // then assemble(), as World does; the device copy follows on the next assemble().
 //
+// The material table is a template parameter (BasicStressProblem<Table>): any table whose
+// operator[](u8) gives a material with the fields E, G, rho, ft, fb, fc, cohesion and friction - the
+// reference's svx::MaterialTable (core/include/svx/material/material.hpp:91-113) as it is, or
+// svxg::MaterialTable below. memory_bytes() / solver_bytes() take any enum with Held / Used (the
+// reference's svx::Bytes, core/include/svx/base/mem.hpp:14). So World's call sites (`P.mats =
+// mats_.get()`, world.cpp:455; `memory_bytes(k)`, world.cpp:2493; `bond_utilization(B, L,
+// fragility, mats())`) compile unchanged against `using StressProblem =
+// svxg::BasicStressProblem<svx::MaterialTable>`.
+//
 // Header-only; link with libsvxg.so. No exceptions (the reference's libraries build without them).
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "svxg.h"
@@ -57,6 +67,44 @@ struct SBond {
 using Material = svxg_material;
 using BondLoad = svxg_bond_load;
 enum class FailMode : u8 { None = 0, Tension, Crush, Shear };
+// what a memory measure counts (the reference's Bytes, mem.hpp:10-14): Held what containers hold
+// (their capacities), Used what they use (their elements)
+enum class Bytes : u8 { Held, Used };
+
+// A material table shaped like the reference's (material.hpp:91-113): 127 ids, operator[] of an id
+// never set gives id 0's material. Default: id 0 reinforced concrete, id 1 concrete (the reference's
+// presets' strengths; a world sets its own).
+constexpr int kMaxMaterials = 127;
+class MaterialTable {
+ public:
+  MaterialTable() {
+    m_[0] = Material{30e9, 12.5e9, 2400, 1.2e6, 3.5e6, 30e6, 1.6e6, 0.7};
+    m_[1] = Material{28e9, 11.5e9, 2300, 0.8e6, 1.2e6, 25e6, 1.2e6, 0.7};
+    used_[0] = used_[1] = true;
+  }
+  // ids 0 .. size() - 1 from a list
+  explicit MaterialTable(const std::vector<Material>& list) {
+    for (size_t i = 0; i < list.size() && i < size_t(kMaxMaterials); ++i) set(u8(i), list[i]);
+  }
+  const Material& operator[](u8 id) const { return id < kMaxMaterials && used_[id] ? m_[id] : m_[0]; }
+  const Material& operator[](int id) const { return (*this)[u8(id)]; }
+  void set(u8 id, const Material& m) {
+    if (id >= kMaxMaterials) return;
+    m_[id] = m;
+    used_[id] = true;
+  }
+  bool registered(u8 id) const { return id < kMaxMaterials && used_[id]; }
+
+ private:
+  Material m_[kMaxMaterials]{};
+  bool used_[kMaxMaterials]{};
+};
+
+// the C ABI's material record of a table entry
+template <class M>
+inline svxg_material to_c_material(const M& m) {
+  return svxg_material{f64(m.E), f64(m.G), f64(m.rho), f64(m.ft), f64(m.fb), f64(m.fc), f64(m.cohesion), f64(m.friction)};
+}
 
 struct StressOptions {
   f64 rtol = 2e-3;
@@ -127,16 +175,18 @@ inline svxg_ctx* default_context() {
   return c;
 }
 
-class StressProblem {
+template <class Table = MaterialTable>
+class BasicStressProblem {
  public:
   std::vector<SNode> nodes;
   std::vector<SBond> bonds;
-  const std::vector<Material>* mats = nullptr;  // the bonds' materials (a world's table)
+  const Table* mats = nullptr;  // the bonds' materials (a world's; nullptr: the default table)
+  const Table& materials() const { return mats ? *mats : default_table(); }
 
-  StressProblem() = default;
-  StressProblem(const StressProblem&) = delete;
-  StressProblem& operator=(const StressProblem&) = delete;
-  ~StressProblem() { drop(); }
+  BasicStressProblem() = default;
+  BasicStressProblem(const BasicStressProblem&) = delete;
+  BasicStressProblem& operator=(const BasicStressProblem&) = delete;
+  ~BasicStressProblem() { drop(); }
 
   bool assemble(const StressOptions& opt = {}) {
     opt_ = opt;
@@ -145,7 +195,9 @@ class StressProblem {
     std::vector<svxg_bond> b(bonds.size());
     for (size_t i = 0; i < nodes.size(); ++i) n[i] = to_c(nodes[i]);
     for (size_t i = 0; i < bonds.size(); ++i) b[i] = to_c(bonds[i]);
-    const std::vector<Material>& m = mats ? *mats : fallback_materials();
+    // (every id of the table: an id a bond names resolves as the table resolves it)
+    std::vector<svxg_material> m(static_cast<size_t>(kMaxMaterials));
+    for (int i = 0; i < kMaxMaterials; ++i) m[size_t(i)] = to_c_material(materials()[u8(i)]);
     if (svxg_problem_create(default_context(), n.data(), i32(n.size()), b.data(), i32(b.size()), m.data(),
                             i32(m.size()), &p_) != SVXG_OK)
       return false;
@@ -165,11 +217,13 @@ class StressProblem {
     if (p_) svxg_problem_release(p_);
     for (SBond& b : bonds) b.in_k = false;
   }
+  // (the bond's own flags follow the library's: broken, and out of an assembled K - the other bonds'
+  // do not change, so a judge's picks cost no copy of the bond list each)
   void remove_bond(i32 b) {
     bonds[size_t(b)].broken = true;
     if (p_) {
       svxg_problem_remove_bonds(p_, &b, 1);
-      pull_flags();
+      if (assembled()) bonds[size_t(b)].in_k = false;
     }
   }
   void retire_nodes(const std::vector<i32>& list) {
@@ -197,10 +251,19 @@ class StressProblem {
   i32 free_nodes() const { return info().nfree; }
   i64 matrix_blocks() const { return info().blocks; }
   i64 work_per_iteration() const { return info().work_per_iteration; }
-  // (device bytes: the operator, hierarchy and solve vectors that release() frees)
-  i64 solver_bytes() const { return info().device_bytes; }
-  i64 memory_bytes() const {
-    return i64(nodes.capacity() * sizeof(SNode) + bonds.capacity() * sizeof(SBond)) + solver_bytes();
+  // (stress.cpp:341-347: the graph's vectors - capacities for Held, sizes for Used - and the solver's
+  // device bytes: the operator, hierarchy and solve vectors that release() frees, the same either way)
+  template <class K = Bytes>
+  i64 solver_bytes(K kind = K::Held) const {
+    (void)kind;
+    return info().device_bytes;
+  }
+  template <class K = Bytes>
+  i64 memory_bytes(K kind = K::Held) const {
+    const bool held = kind == K::Held;
+    return i64((held ? nodes.capacity() : nodes.size()) * sizeof(SNode) +
+               (held ? bonds.capacity() : bonds.size()) * sizeof(SBond)) +
+           solver_bytes(kind);
   }
 
   PcgResult solve(const std::vector<f64>& f, std::vector<f64>& u, f64 rtol, int maxit, bool warm) {
@@ -232,10 +295,21 @@ class StressProblem {
     u.assign(6 * nodes.size(), 0.0);
     if (p_) svxg_problem_current(p_, u.data());
   }
+  // Bond b's load from u (stress.cpp:477-502). The device computes every bond's load from u in one
+  // pass and this keeps them with a copy of u: a later call whose bond reads the same values of its
+  // two nodes in u gets its load from there (a bond's load depends on those twelve values alone), so
+  // World's loop over every bond (world.cpp:1171-1193) costs one device pass, not one a bond.
   BondLoad bond_load(i32 b, const std::vector<f64>& u) const {
-    std::vector<BondLoad> all(bonds.size());
-    if (p_) svxg_problem_bond_loads(p_, u.data(), all.data());
-    return all[size_t(b)];
+    if (!p_ || b < 0 || size_t(b) >= bonds.size() || u.size() != 6 * nodes.size()) return BondLoad{};
+    const SBond& B = bonds[size_t(b)];
+    if (!loads_.empty() && same_nodes(u, B.a) && (B.b < 0 || same_nodes(u, B.b))) return loads_[size_t(b)];
+    loads_.assign(bonds.size(), BondLoad{});
+    if (svxg_problem_bond_loads(p_, u.data(), loads_.data()) != SVXG_OK) {
+      loads_.clear();
+      return BondLoad{};
+    }
+    loads_u_ = u;
+    return loads_[size_t(b)];
   }
   // every bond's load at once (the judge loop's form: one device pass)
   std::vector<BondLoad> bond_loads(const std::vector<f64>* u = nullptr) const {
@@ -250,6 +324,11 @@ class StressProblem {
   svxg_problem* handle() const { return p_; }
 
  private:
+  // node i's six values in u are those the kept loads were made from (bitwise)
+  bool same_nodes(const std::vector<f64>& u, i32 i) const {
+    return loads_u_.size() == u.size() && i >= 0 &&
+           std::memcmp(u.data() + 6 * size_t(i), loads_u_.data() + 6 * size_t(i), 6 * sizeof(f64)) == 0;
+  }
   svxg_problem_info info() const {
     svxg_problem_info i{};
     if (p_) svxg_problem_info_get(p_, &i);
@@ -271,13 +350,16 @@ class StressProblem {
     if (p_) svxg_problem_destroy(p_);
     p_ = nullptr;
     running_ = false;
+    loads_.clear();
+    loads_u_.clear();
   }
-  static const std::vector<Material>& fallback_materials() {
-    static const std::vector<Material> m = {{30e9, 12.5e9, 2400, 1.2e6, 3.5e6, 30e6, 1.6e6, 0.7},
-                                            {28e9, 11.5e9, 2300, 0.8e6, 1.2e6, 25e6, 1.2e6, 0.7}};
-    return m;
+  static const Table& default_table() {
+    static const Table t{};
+    return t;
   }
   svxg_problem* p_ = nullptr;
+  mutable std::vector<BondLoad> loads_;  // (bond_load: every bond's load from loads_u_)
+  mutable std::vector<f64> loads_u_;
   StressOptions opt_{};
   bool running_ = false;
 };
@@ -288,16 +370,18 @@ struct BondStrength {
   f64 ft, fb, fc, coh, mu;
 };
 
-inline BondStrength bond_strength(const SBond& b, f64 fragility, const std::vector<Material>& mats) {
+template <class Table>
+inline BondStrength bond_strength(const SBond& b, f64 fragility, const Table& mats) {
   const f64 s = b.strength / std::max(1e-6, fragility);
   if (b.sectioned) return {s * b.ft, s * b.fb, s * b.fc, s * b.coh, f64(b.mu)};
-  const Material& A = mats[b.ma < mats.size() ? b.ma : 0];
-  const Material& B = mats[b.mb < mats.size() ? b.mb : 0];
+  const auto& A = mats[b.ma];
+  const auto& B = mats[b.mb];
   return {s * std::min(A.ft, B.ft), s * std::min(A.fb, B.fb), s * std::min(A.fc, B.fc),
           s * std::min(A.cohesion, B.cohesion), std::min(A.friction, B.friction)};
 }
 
-inline f64 bond_utilization(const SBond& b, const BondLoad& L, f64 fragility, const std::vector<Material>& mats,
+template <class Table>
+inline f64 bond_utilization(const SBond& b, const BondLoad& L, f64 fragility, const Table& mats,
                             FailMode* mode = nullptr) {
   BondStrength S = bond_strength(b, fragility, mats);
   const f64 weak = 1e-9;
@@ -326,5 +410,7 @@ inline f64 bond_utilization(const SBond& b, const BondLoad& L, f64 fragility, co
   if (mode) *mode = phi > 0 ? m : FailMode::None;
   return phi;
 }
+
+using StressProblem = BasicStressProblem<>;
 
 }  // namespace svxg

@@ -3,6 +3,7 @@
   libsvxg.so    the device library: HIP kernels + host driver + C ABI (include/svxg.h)
   libsvxgen.so  the deterministic synthetic-workload generator (CPU)
   tests/cpp/test_stress_facade  the reference's stress tests restated over include/svxg_stress.hpp
+  tests/cpp/test_world_loops    World's loops over a structure's bonds against the same header
 
 Kernels are compiled with -ffp-contract=off (assembly, P and the bond checks reproduce the
 reference's operation order) and without packed FP32 instructions.
@@ -64,13 +65,17 @@ def build_generator():
 
 
 def build_cpp_tests():
-    """The C++ restatement of the reference's stress tests over include/svxg_stress.hpp."""
-    out = os.path.join(ROOT, "tests", "cpp", "test_stress_facade")
-    _run(["g++", "-std=c++20", "-O2", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp",
-          os.path.join(ROOT, "tests", "cpp", "test_stress_facade.cpp"), "-L", HERE, "-lsvxg",
-          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../opus_destruct_2_amd"])
-    os.replace(out + ".tmp", out)
-    return out
+    """The C++ tests over include/svxg_stress.hpp: the reference's stress tests restated, and World's
+    loops over a structure's bonds (linked with the workload generator for the block)."""
+    outs = []
+    for name, extra in (("test_stress_facade", []), ("test_world_loops", ["-lsvxgen"])):
+        out = os.path.join(ROOT, "tests", "cpp", name)
+        _run(["g++", "-std=c++20", "-O2", "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp",
+              os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-L", HERE, "-lsvxg"] + extra +
+             [f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../opus_destruct_2_amd"])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_all():

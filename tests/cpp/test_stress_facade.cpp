@@ -32,7 +32,7 @@ static std::vector<Material> presets() {
   m[1] = concrete;
   return m;
 }
-static const std::vector<Material> kMats = presets();
+static const MaterialTable kMats(presets());
 
 // test_stress.cpp:14-32
 static SBond bond(const StressProblem& P, i32 A, i32 B, const V3& n, const V3& p, f64 a) {
