@@ -510,6 +510,25 @@ typedef struct svxg_server_options {
   int32_t restart_iters;
   int32_t rebuild_iters;
   int32_t setup_workers;  /* setup workers (2), each with its own stream: groups of arrivals set up side by side */
+  /* step_structures' failure policy (world.cpp:1056-1100). Every run of a solve has a budget of
+     restart_iters iterations (0: none); at the budget, unconverged, it restarts from its current
+     iterate - on a rebuilt preconditioner when its own is stale (bonds broke since it was built) or
+     block-Jacobi, else on the same one. With restart_diverging (1), a run whose residual has grown to
+     10 x |b| on a preconditioner it may rebuild restarts at the next look on a rebuilt one, from the
+     problem's last solution (0 without one). A breakdown (a non-finite value, a direction of no
+     curvature) rebuilds the preconditioner, zeroes what is not finite in the load and the last
+     solution, retires the nodes that lost every path to a support bond (detach_unsupported,
+     world.cpp:1371-1380) and restarts from that solution. After solve_restarts (60) restarts in a
+     row the solve is abandoned: the request completes unconverged with `abandoned` set (0: no
+     restarts at all - a breakdown or a divergence is final, a run at its budget is abandoned). */
+  int32_t solve_restarts;
+  int32_t restart_diverging;
+  /* 1: each converged request is judged on its solution (World::Impl::judge, world.cpp:1111) with
+     judge_opt and state {shock: the request removed bonds, not transient, round 0}: the counters in
+     svxg_serve_done.judge, the picks through svxg_server_picks. 0 (default): not judged. */
+  int32_t judge;
+  int32_t pad_;
+  svxg_judge_options judge_opt;
 } svxg_server_options;
 
 typedef struct svxg_serve_done {
@@ -522,6 +541,11 @@ typedef struct svxg_serve_done {
   double t_ready_ms;      /* setup and loads done */
   double t_admit_ms;      /* joined the running batch */
   double t_done_ms;       /* solution written */
+  int32_t restarts;       /* runs restarted (budget, divergence, breakdown) before this result */
+  int32_t abandoned;      /* 1: left unconverged after solve_restarts restarts */
+  int32_t breakdowns;     /* runs that broke down and were restarted */
+  int32_t detached;       /* nodes retired because they lost every path to a support */
+  svxg_judge_result judge;/* the judgement of a converged request (svxg_server_options.judge) */
 } svxg_serve_done;
 
 typedef struct svxg_server_stats {
@@ -536,7 +560,11 @@ typedef struct svxg_server_stats {
   double assemble_ms;     /* of which assemblies */
   int64_t setup_groups;   /* groups of requests the setup worker took at once */
   int64_t rebuilds;
-  int64_t restarts;       /* solves restarted on a rebuilt preconditioner (world.cpp:1068-1110) */
+  int64_t restarts;       /* runs restarted (world.cpp:1068-1110) */
+  int64_t solves_abandoned; /* requests left unconverged after solve_restarts restarts (WorldStats::solves_abandoned) */
+  int64_t breakdowns;     /* runs that broke down */
+  int64_t diverged;       /* runs restarted for a growing residual */
+  int64_t judged;         /* requests judged */
 } svxg_server_stats;
 
 void svxg_server_default_options(svxg_server_options* out);
@@ -557,6 +585,9 @@ int svxg_server_submit(svxg_server* srv, svxg_problem* p, const double* f, doubl
 int svxg_server_submit_many(svxg_server* srv, int32_t count, svxg_problem* const* problems, const double* const* f,
                             double* const* u, const int32_t* warm, const int32_t* const* remove, const int32_t* nremove,
                             uint64_t* tickets);
+/* The picks of problem p's last judgement by the server (svxg_server_options.judge), in order:
+   min(count, cap) records to out, *count the number picked. Valid once its request completed. */
+int svxg_server_picks(svxg_server* srv, const svxg_problem* p, svxg_break* out, int32_t cap, int32_t* count);
 /* Completed requests, oldest first (up to cap); waits up to wait_ms for at least one. */
 int svxg_server_poll(svxg_server* srv, svxg_serve_done* out, int32_t cap, int32_t* count, int32_t wait_ms);
 /* Destroys p (its device memory returns to the budget) once no request of it is in flight. */

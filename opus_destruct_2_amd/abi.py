@@ -90,24 +90,13 @@ class ProblemInfo(ctypes.Structure):
         }
 
 
-class ServerOptions(ctypes.Structure):
-    _fields_ = [("precision", ctypes.c_int32), ("max_active", ctypes.c_int32), ("rtol", ctypes.c_double),
-                ("maxit", ctypes.c_int32), ("amg_min_nodes", ctypes.c_int32), ("window", ctypes.c_int32),
-                ("restart_iters", ctypes.c_int32), ("rebuild_iters", ctypes.c_int32), ("setup_workers", ctypes.c_int32)]
-
-
-class ServeDone(ctypes.Structure):
-    _fields_ = [("ticket", ctypes.c_uint64), ("problem", ctypes.c_void_p), ("result", PcgResult),
-                ("status", ctypes.c_int32), ("assembled", ctypes.c_int32), ("t_submit_ms", ctypes.c_double),
-                ("t_ready_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double)]
-
-
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
                 ("steps", ctypes.c_int64), ("looks", ctypes.c_int64), ("assembled", ctypes.c_int64),
                 ("max_active", ctypes.c_int64), ("step_ms", ctypes.c_double), ("rebuild_ms", ctypes.c_double),
                 ("setup_ms", ctypes.c_double), ("assemble_ms", ctypes.c_double), ("setup_groups", ctypes.c_int64),
-                ("rebuilds", ctypes.c_int64), ("restarts", ctypes.c_int64)]
+                ("rebuilds", ctypes.c_int64), ("restarts", ctypes.c_int64), ("solves_abandoned", ctypes.c_int64),
+                ("breakdowns", ctypes.c_int64), ("diverged", ctypes.c_int64), ("judged", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -154,6 +143,22 @@ class JudgeResult(ctypes.Structure):
 
     def as_dict(self):
         return {"max_phi": self.max_phi, "over": self.over, "picked": self.picked, "hinges": self.hinges}
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("max_active", ctypes.c_int32), ("rtol", ctypes.c_double),
+                ("maxit", ctypes.c_int32), ("amg_min_nodes", ctypes.c_int32), ("window", ctypes.c_int32),
+                ("restart_iters", ctypes.c_int32), ("rebuild_iters", ctypes.c_int32), ("setup_workers", ctypes.c_int32),
+                ("solve_restarts", ctypes.c_int32), ("restart_diverging", ctypes.c_int32), ("judge", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("judge_opt", JudgeOptions)]
+
+
+class ServeDone(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("problem", ctypes.c_void_p), ("result", PcgResult),
+                ("status", ctypes.c_int32), ("assembled", ctypes.c_int32), ("t_submit_ms", ctypes.c_double),
+                ("t_ready_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double),
+                ("restarts", ctypes.c_int32), ("abandoned", ctypes.c_int32), ("breakdowns", ctypes.c_int32),
+                ("detached", ctypes.c_int32), ("judge", JudgeResult)]
 
 
 class CascadeResult(ctypes.Structure):

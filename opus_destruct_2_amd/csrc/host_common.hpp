@@ -190,6 +190,7 @@ struct svxg_problem {
   std::vector<svxg::HostLevel> lv;
   svxg::DevBuf x, r, z, p, q, b, ix, ir, iz, ip, iq;
   svxg::DevBuf u_all;  // 6 per node: the last solve's displacement (device-resident)
+  std::vector<svxg_break> last_picks;  // the picks of its last judgement by a server
   svxg::DevBuf d_base, d_has_base;  // the judge's baseline loads (one svxg_bond_load a bond) and flags
   bool have_solution = false;
   bool have_load = false;  // b holds a load (a solve or a request gave one)

@@ -1581,6 +1581,21 @@ __global__ void k_scatter_u(const double* __restrict__ x, const int32_t* __restr
   u[t] = d >= 0 ? x[6 * size_t(d) + q] : 0.0;
 }
 
+// x (free DOFs) from u (all nodes): the inverse of k_scatter_u
+__global__ void k_gather_u(const double* __restrict__ u, const int32_t* __restrict__ dof, double* __restrict__ x,
+                           int nnodes) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * nnodes) return;
+  const int d = dof[t / 6];
+  if (d >= 0) x[6 * size_t(d) + t % 6] = u[t];
+}
+
+// every value of v that is not finite becomes 0
+__global__ void k_zero_nonfinite(double* __restrict__ v, int64_t n) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < n && !isfinite(v[t])) v[t] = 0.0;
+}
+
 __global__ void __launch_bounds__(kWG) k_scatter_many(const ScatterJob* __restrict__ jobs, int n) {
   for (int k = blockIdx.x; k < n; k += gridDim.x) {
     const ScatterJob j = jobs[k];
@@ -1755,6 +1770,12 @@ void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, 
 }
 void launch_scatter_many(hipStream_t s, const ScatterJob* jobs, int n) {
   if (n > 0) hipLaunchKernelGGL(k_scatter_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, jobs, n);
+}
+void launch_gather_u(hipStream_t s, const double* u, const int32_t* dof, double* x, int nnodes) {
+  if (nnodes > 0) hipLaunchKernelGGL(k_gather_u, dim3((6 * nnodes + 255) / 256), dim3(256), 0, s, u, dof, x, nnodes);
+}
+void launch_zero_nonfinite(hipStream_t s, double* v, int64_t n) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_nonfinite, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, v, n);
 }
 void launch_scatter_u(hipStream_t s, const double* x, const int32_t* dof, double* u, int nnodes) {
   if (nnodes > 0) hipLaunchKernelGGL(k_scatter_u, dim3((6 * nnodes + 255) / 256), dim3(256), 0, s, x, dof, u, nnodes);

@@ -287,6 +287,10 @@ struct CheckDev {
 void launch_bond_checks(hipStream_t s, const CheckDev* jobs, const Tile* tiles, int ntiles);
 // u over all nodes (6 per node) from the free-DOF solution
 void launch_scatter_u(hipStream_t s, const double* x, const int32_t* dof, double* u, int nnodes);
+// x (free DOFs) from u (6 per node, all nodes)
+void launch_gather_u(hipStream_t s, const double* u, const int32_t* dof, double* x, int nnodes);
+// v[i] = 0 where v[i] is not finite
+void launch_zero_nonfinite(hipStream_t s, double* v, int64_t n);
 // u (all nodes) from x (free DOFs) for many problems, one workgroup a problem at a time
 struct ScatterJob {
   const double* x;

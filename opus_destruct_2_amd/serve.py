@@ -10,19 +10,27 @@ import threading
 
 import numpy as np
 
-from .abi import SVXG_MIXED, ServeDone, ServerOptions, ServerStats, ptr
+from .abi import BREAK, SVXG_MIXED, ServeDone, ServerOptions, ServerStats, ptr
 from .stress import SvxgError, _check, default_context, lib
 
 
 class Server:
     def __init__(self, ctx=None, precision=SVXG_MIXED, max_active=4096, rtol=1e-8, maxit=5000,
-                 amg_min_nodes=160, window=8, restart_iters=120, rebuild_iters=60, setup_workers=2):
+                 amg_min_nodes=160, window=8, restart_iters=120, rebuild_iters=60, setup_workers=2,
+                 solve_restarts=60, restart_diverging=True, judge=None):
+        """judge: None (requests are not judged) or judge options (stress.default_judge_options):
+        each converged request is judged on its solution (the counters in its record, the picks
+        through picks())."""
         self.ctx = ctx or default_context()
         o = ServerOptions()
         lib().svxg_server_default_options(ctypes.byref(o))
         o.precision, o.max_active, o.rtol, o.maxit = precision, max_active, rtol, maxit
         o.amg_min_nodes, o.window = amg_min_nodes, window
         o.restart_iters, o.rebuild_iters, o.setup_workers = restart_iters, rebuild_iters, setup_workers
+        o.solve_restarts, o.restart_diverging = solve_restarts, int(bool(restart_diverging))
+        if judge is not None:
+            o.judge = 1
+            o.judge_opt = judge
         h = ctypes.c_void_p()
         _check(lib().svxg_server_create(self.ctx.h, ctypes.byref(o), ctypes.byref(h)), "svxg_server_create")
         self.h = h
@@ -87,7 +95,9 @@ class Server:
                 out[d.ticket] = {
                     "problem": problem, "u": u, "status": d.status, "assembled": bool(d.assembled),
                     "result": d.result.as_dict(), "t_submit_ms": d.t_submit_ms, "t_ready_ms": d.t_ready_ms,
-                    "t_admit_ms": d.t_admit_ms, "t_done_ms": d.t_done_ms,
+                    "t_admit_ms": d.t_admit_ms, "t_done_ms": d.t_done_ms, "restarts": d.restarts,
+                    "abandoned": bool(d.abandoned), "breakdowns": d.breakdowns, "detached": d.detached,
+                    "judge": d.judge.as_dict(),
                 }
         return out
 
@@ -102,6 +112,13 @@ class Server:
             if self.now_ms() - t0 > timeout_ms:
                 raise SvxgError(f"server: {len(want) - len(got)} requests still in flight after {timeout_ms} ms")
         return got
+
+    def picks(self, problem, cap=4096):
+        """The picks of problem's last judgement by this server (BREAK records, in order)."""
+        out = np.zeros(cap, BREAK)
+        n = ctypes.c_int32()
+        _check(lib().svxg_server_picks(self.h, problem.h, out.ctypes.data, cap, ctypes.byref(n)), "server picks")
+        return out[:min(cap, n.value)].copy()
 
     def release(self, problem):
         """Destroy `problem` once its request (if any) completes."""

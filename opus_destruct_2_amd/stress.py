@@ -79,6 +79,7 @@ def lib():
             "svxg_server_release": ([vp, vp], ctypes.c_int),
             "svxg_server_now_ms": ([vp], f64),
             "svxg_server_stats_get": ([vp, vp], ctypes.c_int),
+            "svxg_server_picks": ([vp, vp, vp, i32, ctypes.POINTER(i32)], ctypes.c_int),
             "svxg_default_judge_options": ([vp], None),
             "svxg_problem_set_judge_baseline": ([vp, vp, vp], ctypes.c_int),
             "svxg_problem_get_judge_baseline": ([vp, vp, vp], ctypes.c_int),

@@ -94,9 +94,10 @@ def test_removals_and_warm_starts_equal_the_problem_api(precision, mats, ctx):
         u1, r1 = p.solve(g.f, u=u0, rtol=1e-9, maxit=5000, warm=True)
         ref.append((u0, r0, u1, r1))
         p.close()
-    # served: a first request, then removals + warm start with the resident load (the rebuild policy
-    # off: the problem API has none)
-    srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=3, restart_iters=0, rebuild_iters=0)
+    # served: a first request, then removals + warm start with the resident load (the rebuild and
+    # failure policies off: the problem API has none)
+    srv = serve.Server(ctx, precision=precision, rtol=1e-9, window=3, restart_iters=0, rebuild_iters=0,
+                       solve_restarts=0, restart_diverging=False)
     ps = problems(gs, mats, ctx)
     first = {srv.submit(p, g.f): k for k, (p, g) in enumerate(zip(ps, gs))}
     got = srv.wait(list(first))
@@ -125,7 +126,9 @@ def test_stale_and_block_jacobi_solves_restart_on_a_rebuilt_multigrid(mats, ctx)
         rec = srv.wait([t])[t]
         st = srv.stats()
         srv.close()
-        assert rec["result"]["converged"] and st["restarts"] == 1, (rec, st)
+        # (the block-Jacobi run stops at its budget and restarts on a multigrid; the multigrid's runs
+        # restart from their iterates at the same budget until one converges)
+        assert rec["result"]["converged"] and st["restarts"] >= 1 and rec["restarts"] == st["restarts"], (rec, st)
         assert p.info()["levels"] > 1  # (a multigrid now)
         assert np.abs(rec["u"] - uo).max() <= 1e-5 * np.abs(uo).max()
         res.append(rec)
