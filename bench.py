@@ -739,7 +739,7 @@ def main():
             prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
             head = run_path(g, mats, prec, args.steps, args.warmup, ctx)
             rec = {"solve_ms": round(head["solve_ms"], 3), "iters": head["iters"],
-                   "roofline": roofline(head, args.precision, (args.nx, args.ny, args.nz))}
+                   "roofline": roofline(head, args.precision, (args.nx, args.ny, args.nz)), "autotune": ctx.tuned()}
         elif args.only == "batch":
             rec, _ = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample, args.batch_sigma)
         elif args.only == "cpu":  # (the CPU protocol on the block: every thread and one thread)
@@ -860,6 +860,8 @@ def main():
             cb.pop(k)
     line["cpu_baseline"] = cb
     line["cpu_baselines"] = cpu_all or None
+    # the autotuner's choices on the headline's context (the block, the tower, the batch)
+    line["autotune"] = ctx.tuned()
     print(json.dumps(line), flush=True)
 
 

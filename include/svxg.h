@@ -297,6 +297,32 @@ int svxg_ctx_set_resident_rows(svxg_ctx* ctx, int32_t rows);
 int svxg_ctx_set_step_graphs(svxg_ctx* ctx, int32_t on);
 int svxg_ctx_kernel_stats(svxg_ctx* ctx, int32_t kind, int64_t* launches, double* total_ms, double* bytes);
 
+/* The autotuner (no counterpart in the reference). Per level shape - what a level's launches serve:
+   its rows over the batch (log2), its entries a row, its precision and block storage - it picks the
+   tiling of the block-a-lane launches (blocks a workgroup pass: 64, 128, 256 or 512, which sets the
+   rows a workgroup takes) of the smoothing and residual, the prolongation and the FP16 restriction,
+   and whether a dense coarsest solve of few chunks runs a wavefront a span or a workgroup a chunk.
+   A shape met for the first time is tuned online on the batch's first V-cycles: each candidate's
+   launches are timed with events in turn (the same kernels on the same rows: a row's sums do not
+   depend on its tile, so the answer's bits do not change), and the fastest mean is kept for the
+   context's later batches of that shape. on = 0 uses the untuned defaults (256 blocks; split dense
+   solves for up to 64 chunks). */
+typedef struct svxg_tune_record {
+  int32_t kind;              /* 0 smoothing + residual, 1 prolongation, 2 FP16 restriction, 3 dense coarsest solve */
+  int32_t fp64;              /* the V-cycle's precision: 1 FP64, 0 FP32 */
+  int32_t half;              /* the level's blocks stored as FP16 */
+  int32_t rows_log2;         /* the level's rows over the batch, log2 (kind 3: its dense rows) */
+  int32_t per_row;           /* entries a row (kind 3: 64-row chunks) */
+  int32_t ncand;             /* candidates timed */
+  int32_t choice;            /* the index of the one kept */
+  int32_t samples;           /* launches timed a candidate */
+  int32_t cand[4];           /* kinds 0-2: blocks a pass; kind 3: 0 a workgroup a chunk, 1 a wavefront a span */
+  double us[4];              /* each candidate's mean launch time (microseconds) */
+} svxg_tune_record;
+int svxg_ctx_set_autotune(svxg_ctx* ctx, int32_t on);
+/* The shapes tuned so far: min(count, cap) records to out, *count the number tuned. */
+int svxg_ctx_tuned(svxg_ctx* ctx, svxg_tune_record* out, int32_t cap, int32_t* count);
+
 /* ---- the judge and break rounds ------------------------------------------------------------
    The World's judgement of a solved structure (World::Impl::judge, world.cpp:1162-1277) and of a
    piece's check (world_pieces.cpp:647-671): every intact bond's loads from the solution, the DIF
@@ -513,9 +539,9 @@ typedef struct svxg_server_options {
   /* step_structures' failure policy (world.cpp:1056-1100). Every run of a solve has a budget of
      restart_iters iterations (0: none); at the budget, unconverged, it restarts from its current
      iterate - on a rebuilt preconditioner when its own is stale (bonds broke since it was built) or
-     block-Jacobi, else on the same one. With restart_diverging (1), a run whose residual has grown to
-     10 x |b| on a preconditioner it may rebuild restarts at the next look on a rebuilt one, from the
-     problem's last solution (0 without one). A breakdown (a non-finite value, a direction of no
+     block-Jacobi, else on the same one. With restart_diverging (1), a run that ends at its budget with
+     a residual of 10 x |b| or more, on a preconditioner it may rebuild, restarts on a rebuilt one from
+     the problem's last solution (0 without one). A breakdown (a non-finite value, a direction of no
      curvature) rebuilds the preconditioner, zeroes what is not finite in the load and the last
      solution, retires the nodes that lost every path to a support bond (detach_unsupported,
      world.cpp:1371-1380) and restarts from that solution. After solve_restarts (60) restarts in a

@@ -519,13 +519,34 @@ struct Batch {
   int nres = 0;
   // per level: smoothing tiles (S = 1 and S = 10 sub-groups a row), prolongation tiles, restriction
   // tiles (coarse rows, S = 1 / 10), stalled-coarsest tiles (S = 1 / 10), dense-coarsest graphs
+  // the autotuner's state of one launch kind of a level (0 smoothing, 1 prolongation, 2 FP16
+  // restriction, 3 dense solve): its shape key and, while the shape is tuned, every candidate's table
+  struct Tune {
+    uint64_t key = 0;
+    int ncand = 0;       // candidates (0: not tuned here)
+    bool tuning = false;
+    int next = 0;        // the candidate the next launch takes
+    DevBuf tiles[4];
+    int nt[4] = {0, 0, 0, 0};
+  };
   struct Lv {
     DevBuf ts[2], tp, tc[2], tstall[2], dense;
     int nts[2] = {0, 0}, ntp = 0, ntc[2] = {0, 0}, ntstall[2] = {0, 0}, ndense = 0;
     DevBuf dpart, dcnt;  // (few dense chunks: the solve a wavefront a (chunk, span), its partials and counters)
+    bool dsplit = false;  // the dense solve a wavefront a span (the tuned or default choice)
+    Tune tune[4];
   };
   // dense coarsest solves of at most this many 64-row chunks a level are spread a wavefront a span
   static constexpr int kDenseSplitChunks = 64;
+  // the autotuner's timed launches not yet read (events, key, candidate); its event pairs for reuse
+  struct TunePending {
+    hipEvent_t a, b;
+    uint64_t key;
+    int cand;
+  };
+  std::vector<TunePending> tune_pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> tune_events;
+  int tuning = 0;  // level kinds still tuning (no step graph is recorded meanwhile)
   std::vector<Lv> lvl;
   DevBuf partial;
   DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
@@ -547,6 +568,14 @@ struct Batch {
   ~Batch() {
     for (hipGraphExec_t& g : step_graph)
       if (g) (void)hipGraphExecDestroy(g);
+    for (auto& e : tune_events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    for (auto& e : tune_pending) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
   }
 };
 
@@ -558,11 +587,48 @@ static bool long_rows(int64_t entries, int64_t rows) { return rows > 0 && entrie
 // rows' blocks kLpbBlocks a pass; a level's tiles hold as many rows (at most kTileRows) as fill about
 // one pass at its mean entries a row, so that long coarse rows still make many workgroups. Every row's
 // sums are the same whatever tile holds it, so this sets only the time.
+// (a pass is kWG blocks over FP32 / FP16 blocks, kWG / 2 over FP64 ones: two lanes a block; target:
+// the blocks a tile should hold, the autotuner's variable, kLpbBlocks untuned)
 constexpr int kLpbBlocks = 256;
-static int lpb_tile_rows(int64_t entries, int64_t rows) {
+static int lpb_tile_rows(int64_t entries, int64_t rows, bool fp64, int target = kLpbBlocks) {
   if (rows <= 0) return kTileRows;
   const int64_t per = std::max<int64_t>(1, (entries + rows - 1) / rows);
-  return int(std::clamp<int64_t>(kLpbBlocks / per, 1, kTileRows));
+  return int(std::clamp<int64_t>((fp64 ? target / 2 : target) / per, 1, kTileRows));
+}
+
+// ---- the autotuner's shapes -------------------------------------------------------------------------
+
+constexpr int kTuneTargets[4] = {64, 128, 256, 512};
+constexpr int kTuneDefault = 2;  // (256 blocks)
+constexpr int kTuneSamples = 3;  // launches timed a candidate
+
+static int ilog2(int64_t v) {
+  int k = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++k;
+  }
+  return k;
+}
+static uint64_t tune_key(int kind, bool fp64, bool half, int64_t rows, int64_t entries) {
+  const int64_t per = rows > 0 ? std::min<int64_t>(255, (entries + rows / 2) / rows) : 0;
+  return uint64_t(kind) | uint64_t(fp64) << 4 | uint64_t(half) << 5 | uint64_t(ilog2(rows)) << 8 | uint64_t(per) << 16;
+}
+static svxg_tune_record tune_record_of(uint64_t key) {
+  svxg_tune_record r;
+  std::memset(&r, 0, sizeof r);
+  r.kind = int32_t(key & 15);
+  r.fp64 = int32_t(key >> 4 & 1);
+  r.half = int32_t(key >> 5 & 1);
+  r.rows_log2 = int32_t(key >> 8 & 255);
+  r.per_row = int32_t(key >> 16 & 255);
+  return r;
+}
+// the cached choice of a shape (-1: none yet)
+static int tune_choice(svxg_ctx* c, uint64_t key) {
+  std::lock_guard<std::mutex> lk(c->tune_mu);
+  auto it = c->tune_choice.find(key);
+  return it == c->tune_choice.end() ? -1 : it->second;
 }
 
 static void fill_desc(svxg_problem* P, GraphDev& g, PcgState* st) {
@@ -694,19 +760,60 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     std::vector<Tile> dense;  // (64-row chunks of the dense inverses)
   };
   std::vector<LvTiles> lt(static_cast<size_t>(B.maxlev));
+  // (the V-cycle's precision: FP64 on the plain path)
+  const bool v64 = count > 0 && probs[0]->opt.precision == SVXG_FP64;
+  std::vector<std::vector<Tile>> cand_tiles;  // (the candidate tables of shapes tuned here, until the flush)
+  cand_tiles.reserve(size_t(B.maxlev) * 12);
   for (int l = 0; l < B.maxlev; ++l) {
     LvTiles& T = lt[size_t(l)];
+    Batch::Lv& V = B.lvl[size_t(l)];
+    // the level's shape a launch kind: rows and entries over the batch's tiled graphs
+    int64_t rows[3] = {0, 0, 0}, ent[3] = {0, 0, 0}, dense_rows = 0;
+    bool half = true, r16 = true;
+    std::vector<int> members;
     for (int g = 0; g < count; ++g) {
       svxg_problem* P = probs[g];
       if (!tiled[size_t(g)] || P->jacobi_only || l >= int(P->lv.size())) continue;
       const HostLevel& H = P->lv[size_t(l)];
-      const int srows = lpb_tile_rows(H.nnzb, H.n);
+      members.push_back(g);
+      half = half && H.A16.as<void>() != nullptr;
+      if (!H.terminal) {
+        rows[0] += H.n;
+        ent[0] += H.nnzb;
+        rows[1] += H.n;
+        ent[1] += H.np;
+        rows[2] += P->lv[size_t(l) + 1].n;
+        ent[2] += H.np;
+        r16 = r16 && H.R16.as<void>() != nullptr;
+      } else if (H.dense_m > 0) {
+        dense_rows += H.dense_m;
+      }
+    }
+    // each kind's blocks a tile: the shape's tuned choice, else (tuning it here, or untuned) the default
+    int target[3] = {kTuneTargets[kTuneDefault], kTuneTargets[kTuneDefault], kTuneTargets[kTuneDefault]};
+    for (int k = 0; k < 3; ++k) {
+      Batch::Tune& tu = V.tune[k];
+      if (rows[k] == 0 || (k == 2 && !r16)) continue;
+      tu.key = tune_key(k, v64, half, rows[k], ent[k]);
+      const int ch = c ? tune_choice(c, tu.key) : -1;
+      if (ch >= 0) {
+        target[k] = kTuneTargets[ch];
+      } else if (c && c->autotune) {
+        tu.ncand = 4;
+        tu.tuning = true;
+        ++B.tuning;
+      }
+    }
+    for (int g : members) {
+      svxg_problem* P = probs[g];
+      const HostLevel& H = P->lv[size_t(l)];
+      const int srows = lpb_tile_rows(H.nnzb, H.n, v64, target[0]);
       if (!H.terminal) {
         add_tiles(T.ts[0], g, H.n, srows);
-        add_tiles(T.tp, g, H.n, lpb_tile_rows(H.np, H.n));
+        add_tiles(T.tp, g, H.n, lpb_tile_rows(H.np, H.n, v64, target[1]));
         const int nc = P->lv[size_t(l) + 1].n;
         if (H.R16.as<void>() || !long_rows(H.np, H.nc))
-          add_tiles(T.tc[0], g, nc, H.R16.as<void>() ? lpb_tile_rows(H.np, nc) : tile_rows<1>());
+          add_tiles(T.tc[0], g, nc, H.R16.as<void>() ? lpb_tile_rows(H.np, nc, v64, target[2]) : tile_rows<1>());
         else
           add_tiles(T.tc[1], g, nc, tile_rows<10>());
       } else if (H.dense_m > 0) {
@@ -715,7 +822,38 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
         add_tiles(T.tstall[0], g, H.n, srows);
       }
     }
-    Batch::Lv& V = B.lvl[size_t(l)];
+    // candidate tables of the kinds tuned here (the same rows, other tilings)
+    for (int k = 0; k < 3; ++k) {
+      Batch::Tune& tu = V.tune[k];
+      if (!tu.tuning) continue;
+      for (int ci = 0; ci < tu.ncand; ++ci) {
+        cand_tiles.emplace_back();
+        std::vector<Tile>& ct = cand_tiles.back();
+        for (int g : members) {
+          const svxg_problem* P = probs[g];
+          const HostLevel& H = P->lv[size_t(l)];
+          if (H.terminal) continue;
+          const int tg = kTuneTargets[ci];
+          if (k == 0) add_tiles(ct, g, H.n, lpb_tile_rows(H.nnzb, H.n, v64, tg));
+          else if (k == 1) add_tiles(ct, g, H.n, lpb_tile_rows(H.np, H.n, v64, tg));
+          else add_tiles(ct, g, P->lv[size_t(l) + 1].n, lpb_tile_rows(H.np, P->lv[size_t(l) + 1].n, v64, tg));
+        }
+        tu.nt[ci] = int(ct.size());
+        SVXG_TRY(up.add(tu.tiles[ci], A, ct));
+      }
+    }
+    // the dense solve: a wavefront a span for few chunks (tuned against a workgroup a chunk)
+    if (!T.dense.empty() && int(T.dense.size()) <= Batch::kDenseSplitChunks) {
+      Batch::Tune& tu = V.tune[3];
+      tu.key = tune_key(3, v64, false, dense_rows, int64_t(T.dense.size()) * dense_rows);
+      const int ch = c ? tune_choice(c, tu.key) : -1;
+      V.dsplit = ch >= 0 ? ch == 1 : true;
+      if (ch < 0 && c && c->autotune) {
+        tu.ncand = 2;
+        tu.tuning = true;
+        ++B.tuning;
+      }
+    }
     for (int v = 0; v < 2; ++v) {
       V.nts[v] = int(T.ts[v].size());
       V.ntc[v] = int(T.tc[v].size());
@@ -728,7 +866,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     V.ndense = int(T.dense.size());
     SVXG_TRY(up.add(V.tp, A, T.tp));
     SVXG_TRY(up.add(V.dense, A, T.dense));
-    if (V.ndense > 0 && V.ndense <= Batch::kDenseSplitChunks) {
+    if (V.ndense > 0 && V.ndense <= Batch::kDenseSplitChunks && (V.dsplit || V.tune[3].tuning)) {
       SVXG_TRY(V.dpart.make(A, sizeof(double) * size_t(V.ndense) * kDenseSpans * 64));
       SVXG_TRY(V.dcnt.make(A, sizeof(int32_t) * size_t(V.ndense)));
       SVXG_TRY(up.add_raw(V.dcnt.as<void>(), zc.data(), sizeof(int32_t) * size_t(V.ndense)));
@@ -826,7 +964,119 @@ static int batch_degree(const svxg_problem* P) {
   return d0 | (dc << 8);
 }
 
-// the V-cycle of a batch; `degree` packs the smoother's degrees (level_degree)
+// ---- the autotuner at work -------------------------------------------------------------------------
+
+// the table a launch of a tuned kind takes now: while its shape is tuned, the candidate in turn
+struct TuneUse {
+  const Tile* t;
+  int n;
+  int cand;  // (-1: not timed)
+};
+static TuneUse tune_pick(const Batch::Tune& tu, const DevBuf& own, int nown) {
+  if (!tu.tuning) return {own.as<Tile>(), nown, -1};
+  return {tu.tiles[tu.next].as<Tile>(), tu.nt[tu.next], tu.next};
+}
+// an event pair around one timed launch (read at the next synchronisation: tune_collect)
+struct TuneTimer {
+  Batch& B;
+  hipStream_t s;
+  uint64_t key;
+  int cand;
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  TuneTimer(Batch& b, hipStream_t st, uint64_t k, int c) : B(b), s(st), key(k), cand(c) {
+    if (cand < 0) return;
+    if (!B.tune_events.empty()) {
+      ev = B.tune_events.back();
+      B.tune_events.pop_back();
+    } else if (hipEventCreate(&ev.first) != hipSuccess || hipEventCreate(&ev.second) != hipSuccess) {
+      cand = -1;
+      return;
+    }
+    (void)hipEventRecord(ev.first, s);
+  }
+  ~TuneTimer() {
+    if (cand < 0) return;
+    (void)hipEventRecord(ev.second, s);
+    B.tune_pending.push_back(Batch::TunePending{ev.first, ev.second, key, cand});
+  }
+};
+// after a V-cycle: every kind still tuned takes its next candidate
+static void tune_advance(Batch& B) {
+  if (!B.tuning) return;
+  for (Batch::Lv& V : B.lvl)
+    for (Batch::Tune& tu : V.tune)
+      if (tu.tuning) tu.next = (tu.next + 1) % tu.ncand;
+}
+
+// After a synchronisation: the timed launches' times added to their shapes' samples; a shape with
+// kTuneSamples of every candidate gets the fastest mean as its choice (kept in the context), and the
+// batch's levels of that shape take its table from now on.
+static void tune_collect(Batch& B) {
+  if (B.probs.empty() || (B.tune_pending.empty() && !B.tuning)) return;
+  svxg_ctx* c = B.probs[0]->ctx;
+  std::lock_guard<std::mutex> lk(c->tune_mu);
+  for (const Batch::TunePending& p : B.tune_pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      svxg_ctx::TuneAcc& acc = c->tune_acc[p.key];
+      acc.ms[p.cand] += ms;
+      ++acc.n[p.cand];
+    }
+    B.tune_events.emplace_back(p.a, p.b);
+  }
+  B.tune_pending.clear();
+  for (Batch::Lv& V : B.lvl)
+    for (int k = 0; k < 4; ++k) {
+      Batch::Tune& tu = V.tune[k];
+      if (!tu.tuning) continue;
+      int choice = -1;
+      auto known = c->tune_choice.find(tu.key);
+      if (known != c->tune_choice.end()) {
+        choice = known->second;  // (decided by another batch meanwhile)
+      } else {
+        const svxg_ctx::TuneAcc& acc = c->tune_acc[tu.key];
+        bool all = true;
+        for (int ci = 0; ci < tu.ncand; ++ci) all = all && acc.n[ci] >= kTuneSamples;
+        if (!all) continue;
+        svxg_tune_record rec = tune_record_of(tu.key);
+        rec.ncand = tu.ncand;
+        rec.samples = kTuneSamples;
+        double best = 0.0;
+        for (int ci = 0; ci < tu.ncand; ++ci) {
+          const double us = 1e3 * acc.ms[ci] / acc.n[ci];
+          rec.us[ci] = us;
+          rec.cand[ci] = k == 3 ? ci : kTuneTargets[ci];
+          if (choice < 0 || us < best) {
+            choice = ci;
+            best = us;
+          }
+        }
+        rec.choice = choice;
+        c->tune_choice[tu.key] = choice;
+        c->tune_log.push_back(rec);
+        c->tune_acc.erase(tu.key);
+      }
+      // (the chosen table becomes the level's own; no launch is pending: a synchronisation came first)
+      tu.tuning = false;
+      --B.tuning;
+      if (k == 0) {
+        V.ts[0] = std::move(tu.tiles[choice]);
+        V.nts[0] = tu.nt[choice];
+      } else if (k == 1) {
+        V.tp = std::move(tu.tiles[choice]);
+        V.ntp = tu.nt[choice];
+      } else if (k == 2) {
+        V.tc[0] = std::move(tu.tiles[choice]);
+        V.ntc[0] = tu.nt[choice];
+      } else {
+        V.dsplit = choice == 1;
+      }
+      for (DevBuf& d : tu.tiles) d = DevBuf();
+    }
+}
+
+// the V-cycle of a batch; `degree` packs the smoother's degrees (level_degree). Launches of kinds
+// whose shape is being tuned take the candidate tables in turn (tune_pick), one launch of each timed.
 template <typename T>
 static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   const GraphDev* G = B.dG.as<GraphDev>();
@@ -837,18 +1087,28 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   for (int l = 0; l < L; ++l) {
     Batch::Lv& V = B.lvl[size_t(l)];
     const int dg = level_degree(degree, l);
-    for (int v = 0; v < 2; ++v) {
-      if (!V.nts[v]) continue;
+    const TuneUse sm = tune_pick(V.tune[0], V.ts[0], V.nts[0]);
+    if (sm.n) {
       for (int st = 0; st < dg; ++st) {
         ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
-        launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr,
-                            SV[v], ps.pa());
+        launch_cheb_step<T>(s, G, sm.t, sm.n, l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr, 1, ps.pa());
       }
-      launch_residual<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, (dg - 1) & 1, inner, SV[v]);
+      TuneTimer tt(B, s, V.tune[0].key, sm.cand);
+      launch_residual<T>(s, G, sm.t, sm.n, l, (dg - 1) & 1, inner, 1);
     }
-    for (int v = 0; v < 2; ++v) launch_restrict<T>(s, G, V.tc[v].as<Tile>(), V.ntc[v], l, inner, SV[v]);
-    if (V.ndense)
-      launch_dense_solve<T>(s, G, V.dense.as<Tile>(), V.ndense, l, inner, V.dpart.as<double>(), V.dcnt.as<int32_t>());
+    {
+      const TuneUse rs = tune_pick(V.tune[2], V.tc[0], V.ntc[0]);
+      TuneTimer tt(B, s, V.tune[2].key, rs.cand);
+      launch_restrict<T>(s, G, rs.t, rs.n, l, inner, 1);
+    }
+    launch_restrict<T>(s, G, V.tc[1].as<Tile>(), V.ntc[1], l, inner, 10);
+    if (V.ndense) {
+      const Batch::Tune& tu = V.tune[3];
+      const bool split = tu.tuning ? tu.next == 1 : V.dsplit;
+      TuneTimer tt(B, s, tu.key, tu.tuning ? tu.next : -1);
+      launch_dense_solve<T>(s, G, V.dense.as<Tile>(), V.ndense, l, inner, split ? V.dpart.as<double>() : nullptr,
+                            split ? V.dcnt.as<int32_t>() : nullptr);
+    }
     for (int v = 0; v < 2; ++v)
       if (V.ntstall[v])
         for (int st = 0; st < kStallDegree; ++st)
@@ -857,20 +1117,23 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
   }
   for (int l = L - 2; l >= 0; --l) {
     Batch::Lv& V = B.lvl[size_t(l)];
-    if (!V.ntp) continue;
+    const TuneUse pr = tune_pick(V.tune[1], V.tp, V.ntp);
+    if (!pr.n) continue;
     const int dg = level_degree(degree, l);
     const int cur = (dg - 1) & 1;
-    launch_prolong<T>(s, G, V.tp.as<Tile>(), V.ntp, l, cur, inner);
+    {
+      TuneTimer tt(B, s, V.tune[1].key, pr.cand);
+      launch_prolong<T>(s, G, pr.t, pr.n, l, cur, inner);
+    }
+    const TuneUse sm = tune_pick(V.tune[0], V.ts[0], V.nts[0]);
     for (int j = 0; j < dg; ++j) {
       const int gs = dg + j;
-      for (int v = 0; v < 2; ++v) {
-        if (!V.nts[v]) continue;
-        ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
-        launch_cheb_step<T>(s, G, V.ts[v].as<Tile>(), V.nts[v], l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, SV[v],
-                            ps.pa());
-      }
+      if (!sm.n) continue;
+      ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
+      launch_cheb_step<T>(s, G, sm.t, sm.n, l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, 1, ps.pa());
     }
   }
+  tune_advance(B);
 }
 
 static void prof_collect(svxg_ctx* c);
@@ -878,6 +1141,7 @@ static int read_states(Batch& B, hipStream_t s) {
   SVXG_HIP(hipMemcpyAsync(B.hstate.data(), B.dstate.as<void>(), sizeof(PcgState) * size_t(B.G), hipMemcpyDeviceToHost, s));
   SVXG_HIP(hipStreamSynchronize(s));
   if (!B.probs.empty()) prof_collect(B.probs[0]->ctx);
+  tune_collect(B);
   return SVXG_OK;
 }
 
@@ -967,7 +1231,7 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
   if (!B.nt0run) return;
   const svxg_ctx* c = B.probs[0]->ctx;
   const bool record = c->step_graphs == 2 || (c->step_graphs == 1 && B.nt0run >= Batch::kGraphMinTiles && ++B.windows > 2);
-  if (record && !c->profiling && !B.graphs_failed) {
+  if (record && !c->profiling && !B.graphs_failed && !B.tuning) {
     hipGraphExec_t& ex = B.step_graph[(lean ? 8 : 0) + (mixed ? 4 : 0) + (p32 ? 2 : 0) + (p64 ? 1 : 0)];
     if (!ex) {
       // (recorded on first use; a failed recording leaves this batch to launch kernel by kernel)

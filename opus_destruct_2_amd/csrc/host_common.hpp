@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -169,6 +170,16 @@ struct svxg_ctx {
   // the device address of this device's copy of the library's fixed scratch array (the
   // allocation-free entry point's), taken at creation with the device current
   uint8_t* scratch = nullptr;
+  // the autotuner: samples by level-shape key while a shape is tuned, the choices once it is
+  int autotune = 1;
+  std::mutex tune_mu;
+  struct TuneAcc {
+    double ms[4] = {0, 0, 0, 0};
+    int n[4] = {0, 0, 0, 0};
+  };
+  std::unordered_map<uint64_t, TuneAcc> tune_acc;
+  std::unordered_map<uint64_t, int> tune_choice;
+  std::vector<svxg_tune_record> tune_log;
 };
 
 struct svxg_problem {

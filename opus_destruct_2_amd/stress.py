@@ -13,7 +13,7 @@ import numpy as np
 
 from .abi import (BOND, BOND_LOAD, NODE, SVXG_FP64, SVXG_MIXED, SVXG_OK, SVXG_SINGULAR, PcgResult, ProblemInfo,
                   StressOptions, reference_options,
-                  BREAK, CascadeResult, JudgeOptions, JudgeResult, JudgeState, ductile_mask)
+                  BREAK, CascadeResult, JudgeOptions, JudgeResult, JudgeState, TuneRecord, ductile_mask)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsvxg.so")
@@ -68,6 +68,8 @@ def lib():
             "svxg_ctx_set_profiling": ([vp, i32], ctypes.c_int),
             "svxg_ctx_set_resident_rows": ([vp, i32], ctypes.c_int),
             "svxg_ctx_set_step_graphs": ([vp, i32], ctypes.c_int),
+            "svxg_ctx_set_autotune": ([vp, i32], ctypes.c_int),
+            "svxg_ctx_tuned": ([vp, vp, i32, ctypes.POINTER(i32)], ctypes.c_int),
             "svxg_ctx_kernel_stats": ([vp, i32, ctypes.POINTER(i64), ctypes.POINTER(f64), ctypes.POINTER(f64)],
                                       ctypes.c_int),
             "svxg_server_default_options": ([vp], None),
@@ -137,6 +139,20 @@ class Context:
 
     def set_profiling(self, on):
         _check(lib().svxg_ctx_set_profiling(self.h, int(bool(on))), "set_profiling")
+
+    def set_autotune(self, on):
+        """Tile shapes tuned online per level shape (default on) or the untuned defaults; results are
+        unchanged either way."""
+        _check(lib().svxg_ctx_set_autotune(self.h, int(bool(on))), "set_autotune")
+
+    def tuned(self):
+        """The level shapes the autotuner has decided: a list of dicts (kind, shape, candidates, their
+        mean launch times in microseconds, the choice)."""
+        n = ctypes.c_int32()
+        _check(lib().svxg_ctx_tuned(self.h, None, 0, ctypes.byref(n)), "tuned")
+        buf = (TuneRecord * max(1, n.value))()
+        _check(lib().svxg_ctx_tuned(self.h, buf, n.value, ctypes.byref(n)), "tuned")
+        return [buf[k].as_dict() for k in range(n.value)]
 
     def kernel_stats(self, kind):
         """(launches, device ms, algorithmic bytes) of a profiled kernel kind since the last call."""
