@@ -547,6 +547,9 @@ struct Batch {
   std::vector<TunePending> tune_pending;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tune_events;
   int tuning = 0;  // level kinds still tuning (no step graph is recorded meanwhile)
+  // every tiled graph has a multigrid with smoothing at level 0: the PCG's r . z comes from the
+  // V-cycle's last level-0 step, run over the PCG's tiles (no dot_rz launch)
+  bool fuse_rz = false;
   std::vector<Lv> lvl;
   DevBuf partial;
   DevBuf prof_gb;  // kernel timing: 4 x G algorithmic bytes (kind-major) of one launch, per graph
@@ -744,6 +747,10 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   const std::vector<int32_t> zc(size_t(Batch::kDenseSplitChunks), 0);
   B.nt0 = int(t0.size());
   B.nt0run = int(t0run.size());
+  B.fuse_rz = !t0run.empty() && tj.empty();
+  for (int g = 0; g < count && B.fuse_rz; ++g)
+    if (tiled[size_t(g)] && (probs[g]->jacobi_only || probs[g]->lv.empty() || probs[g]->lv[0].terminal))
+      B.fuse_rz = false;
   B.ntj = int(tj.size());
   Stager up(c, s);
   SVXG_TRY(up.add(B.dG, A, B.hG));
@@ -1078,7 +1085,7 @@ static void tune_collect(Batch& B) {
 // the V-cycle of a batch; `degree` packs the smoother's degrees (level_degree). Launches of kinds
 // whose shape is being tuned take the candidate tables in turn (tune_pick), one launch of each timed.
 template <typename T>
-static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
+static void vcycle(Batch& B, hipStream_t s, int inner, int degree, bool dot = false) {
   const GraphDev* G = B.dG.as<GraphDev>();
   if (B.ntj) launch_jacobi<T>(s, G, B.tj.as<Tile>(), B.ntj, inner);
   if (!B.any_amg) return;
@@ -1129,6 +1136,12 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree) {
     for (int j = 0; j < dg; ++j) {
       const int gs = dg + j;
       if (!sm.n) continue;
+      if (dot && B.fuse_rz && l == 0 && j == dg - 1) {
+        // (the PCG's r . z partials on its own tiles: k_cheb_step_rz, not among the timed plain steps)
+        launch_cheb_step<T>(s, G, B.t0run.as<Tile>(), B.nt0run, l, j, (gs + 1) & 1, gs & 1, 0, inner,
+                            B.partial.as<double>(), 1);
+        continue;
+      }
       ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
       launch_cheb_step<T>(s, G, sm.t, sm.n, l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, 1, ps.pa());
     }
@@ -1215,8 +1228,8 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
   launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
   launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_OUTER, part);
-  vcycle<double>(B, s, SERVE_OUTER, degree);
-  launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
+  vcycle<double>(B, s, SERVE_OUTER, degree, true);
+  if (!B.fuse_rz) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
   launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_OUTER, part);
   launch_update_p<double>(s, G, t0, B.nt0run, SERVE_OUTER);
 }
@@ -1302,9 +1315,9 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
   if (p32) launch_axpy_rr<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
   if (p64) launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Rn, G, gt, B.G, SERVE_ITER, part);
-  vcycle<float>(B, s, SERVE_INNER, degree);
-  if (p32) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
-  if (p64) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
+  vcycle<float>(B, s, SERVE_INNER, degree, true);
+  if (p32 && !B.fuse_rz) launch_dot_rz<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part);
+  if (p64 && !B.fuse_rz) launch_dot_rz<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Beta, G, gt, B.G, SERVE_ITER, part);
   if (p32) launch_update_p<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT);
   if (p64) launch_update_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT);
@@ -1313,8 +1326,9 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
     if (p64) launch_start_p<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_ST, part);
     launch_finish(s, FinishKind::Start, G, gt, B.G, SERVE_START, part);
   }
-  if (p32) launch_inner_apply(s, G, t0, B.nt0run);  // (FP64-product inner solves update x themselves)
-  launch_inner_close(s, G, B.G);
+  // (FP64-product inner solves update x themselves and close in their finishes)
+  if (p32) launch_inner_apply(s, G, t0, B.nt0run);
+  if (p32) launch_inner_close(s, G, B.G);
 }
 
 // which product precisions the next window launches (states just read): FP64 products once a

@@ -385,7 +385,7 @@ __device__ __forceinline__ double graph_sum(int k0, int k1, const double* partia
 // One Chebyshev step on `level`: r = b - A x_in (b when fresh: x_in = 0), d = c1 d + c2 D^-1 r,
 // x_out = x_in + d. S: six-lane sub-groups a row. FRESH: the first step from x_in = 0 (no matrix
 // pass; its own instantiation so that timings of the two kinds of step stay apart). With
-// `partial`, the partial sums of b . x_out.
+// `partial` (level 0's last step, on the PCG's tiles), the partial sums of the PCG's r . z.
 template <typename T, int S, bool FRESH>
 __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl, int level, int step, int in_buf, int out_buf, int inner, double* __restrict__ partial) {
   const LevelDev& L = g.lv[level];
@@ -433,7 +433,19 @@ __device__ __forceinline__ void cheb_step_tile(const GraphDev& g, const Tile& tl
     xo = xin + d;
     a.x[out_buf][e] = xo;
   }
-  if (partial) finish_partials(on && sub == 0 ? double(bq) * double(xo) : 0.0, 0.0, partial, tl.slot, false);
+  if (partial) {
+    // (the V-cycle's last step at level 0, over the PCG's tiles: r . z of the PCG - the residual r of
+    // the iteration's precision, FP64 for FP64-product inner solves, times z = x_out - as dot_rz_tile)
+    double rz = 0.0;
+    if (on && sub == 0) {
+      double rp = double(bq);
+      if constexpr (std::is_same<T, float>::value) {
+        if (g.st->inner64) rp = g.r[e];
+      }
+      rz = rp * double(xo);
+    }
+    finish_partials(rz, 0.0, partial, tl.slot, false);
+  }
 
 }
 template <typename T, int S, bool FRESH>
@@ -447,6 +459,19 @@ __global__ void __launch_bounds__(kWG) k_cheb_step(const GraphDev* __restrict__ 
   if (g.resident || !live(g, inner)) return;
   if (pa.acc && tl.row0 == 0 && threadIdx.x == 0) atomicAdd(pa.acc, pa.gb[tl.g]);
   cheb_step_tile<T, S, FRESH>(g, tl, level, step, in_buf, out_buf, inner, partial);
+}
+// the V-cycle's last level-0 step with the PCG's r . z partials (its own kernel, so that the plain
+// steps' timings stay apart)
+template <typename T>
+__global__ void __launch_bounds__(kWG) k_cheb_step_rz(const GraphDev* __restrict__ G, const Tile* __restrict__ tiles,
+                                                      int ntiles, int step, int in_buf, int out_buf, int inner,
+                                                      double* __restrict__ partial) {
+  const int t = xcd_tile(ntiles);
+  if (t >= ntiles) return;
+  const Tile tl = tiles[t];
+  const GraphDev& g = G[tl.g];
+  if (g.resident || !live(g, inner)) return;
+  cheb_step_tile<T, 1, false>(g, tl, 0, step, in_buf, out_buf, inner, partial);
 }
 
 // r = b - A x
@@ -1040,6 +1065,18 @@ __global__ void __launch_bounds__(kWG) k_start_p(const GraphDev* __restrict__ G,
 // ---- per-graph finishes (one workgroup a graph; partials summed in tile order) ----------------
 
 
+// an inner solve ends (PH_APPLY); one with FP64 products updated x itself, so it closes at once: its
+// iterations count towards the solve's and the graph goes on to its next refinement step (what
+// inner_close_g does after the FP32-product corrections are applied)
+__device__ __forceinline__ void inner_end(PcgState* s) {
+  s->phase = PH_APPLY;
+  if (!s->inner64) return;
+  s->iters += s->inner_iters;
+  if (s->inner_state == ST_BREAKDOWN && s->inner_iters == 0) s->state = ST_BREAKDOWN;
+  s->inner_state = ST_IDLE;
+  s->phase = PH_OUTER;
+}
+
 __device__ __forceinline__ PcgState* finish_state(const GraphDev& g, int mode, bool& go) {
   go = live(g, mode);
   return g.st;
@@ -1063,12 +1100,12 @@ __device__ __forceinline__ void finish_start_g(const GraphDev& g, int k0, int k1
     s->phase = PH_INNER;
     if (!(rn > s->inner_target)) {
       s->inner_state = ST_CONVERGED;
-      s->phase = PH_APPLY;
       if (s->inner64) {
         s->rn = rn;
         s->rel_res = rn / s->bn;
         s->state = ST_CONVERGED;
       }
+      inner_end(s);
     }
     return;
   }
@@ -1098,13 +1135,13 @@ __device__ __forceinline__ void finish_alpha_g(const GraphDev& g, int k0, int k1
   if (inner_mode(mode)) {
     if (s->inner_iters >= s->inner_maxit) {
       s->inner_state = ST_STOPPED;
-      s->phase = PH_APPLY;
+      inner_end(s);
       return;
     }
     ++s->inner_iters;
     if (!(pq > 0)) {
       s->inner_state = ST_BREAKDOWN;
-      s->phase = PH_APPLY;
+      inner_end(s);
       return;
     }
   } else {
@@ -1138,12 +1175,12 @@ __device__ __forceinline__ void finish_rn_g(const GraphDev& g, int k0, int k1, i
     s->inner_rn = rn;
     if (rn <= s->inner_target) {
       s->inner_state = ST_CONVERGED;
-      s->phase = PH_APPLY;
       if (s->inner64) {
         s->rn = rn;
         s->rel_res = rn / s->bn;
         s->state = ST_CONVERGED;
       }
+      inner_end(s);
     }
     return;
   }
@@ -1166,7 +1203,7 @@ __device__ __forceinline__ void finish_beta_g(const GraphDev& g, int k0, int k1,
   if (!(rz1 > 0)) {
     if (inner_mode(mode)) {
       s->inner_state = ST_BREAKDOWN;
-      s->phase = PH_APPLY;
+      inner_end(s);
     } else {
       s->state = ST_BREAKDOWN;
     }
@@ -1508,18 +1545,20 @@ void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int n
                       int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa) {
   if (ntiles <= 0) return;
   const dim3 grid(grid_for(ntiles)), wg(kWG);
-  if (S == 10 && fresh)
+  if (partial)
+    hipLaunchKernelGGL(k_cheb_step_rz<T>, grid, wg, 0, s, G, tiles, ntiles, step, in_buf, out_buf, inner, partial);
+  else if (S == 10 && fresh)
     hipLaunchKernelGGL((k_cheb_step<T, 10, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
-                       partial, pa);
+                       nullptr, pa);
   else if (S == 10)
     hipLaunchKernelGGL((k_cheb_step<T, 10, false>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf,
-                       inner, partial, pa);
+                       inner, nullptr, pa);
   else if (fresh)
     hipLaunchKernelGGL((k_cheb_step<T, 1, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
-                       partial, pa);
+                       nullptr, pa);
   else
     hipLaunchKernelGGL((k_cheb_step<T, 1, false>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf,
-                       inner, partial, pa);
+                       inner, nullptr, pa);
 }
 template <typename T>
 void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
