@@ -682,6 +682,9 @@ def main():
                     help="mixed precision: graphs below this many free nodes take FP64 K products (default: the library's)")
     ap.add_argument("--resident-rows", type=int, default=-1,
                     help="graphs up to this many free nodes step in one workgroup each (-1: the library's default)")
+    ap.add_argument("--autotune", type=int, default=1, choices=[0, 1],
+                    help="tile shapes tuned online per level shape (1, the library's default) or the untuned "
+                         "defaults (0: fixed launch shapes, e.g. for counter passes)")
     ap.add_argument("--step-graphs", type=int, default=None, choices=[0, 1, 2],
                     help="--only legs: record tiled solve steps as HIP graphs (0 never, 1 large batches, 2 always; "
                          "default: the library's)")
@@ -730,6 +733,7 @@ def main():
         k, v = kv.split("=")
         AMG_KW[k.strip()] = float(v) if "." in v or "e" in v else int(v)
     ctx = stress.default_context()
+    ctx.set_autotune(bool(args.autotune))
     if args.resident_rows >= 0:
         ctx.set_resident_rows(args.resident_rows)
     if args.only is not None:
