@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one live launch of a kernel on a workload, from two rocprofv3 counter passes of this
 # script (FETCH_SIZE scaled by 2.0, the factor a plain 1 GiB copy reads it with; WRITE_SIZE as read)
-MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0, FP16 block copies)", 64, 64, 48): 117.7e6,
+MEASURED_TRAFFIC = {("k_cheb_step<float, 1, false> (level 0, FP16 block copies)", 64, 64, 48): 114.2e6,
                     ("k_cheb_step<float, 1, false> (level 0)", 64, 64, 48): 202.7e6}
 METRIC = "PCG-solved DOFs/s at 1e-8 residual, batched vs plain; p50/p99 tick latency"
 PRECISION_MIXED = ("FP64 PCG (FP64 K products, dots and vector updates) around an FP32-arithmetic V-cycle with an "
