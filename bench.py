@@ -107,6 +107,43 @@ def roofline(res, precision, shape):
             "launches": n, "avg_us": round(avg_s * 1e6, 2), "bytes_per_launch": int(per_launch)}
 
 
+def solve_bytes(info, mixed, iters, refinements, cheb_degree=2, coarse_degree=1):
+    """Algorithmic HBM bytes of one solve (each array a kernel reads or writes counted once a launch,
+    gathered x once): per PCG iteration the FP64 K product, the PCG's vector updates and the V-cycle
+    of every level (its Chebyshev steps, residual, restriction and prolongation over the level's
+    block storage: FP16 blocks of 80 bytes, else 36 values of the cycle's precision, 4 bytes of column
+    index a block; vectors of the cycle's precision), the dense coarsest solve (its FP64 inverse);
+    plus an FP64 residual a refinement. The formula of the path's algorithmic bytes per unit of work."""
+    rows, blocks, pblocks = info["level_rows"], info["level_blocks"], info["level_pblocks"]
+    L = len(rows)
+    w = 4 if mixed else 8                      # the V-cycle's vector and arithmetic precision
+    blk = 80 if info["vcycle_fp16"] else 36 * w  # a stored block
+    n0, nb0 = rows[0], blocks[0]
+    k_pass = nb0 * (288 + 4) + 4 * (n0 + 1)
+    per_it = k_pass + 2 * 6 * n0 * 8                       # q = K p: p gathered once, q written
+    per_it += 6 * n0 * (6 * 8 + (4 if mixed else 0))       # x += a p, r -= a q (x, p, r, q read; x, r written; FP32 r)
+    per_it += 6 * n0 * ((4 if mixed else 8) + 8 + 8)       # p = z + b p
+    per_it += 6 * n0 * 8                                   # r read again for r . z
+    dense = info.get("dense_rows", 0)
+    for l in range(L):
+        n, nb = rows[l], blocks[l]
+        terminal = l == L - 1
+        if terminal:
+            if dense:
+                per_it += 8 * dense * dense + 2 * dense * w
+            continue
+        dg = cheb_degree if l == 0 else coarse_degree
+        nc, npb = rows[l + 1], pblocks[l]
+        mat = nb * (blk + 4) + 4 * (n + 1)
+        step = mat + blk * n + 5 * 6 * n * w                # r = b - A x, d = c1 d + c2 D^-1 r, x += d
+        fresh = blk * n + 4 * 6 * n * w                     # from x = 0: b, D^-1 b, d and x written
+        per_it += fresh + (2 * dg - 1) * step
+        per_it += mat + 3 * 6 * n * w                       # residual
+        per_it += npb * (blk + 4) + 4 * (nc + 1) + 6 * n * w + 6 * nc * w  # restriction (transposed copies)
+        per_it += npb * (blk + 4) + 4 * (n + 1) + 6 * nc * w + 2 * 6 * n * w  # prolongation
+    return iters * per_it + refinements * (k_pass + 3 * 6 * n0 * 8)
+
+
 def copy_bandwidth(mib=1024):
     """A plain device copy's rate (bytes read + written / time, best of five) measured in a child
     process before this one touches the GPU (torch's HIP runtime and the library's cannot share a
@@ -771,6 +808,11 @@ def main():
     rl = roofline(head, args.precision, (args.nx, args.ny, args.nz))
     if rl is not None:
         rl["copy_gbs"] = copy_gbs
+        # the whole solve against the same roofline: its algorithmic bytes / its device time / peak
+        sb = solve_bytes(head["info"], head_prec == SVXG_MIXED, head["iters"], head["refinements"])
+        rl["solve_bytes"] = int(sb)
+        rl["solve_achieved"] = round(sb / (head["solve_ms"] / 1e3) / 1e9, 1)
+        rl["solve_frac"] = round(sb / (head["solve_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",

@@ -915,16 +915,17 @@ static int prof_table(Batch& B, Arena* A, hipStream_t s) {
 // with which the workgroups of each served graph's first tile add that graph's bytes (live for
 // `inner`) to the context's device counter, so launches issued after a graph stopped add time and
 // no bytes.
+// (the events go to the launch, whose own dispatch records them at the kernel's start and end:
+// t0(), t1(); no marker packets around it, so the gap before a kernel starts is not counted)
 struct ProfScope {
   svxg_ctx* c;
   const Batch& B;
   int kind, inner;
   hipStream_t s;
-  hipEvent_t e1 = nullptr;
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   ProfScope(const Batch& b, int k, hipStream_t st, int in) : c(b.probs[0]->ctx), B(b), kind(k), inner(in), s(st) {
     if (!c->profiling || kind < 0 || !c->prof_acc.as<void>() || !B.prof_gb.as<void>()) return;
     auto& P = c->prof[kind];
-    std::pair<hipEvent_t, hipEvent_t> ev;
     if (!P.spare.empty()) {
       ev = P.spare.back();
       P.spare.pop_back();
@@ -932,17 +933,14 @@ struct ProfScope {
       (void)hipEventCreate(&ev.first);
       (void)hipEventCreate(&ev.second);
     }
-    (void)hipEventRecord(ev.first, s);
     P.pending.push_back(ev);
-    e1 = ev.second;
   }
   ProfAcc pa() const {
-    if (!e1) return {};
+    if (!ev.first) return {};
     return {B.prof_gb.as<double>() + size_t(kind) * size_t(B.G), c->prof_acc.as<double>() + kind};
   }
-  ~ProfScope() {
-    if (e1) (void)hipEventRecord(e1, s);
-  }
+  hipEvent_t t0() const { return ev.first; }
+  hipEvent_t t1() const { return ev.second; }
 };
 
 // after a synchronisation: the pending pairs' times added up
@@ -1098,7 +1096,8 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree, bool dot = fa
     if (sm.n) {
       for (int st = 0; st < dg; ++st) {
         ProfScope ps(B, l == 0 && st > 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
-        launch_cheb_step<T>(s, G, sm.t, sm.n, l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr, 1, ps.pa());
+        launch_cheb_step<T>(s, G, sm.t, sm.n, l, st, (st + 1) & 1, st & 1, st == 0, inner, nullptr, 1, ps.pa(), ps.t0(),
+                            ps.t1());
       }
       TuneTimer tt(B, s, V.tune[0].key, sm.cand);
       launch_residual<T>(s, G, sm.t, sm.n, l, (dg - 1) & 1, inner, 1);
@@ -1143,7 +1142,7 @@ static void vcycle(Batch& B, hipStream_t s, int inner, int degree, bool dot = fa
         continue;
       }
       ProfScope ps(B, l == 0 ? (sizeof(T) == 4 ? 3 : 2) : -1, s, inner);
-      launch_cheb_step<T>(s, G, sm.t, sm.n, l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, 1, ps.pa());
+      launch_cheb_step<T>(s, G, sm.t, sm.n, l, j, (gs + 1) & 1, gs & 1, 0, inner, nullptr, 1, ps.pa(), ps.t0(), ps.t1());
     }
   }
   tune_advance(B);
@@ -1223,7 +1222,7 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
   double* part = B.partial.as<double>();
   {
     ProfScope ps(B, 0, s, SERVE_OUTER);
-    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa());
+    launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_OUTER, part, ps.pa(), ps.t0(), ps.t1());
   }
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_OUTER, part);
   launch_axpy_rr<double>(s, G, t0, B.nt0run, SERVE_OUTER, part);
@@ -1308,7 +1307,7 @@ static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, 
   }
   {
     ProfScope ps(B, p32 ? 1 : -1, s, SERVE_P32 | SERVE_IT);
-    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa());
+    if (p32) launch_pcg_spmv<float>(s, G, t0, B.nt0run, SERVE_P32 | SERVE_IT, part, ps.pa(), ps.t0(), ps.t1());
   }
   if (p64) launch_pcg_spmv<double>(s, G, t0, B.nt0run, SERVE_P64 | SERVE_IT, part);
   launch_finish(s, FinishKind::Alpha, G, gt, B.G, SERVE_ITER, part);

@@ -10,6 +10,7 @@
 //   solve_coarsest amg.cpp:580-592    -> k_dense_solve (the coarsest level's inverse, FP64)
 //   StressProblem::iterate            -> k_pcg_spmv, k_axpy_rr, k_update_p, k_finish_*
 //     stress.cpp:403-464                 (the dots: per-tile partials summed per graph in order)
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 #include <type_traits>
@@ -1542,10 +1543,14 @@ int grid_for(int ntiles) {
 
 template <typename T>
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
-                      int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa) {
+                      int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa, hipEvent_t t0,
+                      hipEvent_t t1) {
   if (ntiles <= 0) return;
   const dim3 grid(grid_for(ntiles)), wg(kWG);
-  if (partial)
+  if ((t0 || t1) && S == 1 && !fresh && !partial)
+    hipExtLaunchKernelGGL((k_cheb_step<T, 1, false>), grid, wg, 0, s, t0, t1, 0, G, tiles, ntiles, level, step, in_buf,
+                          out_buf, inner, (double*)nullptr, pa);
+  else if (partial)
     hipLaunchKernelGGL(k_cheb_step_rz<T>, grid, wg, 0, s, G, tiles, ntiles, step, in_buf, out_buf, inner, partial);
   else if (S == 10 && fresh)
     hipLaunchKernelGGL((k_cheb_step<T, 10, true>), grid, wg, 0, s, G, tiles, ntiles, level, step, in_buf, out_buf, inner,
@@ -1616,9 +1621,13 @@ void launch_spmv_plain(hipStream_t s, const int32_t* rowptr, const int32_t* col,
 }
 template <typename T>
 void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                     ProfAcc pa) {
+                     ProfAcc pa, hipEvent_t t0, hipEvent_t t1) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa);
+  if (t0 || t1)
+    hipExtLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, t0, t1, 0, G, tiles, ntiles, inner,
+                          partial, pa);
+  else
+    hipLaunchKernelGGL(k_pcg_spmv<T>, dim3(grid_for(ntiles)), dim3(kWG), 0, s, G, tiles, ntiles, inner, partial, pa);
 }
 template <typename T>
 void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial) {
@@ -1702,7 +1711,7 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
 
 #define SVXG_INST(T)                                                                                                \
   template void launch_cheb_step<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int, int, int,   \
-                                    double*, int, ProfAcc);                                                         \
+                                    double*, int, ProfAcc, hipEvent_t, hipEvent_t);                                 \
   template void launch_residual<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int, int);              \
   template void launch_restrict<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                   \
   template void launch_prolong<T>(hipStream_t, const GraphDev*, const Tile*, int, int, int, int);                    \
@@ -1711,7 +1720,8 @@ void launch_inner_close(hipStream_t s, const GraphDev* G, int count) {
   template void launch_dot_rz<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_dot_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                      \
   template void launch_spmv_plain<T>(hipStream_t, const int32_t*, const int32_t*, const T*, const T*, T*, int);      \
-  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc);          \
+  template void launch_pcg_spmv<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*, ProfAcc, hipEvent_t, \
+                                   hipEvent_t);                                                                     \
   template void launch_axpy_rr<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);                     \
   template void launch_update_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int);                             \
   template void launch_start_p<T>(hipStream_t, const GraphDev*, const Tile*, int, int, double*);

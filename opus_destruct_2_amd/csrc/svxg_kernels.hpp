@@ -22,8 +22,10 @@ struct ProfAcc {
 };
 // S: six-lane sub-groups a row (1: 10 rows a wavefront; 10: one row a wavefront, for long rows)
 template <typename T>
+// (t0, t1: events the kernel's own dispatch records at its start and end - kernel timing - or null)
 void launch_cheb_step(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int step, int in_buf,
-                      int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa = {});
+                      int out_buf, int fresh, int inner, double* partial, int S, ProfAcc pa = {},
+                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 template <typename T>
 void launch_residual(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int level, int buf, int inner,
                      int S);
@@ -41,7 +43,7 @@ template <typename T>
 void launch_jacobi(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner);
 template <typename T>
 void launch_pcg_spmv(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial,
-                     ProfAcc pa = {});
+                     ProfAcc pa = {}, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 template <typename T>
 void launch_axpy_rr(hipStream_t s, const GraphDev* G, const Tile* tiles, int ntiles, int inner, double* partial);
 template <typename T>
