@@ -266,6 +266,10 @@ int svxg_assemble_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_t co
 /* K as assembled, in the reference's Bsr6 layout (amg.hpp:20-27): rowptr[nfree+1], col[blocks],
    val[36 blocks] row-major 6x6, diagonal block first in each row. Null pointers: skipped. */
 int svxg_problem_get_matrix(const svxg_problem* p, int32_t* rowptr, int32_t* col, double* val);
+/* A level's inverse diagonal blocks (Amg::finalize, amg.cpp:227-244: inv6 of each diagonal block,
+   regularised by 1e-9 x its |trace| where singular), 36 a row, row-major: the FP64 blocks where the
+   level holds them (*fp64 = 1), else its FP32 copies widened (*fp64 = 0). */
+int svxg_problem_get_diag_inverse(const svxg_problem* p, int32_t level, double* out, int32_t* fp64);
 /* y = K x on the device (Bsr6::apply, amg.cpp:174-183) over free DOFs (6 nfree each), FP64 or,
    with fp32 != 0, through the FP32 copy of K. */
 int svxg_problem_apply_matrix(svxg_problem* p, const double* x, double* y, int32_t fp32);

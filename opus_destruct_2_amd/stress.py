@@ -60,6 +60,7 @@ def lib():
             "svxg_problem_get_nodes": ([vp, vp], ctypes.c_int),
             "svxg_solve_batch": ([vp, vp, i32, vp, vp, f64, i32, i32, vp], ctypes.c_int),
             "svxg_problem_get_matrix": ([vp, vp, vp, vp], ctypes.c_int),
+            "svxg_problem_get_diag_inverse": ([vp, ctypes.c_int32, vp, vp], ctypes.c_int),
             "svxg_problem_apply_matrix": ([vp, vp, vp, i32], ctypes.c_int),
             "svxg_problem_apply_preconditioner": ([vp, vp, vp], ctypes.c_int),
             "svxg_problem_last_timing": ([vp, ctypes.POINTER(f64), ctypes.POINTER(f64)], ctypes.c_int),
@@ -329,6 +330,16 @@ class StressProblem:
         val = np.zeros((inf["blocks"], 6, 6))
         _check(lib().svxg_problem_get_matrix(self.h, rowptr.ctypes.data, col.ctypes.data, val.ctypes.data), "matrix")
         return rowptr, col, val
+
+    def diag_inverse(self, level=0):
+        """(blocks [rows, 6, 6], held in FP64) of a level's inverse diagonal (amg.cpp:227-244)."""
+        inf = self.info()
+        rows = inf["nfree"] if level == 0 else inf["level_rows"][level]  # (block-Jacobi: no hierarchy)
+        out = np.zeros((rows, 6, 6))
+        f64 = ctypes.c_int32(0)
+        _check(lib().svxg_problem_get_diag_inverse(self.h, int(level), out.ctypes.data, ctypes.byref(f64)),
+               "diag_inverse")
+        return out, bool(f64.value)
 
     def apply_matrix(self, x, fp32=False):
         x = np.ascontiguousarray(x, np.float64)

@@ -47,6 +47,7 @@ def _bind(path, pre):
         "set_threads": ([i32], None),
     }
     sig["extend_warm_start"] = ([vp, i32, vp, i32, vp, i32], None)
+    sig["inv6"] = ([vp, vp], i32)
     if pre == "orc_":
         sig.update({"get_matrix": ([vp, vp, vp, vp], None), "apply_matrix": ([vp, vp, vp], None),
                     "apply_preconditioner": ([vp, vp, vp], None), "get_nodes": ([vp, vp], None),
@@ -372,6 +373,34 @@ def cascade(problem, judge, f, u, opts, state, rtol=1e-10, maxit=20000, max_call
     return u, out
 
 
+def inv6(a, kind="oracle"):
+    """The 6x6 block inverse of the reference's level set-up (amg.cpp:105-135): (True, A^-1), or
+    (False, None) where the reference's inv6 reports the block singular."""
+    a = np.ascontiguousarray(a, np.float64).reshape(36)
+    out = np.zeros(36)
+    L, pre = library(kind)
+    ok = getattr(L, pre + "inv6")(a.ctypes.data, out.ctypes.data)
+    return (True, out.reshape(6, 6)) if ok else (False, None)
+
+
+def diag_inverse(a, kind="oracle"):
+    """A level's diagonal inverse as the reference's finalize builds it (amg.cpp:228-244): inv6 of the
+    block, else inv6 of the block with 1e-9 x the sum of its |diagonal| (1 if that is 0) added to the
+    diagonal; None if that too is singular."""
+    ok, out = inv6(a, kind)
+    if ok:
+        return out
+    r = np.array(a, np.float64).reshape(6, 6).copy()
+    tr = 0.0
+    for q in range(6):
+        tr += abs(r[q, q])
+    eps = 1e-9 * tr if tr > 0 else 1.0
+    for q in range(6):
+        r[q, q] += eps
+    ok, out = inv6(r, kind)
+    return out if ok else None
+
+
 def set_ref_threads(n):
     L, pre = library("ref")
     L.ref_set_threads(int(n))
@@ -402,4 +431,4 @@ def bond_strength(bond, fragility, mats, kind="oracle"):
 
 __all__ = ["CpuProblem", "CpuJudge", "build", "library", "ref_available", "bond_utilization", "bond_strength",
            "reference_options", "StressOptions", "set_ref_threads", "hinge_of", "lerp_load", "graph_components",
-           "cascade", "bond_sections"]
+           "cascade", "bond_sections", "inv6", "diag_inverse"]

@@ -1565,6 +1565,8 @@ void orc_get_nodes(orc_problem* h, svxg_node* out) { std::copy(h->nodes.begin(),
 void orc_set_threads(int32_t) {}
 
 // StressOptions{} / AmgOptions{} defaults (stress.hpp:68-72, amg.hpp:46-69).
+int32_t orc_inv6(const double* A, double* out) { return inv6(A, out) ? 1 : 0; }
+
 void orc_default_options(svxg_stress_options* o) {
   o->rtol = 2e-3;
   o->amg_min_nodes = 160;

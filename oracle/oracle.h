@@ -70,6 +70,9 @@ void orc_bond_sections(const svxg_node* nodes, int32_t n, const svxg_section_hea
 int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp);
 
 void orc_default_options(svxg_stress_options* out);
+/* the 6x6 block inverse of a level's diagonal (amg.cpp:105-135 blk6::inv6): 1 and out = A^-1, or 0
+   (zero or singular at the reference's pivot tolerance) */
+int32_t orc_inv6(const double* A, double* out);
 
 #ifdef __cplusplus
 }

@@ -235,6 +235,8 @@ void ref_get_bonds(ref_problem* h, svxg_bond* out) {
   for (size_t k = 0; k < h->P.bonds.size(); ++k) from_ref(h->P.bonds[k], &out[k]);
 }
 
+int32_t ref_inv6(const double* A, double* out) { return blk6::inv6(A, out) ? 1 : 0; }
+
 void ref_set_threads(int32_t n) { set_num_threads(n); }
 int32_t ref_threads(void) { return num_threads(); }
 

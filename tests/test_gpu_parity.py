@@ -12,6 +12,7 @@ Bars (the path's contract):
 import numpy as np
 import pytest
 
+import oracle
 from oracle import CpuProblem, reference_options
 from opus_destruct_2_amd import gen, stress
 from opus_destruct_2_amd.abi import SVXG_FP64, SVXG_MIXED
@@ -76,6 +77,31 @@ def test_matrix_bit_identical(case, mats, ctx):
     assert np.array_equal(rd, ro) and np.array_equal(cd, co)
     assert np.array_equal(bits(vd), bits(vo))
     assert np.array_equal(d.bonds["in_k"], o.get_bonds()["in_k"])
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("prec", [SVXG_FP64, SVXG_MIXED], ids=["fp64", "mixed"])
+def test_diag_inverse_bit_identical(case, prec, mats, ctx):
+    """Level 0's inverse diagonal blocks (Amg::finalize, amg.cpp:227-244; block-Jacobi's, stress.cpp:
+    149-157) against the reference's inv6 applied to the device's own K: bit for bit where the level
+    holds FP64 blocks (the FP64 path), the FP32 rounding of those bits where it holds FP32 copies."""
+    name, nodes, bonds, f, kw = case
+    d = stress.StressProblem(nodes, bonds, mats, ctx)
+    assert d.assemble(stress.default_options(precision=prec, **kw))
+    rowptr, col, val = d.matrix()
+    dinv, held64 = d.diag_inverse(0)
+    assert held64 == (prec == SVXG_FP64 or name.endswith("_jacobi"))
+    kind = "ref" if oracle.ref_available() else "oracle"
+    want = np.zeros_like(dinv)
+    for i in range(len(rowptr) - 1):
+        assert col[rowptr[i]] == i
+        w = oracle.diag_inverse(val[rowptr[i]], kind)
+        assert w is not None, (name, i)
+        want[i] = w
+    if held64:
+        assert np.array_equal(bits(dinv), bits(want)), name
+    else:
+        assert np.array_equal(bits(dinv), bits(want.astype(np.float32).astype(np.float64))), name
 
 
 @pytest.mark.parametrize("case", CASES, ids=IDS)

@@ -134,6 +134,66 @@ def test_rigid_motion_loads_nothing(mats):
         assert max(abs(L[k]) for k in ("N", "V1", "V2", "T", "M1", "M2")) < 1e-3
 
 
+# ---- the 6x6 block inverse (amg.cpp:105-135) and a level's diagonal inverse (:227-244) ---------------
+
+def _blocks(seed):
+    """SPD stiffness-like blocks, general blocks whose elimination pivots, and badly scaled ones."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(40):
+        m = rng.standard_normal((6, 6))
+        out.append(m @ m.T + 0.1 * np.eye(6))
+        out.append(rng.standard_normal((6, 6)))
+        out.append(np.diag(10.0 ** rng.uniform(-6, 9, 6)) @ rng.standard_normal((6, 6)))
+    return out
+
+
+def test_inv6_closed_forms():
+    # diagonal: exact reciprocals; a permutation: its transpose, exactly; a rigid-motion map
+    # [[I, -[r]x], [0, I]]: [[I, [r]x], [0, I]] exactly (its entries are small integers)
+    d = np.array([2.0, 4.0, 0.5, 8.0, 1.0, 0.25])
+    ok, inv = oracle.inv6(np.diag(d))
+    assert ok and np.array_equal(inv, np.diag(1.0 / d))
+    perm = np.eye(6)[[3, 0, 5, 1, 4, 2]]
+    ok, inv = oracle.inv6(perm)
+    assert ok and np.array_equal(inv, perm.T)
+    r = np.array([1.0, -2.0, 3.0])
+    X = np.array([[0, -r[2], r[1]], [r[2], 0, -r[0]], [-r[1], r[0], 0]])
+    Pm = np.eye(6)
+    Pm[:3, 3:] = -X
+    ok, inv = oracle.inv6(Pm)
+    want = np.eye(6)
+    want[:3, 3:] = X
+    assert ok and np.array_equal(inv, want)
+    # A A^-1 = I on well-conditioned blocks; against numpy's LU solve
+    for a in _blocks(3)[:60]:
+        ok, inv = oracle.inv6(a)
+        assert ok
+        assert np.abs(a @ inv - np.eye(6)).max() <= 1e-9 * np.linalg.cond(a)
+        assert np.allclose(inv, np.linalg.inv(a), rtol=1e-8 * np.linalg.cond(a), atol=0)
+
+
+def test_inv6_singular_rules():
+    # zero block; an exactly rank-5 block; a pivot at 1e-16 of the largest entry: all singular.
+    # At 1e-14 (above the 1e-15 pivot tolerance) it inverts.
+    assert oracle.inv6(np.zeros((6, 6)))[0] is False
+    a = np.eye(6)
+    a[5, 5] = 0.0
+    assert oracle.inv6(a)[0] is False
+    a[5, 5] = 1e-16
+    assert oracle.inv6(a)[0] is False
+    a[5, 5] = 1e-14
+    ok, inv = oracle.inv6(a)
+    assert ok and inv[5, 5] == 1e14
+    # the level's regularisation: a singular block gains 1e-9 x sum |diag| on its diagonal
+    a = np.eye(6)
+    a[5, 5] = 0.0
+    got = oracle.diag_inverse(a)
+    assert got is not None
+    assert got[5, 5] == 1.0 / 5e-9 and got[0, 0] == 1.0 / (1.0 + 5e-9)
+    assert oracle.diag_inverse(np.zeros((6, 6)))[0, 0] == 1.0  # (trace 0: +1 on the diagonal)
+
+
 # ---- the compiled reference (where it builds) ------------------------------------------------------
 
 ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (reference sources absent)")
@@ -255,3 +315,23 @@ def test_extend_warm_start_parity_and_rigid_motion():
     ur = u.reshape(-1, 6)[first:]
     assert np.allclose(ue[first:][reached], ur[reached], rtol=1e-12, atol=1e-15)
     assert np.array_equal(ue[:first], u.reshape(-1, 6)[:first])
+
+
+@ref
+def test_inv6_bit_identical_to_reference():
+    """The oracle's inv6 and the level regularisation against blk6::inv6 itself, bit for bit (and its
+    singular verdicts), on well-conditioned, pivoting, badly scaled and singular blocks."""
+    blocks = _blocks(5)
+    a = np.eye(6)
+    a[2, 2] = 0.0
+    blocks += [np.zeros((6, 6)), a, np.ones((6, 6))]
+    for k, b in enumerate(blocks):
+        ok_o, inv_o = oracle.inv6(b, "oracle")
+        ok_r, inv_r = oracle.inv6(b, "ref")
+        assert ok_o == ok_r, k
+        if ok_o:
+            assert np.array_equal(_bits(inv_o), _bits(inv_r)), k
+        do, dr = oracle.diag_inverse(b, "oracle"), oracle.diag_inverse(b, "ref")
+        assert (do is None) == (dr is None), k
+        if do is not None:
+            assert np.array_equal(_bits(do), _bits(dr)), k
