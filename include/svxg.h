@@ -456,7 +456,7 @@ int svxg_break_rounds_batch(svxg_ctx* ctx, svxg_problem* const* problems, int32_
    moments, extreme fibres and torsion radius, the spring lengths la / lb from the two centres, and
    the section strengths (each face as strong as the weaker of its two voxels' materials, times the
    condition of the more damaged; the faces' mean). One lattice, unrotated (the reference's
-   finish_lattice case); junction samples between lattices are not covered. */
+   finish_lattice case); svxg_bond_sections_general below takes placed lattices and junctions. */
 
 /* Vox (core/include/svx/world/grid.hpp:28-36): 0 air; else material id + 1 in the low 7 bits,
    0x80 for an anchored (ground) voxel. A box of voxels lo .. lo + dim - 1, x fastest; voxels
@@ -496,6 +496,66 @@ typedef struct svxg_section_face {
 int svxg_bond_sections(svxg_ctx* ctx, const svxg_node* nodes, int32_t n, const svxg_section_head* heads, int32_t m,
                        const svxg_section_face* faces, int32_t nfaces, const svxg_voxel_box* box,
                        const svxg_material* mats, int32_t nmats, svxg_bond* out);
+
+/* ---- bond sections across lattices: placed grids and junction samples -----------------------
+   The general case of the same (SecAcc::finish, world_internal.hpp:120-127, finish_general,
+   :250-362, and section_strengths_general, :402-441; called by append_nodes, world.cpp:824-852, and
+   the piece graphs, world_pieces.cpp:429-440): a bond's faces lie in one lattice (its `grid`),
+   which may be placed in the bond's frame by a rotation and an offset, and it may hold junction
+   samples - S x S points of a voxel face of one lattice that land in a voxel of another - each a
+   small square of side h / S. A bond with no samples on an unplaced lattice takes the one-lattice
+   formulas (finish_lattice) exactly as svxg_bond_sections. */
+
+/* A lattice (LatticeXf, core/include/svx/phys/rigid.hpp:85-95, with its voxels): to(s) = off + R s
+   in the frame (R row-major, as LatticeXf::make leaves it), identity = 1 for an unplaced lattice
+   (off and R unused); voxel size h; its voxels lo .. lo + dim - 1 (x fastest; outside: air). */
+typedef struct svxg_lattice {
+  double off[3];
+  double R[9];
+  double h;
+  int32_t identity;
+  int32_t lo[3];
+  int32_t dim[3];
+  int32_t pad_;
+  const uint8_t* vox;        /* dim[0] * dim[1] * dim[2] Vox codes */
+  const uint8_t* damage;     /* the same count, or null */
+} svxg_lattice;              /* 152 bytes */
+
+/* A bond's accumulator in the general case (SecHead / SecAcc, world_internal.hpp:105-125): the
+   svxg_section_head fields, its faces' lattice `grid`, and samples [sample0, sample0 + nsamples)
+   of the sample list. A support with samples (b < 0, a junction support) takes its normal from
+   them. */
+typedef struct svxg_section_head_general {
+  int32_t a, b;
+  int32_t axis, sign;
+  int32_t face0, nfaces;
+  int32_t sample0, nsamples;
+  uint16_t grid;
+  uint8_t ma, mb;
+  uint8_t pad_[4];
+  double strength_a, strength_b;
+} svxg_section_head_general; /* 56 bytes */
+
+/* A junction sample (JSample with its accumulator's jsg and jw, world_internal.hpp:78-84, 114-117):
+   sample `sub` (0 .. S*S-1) of face `face` (axis * 2 + 1 for the +axis face) of voxel v of lattice
+   vg lands in voxel o of lattice og; kind 1: in the unknown world (its strength is its own voxel's);
+   sign +1 when the face's normal points from a to b (or out of a); weight: its share of the area. */
+typedef struct svxg_junction_sample {
+  int32_t v[3], o[3];
+  uint16_t vg, og;
+  uint8_t face, sub, kind;
+  int8_t sign;
+  float weight;
+} svxg_junction_sample;      /* 36 bytes */
+
+/* out[k] (m bonds): bond k as SecAcc::finish then section_strengths(_general) make it (the fields
+   of svxg_bond_sections; faces = faces + samples). S: samples per face edge (the caller's
+   junction_samples clamped to 1..7, world.cpp:825). nodes: the graph's node centres in the frame. */
+int svxg_bond_sections_general(svxg_ctx* ctx, const svxg_node* nodes, int32_t n,
+                               const svxg_section_head_general* heads, int32_t m, const svxg_section_face* faces,
+                               int32_t nfaces, const svxg_junction_sample* samples, int32_t nsamples,
+                               const svxg_lattice* lattices, int32_t nlattices, int32_t S, const svxg_material* mats,
+                               int32_t nmats, svxg_bond* out);
 
 /* A structure's graph from one voxel lattice, all on the device (the walk of World::Impl::extract
    and append_nodes, world.cpp:425-865, for one unrotated lattice and a node a fragment): frag holds

@@ -211,6 +211,27 @@ SECTION_FACE = np.dtype([("p", "<i4", (3,)), ("axis", "i1"), ("sign", "i1"), ("p
 assert SECTION_HEAD.itemsize == 48 and SECTION_FACE.itemsize == 16
 
 
+# ---- the general case: placed lattices and junction samples (world_internal.hpp:78-125, 250-441) ----
+SECTION_HEAD_GENERAL = np.dtype([
+    ("a", "<i4"), ("b", "<i4"), ("axis", "<i4"), ("sign", "<i4"), ("face0", "<i4"), ("nfaces", "<i4"),
+    ("sample0", "<i4"), ("nsamples", "<i4"), ("grid", "<u2"), ("ma", "u1"), ("mb", "u1"), ("pad_", "u1", (4,)),
+    ("strength_a", "<f8"), ("strength_b", "<f8"),
+], align=True)
+JUNCTION_SAMPLE = np.dtype([
+    ("v", "<i4", (3,)), ("o", "<i4", (3,)), ("vg", "<u2"), ("og", "<u2"), ("face", "u1"), ("sub", "u1"),
+    ("kind", "u1"), ("sign", "i1"), ("weight", "<f4"),
+], align=True)
+assert SECTION_HEAD_GENERAL.itemsize == 56 and JUNCTION_SAMPLE.itemsize == 36
+
+
+class Lattice(ctypes.Structure):
+    """svxg_lattice: a lattice placed in the bond's frame (to(s) = off + R s; identity: unplaced) with
+    its voxels (Vox codes, x fastest)."""
+    _fields_ = [("off", ctypes.c_double * 3), ("R", ctypes.c_double * 9), ("h", ctypes.c_double),
+                ("identity", ctypes.c_int32), ("lo", ctypes.c_int32 * 3), ("dim", ctypes.c_int32 * 3),
+                ("pad_", ctypes.c_int32), ("vox", ctypes.c_void_p), ("damage", ctypes.c_void_p)]
+
+
 class VoxelBox(ctypes.Structure):
     """svxg_voxel_box: Vox codes (0 air, material id + 1, 0x80 anchored), x fastest."""
     _fields_ = [("lo", ctypes.c_int32 * 3), ("dim", ctypes.c_int32 * 3), ("h", ctypes.c_double),

@@ -371,6 +371,28 @@ struct SectionDev {
 };
 void launch_bond_sections(hipStream_t s, const SectionDev* dev, int m);
 
+// the general case: an element (face or junction sample) of a bond as a square in the bond's frame
+struct SecElem {
+  double x[3], nk[3];  // centre, normal from a to b
+  double ar, sd;       // area (times the sample's weight), side
+  double st[5];        // strength shares: ft, fb, fc, cohesion (w x condition x weaker), friction (w x weaker)
+  double w;            // strength weight: 1 a face, 1 / S^2 a sample
+};
+struct SectionGenDev {
+  const svxg_node* nodes;
+  const svxg_section_head_general* heads;
+  const svxg_section_face* faces;
+  const svxg_junction_sample* samples;
+  const svxg_lattice* lat;  // (device voxel pointers)
+  const svxg_material* mats;
+  const int64_t* eoff;      // m + 1: the bonds' element ranges (0 for a one-lattice bond)
+  SecElem* elem;
+  svxg_bond* out;
+  int64_t E;
+  int32_t m, nmats, S, pad_;
+};
+void launch_bond_sections_general(hipStream_t s, const SectionGenDev* dev, int m, int64_t elements);
+
 // ---- one lattice's graph (world.cpp:425-865 for one lattice, a node a fragment) -----------------------
 struct LatticeDev {
   const uint8_t* vox;

@@ -89,6 +89,8 @@ def lib():
             "svxg_judge_batch": ([vp, vp, i32, vp, vp, vp, vp, vp, vp], ctypes.c_int),
             "svxg_break_rounds_batch": ([vp, vp, i32, vp, vp, f64, i32, vp, i32, vp, vp, vp, vp, vp], ctypes.c_int),
             "svxg_bond_sections": ([vp, vp, i32, vp, i32, vp, i32, vp, vp, i32, vp], ctypes.c_int),
+            "svxg_bond_sections_general": ([vp, vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, vp, i32, vp],
+                                           ctypes.c_int),
             "svxg_lattice_graph": ([vp, vp, vp, i32, vp, i32, vp, ctypes.POINTER(i32), ctypes.POINTER(i32), vp, vp, vp],
                                    ctypes.c_int),
         }

@@ -19,7 +19,8 @@ import ctypes
 
 import numpy as np
 
-from .abi import BOND, NODE, SECTION_FACE, SECTION_HEAD, VoxelBox
+from .abi import (BOND, JUNCTION_SAMPLE, NODE, SECTION_FACE, SECTION_HEAD, SECTION_HEAD_GENERAL, Lattice,
+                  VoxelBox)
 
 ANCHOR = 0x80
 
@@ -201,6 +202,247 @@ def lattice_graph(vox, frag, lo, h, mats, damage=None, ctx=None):
                                        nodes.ctypes.data, ctypes.byref(nb), ctypes.byref(nf), heads.ctypes.data,
                                        faces.ctypes.data, bonds.ctypes.data), "svxg_lattice_graph")
     return nodes, heads[:nb.value], faces[:nf.value], bonds[:nb.value]
+
+
+# ---- several lattices: placed grids and junctions (docs of the reference: GRIDS) ---------------------
+
+class Grid:
+    """A voxel lattice placed in the frame (LatticeXf, core/include/svx/phys/rigid.hpp:85-95):
+    to(s) = off + R s; R None: unplaced (the identity). vox, damage: (z, y, x) arrays over the box
+    from lo, Vox codes."""
+
+    def __init__(self, vox, lo, h, R=None, off=None, damage=None):
+        self.vox = np.ascontiguousarray(vox, np.uint8)
+        self.lo = np.asarray(lo, np.int32)
+        self.h = float(h)
+        self.identity = R is None and (off is None or not np.any(off))
+        self.R = np.eye(3) if R is None else np.asarray(R, np.float64)
+        self.off = np.zeros(3) if off is None else np.asarray(off, np.float64)
+        self.damage = None if damage is None else np.ascontiguousarray(damage, np.uint8)
+
+    def to(self, s):
+        s = np.asarray(s, np.float64)
+        return s if self.identity else s @ self.R.T + self.off
+
+    def dir_to(self, d):
+        d = np.asarray(d, np.float64)
+        return d if self.identity else d @ self.R.T
+
+    def from_frame(self, x):
+        x = np.asarray(x, np.float64)
+        return x if self.identity else (x - self.off) @ self.R
+
+    def at(self, p):
+        """Vox codes at integer points p (k, 3); air outside the box."""
+        p = np.asarray(p, np.int64)
+        nz, ny, nx = self.vox.shape
+        r = p - self.lo
+        inside = (r >= 0).all(1) & (r[:, 0] < nx) & (r[:, 1] < ny) & (r[:, 2] < nz)
+        out = np.zeros(len(p), np.uint8)
+        out[inside] = self.vox[r[inside, 2], r[inside, 1], r[inside, 0]]
+        return out
+
+    def lattice(self):
+        L = Lattice()
+        for k in range(3):
+            L.off[k] = float(self.off[k])
+            L.lo[k] = int(self.lo[k])
+        for k in range(9):
+            L.R[k] = float(self.R.reshape(9)[k])
+        L.h = self.h
+        L.identity = int(bool(self.identity))
+        nz, ny, nx = self.vox.shape
+        L.dim[0], L.dim[1], L.dim[2] = nx, ny, nz
+        L.vox = self.vox.ctypes.data
+        L.damage = None if self.damage is None else self.damage.ctypes.data
+        return L
+
+
+def rotation(axis, deg):
+    """The rotation matrix of `deg` degrees about `axis` (Rodrigues)."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    t = np.radians(deg)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * (K @ K)
+
+
+def lattices(grids):
+    arr = (Lattice * len(grids))()
+    for g, G in enumerate(grids):
+        arr[g] = G.lattice()
+    return arr
+
+
+def junction_points(v, face, sub, S, h, push=0.0):
+    """junction_point (world_internal.hpp:94-103) of many samples: (k, 3) lattice points (metres)."""
+    v = np.asarray(v, np.float64)
+    face = np.asarray(face, np.int64)
+    sub = np.asarray(sub, np.int64)
+    a = face >> 1
+    a1, a2 = (a + 1) % 3, (a + 2) % 3
+    sg = np.where(face & 1, 1.0, -1.0)
+    i, j = sub // S, sub % S
+    s = h * v
+    rows = np.arange(len(s))
+    s[rows, a] += sg * (0.5 * h + push)
+    s[rows, a1] += ((i + 0.5) / S - 0.5) * h
+    s[rows, a2] += ((j + 0.5) / S - 0.5) * h
+    return s
+
+
+def extract_grids(grids, frags, mats, S=3, push=None):
+    """(nodes, heads, faces, samples) of a structure over several lattices, the multi-lattice analogue
+    of extract(): frags[g] holds each voxel's node (global over the grids; -1 for none). Within a
+    lattice, faces as extract() finds them; between lattices, junction samples: each face of a node's
+    voxel that looks into air of its own lattice is split S x S, each sample point (pushed out of the
+    face by `push`, h / 8 by default) is looked up in the older lattices (lower index first from the
+    newest down) and, on a solid voxel, joins the two nodes (or, on an anchored voxel, rests the node
+    on it: a junction support). Bonds ordered by (node a, pairs before supports, node b / key), faces
+    and samples in scan order. Node centres and masses in the frame."""
+    rho = np.asarray(mats["rho"], np.float64)
+    nn = max(int(f.max()) + 1 if f.size else 0 for f in frags)
+    mass = np.zeros(nn)
+    mom = np.zeros((nn, 3))
+    counts = np.zeros((nn, 128), np.int64)
+    frec, srec = [], []  # face: (a, kind, key2, g, scan, px, py, pz, axis, sign, mb); sample: (a, kind, key2, g, scan, vx, vy, vz, ox, oy, oz, vg, og, face, sub, kind, sign)
+    for g, (G, frag) in enumerate(zip(grids, frags)):
+        frag = np.ascontiguousarray(frag, np.int32)
+        vox = G.vox
+        nzb, nyb, nxb = frag.shape
+        zz, yy, xx = np.nonzero(frag >= 0)
+        f = frag[zz, yy, xx]
+        mid = (vox[zz, yy, xx] & 0x7F).astype(np.int64) - 1
+        m = rho[mid] * G.h ** 3
+        cen = G.to(np.stack([xx + G.lo[0], yy + G.lo[1], zz + G.lo[2]], -1) * G.h)
+        np.add.at(mass, f, m)
+        np.add.at(mom, f, m[:, None] * cen)
+        np.add.at(counts, (f, mid), 1)
+        flat = lambda z, y, x: (x + nxb * (y + nyb * z)).astype(np.int64)
+        for axis in range(3):
+            sl_p, sl_q = [slice(None)] * 3, [slice(None)] * 3
+            ax_np = 2 - axis
+            sl_p[ax_np] = slice(0, frag.shape[ax_np] - 1)
+            sl_q[ax_np] = slice(1, frag.shape[ax_np])
+            fp, fq = frag[tuple(sl_p)], frag[tuple(sl_q)]
+            vp, vq = vox[tuple(sl_p)], vox[tuple(sl_q)]
+            z, y, x = np.nonzero((fp >= 0) | (fq >= 0))
+            a_, b_ = fp[z, y, x], fq[z, y, x]
+            va, vb = vp[z, y, x], vq[z, y, x]
+            scan = flat(z, y, x)
+            px, py, pz = x + G.lo[0], y + G.lo[1], z + G.lo[2]
+            pr = (a_ >= 0) & (b_ >= 0) & (a_ != b_)
+            k = int(pr.sum())
+            frec.append(np.stack([np.minimum(a_, b_)[pr], np.zeros(k, np.int64), np.maximum(a_, b_)[pr],
+                                  np.full(k, g), scan[pr], px[pr], py[pr], pz[pr], np.full(k, axis),
+                                  np.where(a_ < b_, 1, -1)[pr], np.zeros(k, np.int64)], -1))
+            up = (a_ >= 0) & (b_ < 0) & ((vb & ANCHOR) != 0)
+            dn = (b_ >= 0) & (a_ < 0) & ((va & ANCHOR) != 0)
+            for sel, node, s_, mbv in ((up, a_, 1, vb), (dn, b_, -1, va)):
+                k = int(sel.sum())
+                frec.append(np.stack([node[sel], np.ones(k, np.int64), np.full(k, 2 * axis + (1 if s_ > 0 else 0)),
+                                      np.full(k, g), scan[sel], px[sel], py[sel], pz[sel], np.full(k, axis),
+                                      np.full(k, s_), ((mbv[sel] & 0x7F).astype(np.int64) - 1)], -1))
+        if g == 0:
+            continue
+        # junction samples of this lattice's open faces into the older lattices
+        pz_ = push if push is not None else G.h / 8
+        vv = np.stack([xx + G.lo[0], yy + G.lo[1], zz + G.lo[2]], -1)
+        for face in range(6):
+            e = np.zeros(3, np.int64)
+            e[face >> 1] = 1 if face & 1 else -1
+            open_ = G.at(vv + e) == 0
+            if not open_.any():
+                continue
+            vs, fs = vv[open_], f[open_]
+            for sub in range(S * S):
+                pts = G.to(junction_points(vs, np.full(len(vs), face), np.full(len(vs), sub), S, G.h, pz_))
+                found = np.zeros(len(vs), bool)
+                for og in range(g - 1, -1, -1):
+                    O = grids[og]
+                    o = np.floor(O.from_frame(pts) / O.h + 0.5).astype(np.int64)
+                    ov = O.at(o)
+                    hit = ~found & (ov != 0)
+                    if not hit.any():
+                        continue
+                    found |= hit
+                    of = np.full(len(vs), -1, np.int64)
+                    r = o[hit] - O.lo
+                    of[hit] = frags[og][r[:, 2], r[:, 1], r[:, 0]]
+                    anch = hit & ((ov & ANCHOR) != 0)
+                    pair = hit & ~anch & (of >= 0) & (of != fs)
+                    for sel, kind in ((pair, 0), (anch, 1)):
+                        k = int(sel.sum())
+                        if not k:
+                            continue
+                        a = np.where(kind == 0, np.minimum(fs[sel], of[sel]), fs[sel])
+                        key2 = np.maximum(fs[sel], of[sel]) if kind == 0 else np.full(k, 6 + og)
+                        sign = np.where(a == fs[sel], 1, -1)
+                        scan = (np.flatnonzero(open_)[sel] * 6 + face) * (S * S) + sub
+                        srec.append(np.stack([a, np.full(k, kind), key2, np.full(k, g), scan,
+                                              vs[sel, 0], vs[sel, 1], vs[sel, 2], o[sel, 0], o[sel, 1], o[sel, 2],
+                                              np.full(k, g), np.full(k, og), np.full(k, face), np.full(k, sub),
+                                              np.zeros(k, np.int64), sign,
+                                              ((ov[sel] & 0x7F).astype(np.int64) - 1)], -1))
+    nodes = np.zeros(nn, NODE)
+    nodes["c"] = mom / np.maximum(mass, 1e-300)[:, None]
+    nodes["mass"] = mass
+    nmat = counts.argmax(1).astype(np.uint8)
+    F = np.concatenate(frec, 0) if frec else np.zeros((0, 11), np.int64)
+    J = np.concatenate(srec, 0) if srec else np.zeros((0, 18), np.int64)
+    F = F[np.lexsort((F[:, 8], F[:, 4], F[:, 3], F[:, 2], F[:, 1], F[:, 0]))]
+    J = J[np.lexsort((J[:, 4], J[:, 3], J[:, 2], J[:, 1], J[:, 0]))]
+    keys = sorted(set(map(tuple, F[:, :3].tolist())) | set(map(tuple, J[:, :3].tolist())))
+    fk = {k: [] for k in keys}
+    for i, r in enumerate(F[:, :3].tolist()):
+        fk[tuple(r)].append(i)
+    jk = {k: [] for k in keys}
+    for i, r in enumerate(J[:, :3].tolist()):
+        jk[tuple(r)].append(i)
+    faces = np.zeros(len(F), SECTION_FACE)
+    samples = np.zeros(len(J), JUNCTION_SAMPLE)
+    heads = np.zeros(len(keys), SECTION_HEAD_GENERAL)
+    f0 = s0 = 0
+    for h_, key in enumerate(keys):
+        a, kind, key2 = key
+        fi, ji = fk[key], jk[key]
+        for q, i in enumerate(fi):
+            faces[f0 + q] = ((F[i, 5], F[i, 6], F[i, 7]), F[i, 8], F[i, 9], (0, 0))
+        for q, i in enumerate(ji):
+            samples[s0 + q] = ((J[i, 5], J[i, 6], J[i, 7]), (J[i, 8], J[i, 9], J[i, 10]), J[i, 11], J[i, 12],
+                               J[i, 13], J[i, 14], J[i, 15], J[i, 16], 1.0)
+        grid = int(F[fi[0], 3]) if fi else int(J[ji[0], 3])
+        pair = kind == 0
+        if pair:
+            axis, sign, mb = 0, 1, int(nmat[key2])
+        elif key2 < 6:
+            axis, sign, mb = key2 // 2, (1 if key2 % 2 else -1), int(F[fi[0], 10])
+        else:
+            axis, sign, mb = 3 + 6 * (key2 - 6), 1, int(J[ji[0], 17])
+        heads[h_] = (a, key2 if pair else -1, axis, sign, f0, len(fi), s0, len(ji), grid, int(nmat[a]), mb,
+                     (0,) * 4, 1.0, 1.0 if pair else 1e9)
+        f0 += len(fi)
+        s0 += len(ji)
+    return nodes, heads, faces, samples
+
+
+def bond_sections_general(nodes, heads, faces, samples, grids, S, mats, ctx=None):
+    """svxg_bond_sections_general: the bonds (BOND records) of the heads over placed lattices and
+    junction samples, on the device."""
+    from . import stress
+    ctx = ctx or stress.default_context()
+    nodes = np.ascontiguousarray(nodes, NODE)
+    heads = np.ascontiguousarray(heads, SECTION_HEAD_GENERAL)
+    faces = np.ascontiguousarray(faces, SECTION_FACE)
+    samples = np.ascontiguousarray(samples, JUNCTION_SAMPLE)
+    mats = np.ascontiguousarray(mats)
+    lat = lattices(grids)
+    out = np.zeros(len(heads), BOND)
+    stress._check(stress.lib().svxg_bond_sections_general(
+        ctx.h, nodes.ctypes.data, len(nodes), heads.ctypes.data, len(heads), faces.ctypes.data, len(faces),
+        samples.ctypes.data, len(samples), lat, len(grids), int(S), mats.ctypes.data, len(mats), out.ctypes.data),
+        "svxg_bond_sections_general")
+    return out
 
 
 def gravity(nodes):

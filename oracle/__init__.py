@@ -55,7 +55,9 @@ def _bind(path, pre):
                     "hinge_of": ([vp, vp, vp, vp, vp, i32, vp, f64, f64, vp], i32),
                     "lerp_load": ([vp, vp, f64, vp], None),
                     "graph_components": ([i32, vp, i32, vp, vp], i32),
-                    "bond_sections": ([vp, i32, vp, i32, vp, i32, vp, vp, i32, vp], None)})
+                    "bond_sections": ([vp, i32, vp, i32, vp, i32, vp, vp, i32, vp], None),
+                    "bond_sections_general": ([vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, vp, i32, vp],
+                                              None)})
     for name, (args, res) in sig.items():
         fn = getattr(L, pre + name)
         fn.argtypes = args
@@ -303,6 +305,26 @@ def bond_sections(nodes, heads, faces, vox, lo, h, mats, damage=None):
     return out
 
 
+def bond_sections_general(nodes, heads, faces, samples, grids, S, mats):
+    """orc_bond_sections_general: SecAcc::finish (finish_lattice / finish_general) and
+    section_strengths(_general) (world_internal.hpp:120-127, 250-362, 374-441) of every head over
+    placed lattices (opus_destruct_2_amd.voxels.Grid) and junction samples, on the CPU."""
+    from opus_destruct_2_amd.voxels import lattices
+    from opus_destruct_2_amd.abi import JUNCTION_SAMPLE, SECTION_FACE, SECTION_HEAD_GENERAL
+    L, _ = library("oracle")
+    nodes = np.ascontiguousarray(nodes, NODE)
+    heads = np.ascontiguousarray(heads, SECTION_HEAD_GENERAL)
+    faces = np.ascontiguousarray(faces, SECTION_FACE)
+    samples = np.ascontiguousarray(samples, JUNCTION_SAMPLE)
+    mats = np.ascontiguousarray(mats)
+    lat = lattices(grids)
+    out = np.zeros(len(heads), BOND)
+    L.orc_bond_sections_general(nodes.ctypes.data, len(nodes), heads.ctypes.data, len(heads), faces.ctypes.data,
+                                len(faces), samples.ctypes.data, len(samples), lat, len(grids), int(S),
+                                mats.ctypes.data, len(mats), out.ctypes.data)
+    return out
+
+
 def cascade(problem, judge, f, u, opts, state, rtol=1e-10, maxit=20000, max_call_rounds=0, assemble_opt=None):
     """The break-round cascade of one structure on the CPU (World::Impl::step_structures' solve,
     judge and detach_unsupported, world.cpp:986-1129, 1162-1277, 1371-1444, round after round),
@@ -431,4 +453,4 @@ def bond_strength(bond, fragility, mats, kind="oracle"):
 
 __all__ = ["CpuProblem", "CpuJudge", "build", "library", "ref_available", "bond_utilization", "bond_strength",
            "reference_options", "StressOptions", "set_ref_threads", "hinge_of", "lerp_load", "graph_components",
-           "cascade", "bond_sections", "inv6", "diag_inverse"]
+           "cascade", "bond_sections", "bond_sections_general", "inv6", "diag_inverse"]

@@ -1494,6 +1494,265 @@ void orc_bond_sections(const svxg_node* nodes, int32_t, const svxg_section_head*
   }
 }
 
+// The general case (SecAcc::finish, world_internal.hpp:120-127; finish_general, :250-362;
+// section_strengths_general, :402-441): faces of a placed lattice and junction samples, each a small
+// square (point, normal, area, side) in the bond's frame. A bond with no samples on an unplaced
+// lattice is finish_lattice's: orc_bond_sections above on that lattice's voxels.
+void orc_bond_sections_general(const svxg_node* nodes, int32_t n, const svxg_section_head_general* heads, int32_t m,
+                               const svxg_section_face* faces, int32_t nfaces, const svxg_junction_sample* samples,
+                               int32_t, const svxg_lattice* lat, int32_t, int32_t S, const svxg_material* mats,
+                               int32_t nmats, svxg_bond* out) {
+  auto comp = [](V3& a, int i) -> f64& { return i == 0 ? a.x : (i == 1 ? a.y : a.z); };
+  auto add = [](const V3& a, const V3& b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; };
+  auto scale = [](const V3& a, f64 k) { return V3{a.x * k, a.y * k, a.z * k}; };
+  auto unit = [&](const V3& a) {
+    const f64 l = std::sqrt(vdot(a, a));
+    return l > 0 ? scale(a, 1.0 / l) : V3{0, 0, 0};
+  };
+  // LatticeXf::to / dir_to (rigid.hpp:91-93): R s row by row, then the offset
+  auto rot = [](const svxg_lattice& L, const V3& s) {
+    const double* R = L.R;
+    return V3{R[0] * s.x + R[1] * s.y + R[2] * s.z, R[3] * s.x + R[4] * s.y + R[5] * s.z,
+              R[6] * s.x + R[7] * s.y + R[8] * s.z};
+  };
+  auto to = [&](const svxg_lattice& L, const V3& s) {
+    if (L.identity) return s;
+    const V3 r = rot(L, s);
+    return V3{L.off[0] + r.x, L.off[1] + r.y, L.off[2] + r.z};
+  };
+  auto dir_to = [&](const svxg_lattice& L, const V3& d) { return L.identity ? d : rot(L, d); };
+  auto voxel = [](const svxg_lattice& L, const int32_t p[3], uint8_t* v, uint8_t* dmg) {
+    i64 idx = 0, mul = 1;
+    for (int c = 0; c < 3; ++c) {
+      const i32 r = p[c] - L.lo[c];
+      if (r < 0 || r >= L.dim[c]) {
+        *v = 0;
+        *dmg = 0;
+        return;
+      }
+      idx += mul * r;
+      mul *= L.dim[c];
+    }
+    *v = L.vox[idx];
+    *dmg = L.damage ? L.damage[idx] : 0;
+  };
+  auto material = [&](uint8_t v) -> const svxg_material& {
+    const int id = (v & 0x7F) - 1;
+    return mats[(id >= 0 && id < nmats) ? id : 0];
+  };
+  for (i32 k = 0; k < m; ++k) {
+    const svxg_section_head_general& hd = heads[k];
+    const svxg_lattice& G = lat[hd.grid];
+    const i32 nf = hd.nfaces, nj = hd.nsamples;
+    const svxg_section_face* F = faces + hd.face0;
+    const svxg_junction_sample* J = samples + hd.sample0;
+    svxg_bond B{};
+    if (nj == 0 && G.identity) {
+      // finish_lattice and section_strengths on the faces' own lattice
+      svxg_section_head one{};
+      one.a = hd.a;
+      one.b = hd.b;
+      one.axis = hd.axis;
+      one.sign = hd.sign;
+      one.face0 = 0;
+      one.nfaces = nf;
+      one.ma = hd.ma;
+      one.mb = hd.mb;
+      one.strength_a = hd.strength_a;
+      one.strength_b = hd.strength_b;
+      svxg_voxel_box box{};
+      for (int c = 0; c < 3; ++c) {
+        box.lo[c] = G.lo[c];
+        box.dim[c] = G.dim[c];
+      }
+      box.h = G.h;
+      box.vox = G.vox;
+      box.damage = G.damage;
+      orc_bond_sections(nodes, n, &one, 1, F, nf, &box, mats, nmats, &B);
+      B.tag = k;
+      out[k] = B;
+      continue;
+    }
+    // each face and sample as a square: centre, normal (from a to b), area, side
+    const size_t ne = size_t(nf) + size_t(nj);
+    std::vector<V3> xs(ne), nk(ne);
+    std::vector<f64> ar(ne), sd(ne);
+    const f64 h = G.h, A1 = h * h;
+    for (i32 f = 0; f < nf; ++f) {
+      V3 x{h * F[f].p[0], h * F[f].p[1], h * F[f].p[2]};
+      comp(x, F[f].axis) += 0.5 * h;
+      xs[size_t(f)] = to(G, x);
+      V3 nn{0, 0, 0};
+      comp(nn, F[f].axis) = F[f].sign;
+      nk[size_t(f)] = dir_to(G, nn);
+      ar[size_t(f)] = A1;
+      sd[size_t(f)] = h;
+    }
+    for (i32 q = 0; q < nj; ++q) {
+      const svxg_junction_sample& js = J[q];
+      const svxg_lattice& X = lat[js.vg];
+      const f64 hs = X.h, sj = hs / static_cast<f64>(std::max(1, S));
+      // the sample's point: the centre of cell (i, j) of an S x S split of the face (push 0)
+      const int ax = js.face >> 1, a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+      const f64 sg = (js.face & 1) ? 1.0 : -1.0;
+      const int i = js.sub / S, j = js.sub % S;
+      V3 pt{hs * js.v[0], hs * js.v[1], hs * js.v[2]};
+      comp(pt, ax) += sg * (0.5 * hs + 0.0);
+      comp(pt, a1) += ((i + 0.5) / static_cast<f64>(S) - 0.5) * hs;
+      comp(pt, a2) += ((j + 0.5) / static_cast<f64>(S) - 0.5) * hs;
+      const size_t e = size_t(nf) + size_t(q);
+      xs[e] = to(X, pt);
+      V3 fn{0, 0, 0};
+      comp(fn, ax) = (js.face & 1) ? 1.0 : -1.0;
+      nk[e] = scale(dir_to(X, fn), static_cast<f64>(js.sign));
+      ar[e] = sj * sj * static_cast<f64>(js.weight);
+      sd[e] = sj;
+    }
+    f64 atot = 0.0;
+    for (size_t e = 0; e < ne; ++e) atot += ar[e];
+    const bool pair = hd.b >= 0;
+    const V3 ca = v3(nodes[hd.a].c);
+    const V3 cb = pair ? v3(nodes[hd.b].c) : V3{0, 0, 0};
+    // the interface normal: the area-weighted mean normal (a lattice support: its face axis)
+    V3 nrm{0, 0, 0};
+    if (pair || nj) {
+      for (size_t e = 0; e < ne; ++e) nrm = add(nrm, scale(nk[e], ar[e]));
+      const f64 tiny = 1e-12 * atot * atot;
+      if (vdot(nrm, nrm) <= tiny && pair) nrm = vsub(cb, ca);
+      if (vdot(nrm, nrm) <= tiny) nrm = V3{0, 0, 1};
+      nrm = unit(nrm);
+    } else {
+      V3 e{0, 0, 0};
+      comp(e, hd.axis) = hd.sign;
+      nrm = dir_to(G, e);
+    }
+    const f64 ax = std::abs(nrm.x), ay = std::abs(nrm.y), az = std::abs(nrm.z);
+    const V3 least = (ax <= ay && ax <= az) ? V3{1, 0, 0} : (ay <= az ? V3{0, 1, 0} : V3{0, 0, 1});
+    V3 t1 = unit(vcross(nrm, least));
+    V3 t2 = vcross(nrm, t1);
+    std::vector<f64> ws(ne);
+    f64 W = 0.0;
+    V3 pc{0, 0, 0};
+    for (size_t e = 0; e < ne; ++e) {
+      ws[e] = ar[e] * std::max(0.05, std::abs(vdot(nrm, nk[e])));
+      W += ws[e];
+      pc = add(pc, scale(xs[e], ws[e]));
+    }
+    pc = scale(pc, 1.0 / W);
+    {
+      f64 s11 = 0.0, s22 = 0.0, s12 = 0.0;
+      for (size_t e = 0; e < ne; ++e) {
+        const V3 d = vsub(xs[e], pc);
+        const f64 y1 = vdot(d, t1), y2 = vdot(d, t2);
+        s11 += ws[e] * y1 * y1;
+        s22 += ws[e] * y2 * y2;
+        s12 += ws[e] * y1 * y2;
+      }
+      if (std::abs(s12) > 1e-9 * (s11 + s22)) {
+        const f64 half = 0.5 * (s11 - s22);
+        const f64 l1 = 0.5 * (s11 + s22) + std::sqrt(half * half + s12 * s12);
+        f64 e1 = s12, e2 = l1 - s11;
+        if (std::abs(l1 - s22) > std::abs(e2)) {
+          e1 = l1 - s22;
+          e2 = s12;
+        }
+        const f64 en = std::sqrt(e1 * e1 + e2 * e2);
+        if (en > 0.0) {
+          e1 /= en;
+          e2 /= en;
+          const V3 u1 = add(scale(t1, e1), scale(t2, e2));
+          const V3 u2 = vsub(scale(t2, e1), scale(t1, e2));
+          t1 = u1;
+          t2 = u2;
+        }
+      }
+    }
+    B.a = hd.a;
+    B.b = hd.b;
+    B.ma = hd.ma;
+    B.mb = hd.mb;
+    B.faces = static_cast<i32>(ne);
+    B.tag = k;
+    for (int c = 0; c < 3; ++c) {
+      B.n[c] = vget(nrm, c);
+      B.t1[c] = vget(t1, c);
+      B.t2[c] = vget(t2, c);
+      B.p[c] = vget(pc, c);
+    }
+    B.area = std::max(W, 0.25 * atot);
+    f64 s1 = 0.0, s2 = 0.0, m1 = 0.0, m2 = 0.0, own = 0.0;
+    for (size_t e = 0; e < ne; ++e) {
+      const V3 d = vsub(xs[e], pc);
+      const f64 y1 = vdot(d, t1), y2 = vdot(d, t2);
+      s1 += ws[e] * y1 * y1;
+      s2 += ws[e] * y2 * y2;
+      m1 = std::max(m1, std::abs(y1) + 0.5 * sd[e]);
+      m2 = std::max(m2, std::abs(y2) + 0.5 * sd[e]);
+      own += ws[e] * sd[e] * sd[e];
+    }
+    own /= W;
+    B.s1 = s1 + B.area * own / 12.0;
+    B.s2 = s2 + B.area * own / 12.0;
+    B.c1 = m1;
+    B.c2 = m2;
+    B.rmax = std::sqrt(B.c1 * B.c1 + B.c2 * B.c2);
+    if (pair) {
+      const V3 d = vsub(cb, ca);
+      const f64 L = std::max({std::abs(vdot(d, nrm)), 0.5 * std::sqrt(vdot(d, d)), 0.5 * h});
+      B.la = std::clamp(vdot(vsub(pc, ca), nrm), 0.1 * L, 0.9 * L);
+      B.lb = L - B.la;
+    } else {
+      B.la = std::max(0.25 * h, vdot(vsub(pc, ca), nrm));
+      B.lb = 0.5 * h;
+    }
+    B.strength = std::min(hd.strength_a, hd.strength_b);
+    // section_strengths_general: a face weighs 1, a sample 1 / S^2 (an S x S split of a face); each
+    // the weaker material of its two voxels times the more damaged one's condition; the mean
+    f64 ft = 0, fb = 0, fc = 0, coh = 0, mu = 0, wsum = 0;
+    auto one = [&](uint8_t va, uint8_t da, uint8_t vb, uint8_t db, f64 w) {
+      const svxg_material& Ma = material(va ? va : vb);
+      const svxg_material& Mb = material(vb ? vb : va);
+      const f64 cond = std::max(1e-6, 1.0 - static_cast<f64>(std::max(da, db)) / 255.0);
+      ft += w * cond * std::min(Ma.ft, Mb.ft);
+      fb += w * cond * std::min(Ma.fb, Mb.fb);
+      fc += w * cond * std::min(Ma.fc, Mb.fc);
+      coh += w * cond * std::min(Ma.cohesion, Mb.cohesion);
+      mu += w * std::min(Ma.friction, Mb.friction);
+      wsum += w;
+    };
+    const f64 sw = 1.0 / static_cast<f64>(std::max(1, S * S));
+    for (i32 f = 0; f < nf; ++f) {
+      uint8_t va, da, vb, db;
+      voxel(G, F[f].p, &va, &da);
+      int32_t q[3] = {F[f].p[0], F[f].p[1], F[f].p[2]};
+      q[F[f].axis] += 1;
+      voxel(G, q, &vb, &db);
+      one(va, da, vb, db, 1.0);
+    }
+    for (i32 q = 0; q < nj; ++q) {
+      uint8_t va, da, vb, db;
+      voxel(lat[J[q].vg], J[q].v, &va, &da);
+      if (J[q].kind == 1) {
+        vb = va;
+        db = da;
+      } else {
+        voxel(lat[J[q].og], J[q].o, &vb, &db);
+      }
+      one(va, da, vb, db, sw * static_cast<f64>(1.0f));
+    }
+    if (ne > 0 && wsum > 0.0) {
+      const f64 w = 1.0 / wsum;
+      B.sectioned = 1;
+      B.ft = static_cast<f32>(ft * w);
+      B.fb = static_cast<f32>(fb * w);
+      B.fc = static_cast<f32>(fc * w);
+      B.coh = static_cast<f32>(coh * w);
+      B.mu = static_cast<f32>(mu * w);
+    }
+    out[k] = B;
+  }
+}
+
 int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp) {
   // adjacency of the intact bonds between nodes, then floods: the seeds first (component 0), then
   // every node not reached yet, in index order

@@ -67,6 +67,10 @@ void orc_bond_sections(const svxg_node* nodes, int32_t n, const svxg_section_hea
                        const svxg_section_face* faces, int32_t nfaces, const svxg_voxel_box* box,
                        const svxg_material* mats, int32_t nmats, svxg_bond* out);
 /* graph_components (world_internal.hpp:660-706): comp[n]; returns the number of components */
+void orc_bond_sections_general(const svxg_node* nodes, int32_t n, const svxg_section_head_general* heads, int32_t m,
+                               const svxg_section_face* faces, int32_t nfaces, const svxg_junction_sample* samples,
+                               int32_t nsamples, const svxg_lattice* lattices, int32_t nlattices, int32_t S,
+                               const svxg_material* mats, int32_t nmats, svxg_bond* out);
 int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp);
 
 void orc_default_options(svxg_stress_options* out);

@@ -52,8 +52,11 @@ def test_struct_layouts_match_a_c_compiler(tmp_path):
                "svxg_server_options": abi.ServerOptions, "svxg_serve_done": abi.ServeDone,
                "svxg_server_stats": abi.ServerStats, "svxg_judge_options": abi.JudgeOptions,
                "svxg_judge_state": abi.JudgeState, "svxg_judge_result": abi.JudgeResult,
-               "svxg_cascade_result": abi.CascadeResult}
-    dtypes = {"svxg_hinge_cut": abi.HINGE_CUT, "svxg_break": abi.BREAK, "svxg_bond_load": abi.BOND_LOAD}
+               "svxg_cascade_result": abi.CascadeResult, "svxg_voxel_box": abi.VoxelBox,
+               "svxg_lattice": abi.Lattice}
+    dtypes = {"svxg_hinge_cut": abi.HINGE_CUT, "svxg_break": abi.BREAK, "svxg_bond_load": abi.BOND_LOAD,
+              "svxg_section_head": abi.SECTION_HEAD, "svxg_section_face": abi.SECTION_FACE,
+              "svxg_section_head_general": abi.SECTION_HEAD_GENERAL, "svxg_junction_sample": abi.JUNCTION_SAMPLE}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svxg.h"', "int main(void) {"]
     for c, py in mirrors.items():
         lines.append(f'  printf("{c} %zu\\n", sizeof({c}));')

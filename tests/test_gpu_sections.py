@@ -8,7 +8,7 @@ import pytest
 import oracle
 from oracle import CpuProblem
 from opus_destruct_2_amd import stress, voxels
-from opus_destruct_2_amd.abi import SVXG_MIXED
+from opus_destruct_2_amd.abi import JUNCTION_SAMPLE, SVXG_MIXED
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +74,75 @@ def test_lattice_graph_on_device_is_the_host_extraction(shape, cell, voids, seed
     assert dh.tobytes() == heads.tobytes()
     assert df.tobytes() == faces.tobytes()
     assert db.tobytes() == bonds.tobytes()
+
+
+# ---- the general case: placed lattices and junction samples (svxg_bond_sections_general) -------------
+
+from tests.test_sections_oracle import cantilever_scene, general_heads, grids_scene, turned  # noqa: E402
+
+
+@pytest.mark.parametrize("S", [1, 2, 3, 5, 7])
+def test_general_sections_are_the_oracles(S, mats, ctx):
+    """Three lattices joined by junctions (pairs across lattices, a junction support, faces within the
+    turned ones), damaged voxels: every bond bit for bit the oracle's."""
+    grids, frags = grids_scene(seed=S)
+    nodes, heads, faces, samples = voxels.extract_grids(grids, frags, mats, S=S)
+    assert (heads["nsamples"] > 0).sum() >= 10
+    dev = voxels.bond_sections_general(nodes, heads, faces, samples, grids, S, mats, ctx=ctx)
+    orc = oracle.bond_sections_general(nodes, heads, faces, samples, grids, S, mats)
+    assert dev.tobytes() == orc.tobytes()
+
+
+def test_general_entry_on_one_unplaced_lattice_is_the_lattice_entry(mats, ctx):
+    vox, frag, lo = voxels.voronoi_block(20, 16, 12, cell=2, voids=0.1, seed=12)
+    rng = np.random.default_rng(12)
+    damage = np.where(rng.random(vox.shape) < 0.05, rng.integers(1, 256, vox.shape), 0).astype(np.uint8)
+    nodes, heads, faces = voxels.extract(vox, frag, lo, 0.25, mats)
+    one = voxels.bond_sections(nodes, heads, faces, vox, lo, 0.25, mats, damage=damage, ctx=ctx)
+    G = voxels.Grid(vox, lo, 0.25, damage=damage)
+    gen_ = voxels.bond_sections_general(nodes, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE), [G], 3,
+                                        mats, ctx=ctx)
+    assert one.tobytes() == gen_.tobytes()
+    # (the same lattice marked placed by the identity: the general formulas, the oracle's bits)
+    G.identity = False
+    dev = voxels.bond_sections_general(nodes, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE), [G], 3,
+                                       mats, ctx=ctx)
+    orc = oracle.bond_sections_general(nodes, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE), [G], 3, mats)
+    assert dev.tobytes() == orc.tobytes()
+
+
+def test_turned_cantilever_on_the_device(mats, ctx):
+    """tests/core/test_grids.cpp:55-80 on the device: the cantilever's sections in an unplaced grid
+    and in one turned 30 degrees about z and moved, each solved on the device: utilisations within
+    1e-6 of each other and of the oracle's."""
+    phis = []
+    for R, off in ((None, None), (turned(30.0), (3.2, -1.7, 0.0))):
+        G, frag = cantilever_scene(R, off)
+        nodes, heads, faces, samples = voxels.extract_grids([G], [frag], mats, S=3)
+        b = voxels.bond_sections_general(nodes, heads, faces, samples, [G], 3, mats, ctx=ctx)
+        assert b.tobytes() == oracle.bond_sections_general(nodes, heads, faces, samples, [G], 3, mats).tobytes()
+        p = stress.StressProblem(nodes, b, mats, ctx, options=stress.default_options())
+        assert p.assemble()
+        u, r = p.solve(voxels.gravity(nodes), rtol=1e-12, maxit=5000)
+        assert r["converged"]
+        phis.append(p.bond_checks(None)[0])
+        p.close()
+    assert phis[0].max() > 0.05
+    assert np.allclose(phis[1], phis[0], rtol=1e-6, atol=1e-9)
+
+
+def test_general_sections_edge_cases(mats, ctx):
+    grids, frags = grids_scene(seed=2)
+    nodes, heads, faces, samples = voxels.extract_grids(grids, frags, mats, S=3)
+    assert len(voxels.bond_sections_general(nodes, heads[:0], faces, samples, grids, 3, mats, ctx=ctx)) == 0
+    for field, value in (("face", 6), ("sub", 9), ("vg", 3), ("kind", 2)):
+        bad = samples.copy()
+        bad[0][field] = value
+        with pytest.raises(stress.SvxgError):
+            voxels.bond_sections_general(nodes, heads, faces, bad, grids, 3, mats, ctx=ctx)
+    bad = heads.copy()
+    bad[0]["grid"] = 3
+    with pytest.raises(stress.SvxgError):
+        voxels.bond_sections_general(nodes, bad, faces, samples, grids, 3, mats, ctx=ctx)
+    with pytest.raises(stress.SvxgError):
+        voxels.bond_sections_general(nodes, heads, faces, samples, grids, 0, mats, ctx=ctx)

@@ -19,7 +19,7 @@ import pytest
 
 import oracle
 from opus_destruct_2_amd import gen, voxels
-from opus_destruct_2_amd.abi import NODE, SECTION_FACE, SECTION_HEAD
+from opus_destruct_2_amd.abi import JUNCTION_SAMPLE, NODE, SECTION_FACE, SECTION_HEAD
 
 H = 0.25
 
@@ -146,6 +146,267 @@ def test_voronoi_graph_from_voxels_solves(mats):
     u, r = p.solve(voxels.gravity(nodes), rtol=1e-10, maxit=5000)
     assert r["converged"]
     # (the weight is carried: the support bonds' forces, N n + V1 t1 + V2 t2, add up to it)
+    loads = p.bond_loads(u)
+    sup = b["b"] < 0
+    Fz = loads["N"][sup] * b["n"][sup, 2] + loads["V1"][sup] * b["t1"][sup, 2] + loads["V2"][sup] * b["t2"][sup, 2]
+    assert np.isclose(Fz.sum(), 9.81 * nodes["mass"].sum(), rtol=1e-6)
+
+
+# ---- the general case: placed lattices and junction samples (orc_bond_sections_general:
+# finish_general and section_strengths_general, world_internal.hpp:250-362, 402-441) ------------------
+#
+# Pinned by what the geometry fixes: an unplaced lattice without samples is the one-lattice case bit
+# for bit (SecAcc::finish's dispatch); the general formulas on an unplaced lattice agree with the
+# one-lattice ones (the same section measured another way); a lattice turned and moved gives the
+# turned and moved section (n -> R n, p -> off + R p, the same area, moments, springs and strengths);
+# a face split into S x S junction samples measures as the face (the samples' own squares and their
+# spread add up to the face's second moment, the parallel-axis theorem); and, as the reference's
+# tests/core/test_grids.cpp:55-80, a cantilever in a grid turned about z carries its weight as in the
+# world grid (utilisations within 1e-6).
+
+ROCK, CONCRETE = 5, 1
+
+
+def general_heads(heads, grid=0):
+    """svxg_section_head records as svxg_section_head_general ones of lattice `grid`, no samples."""
+    from opus_destruct_2_amd.abi import SECTION_HEAD_GENERAL
+    g = np.zeros(len(heads), SECTION_HEAD_GENERAL)
+    for k in ("a", "b", "axis", "sign", "face0", "nfaces", "ma", "mb", "strength_a", "strength_b"):
+        g[k] = heads[k]
+    g["grid"] = grid
+    return g
+
+
+def cantilever_scene(R=None, off=None, h=0.125, cell=2):
+    """The reference's grid cantilever (test_grids.cpp:46-50): an anchored rock pier [0, 8) x [0, 8) x
+    [0, 24) and a concrete beam [8, 32) x [2, 6) x [16, 22) cut into cell^3 fragments, in one lattice
+    placed by (R, off). Returns (grid, frag)."""
+    vox = np.zeros((24, 8, 32), np.uint8)
+    vox[:, :, :8] = voxels.vox_code(ROCK, anchored=True)
+    vox[16:22, 2:6, 8:32] = voxels.vox_code(CONCRETE)
+    frag = np.full(vox.shape, -1, np.int32)
+    z, y, x = np.nonzero(vox == voxels.vox_code(CONCRETE))
+    cid = (x - 8) // cell + 100 * ((y - 2) // cell) + 10000 * ((z - 16) // cell)
+    _, frag[z, y, x] = np.unique(cid, return_inverse=True)
+    return voxels.Grid(vox, (0, 0, 0), h, R=R, off=off), frag
+
+
+def turned(deg_z=30.0, tilt=0.0):
+    R = voxels.rotation((0, 0, 1), deg_z)
+    if tilt:
+        R = voxels.rotation((1, 0.3, 0), tilt) @ R
+    return R
+
+
+def _frame_ok(b):
+    n, t1, t2 = b["n"], b["t1"], b["t2"]
+    assert np.allclose(np.einsum("ij,ij->i", n, t1), 0, atol=1e-12)
+    assert np.allclose(np.einsum("ij,ij->i", t1, t2), 0, atol=1e-12)
+    assert np.allclose(np.cross(n, t1), t2, atol=1e-12)
+
+
+def test_general_unplaced_without_samples_is_the_lattice_case(mats):
+    vox, frag, lo = voxels.voronoi_block(14, 12, 10, cell=2, voids=0.1, seed=3)
+    rng = np.random.default_rng(3)
+    dmg = np.where(rng.random(vox.shape) < 0.1, rng.integers(1, 256, vox.shape), 0).astype(np.uint8)
+    nodes, heads, faces = voxels.extract(vox, frag, lo, H, mats)
+    one = oracle.bond_sections(nodes, heads, faces, vox, lo, H, mats, damage=dmg)
+    G = voxels.Grid(vox, lo, H, damage=dmg)
+    gen_ = oracle.bond_sections_general(nodes, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE), [G], 3, mats)
+    assert one.tobytes() == gen_.tobytes()
+
+
+def test_general_formulas_agree_with_the_lattice_ones(mats):
+    vox, frag, lo = voxels.voronoi_block(14, 12, 10, cell=2, voids=0.1, seed=5)
+    rng = np.random.default_rng(5)
+    dmg = np.where(rng.random(vox.shape) < 0.1, rng.integers(1, 256, vox.shape), 0).astype(np.uint8)
+    nodes, heads, faces = voxels.extract(vox, frag, lo, H, mats)
+    one = oracle.bond_sections(nodes, heads, faces, vox, lo, H, mats, damage=dmg)
+    # (the same bonds through the general formulas: the lattice marked placed, by the identity)
+    gen_ = oracle.bond_sections_general(nodes, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE),
+                                        [_placed(voxels.Grid(vox, lo, H, damage=dmg))], 3, mats)
+    for k in ("n", "p", "area", "s1", "s2", "c1", "c2", "rmax", "la", "lb"):
+        assert np.allclose(gen_[k], one[k], rtol=1e-12, atol=1e-15), k
+    # (the strengths of a bond without samples: the faces' mean either way, bit for bit)
+    for k in ("ft", "fb", "fc", "coh", "mu", "strength", "faces", "sectioned", "a", "b", "ma", "mb"):
+        assert np.array_equal(gen_[k], one[k]), k
+    _frame_ok(gen_)
+
+
+@pytest.mark.parametrize("deg,tilt,off", [(30.0, 0.0, (3.2, -1.7, 0.0)), (90.0, 0.0, (1.0, 2.0, 0.5)),
+                                          (17.0, 40.0, (-0.3, 0.9, 2.2))])
+def test_turned_lattice_turns_its_sections(deg, tilt, off, mats):
+    G0, frag = cantilever_scene()
+    nodes0, heads, faces = voxels.extract(G0.vox, frag, G0.lo, G0.h, mats)
+    R = turned(deg, tilt)
+    G1 = voxels.Grid(G0.vox, G0.lo, G0.h, R=R, off=off)
+    nodes1 = nodes0.copy()
+    nodes1["c"] = G1.to(nodes0["c"])
+    b0 = oracle.bond_sections_general(nodes0, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE),
+                                      [_placed(G0)], 3, mats)
+    b1 = oracle.bond_sections_general(nodes1, general_heads(heads), faces, np.zeros(0, JUNCTION_SAMPLE), [G1], 3, mats)
+    assert (b1["b"] < 0).sum() >= 6 and (b1["b"] >= 0).sum() > 100
+    assert np.allclose(b1["n"], b0["n"] @ R.T, atol=1e-12)
+    assert np.allclose(b1["p"], G1.to(b0["p"]), atol=1e-12)
+    for k in ("area", "la", "lb"):
+        assert np.allclose(b1[k], b0[k], rtol=1e-11), k
+    assert np.allclose(b1["s1"] + b1["s2"], b0["s1"] + b0["s2"], rtol=1e-11)
+    assert np.allclose(b1["s1"] * b1["s2"], b0["s1"] * b0["s2"], rtol=1e-9)
+    for k in ("ft", "fb", "fc", "coh", "mu", "strength"):
+        assert np.array_equal(b1[k], b0[k]), k
+    _frame_ok(b1)
+
+
+def _placed(G):
+    P = voxels.Grid(G.vox, G.lo, G.h, damage=G.damage)
+    P.identity = False
+    return P
+
+
+@pytest.mark.parametrize("S,weight", [(1, 1.0), (2, 1.0), (3, 1.0), (7, 1.0), (3, 0.5)])
+def test_face_split_into_samples_measures_as_the_face(S, weight, mats):
+    """Every pair bond of a Voronoi block given again as the S x S junction samples of its faces
+    (alternately on a's voxel's face looking into b's, sign +1, and on b's looking into a's, sign
+    -1): the same section up to rounding; with every sample weighing `weight`, the area and second
+    moments scaled by it, the rest unchanged."""
+    from opus_destruct_2_amd.abi import JUNCTION_SAMPLE as JS
+    vox, frag, lo = voxels.voronoi_block(12, 10, 8, cell=2, voids=0.0, seed=9)
+    nodes, heads, faces = voxels.extract(vox, frag, lo, H, mats)
+    pairs = heads[heads["b"] >= 0]
+    G = voxels.Grid(vox, lo, H)
+    gh = general_heads(pairs)
+    samples = []
+    for k, hd in enumerate(pairs):
+        gh[k]["sample0"] = len(samples)
+        for f in faces[hd["face0"]:hd["face0"] + hd["nfaces"]]:
+            ax = int(f["axis"])
+            lo_v = np.array(f["p"])
+            hi_v = lo_v.copy()
+            hi_v[ax] += 1
+            # a's voxel: the lower one when a is on the lower side; its face towards b, normal +1;
+            # every other face from b's voxel, its normal pointing to a: sign -1
+            av, bv = (lo_v, hi_v) if f["sign"] > 0 else (hi_v, lo_v)
+            a_face = 2 * ax + (1 if f["sign"] > 0 else 0)
+            v, o, face, sg = (av, bv, a_face, 1) if len(samples) % 2 == 0 else (bv, av, a_face ^ 1, -1)
+            for sub in range(S * S):
+                samples.append((tuple(v), tuple(o), 0, 0, face, sub, 0, sg, weight))
+        gh[k]["nsamples"] = len(samples) - gh[k]["sample0"]
+        gh[k]["nfaces"] = 0
+    samples = np.array(samples, JS)
+    bf = oracle.bond_sections_general(nodes, general_heads(pairs), faces, np.zeros(0, JS), [G], S, mats)
+    bs = oracle.bond_sections_general(nodes, gh, faces, samples, [G], S, mats)
+    assert np.all(bs["faces"] == S * S * bf["faces"])
+    # (a face's samples average to its centre with its weight: the same normal, centroid, area and
+    # springs on every bond; on a plane interface, all its faces of one axis, also the same second
+    # moments - out of the plane a face's samples spread along it, which its square does not - and,
+    # where its principal axes are the lattice's, the same extreme fibres: a face's square is taken
+    # square to them)
+    for k in ("n", "p", "la", "lb"):
+        assert np.allclose(bs[k], bf[k], rtol=1e-10, atol=1e-14), k
+    assert np.allclose(bs["area"], weight * bf["area"], rtol=1e-10)
+    plane = np.array([len(set(faces["axis"][h["face0"]:h["face0"] + h["nfaces"]])) == 1 for h in pairs])
+    assert plane.sum() > 20
+    for k in ("s1", "s2"):
+        assert np.allclose(bs[k][plane], weight * bf[k][plane], rtol=1e-10, atol=1e-14), k
+    square = plane & (np.abs(bf["t1"]).max(1) > 1 - 1e-12)
+    assert square.sum() > 10
+    for k in ("c1", "c2", "rmax"):
+        assert np.allclose(bs[k][square], bf[k][square], rtol=1e-10, atol=1e-14), k
+    for k in ("ft", "fb", "fc", "coh", "mu"):
+        assert np.allclose(bs[k], bf[k], rtol=1e-6), k
+
+
+def test_junction_support_takes_the_samples_normal(mats):
+    """A node resting through junction samples of its bottom faces (a junction support): the normal
+    is the samples' mean normal out of a (down), la = max(h / 4, centroid depth), lb = h / 2, and its
+    area the faces'."""
+    from opus_destruct_2_amd.abi import JUNCTION_SAMPLE as JS
+    vox = np.zeros((3, 4, 4), np.uint8)
+    vox[0] = voxels.vox_code(ROCK, anchored=True)
+    vox[1:] = voxels.vox_code(CONCRETE)
+    G = voxels.Grid(vox, (0, 0, -1), H)
+    nodes = _nodes([1.5 * H, 1.5 * H, 0.5 * H])
+    S = 3
+    samples = [((x, y, 0), (x, y, -1), 0, 0, 4, sub, 0, 1, 1.0) for y in range(4) for x in range(4) for sub in range(S * S)]
+    from opus_destruct_2_amd.abi import SECTION_HEAD_GENERAL
+    hd = np.zeros(1, SECTION_HEAD_GENERAL)
+    hd[0] = (0, -1, 3, 1, 0, 0, 0, len(samples), 0, CONCRETE, ROCK, (0,) * 4, 1.0, 1e9)
+    b = oracle.bond_sections_general(nodes, hd, np.zeros(0, SECTION_FACE), np.array(samples, JS), [G], S, mats)[0]
+    assert np.allclose(b["n"], [0, 0, -1]) and b["b"] == -1 and b["faces"] == 16 * S * S
+    assert np.isclose(b["area"], 16 * H * H) and np.allclose(b["p"], [1.5 * H, 1.5 * H, -0.5 * H])
+    assert np.isclose(b["s1"], (4 * H) ** 4 / 12) and np.isclose(b["s2"], (4 * H) ** 4 / 12)
+    assert np.isclose(b["la"], H) and np.isclose(b["lb"], 0.5 * H)
+    m_c, m_r = mats[CONCRETE], mats[ROCK]
+    assert np.isclose(b["ft"], min(m_c["ft"], m_r["ft"]), rtol=1e-6)
+
+
+def test_turned_cantilever_carries_its_weight_as_in_the_world_grid(mats):
+    """tests/core/test_grids.cpp:55-80: the same cantilever in an unplaced grid and in one turned 30
+    degrees about z and moved (gravity along both lattices' z): the same utilisations to 1e-6, the
+    same bonds over 1."""
+    phi = []
+    for R, off in ((None, None), (turned(30.0), (3.2, -1.7, 0.0))):
+        G, frag = cantilever_scene(R, off)
+        nodes, heads, faces, samples = voxels.extract_grids([G], [frag], mats, S=3)
+        assert len(samples) == 0
+        b = oracle.bond_sections_general(nodes, heads, faces, samples, [G], 3, mats)
+        p = oracle.CpuProblem(nodes, b, mats)
+        assert p.assemble()
+        u, r = p.solve(voxels.gravity(nodes), rtol=1e-12, maxit=5000)
+        assert r["converged"]
+        phi.append(p.bond_checks(u)[0])
+    assert phi[0].max() > 0.05
+    assert np.allclose(phi[1], phi[0], rtol=1e-6, atol=1e-9)
+    assert int((phi[0] >= 1).sum()) == int((phi[1] >= 1).sum())
+
+
+def grids_scene(seed=1, h2=0.1, tilt=8.0, damage=0.05):
+    """Three lattices: the world grid (h 0.125: an anchored rock ground 4 x 3 m and on it a Voronoi
+    block of concrete and masonry fragments, 2 x 1.75 x 1.25 m), a grid turned 30 degrees about z
+    holding a 3 m beam cast into the block's top and reaching out of it, and a grid of voxel size h2
+    turned about a slanted axis holding a strut from the ground into the beam's far end. Returns
+    (grids, frags), node ids global over the grids; random damage on a share of each lattice's
+    voxels."""
+    rng = np.random.default_rng(seed)
+    bv_, bf_, _ = voxels.voronoi_block(16, 14, 10, cell=3, voids=0.1, ground=False, seed=seed, materials=(1, 3))
+    wv = np.zeros((11, 24, 32), np.uint8)
+    wf = np.full(wv.shape, -1, np.int32)
+    wv[0] = voxels.vox_code(6, anchored=True)
+    wv[1:, :14, :16] = bv_
+    wf[1:, :14, :16] = bf_
+    W = voxels.Grid(wv, (0, 0, -1), 0.125)
+    n0 = int(wf.max()) + 1
+    bv = np.full((3, 3, 24), voxels.vox_code(CONCRETE), np.uint8)
+    bf = (np.arange(24)[None, None, :] // 3 + np.zeros((3, 3, 1), np.int64)).astype(np.int32) + n0
+    T = voxels.Grid(bv, (0, 0, 0), 0.125, R=turned(30.0), off=(1.0, 0.7, 0.9))
+    n1 = int(bf.max()) + 1
+    sv = np.full((12, 4, 4), voxels.vox_code(ROCK), np.uint8)
+    sf = (np.arange(12)[:, None, None] // 3 + np.zeros((1, 4, 4), np.int64)).astype(np.int32) + n1
+    P = voxels.Grid(sv, (0, 0, 0), h2, R=turned(17.0, tilt), off=(3.05, 1.95, -0.05))
+    grids, frags = [W, T, P], [wf, bf, sf]
+    for G in grids:
+        G.damage = np.where(rng.random(G.vox.shape) < damage, rng.integers(1, 256, G.vox.shape), 0).astype(np.uint8)
+    return grids, frags
+
+
+@pytest.mark.parametrize("S", [1, 3, 7])
+def test_grids_scene_carries_its_weight(S, mats):
+    """A structure over three lattices joined by junctions (pairs across lattices and supports on the
+    ground through samples): every bond sectioned with a right-handed orthonormal frame, and after the
+    solve the support bonds carry the structure's weight (their forces' vertical parts add up to it)."""
+    grids, frags = grids_scene()
+    nodes, heads, faces, samples = voxels.extract_grids(grids, frags, mats, S=S)
+    cross = heads["nsamples"] > 0
+    vg = {(int(samples["vg"][h["sample0"]]), int(samples["og"][h["sample0"]])) for h in heads[cross]}
+    assert vg == {(1, 0), (2, 1), (2, 0)}, vg  # (beam into block, strut into beam, strut on the ground)
+    assert cross.sum() >= 10 and ((heads["b"] < 0) & cross).any() and ((heads["b"] >= 0) & cross).any()
+    b = oracle.bond_sections_general(nodes, heads, faces, samples, grids, S, mats)
+    assert np.all(b["sectioned"] == 1) and np.all(b["area"] > 0) and np.all(b["la"] > 0) and np.all(b["lb"] > 0)
+    _frame_ok(b)
+    p = oracle.CpuProblem(nodes, b, mats)
+    assert p.assemble()
+    u, r = p.solve(voxels.gravity(nodes), rtol=1e-11, maxit=5000)
+    assert r["converged"]
     loads = p.bond_loads(u)
     sup = b["b"] < 0
     Fz = loads["N"][sup] * b["n"][sup, 2] + loads["V1"][sup] * b["t1"][sup, 2] + loads["V2"][sup] * b["t2"][sup, 2]
