@@ -1233,7 +1233,13 @@ static void fp64_step(Batch& B, hipStream_t s, int degree) {
   launch_update_p<double>(s, G, t0, B.nt0run, SERVE_OUTER);
 }
 
-// a window of w steps: the resident graphs' in one launch, the tiled graphs' step by step
+// a window of w steps: the resident graphs' in one launch, the tiled graphs' step by step.
+// Mixed windows: only the first and last steps of a window launch the refinement and inner-start
+// kernels (none when every running graph began the window iterating an inner solve, `lean`, but
+// for the last step); the steps between run the inner iteration alone. A graph that leaves its
+// inner solve in the middle of a window applies its correction and waits, untouched, for the last
+// step, so it still finishes before the host's look at the window's end, and its arithmetic is the
+// same - only the idle steps differ.
 static void mixed_step(Batch& B, hipStream_t s, int degree, bool p32, bool p64, bool lean);
 static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, bool p64, bool lean = false) {
   const int degree = batch_degree(B.probs[0]);
@@ -1241,32 +1247,39 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
     launch_resident_steps(s, B.dG.as<GraphDev>(), B.gtile.as<int32_t>(), B.res.as<int32_t>(), B.nres, w, degree,
                           mixed ? 1 : 0, p32 ? 1 : 0, p64 ? 1 : 0, B.partial.as<double>());
   if (!B.nt0run) return;
+  auto lean_at = [&](int k) { return mixed && k < w - 1 && (lean || k > 0); };
   const svxg_ctx* c = B.probs[0]->ctx;
   const bool record = c->step_graphs == 2 || (c->step_graphs == 1 && B.nt0run >= Batch::kGraphMinTiles && ++B.windows > 2);
   if (record && !c->profiling && !B.graphs_failed && !B.tuning) {
-    hipGraphExec_t& ex = B.step_graph[(lean ? 8 : 0) + (mixed ? 4 : 0) + (p32 ? 2 : 0) + (p64 ? 1 : 0)];
-    if (!ex) {
-      // (recorded on first use; a failed recording leaves this batch to launch kernel by kernel)
-      hipGraph_t gr = nullptr;
-      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-        if (mixed) mixed_step(B, s, degree, p32, p64, lean);
-        else fp64_step(B, s, degree);
-        if (hipStreamEndCapture(s, &gr) != hipSuccess) gr = nullptr;
+    // (each step form recorded on first use; a failed recording leaves this batch to launch kernel
+    // by kernel)
+    auto exec_for = [&](bool ln) -> hipGraphExec_t {
+      hipGraphExec_t& ex = B.step_graph[(ln ? 8 : 0) + (mixed ? 4 : 0) + (p32 ? 2 : 0) + (p64 ? 1 : 0)];
+      if (!ex && !B.graphs_failed) {
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+          if (mixed) mixed_step(B, s, degree, p32, p64, ln);
+          else fp64_step(B, s, degree);
+          if (hipStreamEndCapture(s, &gr) != hipSuccess) gr = nullptr;
+        }
+        if (!gr || hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+        if (gr) (void)hipGraphDestroy(gr);
+        if (!ex) {
+          B.graphs_failed = true;
+          (void)hipGetLastError();
+        }
       }
-      if (!gr || hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
-      if (gr) (void)hipGraphDestroy(gr);
-      if (!ex) {
-        B.graphs_failed = true;
-        (void)hipGetLastError();
-      }
-    }
-    if (ex) {
-      for (int k = 0; k < w; ++k) (void)hipGraphLaunch(ex, s);
+      return ex;
+    };
+    const hipGraphExec_t full = exec_for(false);
+    const hipGraphExec_t thin = (mixed && w > 1) ? exec_for(true) : full;
+    if (full && thin) {
+      for (int k = 0; k < w; ++k) (void)hipGraphLaunch(lean_at(k) ? thin : full, s);
       return;
     }
   }
   for (int k = 0; k < w; ++k) {
-    if (mixed) mixed_step(B, s, degree, p32, p64, lean);
+    if (mixed) mixed_step(B, s, degree, p32, p64, lean_at(k));
     else fp64_step(B, s, degree);
   }
 }
