@@ -776,7 +776,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     Batch::Lv& V = B.lvl[size_t(l)];
     // the level's shape a launch kind: rows and entries over the batch's tiled graphs
     int64_t rows[3] = {0, 0, 0}, ent[3] = {0, 0, 0}, dense_rows = 0;
-    bool half = true, r16 = true;
+    bool half = true;
     std::vector<int> members;
     for (int g = 0; g < count; ++g) {
       svxg_problem* P = probs[g];
@@ -791,7 +791,6 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
         ent[1] += H.np;
         rows[2] += P->lv[size_t(l) + 1].n;
         ent[2] += H.np;
-        r16 = r16 && H.R16.as<void>() != nullptr;
       } else if (H.dense_m > 0) {
         dense_rows += H.dense_m;
       }
@@ -800,7 +799,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     int target[3] = {kTuneTargets[kTuneDefault], kTuneTargets[kTuneDefault], kTuneTargets[kTuneDefault]};
     for (int k = 0; k < 3; ++k) {
       Batch::Tune& tu = V.tune[k];
-      if (rows[k] == 0 || (k == 2 && !r16)) continue;
+      if (rows[k] == 0 || (k == 2 && v64)) continue;
       tu.key = tune_key(k, v64, half, rows[k], ent[k]);
       const int ch = c ? tune_choice(c, tu.key) : -1;
       if (ch >= 0) {
@@ -819,8 +818,10 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
         add_tiles(T.ts[0], g, H.n, srows);
         add_tiles(T.tp, g, H.n, lpb_tile_rows(H.np, H.n, v64, target[1]));
         const int nc = P->lv[size_t(l) + 1].n;
-        if (H.R16.as<void>() || !long_rows(H.np, H.nc))
-          add_tiles(T.tc[0], g, nc, H.R16.as<void>() ? lpb_tile_rows(H.np, nc, v64, target[2]) : tile_rows<1>());
+        if (!v64)  // (FP16 copies, or FP32 P's blocks read transposed in place: a block a lane)
+          add_tiles(T.tc[0], g, nc, lpb_tile_rows(H.np, nc, v64, target[2]));
+        else if (!long_rows(H.np, H.nc))
+          add_tiles(T.tc[0], g, nc, tile_rows<1>());
         else
           add_tiles(T.tc[1], g, nc, tile_rows<10>());
       } else if (H.dense_m > 0) {

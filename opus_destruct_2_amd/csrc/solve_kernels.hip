@@ -247,14 +247,40 @@ __device__ __forceinline__ void lpb_block(const void* __restrict__ A, const int3
   }
 }
 
+// a block of the restriction read in place from the prolongator (FP32): block ent[kb] of P taken
+// transposed - its column q dotted with x_j - in the row walk's order of terms
+__device__ __forceinline__ void lpb_block_pt(const float* __restrict__ P, const int32_t* __restrict__ ent,
+                                             const int32_t* __restrict__ col, const float* __restrict__ x, int kb,
+                                             float y[6]) {
+  const int j = ldg(col + kb);
+  const int pe = ldg(ent + kb);
+  using F2 = typename V2<float>::t;
+  const F2* bp = reinterpret_cast<const F2*>(P + 36 * size_t(pe));
+  F2 m[18];
+#pragma unroll
+  for (int c = 0; c < 18; ++c) m[c] = ldg(bp + c);
+  float v[6];
+  load6(x + 6 * size_t(j), v);
+  float a[36];
+#pragma unroll
+  for (int c = 0; c < 18; ++c) {
+    a[2 * c] = m[c].x;
+    a[2 * c + 1] = m[c].y;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    y[q] = a[q] * v[0] + a[6 + q] * v[1] + a[12 + q] * v[2] + a[18 + q] * v[3] + a[24 + q] * v[4] + a[30 + q] * v[5];
+}
+
 // acc0 + (A x)_q of row `row` (lanes with on = false get 0) over the tile's rows [row0, row0 + nrows),
 // the row's block sums added to acc0 in block order; every lane of the workgroup must call it (it
 // synchronises the workgroup). H lanes share a block, each
 // taking 6 / H of its rows (FP64 blocks: two lanes of 144 bytes each keep more blocks in flight)
-template <typename T, bool HALF>
+template <typename T, bool HALF, bool PT = false>
 __device__ __forceinline__ T lpb_rows(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                       const void* __restrict__ A, const T* __restrict__ x, int row0, int nrows,
-                                      bool on, int row, int q, T acc0 = T(0)) {
+                                      bool on, int row, int q, T acc0 = T(0), const int32_t* __restrict__ ent = nullptr) {
+  static_assert(!PT || (std::is_same<T, float>::value && !HALF), "the in-place transposed blocks are FP32");
   constexpr int H = (!HALF && sizeof(T) == 8) ? 2 : 1;
   constexpr int NQ = 6 / H, PER = kWG / H;
   __shared__ T yb[6 * PER];
@@ -270,7 +296,10 @@ __device__ __forceinline__ T lpb_rows(const int32_t* __restrict__ rowptr, const 
     const int kb = c + lb;
     if (kb < k1) {
       T y[NQ];
-      lpb_block<T, HALF, NQ>(A, col, x, kb, q0, y);
+      if constexpr (PT)
+        lpb_block_pt(static_cast<const float*>(A), ent, col, x, kb, y);
+      else
+        lpb_block<T, HALF, NQ>(A, col, x, kb, q0, y);
 #pragma unroll
       for (int u = 0; u < NQ; ++u) yb[6 * lb + q0 + u] = y[u];
     }
@@ -532,6 +561,12 @@ __device__ __forceinline__ void restrict_tile(const GraphDev& g, const Tile& tl,
     if (half) {
       acc = T(lpb_rows<float, true>(F.Rrow, F.Rfine, F.R16, reinterpret_cast<const float*>(af.r), tl.row0, tl.nrows,
                                     on, I, q));
+      if (on) ac.b[6 * size_t(I) + q] = acc;
+      return;
+    }
+    if constexpr (std::is_same<T, float>::value) {
+      // (FP32: the prolongator's blocks read in place through Rent, each taken transposed)
+      acc = lpb_rows<float, false, true>(F.Rrow, F.Rfine, af.P, af.r, tl.row0, tl.nrows, on, I, q, 0.0f, F.Rent);
       if (on) ac.b[6 * size_t(I) + q] = acc;
       return;
     }
@@ -1414,7 +1449,7 @@ __device__ void vcycle_g(const GraphDev& g, int gi, int mode, int degree) {
       for (int st = 0; st < dg; ++st) smooth_any<T>(g, gi, l, st, (st + 1) & 1, st & 1, st == 0, mode);
       if (L.sub == 10) residual_g<T, 10>(g, gi, l, (dg - 1) & 1, mode);
       else residual_g<T, 1>(g, gi, l, (dg - 1) & 1, mode);
-      if (L.rsub == 10) restrict_g<T, 10>(g, gi, l, mode);
+      if (L.rsub == 10 && !std::is_same<T, float>::value) restrict_g<T, 10>(g, gi, l, mode);
       else restrict_g<T, 1>(g, gi, l, mode);
     } else if (L.dense_m > 0) {
       dense_solve_g<T>(g, l);

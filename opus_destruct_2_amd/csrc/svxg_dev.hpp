@@ -131,7 +131,7 @@ struct LevelDev {
   double lmax;           // lambda_max(D^-1 A) estimate (x 1.1)
   int32_t cheb_degree;
   int32_t sub;           // six-lane sub-groups a row in the tiled launches of this level (1 or 10)
-  int32_t rsub;          // the same for its restriction (coarse rows of the next level)
+  int32_t rsub;          // the same for its FP64 restriction (coarse rows of the next level; FP32 / FP16: a block a lane)
   int32_t pad_;
   const uint8_t* A16;    // A in FP16 blocks scaled by their largest entry (kHalfBlock bytes a block), or null
   const uint8_t* P16;    // P the same way, by prolongation entry (row-major blocks), or null
