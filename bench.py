@@ -779,8 +779,11 @@ def main():
         if args.only == "head":  # (the headline solves alone, with their live roofline)
             prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
             head = run_path(g, mats, prec, args.steps, args.warmup, ctx)
+            sb = solve_bytes(head["info"], prec == SVXG_MIXED, head["iters"], head["refinements"])
             rec = {"solve_ms": round(head["solve_ms"], 3), "iters": head["iters"],
-                   "roofline": roofline(head, args.precision, (args.nx, args.ny, args.nz)), "autotune": ctx.tuned()}
+                   "roofline": roofline(head, args.precision, (args.nx, args.ny, args.nz)),
+                   "solve_bytes": int(sb), "solve_frac": round(sb / (head["solve_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "autotune": ctx.tuned()}
         elif args.only == "batch":
             rec, _ = run_batch(mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample, args.batch_sigma)
         elif args.only == "cpu":  # (the CPU protocol on the block: every thread and one thread)
