@@ -861,11 +861,9 @@ def main():
     if args.serve_lambdas:
         lams = [int(x) for x in args.serve_lambdas.split(",") if x]
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
-        # (a context of its own, as a server's would be)
-        sctx = stress.Context(0)
-        line["serving"] = run_serving(mats, sctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
+        # (on the context the headline and the batch used: a long-lived server's state)
+        line["serving"] = run_serving(mats, ctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
                                       args.serve_pool, setup_workers=args.serve_setup_workers)
-        sctx.close()
     if args.city > 0:
         cctx = stress.Context(0)  # (a context of its own, freed after the leg)
         line["city"] = run_city(mats, cctx, args.city, args.city_rounds, args.city_rate)
