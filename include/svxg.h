@@ -572,6 +572,24 @@ int svxg_lattice_graph(svxg_ctx* ctx, const svxg_voxel_box* box, const int32_t* 
                        const svxg_material* mats, int32_t nmats, svxg_node* nodes, int32_t* nbonds, int32_t* nfaces,
                        svxg_section_head* heads, svxg_section_face* faces, svxg_bond* bonds);
 
+/* The same with fragments clustered into nodes (append_nodes, world.cpp:741-800, for one lattice):
+   touching fragments whose first voxels (the least (x, y, z) of their voxels, x first) share a cell
+   make one node, clusters numbered in order of their least fragment (cluster_items,
+   world_internal.hpp:466-491). The cell: with cubes != 0 a cube of `cell` voxels; else the
+   reference's default key, whose packing drops the cell along x (WorldConfig::cluster_cubes off,
+   world.hpp:234-241): `cell` voxels in y and z and a chunk (32 voxels) in x. cell: 0 (a node a
+   fragment, as svxg_lattice_graph), or a divisor of 32 (World::Impl::cluster_cell gives 8 or 16).
+   A node's mass is its fragments' masses summed in index order, its centre their centres weighted
+   by their masses (summed, then times 1 / mass), its material the heaviest fragment's; faces between
+   fragments of one node drop out. frag_node (nfrag, or null) receives each fragment's node and
+   *nnodes the node count; nodes then has room for *nnodes. The two-call protocol of
+   svxg_lattice_graph. */
+int svxg_lattice_graph_clustered(svxg_ctx* ctx, const svxg_voxel_box* box, const int32_t* frag, int32_t nfrag,
+                                 int32_t cell, int32_t cubes, const svxg_material* mats, int32_t nmats,
+                                 int32_t* frag_node, int32_t* nnodes, svxg_node* nodes, int32_t* nbonds,
+                                 int32_t* nfaces, svxg_section_head* heads, svxg_section_face* faces,
+                                 svxg_bond* bonds);
+
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------
    Replaces the reference's per-tick structure loop (step_structures, world.cpp:986-1129: the
    structures of a tick solved one after another within a block-operation budget) and its piece

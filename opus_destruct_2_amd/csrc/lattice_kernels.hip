@@ -44,6 +44,10 @@ __device__ __forceinline__ bool neighbour(const LatticeDev& L, int64_t slot, Nbr
   const int64_t w = q[0] + int64_t(nx) * (q[1] + int64_t(ny) * q[2]);
   o->a = L.frag[v];
   o->b = L.frag[w];
+  if (L.fnode) {
+    if (o->a >= 0) o->a = L.fnode[o->a];
+    if (o->b >= 0) o->b = L.fnode[o->b];
+  }
   o->va = L.vox[v];
   o->vb = L.vox[w];
   o->x = x;
@@ -223,7 +227,166 @@ __global__ void __launch_bounds__(kWG) k_node_sums(const LatticeDev* __restrict_
   nmat[f] = uint8_t(best);
 }
 
+// ---- clusters of fragments ----------------------------------------------------------------------
+
+constexpr int64_t kCoordBias = int64_t(1) << 20;
+
+// a fragment's first voxel: the least (x, y, z) of its voxels, x first, in global coordinates (the
+// complement kept, so one atomicMax over a zeroed array finds the least)
+__global__ void __launch_bounds__(kWG) k_frag_first(const LatticeDev* __restrict__ Lp, uint64_t* __restrict__ nfirst) {
+  const LatticeDev& L = *Lp;
+  const int64_t v = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (v >= L.nvox) return;
+  const int f = L.frag[v];
+  if (f < 0) return;
+  const int nx = L.dim[0], ny = L.dim[1];
+  const int64_t x = v % nx + L.lo[0], y = (v / nx) % ny + L.lo[1], z = v / (int64_t(nx) * ny) + L.lo[2];
+  const uint64_t packed = (uint64_t(x + kCoordBias) << 42) | (uint64_t(y + kCoordBias) << 21) | uint64_t(z + kCoordBias);
+  atomicMax(reinterpret_cast<unsigned long long*>(nfirst + f), (unsigned long long)(~packed));
+}
+
+__device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
+  const int64_t q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+// its cell (append_nodes' key): cubes of `cell` voxels, or the reference's default key - a chunk
+// (32 voxels) long in x, `cell` in y and z; each label starts as the fragment itself
+__global__ void __launch_bounds__(kWG) k_frag_cells(const uint64_t* __restrict__ nfirst, int cell, int cubes,
+                                                    uint64_t* __restrict__ cellkey, int32_t* __restrict__ label,
+                                                    int nfrag) {
+  const int f = int(blockIdx.x) * kWG + threadIdx.x;
+  if (f >= nfrag) return;
+  const uint64_t packed = ~nfirst[f];
+  const int64_t mask = (int64_t(1) << 21) - 1;
+  const int64_t x = int64_t(packed >> 42) - kCoordBias, y = int64_t((packed >> 21) & mask) - kCoordBias,
+                z = int64_t(packed & mask) - kCoordBias;
+  const int64_t cx = floor_div(x, cubes ? cell : 32), cy = floor_div(y, cell), cz = floor_div(z, cell);
+  cellkey[f] = (uint64_t(cx + kCoordBias) << 42) | (uint64_t(cy + kCoordBias) << 21) | uint64_t(cz + kCoordBias);
+  label[f] = f;
+}
+
+// a round: every link (two touching fragments of one cell) takes both ends, and their labels' own
+// labels, to the lesser label; then each label follows its label once. Labels only fall, and only
+// to labels of the same cluster, so a round that changes nothing leaves each cluster its least
+// fragment.
+__global__ void __launch_bounds__(kWG) k_cluster_link(const LatticeDev* __restrict__ Lp,
+                                                      const uint64_t* __restrict__ cellkey, int32_t* __restrict__ label,
+                                                      int32_t* __restrict__ changed) {
+  const LatticeDev& L = *Lp;
+  const int64_t slot = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (slot >= 3 * L.nvox) return;
+  Nbr n;
+  if (!neighbour(L, slot, &n) || n.a < 0 || n.b < 0 || n.a == n.b || cellkey[n.a] != cellkey[n.b]) return;
+  const int la = label[n.a], lb = label[n.b];
+  if (la == lb) return;
+  const int m = min(la, lb);
+  atomicMin(label + n.a, m);
+  atomicMin(label + n.b, m);
+  atomicMin(label + la, m);
+  atomicMin(label + lb, m);
+  *changed = 1;
+}
+__global__ void __launch_bounds__(kWG) k_cluster_jump(int32_t* __restrict__ label, int nfrag) {
+  const int f = int(blockIdx.x) * kWG + threadIdx.x;
+  if (f >= nfrag) return;
+  const int l = label[f];
+  const int ll = label[l];
+  if (ll < l) atomicMin(label + f, ll);
+}
+
+__global__ void __launch_bounds__(kWG) k_cluster_roots(const int32_t* __restrict__ label, int32_t* __restrict__ root,
+                                                       int nfrag) {
+  const int f = int(blockIdx.x) * kWG + threadIdx.x;
+  if (f < nfrag) root[f] = label[f] == f ? 1 : 0;
+}
+
+// a fragment's node: its cluster's rank (clusters in order of their least fragment); the sort keys
+// of the fragments by node, and the nodes' member counts
+__global__ void __launch_bounds__(kWG) k_cluster_map(const int32_t* __restrict__ label, const int32_t* __restrict__ rank,
+                                                     int32_t* __restrict__ fnode, uint32_t* __restrict__ key,
+                                                     int32_t* __restrict__ idx, int32_t* __restrict__ count, int nfrag) {
+  const int f = int(blockIdx.x) * kWG + threadIdx.x;
+  if (f >= nfrag) return;
+  const int nd = rank[label[f]];
+  fnode[f] = nd;
+  key[f] = uint32_t(nd);
+  idx[f] = f;
+  atomicAdd(count + nd, 1);
+}
+
+// a thread a node, over its fragments in index order (world.cpp:771-797): the masses summed, the
+// centres weighted by them, then times 1 / mass; the heaviest fragment's material (the first of
+// equal ones)
+__global__ void __launch_bounds__(kWG) k_cluster_nodes(const int32_t* __restrict__ member, const int32_t* __restrict__ start,
+                                                       const svxg_node* __restrict__ fnodes,
+                                                       const uint8_t* __restrict__ fmat, svxg_node* __restrict__ nodes,
+                                                       uint8_t* __restrict__ nmat, int nnodes) {
+  const int c = int(blockIdx.x) * kWG + threadIdx.x;
+  if (c >= nnodes) return;
+  double mass = 0.0, heavy = -1.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  uint8_t mat = 0;
+  for (int k = start[c]; k < start[c + 1]; ++k) {
+    const svxg_node& fn = fnodes[member[k]];
+    mass += fn.mass;
+    cx += fn.c[0] * fn.mass;
+    cy += fn.c[1] * fn.mass;
+    cz += fn.c[2] * fn.mass;
+    if (fn.mass > heavy) {
+      heavy = fn.mass;
+      mat = fmat[member[k]];
+    }
+  }
+  svxg_node nd;
+  memset(&nd, 0, sizeof nd);
+  if (mass > 0) {
+    const double inv = 1.0 / mass;
+    nd.c[0] = cx * inv;
+    nd.c[1] = cy * inv;
+    nd.c[2] = cz * inv;
+  } else {
+    nd.c[0] = cx;
+    nd.c[1] = cy;
+    nd.c[2] = cz;
+  }
+  nd.mass = mass;
+  nodes[c] = nd;
+  nmat[c] = mat;
+}
+
 }  // namespace
+
+void launch_frag_first(hipStream_t s, const LatticeDev* L, uint64_t* nfirst, int64_t nvox) {
+  if (nvox > 0) hipLaunchKernelGGL(k_frag_first, dim3(unsigned((nvox + kWG - 1) / kWG)), dim3(kWG), 0, s, L, nfirst);
+}
+void launch_frag_cells(hipStream_t s, const LatticeDev*, const uint64_t* nfirst, int cell, int cubes, uint64_t* cellkey,
+                       int32_t* label, int nfrag) {
+  if (nfrag > 0)
+    hipLaunchKernelGGL(k_frag_cells, dim3((nfrag + kWG - 1) / kWG), dim3(kWG), 0, s, nfirst, cell, cubes, cellkey,
+                       label, nfrag);
+}
+void launch_cluster_round(hipStream_t s, const LatticeDev* L, const uint64_t* cellkey, int32_t* label, int32_t* changed,
+                          int64_t nslots, int nfrag) {
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_cluster_link, dim3(unsigned((nslots + kWG - 1) / kWG)), dim3(kWG), 0, s, L, cellkey, label,
+                       changed);
+  if (nfrag > 0) hipLaunchKernelGGL(k_cluster_jump, dim3((nfrag + kWG - 1) / kWG), dim3(kWG), 0, s, label, nfrag);
+}
+void launch_cluster_ids(hipStream_t s, const int32_t* label, int32_t* root, int nfrag) {
+  if (nfrag > 0) hipLaunchKernelGGL(k_cluster_roots, dim3((nfrag + kWG - 1) / kWG), dim3(kWG), 0, s, label, root, nfrag);
+}
+void launch_cluster_map(hipStream_t s, const int32_t* label, const int32_t* rank, int32_t* fnode, uint32_t* key,
+                        int32_t* idx, int32_t* count, int nfrag) {
+  if (nfrag > 0)
+    hipLaunchKernelGGL(k_cluster_map, dim3((nfrag + kWG - 1) / kWG), dim3(kWG), 0, s, label, rank, fnode, key, idx,
+                       count, nfrag);
+}
+void launch_cluster_nodes(hipStream_t s, const int32_t* member, const int32_t* start, const svxg_node* fnodes,
+                          const uint8_t* fmat, svxg_node* nodes, uint8_t* nmat, int nnodes) {
+  if (nnodes > 0)
+    hipLaunchKernelGGL(k_cluster_nodes, dim3((nnodes + kWG - 1) / kWG), dim3(kWG), 0, s, member, start, fnodes, fmat,
+                       nodes, nmat, nnodes);
+}
 
 void launch_face_flags(hipStream_t s, const LatticeDev* L, int32_t* flag, int64_t nslots) {
   if (nslots > 0) hipLaunchKernelGGL(k_face_flags, dim3(unsigned((nslots + kWG - 1) / kWG)), dim3(kWG), 0, s, L, flag);

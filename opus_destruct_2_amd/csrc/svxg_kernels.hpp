@@ -397,12 +397,25 @@ void launch_bond_sections_general(hipStream_t s, const SectionGenDev* dev, int m
 struct LatticeDev {
   const uint8_t* vox;
   const int32_t* frag;
+  const int32_t* fnode;  // a fragment's node (clusters), or null: a node a fragment
   const svxg_material* mats;
   int32_t lo[3], dim[3];
   int64_t nvox;
   int32_t nfrag, nmats;
   double h;
 };
+// clusters of fragments (append_nodes, world.cpp:741-800): each fragment's first voxel and cell,
+// rounds of min-label propagation over the links inside cells, the labels' ranks, the nodes
+void launch_frag_first(hipStream_t s, const LatticeDev* L, uint64_t* nfirst, int64_t nvox);
+void launch_frag_cells(hipStream_t s, const LatticeDev* L, const uint64_t* nfirst, int cell, int cubes,
+                       uint64_t* cellkey, int32_t* label, int nfrag);
+void launch_cluster_round(hipStream_t s, const LatticeDev* L, const uint64_t* cellkey, int32_t* label,
+                          int32_t* changed, int64_t nslots, int nfrag);
+void launch_cluster_ids(hipStream_t s, const int32_t* label, int32_t* root, int nfrag);
+void launch_cluster_map(hipStream_t s, const int32_t* label, const int32_t* rank, int32_t* fnode, uint32_t* key,
+                        int32_t* idx, int32_t* count, int nfrag);
+void launch_cluster_nodes(hipStream_t s, const int32_t* member, const int32_t* start, const svxg_node* fnodes,
+                          const uint8_t* fmat, svxg_node* nodes, uint8_t* nmat, int nnodes);
 void launch_face_flags(hipStream_t s, const LatticeDev* L, int32_t* flag, int64_t nslots);
 void launch_face_write(hipStream_t s, const LatticeDev* L, const int32_t* flag, const int32_t* at, uint64_t* key,
                        int32_t* idx, svxg_section_face* face, uint8_t* mb, int64_t nslots);

@@ -66,10 +66,56 @@ def voronoi_block(nx, ny, nz, cell=2, voids=0.1, ground=True, seed=1, materials=
     return vox, frag, lo
 
 
-def extract(vox, frag, lo, h, mats):
+CHUNK = 32  # (the reference's chunk edge, core/include/svx/world/grid.hpp:41)
+
+
+def cluster_cell(fragments, limit=2500):
+    """World::Impl::cluster_cell (world.cpp:733-737): no clusters up to `limit` fragments
+    (WorldConfig::cluster_nodes, 2500), cells of 8 voxels up to six times that, else 16."""
+    if fragments <= limit:
+        return 0
+    return 8 if fragments <= 6 * limit else 16
+
+
+def cell_keys(first, cell, cubes=False):
+    """A fragment's cell (append_nodes' key, world.cpp:741-755) from the global coordinates of its
+    first voxel (its voxels' least (x, y, z), x first): cubes of `cell` voxels with `cubes`; else
+    the reference's default key, whose packing drops the cell along x - cells `cell` voxels in y and
+    z and a chunk long in x."""
+    first = np.asarray(first, np.int64)
+    cx = np.floor_divide(first[:, 0], cell if cubes else CHUNK)
+    return np.stack([cx, np.floor_divide(first[:, 1], cell), np.floor_divide(first[:, 2], cell)], -1)
+
+
+def cluster_items(keys, links):
+    """cluster_items (world_internal.hpp:466-491): items with the same key that are linked, directly
+    or through each other, form a cluster; clusters numbered in order of first appearance (the least
+    item index). keys: (n, k) array; links: (m, 2) item pairs. Returns (cluster of each item, count)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    n = len(keys)
+    links = np.asarray(links, np.int64).reshape(-1, 2)
+    same = np.all(keys[links[:, 0]] == keys[links[:, 1]], 1) if len(links) else np.zeros(0, bool)
+    L = links[same]
+    g = coo_matrix((np.ones(len(L)), (L[:, 0], L[:, 1])), shape=(n, n))
+    _, comp = connected_components(g, directed=False)
+    root = np.full(comp.max() + 1 if n else 0, n, np.int64)
+    np.minimum.at(root, comp, np.arange(n))
+    order = np.argsort(root, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return rank[comp].astype(np.int32), len(order)
+
+
+def extract(vox, frag, lo, h, mats, cell=0, cubes=False):
     """(nodes, heads, faces): a node a fragment (its centre of mass and mass from the materials'
     densities), a section head and face list a bond (pairs of touching fragments; a fragment on an
-    anchored voxel, one a side of an axis). vox, frag: (z, y, x) arrays over the box from lo."""
+    anchored voxel, one a side of an axis). vox, frag: (z, y, x) arrays over the box from lo.
+    cell > 0: touching fragments whose first voxels share a cell (cell_keys) make one node
+    (append_nodes, world.cpp:741-800): its mass the fragments' masses summed in index order, its
+    centre their centres weighted by their masses (summed, then times 1 / mass), its material the
+    heaviest fragment's; faces between fragments of one node drop out. Returns
+    (nodes, heads, faces, frag_node) then."""
     vox = np.ascontiguousarray(vox, np.uint8)
     frag = np.ascontiguousarray(frag, np.int32)
     nzb, nyb, nxb = frag.shape
@@ -91,6 +137,34 @@ def extract(vox, frag, lo, h, mats):
     nodes = np.zeros(nf, NODE)
     nodes["c"] = com
     nodes["mass"] = mass
+    frag_node = None
+    if cell > 0:
+        # each fragment's first voxel, its cell, the links between touching fragments, the clusters
+        g = np.stack([xx + lo[0], yy + lo[1], zz + lo[2]], -1).astype(np.int64)
+        B = 1 << 20
+        packed = ((g[:, 0] + B) << 42) | ((g[:, 1] + B) << 21) | (g[:, 2] + B)
+        fmin = np.full(nf, np.iinfo(np.int64).max, np.int64)
+        np.minimum.at(fmin, f, packed)
+        first = np.stack([(fmin >> 42) - B, ((fmin >> 21) & ((1 << 21) - 1)) - B, (fmin & ((1 << 21) - 1)) - B], -1)
+        links = []
+        for axis in range(3):
+            sl_p, sl_q = [slice(None)] * 3, [slice(None)] * 3
+            sl_p[2 - axis] = slice(0, frag.shape[2 - axis] - 1)
+            sl_q[2 - axis] = slice(1, frag.shape[2 - axis])
+            a_, b_ = frag[tuple(sl_p)].ravel(), frag[tuple(sl_q)].ravel()
+            pr = (a_ >= 0) & (b_ >= 0) & (a_ != b_)
+            links.append(np.stack([a_[pr], b_[pr]], -1))
+        frag_node, nn = cluster_items(cell_keys(first, cell, cubes), np.concatenate(links, 0))
+        # nodes: the members in index order (bincount sums in input order)
+        cmass = np.bincount(frag_node, weights=mass, minlength=nn)
+        ccom = np.stack([np.bincount(frag_node, weights=com[:, k] * mass, minlength=nn) for k in range(3)], -1)
+        order = np.lexsort((np.arange(nf), -mass, frag_node))
+        heavy = order[np.r_[True, frag_node[order][1:] != frag_node[order][:-1]]]
+        nmat = nmat[heavy]
+        nodes = np.zeros(nn, NODE)
+        nodes["c"] = ccom * (1.0 / cmass)[:, None]
+        nodes["mass"] = cmass
+        frag = np.where(frag >= 0, frag_node[np.maximum(frag, 0)], -1).astype(np.int32)
     # faces: for each axis, voxel p and its neighbour q = p + e
     recs = []  # (a, kind, key2, scan, px, py, pz, axis, sign, mb)
     flat = lambda z, y, x: (x + nxb * (y + nyb * z)).astype(np.int64)
@@ -145,6 +219,8 @@ def extract(vox, frag, lo, h, mats):
     heads["strength_b"] = np.where(pair, 1.0, 1e9)
     # a pair face's sign: +1 when node a is on its lower side; a support face: the side of a
     faces["sign"] = R[:, 8]
+    if cell > 0:
+        return nodes, heads, faces, frag_node
     return nodes, heads, faces
 
 
@@ -443,6 +519,36 @@ def bond_sections_general(nodes, heads, faces, samples, grids, S, mats, ctx=None
         samples.ctypes.data, len(samples), lat, len(grids), int(S), mats.ctypes.data, len(mats), out.ctypes.data),
         "svxg_bond_sections_general")
     return out
+
+
+def lattice_graph_clustered(vox, frag, lo, h, mats, cell, cubes=False, damage=None, ctx=None):
+    """svxg_lattice_graph_clustered: (nodes, heads, faces, bonds, frag_node), the records of
+    extract(cell=...) followed by bond_sections(), all on the device."""
+    from . import stress
+    ctx = ctx or stress.default_context()
+    vox = np.ascontiguousarray(vox, np.uint8)
+    frag = np.ascontiguousarray(frag, np.int32)
+    damage = None if damage is None else np.ascontiguousarray(damage, np.uint8)
+    mats = np.ascontiguousarray(mats)
+    box = voxel_box(vox, lo, h, damage)
+    nfrag = int(frag.max()) + 1
+    nn, nb, nf = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    fnode = np.zeros(nfrag, np.int32)
+    L = stress.lib()
+    stress._check(L.svxg_lattice_graph_clustered(ctx.h, ctypes.byref(box), frag.ctypes.data, nfrag, int(cell),
+                                                 int(bool(cubes)), mats.ctypes.data, len(mats), fnode.ctypes.data,
+                                                 ctypes.byref(nn), None, ctypes.byref(nb), ctypes.byref(nf), None,
+                                                 None, None), "svxg_lattice_graph_clustered")
+    nodes = np.zeros(max(1, nn.value), NODE)
+    heads = np.zeros(max(1, nb.value), SECTION_HEAD)
+    faces = np.zeros(max(1, nf.value), SECTION_FACE)
+    bonds = np.zeros(max(1, nb.value), BOND)
+    stress._check(L.svxg_lattice_graph_clustered(ctx.h, ctypes.byref(box), frag.ctypes.data, nfrag, int(cell),
+                                                 int(bool(cubes)), mats.ctypes.data, len(mats), fnode.ctypes.data,
+                                                 ctypes.byref(nn), nodes.ctypes.data, ctypes.byref(nb),
+                                                 ctypes.byref(nf), heads.ctypes.data, faces.ctypes.data,
+                                                 bonds.ctypes.data), "svxg_lattice_graph_clustered")
+    return nodes[:nn.value], heads[:nb.value], faces[:nf.value], bonds[:nb.value], fnode
 
 
 def gravity(nodes):

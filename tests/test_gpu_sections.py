@@ -146,3 +146,39 @@ def test_general_sections_edge_cases(mats, ctx):
         voxels.bond_sections_general(nodes, bad, faces, samples, grids, 3, mats, ctx=ctx)
     with pytest.raises(stress.SvxgError):
         voxels.bond_sections_general(nodes, heads, faces, samples, grids, 0, mats, ctx=ctx)
+
+
+@pytest.mark.parametrize("cell,cubes,mixed", [(8, False, False), (8, True, False), (16, False, True), (16, True, False),
+                                              (4, True, True)])
+def test_clustered_lattice_graph_is_the_host_extraction(cell, cubes, mixed, mats, ctx):
+    """svxg_lattice_graph_clustered (fragments' first voxels and cells, labels propagated over the
+    links inside cells, clusters ranked, nodes summed, then the graph over the nodes, all on the
+    device) gives extract(cell=...)'s nodes, heads, faces and fragment map and the oracle's sections
+    of them, bit for bit."""
+    vox, frag, lo = voxels.voronoi_block(48, 40, 30, cell=2, voids=0.1, seed=cell + 3 * cubes)
+    rng = np.random.default_rng(cell)
+    if mixed:
+        solid = frag >= 0
+        vox[solid] = rng.integers(0, 12, int(solid.sum())) + 1
+    damage = np.where(rng.random(vox.shape) < 0.05, rng.integers(1, 256, vox.shape), 0).astype(np.uint8)
+    nodes, heads, faces, fn = voxels.extract(vox, frag, lo, 0.25, mats, cell=cell, cubes=cubes)
+    bonds = oracle.bond_sections(nodes, heads, faces, vox, lo, 0.25, mats, damage=damage)
+    dn, dh, df, db, dfn = voxels.lattice_graph_clustered(vox, frag, lo, 0.25, mats, cell, cubes, damage=damage, ctx=ctx)
+    assert len(dn) < int(frag.max()) + 1
+    assert np.array_equal(dfn, fn)
+    assert dn.tobytes() == nodes.tobytes()
+    assert dh.tobytes() == heads.tobytes()
+    assert df.tobytes() == faces.tobytes()
+    assert db.tobytes() == bonds.tobytes()
+
+
+def test_clustered_lattice_graph_edge_cases(mats, ctx):
+    vox, frag, lo = voxels.voronoi_block(12, 10, 8, cell=2, voids=0.0, seed=2)
+    # cell 0: the unclustered graph, each fragment its own node
+    dn, dh, df, db, dfn = voxels.lattice_graph_clustered(vox, frag, lo, 0.25, mats, 0, ctx=ctx)
+    n2, h2, f2, b2 = voxels.lattice_graph(vox, frag, lo, 0.25, mats, ctx=ctx)
+    assert dn.tobytes() == n2.tobytes() and db.tobytes() == b2.tobytes()
+    assert np.array_equal(dfn, np.arange(int(frag.max()) + 1))
+    for bad in (3, 64, -1):  # (a cell divides the 32-voxel chunk)
+        with pytest.raises(stress.SvxgError):
+            voxels.lattice_graph_clustered(vox, frag, lo, 0.25, mats, bad, ctx=ctx)

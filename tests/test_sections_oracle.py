@@ -411,3 +411,101 @@ def test_grids_scene_carries_its_weight(S, mats):
     sup = b["b"] < 0
     Fz = loads["N"][sup] * b["n"][sup, 2] + loads["V1"][sup] * b["t1"][sup, 2] + loads["V2"][sup] * b["t2"][sup, 2]
     assert np.isclose(Fz.sum(), 9.81 * nodes["mass"].sum(), rtol=1e-6)
+
+
+# ---- clusters of fragments (append_nodes, world.cpp:733-800; cluster_items, world_internal.hpp:466-491)
+
+def _bfs_clusters(keys, links):
+    """Clusters by breadth-first search from each unvisited item in index order (numbered as found)."""
+    n = len(keys)
+    adj = [[] for _ in range(n)]
+    for a, b in links:
+        if tuple(keys[a]) == tuple(keys[b]):
+            adj[a].append(b)
+            adj[b].append(a)
+    out, k = [-1] * n, 0
+    for i in range(n):
+        if out[i] >= 0:
+            continue
+        out[i], todo = k, [i]
+        while todo:
+            x = todo.pop()
+            for y in adj[x]:
+                if out[y] < 0:
+                    out[y] = k
+                    todo.append(y)
+        k += 1
+    return np.array(out), k
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_cluster_items_against_a_search(seed):
+    rng = np.random.default_rng(seed)
+    n = 60
+    keys = rng.integers(0, 3, (n, 2))
+    links = rng.integers(0, n, (90, 2))
+    got, k = voxels.cluster_items(keys, links)
+    want, kw = _bfs_clusters(keys, links)
+    assert k == kw and np.array_equal(got, want)
+
+
+def test_cluster_cell_rule():
+    assert voxels.cluster_cell(2500) == 0 and voxels.cluster_cell(2501) == 8
+    assert voxels.cluster_cell(15000) == 8 and voxels.cluster_cell(15001) == 16
+    # the default key drops the cell along x: a chunk (32 voxels) long; cubes keep it
+    first = np.array([[0, 0, 0], [31, 7, 7], [32, 0, 0], [-1, 8, 0]])
+    assert [tuple(k) for k in voxels.cell_keys(first, 8)] == [(0, 0, 0), (0, 0, 0), (1, 0, 0), (-1, 1, 0)]
+    assert [tuple(k) for k in voxels.cell_keys(first, 8, cubes=True)] == [(0, 0, 0), (3, 0, 0), (4, 0, 0), (-1, 1, 0)]
+
+
+@pytest.mark.parametrize("cell,cubes", [(8, False), (8, True), (16, False), (16, True)])
+def test_clustered_graph_properties(cell, cubes, mats):
+    """Clustered nodes: each node's fragments touch (connected through links inside it) and share a
+    cell, no link joins two nodes of one cell, the mass and the centre are the voxels' (to rounding),
+    no face joins a node to itself, and the structure carries its weight through its supports."""
+    vox, frag, lo = voxels.voronoi_block(34, 20, 18, cell=2, voids=0.1, seed=cell + cubes)
+    n0, h0, f0 = voxels.extract(vox, frag, lo, H, mats)
+    nodes, heads, faces, fn = voxels.extract(vox, frag, lo, H, mats, cell=cell, cubes=cubes)
+    assert 1 < len(nodes) < len(n0)
+    # first voxels and keys of the fragments
+    zz, yy, xx = np.nonzero(frag >= 0)
+    g = np.stack([xx + lo[0], yy + lo[1], zz + lo[2]], -1)
+    f = frag[zz, yy, xx]
+    first = np.zeros((len(n0), 3), np.int64)
+    for k in range(len(n0)):
+        sel = g[f == k]
+        first[k] = sel[np.lexsort((sel[:, 2], sel[:, 1], sel[:, 0]))[0]]
+    keys = voxels.cell_keys(first, cell, cubes)
+    for nd in range(len(nodes)):
+        members = np.flatnonzero(fn == nd)
+        assert len({tuple(k) for k in keys[members]}) == 1
+    # links inside a cell join fragments of one node only (maximal), across nodes none
+    links = set()
+    for axis in range(3):
+        sl_p, sl_q = [slice(None)] * 3, [slice(None)] * 3
+        sl_p[2 - axis] = slice(0, frag.shape[2 - axis] - 1)
+        sl_q[2 - axis] = slice(1, frag.shape[2 - axis])
+        a_, b_ = frag[tuple(sl_p)].ravel(), frag[tuple(sl_q)].ravel()
+        pr = (a_ >= 0) & (b_ >= 0) & (a_ != b_)
+        links |= set(zip(a_[pr].tolist(), b_[pr].tolist()))
+    for a, b in links:
+        if tuple(keys[a]) == tuple(keys[b]):
+            assert fn[a] == fn[b]
+    # numbering in order of the least fragment
+    least = np.array([np.flatnonzero(fn == nd)[0] for nd in range(len(nodes))])
+    assert np.all(np.diff(least) > 0)
+    assert np.isclose(nodes["mass"].sum(), n0["mass"].sum(), rtol=1e-13)
+    for nd in range(0, len(nodes), max(1, len(nodes) // 7)):
+        sel = fn[f] == nd
+        m = n0["mass"][f[sel]] / np.bincount(f)[f[sel]]  # (each voxel's mass: its fragment's share)
+        assert np.allclose(nodes["c"][nd], (g[sel] * H * m[:, None]).sum(0) / m.sum(), rtol=1e-12, atol=1e-12)
+    assert np.all((heads["b"] < 0) | (heads["a"] < heads["b"]))
+    b = oracle.bond_sections(nodes, heads, faces, vox, lo, H, mats)
+    p = oracle.CpuProblem(nodes, b, mats)
+    assert p.assemble()
+    u, r = p.solve(voxels.gravity(nodes), rtol=1e-11, maxit=5000)
+    assert r["converged"]
+    loads = p.bond_loads(u)
+    sup = b["b"] < 0
+    Fz = loads["N"][sup] * b["n"][sup, 2] + loads["V1"][sup] * b["t1"][sup, 2] + loads["V2"][sup] * b["t2"][sup, 2]
+    assert np.isclose(Fz.sum(), 9.81 * nodes["mass"].sum(), rtol=1e-6)
