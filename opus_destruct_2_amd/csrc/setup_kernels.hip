@@ -1349,13 +1349,14 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
-  if (int(blockIdx.x) >= count) return;
-  const DenseDev& d = J[blockIdx.x];
-  const int m = d.m, mp = d.mp, NB = mp / 16;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  double* L = d.L;  // mp x mp: the matrix, then its factor (lower tiles)
-  double* W = d.W;  // mp x mp: L^-1 (lower tiles)
+// the padded matrix in L (identity below m), W zeroed, the blocks scattered, the lower triangle
+// symmetrised (amg.cpp:535-578); returns max |a_ii| (the pivot floor's scale). A 1024-thread
+// workgroup.
+__device__ double dense_prep(const DenseDev& d) {
+  const int m = d.m, mp = d.mp;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* L = d.L;
+  double* W = d.W;
   auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(mp) + size_t(c)]; };
   // (rows a wave, columns its lanes)
   for (int r = wave; r < mp; r += kDenseWaves)
@@ -1384,80 +1385,133 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
   __syncthreads();
   double dmax = 0.0;
   for (int w = 0; w < kDenseWaves; ++w) dmax = fmax(dmax, red[w]);
+  return dmax;
+}
+
+// block column kb: wave 0 factors the diagonal tile in LDS (the reference's pivot floor) and forms
+// its triangular inverse Ik (also W's diagonal tile); then the waves take the panel below,
+// L_ib,kb = A_ib,kb (L_kb,kb)^-T. A 1024-thread workgroup.
+__device__ void dense_panel(const DenseDev& d, int kb, double dmax) {
+  const int mp = d.mp, NB = mp / 16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* L = d.L;
+  double* W = d.W;
+  auto at = [&](int r, int c) -> double& { return L[size_t(r) * size_t(mp) + size_t(c)]; };
   __shared__ double Dk[16][17];
   __shared__ double Ik[16][17];
-  __shared__ double Sw[kDenseWaves][16][17];
+  const int k0 = 16 * kb;
+  if (wave == 0) {
+    for (int e = lane; e < 256; e += 64) {
+      const int r = e >> 4, c = e & 15;
+      Dk[r][c] = c <= r ? at(k0 + r, k0 + c) : 0.0;
+    }
+    wave_lds_sync();
+    for (int j = 0; j < 16; ++j) {
+      double dd = Dk[j][j];
+      if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
+      const double lj = sqrt(dd);
+      wave_lds_sync();
+      if (lane == j) Dk[j][j] = lj;
+      else if (lane > j && lane < 16) Dk[lane][j] /= lj;
+      wave_lds_sync();
+      for (int e = lane; e < 256; e += 64) {
+        const int r = e >> 4, c = e & 15;
+        if (c > j && c <= r) Dk[r][c] -= Dk[r][j] * Dk[c][j];
+      }
+      wave_lds_sync();
+    }
+    if (lane < 16) {  // column `lane` of the inverse by forward substitution
+      const int c = lane;
+      for (int r = 0; r < c; ++r) Ik[r][c] = 0.0;
+      Ik[c][c] = 1.0 / Dk[c][c];
+      for (int r = c + 1; r < 16; ++r) {
+        double s = 0.0;
+        for (int q = c; q < r; ++q) s += Dk[r][q] * Ik[q][c];
+        Ik[r][c] = -s / Dk[r][r];
+      }
+    }
+    wave_lds_sync();
+    for (int e = lane; e < 256; e += 64) {
+      const int r = e >> 4, c = e & 15;
+      at(k0 + r, k0 + c) = Dk[r][c];
+      W[size_t(k0 + r) * size_t(mp) + size_t(k0 + c)] = Ik[r][c];
+    }
+  }
+  __syncthreads();
+  for (int ib = kb + 1 + wave; ib < NB; ib += kDenseWaves) {
+    double* T = &at(16 * ib, k0);
+    dbl4 c = {0.0, 0.0, 0.0, 0.0};
+    c = tile_mma<false, true>(c, T, mp, &Ik[0][0], 17, 1.0);
+    tile_store(T, mp, c);
+  }
+  __syncthreads();
+}
+
+// tile t of a lower triangle of n tile rows: (row a, column t - a(a+1)/2)
+__device__ __forceinline__ void tri_tile(int t, int* a_out, int* b_out) {
+  int a = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (a * (a + 1) / 2 > t) --a;
+  while ((a + 1) * (a + 2) / 2 <= t) ++a;
+  *a_out = a;
+  *b_out = t - a * (a + 1) / 2;
+}
+
+// the trailing lower triangle's tile t after block column kb: A_ib,jb -= L_ib,kb L_jb,kb^T
+__device__ __forceinline__ void dense_update_tile(const DenseDev& d, int kb, int t) {
+  const int mp = d.mp, k0 = 16 * kb;
+  int a, b;
+  tri_tile(t, &a, &b);
+  const int ib = kb + 1 + a, jb = kb + 1 + b;
+  double* L = d.L;
+  double* C = L + size_t(16 * ib) * size_t(mp) + size_t(16 * jb);
+  dbl4 c = tile_load(C, mp);
+  c = tile_mma<false, true>(c, L + size_t(16 * ib) * size_t(mp) + size_t(k0), mp,
+                            L + size_t(16 * jb) * size_t(mp) + size_t(k0), mp, -1.0);
+  tile_store(C, mp, c);
+}
+
+// M^-1's lower tile t = sum over kb >= ib of W_kb,ib^T W_kb,jb, written and mirrored
+__device__ __forceinline__ void dense_wtw_tile(const DenseDev& d, int t) {
+  const int m = d.m, mp = d.mp, NB = mp / 16;
+  const int lane = threadIdx.x & 63;
+  int ib, jb;
+  tri_tile(t, &ib, &jb);
+  const double* W = d.W;
+  dbl4 c = {0.0, 0.0, 0.0, 0.0};
+  for (int kb = ib; kb < NB; ++kb)
+    c = tile_mma<true, false>(c, &W[size_t(16 * kb) * size_t(mp) + size_t(16 * ib)], mp,
+                              &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)], mp, 1.0);
+  const int i = lane & 15, g = lane >> 4;
+  for (int r = 0; r < 4; ++r) {
+    const int row = 16 * ib + g + 4 * r, col = 16 * jb + i;
+    if (row >= m || col >= m || col > row) continue;
+    d.Minv[size_t(row) * size_t(m) + size_t(col)] = c[r];
+    d.Minv[size_t(col) * size_t(m) + size_t(row)] = c[r];
+  }
+}
+
+__global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __restrict__ J, int count) {
+  if (int(blockIdx.x) >= count) return;
+  const DenseDev& d = J[blockIdx.x];
+  const int mp = d.mp, NB = mp / 16;
+  const int wave = threadIdx.x >> 6;
+  double* L = d.L;
+  double* W = d.W;
+  const double dmax = dense_prep(d);
   for (int kb = 0; kb < NB; ++kb) {
-    const int k0 = 16 * kb;
-    if (wave == 0) {
-      // the diagonal tile: unblocked Cholesky in LDS, then its triangular inverse
-      for (int e = lane; e < 256; e += 64) {
-        const int r = e >> 4, c = e & 15;
-        Dk[r][c] = c <= r ? at(k0 + r, k0 + c) : 0.0;
-      }
-      wave_lds_sync();
-      for (int j = 0; j < 16; ++j) {
-        double dd = Dk[j][j];
-        if (!(dd > 1e-12 * dmax)) dd = fmax(1e-12 * dmax, 1e-300);
-        const double lj = sqrt(dd);
-        wave_lds_sync();
-        if (lane == j) Dk[j][j] = lj;
-        else if (lane > j && lane < 16) Dk[lane][j] /= lj;
-        wave_lds_sync();
-        for (int e = lane; e < 256; e += 64) {
-          const int r = e >> 4, c = e & 15;
-          if (c > j && c <= r) Dk[r][c] -= Dk[r][j] * Dk[c][j];
-        }
-        wave_lds_sync();
-      }
-      if (lane < 16) {  // column `lane` of the inverse by forward substitution
-        const int c = lane;
-        for (int r = 0; r < c; ++r) Ik[r][c] = 0.0;
-        Ik[c][c] = 1.0 / Dk[c][c];
-        for (int r = c + 1; r < 16; ++r) {
-          double s = 0.0;
-          for (int q = c; q < r; ++q) s += Dk[r][q] * Ik[q][c];
-          Ik[r][c] = -s / Dk[r][r];
-        }
-      }
-      wave_lds_sync();
-      for (int e = lane; e < 256; e += 64) {
-        const int r = e >> 4, c = e & 15;
-        at(k0 + r, k0 + c) = Dk[r][c];
-        W[size_t(k0 + r) * size_t(mp) + size_t(k0 + c)] = Ik[r][c];
-      }
-    }
-    __syncthreads();
-    // the panel: L_ib,kb = A_ib,kb (L_kb,kb)^-T
-    for (int ib = kb + 1 + wave; ib < NB; ib += kDenseWaves) {
-      double* T = &at(16 * ib, k0);
-      dbl4 c = {0.0, 0.0, 0.0, 0.0};
-      c = tile_mma<false, true>(c, T, mp, &Ik[0][0], 17, 1.0);
-      tile_store(T, mp, c);
-    }
-    __syncthreads();
-    // the trailing lower triangle: A_ib,jb -= L_ib,kb L_jb,kb^T, kb < jb <= ib
+    dense_panel(d, kb, dmax);
     const int n = NB - kb - 1;
-    const int tiles = n * (n + 1) / 2;
-    for (int t = wave; t < tiles; t += kDenseWaves) {
-      int a = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-      while (a * (a + 1) / 2 > t) --a;
-      while ((a + 1) * (a + 2) / 2 <= t) ++a;
-      const int ib = kb + 1 + a, jb = kb + 1 + (t - a * (a + 1) / 2);
-      double* C = &at(16 * ib, 16 * jb);
-      dbl4 c = tile_load(C, mp);
-      c = tile_mma<false, true>(c, &at(16 * ib, k0), mp, &at(16 * jb, k0), mp, -1.0);
-      tile_store(C, mp, c);
-    }
+    for (int t = wave; t < n * (n + 1) / 2; t += kDenseWaves) dense_update_tile(d, kb, t);
     __syncthreads();
   }
   // W = L^-1 below the diagonal: W_ib,jb = -W_ib,ib sum_{jb <= kb < ib} L_ib,kb W_kb,jb
+  __shared__ double Sw[kDenseWaves][16][17];
   for (int ib = 1; ib < NB; ++ib) {
     for (int jb = wave; jb < ib; jb += kDenseWaves) {
       dbl4 c = {0.0, 0.0, 0.0, 0.0};
       for (int kb = jb; kb < ib; ++kb)
-        c = tile_mma<false, false>(c, &at(16 * ib, 16 * kb), mp, &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)],
-                                   mp, 1.0);
+        c = tile_mma<false, false>(c, &L[size_t(16 * ib) * size_t(mp) + size_t(16 * kb)], mp,
+                                   &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)], mp, 1.0);
       tile_store(&Sw[wave][0][0], 17, c);
       wave_lds_sync();
       dbl4 z = {0.0, 0.0, 0.0, 0.0};
@@ -1467,25 +1521,80 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
     }
     __syncthreads();
   }
-  // M^-1 = W^T W: tile (ib, jb), ib >= jb, sums over kb >= ib; lower part written and mirrored
-  const int tiles = NB * (NB + 1) / 2;
-  for (int t = wave; t < tiles; t += kDenseWaves) {
-    int a = int((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while (a * (a + 1) / 2 > t) --a;
-    while ((a + 1) * (a + 2) / 2 <= t) ++a;
-    const int ib = a, jb = t - a * (a + 1) / 2;
+  for (int t = wave; t < NB * (NB + 1) / 2; t += kDenseWaves) dense_wtw_tile(d, t);
+}
+
+// ---- the same over many workgroups (coarsest levels above kDenseOneBlocks tile rows) ------------------
+//
+// A level of C2's size (864 rows, 54 tile rows) leaves one workgroup with some 27k tile products in
+// sequence. Spread: per block column, a workgroup a graph factors the diagonal tile and solves the
+// panel (the sequential part), then a wave a trailing tile updates the lower triangle; L^-1 goes
+// column block by column block, a workgroup each (its rows in order, each row's sum split over the
+// 16 waves and added in wave order); M^-1 = W^T W a wave a tile. The arithmetic of a tile product is
+// the one-workgroup kernel's; L^-1's sums are grouped differently (same terms, a fixed order), so a
+// graph's inverse depends only on its size class, never on its batch.
+
+__global__ void __launch_bounds__(kDenseWG) k_dense_prep(const DenseDev* __restrict__ J, int count,
+                                                         double* __restrict__ dmax) {
+  if (int(blockIdx.x) >= count) return;
+  const double dm = dense_prep(J[blockIdx.x]);
+  if (threadIdx.x == 0) dmax[blockIdx.x] = dm;
+}
+
+__global__ void __launch_bounds__(kDenseWG) k_dense_panel(const DenseDev* __restrict__ J, int count,
+                                                          const double* __restrict__ dmax, int kb) {
+  if (int(blockIdx.x) >= count) return;
+  const DenseDev& d = J[blockIdx.x];
+  if (kb >= d.mp / 16) return;
+  dense_panel(d, kb, dmax[blockIdx.x]);
+}
+
+// a wave a tile: grid (tile groups of four, graphs)
+__global__ void __launch_bounds__(256) k_dense_update(const DenseDev* __restrict__ J, int kb) {
+  const DenseDev& d = J[blockIdx.y];
+  const int n = d.mp / 16 - kb - 1;
+  const int t = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+  if (n <= 0 || t >= n * (n + 1) / 2) return;
+  dense_update_tile(d, kb, t);
+}
+
+// column block jb of W = L^-1: grid (column blocks, graphs)
+__global__ void __launch_bounds__(kDenseWG) k_dense_winv(const DenseDev* __restrict__ J) {
+  const DenseDev& d = J[blockIdx.y];
+  const int mp = d.mp, NB = mp / 16, jb = blockIdx.x;
+  if (jb >= NB - 1) return;
+  const int wave = threadIdx.x >> 6;
+  const double* L = d.L;
+  double* W = d.W;
+  __shared__ double Sp[kDenseWaves][16][17];
+  for (int ib = jb + 1; ib < NB; ++ib) {
     dbl4 c = {0.0, 0.0, 0.0, 0.0};
-    for (int kb = ib; kb < NB; ++kb)
-      c = tile_mma<true, false>(c, &W[size_t(16 * kb) * size_t(mp) + size_t(16 * ib)], mp,
-                                &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)], mp, 1.0);
-    const int i = lane & 15, g = lane >> 4;
-    for (int r = 0; r < 4; ++r) {
-      const int row = 16 * ib + g + 4 * r, col = 16 * jb + i;
-      if (row >= m || col >= m || col > row) continue;
-      d.Minv[size_t(row) * size_t(m) + size_t(col)] = c[r];
-      d.Minv[size_t(col) * size_t(m) + size_t(row)] = c[r];
+    for (int kb = jb + wave; kb < ib; kb += kDenseWaves)
+      c = tile_mma<false, false>(c, &L[size_t(16 * ib) * size_t(mp) + size_t(16 * kb)], mp,
+                                 &W[size_t(16 * kb) * size_t(mp) + size_t(16 * jb)], mp, 1.0);
+    tile_store(&Sp[wave][0][0], 17, c);
+    __syncthreads();
+    if (wave == 0) {
+      // (the waves' partial sums added in wave order, then times -W_ib,ib)
+      const int i = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
+      for (int w = 1; w < kDenseWaves && jb + w < ib; ++w)
+        for (int r = 0; r < 4; ++r) Sp[0][g + 4 * r][i] += Sp[w][g + 4 * r][i];
+      wave_lds_sync();
+      dbl4 z = {0.0, 0.0, 0.0, 0.0};
+      z = tile_mma<false, false>(z, &W[size_t(16 * ib) * size_t(mp) + size_t(16 * ib)], mp, &Sp[0][0][0], 17, -1.0);
+      tile_store(&W[size_t(16 * ib) * size_t(mp) + size_t(16 * jb)], mp, z);
     }
+    __syncthreads();
   }
+}
+
+// a wave a tile of M^-1's lower triangle: grid (tile groups of four, graphs)
+__global__ void __launch_bounds__(256) k_dense_wtw(const DenseDev* __restrict__ J) {
+  const DenseDev& d = J[blockIdx.y];
+  const int NB = d.mp / 16;
+  const int t = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+  if (t >= NB * (NB + 1) / 2) return;
+  dense_wtw_tile(d, t);
 }
 
 // ---- conversions -----------------------------------------------------------------------------------
@@ -1750,6 +1859,20 @@ void launch_power_iter(hipStream_t s, const PowerDev* jobs, const Tile* tiles, i
 }
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count) {
   if (count > 0) hipLaunchKernelGGL(k_dense_factor, dim3(count), dim3(kDenseWG), 0, s, jobs, count);
+}
+void launch_dense_factor_wide(hipStream_t s, const DenseDev* jobs, int count, int max_nb, double* dmax) {
+  if (count <= 0 || max_nb <= 0) return;
+  hipLaunchKernelGGL(k_dense_prep, dim3(count), dim3(kDenseWG), 0, s, jobs, count, dmax);
+  for (int kb = 0; kb < max_nb; ++kb) {
+    hipLaunchKernelGGL(k_dense_panel, dim3(count), dim3(kDenseWG), 0, s, jobs, count, dmax, kb);
+    const int n = max_nb - kb - 1;
+    if (n > 0)
+      hipLaunchKernelGGL(k_dense_update, dim3(unsigned((n * (n + 1) / 2 + 3) / 4), unsigned(count)), dim3(256), 0, s,
+                         jobs, kb);
+  }
+  if (max_nb > 1) hipLaunchKernelGGL(k_dense_winv, dim3(unsigned(max_nb - 1), unsigned(count)), dim3(kDenseWG), 0, s, jobs);
+  hipLaunchKernelGGL(k_dense_wtw, dim3(unsigned((max_nb * (max_nb + 1) / 2 + 3) / 4), unsigned(count)), dim3(256), 0,
+                     s, jobs);
 }
 void launch_to_f32(hipStream_t s, const double* in, float* out, int64_t count) {
   if (count <= 0) return;

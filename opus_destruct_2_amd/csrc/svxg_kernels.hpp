@@ -252,6 +252,10 @@ struct DenseDev {
   double* Minv;   // m x m
 };
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count);
+// the same spread over many workgroups (graphs whose padded size is above kDenseOneBlocks tile rows;
+// max_nb: the largest graph's tile rows; dmax: a double a graph of scratch)
+constexpr int kDenseOneBlocks = 18;
+void launch_dense_factor_wide(hipStream_t s, const DenseDev* jobs, int count, int max_nb, double* dmax);
 
 // FP32 copies
 void launch_to_f32(hipStream_t s, const double* in, float* out, int64_t count);

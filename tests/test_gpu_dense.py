@@ -46,3 +46,26 @@ def test_single_dense_level_is_the_exact_inverse(count, pinned, coarse_max, prec
     if precision == SVXG_FP64:
         assert res["iters"] <= 2, res
     p.close()
+
+
+def test_dense_levels_of_many_sizes_in_one_setup(mats, ctx):
+    """Dense levels of one tile row up to 72 (the one-workgroup factorisation up to 18 tile rows, the
+    many-workgroup one above), set up in one batch: each the exact inverse, and each bit for bit the
+    same as set up alone."""
+    sizes = [3, 31, 48, 49, 80, 120, 192]
+    opts = stress.default_options(precision=SVXG_FP64, amg_min_nodes=0, coarse_max=192)
+    gs = [gen.piece(n, pinned=bool(k & 1), seed=700 + n) for k, n in enumerate(sizes)]
+    batch = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
+    stress.assemble_batch(batch, ctx)
+    for g, p in zip(gs, batch):
+        assert p.info()["levels"] == 1
+        K = dense_k(p)
+        r = gen.uniform(77, K.shape[0]) - 0.5
+        z = p.apply_preconditioner(r)
+        ref = np.linalg.solve(K, r)
+        assert np.abs(z - ref).max() <= 1e-8 * np.abs(ref).max(), g.n
+        alone = stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts)
+        assert alone.assemble(opts)
+        assert np.array_equal(alone.apply_preconditioner(r).view(np.uint64), z.view(np.uint64)), g.n
+        alone.close()
+        p.close()
