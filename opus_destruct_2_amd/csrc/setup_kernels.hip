@@ -316,28 +316,50 @@ __global__ void k_identity_rows(const AssemblyDev* __restrict__ J, const Tile* _
   }
 }
 
-// bonds leave K in list order (one workgroup a graph): each one's products subtracted in place
-__global__ void __launch_bounds__(64) k_remove_bonds(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ ids,
-                                                    const int32_t* __restrict__ offs, int njobs) {
-  if (int(blockIdx.x) >= njobs) return;
-  const AssemblyDev& a = J[blockIdx.x];
+// Bonds leaving K in list order (StressProblem::remove_bond, stress.cpp:260-284), spread: a workgroup
+// a removed bond forms its four products (aa, bb, ab, ba) into scratch; then a thread a (block,
+// element) of K adds the products that land on its block in the removal list's order - each element
+// of K gets the same additions in the same order as removing the bonds one after another, so the
+// bits are the sequential removal's.
+__global__ void __launch_bounds__(64) k_removal_products(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ ids,
+                                                         const int32_t* __restrict__ tjob, double* __restrict__ prod,
+                                                         int nids) {
+  const int t = blockIdx.x;
+  if (t >= nids) return;
+  const AssemblyDev& a = J[tjob[t]];
   __shared__ double sD[36], sA[36], sB[36];
   const int e = threadIdx.x;
-  for (int t = offs[blockIdx.x]; t < offs[blockIdx.x + 1]; ++t) {
-    const int bi = ids[t];
-    const int32_t* sl = a.bond_slot + 4 * size_t(bi);
-    if (e < 36) bond_blocks_lds(a.bonds[bi], a.nodec, a.mats, a.nmats, e, sD, sA, sB);
-    __syncthreads();
-    if (e < 36) {
-      const int R = e / 6, C = e % 6;
-      // (stress.cpp:269-274: aa, bb subtracted, ab, ba added back: C + (-1) acc, C + (+1) acc)
-      if (sl[0] >= 0) a.K[36 * size_t(sl[0]) + e] += -1.0 * atbd_elem(sA, sD, sA, R, C);
-      if (sl[1] >= 0) a.K[36 * size_t(sl[1]) + e] += -1.0 * atbd_elem(sB, sD, sB, R, C);
-      if (sl[2] >= 0) a.K[36 * size_t(sl[2]) + e] += 1.0 * atbd_elem(sA, sD, sB, R, C);
-      if (sl[3] >= 0) a.K[36 * size_t(sl[3]) + e] += 1.0 * atbd_elem(sB, sD, sA, R, C);
-    }
-    __syncthreads();
+  const int bi = ids[t];
+  if (e < 36) bond_blocks_lds(a.bonds[bi], a.nodec, a.mats, a.nmats, e, sD, sA, sB);
+  __syncthreads();
+  if (e < 36) {
+    const int R = e / 6, C = e % 6;
+    double* o = prod + 144 * size_t(t);
+    o[e] = atbd_elem(sA, sD, sA, R, C);
+    o[36 + e] = atbd_elem(sB, sD, sB, R, C);
+    o[72 + e] = atbd_elem(sA, sD, sB, R, C);
+    o[108 + e] = atbd_elem(sB, sD, sA, R, C);
   }
+}
+
+// group g: block slot[g] of job gjob[g], its products cont[gstart[g] .. gstart[g + 1]) (4 t + which,
+// in list order); aa and bb subtracted, ab and ba added back (stress.cpp:269-274)
+__global__ void __launch_bounds__(kWG) k_removal_apply(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ gjob,
+                                                       const int32_t* __restrict__ gslot,
+                                                       const int32_t* __restrict__ gstart,
+                                                       const int32_t* __restrict__ cont, const double* __restrict__ prod,
+                                                       int ngroups) {
+  const int64_t t = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (t >= 36 * int64_t(ngroups)) return;
+  const int g = int(t / 36), e = int(t % 36);
+  double* K = J[gjob[g]].K + 36 * size_t(gslot[g]) + e;
+  double v = *K;
+  for (int k = gstart[g]; k < gstart[g + 1]; ++k) {
+    const int c = cont[k];
+    const double p = prod[36 * size_t(c) + e];
+    v += ((c & 3) < 2 ? -1.0 : 1.0) * p;
+  }
+  *K = v;
 }
 
 __global__ void k_retire_rows(const AssemblyDev* __restrict__ J, const int32_t* __restrict__ rows, int count) {
@@ -1747,8 +1769,15 @@ void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count) {
 void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_identity_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
-void launch_remove_bonds(hipStream_t s, const AssemblyDev* jobs, const int32_t* ids, const int32_t* offs, int njobs) {
-  if (njobs > 0) hipLaunchKernelGGL(k_remove_bonds, dim3(njobs), dim3(64), 0, s, jobs, ids, offs, njobs);
+void launch_removal_products(hipStream_t s, const AssemblyDev* jobs, const int32_t* ids, const int32_t* tjob,
+                             double* prod, int nids) {
+  if (nids > 0) hipLaunchKernelGGL(k_removal_products, dim3(nids), dim3(64), 0, s, jobs, ids, tjob, prod, nids);
+}
+void launch_removal_apply(hipStream_t s, const AssemblyDev* jobs, const int32_t* gjob, const int32_t* gslot,
+                          const int32_t* gstart, const int32_t* cont, const double* prod, int ngroups) {
+  if (ngroups > 0)
+    hipLaunchKernelGGL(k_removal_apply, dim3(unsigned((36 * int64_t(ngroups) + kWG - 1) / kWG)), dim3(kWG), 0, s, jobs,
+                       gjob, gslot, gstart, cont, prod, ngroups);
 }
 void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* rows, int count) {
   if (count > 0) hipLaunchKernelGGL(k_retire_rows, dim3((36 * count + 255) / 256), dim3(256), 0, s, job, rows, count);

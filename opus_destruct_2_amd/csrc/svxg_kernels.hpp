@@ -112,7 +112,13 @@ void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count);
 void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles);
 // in-place removal of bonds from K (stress.cpp:260-284): the listed bonds' products, subtracted
 // (one workgroup a job: job j's bonds are ids[offs[j] .. offs[j + 1]))
-void launch_remove_bonds(hipStream_t s, const AssemblyDev* jobs, const int32_t* ids, const int32_t* offs, int njobs);
+// bonds leaving K in list order: each removed bond's four products (prod: 144 doubles a bond), then
+// each touched block's products added in list order (groups: a job's block slot and its products'
+// range in cont)
+void launch_removal_products(hipStream_t s, const AssemblyDev* jobs, const int32_t* ids, const int32_t* tjob,
+                             double* prod, int nids);
+void launch_removal_apply(hipStream_t s, const AssemblyDev* jobs, const int32_t* gjob, const int32_t* gslot,
+                          const int32_t* gstart, const int32_t* cont, const double* prod, int ngroups);
 // rows of retired nodes become id_scale I (stress.cpp:299-308)
 void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* rows, int count);
 

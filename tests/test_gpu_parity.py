@@ -197,3 +197,27 @@ def test_vcycle_fp16_copies(mats, ctx):
     assert abs(its[0] - its[1]) <= 3
     scale = np.abs(us[1]).max()
     assert np.abs(us[0] - us[1]).max() <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("case", [CASES[3], CASES[7]], ids=[IDS[3], IDS[7]])
+def test_matrix_after_removals_bit_identical(case, mats, ctx):
+    """K after bond removals in place (StressProblem::remove_bond, stress.cpp:260-284): many bonds,
+    neighbours among them (their diagonal blocks take several products, in the list's order), in
+    two calls and in one batch of two problems, bit for bit the oracle's removing them one by one."""
+    name, nodes, bonds, f, kw = case
+    rng = np.random.default_rng(11)
+    live = np.flatnonzero(bonds["broken"] == 0)
+    cut1 = rng.choice(live, size=len(live) // 8, replace=False)
+    cut2 = rng.choice(np.setdiff1d(live, cut1), size=len(live) // 10, replace=False)
+    o = CpuProblem(nodes, bonds, mats)
+    assert o.assemble(reference_options(**kw))
+    for b in list(cut1) + list(cut2):
+        o.remove_bond(int(b))
+    ro, co, vo = o.matrix()
+    d = stress.StressProblem(nodes, bonds, mats, ctx)
+    assert d.assemble(stress.default_options(**kw))
+    d.remove_bonds(cut1)
+    d.remove_bonds(cut2)
+    rd, cd, vd = d.matrix()
+    assert np.array_equal(rd, ro) and np.array_equal(cd, co)
+    assert np.array_equal(bits(vd), bits(vo))
