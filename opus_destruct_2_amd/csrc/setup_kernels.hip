@@ -1238,10 +1238,15 @@ __global__ void __launch_bounds__(kWG) k_power_init(const PowerDev* __restrict__
   p.v[0][e] = 1.0 + 0.001 * double(e % 7);
 }
 
-// v_{s+1} = D^-1 A (v_s / |v_s|) over a tile, its partial |v_{s+1}|^2 (S six-lane sub-groups a row:
-// S = 10 serves the long rows of coarse levels a wavefront each)
+// v_{s+1} = D^-1 A (v_s / |v_s|), partial |v_{s+1}|^2 (S six-lane sub-groups a row: S = 10 serves
+// the long rows of coarse levels a wavefront each)
 template <int S>
-__device__ __forceinline__ void power_step_tile(const PowerDev& p, const Tile& tl, int step, double* partial) {
+__global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                    int ntiles, int step, double* __restrict__ partial) {
+  if (int(blockIdx.x) >= ntiles) return;
+  const Tile tl = tiles[blockIdx.x];
+  const PowerDev& p = J[tl.g];
+  if (p.lam[1] <= 0.0) return;  // (stopped: a zero vector)
   const double inv = step == 0 ? 1.0 : 1.0 / p.lam[1];
   const double* in = p.v[step & 1];
   double* out = p.v[(step + 1) & 1];
@@ -1273,34 +1278,14 @@ __device__ __forceinline__ void power_step_tile(const PowerDev& p, const Tile& t
   if (threadIdx.x == 0) partial[2 * tl.slot] = s;
 }
 
-// One power iteration of every graph: its tiles (the first nshort with six lanes a row, the rest a
-// wavefront a row), and the graph's last tile to finish sums the graph's partials in slot order
-// (the same workgroup-wide sum whichever tile is last) and sets |v| - a step and its finish in one
-// launch. cnt: a counter a graph, zero between launches (the last tile resets it).
-__global__ void __launch_bounds__(kWG) k_power_iter(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
-                                                    int ntiles, int nshort, int step, double* __restrict__ partial,
-                                                    int32_t* __restrict__ cnt, const int32_t* __restrict__ gtile) {
-  if (int(blockIdx.x) >= ntiles) return;
-  const Tile tl = tiles[blockIdx.x];
-  const PowerDev& p = J[tl.g];
-  if (p.lam[1] <= 0.0) return;  // (stopped: a zero vector)
-  if (int(blockIdx.x) < nshort) power_step_tile<1>(p, tl, step, partial);
-  else power_step_tile<10>(p, tl, step, partial);
-  const int k0 = gtile[tl.g], k1 = gtile[tl.g + 1];
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(cnt + tl.g, 1) == k1 - k0 - 1 ? 1 : 0;
-    if (last) {
-      cnt[tl.g] = 0;
-      __threadfence();
-    }
-  }
-  __syncthreads();
-  if (!last) return;
+__global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict__ J, const int32_t* __restrict__ gtile,
+                                                      int count, const double* __restrict__ partial) {
+  if (int(blockIdx.x) >= count) return;
+  const PowerDev& p = J[blockIdx.x];
+  if (p.lam[1] <= 0.0) return;
   __shared__ double lds[4];
   double a = 0.0;
-  for (int k = k0 + int(threadIdx.x); k < k1; k += kWG) a += partial[2 * k];
+  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k];
   const double s = block_sum(a, lds);
   if (threadIdx.x == 0) {
     const double nv = sqrt(s);
@@ -1880,11 +1865,17 @@ void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, i
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_power_init, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, count);
 }
-void launch_power_iter(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int nshort, int step,
-                       double* partial, int32_t* cnt, const int32_t* gtile) {
-  if (ntiles > 0)
-    hipLaunchKernelGGL(k_power_iter, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, nshort, step, partial, cnt,
-                       gtile);
+void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial,
+                       bool long_rows) {
+  if (ntiles <= 0) return;
+  if (long_rows)
+    hipLaunchKernelGGL(k_power_step<10>, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
+  else
+    hipLaunchKernelGGL(k_power_step<1>, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
+}
+void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_power_finish, dim3(count), dim3(kWG), 0, s, jobs, gtile, count, partial);
 }
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count) {
   if (count > 0) hipLaunchKernelGGL(k_dense_factor, dim3(count), dim3(kDenseWG), 0, s, jobs, count);

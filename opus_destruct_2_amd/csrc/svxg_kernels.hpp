@@ -234,9 +234,9 @@ struct PowerDev {
   double* lam;   // 2 doubles: [0] the estimate, [1] the last norm
 };
 void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int count);
-// one power iteration of every graph, its |v| set by the graph's last tile (cnt: zeroed, a graph)
-void launch_power_iter(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int nshort, int step,
-                       double* partial, int32_t* cnt, const int32_t* gtile);
+void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial,
+                       bool long_rows);
+void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
 
 // the coarsest level's dense inverse (amg.cpp:535-578, then inverted)
 // the P, A P and P^T A P fills run 36 lanes an entry (one a block element, seven entries a
