@@ -590,6 +590,70 @@ int svxg_lattice_graph_clustered(svxg_ctx* ctx, const svxg_voxel_box* box, const
                                  int32_t* nfaces, svxg_section_head* heads, svxg_section_face* faces,
                                  svxg_bond* bonds);
 
+/* ---- a structure's extraction: the walk from a seed fragment ----------------------------------
+   World::Impl::extract's walk and append_nodes (world.cpp:425-620, 739-865) for a structure of the
+   world grid (grid 0, no junctions), on the device, in the reference's order: the members in the
+   order the walk takes them (breadth first from the seed; a member's voxels in its chunk's scan
+   order, x slowest; each voxel's faces -x, +x, -y, +y, -z, +z), its bonds in the order their first
+   faces are met and their faces in the order met (SectionLog, world_internal.hpp:533-620), nodes
+   clustered as append_nodes clusters them, a pair bond's ends in the order of their identities.
+   A fragment is a FragInfo of one chunk (fragments.cpp:283-345): chunk-local, 32^3 voxels a
+   chunk. */
+typedef struct svxg_fragment {
+  double com[3];             /* FragInfo::com (world, m) */
+  double mass;               /* FragInfo::mass */
+  double strength;           /* class_mult(frag_class(f)) (world.cpp:408-421; 1 undesigned) */
+  int64_t owner;             /* owner_of(f): the registered structure that holds it, 0 none */
+  int32_t first[3];          /* FragInfo::first as world voxel coordinates (its chunk is the fragment's) */
+  uint8_t mat;               /* FragInfo::mat */
+  uint8_t pad_[3];
+} svxg_fragment;             /* 64 bytes */
+
+typedef struct svxg_extract_options {
+  int32_t max_nodes;         /* WorldConfig::structure_max_nodes (60000) */
+  int32_t cluster_nodes;     /* WorldConfig::cluster_nodes (2500), cluster_cell's limit */
+  double max_radius;         /* WorldConfig::structure_max_radius (60 m) from the seed's centre */
+  int64_t fresh_from;        /* owners >= fresh_from are structures of this refresh: held fixed
+                                (World::Impl::fresh_from_; INT64_MAX: none) */
+  int32_t cluster_cubes;     /* WorldConfig::cluster_cubes (0) */
+  int32_t outside;           /* voxels outside the box: 0 air; 1 chunks not resident, which hold
+                                what touches them (the walk's `unloaded`) */
+} svxg_extract_options;      /* 32 bytes */
+
+typedef struct svxg_extract_result {
+  int32_t members;           /* fragments taken, in walk order */
+  int32_t nodes;             /* after clustering */
+  int32_t bonds;
+  int32_t faces;
+  int32_t cell;              /* cluster_cell(members): 0, 8 or 16 */
+  int32_t truncated;         /* a fragment met was held fixed (radius, node limit, a fresh owner) */
+  int32_t any_support;       /* 0: nothing holds it - extract makes no structure (a free piece) */
+  int32_t touched_unloaded;  /* it reached outside the box with outside = 1 */
+} svxg_extract_result;
+
+void svxg_extract_default_options(svxg_extract_options* out);
+
+/* The walk from fragment `seed` over box (Vox codes, damage), broken (per voxel, bit a: the face to
+   its +a neighbour is broken; or null) and frag (each voxel's fragment, -1 none; fragments only on
+   solid unanchored voxels, each inside the chunk of its `first`). Outputs, each with room for what
+   res holds on entry (members, nodes, bonds, faces):
+     members[k]      the fragment of member k (walk order)
+     member_node[k]  its node
+     nodes           centre and mass (append_nodes: masses summed in member order, centres weighted
+                     by them, then times 1 / mass)
+     node_ident      frag_ident of each node's first fragment (world_internal.hpp:24)
+     heads, faces    each bond's section head (ma, mb and strengths as append_nodes sets them) and
+                     faces, the svxg_bond_sections inputs
+     bonds           the bonds, sections computed (SecAcc::finish_lattice + section_strengths)
+     bond_ident      bond_ident of each bond (world_internal.hpp:25-27)
+   With every output null only the counts are made (res); SVXG_BAD_INPUT when a room is short (the
+   counts are set). Any output may be null. */
+int svxg_extract_structure(svxg_ctx* ctx, const svxg_voxel_box* box, const uint8_t* broken, const int32_t* frag,
+                           const svxg_fragment* frags, int32_t nfrag, int32_t seed, const svxg_extract_options* opt,
+                           const svxg_material* mats, int32_t nmats, svxg_extract_result* res, int32_t* members,
+                           int32_t* member_node, svxg_node* nodes, uint64_t* node_ident, svxg_section_head* heads,
+                           svxg_section_face* faces, svxg_bond* bonds, uint64_t* bond_ident);
+
 /* ---- serving: a per-tick request loop with continuous batching ------------------------------
    Replaces the reference's per-tick structure loop (step_structures, world.cpp:986-1129: the
    structures of a tick solved one after another within a block-operation budget) and its piece

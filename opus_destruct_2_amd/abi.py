@@ -238,6 +238,42 @@ class VoxelBox(ctypes.Structure):
                 ("vox", ctypes.c_void_p), ("damage", ctypes.c_void_p)]
 
 
+# ---- a structure's extraction (World::Impl::extract + append_nodes, world.cpp:425-865) ----------
+FRAGMENT = np.dtype([
+    ("com", "<f8", (3,)), ("mass", "<f8"), ("strength", "<f8"), ("owner", "<i8"), ("first", "<i4", (3,)),
+    ("mat", "u1"), ("pad_", "u1", (3,)),
+], align=True)
+assert FRAGMENT.itemsize == 64
+
+
+class ExtractOptions(ctypes.Structure):
+    """svxg_extract_options (WorldConfig's extraction settings, world.hpp:142-148, 234-265)."""
+    _fields_ = [("max_nodes", ctypes.c_int32), ("cluster_nodes", ctypes.c_int32), ("max_radius", ctypes.c_double),
+                ("fresh_from", ctypes.c_int64), ("cluster_cubes", ctypes.c_int32), ("outside", ctypes.c_int32)]
+
+
+class ExtractResult(ctypes.Structure):
+    """svxg_extract_result: in, the room of the outputs; out, the counts and the walk's flags."""
+    _fields_ = [("members", ctypes.c_int32), ("nodes", ctypes.c_int32), ("bonds", ctypes.c_int32),
+                ("faces", ctypes.c_int32), ("cell", ctypes.c_int32), ("truncated", ctypes.c_int32),
+                ("any_support", ctypes.c_int32), ("touched_unloaded", ctypes.c_int32)]
+
+
+assert ctypes.sizeof(ExtractOptions) == 32 and ctypes.sizeof(ExtractResult) == 32
+
+
+def extract_options(**kw):
+    """svxg_extract_options at the reference's defaults (max_nodes 60000, cluster_nodes 2500,
+    max_radius 60 m, no fresh owners, cluster_cubes off, air outside the box), then kw."""
+    o = ExtractOptions(max_nodes=60000, cluster_nodes=2500, max_radius=60.0, fresh_from=(1 << 63) - 1,
+                       cluster_cubes=0, outside=0)
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
 def ptr(a):
     """The data pointer of a numpy array (None for None)."""
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)

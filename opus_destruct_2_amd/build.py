@@ -22,7 +22,7 @@ ARCH = os.environ.get("SVXG_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 SOURCES = ["solve_kernels.hip", "setup_kernels.hip", "judge_kernels.hip", "section_kernels.hip", "lattice_kernels.hip",
-           "host.hip"]
+           "extract_kernels.hip", "host.hip"]
 
 
 def _run(cmd):

@@ -1395,6 +1395,7 @@ static int recheck_residual(Batch& B, hipStream_t s) {
 #include "host_serve.inc"
 #include "host_judge.inc"
 #include "host_lattice.inc"
+#include "host_extract.inc"
 
 namespace svxg {
 // (a kept batch-of-one holds device pointers: rebuilt when K's arrays change)

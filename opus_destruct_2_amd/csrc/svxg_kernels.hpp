@@ -436,6 +436,89 @@ void launch_heads(hipStream_t s, const uint64_t* key, const int32_t* first, cons
 void launch_voxel_keys(hipStream_t s, const LatticeDev* L, uint32_t* key, int32_t* idx, int32_t* count, int64_t nvox);
 void launch_node_sums(hipStream_t s, const LatticeDev* L, const int32_t* vidx, const int32_t* start, int64_t skip,
                       svxg_node* nodes, uint8_t* nmat, int nfrag);
+// ---- a structure's extraction: the walk from a seed (world.cpp:425-620, 739-865) -------------------
+struct WalkState {
+  int32_t count;         // members so far
+  int32_t level_begin;   // the level being walked: members [level_begin, level_end)
+  int32_t level_end;
+  int32_t ncand;         // fragments the level met for the first time
+  int32_t truncated, any_support, touched_unloaded, bad;
+};
+struct WalkDev {
+  const uint8_t* vox;
+  const uint8_t* broken;  // or null
+  const int32_t* frag;
+  const svxg_fragment* frags;
+  int32_t* bbox;          // 6 a fragment: its voxels' least and greatest box-local x, y, z
+  int32_t* slot;          // a fragment's member index; -1 not met; -2 held fixed
+  uint64_t* meet;         // the least (member, voxel, face) key that met a fragment
+  int32_t* candflag;
+  int32_t* cand;
+  int32_t* members;
+  WalkState* st;
+  int32_t lo[3], dim[3];
+  int64_t nvox;
+  int32_t nfrag, seed, max_nodes, outside;
+  double max_radius;
+  int64_t fresh_from;
+};
+// the merge's arrays (k_walk_heads, k_walk_faces)
+struct WalkMerge {
+  const uint64_t* mkey;      // node-level keys, sorted (over the fine bonds in the order made)
+  const int32_t* mstart;     // a merged bond's first entry in that list (nmerged + 1)
+  const int32_t* mpos;       // its first fine bond's place in the order made
+  const int32_t* mseg;       // merged bonds started before an entry (exclusive scan of mfirst)
+  const int32_t* mfirst;     // an entry starts a merged bond
+  const int32_t* kinv;       // a fine bond's (by order) entry in the list
+  const int32_t* final_of;   // a merged bond's place among the bonds
+  const int32_t* face0;      // a bond's first face (bonds + 1)
+  const int32_t* S;          // faces before an entry of the list (exclusive scan)
+  const int32_t* byfo;       // the fine bond made o-th (key order index)
+  const int32_t* fo;         // a fine bond's place in the order made
+  const uint64_t* fkey;      // a fine bond's key
+  const int32_t* fstart;     // its first record in the sorted records (nfine + 1)
+  const int32_t* fpos;       // its first record's place in the walk
+  const int32_t* seg;        // fine bonds started before a sorted record (exclusive scan of ffirst)
+  const int32_t* ffirst;     // a sorted record starts a fine bond
+  const int32_t* sidx;       // a sorted record's place in the walk
+  const svxg_section_face* rface;
+  const uint8_t* rmb;
+  const int32_t* fnode;      // a member's node
+  const uint8_t* nmat;
+  const double* nstr;
+  const uint64_t* ident;
+  uint8_t* swapped;          // a merged bond's ends swapped for their identities
+};
+void launch_walk_init(hipStream_t s, const WalkDev* W, int nfrag, int64_t nvox);
+void launch_walk_meet(hipStream_t s, const WalkDev* W, int level_upper);
+void launch_walk_keys(hipStream_t s, const WalkDev* W, int n, uint64_t* key, int32_t* val);
+void launch_walk_eligible(hipStream_t s, const WalkDev* W, const int32_t* val, int n, int32_t* e);
+void launch_walk_assign(hipStream_t s, const WalkDev* W, const int32_t* val, const int32_t* e, const int32_t* pre,
+                        int n);
+void launch_walk_count(hipStream_t s, const WalkDev* W, int nm, int32_t* count);
+void launch_walk_emit(hipStream_t s, const WalkDev* W, int nm, const int32_t* off, uint64_t* rkey,
+                      svxg_section_face* rface, uint8_t* rmb);
+void launch_seg_first(hipStream_t s, const uint64_t* key, int n, int32_t* first);
+void launch_fine(hipStream_t s, const uint64_t* skey, const int32_t* sidx, const int32_t* first, const int32_t* seg,
+                 int n, int32_t* fstart, int32_t* fpos, uint64_t* fkey, int32_t* flag);
+void launch_fine_order(hipStream_t s, const int32_t* fpos, const int32_t* rank, int nfine, int32_t* fo, int32_t* byfo);
+void launch_walk_cells(hipStream_t s, const WalkDev* W, int cell, int cubes, uint64_t* cellkey, int32_t* label, int nm);
+void launch_walk_link_round(hipStream_t s, const uint64_t* fkey, int nfine, const uint64_t* cellkey, int32_t* label,
+                            int32_t* changed, int nm);
+void launch_walk_iota(hipStream_t s, int32_t* a, int32_t* b, int n);
+void launch_walk_nodes(hipStream_t s, const WalkDev* W, const int32_t* mlist, const int32_t* start, int nn,
+                       svxg_node* nodes, uint8_t* nmat, double* nstr, uint64_t* ident);
+void launch_merge_keys(hipStream_t s, const uint64_t* fkey, const int32_t* byfo, const int32_t* fnode, int nfine,
+                       uint64_t* mkey, int32_t* mval);
+void launch_merge_counts(hipStream_t s, const uint64_t* mkey, const int32_t* mval, const int32_t* byfo,
+                         const int32_t* fstart, int n, int32_t* cnt, int32_t* kinv);
+void launch_merged(hipStream_t s, const uint64_t* mkey, const int32_t* mval, const int32_t* first, const int32_t* seg,
+                   int n, int32_t* mstart, int32_t* mpos, int32_t* flag);
+void launch_merged_order(hipStream_t s, const int32_t* mpos, const int32_t* rank, const int32_t* mstart,
+                         const int32_t* S, int nmerged, int32_t* final_of, int32_t* fcount);
+void launch_walk_heads(hipStream_t s, const WalkMerge& M, int nmerged, svxg_section_head* heads, uint64_t* bid);
+void launch_walk_faces(hipStream_t s, const WalkMerge& M, int nrec, svxg_section_face* out);
+
 hipError_t scan_exclusive_i32(void* tmp, size_t& bytes, const int32_t* in, int32_t* out, int64_t n, hipStream_t s);
 hipError_t sort_pairs_u64(void* tmp, size_t& bytes, const uint64_t* kin, uint64_t* kout, const int32_t* vin,
                           int32_t* vout, int64_t n, int bits, hipStream_t s);

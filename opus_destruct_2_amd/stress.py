@@ -95,6 +95,8 @@ def lib():
                                    ctypes.c_int),
             "svxg_lattice_graph_clustered": ([vp, vp, vp, i32, i32, i32, vp, i32, vp, ctypes.POINTER(i32), vp,
                                               ctypes.POINTER(i32), ctypes.POINTER(i32), vp, vp, vp], ctypes.c_int),
+            "svxg_extract_default_options": ([vp], None),
+            "svxg_extract_structure": ([vp, vp, vp, vp, vp, i32, i32, vp, vp, i32, vp] + [vp] * 8, ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

@@ -19,7 +19,7 @@ import ctypes
 
 import numpy as np
 
-from .abi import (BOND, JUNCTION_SAMPLE, NODE, SECTION_FACE, SECTION_HEAD, SECTION_HEAD_GENERAL, Lattice,
+from .abi import (BOND, FRAGMENT, JUNCTION_SAMPLE, NODE, SECTION_FACE, SECTION_HEAD, SECTION_HEAD_GENERAL, Lattice,
                   VoxelBox)
 
 ANCHOR = 0x80
@@ -549,6 +549,97 @@ def lattice_graph_clustered(vox, frag, lo, h, mats, cell, cubes=False, damage=No
                                                  ctypes.byref(nf), heads.ctypes.data, faces.ctypes.data,
                                                  bonds.ctypes.data), "svxg_lattice_graph_clustered")
     return nodes[:nn.value], heads[:nb.value], faces[:nf.value], bonds[:nb.value], fnode
+
+
+def chunk_fragments(frag, lo):
+    """The fragments cut at chunk boundaries (the reference fragments each chunk on its own,
+    fragments.cpp:283-345: a fragment lies in one chunk), numbered in order of (fragment, chunk)."""
+    frag = np.ascontiguousarray(frag, np.int32)
+    nz, ny, nx = frag.shape
+    z, y, x = np.nonzero(frag >= 0)
+    cx = np.floor_divide(x + int(lo[0]), CHUNK)
+    cy = np.floor_divide(y + int(lo[1]), CHUNK)
+    cz = np.floor_divide(z + int(lo[2]), CHUNK)
+    key = np.stack([frag[z, y, x].astype(np.int64), cx, cy, cz], -1)
+    _, inv = np.unique(key, axis=0, return_inverse=True)
+    out = np.full(frag.shape, -1, np.int32)
+    out[z, y, x] = inv.reshape(-1).astype(np.int32)
+    return out
+
+
+def fragments(vox, frag, lo, h, mats):
+    """FRAGMENT records (FragInfo as the walk reads it, fragments.cpp:283-345) of a box's fragments:
+    mass = the voxels' densities times h^3, centre of mass (voxel centres at h times their integer
+    coordinates), first = the fragment's least voxel in its chunk's scan order (x slowest) and its
+    material; strength 1 (undesigned), owner 0 (none)."""
+    vox = np.ascontiguousarray(vox, np.uint8)
+    frag = np.ascontiguousarray(frag, np.int32)
+    nf = int(frag.max()) + 1 if frag.size else 0
+    rho = np.asarray(mats["rho"], np.float64)
+    z, y, x = np.nonzero(frag >= 0)
+    f = frag[z, y, x]
+    mid = (vox[z, y, x] & 0x7F).astype(np.int64) - 1
+    m = rho[mid] * (h * h * h)
+    g = np.stack([x + int(lo[0]), y + int(lo[1]), z + int(lo[2])], -1).astype(np.int64)
+    out = np.zeros(nf, FRAGMENT)
+    out["mass"] = np.bincount(f, weights=m, minlength=nf)
+    for k in range(3):
+        out["com"][:, k] = np.bincount(f, weights=m * g[:, k] * h, minlength=nf) / out["mass"]
+    order = np.lexsort((g[:, 2], g[:, 1], g[:, 0], f))
+    head = order[np.r_[True, f[order][1:] != f[order][:-1]]]
+    out["first"][f[head]] = g[head]
+    out["mat"][f[head]] = mid[head]
+    out["strength"] = 1.0
+    return out
+
+
+def _extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts):
+    """The two calls of svxg_extract_structure (or its restatement): the counts, then the records.
+    call(box, broken, frag, frags, nfrag, seed, opts, mats, nmats, res, *outputs) -> status."""
+    from .abi import ExtractResult, extract_options
+    vox = np.ascontiguousarray(vox, np.uint8)
+    frag = np.ascontiguousarray(frag, np.int32)
+    frags = np.ascontiguousarray(frags, FRAGMENT)
+    broken = None if broken is None else np.ascontiguousarray(broken, np.uint8)
+    damage = None if damage is None else np.ascontiguousarray(damage, np.uint8)
+    mats = np.ascontiguousarray(mats)
+    opts = opts if opts is not None else extract_options()
+    box = voxel_box(vox, lo, h, damage)
+    bp = None if broken is None else broken.ctypes.data
+    res = ExtractResult()
+    args = (ctypes.byref(box), bp, frag.ctypes.data, frags.ctypes.data, len(frags), int(seed), ctypes.byref(opts),
+            mats.ctypes.data, len(mats), ctypes.byref(res))
+    call(*args, *([None] * 8))
+    counts = {k: getattr(res, k) for k in ("members", "nodes", "bonds", "faces")}
+    out = {"members": np.zeros(max(1, counts["members"]), np.int32),
+           "member_node": np.zeros(max(1, counts["members"]), np.int32),
+           "nodes": np.zeros(max(1, counts["nodes"]), NODE), "node_ident": np.zeros(max(1, counts["nodes"]), np.uint64),
+           "heads": np.zeros(max(1, counts["bonds"]), SECTION_HEAD), "faces": np.zeros(max(1, counts["faces"]), SECTION_FACE),
+           "bonds": np.zeros(max(1, counts["bonds"]), BOND), "bond_ident": np.zeros(max(1, counts["bonds"]), np.uint64)}
+    call(*args, *(out[k].ctypes.data for k in ("members", "member_node", "nodes", "node_ident", "heads", "faces",
+                                                "bonds", "bond_ident")))
+    n = {"members": counts["members"], "member_node": counts["members"], "nodes": counts["nodes"],
+         "node_ident": counts["nodes"], "heads": counts["bonds"], "faces": counts["faces"], "bonds": counts["bonds"],
+         "bond_ident": counts["bonds"]}
+    out = {k: v[:n[k]] for k, v in out.items()}
+    for k in ("cell", "truncated", "any_support", "touched_unloaded"):
+        out[k] = int(getattr(res, k))
+    return out
+
+
+def extract_structure(vox, frag, lo, h, frags, seed, mats, broken=None, damage=None, opts=None, ctx=None):
+    """svxg_extract_structure: the structure the walk from fragment `seed` takes (World::Impl::extract
+    and append_nodes, world.cpp:425-865, for the world grid), on the device. vox, frag, broken,
+    damage: (z, y, x) arrays over the box from lo; frags: FRAGMENT records (fragments()); opts:
+    abi.extract_options(...). Returns a dict: members, member_node, nodes, node_ident, heads, faces,
+    bonds, bond_ident, and the walk's cell, truncated, any_support and touched_unloaded."""
+    from . import stress
+    ctx = ctx or stress.default_context()
+    L = stress.lib()
+
+    def call(*a):
+        stress._check(L.svxg_extract_structure(ctx.h, *a), "svxg_extract_structure")
+    return _extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts)
 
 
 def gravity(nodes):

@@ -57,7 +57,9 @@ def _bind(path, pre):
                     "graph_components": ([i32, vp, i32, vp, vp], i32),
                     "bond_sections": ([vp, i32, vp, i32, vp, i32, vp, vp, i32, vp], None),
                     "bond_sections_general": ([vp, i32, vp, i32, vp, i32, vp, i32, vp, i32, i32, vp, i32, vp],
-                                              None)})
+                                              None),
+                    "extract_structure": ([vp, vp, vp, vp, i32, i32, vp, vp, i32, vp] + [vp] * 8, i32),
+                    "mix64_value": ([ctypes.c_uint64], ctypes.c_uint64)})
     for name, (args, res) in sig.items():
         fn = getattr(L, pre + name)
         fn.argtypes = args
@@ -323,6 +325,26 @@ def bond_sections_general(nodes, heads, faces, samples, grids, S, mats):
                                 len(faces), samples.ctypes.data, len(samples), lat, len(grids), int(S),
                                 mats.ctypes.data, len(mats), out.ctypes.data)
     return out
+
+
+def extract_structure(vox, frag, lo, h, frags, seed, mats, broken=None, damage=None, opts=None):
+    """orc_extract_structure: World::Impl::extract's walk and append_nodes (world.cpp:425-620,
+    739-865) for the world grid, taken sequentially on the CPU; the dict of
+    opus_destruct_2_amd.voxels.extract_structure."""
+    from opus_destruct_2_amd import voxels
+    L, _ = library("oracle")
+
+    def call(*a):
+        st = L.orc_extract_structure(*a)
+        if st != 0 and a[10] is not None:
+            raise RuntimeError(f"orc_extract_structure: status {st}")
+    return voxels._extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts)
+
+
+def mix64(x):
+    """The SplitMix64 output function of the identities (world_internal.hpp:17-23)."""
+    L, _ = library("oracle")
+    return int(L.orc_mix64_value(ctypes.c_uint64(int(x) & ((1 << 64) - 1))))
 
 
 def cascade(problem, judge, f, u, opts, state, rtol=1e-10, maxit=20000, max_call_rounds=0, assemble_opt=None):

@@ -1842,3 +1842,391 @@ void orc_default_options(svxg_stress_options* o) {
 }
 
 }  // extern "C"
+
+// ---- a structure's extraction: the walk from a seed ---------------------------------------------
+// World::Impl::extract's walk (world.cpp:425-620) and append_nodes (world.cpp:739-865) for the world
+// grid alone, over a box of voxels: the sequential walk as the reference takes it - a queue of
+// members, each member's voxels in its chunk's scan order, a voxel's six faces in turn - with its
+// SectionLog (world_internal.hpp:533-620) and cluster_items (:466-491).
+namespace {
+
+constexpr i32 kChunkEdge = 32;  // core/include/svx/world/grid.hpp:41
+
+// SplitMix64's output function (Steele, Lea and Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014), as world_internal.hpp:17-23 uses it
+uint64_t orc_mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+uint64_t orc_key3(i64 x, i64 y, i64 z) {  // grid.hpp:56-59
+  const i64 off = i64(1) << 20;
+  return (uint64_t(x + off) << 42) | (uint64_t(y + off) << 21) | uint64_t(z + off);
+}
+i32 orc_fdiv(i32 a, i32 b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+uint64_t orc_acc_key(i32 a, i32 b, int axis, int sign) {  // world_internal.hpp:447-450
+  if (b >= 0) return (uint64_t(std::min(a, b)) << 32) | (uint64_t(std::max(a, b)) << 1) | 1u;
+  return (uint64_t(a) << 32) | (uint64_t(axis * 2 + (sign > 0 ? 1 : 0)) << 1);
+}
+
+struct WalkHead {
+  i32 a = -1, b = -1;
+  int axis = 0;
+  int sign = 1;
+  uint8_t mb = 0;
+};
+struct WalkFace {
+  i32 p[3];
+  i32 s;
+  int axis;
+  int sign;
+};
+
+// SectionLog: fine bonds in the order made, faces in the order met
+struct WalkLog {
+  std::vector<WalkHead> heads;
+  std::vector<WalkFace> faces;
+  std::vector<std::pair<uint64_t, i32>> index;  // (key, head), looked up linearly by key
+  std::vector<i32> table;                        // open-addressed over index
+  i32 at(i32 a, i32 b, int axis, int sign) {
+    const uint64_t k = orc_acc_key(a, b, axis, sign);
+    if (table.empty()) table.assign(1024, -1);
+    if (index.size() * 2 >= table.size()) {
+      std::vector<i32> t(table.size() * 2, -1);
+      for (size_t e = 0; e < index.size(); ++e) {
+        size_t i = size_t(orc_mix64(index[e].first)) & (t.size() - 1);
+        while (t[i] >= 0) i = (i + 1) & (t.size() - 1);
+        t[i] = i32(e);
+      }
+      table.swap(t);
+    }
+    size_t i = size_t(orc_mix64(k)) & (table.size() - 1);
+    while (table[i] >= 0) {
+      if (index[size_t(table[i])].first == k) return index[size_t(table[i])].second;
+      i = (i + 1) & (table.size() - 1);
+    }
+    const i32 s = i32(heads.size());
+    table[i] = i32(index.size());
+    index.push_back({k, s});
+    WalkHead H;
+    H.a = b >= 0 ? std::min(a, b) : a;
+    H.b = b >= 0 ? std::max(a, b) : b;
+    H.axis = axis;
+    H.sign = sign;
+    heads.push_back(H);
+    return s;
+  }
+  void face(i32 s, const i32 lower[3], int axis, int sign_from_a) {
+    WalkFace F;
+    for (int c = 0; c < 3; ++c) F.p[c] = lower[c];
+    F.s = s;
+    F.axis = axis;
+    F.sign = sign_from_a;
+    faces.push_back(F);
+  }
+};
+
+}  // namespace
+
+extern "C" int32_t orc_extract_structure(const svxg_voxel_box* box, const uint8_t* broken, const int32_t* frag,
+                                         const svxg_fragment* frags, int32_t nfrag, int32_t seed,
+                                         const svxg_extract_options* opt, const svxg_material* mats, int32_t nmats,
+                                         svxg_extract_result* res, int32_t* members_out, int32_t* member_node,
+                                         svxg_node* nodes_out, uint64_t* node_ident, svxg_section_head* heads_out,
+                                         svxg_section_face* faces_out, svxg_bond* bonds_out, uint64_t* bond_ident) {
+  const i32 nx = box->dim[0], ny = box->dim[1], nz = box->dim[2];
+  const i64 nvox = i64(nx) * ny * nz;
+  auto vidx = [&](const i32 g[3], i64* v) {
+    const i32 x = g[0] - box->lo[0], y = g[1] - box->lo[1], z = g[2] - box->lo[2];
+    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return false;
+    *v = x + i64(nx) * (y + i64(ny) * z);
+    return true;
+  };
+  // each fragment's voxels in its chunk's scan order (x slowest: local index x * 1024 + y * 32 + z)
+  std::vector<i32> vstart(size_t(nfrag) + 1, 0);
+  for (i64 v = 0; v < nvox; ++v)
+    if (frag[v] >= 0) ++vstart[size_t(frag[v]) + 1];
+  for (i32 f = 0; f < nfrag; ++f) vstart[size_t(f) + 1] += vstart[size_t(f)];
+  std::vector<i64> vlist(size_t(vstart[size_t(nfrag)]));
+  {
+    std::vector<i32> fill(vstart.begin(), vstart.end() - 1);
+    for (i32 x = 0; x < nx; ++x)
+      for (i32 y = 0; y < ny; ++y)
+        for (i32 z = 0; z < nz; ++z) {
+          const i64 v = x + i64(nx) * (y + i64(ny) * z);
+          if (frag[v] >= 0) vlist[size_t(fill[size_t(frag[v])]++)] = v;
+        }
+  }
+  auto mat_of = [](uint8_t v) { return uint8_t((v & 0x7F) - 1); };
+  // the walk (world.cpp:475-620)
+  std::vector<i32> slot(static_cast<size_t>(nfrag), -1), members;
+  const V3 seed_pos = v3(frags[seed].com);
+  slot[size_t(seed)] = 0;
+  members.push_back(seed);
+  bool truncated = false, any_support = false, touched_unloaded = false;
+  auto meet = [&](i32 G) -> i32 {
+    i32& s = slot[size_t(G)];
+    if (s == -1) {
+      const V3 d = vsub(v3(frags[G].com), seed_pos);
+      const i64 other = frags[G].owner;
+      const bool owned = other != 0;
+      const bool fresh = owned && other >= opt->fresh_from;
+      const bool room = i32(members.size()) < opt->max_nodes;
+      if (!fresh && room && (owned || std::sqrt(vdot(d, d)) <= opt->max_radius)) {
+        s = i32(members.size());
+        members.push_back(G);
+      } else {
+        s = -2;
+        truncated = true;
+      }
+    }
+    return s;
+  };
+  WalkLog fine;
+  for (size_t qi = 0; qi < members.size(); ++qi) {
+    const i32 F = members[qi];
+    const i32 nF = i32(qi);
+    for (i32 k = vstart[size_t(F)]; k < vstart[size_t(F) + 1]; ++k) {
+      const i64 v = vlist[size_t(k)];
+      const i32 p[3] = {i32(v % nx) + box->lo[0], i32((v / nx) % ny) + box->lo[1], i32(v / (i64(nx) * ny)) + box->lo[2]};
+      const uint8_t brk_p = broken ? broken[v] : 0;
+      for (int a = 0; a < 3; ++a)
+        for (int sg = -1; sg <= 1; sg += 2) {
+          i32 q[3] = {p[0], p[1], p[2]};
+          q[a] += sg;
+          i64 w = -1;
+          const bool in = vidx(q, &w);
+          if (in && frag[w] == F) continue;
+          const uint8_t vq = in ? box->vox[w] : 0;
+          const bool unloaded = !in && opt->outside == 1;
+          if (!unloaded) {
+            if (vq == 0) continue;
+            const bool face_broken = sg > 0 ? ((brk_p >> a) & 1) != 0 : broken && ((broken[w] >> a) & 1) != 0;
+            if (face_broken) continue;
+          }
+          const i32* lower = sg > 0 ? p : q;
+          if (unloaded) touched_unloaded = true;
+          if ((vq & 0x80) || unloaded) {
+            const i32 A = fine.at(nF, -1, a, sg);
+            fine.heads[size_t(A)].mb = unloaded ? mat_of(box->vox[v]) : mat_of(vq);
+            fine.face(A, lower, a, 1);
+            any_support = true;
+            continue;
+          }
+          const i32 g2 = frag[w];
+          if (g2 < 0) continue;
+          const i32 s = meet(g2);
+          if (s >= 0) {
+            if (sg > 0) fine.face(fine.at(nF, s, a, 1), lower, a, nF < s ? 1 : -1);
+          } else {
+            const i32 A = fine.at(nF, -1, a, sg);
+            fine.heads[size_t(A)].mb = mat_of(vq);
+            fine.face(A, lower, a, 1);
+            any_support = true;
+          }
+        }
+    }
+  }
+  // append_nodes: clusters (cluster_cell, world.cpp:733-737; cluster_items)
+  const i32 m = i32(members.size());
+  const i32 cell = m <= opt->cluster_nodes ? 0 : (i64(m) <= 6 * i64(opt->cluster_nodes) ? 8 : 16);
+  std::vector<uint64_t> key(static_cast<size_t>(m));
+  for (i32 f = 0; f < m; ++f) {
+    if (cell <= 0) {
+      key[size_t(f)] = uint64_t(f);
+      continue;
+    }
+    const svxg_fragment& fi = frags[members[size_t(f)]];
+    i32 cc[3], l[3];
+    for (int c = 0; c < 3; ++c) {
+      cc[c] = orc_fdiv(fi.first[c], kChunkEdge);
+      l[c] = fi.first[c] - cc[c] * kChunkEdge;
+    }
+    const uint64_t c = opt->cluster_cubes
+                           ? uint64_t(((l[0] / cell) << 4) | ((l[1] / cell) << 2) | (l[2] / cell))
+                           : uint64_t((l[0] / cell) * 64 + (l[1] / cell) * 8 + l[2] / cell);
+    key[size_t(f)] = orc_mix64(orc_key3(cc[0], cc[1], cc[2]) ^ (c << 58));
+  }
+  std::vector<i32> par(static_cast<size_t>(m));
+  for (i32 i = 0; i < m; ++i) par[size_t(i)] = i;
+  auto find = [&](i32 x) {
+    while (par[size_t(x)] != x) {
+      par[size_t(x)] = par[size_t(par[size_t(x)])];
+      x = par[size_t(x)];
+    }
+    return x;
+  };
+  for (const WalkHead& H : fine.heads) {
+    if (H.a < 0 || H.a >= m || H.b < 0 || H.b >= m) continue;
+    if (key[size_t(H.a)] != key[size_t(H.b)]) continue;
+    const i32 ra = find(H.a), rb = find(H.b);
+    if (ra != rb) par[size_t(std::max(ra, rb))] = std::min(ra, rb);
+  }
+  std::vector<i32> fnode(static_cast<size_t>(m), -1), id(size_t(m), -1);
+  i32 K = 0;
+  for (i32 i = 0; i < m; ++i) {
+    const i32 r = find(i);
+    if (id[size_t(r)] < 0) id[size_t(r)] = K++;
+    fnode[size_t(i)] = id[size_t(r)];
+  }
+  std::vector<std::vector<i32>> cm(static_cast<size_t>(K));
+  for (i32 f = 0; f < m; ++f) cm[size_t(fnode[size_t(f)])].push_back(f);
+  std::vector<svxg_node> nodes(static_cast<size_t>(K));
+  std::vector<uint8_t> nmat(static_cast<size_t>(K));
+  std::vector<f64> nstrength(static_cast<size_t>(K));
+  std::vector<uint64_t> ident(static_cast<size_t>(K));
+  for (i32 c = 0; c < K; ++c) {
+    f64 mass = 0.0, heavy = -1.0, strength = 1e30;
+    V3 com;
+    uint8_t mat = 0;
+    for (i32 f : cm[size_t(c)]) {
+      const svxg_fragment& fi = frags[members[size_t(f)]];
+      mass += fi.mass;
+      com.x += fi.com[0] * fi.mass;
+      com.y += fi.com[1] * fi.mass;
+      com.z += fi.com[2] * fi.mass;
+      if (fi.mass > heavy) {
+        heavy = fi.mass;
+        mat = fi.mat;
+      }
+      strength = std::min(strength, fi.strength);
+    }
+    svxg_node nd{};
+    if (mass > 0) {
+      const f64 inv = 1.0 / mass;
+      nd.c[0] = com.x * inv;
+      nd.c[1] = com.y * inv;
+      nd.c[2] = com.z * inv;
+    } else {
+      nd.c[0] = com.x;
+      nd.c[1] = com.y;
+      nd.c[2] = com.z;
+    }
+    nd.mass = mass;
+    nodes[size_t(c)] = nd;
+    nmat[size_t(c)] = mat;
+    nstrength[size_t(c)] = strength;
+    const svxg_fragment& f0 = frags[members[size_t(cm[size_t(c)].front())]];
+    i32 cc[3], l[3];
+    for (int q = 0; q < 3; ++q) {
+      cc[q] = orc_fdiv(f0.first[q], kChunkEdge);
+      l[q] = f0.first[q] - cc[q] * kChunkEdge;
+    }
+    const i32 first = l[0] * kChunkEdge * kChunkEdge + l[1] * kChunkEdge + l[2];
+    ident[size_t(c)] = orc_mix64(orc_key3(cc[0], cc[1], cc[2]) * 0x9E3779B97F4A7C15ull ^ uint64_t(first));
+  }
+  // SectionLog::merge: node-level bonds in the order their first fine bond was made
+  struct Merged {
+    WalkHead H;
+    std::vector<WalkFace> faces;
+  };
+  std::vector<Merged> out;
+  std::vector<std::pair<uint64_t, i32>> mindex;
+  std::vector<i32> to(fine.heads.size(), -1);
+  std::vector<uint8_t> flip(fine.heads.size(), 0);
+  {
+    std::vector<std::pair<uint64_t, i32>> sorted;  // (key, merged) for lookups
+    for (size_t f = 0; f < fine.heads.size(); ++f) {
+      const WalkHead& F = fine.heads[f];
+      const i32 a = fnode[size_t(F.a)];
+      const i32 b = F.b >= 0 ? fnode[size_t(F.b)] : -1;
+      if (b >= 0 && a == b) continue;
+      const uint64_t k = orc_acc_key(a, b, F.axis, F.sign);
+      auto it = std::lower_bound(sorted.begin(), sorted.end(), std::make_pair(k, i32(-1)));
+      i32 mi;
+      if (it != sorted.end() && it->first == k) {
+        mi = it->second;
+      } else {
+        mi = i32(out.size());
+        sorted.insert(it, {k, mi});
+        Merged M;
+        M.H = F;
+        M.H.a = b >= 0 ? std::min(a, b) : a;
+        M.H.b = b >= 0 ? std::max(a, b) : b;
+        out.push_back(std::move(M));
+      }
+      to[f] = mi;
+      flip[f] = b >= 0 && a != out[size_t(mi)].H.a;
+    }
+  }
+  {
+    std::vector<std::vector<i32>> per(fine.heads.size());
+    for (size_t e = 0; e < fine.faces.size(); ++e) per[size_t(fine.faces[e].s)].push_back(i32(e));
+    for (size_t f = 0; f < fine.heads.size(); ++f) {
+      if (to[f] < 0) continue;
+      Merged& M = out[size_t(to[f])];
+      for (i32 e : per[f]) {
+        WalkFace F = fine.faces[size_t(e)];
+        F.sign = (flip[f] ? -1 : 1) * F.sign;
+        M.faces.push_back(F);
+      }
+    }
+  }
+  // the bonds (world.cpp:824-852): ends in the order of their identities, materials, strengths
+  const i32 nb = i32(out.size());
+  std::vector<svxg_section_head> heads(static_cast<size_t>(nb));
+  std::vector<svxg_section_face> faces;
+  std::vector<uint64_t> bid(static_cast<size_t>(nb));
+  for (i32 k = 0; k < nb; ++k) {
+    Merged& M = out[size_t(k)];
+    if (M.H.b >= 0 && ident[size_t(M.H.a)] > ident[size_t(M.H.b)]) {
+      std::swap(M.H.a, M.H.b);
+      for (WalkFace& F : M.faces) F.sign = -F.sign;
+    }
+    svxg_section_head& h = heads[size_t(k)];
+    std::memset(&h, 0, sizeof h);
+    h.a = M.H.a;
+    h.b = M.H.b;
+    h.axis = M.H.axis;
+    h.sign = M.H.sign;
+    h.face0 = i32(faces.size());
+    h.nfaces = i32(M.faces.size());
+    h.ma = nmat[size_t(M.H.a)];
+    h.strength_a = nstrength[size_t(M.H.a)];
+    if (M.H.b >= 0) {
+      h.mb = nmat[size_t(M.H.b)];
+      h.strength_b = nstrength[size_t(M.H.b)];
+    } else {
+      h.mb = M.H.mb;
+      h.strength_b = 1e9;
+    }
+    for (const WalkFace& F : M.faces) {
+      svxg_section_face o;
+      std::memset(&o, 0, sizeof o);
+      for (int c = 0; c < 3; ++c) o.p[c] = F.p[c];
+      o.axis = int8_t(F.axis);
+      o.sign = int8_t(F.sign);
+      faces.push_back(o);
+    }
+    const uint64_t ia = ident[size_t(M.H.a)], ib = M.H.b >= 0 ? ident[size_t(M.H.b)] : 0;
+    const int ax = M.H.b >= 0 ? 0 : M.H.axis, sn = M.H.b >= 0 ? 1 : M.H.sign;
+    bid[size_t(k)] = orc_mix64(ia * 0xD6E8FEB86659FD93ull ^ orc_mix64(ib + uint64_t(ax * 2 + (sn > 0 ? 1 : 0))));
+  }
+  const i32 nf = i32(faces.size());
+  svxg_extract_result r{};
+  r.members = m;
+  r.nodes = K;
+  r.bonds = nb;
+  r.faces = nf;
+  r.cell = cell;
+  r.truncated = truncated ? 1 : 0;
+  r.any_support = any_support ? 1 : 0;
+  r.touched_unloaded = touched_unloaded ? 1 : 0;
+  const bool want = members_out || member_node || nodes_out || node_ident || heads_out || faces_out || bonds_out || bond_ident;
+  const bool short_room = want && (m > res->members || K > res->nodes || nb > res->bonds || nf > res->faces);
+  *res = r;
+  if (short_room) return SVXG_BAD_INPUT;
+  if (members_out) std::copy(members.begin(), members.end(), members_out);
+  if (member_node) std::copy(fnode.begin(), fnode.end(), member_node);
+  if (nodes_out) std::copy(nodes.begin(), nodes.end(), nodes_out);
+  if (node_ident) std::copy(ident.begin(), ident.end(), node_ident);
+  if (heads_out) std::copy(heads.begin(), heads.end(), heads_out);
+  if (faces_out) std::copy(faces.begin(), faces.end(), faces_out);
+  if (bond_ident) std::copy(bid.begin(), bid.end(), bond_ident);
+  if (bonds_out && nb > 0)
+    orc_bond_sections(nodes.data(), K, heads.data(), nb, faces.data(), nf, box, mats, nmats, bonds_out);
+  return SVXG_OK;
+}
+
+extern "C" uint64_t orc_mix64_value(uint64_t x) { return orc_mix64(x); }

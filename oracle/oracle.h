@@ -72,6 +72,15 @@ void orc_bond_sections_general(const svxg_node* nodes, int32_t n, const svxg_sec
                                int32_t nsamples, const svxg_lattice* lattices, int32_t nlattices, int32_t S,
                                const svxg_material* mats, int32_t nmats, svxg_bond* out);
 int32_t orc_graph_components(int32_t n, const svxg_bond* bonds, int32_t m, const uint8_t* seed, int32_t* comp);
+/* World::Impl::extract's walk and append_nodes (world.cpp:425-620, 739-865) for the world grid over a
+   box: the restatement of svxg_extract_structure, taken sequentially as the reference takes it */
+int32_t orc_extract_structure(const svxg_voxel_box* box, const uint8_t* broken, const int32_t* frag,
+                              const svxg_fragment* frags, int32_t nfrag, int32_t seed, const svxg_extract_options* opt,
+                              const svxg_material* mats, int32_t nmats, svxg_extract_result* res, int32_t* members,
+                              int32_t* member_node, svxg_node* nodes, uint64_t* node_ident, svxg_section_head* heads,
+                              svxg_section_face* faces, svxg_bond* bonds, uint64_t* bond_ident);
+/* the SplitMix64 output function the identities use (world_internal.hpp:17-23) */
+uint64_t orc_mix64_value(uint64_t x);
 
 void orc_default_options(svxg_stress_options* out);
 /* the 6x6 block inverse of a level's diagonal (amg.cpp:105-135 blk6::inv6): 1 and out = A^-1, or 0

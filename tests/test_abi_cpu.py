@@ -53,10 +53,12 @@ def test_struct_layouts_match_a_c_compiler(tmp_path):
                "svxg_server_stats": abi.ServerStats, "svxg_judge_options": abi.JudgeOptions,
                "svxg_judge_state": abi.JudgeState, "svxg_judge_result": abi.JudgeResult,
                "svxg_cascade_result": abi.CascadeResult, "svxg_voxel_box": abi.VoxelBox,
-               "svxg_lattice": abi.Lattice}
+               "svxg_lattice": abi.Lattice, "svxg_extract_options": abi.ExtractOptions,
+               "svxg_extract_result": abi.ExtractResult}
     dtypes = {"svxg_hinge_cut": abi.HINGE_CUT, "svxg_break": abi.BREAK, "svxg_bond_load": abi.BOND_LOAD,
               "svxg_section_head": abi.SECTION_HEAD, "svxg_section_face": abi.SECTION_FACE,
-              "svxg_section_head_general": abi.SECTION_HEAD_GENERAL, "svxg_junction_sample": abi.JUNCTION_SAMPLE}
+              "svxg_section_head_general": abi.SECTION_HEAD_GENERAL, "svxg_junction_sample": abi.JUNCTION_SAMPLE,
+              "svxg_fragment": abi.FRAGMENT}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svxg.h"', "int main(void) {"]
     for c, py in mirrors.items():
         lines.append(f'  printf("{c} %zu\\n", sizeof({c}));')
@@ -91,6 +93,17 @@ def test_default_options_without_a_device():
     j = stress.default_judge_options()
     assert (j.fragility, j.dif, j.break_band, j.quota_fraction, j.hinge_shape) == (1.0, 1.5, 0.85, 0.25, 1.3)
     assert (j.max_breaks, j.max_rounds, j.plastic_hinges, j.rule) == (256, 400, 1, 0)
+
+
+def test_extract_default_options_without_a_device():
+    """svxg_extract_default_options: WorldConfig's extraction defaults (world.hpp:142-148, 265), the
+    same as abi.extract_options()."""
+    from opus_destruct_2_amd import stress
+    o = abi.ExtractOptions()
+    stress.lib().svxg_extract_default_options(ctypes.byref(o))
+    want = abi.extract_options()
+    assert bytes(o) == bytes(want)
+    assert (o.max_nodes, o.cluster_nodes, o.max_radius, o.cluster_cubes, o.outside) == (60000, 2500, 60.0, 0, 0)
 
 
 def _digest(g):
