@@ -439,8 +439,46 @@ def run_graph(mats, ctx, shape, cpu=True):
     rec["iters"] = r["iters"]
     rec["dofs"] = int(6 * p.info()["nfree"])
     p.close()
+    rec["extraction"] = run_extract(mats, ctx, vox, frag, lo, h, cpu=cpu)
     log(f"  graph from voxels: {rec}")
     return rec
+
+
+def run_extract(mats, ctx, vox, frag, lo, h, cpu=True):
+    """The structure the walk from a seed takes (svxg_extract_structure: World::Impl::extract and
+    append_nodes on the device) over the same block, its fragments cut at chunk boundaries: with the
+    reference's limits (60000 members, clusters of 16-voxel cells past 15000) and with none (every
+    fragment a node). Wall time of the two calls (the counts, then the records) with the copies,
+    against the oracle's sequential walk on one host thread, and the bits compared."""
+    import oracle
+    from opus_destruct_2_amd import voxels
+    from opus_destruct_2_amd.abi import extract_options
+    frag = voxels.chunk_fragments(frag, lo)
+    fr = voxels.fragments(vox, frag, lo, h, mats)
+    nz, ny, nx = frag.shape
+    seed = int(frag[min(3, nz - 1), ny // 2, nx // 2])
+    out = {"fragments": int(len(fr)), "seed": seed}
+    for name, kw in (("limits", {}), ("whole", {"max_nodes": 1 << 20, "cluster_nodes": 1 << 20, "max_radius": 1e9})):
+        opts = extract_options(**kw)
+        walls = []
+        for _ in range(3):
+            t = time.perf_counter()
+            d = voxels.extract_structure(vox, frag, lo, h, fr, seed, mats, opts=opts, ctx=ctx)
+            walls.append((time.perf_counter() - t) * 1e3)
+        r = {"members": int(len(d["members"])), "nodes": int(len(d["nodes"])), "bonds": int(len(d["bonds"])),
+             "faces": int(len(d["faces"])), "cell": d["cell"], "truncated": d["truncated"],
+             "device_wall_ms": round(float(np.median(walls)), 2),
+             "members_per_s": round(len(d["members"]) / (float(np.median(walls)) / 1e3), 1)}
+        if cpu:
+            t = time.perf_counter()
+            o = oracle.extract_structure(vox, frag, lo, h, fr, seed, mats, opts=opts)
+            r["cpu_walk_s"] = round(time.perf_counter() - t, 3)
+            r["cpu_cores"] = 1
+            r["bitwise_equal_cpu"] = bool(all(o[k].tobytes() == d[k].tobytes() for k in
+                                              ("members", "member_node", "nodes", "node_ident", "heads", "faces",
+                                               "bonds", "bond_ident")))
+        out[name] = r
+    return out
 
 
 class _Piece:

@@ -6,14 +6,18 @@
 // held fixed: past the radius or the node limit, or a fresh structure's). Its order is that of a
 // breadth-first search whose ties go to the earliest (member, voxel, face) that met a fragment, so
 // the device takes it level by level:
-//   k_walk_meet    a workgroup a member of the level: every fragment it meets that has no slot yet
+//   k_walk_meet    a wavefront a member of the level: every fragment it meets that has no slot yet
 //                  keeps the least key (member, voxel's place in the box's x-slowest scan, face)
-//                  that met it (atomicMin) and joins the level's candidates
-//   sort           the candidates by that key: the order the queue would have appended them in
-//   k_walk_assign  slots in that order (a scan of the eligible ones), up to the node limit
+//                  that met it (atomicMin) - the member whose key it keeps met it first
+//   k_walk_own     a wavefront a member: how many fragments it met first may be taken
+//   k_walk_scan    those counts scanned over the level: each member's first slot
+//   k_walk_assign  a wavefront a member: its first-met fragments in the order it met them (a lane a
+//                  voxel in scan order, faces in turn, a wavefront scan) take the next slots while
+//                  there is room, the rest are held fixed - the order the queue appends them in
+// The host runs levels in batches (five launches a level, a look at the level's size a batch).
 // Then the SectionLog (world_internal.hpp:533-620) as records in the walk's order:
-//   k_walk_emit    a workgroup a member, its voxels in scan order, a block scan placing each face's
-//                  record: a fine bond's key (acc_key, world_internal.hpp:447-450), the face, a
+//   k_walk_emit    a wavefront a member, its voxels in scan order, a wavefront scan placing each
+//                  face's record: a fine bond's key (acc_key, world_internal.hpp:447-450), the face, a
 //                  support's material
 //   sort           records by key (stable: a fine bond's faces in the order met); a fine bond's
 //                  first record gives its place in the order fine bonds were made
@@ -22,6 +26,7 @@
 // made (a stable sort of the node-level keys over the fine bonds in order), a pair's ends in the
 // order of their identities (k_walk_heads), and every face placed (k_walk_faces). All of it integer
 // and ordering work over the box (HBM-bound: a few bytes a voxel and a record a face).
+#include <algorithm>
 #include <cstring>
 
 #include "svxg_kernels.hpp"
@@ -103,13 +108,44 @@ __device__ __forceinline__ void member_voxel(const WalkDev& W, const MemberBox& 
   *v = *x + int64_t(W.dim[0]) * (*y + int64_t(W.dim[1]) * *z);
 }
 
+constexpr int kWaves = kWG / 64;  // wavefronts a workgroup: a member a wavefront
+
+// an exclusive scan over the wavefront (every lane calls it); *total: the wavefront's sum
+__device__ __forceinline__ int wave_scan(int v, int* total) {
+  const int lane = threadIdx.x & 63;
+  int incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  *total = __shfl(incl, 63, 64);
+  return incl - v;
+}
+
+__device__ __forceinline__ uint64_t walk_key(int i, int x, int y, int z, int d, const WalkDev& W) {
+  const uint64_t t = (uint64_t(x) * uint64_t(W.dim[1]) + uint64_t(y)) * uint64_t(W.dim[2]) + uint64_t(z);
+  return (uint64_t(i) << 34) | (t * 6 + uint64_t(d));
+}
+
+// met first: taken while there is room if not a fresh structure's and within the radius (or another
+// structure's), else held fixed (world.cpp:484-506)
+__device__ __forceinline__ bool eligible(const WalkDev& W, int G) {
+  const svxg_fragment& g = W.frags[G];
+  const bool owned = g.owner != 0;
+  const bool fresh = owned && g.owner >= W.fresh_from;
+  if (fresh) return false;
+  if (owned) return true;
+  const svxg_fragment& s0 = W.frags[W.seed];
+  const double dx = g.com[0] - s0.com[0], dy = g.com[1] - s0.com[1], dz = g.com[2] - s0.com[2];
+  return sqrt(dx * dx + dy * dy + dz * dz) <= W.max_radius;
+}
+
 __global__ void __launch_bounds__(kWG) k_walk_init(const WalkDev* __restrict__ Wp) {
   const WalkDev& W = *Wp;
   const int f = int(blockIdx.x) * kWG + threadIdx.x;
   if (f >= W.nfrag) return;
   W.slot[f] = -1;
   W.meet[f] = ~uint64_t(0);
-  W.candflag[f] = 0;
   for (int c = 0; c < 3; ++c) {
     W.bbox[6 * f + c] = 0x7FFFFFFF;
     W.bbox[6 * f + 3 + c] = -1;
@@ -121,6 +157,27 @@ __global__ void __launch_bounds__(kWG) k_walk_init(const WalkDev* __restrict__ W
     s.level_end = 1;
     *W.st = s;
   }
+}
+
+// the inputs' rules, a thread a voxel: 1 a fragment index out of range, 2 a fragment on air or an
+// anchored voxel, 4 a fragment's voxel outside the chunk of its first voxel
+__global__ void __launch_bounds__(kWG) k_walk_check(const WalkDev* __restrict__ Wp, int32_t* __restrict__ bad) {
+  const WalkDev& W = *Wp;
+  const int64_t v = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (v >= W.nvox) return;
+  const int f = W.frag[v];
+  if (f == -1) return;
+  if (f < -1 || f >= W.nfrag) {
+    atomicOr(bad, 1);
+    return;
+  }
+  const uint8_t c = W.vox[v];
+  if (c == 0 || (c & 0x80)) atomicOr(bad, 2);
+  const int nx = W.dim[0], ny = W.dim[1];
+  const int g[3] = {int(v % nx) + W.lo[0], int((v / nx) % ny) + W.lo[1], int(v / (int64_t(nx) * ny)) + W.lo[2]};
+  const svxg_fragment& fr = W.frags[f];
+  if (fdiv32(g[0]) != fdiv32(fr.first[0]) || fdiv32(g[1]) != fdiv32(fr.first[1]) || fdiv32(g[2]) != fdiv32(fr.first[2]))
+    atomicOr(bad, 4);
 }
 
 __global__ void __launch_bounds__(kWG) k_walk_bbox(const WalkDev* __restrict__ Wp) {
@@ -148,89 +205,153 @@ __global__ void k_walk_seed(const WalkDev* __restrict__ Wp) {
   }
 }
 
-// a workgroup a member of the level: the fragments it meets that have no slot yet
+// a wavefront a member of the level (the grid strides over the level): every fragment it meets
+// that has no slot yet keeps the least key that met it
 __global__ void __launch_bounds__(kWG) k_walk_meet(const WalkDev* __restrict__ Wp) {
   const WalkDev& W = *Wp;
-  const int i = W.st->level_begin + int(blockIdx.x);
-  if (i >= W.st->level_end) return;
-  const int F = W.members[i];
-  const MemberBox b = member_box(W, F);
-  for (int64_t j = threadIdx.x; j < b.total; j += kWG) {
-    int x, y, z;
-    int64_t v;
-    member_voxel(W, b, j, &x, &y, &z, &v);
-    if (W.frag[v] != F) continue;
-    const uint8_t brk = W.broken ? W.broken[v] : 0;
-    const uint64_t t = (uint64_t(x) * uint64_t(W.dim[1]) + uint64_t(y)) * uint64_t(W.dim[2]) + uint64_t(z);
-    for (int d = 0; d < 6; ++d) {
-      int G = -1, unl = 0;
-      uint8_t mb = 0;
-      if (walk_face(W, F, v, x, y, z, d, brk, &G, &mb, &unl) != kFaceFrag || W.slot[G] != -1) continue;
-      const uint64_t key = (uint64_t(i) << 34) | (t * 6 + uint64_t(d));
-      atomicMin(reinterpret_cast<unsigned long long*>(W.meet + G), (unsigned long long)key);
-      if (atomicCAS(W.candflag + G, 0, 1) == 0) W.cand[atomicAdd(&W.st->ncand, 1)] = G;
+  const int lane = threadIdx.x & 63;
+  const int w = int(blockIdx.x) * kWaves + int(threadIdx.x >> 6), nw = int(gridDim.x) * kWaves;
+  const int end = W.st->level_end;
+  for (int i = W.st->level_begin + w; i < end; i += nw) {
+    const int F = W.members[i];
+    const MemberBox b = member_box(W, F);
+    for (int64_t j = lane; j < b.total; j += 64) {
+      int x, y, z;
+      int64_t v;
+      member_voxel(W, b, j, &x, &y, &z, &v);
+      if (W.frag[v] != F) continue;
+      const uint8_t brk = W.broken ? W.broken[v] : 0;
+      for (int d = 0; d < 6; ++d) {
+        int G = -1, unl = 0;
+        uint8_t mb = 0;
+        if (walk_face(W, F, v, x, y, z, d, brk, &G, &mb, &unl) != kFaceFrag || W.slot[G] != -1) continue;
+        atomicMin(reinterpret_cast<unsigned long long*>(W.meet + G), (unsigned long long)walk_key(i, x, y, z, d, W));
+      }
     }
   }
 }
 
-__global__ void __launch_bounds__(kWG) k_walk_keys(const WalkDev* __restrict__ Wp, int n, uint64_t* __restrict__ key,
-                                                   int32_t* __restrict__ val) {
+// a wavefront a member: the fragments it met first (its key is their least) that may be taken
+__global__ void __launch_bounds__(kWG) k_walk_own(const WalkDev* __restrict__ Wp) {
   const WalkDev& W = *Wp;
-  const int k = int(blockIdx.x) * kWG + threadIdx.x;
-  if (k >= n) return;
-  const int G = W.cand[k];
-  key[k] = W.meet[G];
-  val[k] = G;
-}
-
-// met in order: taken while there is room if not a fresh structure's and within the radius (or
-// another structure's), else held fixed (world.cpp:484-506)
-__global__ void __launch_bounds__(kWG) k_walk_eligible(const WalkDev* __restrict__ Wp, const int32_t* __restrict__ val,
-                                                       int n, int32_t* __restrict__ e) {
-  const WalkDev& W = *Wp;
-  const int k = int(blockIdx.x) * kWG + threadIdx.x;
-  if (k >= n) return;
-  const svxg_fragment& g = W.frags[val[k]];
-  const svxg_fragment& s0 = W.frags[W.seed];
-  const bool owned = g.owner != 0;
-  const bool fresh = owned && g.owner >= W.fresh_from;
-  bool take = !fresh && owned;
-  if (!fresh && !owned) {
-    const double dx = g.com[0] - s0.com[0], dy = g.com[1] - s0.com[1], dz = g.com[2] - s0.com[2];
-    take = sqrt(dx * dx + dy * dy + dz * dz) <= W.max_radius;
-  }
-  e[k] = take ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kWG) k_walk_assign(const WalkDev* __restrict__ Wp, const int32_t* __restrict__ val,
-                                                     const int32_t* __restrict__ e, const int32_t* __restrict__ pre,
-                                                     int n) {
-  const WalkDev& W = *Wp;
-  const int k = int(blockIdx.x) * kWG + threadIdx.x;
-  if (k >= n) return;
-  const int G = val[k];
-  const int s = W.st->count + pre[k];
-  if (e[k] && s < W.max_nodes) {
-    W.slot[G] = s;
-    W.members[s] = G;
-  } else {
-    W.slot[G] = -2;
-    W.st->truncated = 1;
+  const int lane = threadIdx.x & 63;
+  const int w = int(blockIdx.x) * kWaves + int(threadIdx.x >> 6), nw = int(gridDim.x) * kWaves;
+  const int end = W.st->level_end;
+  for (int i = W.st->level_begin + w; i < end; i += nw) {
+    const int F = W.members[i];
+    const MemberBox b = member_box(W, F);
+    int ne = 0;
+    for (int64_t j = lane; j < b.total; j += 64) {
+      int x, y, z;
+      int64_t v;
+      member_voxel(W, b, j, &x, &y, &z, &v);
+      if (W.frag[v] != F) continue;
+      const uint8_t brk = W.broken ? W.broken[v] : 0;
+      for (int d = 0; d < 6; ++d) {
+        int G = -1, unl = 0;
+        uint8_t mb = 0;
+        if (walk_face(W, F, v, x, y, z, d, brk, &G, &mb, &unl) != kFaceFrag) continue;
+        if (W.meet[G] == walk_key(i, x, y, z, d, W) && eligible(W, G)) ++ne;
+      }
+    }
+    int total;
+    wave_scan(ne, &total);
+    if (lane == 0) W.ecnt[i] = total;
   }
 }
 
-__global__ void k_walk_level(const WalkDev* __restrict__ Wp, const int32_t* __restrict__ e,
-                             const int32_t* __restrict__ pre, int n) {
+// one workgroup: the level's members' counts scanned (their first slots among the level's), and the
+// level's total
+__global__ void __launch_bounds__(1024) k_walk_scan(const WalkDev* __restrict__ Wp) {
+  const WalkDev& W = *Wp;
+  __shared__ int lds[1024];
+  __shared__ int carry;
+  const int t = threadIdx.x;
+  const int begin = W.st->level_begin, end = W.st->level_end;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = begin; base < end; base += 1024) {
+    const int i = base + t;
+    const int v = i < end ? W.ecnt[i] : 0;
+    lds[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int add = t >= o ? lds[t - o] : 0;
+      __syncthreads();
+      lds[t] += add;
+      __syncthreads();
+    }
+    if (i < end) W.ebase[i] = carry + lds[t] - v;
+    __syncthreads();
+    if (t == 0) carry += lds[1023];
+    __syncthreads();
+  }
+  if (t == 0) W.st->eligible = carry;
+}
+
+// a wavefront a member: its first-met fragments in the order it met them (voxels in scan order,
+// a lane a voxel, faces in turn): slots while there is room, else held fixed
+__global__ void __launch_bounds__(kWG) k_walk_assign(const WalkDev* __restrict__ Wp) {
+  const WalkDev& W = *Wp;
+  const int lane = threadIdx.x & 63;
+  const int w = int(blockIdx.x) * kWaves + int(threadIdx.x >> 6), nw = int(gridDim.x) * kWaves;
+  const int end = W.st->level_end;
+  const int count = W.st->count;
+  for (int i = W.st->level_begin + w; i < end; i += nw) {
+    const int F = W.members[i];
+    const MemberBox b = member_box(W, F);
+    int next = count + W.ebase[i];
+    for (int64_t j0 = 0; j0 < b.total; j0 += 64) {
+      const int64_t j = j0 + lane;
+      int G6[6];
+      uint8_t e6[6];
+      int n = 0, ne = 0;
+      if (j < b.total) {
+        int x, y, z;
+        int64_t v;
+        member_voxel(W, b, j, &x, &y, &z, &v);
+        if (W.frag[v] == F) {
+          const uint8_t brk = W.broken ? W.broken[v] : 0;
+          for (int d = 0; d < 6; ++d) {
+            int G = -1, unl = 0;
+            uint8_t mb = 0;
+            if (walk_face(W, F, v, x, y, z, d, brk, &G, &mb, &unl) != kFaceFrag) continue;
+            if (W.meet[G] != walk_key(i, x, y, z, d, W)) continue;
+            G6[n] = G;
+            e6[n] = eligible(W, G) ? 1 : 0;
+            ne += e6[n];
+            ++n;
+          }
+        }
+      }
+      int total;
+      int s = next + wave_scan(ne, &total);
+      for (int k = 0; k < n; ++k) {
+        const int G = G6[k];
+        if (e6[k] && s < W.max_nodes) {
+          W.slot[G] = s;
+          W.members[s] = G;
+        } else {
+          W.slot[G] = -2;
+          W.st->truncated = 1;
+        }
+        if (e6[k]) ++s;
+      }
+      next += total;
+    }
+  }
+}
+
+// the next level: the members just taken
+__global__ void k_walk_next(const WalkDev* __restrict__ Wp) {
   const WalkDev& W = *Wp;
   if (threadIdx.x != 0) return;
   WalkState& s = *W.st;
-  const int total = pre[n - 1] + e[n - 1];
-  const int room = max(0, W.max_nodes - s.count);
-  const int taken = min(total, room);
+  const int taken = min(s.eligible, max(0, W.max_nodes - s.count));
   s.level_begin = s.count;
   s.count += taken;
   s.level_end = s.count;
-  s.ncand = 0;
+  s.eligible = 0;
 }
 
 // a member's records: how many (pass 0) or written in order (pass 1)
@@ -280,81 +401,75 @@ __device__ __forceinline__ int voxel_records(const WalkDev& W, int i, int F, int
   return c;
 }
 
-// an exclusive scan over the workgroup (LDS, kWG entries); returns the total
-__device__ __forceinline__ int block_scan(int v, int* lds, int* excl) {
-  const int t = threadIdx.x;
-  lds[t] = v;
-  __syncthreads();
-  for (int o = 1; o < kWG; o <<= 1) {
-    const int add = t >= o ? lds[t - o] : 0;
-    __syncthreads();
-    lds[t] += add;
-    __syncthreads();
-  }
-  *excl = lds[t] - v;
-  const int total = lds[kWG - 1];
-  __syncthreads();
-  return total;
-}
-
-__global__ void __launch_bounds__(kWG) k_walk_count(const WalkDev* __restrict__ Wp, int32_t* __restrict__ count) {
-  const WalkDev& W = *Wp;
-  __shared__ int lds[kWG];
-  const int i = blockIdx.x;
-  const int F = W.members[i];
-  const MemberBox b = member_box(W, F);
-  int c = 0, flags = 0;
-  Rec r[6];
-  for (int64_t j = threadIdx.x; j < b.total; j += kWG) {
-    int x, y, z;
-    int64_t v;
-    member_voxel(W, b, j, &x, &y, &z, &v);
-    if (W.frag[v] != F) continue;
-    c += voxel_records(W, i, F, x, y, z, v, r, &flags);
-  }
-  int excl;
-  const int total = block_scan(c, lds, &excl);
+__device__ __forceinline__ void raise_flags(const WalkDev& W, int flags) {
   if (flags & 1) W.st->any_support = 1;
   if (flags & 2) W.st->touched_unloaded = 1;
   if (flags & 4) W.st->bad = 1;
-  if (threadIdx.x == 0) count[i] = total;
 }
 
-__global__ void __launch_bounds__(kWG) k_walk_emit(const WalkDev* __restrict__ Wp, const int32_t* __restrict__ off,
-                                                   uint64_t* __restrict__ rkey, svxg_section_face* __restrict__ rface,
-                                                   uint8_t* __restrict__ rmb) {
+// a wavefront a member (the grid strides over the members): its record count
+__global__ void __launch_bounds__(kWG) k_walk_count(const WalkDev* __restrict__ Wp, int nm, int32_t* __restrict__ count) {
   const WalkDev& W = *Wp;
-  __shared__ int lds[kWG];
-  const int i = blockIdx.x;
-  const int F = W.members[i];
-  const MemberBox b = member_box(W, F);
-  int64_t at = off[i];
-  for (int64_t j0 = 0; j0 < b.total; j0 += kWG) {
-    const int64_t j = j0 + threadIdx.x;
-    Rec r[6];
+  const int lane = threadIdx.x & 63;
+  const int w = int(blockIdx.x) * kWaves + int(threadIdx.x >> 6), nw = int(gridDim.x) * kWaves;
+  for (int i = w; i < nm; i += nw) {
+    const int F = W.members[i];
+    const MemberBox b = member_box(W, F);
     int c = 0, flags = 0;
-    if (j < b.total) {
+    Rec r[6];
+    for (int64_t j = lane; j < b.total; j += 64) {
       int x, y, z;
       int64_t v;
       member_voxel(W, b, j, &x, &y, &z, &v);
-      if (W.frag[v] == F) c = voxel_records(W, i, F, x, y, z, v, r, &flags);
+      if (W.frag[v] != F) continue;
+      c += voxel_records(W, i, F, x, y, z, v, r, &flags);
     }
-    int excl;
-    const int total = block_scan(c, lds, &excl);
-    for (int k = 0; k < c; ++k) {
-      const int64_t o = at + excl + k;
-      rkey[o] = r[k].key;
-      svxg_section_face f;
-      memset(&f, 0, sizeof f);
-      f.p[0] = r[k].p[0];
-      f.p[1] = r[k].p[1];
-      f.p[2] = r[k].p[2];
-      f.axis = r[k].axis;
-      f.sign = r[k].sign;
-      rface[o] = f;
-      rmb[o] = r[k].mb;
+    raise_flags(W, flags);
+    int total;
+    wave_scan(c, &total);
+    if (lane == 0) count[i] = total;
+  }
+}
+
+// a wavefront a member: its records written in order from off[i] (a lane a voxel, the voxels in
+// scan order)
+__global__ void __launch_bounds__(kWG) k_walk_emit(const WalkDev* __restrict__ Wp, int nm,
+                                                   const int32_t* __restrict__ off, uint64_t* __restrict__ rkey,
+                                                   svxg_section_face* __restrict__ rface, uint8_t* __restrict__ rmb) {
+  const WalkDev& W = *Wp;
+  const int lane = threadIdx.x & 63;
+  const int w = int(blockIdx.x) * kWaves + int(threadIdx.x >> 6), nw = int(gridDim.x) * kWaves;
+  for (int i = w; i < nm; i += nw) {
+    const int F = W.members[i];
+    const MemberBox b = member_box(W, F);
+    int64_t at = off[i];
+    for (int64_t j0 = 0; j0 < b.total; j0 += 64) {
+      const int64_t j = j0 + lane;
+      Rec r[6];
+      int c = 0, flags = 0;
+      if (j < b.total) {
+        int x, y, z;
+        int64_t v;
+        member_voxel(W, b, j, &x, &y, &z, &v);
+        if (W.frag[v] == F) c = voxel_records(W, i, F, x, y, z, v, r, &flags);
+      }
+      int total;
+      const int excl = wave_scan(c, &total);
+      for (int k = 0; k < c; ++k) {
+        const int64_t o = at + excl + k;
+        rkey[o] = r[k].key;
+        svxg_section_face f;
+        memset(&f, 0, sizeof f);
+        f.p[0] = r[k].p[0];
+        f.p[1] = r[k].p[1];
+        f.p[2] = r[k].p[2];
+        f.axis = r[k].axis;
+        f.sign = r[k].sign;
+        rface[o] = f;
+        rmb[o] = r[k].mb;
+      }
+      at += total;
     }
-    at += total;
   }
 }
 
@@ -649,31 +764,32 @@ inline unsigned blocks(int64_t n) { return unsigned((n + kWG - 1) / kWG); }
 
 }  // namespace
 
+constexpr unsigned kLevelGrid = 512;  // workgroups of a level's launches (2048 wavefronts)
+
+static unsigned member_grid(int nm) { return unsigned(std::min<int64_t>(4096, (int64_t(nm) + kWaves - 1) / kWaves)); }
+
+void launch_walk_check(hipStream_t s, const WalkDev* W, int64_t nvox, int32_t* bad) {
+  if (nvox > 0) hipLaunchKernelGGL(k_walk_check, dim3(blocks(nvox)), dim3(kWG), 0, s, W, bad);
+}
 void launch_walk_init(hipStream_t s, const WalkDev* W, int nfrag, int64_t nvox) {
   hipLaunchKernelGGL(k_walk_init, dim3(blocks(nfrag)), dim3(kWG), 0, s, W);
   if (nvox > 0) hipLaunchKernelGGL(k_walk_bbox, dim3(blocks(nvox)), dim3(kWG), 0, s, W);
   hipLaunchKernelGGL(k_walk_seed, dim3(1), dim3(64), 0, s, W);
 }
-void launch_walk_meet(hipStream_t s, const WalkDev* W, int level_upper) {
-  if (level_upper > 0) hipLaunchKernelGGL(k_walk_meet, dim3(unsigned(level_upper)), dim3(kWG), 0, s, W);
-}
-void launch_walk_keys(hipStream_t s, const WalkDev* W, int n, uint64_t* key, int32_t* val) {
-  hipLaunchKernelGGL(k_walk_keys, dim3(blocks(n)), dim3(kWG), 0, s, W, n, key, val);
-}
-void launch_walk_eligible(hipStream_t s, const WalkDev* W, const int32_t* val, int n, int32_t* e) {
-  hipLaunchKernelGGL(k_walk_eligible, dim3(blocks(n)), dim3(kWG), 0, s, W, val, n, e);
-}
-void launch_walk_assign(hipStream_t s, const WalkDev* W, const int32_t* val, const int32_t* e, const int32_t* pre,
-                        int n) {
-  hipLaunchKernelGGL(k_walk_assign, dim3(blocks(n)), dim3(kWG), 0, s, W, val, e, pre, n);
-  hipLaunchKernelGGL(k_walk_level, dim3(1), dim3(64), 0, s, W, e, pre, n);
+void launch_walk_level(hipStream_t s, const WalkDev* W) {
+  hipLaunchKernelGGL(k_walk_meet, dim3(kLevelGrid), dim3(kWG), 0, s, W);
+  hipLaunchKernelGGL(k_walk_own, dim3(kLevelGrid), dim3(kWG), 0, s, W);
+  hipLaunchKernelGGL(k_walk_scan, dim3(1), dim3(1024), 0, s, W);
+  hipLaunchKernelGGL(k_walk_assign, dim3(kLevelGrid), dim3(kWG), 0, s, W);
+  hipLaunchKernelGGL(k_walk_next, dim3(1), dim3(64), 0, s, W);
 }
 void launch_walk_count(hipStream_t s, const WalkDev* W, int nm, int32_t* count) {
-  if (nm > 0) hipLaunchKernelGGL(k_walk_count, dim3(unsigned(nm)), dim3(kWG), 0, s, W, count);
+  if (nm > 0) hipLaunchKernelGGL(k_walk_count, dim3(member_grid(nm)), dim3(kWG), 0, s, W, nm, count);
 }
 void launch_walk_emit(hipStream_t s, const WalkDev* W, int nm, const int32_t* off, uint64_t* rkey,
                       svxg_section_face* rface, uint8_t* rmb) {
-  if (nm > 0) hipLaunchKernelGGL(k_walk_emit, dim3(unsigned(nm)), dim3(kWG), 0, s, W, off, rkey, rface, rmb);
+  if (nm > 0)
+    hipLaunchKernelGGL(k_walk_emit, dim3(member_grid(nm)), dim3(kWG), 0, s, W, nm, off, rkey, rface, rmb);
 }
 void launch_seg_first(hipStream_t s, const uint64_t* key, int n, int32_t* first) {
   if (n > 0) hipLaunchKernelGGL(k_seg_first, dim3(blocks(n)), dim3(kWG), 0, s, key, n, first);

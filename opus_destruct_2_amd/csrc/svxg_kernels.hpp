@@ -441,7 +441,7 @@ struct WalkState {
   int32_t count;         // members so far
   int32_t level_begin;   // the level being walked: members [level_begin, level_end)
   int32_t level_end;
-  int32_t ncand;         // fragments the level met for the first time
+  int32_t eligible;      // fragments the level met first that may be taken
   int32_t truncated, any_support, touched_unloaded, bad;
 };
 struct WalkDev {
@@ -452,8 +452,8 @@ struct WalkDev {
   int32_t* bbox;          // 6 a fragment: its voxels' least and greatest box-local x, y, z
   int32_t* slot;          // a fragment's member index; -1 not met; -2 held fixed
   uint64_t* meet;         // the least (member, voxel, face) key that met a fragment
-  int32_t* candflag;
-  int32_t* cand;
+  int32_t* ecnt;          // a member's first-met fragments that may be taken
+  int32_t* ebase;         // their first slot among the level's
   int32_t* members;
   WalkState* st;
   int32_t lo[3], dim[3];
@@ -489,12 +489,12 @@ struct WalkMerge {
   const uint64_t* ident;
   uint8_t* swapped;          // a merged bond's ends swapped for their identities
 };
+// the inputs' rules checked (*bad: 1 a fragment index out of range, 2 a fragment on air or an
+// anchored voxel, 4 a fragment's voxel outside its chunk); the walk's arrays set up
+void launch_walk_check(hipStream_t s, const WalkDev* W, int64_t nvox, int32_t* bad);
 void launch_walk_init(hipStream_t s, const WalkDev* W, int nfrag, int64_t nvox);
-void launch_walk_meet(hipStream_t s, const WalkDev* W, int level_upper);
-void launch_walk_keys(hipStream_t s, const WalkDev* W, int n, uint64_t* key, int32_t* val);
-void launch_walk_eligible(hipStream_t s, const WalkDev* W, const int32_t* val, int n, int32_t* e);
-void launch_walk_assign(hipStream_t s, const WalkDev* W, const int32_t* val, const int32_t* e, const int32_t* pre,
-                        int n);
+// one level of the walk (meet, own, scan, assign, next): no-ops once a level is empty
+void launch_walk_level(hipStream_t s, const WalkDev* W);
 void launch_walk_count(hipStream_t s, const WalkDev* W, int nm, int32_t* count);
 void launch_walk_emit(hipStream_t s, const WalkDev* W, int nm, const int32_t* off, uint64_t* rkey,
                       svxg_section_face* rface, uint8_t* rmb);

@@ -594,8 +594,11 @@ def fragments(vox, frag, lo, h, mats):
 
 
 def _extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts):
-    """The two calls of svxg_extract_structure (or its restatement): the counts, then the records.
-    call(box, broken, frag, frags, nfrag, seed, opts, mats, nmats, res, *outputs) -> status."""
+    """svxg_extract_structure (or its restatement) with room for a structure of every fragment: the
+    members and nodes at most nfrag, the faces at most six a fragment voxel, the bonds at most the
+    faces (the arrays are zero pages until written); a second call with the exact counts when the
+    bonds outgrow the room given. call(box, broken, frag, frags, nfrag, seed, opts, mats, nmats, res,
+    *outputs) -> status (0 ok, 1 bad input or short room)."""
     from .abi import ExtractResult, extract_options
     vox = np.ascontiguousarray(vox, np.uint8)
     frag = np.ascontiguousarray(frag, np.int32)
@@ -606,22 +609,29 @@ def _extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts):
     opts = opts if opts is not None else extract_options()
     box = voxel_box(vox, lo, h, damage)
     bp = None if broken is None else broken.ctypes.data
-    res = ExtractResult()
-    args = (ctypes.byref(box), bp, frag.ctypes.data, frags.ctypes.data, len(frags), int(seed), ctypes.byref(opts),
-            mats.ctypes.data, len(mats), ctypes.byref(res))
-    call(*args, *([None] * 8))
-    counts = {k: getattr(res, k) for k in ("members", "nodes", "bonds", "faces")}
-    out = {"members": np.zeros(max(1, counts["members"]), np.int32),
-           "member_node": np.zeros(max(1, counts["members"]), np.int32),
-           "nodes": np.zeros(max(1, counts["nodes"]), NODE), "node_ident": np.zeros(max(1, counts["nodes"]), np.uint64),
-           "heads": np.zeros(max(1, counts["bonds"]), SECTION_HEAD), "faces": np.zeros(max(1, counts["faces"]), SECTION_FACE),
-           "bonds": np.zeros(max(1, counts["bonds"]), BOND), "bond_ident": np.zeros(max(1, counts["bonds"]), np.uint64)}
-    call(*args, *(out[k].ctypes.data for k in ("members", "member_node", "nodes", "node_ident", "heads", "faces",
-                                                "bonds", "bond_ident")))
+    nsolid = int(np.count_nonzero(frag >= 0))
+    room = {"members": len(frags), "nodes": len(frags), "faces": max(1, 6 * nsolid)}
+    room["bonds"] = min(room["faces"], 12 * len(frags))
+    for attempt in range(2):
+        res = ExtractResult(**room)
+        out = {"members": np.zeros(room["members"], np.int32), "member_node": np.zeros(room["members"], np.int32),
+               "nodes": np.zeros(room["nodes"], NODE), "node_ident": np.zeros(room["nodes"], np.uint64),
+               "heads": np.zeros(room["bonds"], SECTION_HEAD), "faces": np.zeros(room["faces"], SECTION_FACE),
+               "bonds": np.zeros(room["bonds"], BOND), "bond_ident": np.zeros(room["bonds"], np.uint64)}
+        st = call(ctypes.byref(box), bp, frag.ctypes.data, frags.ctypes.data, len(frags), int(seed),
+                  ctypes.byref(opts), mats.ctypes.data, len(mats), ctypes.byref(res),
+                  *(out[k].ctypes.data for k in ("members", "member_node", "nodes", "node_ident", "heads", "faces",
+                                                 "bonds", "bond_ident")))
+        counts = {k: getattr(res, k) for k in ("members", "nodes", "bonds", "faces")}
+        if st == 0:
+            break
+        if attempt or all(counts[k] <= room[k] for k in room):
+            raise RuntimeError(f"svxg_extract_structure: status {st}")
+        room = {k: max(1, counts[k]) for k in room}
     n = {"members": counts["members"], "member_node": counts["members"], "nodes": counts["nodes"],
          "node_ident": counts["nodes"], "heads": counts["bonds"], "faces": counts["faces"], "bonds": counts["bonds"],
          "bond_ident": counts["bonds"]}
-    out = {k: v[:n[k]] for k, v in out.items()}
+    out = {k: v[:n[k]] for k, v in out.items()}  # (views: the room past them was never written)
     for k in ("cell", "truncated", "any_support", "touched_unloaded"):
         out[k] = int(getattr(res, k))
     return out
@@ -638,8 +648,14 @@ def extract_structure(vox, frag, lo, h, frags, seed, mats, broken=None, damage=N
     L = stress.lib()
 
     def call(*a):
-        stress._check(L.svxg_extract_structure(ctx.h, *a), "svxg_extract_structure")
-    return _extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts)
+        st = L.svxg_extract_structure(ctx.h, *a)
+        if st not in (0, 1):
+            stress._check(st, "svxg_extract_structure")
+        return st
+    try:
+        return _extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts)
+    except RuntimeError as e:
+        raise stress.SvxgError(f"{e}: {L.svxg_last_error().decode(errors='replace')}") from None
 
 
 def gravity(nodes):

@@ -334,11 +334,7 @@ def extract_structure(vox, frag, lo, h, frags, seed, mats, broken=None, damage=N
     from opus_destruct_2_amd import voxels
     L, _ = library("oracle")
 
-    def call(*a):
-        st = L.orc_extract_structure(*a)
-        if st != 0 and a[10] is not None:
-            raise RuntimeError(f"orc_extract_structure: status {st}")
-    return voxels._extract(call, vox, frag, lo, h, frags, seed, mats, broken, damage, opts)
+    return voxels._extract(L.orc_extract_structure, vox, frag, lo, h, frags, seed, mats, broken, damage, opts)
 
 
 def mix64(x):

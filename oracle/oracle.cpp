@@ -1870,6 +1870,32 @@ uint64_t orc_acc_key(i32 a, i32 b, int axis, int sign) {  // world_internal.hpp:
   return (uint64_t(a) << 32) | (uint64_t(axis * 2 + (sign > 0 ? 1 : 0)) << 1);
 }
 
+// keys to dense indices in the order they first came (an open-addressed table, as
+// world_internal.hpp:493-532's KeyIndex does it)
+struct WalkKeys {
+  std::vector<uint64_t> keys;
+  std::vector<i32> table;
+  i32 operator()(uint64_t k) {
+    if (keys.size() * 2 >= table.size()) {
+      std::vector<i32> t(std::max<size_t>(1024, table.size() * 2), -1);
+      for (size_t e = 0; e < keys.size(); ++e) {
+        size_t i = size_t(orc_mix64(keys[e])) & (t.size() - 1);
+        while (t[i] >= 0) i = (i + 1) & (t.size() - 1);
+        t[i] = i32(e);
+      }
+      table.swap(t);
+    }
+    size_t i = size_t(orc_mix64(k)) & (table.size() - 1);
+    while (table[i] >= 0) {
+      if (keys[size_t(table[i])] == k) return table[i];
+      i = (i + 1) & (table.size() - 1);
+    }
+    table[i] = i32(keys.size());
+    keys.push_back(k);
+    return table[i];
+  }
+};
+
 struct WalkHead {
   i32 a = -1, b = -1;
   int axis = 0;
@@ -1887,28 +1913,10 @@ struct WalkFace {
 struct WalkLog {
   std::vector<WalkHead> heads;
   std::vector<WalkFace> faces;
-  std::vector<std::pair<uint64_t, i32>> index;  // (key, head), looked up linearly by key
-  std::vector<i32> table;                        // open-addressed over index
+  WalkKeys index;
   i32 at(i32 a, i32 b, int axis, int sign) {
-    const uint64_t k = orc_acc_key(a, b, axis, sign);
-    if (table.empty()) table.assign(1024, -1);
-    if (index.size() * 2 >= table.size()) {
-      std::vector<i32> t(table.size() * 2, -1);
-      for (size_t e = 0; e < index.size(); ++e) {
-        size_t i = size_t(orc_mix64(index[e].first)) & (t.size() - 1);
-        while (t[i] >= 0) i = (i + 1) & (t.size() - 1);
-        t[i] = i32(e);
-      }
-      table.swap(t);
-    }
-    size_t i = size_t(orc_mix64(k)) & (table.size() - 1);
-    while (table[i] >= 0) {
-      if (index[size_t(table[i])].first == k) return index[size_t(table[i])].second;
-      i = (i + 1) & (table.size() - 1);
-    }
-    const i32 s = i32(heads.size());
-    table[i] = i32(index.size());
-    index.push_back({k, s});
+    const i32 s = index(orc_acc_key(a, b, axis, sign));
+    if (s < i32(heads.size())) return s;
     WalkHead H;
     H.a = b >= 0 ? std::min(a, b) : a;
     H.b = b >= 0 ? std::max(a, b) : b;
@@ -2122,24 +2130,17 @@ extern "C" int32_t orc_extract_structure(const svxg_voxel_box* box, const uint8_
     std::vector<WalkFace> faces;
   };
   std::vector<Merged> out;
-  std::vector<std::pair<uint64_t, i32>> mindex;
   std::vector<i32> to(fine.heads.size(), -1);
   std::vector<uint8_t> flip(fine.heads.size(), 0);
   {
-    std::vector<std::pair<uint64_t, i32>> sorted;  // (key, merged) for lookups
+    WalkKeys mindex;
     for (size_t f = 0; f < fine.heads.size(); ++f) {
       const WalkHead& F = fine.heads[f];
       const i32 a = fnode[size_t(F.a)];
       const i32 b = F.b >= 0 ? fnode[size_t(F.b)] : -1;
       if (b >= 0 && a == b) continue;
-      const uint64_t k = orc_acc_key(a, b, F.axis, F.sign);
-      auto it = std::lower_bound(sorted.begin(), sorted.end(), std::make_pair(k, i32(-1)));
-      i32 mi;
-      if (it != sorted.end() && it->first == k) {
-        mi = it->second;
-      } else {
-        mi = i32(out.size());
-        sorted.insert(it, {k, mi});
+      const i32 mi = mindex(orc_acc_key(a, b, F.axis, F.sign));
+      if (mi == i32(out.size())) {
         Merged M;
         M.H = F;
         M.H.a = b >= 0 ? std::min(a, b) : a;

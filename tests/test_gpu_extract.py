@@ -131,3 +131,27 @@ def test_large_block(mats, ctx):
     o, d = _both(sc, fr, _seed(sc), mats, ctx)
     assert o["cell"] == 8 and len(o["members"]) > 10000
     _same(o, d)
+
+
+@pytest.mark.parametrize("fault", ["index", "air", "chunk"])
+def test_bad_inputs_are_refused(fault, mats, ctx):
+    """The device checks the inputs before it walks (a fragment index in range, on solid unanchored
+    voxels, inside the chunk of its first voxel) and refuses them with an error, as the host entries do."""
+    from opus_destruct_2_amd.stress import SvxgError
+    sc = W.scene(shape=(40, 20, 10), seed=13)
+    fr = W.frags_of(sc, mats)
+    frag = sc["frag"].copy()
+    if fault == "index":
+        frag[4, 5, 5] = len(fr) + 3
+    elif fault == "air":
+        z, y, x = np.argwhere(sc["vox"] == 0)[0]
+        frag[z, y, x] = 0
+    else:
+        # a fragment just below x = 32 given a voxel just above it
+        f = int(frag[4, 5, 31])
+        frag[4, 5, 32] = f
+    with pytest.raises(SvxgError):
+        voxels.extract_structure(sc["vox"], frag, sc["lo"], sc["h"], fr, _seed(sc), mats, ctx=ctx)
+    # the context goes on working
+    o, d = _both(sc, fr, _seed(sc), mats, ctx)
+    _same(o, d)
