@@ -235,8 +235,14 @@ static int upload_into(const DevBuf& d, const std::vector<T>& h, hipStream_t s) 
   return SVXG_OK;
 }
 
-// the staging lane of stream s: pinned host and device staging, at least `bytes` each
-static int staging(svxg_ctx* c, hipStream_t s, size_t bytes, svxg_ctx::Lane** out) {
+// The staging lane of a stream: pinned host and device buffers, used as a ring. A region is taken
+// from the lane's head; a copy queued on the stream that reads it is followed by the lane's event,
+// and the head returns to 0 once that event is seen complete (or the stream was synchronised by
+// its user). A region is therefore never written by the host while a queued copy still reads it,
+// and uploads need not wait for their copies.
+static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static int stage_region(svxg_ctx* c, hipStream_t s, size_t bytes, svxg_ctx::Lane** out, size_t* off) {
   svxg_ctx::Lane* L = nullptr;
   {
     std::lock_guard<std::mutex> lk(c->lanes_mu);
@@ -248,20 +254,49 @@ static int staging(svxg_ctx* c, hipStream_t s, size_t bytes, svxg_ctx::Lane** ou
       L->s = s;
     }
   }
-  if (bytes > L->pinned_bytes) {
-    if (L->pinned) (void)hipHostFree(L->pinned);
-    L->pinned = nullptr;
-    L->pinned_bytes = 0;
-    const size_t want = std::max(bytes, size_t(16) << 20);
-    SVXG_HIP(hipHostMalloc(&L->pinned, want, hipHostMallocDefault));
-    L->pinned_bytes = want;
+  bytes = round256(std::max<size_t>(1, bytes));
+  if (L->pending && hipEventQuery(L->done) == hipSuccess) L->pending = false;
+  if (!L->pending) L->head = 0;
+  const size_t cap = std::min(L->pinned_bytes, L->dstage.bytes());
+  if (L->head + bytes > cap) {
+    // (wait for what the queued copies still read, then start from the front, grown if need be)
+    if (L->pending) SVXG_HIP(hipEventSynchronize(L->done));
+    L->pending = false;
+    L->head = 0;
+    if (bytes > L->pinned_bytes) {
+      if (L->pinned) (void)hipHostFree(L->pinned);
+      L->pinned = nullptr;
+      L->pinned_bytes = 0;
+      const size_t want = std::max(bytes, size_t(16) << 20);
+      SVXG_HIP(hipHostMalloc(&L->pinned, want, hipHostMallocDefault));
+      L->pinned_bytes = want;
+    }
+    if (bytes > L->dstage.bytes()) SVXG_TRY(L->dstage.make(c->arena.get(), std::max(bytes, size_t(16) << 20)));
   }
-  if (bytes > L->dstage.bytes()) SVXG_TRY(L->dstage.make(c->arena.get(), std::max(bytes, size_t(16) << 20)));
+  *off = L->head;
+  L->head += bytes;
   *out = L;
   return SVXG_OK;
 }
-
-static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+// the copies just queued on the lane's stream read its regions: mark them in use until they are done
+static int stage_queued(svxg_ctx::Lane* L) {
+  if (!L->done) SVXG_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
+  SVXG_HIP(hipEventRecord(L->done, L->s));
+  L->pending = true;
+  return SVXG_OK;
+}
+// the lane's stream was just synchronised: every region is free
+static void stage_idle(svxg_ctx::Lane* L) {
+  L->pending = false;
+  L->head = 0;
+}
+// a lane's resources, when its stream goes
+static void stage_free(svxg_ctx::Lane* L) {
+  if (L->pinned) (void)hipHostFree(L->pinned);
+  L->pinned = nullptr;
+  if (L->done) (void)hipEventDestroy(L->done);
+  L->done = nullptr;
+}
 
 // f(k) for k in [0, n) on up to 8 host threads (the memcpy of large batched uploads)
 template <typename F>
@@ -312,11 +347,14 @@ static size_t stage_round(const std::vector<StagePart>& parts, size_t k0, size_t
   return k1;
 }
 
-// Uploads of a batch (many small arrays): packed into the pinned staging buffer with a table of
-// their destinations, sent in one copy, spread by one kernel, one synchronisation a round trip
+// Uploads of a batch (many small arrays): packed into a region of the stream's staging lane with a
+// table of their destinations, sent in one copy, spread by one kernel. wait: the stream is then
+// synchronised (the caller may free what the stream's queued work uses); else flush() returns with
+// the copies queued (the host arrays may go at once, the region stays taken until they are done:
+// the caller keeps every device array its queued work uses until its next synchronisation)
 class Stager {
  public:
-  Stager(svxg_ctx* c, hipStream_t s) : c_(c), s_(s) {}
+  Stager(svxg_ctx* c, hipStream_t s, bool wait = true) : c_(c), s_(s), wait_(wait) {}
   template <typename T>
   int add(DevBuf& d, Arena* a, const std::vector<T>& h) {
     SVXG_TRY(d.make(a, std::max<size_t>(1, h.size()) * sizeof(T)));
@@ -343,9 +381,10 @@ class Stager {
   int send(const std::vector<StagePart>& parts, size_t k0, size_t k1, size_t data) {
     const size_t n = k1 - k0;
     svxg_ctx::Lane* L = nullptr;
-    SVXG_TRY(staging(c_, s_, data + round256(n * sizeof(BulkSeg)), &L));
-    char* hp = static_cast<char*>(L->pinned);
-    char* dp = L->dstage.as<char>();
+    size_t at = 0;
+    SVXG_TRY(stage_region(c_, s_, data + round256(n * sizeof(BulkSeg)), &L, &at));
+    char* hp = static_cast<char*>(L->pinned) + at;
+    char* dp = L->dstage.as<char>() + at;
     BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
     std::vector<size_t> offs(n);
     size_t off = 0;
@@ -358,11 +397,14 @@ class Stager {
     SVXG_HIP(hipMemcpyAsync(dp, hp, data + n * sizeof(BulkSeg), hipMemcpyHostToDevice, s_));
     launch_copy_many(s_, reinterpret_cast<const BulkSeg*>(dp + data), int(n));
     SVXG_HIP(hipGetLastError());
+    if (!wait_) return stage_queued(L);
     SVXG_HIP(hipStreamSynchronize(s_));
+    stage_idle(L);
     return SVXG_OK;
   }
   svxg_ctx* c_;
   hipStream_t s_;
+  bool wait_;
   std::vector<StagePart> jobs_;
 };
 
@@ -391,9 +433,10 @@ class Gatherer {
   int fetch(const std::vector<StagePart>& parts, size_t k0, size_t k1, size_t data) {
     const size_t n = k1 - k0;
     svxg_ctx::Lane* L = nullptr;
-    SVXG_TRY(staging(c_, s_, data + round256(n * sizeof(BulkSeg)), &L));
-    char* hp = static_cast<char*>(L->pinned);
-    char* dp = L->dstage.as<char>();
+    size_t at = 0;
+    SVXG_TRY(stage_region(c_, s_, data + round256(n * sizeof(BulkSeg)), &L, &at));
+    char* hp = static_cast<char*>(L->pinned) + at;
+    char* dp = L->dstage.as<char>() + at;
     BulkSeg* seg = reinterpret_cast<BulkSeg*>(hp + data);
     size_t off = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -405,6 +448,7 @@ class Gatherer {
     SVXG_HIP(hipGetLastError());
     SVXG_HIP(hipMemcpyAsync(hp, dp, data, hipMemcpyDeviceToHost, s_));
     SVXG_HIP(hipStreamSynchronize(s_));
+    stage_idle(L);
     off = 0;
     for (size_t k = k0; k < k1; ++k) {
       std::memcpy(parts[k].dst, hp + off, parts[k].bytes);
@@ -418,7 +462,8 @@ class Gatherer {
 };
 
 // Fills and FP64 -> FP32 copies of a batch setup: small ones gathered into one launch each, large
-// ones (>= 4 MB) issued on their own; flush() before any launch reads what they write
+// ones (>= 4 MB) issued on their own; flush() before any launch reads what they write (wait: as
+// Stager's, the stream synchronised at the end; else the work is only queued)
 class BulkOps {
  public:
   explicit BulkOps(svxg_ctx* c) : c_(c) {}
@@ -440,24 +485,30 @@ class BulkOps {
     const int64_t per = int64_t(kPieceBytes / 8);
     for (int64_t o = 0; o < n; o += per) f32_.push_back(BulkSeg{dst + o, src + o, std::min(per, n - o)});
   }
-  int flush(hipStream_t s) {
+  int flush(hipStream_t s, bool wait = true) {
     for (const BulkSeg& g : big_zero_) SVXG_HIP(hipMemsetAsync(g.dst, 0, size_t(g.count), s));
     for (const BulkSeg& g : big_f32_)
       launch_to_f32(s, static_cast<const double*>(g.src), static_cast<float*>(g.dst), g.count);
+    svxg_ctx::Lane* L = nullptr;
     if (!zero_.empty() || !f32_.empty()) {
       const size_t bytes = (zero_.size() + f32_.size()) * sizeof(BulkSeg);
-      svxg_ctx::Lane* L = nullptr;
-      SVXG_TRY(staging(c_, s, bytes, &L));
-      std::memcpy(L->pinned, zero_.data(), zero_.size() * sizeof(BulkSeg));
-      std::memcpy(static_cast<char*>(L->pinned) + zero_.size() * sizeof(BulkSeg), f32_.data(),
-                  f32_.size() * sizeof(BulkSeg));
-      SVXG_HIP(hipMemcpyAsync(L->dstage.as<void>(), L->pinned, bytes, hipMemcpyHostToDevice, s));
-      const BulkSeg* d = L->dstage.as<BulkSeg>();
+      size_t at = 0;
+      SVXG_TRY(stage_region(c_, s, bytes, &L, &at));
+      char* hp = static_cast<char*>(L->pinned) + at;
+      std::memcpy(hp, zero_.data(), zero_.size() * sizeof(BulkSeg));
+      std::memcpy(hp + zero_.size() * sizeof(BulkSeg), f32_.data(), f32_.size() * sizeof(BulkSeg));
+      SVXG_HIP(hipMemcpyAsync(L->dstage.as<char>() + at, hp, bytes, hipMemcpyHostToDevice, s));
+      const BulkSeg* d = reinterpret_cast<const BulkSeg*>(L->dstage.as<char>() + at);
       launch_zero_many(s, d, int(zero_.size()));
       launch_f32_many(s, d + zero_.size(), int(f32_.size()));
     }
     SVXG_HIP(hipGetLastError());
-    SVXG_HIP(hipStreamSynchronize(s));
+    if (wait) {
+      SVXG_HIP(hipStreamSynchronize(s));
+      if (L) stage_idle(L);
+    } else if (L) {
+      SVXG_TRY(stage_queued(L));
+    }
     zero_.clear();
     f32_.clear();
     big_zero_.clear();
@@ -472,6 +523,14 @@ class BulkOps {
   std::vector<BulkSeg> zero_, f32_, big_zero_, big_f32_;
 };
 
+// the same through the stream's staging lane, queued (Stager without waiting: the caller keeps d, and
+// whatever its queued work uses, until its next synchronisation of s)
+template <typename T>
+static int upload_queued(svxg_ctx* c, DevBuf& d, Arena* a, const std::vector<T>& h, hipStream_t s) {
+  Stager up(c, s, false);
+  SVXG_TRY(up.add(d, a, h));
+  return up.flush();
+}
 template <typename T>
 static int download(std::vector<T>& h, const DevBuf& d, size_t count, hipStream_t s) {
   h.resize(count);

@@ -157,6 +157,11 @@ struct svxg_ctx {
     void* pinned = nullptr;  // host, pinned
     size_t pinned_bytes = 0;
     svxg::DevBuf dstage;     // device
+    // regions [0, head) of both buffers are used by copies queued on s and not known to be done;
+    // `done` is recorded after the last of them (pending: recorded and not yet seen complete)
+    size_t head = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
   };
   std::mutex lanes_mu;
   std::vector<std::unique_ptr<Lane>> lanes;

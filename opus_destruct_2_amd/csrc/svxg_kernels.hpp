@@ -259,8 +259,10 @@ struct DenseDev {
 };
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count);
 // the same spread over many workgroups (graphs whose padded size is above kDenseOneBlocks tile rows;
-// max_nb: the largest graph's tile rows; dmax: a double a graph of scratch)
-constexpr int kDenseOneBlocks = 18;
+// max_nb: the largest graph's tile rows; dmax: a double a graph of scratch). One workgroup runs a
+// graph's tile products one memory round trip after another: 0.9 ms at 288 rows against about a
+// third of that spread, so only the smallest levels stay whole
+constexpr int kDenseOneBlocks = 4;
 void launch_dense_factor_wide(hipStream_t s, const DenseDev* jobs, int count, int max_nb, double* dmax);
 
 // FP32 copies
