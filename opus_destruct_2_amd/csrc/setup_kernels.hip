@@ -30,6 +30,48 @@ namespace {
 
 constexpr int kBondsPerWG = 7;  // 36 lanes a bond / a block
 
+// LDS writes of a wavefront visible to its other lanes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The lane forms of the prolongator and Galerkin fills take a wavefront an entry: its 64 lanes look
+// at 64 of the entry's candidate terms at once (a coupling, a restriction entry: whether it meets
+// the entry's column, by a lookup or a binary search each) and list the ones that do, in order, in
+// LDS; then lanes 0..35, one a block element, add the listed terms in that order - each element's
+// arithmetic that of the thread form, term for term. The terms' lookups overlap instead of following
+// one another.
+constexpr int kFillWaves = kWG / 64;
+// the listed terms of a wavefront's chunk: up to 64, by rank; a = the term's index, b = its partner
+// entry (or column)
+struct FillList {
+  int32_t a[kFillWaves][64];
+  int32_t b[kFillWaves][64];
+};
+// lane `lane` lists (a, b) when `hit`, at its rank among the wave's hits; returns the hit count
+__device__ __forceinline__ int fill_list(FillList& L, int wv, int lane, bool hit, int a, int b) {
+  const unsigned long long mask = __ballot(hit);
+  if (hit) {
+    const int k = int(__popcll(mask & ((1ull << lane) - 1ull)));
+    L.a[wv][k] = a;
+    L.b[wv][k] = b;
+  }
+  wave_sync();
+  return int(__popcll(mask));
+}
+// the first index in [lo, hi) of sorted cols holding c, or -1
+__device__ __forceinline__ int find_sorted(const int32_t* __restrict__ cols, int lo, int hi, int c) {
+  const int end = hi;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cols[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && cols[lo] == c) ? lo : -1;
+}
+
 // the bond's local stiffness in the global frame, element (R, C) (stress.cpp:59-79)
 __device__ double bond_D(const svxg_bond& b, const svxg_material& A, const svxg_material& B, int R, int C) {
   const bool t = R < 3 && C < 3, rr = R >= 3 && C >= 3;
@@ -84,41 +126,6 @@ __device__ void bond_blocks_lds(const svxg_bond& b, const double* nodec, const s
   } else {
     sB[e] = 0.0;
   }
-}
-
-// the 6x6 Gauss-Jordan inverse with partial pivoting of blk6::inv6 (amg.cpp:105-135)
-__device__ bool inv6(const double* A, double* out) {
-  double W[6][12];
-  double big = 0.0;
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 12; ++c) {
-      W[r][c] = c < 6 ? A[r * 6 + c] : (c - 6 == r ? 1.0 : 0.0);
-      if (c < 6) big = fmax(big, fabs(W[r][c]));
-    }
-  if (big == 0.0) return false;
-  for (int c = 0; c < 6; ++c) {
-    int best = c;
-    for (int r = c + 1; r < 6; ++r)
-      if (fabs(W[r][c]) > fabs(W[best][c])) best = r;
-    if (fabs(W[best][c]) <= 1e-15 * big) return false;
-    if (best != c)
-      for (int k = 0; k < 12; ++k) {
-        const double t = W[c][k];
-        W[c][k] = W[best][k];
-        W[best][k] = t;
-      }
-    const double d = 1.0 / W[c][c];
-    for (int k = 0; k < 12; ++k) W[c][k] *= d;
-    for (int r = 0; r < 6; ++r) {
-      if (r == c) continue;
-      const double m = W[r][c];
-      if (m == 0.0) continue;
-      for (int k = 0; k < 12; ++k) W[r][k] -= m * W[c][k];
-    }
-  }
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 6; ++c) out[r * 6 + c] = W[r][6 + c];
-  return true;
 }
 
 // Frobenius norm summed in an order symmetric under transposition (diagonal, then each pair
@@ -370,35 +377,86 @@ __global__ void k_retire_rows(const AssemblyDev* __restrict__ J, const int32_t* 
   a.K[36 * size_t(a.rowptr[i]) + e] = (e % 7 == 0) ? *a.id_scale : 0.0;
 }
 
-__global__ void k_inv_diag(const DiagDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
-  if (int(blockIdx.x) >= ntiles) return;
-  const Tile tl = tiles[blockIdx.x];
-  const DiagDev& a = J[tl.g];
-  if (int(threadIdx.x) >= tl.nrows) return;
-  const int i = tl.row0 + threadIdx.x;
-  const double* D = a.A + 36 * size_t(a.rowptr[i]);
-  double* out = a.Dinv + 36 * size_t(i);
-  double Dl[36], O[36];
-  for (int q = 0; q < 36; ++q) Dl[q] = D[q];
-  if (inv6(Dl, O)) {
-    for (int q = 0; q < 36; ++q) out[q] = O[q];
-    return;
+// The 6x6 Gauss-Jordan inverse with partial pivoting of blk6::inv6 (amg.cpp:105-135), a wavefront a
+// block: W = [A | I] held in LDS, lane l owning elements l and l + 64 of W's 72 (row-major, 12 a
+// row); every element sees the operations blk6::inv6 applies to it, in its order (pivot search, row
+// swap, row scaling, elimination; rows whose multiplier is 0 left alone), so the inverse has its
+// bits. Returns its verdict (false: zero matrix, or a pivot at most 1e-15 of the largest magnitude;
+// the same on every lane); on success the lanes write the inverse to out.
+__device__ bool inv6_wave(const double* A, double* W, int lane, double* __restrict__ out) {
+  // (element x0 = lane of every lane, x1 = lane + 64 of lanes 0..7)
+  const int x0 = lane, x1 = lane + 64;
+  const bool two = x1 < 72;
+  const int r0 = x0 / 12, k0 = x0 % 12, r1 = x1 / 12, k1 = x1 % 12;
+  W[x0] = k0 < 6 ? A[r0 * 6 + k0] : (k0 - 6 == r0 ? 1.0 : 0.0);
+  if (two) W[x1] = k1 < 6 ? A[r1 * 6 + k1] : (k1 - 6 == r1 ? 1.0 : 0.0);
+  wave_sync();
+  double big = 0.0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) big = fmax(big, fabs(W[r * 12 + c]));
+  if (big == 0.0) return false;
+  for (int c = 0; c < 6; ++c) {
+    int best = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(W[r * 12 + c]) > fabs(W[best * 12 + c])) best = r;
+    if (fabs(W[best * 12 + c]) <= 1e-15 * big) return false;
+    if (best != c) {
+      const double v0 = r0 == c ? W[best * 12 + k0] : (r0 == best ? W[c * 12 + k0] : W[x0]);
+      const double v1 = two ? (r1 == c ? W[best * 12 + k1] : (r1 == best ? W[c * 12 + k1] : W[x1])) : 0.0;
+      wave_sync();
+      W[x0] = v0;
+      if (two) W[x1] = v1;
+      wave_sync();
+    }
+    const double d = 1.0 / W[c * 12 + c];
+    wave_sync();
+    if (r0 == c) W[x0] *= d;
+    if (two && r1 == c) W[x1] *= d;
+    wave_sync();
+    const double m0 = W[r0 * 12 + c], p0 = W[c * 12 + k0];
+    const double m1 = two ? W[r1 * 12 + c] : 0.0, p1 = two ? W[c * 12 + k1] : 0.0;
+    wave_sync();
+    if (r0 != c && m0 != 0.0) W[x0] -= m0 * p0;
+    if (two && r1 != c && m1 != 0.0) W[x1] -= m1 * p1;
+    wave_sync();
   }
+  if (k0 >= 6) out[r0 * 6 + (k0 - 6)] = W[x0];
+  if (two && k1 >= 6) out[r1 * 6 + (k1 - 6)] = W[x1];
+  return true;
+}
+
+// a level's inverse diagonal blocks (amg.cpp:227-244): a wavefront a block, four a workgroup, 64
+// workgroups a row tile; a singular block is regularised by 1e-9 of its diagonal's absolute sum
+// (mode 0) or replaced by the identity scale's inverse (mode 1: block-Jacobi's retired rows)
+__global__ void __launch_bounds__(kWG) k_inv_diag(const DiagDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                  int ntiles) {
+  const int tile = int(blockIdx.x) >> 6, part = int(blockIdx.x) & 63;
+  if (tile >= ntiles) return;
+  const Tile tl = tiles[tile];
+  const DiagDev& a = J[tl.g];
+  const int w = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+  const int rl = 4 * part + w;
+  if (rl >= tl.nrows) return;  // (a whole wavefront)
+  const int i = tl.row0 + rl;
+  __shared__ double Dl[kWG / 64][36], W[kWG / 64][72];
+  if (lane < 36) Dl[w][lane] = a.A[36 * size_t(a.rowptr[i]) + size_t(lane)];
+  wave_sync();
+  double* out = a.Dinv + 36 * size_t(i);
+  if (inv6_wave(Dl[w], W[w], lane, out)) return;
   if (a.mode == 1) {
     const double s = 1.0 / *a.id_scale;
-    for (int q = 0; q < 36; ++q) out[q] = (q % 7 == 0) ? s : 0.0;
+    if (lane < 36) out[lane] = (lane % 7 == 0) ? s : 0.0;
     return;
   }
   double tr = 0.0;
-  for (int q = 0; q < 6; ++q) tr += fabs(Dl[q * 7]);
+  for (int q = 0; q < 6; ++q) tr += fabs(Dl[w][q * 7]);
   const double eps = tr > 0 ? 1e-9 * tr : 1.0;
-  for (int q = 0; q < 6; ++q) Dl[q * 7] += eps;
-  if (inv6(Dl, O)) {
-    for (int q = 0; q < 36; ++q) out[q] = O[q];
-  } else {
-    for (int q = 0; q < 36; ++q) out[q] = 0.0;
-    atomicAdd(a.fail, 1);
-  }
+  wave_sync();
+  if (lane < 6) Dl[w][lane * 7] += eps;
+  wave_sync();
+  if (inv6_wave(Dl[w], W[w], lane, out)) return;
+  if (lane < 36) out[lane] = 0.0;
+  if (lane == 0) atomicAdd(a.fail, 1);
 }
 
 // ---- aggregation ------------------------------------------------------------------------------------
@@ -436,12 +494,21 @@ __global__ void k_strong(const CoarsenDev* __restrict__ J, const Tile* __restric
   c.state[i] = (k1 - k0 <= 1) ? MIS_EXC : MIS_UND;
 }
 
-__global__ void k_mis_t1(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+// Round r counts its undecided rows into counts[mis_slot(r)], the two slots taken in turn: a graph
+// whose previous round left nothing undecided skips the round (a round then changes nothing), its
+// slot set to 0, so the host may queue many rounds between two looks
+__device__ __forceinline__ int mis_slot(int round) { return (round & 1) ? 6 : 0; }
+__device__ __forceinline__ bool mis_settled(const CoarsenDev& c, int round) {
+  return round > 0 && c.counts[mis_slot(round - 1)] == 0;
+}
+
+__global__ void k_mis_t1(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles, int round) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
-  if (tl.row0 == 0 && threadIdx.x == 0) c.counts[0] = 0;  // (this round's undecided rows, counted below)
-  if (int(threadIdx.x) >= tl.nrows) return;
+  const bool settled = mis_settled(c, round);
+  if (tl.row0 == 0 && threadIdx.x == 0) c.counts[mis_slot(round)] = 0;  // (this round's undecided rows)
+  if (settled || int(threadIdx.x) >= tl.nrows) return;
   const int i = tl.row0 + threadIdx.x;
   uint64_t m = mis_tuple(c.state[i], i);
   for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k)
@@ -453,39 +520,35 @@ __global__ void k_mis_t1(const CoarsenDev* __restrict__ J, const Tile* __restric
   c.t1[i] = m;
 }
 
-__global__ void k_mis_t2(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
+// the largest first-hop tuple over a row's strong neighbourhood (the distance-2 maximum), and with it
+// the row's decision: an undecided row whose own tuple is that maximum becomes a root, one next to a
+// root (the maximum's tag) is excluded; the undecided rows left are counted (a row's decision reads
+// only its own distance-2 maximum, so it is taken in the same launch)
+__global__ void __launch_bounds__(kWG) k_mis_t2(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                int ntiles, int round) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = tiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
-  if (int(threadIdx.x) >= tl.nrows) return;
-  const int i = tl.row0 + threadIdx.x;
-  uint64_t m = c.t1[i];
-  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k)
-    if (c.strong[k]) {
-      const uint64_t t = c.t1[c.col[k]];
-      m = t > m ? t : m;
-    }
-  c.t2[i] = m;
-}
-
-__global__ void k_mis_update(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
-  if (int(blockIdx.x) >= ntiles) return;
-  const Tile tl = tiles[blockIdx.x];
-  const CoarsenDev& c = J[tl.g];
+  if (mis_settled(c, round)) return;
   __shared__ int und;
   if (threadIdx.x == 0) und = 0;
   __syncthreads();
   if (int(threadIdx.x) < tl.nrows) {
     const int i = tl.row0 + threadIdx.x;
+    uint64_t m = c.t1[i];
+    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k)
+      if (c.strong[k]) {
+        const uint64_t t = c.t1[c.col[k]];
+        m = t > m ? t : m;
+      }
     if (c.state[i] == MIS_UND) {
-      const uint64_t m = c.t2[i];
       if (m == mis_tuple(MIS_UND, i)) c.state[i] = MIS_ROOT;
       else if ((m >> 62) == 3ull) c.state[i] = MIS_EXC;
       else atomicAdd(&und, 1);
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && und) atomicAdd(&c.counts[0], und);
+  if (threadIdx.x == 0 && und) atomicAdd(&c.counts[mis_slot(round)], und);
 }
 
 // pass 1: a root is its own aggregate's anchor; its strong neighbours join it
@@ -711,39 +774,59 @@ __global__ void k_p_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __
 }
 
 // P's blocks: (J == agg(i) ? P_rigid,i : 0) - w sum over couplings k of row i with agg(col k) = J
-// of D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order. 36 lanes an entry
-// (lane = element (r, c)), seven entries a workgroup (entry tiles of kFillEntries)
+// of D_i^-1 (A_k P_rigid,col k), each element in the reference's summation order. A wavefront an
+// entry (lanes 0..35: element (r, c)), four entries a workgroup (entry tiles of kFillEntries); D_i^-1
+// and each listed coupling's block and offset are staged in LDS, a lane an element, and every lane
+// reads the whole block there
 __global__ void __launch_bounds__(kWG) k_p_fill_lanes(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles,
                                                 int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = etiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
-  const int el = int(threadIdx.x) / 36, q = int(threadIdx.x) % 36;
-  if (el >= tl.nrows) return;
-  const int e = tl.row0 + el;
+  const int wv = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+  if (wv >= tl.nrows) return;  // (a whole wavefront)
+  __shared__ FillList L;
+  __shared__ double Dl[kFillWaves][36], Al[kFillWaves][40];  // (Al[36..38]: the coupled row's offset)
+  const int e = tl.row0 + wv;
   const int i = c.rid[e];
   const int Jc = c.Pcol[e];
-  const int r = q / 6, col = q % 6;
-  const double* Di = c.Dinv + 36 * size_t(i);
+  const int q = lane < 36 ? lane : 0, r = q / 6, col = q % 6;
   double acc = (Jc == c.agg[i]) ? rigid_elem(c.off + 3 * size_t(i), r, col) : 0.0;
   if (c.omega != 0.0) {
-    for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
-      const int j = c.col[k];
-      if (c.agg[j] != Jc) continue;
-      const double* A = c.A + 36 * size_t(k);
-      const double* offj = c.off + 3 * size_t(j);
-      double T[6];
-      for (int rr = 0; rr < 6; ++rr) {
-        double s2 = 0.0;
-        for (int p = 0; p < 6; ++p) s2 += A[rr * 6 + p] * rigid_elem(offj, p, col);
-        T[rr] = s2;
+    if (lane < 36) Dl[wv][lane] = c.Dinv[36 * size_t(i) + size_t(lane)];
+    const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
+    for (int kc = k0; kc < k1; kc += 64) {
+      const int k = kc + lane;
+      int j = 0;
+      bool hit = false;
+      if (k < k1) {
+        j = c.col[k];
+        hit = c.agg[j] == Jc;
       }
-      double s2 = 0.0;
-      for (int p = 0; p < 6; ++p) s2 += Di[r * 6 + p] * T[p];
-      acc += -c.omega * s2;
+      const int nh = fill_list(L, wv, lane, hit, k, j);
+      for (int t = 0; t < nh; ++t) {
+        if (lane < 36) Al[wv][lane] = c.A[36 * size_t(L.a[wv][t]) + size_t(lane)];
+        else if (lane < 39) Al[wv][lane] = c.off[3 * size_t(L.b[wv][t]) + size_t(lane - 36)];
+        wave_sync();
+        if (lane < 36) {
+          const double* A = Al[wv];
+          const double* offj = Al[wv] + 36;
+          const double* Di = Dl[wv];
+          double T[6];
+          for (int rr = 0; rr < 6; ++rr) {
+            double s2 = 0.0;
+            for (int p = 0; p < 6; ++p) s2 += A[rr * 6 + p] * rigid_elem(offj, p, col);
+            T[rr] = s2;
+          }
+          double s2 = 0.0;
+          for (int p = 0; p < 6; ++p) s2 += Di[r * 6 + p] * T[p];
+          acc += -c.omega * s2;
+        }
+        wave_sync();  // (the block is read before the next one overwrites it)
+      }
     }
   }
-  c.P[36 * size_t(e) + size_t(q)] = acc;
+  if (lane < 36) c.P[36 * size_t(e) + size_t(q)] = acc;
 }
 
 // restriction lists: P's entries by coarse column, fine rows ascending
@@ -837,18 +920,38 @@ __global__ void k_ap_cand_count(const CoarsenDev* __restrict__ J, const Tile* __
   c.cptr[i] = n;
 }
 
-// A P's candidate columns of row i (unsorted): the P columns of each coupled row
-__global__ void k_ap_cand_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
-  if (int(blockIdx.x) >= ntiles) return;
-  const Tile tl = tiles[blockIdx.x];
+// A P's candidate columns of row i (unsorted): the P columns of each coupled row, in coupling order.
+// A wavefront a row (four rows a workgroup, 64 workgroups a row tile): each lane copies one coupled
+// row's P columns to its offset (a wavefront prefix sum of their counts)
+__global__ void __launch_bounds__(kWG) k_ap_cand_fill(const CoarsenDev* __restrict__ J, const Tile* __restrict__ tiles,
+                                                      int ntiles) {
+  const int tile = int(blockIdx.x) >> 6, part = int(blockIdx.x) & 63;
+  if (tile >= ntiles) return;
+  const Tile tl = tiles[tile];
   const CoarsenDev& c = J[tl.g];
-  if (int(threadIdx.x) >= tl.nrows) return;
-  const int i = tl.row0 + threadIdx.x;
+  const int w = int(threadIdx.x) >> 6, l = int(threadIdx.x) & 63;
+  const int rl = 4 * part + w;
+  if (rl >= tl.nrows) return;
+  const int i = tl.row0 + rl;
   int32_t* cd = c.cand + c.cptr[i];
-  int n = 0;
-  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
-    const int j = c.col[k];
-    for (int e = c.Prow[j]; e < c.Prow[j + 1]; ++e) cd[n++] = c.Pcol[e];
+  int base = 0;
+  const int k1 = c.rowptr[i + 1];
+  for (int k0 = c.rowptr[i]; k0 < k1; k0 += 64) {
+    const int k = k0 + l;
+    int x0 = 0, len = 0;
+    if (k < k1) {
+      const int j = c.col[k];
+      x0 = c.Prow[j];
+      len = c.Prow[j + 1] - x0;
+    }
+    int incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (l >= o) incl += t;
+    }
+    int32_t* dst = cd + base + incl - len;
+    for (int x = 0; x < len; ++x) dst[x] = c.Pcol[x0 + x];
+    base += __shfl(incl, 63, 64);
   }
 }
 
@@ -1121,36 +1224,49 @@ __global__ void k_ap_fill_thread(const CoarsenDev* __restrict__ J, const Tile* _
 }
 
 // (A P)_iJ: the sum over couplings k of row i (in order) of A_k P_e, e the entry of column J in P's
-// row col k (P rows ascending: a binary search). 36 lanes an entry (lane = element), seven entries a
-// workgroup
+// row col k (P rows ascending: a binary search). A wavefront an entry (lanes 0..35: the elements),
+// four entries a workgroup
 __global__ void __launch_bounds__(kWG) k_ap_fill_lanes(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles,
                                                  int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = etiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
-  const int el = int(threadIdx.x) / 36, q = int(threadIdx.x) % 36;
-  if (el >= tl.nrows) return;
-  const int x = tl.row0 + el;
+  const int wv = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+  if (wv >= tl.nrows) return;
+  __shared__ FillList L;
+  __shared__ double Xl[kFillWaves][2][36];
+  const int x = tl.row0 + wv;
   const int i = c.rid[x];
   const int Jc = c.APcol[x];
-  const int r = q / 6, col = q % 6;
+  const int q = lane < 36 ? lane : 0, r = q / 6, col = q % 6;
   double acc = 0.0;
-  for (int k = c.rowptr[i]; k < c.rowptr[i + 1]; ++k) {
-    const int j = c.col[k];
-    int lo = c.Prow[j], hi = c.Prow[j + 1];
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (c.Pcol[mid] < Jc) lo = mid + 1;
-      else hi = mid;
+  const int k0 = c.rowptr[i], k1 = c.rowptr[i + 1];
+  for (int kc = k0; kc < k1; kc += 64) {
+    const int k = kc + lane;
+    int pe = -1;
+    if (k < k1) {
+      const int j = c.col[k];
+      pe = find_sorted(c.Pcol, c.Prow[j], c.Prow[j + 1], Jc);
     }
-    if (lo >= c.Prow[j + 1] || c.Pcol[lo] != Jc) continue;
-    const double* A = c.A + 36 * size_t(k) + 6 * r;
-    const double* P = c.P + 36 * size_t(lo) + col;
-    double s2 = 0.0;
-    for (int t = 0; t < 6; ++t) s2 += A[t] * P[6 * t];
-    acc = acc + s2;
+    const int nh = fill_list(L, wv, lane, pe >= 0, k, pe);
+    for (int t = 0; t < nh; ++t) {
+      // (the coupling's A block and P entry staged in LDS, an element a lane)
+      if (lane < 36) {
+        Xl[wv][0][lane] = c.A[36 * size_t(L.a[wv][t]) + size_t(lane)];
+        Xl[wv][1][lane] = c.P[36 * size_t(L.b[wv][t]) + size_t(lane)];
+      }
+      wave_sync();
+      if (lane < 36) {
+        const double* A = Xl[wv][0] + 6 * r;
+        const double* P = Xl[wv][1] + col;
+        double s2 = 0.0;
+        for (int u = 0; u < 6; ++u) s2 += A[u] * P[6 * u];
+        acc = acc + s2;
+      }
+      wave_sync();
+    }
   }
-  c.AP[36 * size_t(x) + size_t(q)] = acc;
+  if (lane < 36) c.AP[36 * size_t(x) + size_t(q)] = acc;
 }
 
 // (P^T A P)_IJ, one entry a thread: the sum over restriction entries (i, e) of I, in order, of
@@ -1188,35 +1304,50 @@ __global__ void k_c_fill_thread(const CoarsenDev* __restrict__ J, const Tile* __
 }
 
 // (P^T A P)_IJ: the sum over restriction entries (i, e) of I, in order, of P_e^T (A P)_iJ; times
-// the coarse scale (unsmoothed levels). 36 lanes an entry (lane = element), seven entries a workgroup
+// the coarse scale (unsmoothed levels). A wavefront an entry (lanes 0..35: the elements), four
+// entries a workgroup
 __global__ void __launch_bounds__(kWG) k_c_fill_lanes(const CoarsenDev* __restrict__ J, const Tile* __restrict__ etiles,
                                                 int ntiles) {
   if (int(blockIdx.x) >= ntiles) return;
   const Tile tl = etiles[blockIdx.x];
   const CoarsenDev& c = J[tl.g];
-  const int el = int(threadIdx.x) / 36, q = int(threadIdx.x) % 36;
-  if (el >= tl.nrows) return;
-  const int y = tl.row0 + el;
+  const int wv = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+  if (wv >= tl.nrows) return;
+  __shared__ FillList L;
+  __shared__ double Xl[kFillWaves][2][36];
+  const int y = tl.row0 + wv;
   const int I = c.rid[y];
   const int Jc = c.Ccol[y];
-  const int rr = q / 6, col = q % 6;
+  const int q = lane < 36 ? lane : 0, rr = q / 6, col = q % 6;
   double acc = 0.0;
-  for (int r = c.Rrow[I]; r < c.Rrow[I + 1]; ++r) {
-    const int i = c.Rfine[r];
-    int lo = c.AProw[i], hi = c.AProw[i + 1];
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (c.APcol[mid] < Jc) lo = mid + 1;
-      else hi = mid;
+  const int r0 = c.Rrow[I], r1 = c.Rrow[I + 1];
+  for (int rc = r0; rc < r1; rc += 64) {
+    const int r = rc + lane;
+    int lo = -1, ent = 0;
+    if (r < r1) {
+      const int i = c.Rfine[r];
+      lo = find_sorted(c.APcol, c.AProw[i], c.AProw[i + 1], Jc);
+      ent = c.Rent[r];
     }
-    if (lo >= c.AProw[i + 1] || c.APcol[lo] != Jc) continue;
-    const double* P = c.P + 36 * size_t(c.Rent[r]) + rr;
-    const double* X = c.AP + 36 * size_t(lo) + col;
-    double s2 = 0.0;
-    for (int t = 0; t < 6; ++t) s2 += P[6 * t] * X[6 * t];
-    acc = acc + s2;
+    const int nh = fill_list(L, wv, lane, lo >= 0, ent, lo);
+    for (int t = 0; t < nh; ++t) {
+      // (the restriction entry's P block and the A P block staged in LDS, an element a lane)
+      if (lane < 36) {
+        Xl[wv][0][lane] = c.P[36 * size_t(L.a[wv][t]) + size_t(lane)];
+        Xl[wv][1][lane] = c.AP[36 * size_t(L.b[wv][t]) + size_t(lane)];
+      }
+      wave_sync();
+      if (lane < 36) {
+        const double* P = Xl[wv][0] + rr;
+        const double* X = Xl[wv][1] + col;
+        double s2 = 0.0;
+        for (int u = 0; u < 6; ++u) s2 += P[6 * u] * X[6 * u];
+        acc = acc + s2;
+      }
+      wave_sync();
+    }
   }
-  c.C[36 * size_t(y) + size_t(q)] = c.coarse_scale != 1.0 ? acc * c.coarse_scale : acc;
+  if (lane < 36) c.C[36 * size_t(y) + size_t(q)] = c.coarse_scale != 1.0 ? acc * c.coarse_scale : acc;
 }
 
 // ---- power iteration (6 lanes a row, kTileRows-row tiles) ------------------------------------------
@@ -1238,8 +1369,50 @@ __global__ void __launch_bounds__(kWG) k_power_init(const PowerDev* __restrict__
   p.v[0][e] = 1.0 + 0.001 * double(e % 7);
 }
 
+// (A x)_q of `row` over a tile's rows [row0, row0 + nrows) of an FP64 BSR matrix, a block a pair of
+// lanes: the tile's blocks are contiguous, 128 a pass; a lane pair loads one block (three rows each) and
+// the six x values of its column once, and writes the block's row sums (row q . x, left to right) to
+// LDS; lane q of each row then adds its row's sums in block order - the row walk's sum, bit for bit.
+// Every lane of the workgroup calls it (it synchronises the workgroup).
+__device__ __forceinline__ double lpb_rows64(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                             const double* __restrict__ A, const double* __restrict__ x, int row0,
+                                             int nrows, bool on, int row, int q) {
+  constexpr int PER = kWG / 2;
+  __shared__ double yb[6 * PER];
+  const int k0 = rowptr[row0], k1 = rowptr[row0 + nrows];
+  int rb0 = 0, rb1 = 0;
+  if (on) {
+    rb0 = rowptr[row];
+    rb1 = rowptr[row + 1];
+  }
+  const int lb = int(threadIdx.x) >> 1, q0 = (int(threadIdx.x) & 1) * 3;
+  double acc = 0.0;
+  for (int c = k0; c < k1; c += PER) {
+    const int kb = c + lb;
+    if (kb < k1) {
+      const double* a = A + 36 * size_t(kb) + 6 * q0;
+      const double* v = x + 6 * size_t(col[kb]);
+      double av[18], xv[6];
+      for (int u = 0; u < 18; ++u) av[u] = a[u];
+      for (int u = 0; u < 6; ++u) xv[u] = v[u];
+      for (int r = 0; r < 3; ++r) {
+        double s = 0.0;
+        for (int u = 0; u < 6; ++u) s += av[6 * r + u] * xv[u];
+        yb[6 * lb + q0 + r] = s;
+      }
+    }
+    __syncthreads();
+    if (on) {
+      const int e0 = max(rb0, c), e1 = min(rb1, c + PER);
+      for (int k = e0; k < e1; ++k) acc += yb[6 * (k - c) + q];
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
 // v_{s+1} = D^-1 A (v_s / |v_s|), partial |v_{s+1}|^2 (S six-lane sub-groups a row: S = 10 serves
-// the long rows of coarse levels a wavefront each)
+// the long rows of coarse levels a wavefront each; S = 1 takes the products a block a lane pair)
 template <int S>
 __global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
                                                     int ntiles, int step, double* __restrict__ partial) {
@@ -1255,7 +1428,9 @@ __global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__
   const bool on = rl >= 0 && rl < tl.nrows;
   const int row = tl.row0 + (on ? rl : 0);
   double w = 0.0;
-  if (on) {
+  if constexpr (S == 1) {
+    w = lpb_rows64(p.rowptr, p.col, p.A, in, tl.row0, tl.nrows, on, row, q);
+  } else if (on) {
     for (int k = p.rowptr[row] + sub; k < p.rowptr[row + 1]; k += S) {
       const double* a = p.A + 36 * size_t(k) + 6 * q;
       const double* x = in + 6 * size_t(p.col[k]);
@@ -1541,11 +1716,54 @@ __global__ void __launch_bounds__(kDenseWG) k_dense_factor(const DenseDev* __res
 // the one-workgroup kernel's; L^-1's sums are grouped differently (same terms, a fixed order), so a
 // graph's inverse depends only on its size class, never on its batch.
 
-__global__ void __launch_bounds__(kDenseWG) k_dense_prep(const DenseDev* __restrict__ J, int count,
-                                                         double* __restrict__ dmax) {
-  if (int(blockIdx.x) >= count) return;
-  const double dm = dense_prep(J[blockIdx.x]);
-  if (threadIdx.x == 0) dmax[blockIdx.x] = dm;
+// dense_prep spread: kDensePrepParts workgroups a graph, part p owning the block rows i = p mod parts
+// (their six matrix rows) and the padding rows r = m + p mod parts. First the part's rows of L are
+// zeroed (identity on padding rows), W's likewise zeroed, and the part's block rows scattered in;
+// then (after the launch) the lower triangle of the part's rows is symmetrised, 0.5 (a_ij + a_ji),
+// and max |a_ii| kept by an atomic maximum on its bits (magnitudes order as their bits): the same
+// values dense_prep leaves.
+constexpr int kDensePrepParts = 16;
+
+__global__ void __launch_bounds__(kWG) k_dense_scatter(const DenseDev* __restrict__ J, double* __restrict__ dmax) {
+  const DenseDev& d = J[blockIdx.y];
+  const int part = int(blockIdx.x), m = d.m, mp = d.mp;
+  double* L = d.L;
+  double* W = d.W;
+  if (part == 0 && threadIdx.x == 0) dmax[blockIdx.y] = 0.0;
+  auto clear = [&](int r) {
+    for (int c = int(threadIdx.x); c < mp; c += kWG) {
+      L[size_t(r) * size_t(mp) + size_t(c)] = (r == c && r >= m) ? 1.0 : 0.0;
+      W[size_t(r) * size_t(mp) + size_t(c)] = 0.0;
+    }
+  };
+  for (int i = part; i < d.n; i += kDensePrepParts)
+    for (int q = 0; q < 6; ++q) clear(6 * i + q);
+  for (int r = m + part; r < mp; r += kDensePrepParts) clear(r);
+  __syncthreads();
+  for (int i = part; i < d.n; i += kDensePrepParts) {
+    const int k0 = d.rowptr[i], k1 = d.rowptr[i + 1];
+    for (int t = int(threadIdx.x); t < 36 * (k1 - k0); t += kWG) {
+      const int k = k0 + t / 36, e = t % 36;
+      L[size_t(6 * i + e / 6) * size_t(mp) + size_t(6 * d.col[k] + e % 6)] = d.A[36 * size_t(k) + size_t(e)];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kWG) k_dense_sym(const DenseDev* __restrict__ J, double* __restrict__ dmax) {
+  const DenseDev& d = J[blockIdx.y];
+  const int part = int(blockIdx.x), mp = d.mp;
+  double* L = d.L;
+  double dm = 0.0;
+  for (int i = part; i < d.n; i += kDensePrepParts)
+    for (int q = 0; q < 6; ++q) {
+      const int r = 6 * i + q;
+      for (int c = int(threadIdx.x); c < r; c += kWG)
+        L[size_t(r) * size_t(mp) + size_t(c)] =
+            0.5 * (L[size_t(r) * size_t(mp) + size_t(c)] + L[size_t(c) * size_t(mp) + size_t(r)]);
+      if (threadIdx.x == 0) dm = fmax(dm, fabs(L[size_t(r) * size_t(mp) + size_t(r)]));
+    }
+  if (threadIdx.x == 0 && dm > 0.0)
+    atomicMax(reinterpret_cast<unsigned long long*>(dmax + blockIdx.y), __double_as_longlong(dm));
 }
 
 __global__ void __launch_bounds__(kDenseWG) k_dense_panel(const DenseDev* __restrict__ J, int count,
@@ -1768,7 +1986,7 @@ void launch_retire_rows(hipStream_t s, const AssemblyDev* job, const int32_t* ro
   if (count > 0) hipLaunchKernelGGL(k_retire_rows, dim3((36 * count + 255) / 256), dim3(256), 0, s, job, rows, count);
 }
 void launch_inv_diag(hipStream_t s, const DiagDev* jobs, const Tile* tiles, int ntiles) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_inv_diag, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  if (ntiles > 0) hipLaunchKernelGGL(k_inv_diag, dim3(64 * ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
 void launch_strength(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles <= 0) return;
@@ -1778,11 +1996,10 @@ void launch_strong(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_strong, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
-void launch_mis_round(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
+void launch_mis_round(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int round) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_mis_t1, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
-  hipLaunchKernelGGL(k_mis_t2, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
-  hipLaunchKernelGGL(k_mis_update, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  hipLaunchKernelGGL(k_mis_t1, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, round);
+  hipLaunchKernelGGL(k_mis_t2, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, round);
 }
 void launch_aggregate(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int count) {
   if (ntiles <= 0) return;
@@ -1835,7 +2052,7 @@ void launch_ap_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* til
   if (ntiles > 0) hipLaunchKernelGGL(k_ap_cand_count, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
 void launch_ap_cand_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles) {
-  if (ntiles > 0) hipLaunchKernelGGL(k_ap_cand_fill, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
+  if (ntiles > 0) hipLaunchKernelGGL(k_ap_cand_fill, dim3(64 * ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
 void launch_c_cand_count(hipStream_t s, const CoarsenDev* jobs, const Tile* ctiles, int nctiles) {
   if (nctiles > 0) hipLaunchKernelGGL(k_c_cand_count, dim3(nctiles), dim3(kWG), 0, s, jobs, ctiles, nctiles);
@@ -1882,7 +2099,8 @@ void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count) {
 }
 void launch_dense_factor_wide(hipStream_t s, const DenseDev* jobs, int count, int max_nb, double* dmax) {
   if (count <= 0 || max_nb <= 0) return;
-  hipLaunchKernelGGL(k_dense_prep, dim3(count), dim3(kDenseWG), 0, s, jobs, count, dmax);
+  hipLaunchKernelGGL(k_dense_scatter, dim3(kDensePrepParts, unsigned(count)), dim3(kWG), 0, s, jobs, dmax);
+  hipLaunchKernelGGL(k_dense_sym, dim3(kDensePrepParts, unsigned(count)), dim3(kWG), 0, s, jobs, dmax);
   for (int kb = 0; kb < max_nb; ++kb) {
     hipLaunchKernelGGL(k_dense_panel, dim3(count), dim3(kDenseWG), 0, s, jobs, count, dmax, kb);
     const int n = max_nb - kb - 1;

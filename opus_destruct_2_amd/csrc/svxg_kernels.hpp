@@ -195,7 +195,9 @@ struct CoarsenDev {
 // MIS-2 aggregation
 void launch_strength(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
 void launch_strong(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
-void launch_mis_round(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles);
+// one distance-2 MIS round; round r's undecided rows land in the job's counts[r & 1 ? 6 : 0], and a
+// round after one that left a graph nothing undecided skips that graph
+void launch_mis_round(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int round);
 void launch_aggregate(hipStream_t s, const CoarsenDev* jobs, const Tile* tiles, int ntiles, int count);
 // per-graph exclusive scans: scan_a / scan_b (n + 1 each) -> totals in counts[1], counts[2]
 void launch_scan_pair(hipStream_t s, const CoarsenDev* jobs, int count);
@@ -239,10 +241,11 @@ void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, i
 void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
 
 // the coarsest level's dense inverse (amg.cpp:535-578, then inverted)
-// the P, A P and P^T A P fills run 36 lanes an entry (one a block element, seven entries a
-// workgroup) when a setup has fewer than kFillLanesMax entries to fill (latency-bound), else one
-// thread an entry (throughput-bound); both give the same bits
-constexpr int kFillEntries = kWG / 36;
+// the P, A P and P^T A P fills run a wavefront an entry (its lanes find the entry's terms together,
+// 36 of them then add them up, one a block element; four entries a workgroup) when a setup has fewer
+// than kFillLanesMax entries to fill (latency-bound), else one thread an entry (throughput-bound);
+// both give the same bits
+constexpr int kFillEntries = kWG / 64;
 constexpr int64_t kFillLanesMax = 200000;
 
 struct DenseDev {
@@ -261,8 +264,9 @@ void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count);
 // the same spread over many workgroups (graphs whose padded size is above kDenseOneBlocks tile rows;
 // max_nb: the largest graph's tile rows; dmax: a double a graph of scratch). One workgroup runs a
 // graph's tile products one memory round trip after another: 0.9 ms at 288 rows against about a
-// third of that spread, so only the smallest levels stay whole
-constexpr int kDenseOneBlocks = 4;
+// third of that spread; the spread form takes every graph (kDenseOneBlocks = 0: the one-workgroup
+// kernel stays for a caller that wants a graph's inverse in one launch)
+constexpr int kDenseOneBlocks = 0;
 void launch_dense_factor_wide(hipStream_t s, const DenseDev* jobs, int count, int max_nb, double* dmax);
 
 // FP32 copies
