@@ -15,7 +15,7 @@ value: solved DOFs / (device time of the solve + wall time of the checks), input
 checks' phi and modes written to device arrays; the wall rate with the load, the solution and phi
 crossing PCIe is reported beside it (wall_ms_pcie_inclusive), never as value.
 roofline: the dominant kernel's algorithmic bytes per launch / its average launch time, measured
-live with HIP events on the library's stream during the timed steps.
+live with HIP events on the library's stream during as many solves again after the timed steps.
 cpu_baseline: the reference's own solver (compiled from its sources into oracle/_ref) on the same
 workload: the median of --cpu-runs runs of assemble + cold solve on every host thread of this
 process's share (setup and solve timed apart), with a one-thread figure beside it; cpu_baselines
@@ -82,10 +82,12 @@ class DeviceArray:
             self.p = None
 
 
-def run_path(g, mats, precision, steps, warmup, ctx, **kw):
+def run_path(g, mats, precision, steps, warmup, ctx, profile=False, **kw):
     """Steps of one solve from a cold start plus the bond checks on the resident solution into device
     arrays (phi, mode in HBM); the same checks into host arrays, with the copies over PCIe, are timed
-    apart (checks_host_ms)."""
+    apart (checks_host_ms). profile: after the timed steps, as many solves more with the library timing
+    its dominant kernels live (the roofline's figures; a timed launch costs a few microseconds more,
+    so these solves are not the timed ones)."""
     from opus_destruct_2_amd import stress
     p = stress.StressProblem(g.nodes, g.bonds, mats, ctx)
     t = time.perf_counter()
@@ -103,9 +105,6 @@ def run_path(g, mats, precision, steps, warmup, ctx, **kw):
     for _ in range(warmup):
         u, r = p.solve(g.f, rtol=1e-8, maxit=5000)
         checks_on_device()
-    ctx.set_profiling(True)
-    for k in range(4):
-        ctx.kernel_stats(k)
     dev_ms, wall_ms, chk_ms, iters = [], [], [], []
     for s in range(steps):
         t0 = time.perf_counter()
@@ -120,8 +119,16 @@ def run_path(g, mats, precision, steps, warmup, ctx, **kw):
         iters.append(r["iters"])
         log(f"  step {s}: solve {dev_ms[-1]:.2f} ms device ({wall_ms[-1]:.2f} wall), checks {chk_ms[-1]:.2f} ms, "
             f"iters {r['iters']} refinements {r['refinements']} rel_res {r['rel_res']:.2e}")
-    ctx.set_profiling(False)
-    kstats = {k: ctx.kernel_stats(k) for k in range(4)}
+    # (the live kernel timing runs on further solves of its own, so that it does not slow the timed ones)
+    kstats = None
+    if profile:
+        ctx.set_profiling(True)
+        for k in range(4):
+            ctx.kernel_stats(k)
+        for s in range(steps):
+            p.solve(g.f, rtol=1e-8, maxit=5000)
+        ctx.set_profiling(False)
+        kstats = {k: ctx.kernel_stats(k) for k in range(4)}
     # the same checks into host arrays (phi and modes over PCIe), and the device arrays' bits against them
     out = (np.zeros(g.m), np.zeros(g.m, np.uint8))
     host_ms = []
@@ -314,7 +321,8 @@ def cpu_baseline(name, graphs, mats, runs_all, runs_one, opts=None, keep=False):
 
 def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)):
     """Batched uneven graphs: `count` pieces (log-normal sizes, median 200, spread sigma, 16..20000
-    nodes; even ones on supports, odd ones pinned), one batch, mixed precision to 1e-8; against the
+    nodes; even ones on supports, odd ones pinned), one batch, mixed precision to 1e-8 (the V-cycle
+    over FP16 block copies on every piece: the FP64 PCG sets the answer's accuracy); against the
     plain FP64 path solving graphs one at a time (a sample). On the sample: the batch's answers
     against the plain path's and, with cpu_runs, against the reference's CPU solve of the same pieces
     (the piece path: multigrid at every size), which is also timed."""
@@ -326,7 +334,10 @@ def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)
     dofs = sum(g.dofs for g in gs)
     log(f"batched: {count} graphs, {sum(g.n for g in gs)} nodes, {dofs} DOFs (sizes {sizes.min()}..{sizes.max()}, "
         f"median {int(np.median(sizes))}, sigma {sigma})")
-    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0, **AMG_KW)
+    # (a throughput batch: every piece's V-cycle reads FP16 block copies, whatever its size; the
+    # serving stream, which is latency-bound, keeps the default, FP16 copies from 16384 free nodes)
+    bkw = {"vcycle_fp16_rows": 0, **AMG_KW}
+    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0, **bkw)
     probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
     t = time.perf_counter()
     ok = stress.assemble_batch(probs)
@@ -370,7 +381,8 @@ def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)
     for p in probs:
         p.close()
     rec = {"graphs": count, "dofs": dofs, "sizes": [int(sizes.min()), int(np.median(sizes)), int(sizes.max())],
-           "sigma": sigma, "graphs_fp16_vcycle": fp16, "graphs_lean": lean,
+           "sigma": sigma, "vcycle_fp16_rows": bkw["vcycle_fp16_rows"], "graphs_fp16_vcycle": fp16,
+           "graphs_lean": lean,
            "value": round(batched, 1), "unit": "DOF/s", "solve_ms": round(solve_ms, 3),
            "wall_ms_pcie_inclusive": round(float(np.median(wall)), 3), "setup_ms": round(setup_ms, 1),
            "precision": "mixed", "iters_max": max(r["iters"] for r in rs),
@@ -869,7 +881,7 @@ def main():
             ctx.set_step_graphs(args.step_graphs)
         if args.only == "head":  # (the headline solves alone, with their live roofline)
             prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
-            head = run_path(g, mats, prec, args.steps, args.warmup, ctx)
+            head = run_path(g, mats, prec, args.steps, args.warmup, ctx, profile=True)
             sb = solve_bytes(head["info"], prec == SVXG_MIXED, head["iters"], head["refinements"])
             rec = {"solve_ms": round(head["solve_ms"], 3), "iters": head["iters"],
                    "checks_ms": round(head["checks_ms"], 3), "checks_host_ms": round(head["checks_host_ms"], 3),
@@ -894,7 +906,7 @@ def main():
         return
     head_prec = SVXG_MIXED if args.precision == "mixed" else SVXG_FP64
     other_prec = SVXG_FP64 if head_prec == SVXG_MIXED else SVXG_MIXED
-    head = run_path(g, mats, head_prec, args.steps, args.warmup, ctx)
+    head = run_path(g, mats, head_prec, args.steps, args.warmup, ctx, profile=True)
     other = run_path(g, mats, other_prec, max(1, args.steps // 2), 1, ctx)
     plain = other if head_prec == SVXG_MIXED else head
     mixed_res = head if head_prec == SVXG_MIXED else other

@@ -107,6 +107,11 @@ typedef struct svxg_amg_options {
                                 finest prolongator is smoothed, and coarsening stops at the first level
                                 the dense solve takes (at most 4 coarse_max nodes). Their coarse
                                 operators stay sparse and the cycle short, at the same iterations */
+  double lean_strength;      /* device: the strength threshold of graphs that take the lean hierarchy
+                                (default 0.05; 0: strength). On the 64x64x48 block it aggregates
+                                the coarse levels about eight to one instead of two to four: levels
+                                [167129, 18396, 1234, 127] instead of six, 39 PCG iterations instead
+                                of 38, the solve 15% shorter */
 } svxg_amg_options;
 
 /* StressOptions (stress.hpp:68-72). */

@@ -108,7 +108,7 @@ inline svxg_material to_c_material(const M& m) {
 
 struct StressOptions {
   f64 rtol = 2e-3;
-  svxg_amg_options amg{0.1, 0.6, 48, 1, 16, 1, 2, 1, 1, 0, 1, 16384};
+  svxg_amg_options amg{0.1, 0.6, 48, 1, 16, 1, 2, 1, 1, 0, 1, 16384, 0.05};
   i32 amg_min_nodes = 160;
   i32 precision = SVXG_FP64;
   i32 mixed_fp64_rows = 1 << 30;  // (device: see svxg_stress_options)

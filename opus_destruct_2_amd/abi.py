@@ -44,7 +44,7 @@ class AmgOptions(ctypes.Structure):
         ("coarse_max", ctypes.c_int32), ("smoothed", ctypes.c_int32), ("max_levels", ctypes.c_int32),
         ("sweeps", ctypes.c_int32), ("cheb_degree", ctypes.c_int32), ("coarsen_dense", ctypes.c_int32),
         ("reaggregate", ctypes.c_int32), ("smooth_levels", ctypes.c_int32), ("coarse_cheb_degree", ctypes.c_int32),
-        ("lean_rows", ctypes.c_int32),
+        ("lean_rows", ctypes.c_int32), ("lean_strength", ctypes.c_double),
     ]
 
 

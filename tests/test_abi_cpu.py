@@ -89,7 +89,7 @@ def test_default_options_without_a_device():
     o = stress.default_options()
     assert o.rtol == 2e-3 and o.amg_min_nodes == 160 and o.amg.coarse_max == 48 and o.amg.strength == 0.1
     assert o.mixed_fp64_rows == 1 << 30 and o.vcycle_fp16_rows == 16384 and o.amg.coarse_cheb_degree == 1
-    assert o.amg.lean_rows == 16384
+    assert o.amg.lean_rows == 16384 and o.amg.lean_strength == 0.05
     j = stress.default_judge_options()
     assert (j.fragility, j.dif, j.break_band, j.quota_fraction, j.hinge_shape) == (1.0, 1.5, 0.85, 0.25, 1.3)
     assert (j.max_breaks, j.max_rounds, j.plastic_hinges, j.rule) == (256, 400, 1, 0)
