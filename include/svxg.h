@@ -103,15 +103,16 @@ typedef struct svxg_amg_options {
   int32_t coarse_cheb_degree;/* device: the smoother's degree below level 0 (default 1; 0: cheb_degree).
                                 A batch runs its first problem's degrees */
   int32_t lean_rows;         /* device: graphs with at least this many free nodes (default 16384; 0: none)
-                                take a lean hierarchy: unless smooth_levels says otherwise only the
-                                finest prolongator is smoothed, and coarsening stops at the first level
-                                the dense solve takes (at most 4 coarse_max nodes). Their coarse
-                                operators stay sparse and the cycle short, at the same iterations */
+                                take a lean hierarchy: unless smooth_levels says otherwise the two
+                                finest prolongators are smoothed, coarser ones not, aggregation uses
+                                lean_strength, and coarsening stops at the first level the dense
+                                solve takes (at most 4 coarse_max nodes). Their coarse operators stay
+                                sparse and the cycle short */
   double lean_strength;      /* device: the strength threshold of graphs that take the lean hierarchy
-                                (default 0.05; 0: strength). On the 64x64x48 block it aggregates
-                                the coarse levels about eight to one instead of two to four: levels
-                                [167129, 18396, 1234, 127] instead of six, 39 PCG iterations instead
-                                of 38, the solve 15% shorter */
+                                (default 0.05; 0: strength). On the 64x64x48 block the coarse levels
+                                aggregate about twelve to one instead of two to four: four levels
+                                [167129, 18396, 1234, 145] instead of six, 32 PCG iterations instead
+                                of 38 (with the two smoothed prolongators) */
 } svxg_amg_options;
 
 /* StressOptions (stress.hpp:68-72). */
@@ -242,8 +243,9 @@ typedef struct svxg_problem_info {
   int32_t level_rows[16];
   int64_t level_blocks[16];
   int64_t level_pblocks[16];
-  int32_t lean;              /* the lean hierarchy (svxg_amg_options.lean_rows): finest prolongator smoothed
-                                only, coarsening ended at the first level the dense solve takes */
+  int32_t lean;              /* the lean hierarchy (svxg_amg_options.lean_rows): the two finest prolongators
+                                smoothed, aggregation at lean_strength, coarsening ended at the first
+                                level the dense solve takes */
   int32_t dense_rows;        /* rows of the coarsest level's dense inverse (6 x its nodes), 0 if none */
 } svxg_problem_info;
 int svxg_problem_info_get(const svxg_problem* p, svxg_problem_info* out);
