@@ -1843,21 +1843,27 @@ __global__ void __launch_bounds__(kWG) k_zero_many(const BulkSeg* __restrict__ s
 }
 
 // a block's entries over its largest magnitude, rounded to FP16 (A_ij and A_ji = A_ij^T hold the
-// same values, so a symmetric matrix's copy stays symmetric)
-__global__ void __launch_bounds__(kWG) k_half_blocks(const float* __restrict__ A, uint8_t* __restrict__ H, int64_t nnzb) {
-  const int64_t k = int64_t(blockIdx.x) * kWG + threadIdx.x;
-  if (k >= nnzb) return;
-  const float* a = A + 36 * k;
+// same values, so a symmetric matrix's copy stays symmetric); tr: the block's transpose is stored
+// (a restriction entry's block, read as P's block transposed)
+__device__ __forceinline__ void half_block(const float* __restrict__ a, uint8_t* __restrict__ h, bool tr) {
   float m = 0.0f;
   for (int i = 0; i < 36; ++i) m = fmaxf(m, fabsf(a[i]));
   const float inv = m > 0.0f ? 1.0f / m : 0.0f;
-  uint8_t* h = H + kHalfBlock * k;
   for (int i = 0; i < 36; i += 2) {
-    const __half2 v = __floats2half2_rn(a[i] * inv, a[i + 1] * inv);
+    // (the transpose's entries (r, c), (r, c + 1) are the block's (c, r), (c + 1, r))
+    const int r = i / 6, c = i % 6;
+    const __half2 v = tr ? __floats2half2_rn(a[6 * c + r] * inv, a[6 * (c + 1) + r] * inv)
+                         : __floats2half2_rn(a[i] * inv, a[i + 1] * inv);
     *reinterpret_cast<__half2*>(h + 2 * i) = v;
   }
   reinterpret_cast<float*>(h + 72)[0] = m;
   reinterpret_cast<float*>(h + 72)[1] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kWG) k_half_blocks(const float* __restrict__ A, uint8_t* __restrict__ H, int64_t nnzb) {
+  const int64_t k = int64_t(blockIdx.x) * kWG + threadIdx.x;
+  if (k >= nnzb) return;
+  half_block(A + 36 * k, H + kHalfBlock * k, false);
 }
 
 // the same for the transposes of the blocks P[ent[e]] (a restriction entry's block), in entry order
@@ -1865,19 +1871,18 @@ __global__ void __launch_bounds__(kWG) k_half_blocks_t(const float* __restrict__
                                                        uint8_t* __restrict__ H, int64_t n) {
   const int64_t e = int64_t(blockIdx.x) * kWG + threadIdx.x;
   if (e >= n) return;
-  const float* a = P + 36 * size_t(ent[e]);
-  float m = 0.0f;
-  for (int i = 0; i < 36; ++i) m = fmaxf(m, fabsf(a[i]));
-  const float inv = m > 0.0f ? 1.0f / m : 0.0f;
-  uint8_t* h = H + kHalfBlock * e;
-  for (int i = 0; i < 36; i += 2) {
-    // (the transpose's entries (r, c), (r, c + 1) are P's (c, r), (c + 1, r))
-    const int r = i / 6, c = i % 6;
-    const __half2 v = __floats2half2_rn(a[6 * c + r] * inv, a[6 * (c + 1) + r] * inv);
-    *reinterpret_cast<__half2*>(h + 2 * i) = v;
+  half_block(P + 36 * size_t(ent[e]), H + kHalfBlock * e, true);
+}
+
+// many arrays' FP16 copies in one launch: a workgroup a piece of at most kWG blocks
+__global__ void __launch_bounds__(kWG) k_half_many(const HalfSeg* __restrict__ segs, int n) {
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const HalfSeg g = segs[k];
+    const int64_t e = int64_t(threadIdx.x);
+    if (e >= g.count) continue;
+    const float* a = g.ent ? g.src + 36 * size_t(g.ent[e]) : g.src + 36 * e;
+    half_block(a, g.dst + kHalfBlock * e, g.ent != nullptr);
   }
-  reinterpret_cast<float*>(h + 72)[0] = m;
-  reinterpret_cast<float*>(h + 72)[1] = 0.0f;
 }
 
 __global__ void __launch_bounds__(kWG) k_f32_many(const BulkSeg* __restrict__ segs, int n) {
@@ -2129,6 +2134,9 @@ void launch_half_blocks_t(hipStream_t s, const float* P, const int32_t* ent, uin
   if (n > 0) hipLaunchKernelGGL(k_half_blocks_t, dim3(unsigned((n + kWG - 1) / kWG)), dim3(kWG), 0, s, P, ent, H, n);
 }
 
+void launch_half_many(hipStream_t s, const HalfSeg* segs, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_half_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
+}
 void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n) {
   if (n > 0) hipLaunchKernelGGL(k_f32_many, dim3(bulk_grid(n)), dim3(kWG), 0, s, segs, n);
 }

@@ -286,6 +286,15 @@ void launch_f32_many(hipStream_t s, const BulkSeg* segs, int n);
 void launch_half_blocks(hipStream_t s, const float* A, uint8_t* H, int64_t nnzb);
 // the FP16 copy of the transposes of the blocks P[ent[e]], e < n, in the order of e
 void launch_half_blocks_t(hipStream_t s, const float* P, const int32_t* ent, uint8_t* H, int64_t n);
+// many FP16 copies in one launch: piece k converts `count` (at most kWG) blocks into dst, from src's
+// consecutive blocks, or transposed from src's blocks ent[0 .. count) when ent is set
+struct HalfSeg {
+  const float* src;
+  const int32_t* ent;
+  uint8_t* dst;
+  int64_t count;
+};
+void launch_half_many(hipStream_t s, const HalfSeg* segs, int n);
 // each segment's count bytes copied from src to dst (both 16-byte aligned)
 void launch_copy_many(hipStream_t s, const BulkSeg* segs, int n);
 
