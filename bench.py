@@ -321,8 +321,7 @@ def cpu_baseline(name, graphs, mats, runs_all, runs_one, opts=None, keep=False):
 
 def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)):
     """Batched uneven graphs: `count` pieces (log-normal sizes, median 200, spread sigma, 16..20000
-    nodes; even ones on supports, odd ones pinned), one batch, mixed precision to 1e-8 (the V-cycle
-    over FP16 block copies on every piece: the FP64 PCG sets the answer's accuracy); against the
+    nodes; even ones on supports, odd ones pinned), one batch, mixed precision to 1e-8; against the
     plain FP64 path solving graphs one at a time (a sample). On the sample: the batch's answers
     against the plain path's and, with cpu_runs, against the reference's CPU solve of the same pieces
     (the piece path: multigrid at every size), which is also timed."""
@@ -334,10 +333,7 @@ def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)
     dofs = sum(g.dofs for g in gs)
     log(f"batched: {count} graphs, {sum(g.n for g in gs)} nodes, {dofs} DOFs (sizes {sizes.min()}..{sizes.max()}, "
         f"median {int(np.median(sizes))}, sigma {sigma})")
-    # (a throughput batch: every piece's V-cycle reads FP16 block copies, whatever its size; the
-    # serving stream, which is latency-bound, keeps the default, FP16 copies from 16384 free nodes)
-    bkw = {"vcycle_fp16_rows": 0, **AMG_KW}
-    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0, **bkw)
+    opts = stress.default_options(precision=SVXG_MIXED, amg_min_nodes=0, **AMG_KW)
     probs = [stress.StressProblem(g.nodes, g.bonds, mats, ctx, options=opts) for g in gs]
     t = time.perf_counter()
     ok = stress.assemble_batch(probs)
@@ -381,8 +377,7 @@ def run_batch(mats, ctx, count, steps, sample_plain, sigma=0.75, cpu_runs=(0, 0)
     for p in probs:
         p.close()
     rec = {"graphs": count, "dofs": dofs, "sizes": [int(sizes.min()), int(np.median(sizes)), int(sizes.max())],
-           "sigma": sigma, "vcycle_fp16_rows": bkw["vcycle_fp16_rows"], "graphs_fp16_vcycle": fp16,
-           "graphs_lean": lean,
+           "sigma": sigma, "graphs_fp16_vcycle": fp16, "graphs_lean": lean,
            "value": round(batched, 1), "unit": "DOF/s", "solve_ms": round(solve_ms, 3),
            "wall_ms_pcie_inclusive": round(float(np.median(wall)), 3), "setup_ms": round(setup_ms, 1),
            "precision": "mixed", "iters_max": max(r["iters"] for r in rs),
@@ -560,7 +555,59 @@ class _Piece:
         self.h, self.n, self.g = h, g.n, g
 
 
-def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11, setup_workers=None):
+def serving_sample(mats, ctx, pool, count, cpu_runs, seed=13):
+    """A sample of served requests against the reference's CPU solve (the piece path): `count` pool
+    graphs served cold, then each served again warm after losing 1-5 % of its bonds in place (as the
+    stream's damaged residents do); the reference solves the same graphs, the second time with those
+    bonds broken. Reports the served answers' u and phi errors and the failed-bond agreement."""
+    import oracle
+    from opus_destruct_2_amd import gen, serve, stress
+    from opus_destruct_2_amd.abi import SVXG_MIXED
+    rng = np.random.default_rng(seed)
+    graphs = [pool[k] for k in range(0, len(pool), max(1, len(pool) // count))][:count]
+    srv = serve.Server(ctx, precision=SVXG_MIXED, rtol=1e-8, maxit=5000, amg_min_nodes=0, window=4)
+    pieces = [_Piece(g, mats, ctx) for g in graphs]
+    n = len(pieces)
+
+    def served(recs, tickets):
+        out = []
+        for t, pc in zip(tickets, pieces):
+            r = recs[t]
+            assert r["status"] == 0 and r["result"]["converged"], r
+            phi = np.zeros(len(pc.g.bonds))
+            mode = np.zeros(len(pc.g.bonds), np.uint8)
+            st = stress.lib().svxg_problem_bond_checks(pc.h, None, 1.0, phi.ctypes.data, mode.ctypes.data)
+            assert st == 0
+            out.append((r["u"], phi))
+        return out
+
+    tk = srv.submit_many(pieces, [g.f for g in graphs], [False] * n, [None] * n, want_u=True)
+    cold = served(srv.wait(tk), tk)
+    cuts, damaged = [], []
+    for g in graphs:
+        c = max(1, int(len(g.bonds) * rng.uniform(0.01, 0.05)))
+        ids = gen.safe_cuts(g, np.zeros(len(g.bonds), np.uint8), c, int(rng.integers(1 << 62)))
+        cuts.append(ids)
+        b = g.bonds.copy()
+        b["broken"][ids] = 1
+        damaged.append(gen.Graph(g.nodes, b, g.f, g.name + "-damaged"))
+    tk = srv.submit_many(pieces, [None] * n, [True] * n, cuts, want_u=True)
+    warm = served(srv.wait(tk), tk)
+    for pc in pieces:
+        srv.release(pc)
+    srv.close()
+    opts = oracle.reference_options(amg_min_nodes=0)
+    cpu, ref_cold = cpu_baseline("serving sample: new pieces", graphs, mats, cpu_runs, 0, opts, keep=True)
+    _, ref_warm = cpu_baseline("serving sample: the same pieces damaged", damaged, mats, 1, 0, opts, keep=True)
+    rec = {"graphs": n, "dofs": int(sum(g.dofs for g in graphs)),
+           "cold_vs_cpu_reference": merge_compare([compare(u, ph, cu, cp) for (u, ph), (cu, cp) in zip(cold, ref_cold)]),
+           "damaged_warm_vs_cpu_reference": merge_compare(
+               [compare(u, ph, cu, cp) for (u, ph), (cu, cp) in zip(warm, ref_warm)])}
+    return rec, cpu
+
+
+def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11, setup_workers=None, sample=0,
+                cpu_runs=0):
     """Serving stream: 60 Hz ticks; each tick, Poisson(lam) new structures (sizes as the batched
     leg's, half on supports, half pinned; multigrid at every size, as pieces are) arrive and are solved cold (setup + solve), and 10 % of the
     resident structures that are idle lose 1-5 % of their bonds and are solved again, warm, on their
@@ -684,6 +731,9 @@ def run_serving(mats, ctx, lams, ticks, warm_ticks, life, pool_size, seed=11, se
             f"{rec['ticks_over_16.7ms']} of {len(lat)} ticks over 16.7 ms; iters median {rec['iters_median']} "
             f"max {rec['iters_max']}; stages p50 {rec['stage_p50_ms']} p99 {rec['stage_p99_ms']}; server {rec['server']}")
         out["per_lambda"].append(rec)
+    if sample > 0:
+        out["sample"], out["sample_cpu"] = serving_sample(mats, ctx, pool, sample, max(1, cpu_runs))
+        log(f"  serving sample: {out['sample']}")
     return out
 
 
@@ -799,6 +849,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096, help="graphs in the batched leg (0: skip)")
     ap.add_argument("--plain-sample", type=int, default=128)
     ap.add_argument("--batch-sigma", type=float, default=0.75, help="log-normal spread of the batched leg's sizes")
+    ap.add_argument("--batch-wide-sigma", type=float, default=1.32,
+                    help="log-normal spread of the batched leg's full-range variant (0: skip it)")
     ap.add_argument("--serve-lambdas", default="50,200,800", help="arrivals per tick of the serving leg ('' skips)")
     ap.add_argument("--serve-ticks", type=int, default=90)
     ap.add_argument("--serve-warmup", type=int, default=30)
@@ -963,12 +1015,21 @@ def main():
         line["batched"], cpu_all["batched_sample"] = run_batch(
             mats, ctx, args.batch, max(1, args.steps // 2), args.plain_sample, args.batch_sigma,
             (0, 0) if args.no_cpu else (args.cpu_runs, args.cpu_runs))
+        # the same count of pieces over the configuration's whole 16..20,000-node range (a spread
+        # whose largest of 4096 draws reaches the clamp): graphs of 16384+ free nodes among them take
+        # the lean hierarchy inside the batch
+        if args.batch_wide_sigma > 0:
+            line["batched_full_range"], _ = run_batch(mats, ctx, args.batch, 1, min(8, args.plain_sample),
+                                                      args.batch_wide_sigma)
     if args.serve_lambdas:
         lams = [int(x) for x in args.serve_lambdas.split(",") if x]
         log("serving: 60 Hz ticks, lambdas " + ", ".join(map(str, lams)))
         # (on the context the headline and the batch used: a long-lived server's state)
         line["serving"] = run_serving(mats, ctx, lams, args.serve_ticks, args.serve_warmup, args.serve_life,
-                                      args.serve_pool, setup_workers=args.serve_setup_workers)
+                                      args.serve_pool, setup_workers=args.serve_setup_workers,
+                                      sample=0 if args.no_cpu else 16, cpu_runs=args.cpu_runs)
+        if line["serving"].get("sample_cpu") is not None:
+            cpu_all["serving_sample"] = line["serving"].pop("sample_cpu")
     if args.city > 0:
         cctx = stress.Context(0)  # (a context of its own, freed after the leg)
         line["city"] = run_city(mats, cctx, args.city, args.city_rounds, args.city_rate)
