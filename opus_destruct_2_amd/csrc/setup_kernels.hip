@@ -1411,16 +1411,12 @@ __device__ __forceinline__ double lpb_rows64(const int32_t* __restrict__ rowptr,
   return acc;
 }
 
-// v_{s+1} = D^-1 A (v_s / |v_s|), partial |v_{s+1}|^2 (S six-lane sub-groups a row: S = 10 serves
-// the long rows of coarse levels a wavefront each; S = 1 takes the products a block a lane pair)
+// v_{s+1} = D^-1 A (v_s / |v_s|) over a tile, and the tile's partial of |v_{s+1}|^2 (slot parity s & 1:
+// step s + 1 reads it while writing the other). S six-lane sub-groups a row: S = 10 serves the long
+// rows of coarse levels a wavefront each; S = 1 takes the products a block a lane pair. inv: 1 / |v_s|.
 template <int S>
-__global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles,
-                                                    int ntiles, int step, double* __restrict__ partial) {
-  if (int(blockIdx.x) >= ntiles) return;
-  const Tile tl = tiles[blockIdx.x];
-  const PowerDev& p = J[tl.g];
-  if (p.lam[1] <= 0.0) return;  // (stopped: a zero vector)
-  const double inv = step == 0 ? 1.0 : 1.0 / p.lam[1];
+__device__ __forceinline__ void power_step_tile(const PowerDev& p, const Tile& tl, int step, double inv,
+                                                double* __restrict__ partial) {
   const double* in = p.v[step & 1];
   double* out = p.v[(step + 1) & 1];
   int rl, sub, q;
@@ -1450,18 +1446,68 @@ __global__ void __launch_bounds__(kWG) k_power_step(const PowerDev* __restrict__
   }
   __shared__ double lds[4];
   const double s = block_sum(v * v, lds);
-  if (threadIdx.x == 0) partial[2 * tl.slot] = s;
+  if (threadIdx.x == 0) partial[2 * tl.slot + (step & 1)] = s;
 }
 
-__global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict__ J, const int32_t* __restrict__ gtile,
-                                                      int count, const double* __restrict__ partial) {
-  if (int(blockIdx.x) >= count) return;
-  const PowerDev& p = J[blockIdx.x];
-  if (p.lam[1] <= 0.0) return;
+// |v_s|^2 from a graph's per-tile partials of step s - 1 (slots k0 .. k1, parity `which`): each lane's
+// strided sum in slot order, then the workgroup's fixed order (thread 0 holds it)
+__device__ __forceinline__ double power_sum(int k0, int k1, const double* __restrict__ partial, int which) {
   __shared__ double lds[4];
   double a = 0.0;
-  for (int k = gtile[blockIdx.x] + int(threadIdx.x); k < gtile[blockIdx.x + 1]; k += kWG) a += partial[2 * k];
-  const double s = block_sum(a, lds);
+  for (int k = k0 + int(threadIdx.x); k < k1; k += kWG) a += partial[2 * k + which];
+  return block_sum(a, lds);
+}
+
+// One step of every graph's power iteration in one launch: workgroups [0, n1) take tiles1 (S = 1),
+// the rest tiles10 (S = 10). A graph of few tiles (p.self) normalises itself: each of its workgroups
+// sums the last step's partials (the same order everywhere, so the same |v_s|), stops when |v_s| is
+// not positive (writing zero partials, so that it stays stopped) and its first tile keeps |v_s| in
+// lam; a graph of many tiles reads |v_s| from lam, which k_power_finish set (stopped: lam[1] = 0).
+__global__ void __launch_bounds__(kWG) k_power_steps(const PowerDev* __restrict__ J, const Tile* __restrict__ tiles1,
+                                                     int n1, const Tile* __restrict__ tiles10, int n10,
+                                                     const int32_t* __restrict__ gtile, int step,
+                                                     double* __restrict__ partial) {
+  const int b = int(blockIdx.x);
+  if (b >= n1 + n10) return;
+  const bool longr = b >= n1;
+  const Tile tl = longr ? tiles10[b - n1] : tiles1[b];
+  const PowerDev& p = J[tl.g];
+  double inv = 1.0;
+  if (p.self) {
+    if (step > 0) {
+      __shared__ double nvs;
+      const double ss = power_sum(gtile[tl.g], gtile[tl.g + 1], partial, (step - 1) & 1);
+      if (threadIdx.x == 0) nvs = sqrt(ss);
+      __syncthreads();
+      const double nv = nvs;
+      if (!(nv > 0)) {
+        if (threadIdx.x == 0) partial[2 * tl.slot + (step & 1)] = 0.0;
+        return;
+      }
+      if (threadIdx.x == 0 && tl.slot == gtile[tl.g]) {
+        p.lam[0] = nv;
+        p.lam[1] = nv;
+      }
+      inv = 1.0 / nv;
+    }
+  } else {
+    if (p.lam[1] <= 0.0) return;  // (stopped: a zero vector)
+    if (step > 0) inv = 1.0 / p.lam[1];
+  }
+  if (longr) power_step_tile<10>(p, tl, step, inv, partial);
+  else power_step_tile<1>(p, tl, step, inv, partial);
+}
+
+// |v_{s+1}| of the graphs of many tiles (close = 0) after step s, or of the graphs that normalise
+// themselves after the last step (close = 1): lam = |v_{s+1}| while positive, else the graph stops
+// (lam[1] = 0; lam[0] keeps the last estimate). A workgroup a graph.
+__global__ void __launch_bounds__(kWG) k_power_finish(const PowerDev* __restrict__ J, const int32_t* __restrict__ gtile,
+                                                      int count, const double* __restrict__ partial, int which,
+                                                      int close) {
+  if (int(blockIdx.x) >= count) return;
+  const PowerDev& p = J[blockIdx.x];
+  if ((p.self != 0) != (close != 0) || p.lam[1] <= 0.0) return;
+  const double s = power_sum(gtile[blockIdx.x], gtile[blockIdx.x + 1], partial, which);
   if (threadIdx.x == 0) {
     const double nv = sqrt(s);
     if (!(nv > 0)) {
@@ -2087,17 +2133,16 @@ void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, i
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_power_init, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, count);
 }
-void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial,
-                       bool long_rows) {
-  if (ntiles <= 0) return;
-  if (long_rows)
-    hipLaunchKernelGGL(k_power_step<10>, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
-  else
-    hipLaunchKernelGGL(k_power_step<1>, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles, step, partial);
+void launch_power_steps(hipStream_t s, const PowerDev* jobs, const Tile* tiles1, int n1, const Tile* tiles10, int n10,
+                        const int32_t* gtile, int step, double* partial) {
+  if (n1 + n10 <= 0) return;
+  hipLaunchKernelGGL(k_power_steps, dim3(n1 + n10), dim3(kWG), 0, s, jobs, tiles1, n1, tiles10, n10, gtile, step,
+                     partial);
 }
-void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial) {
+void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial,
+                         int which, bool close) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_power_finish, dim3(count), dim3(kWG), 0, s, jobs, gtile, count, partial);
+  hipLaunchKernelGGL(k_power_finish, dim3(count), dim3(kWG), 0, s, jobs, gtile, count, partial, which, close ? 1 : 0);
 }
 void launch_dense_factor(hipStream_t s, const DenseDev* jobs, int count) {
   if (count > 0) hipLaunchKernelGGL(k_dense_factor, dim3(count), dim3(kDenseWG), 0, s, jobs, count);

@@ -227,7 +227,7 @@ void launch_c_fill(hipStream_t s, const CoarsenDev* jobs, const Tile* etiles, in
 // power iteration for lambda_max(D^-1 A) (amg.cpp:256-275): 15 steps, per-graph norms
 struct PowerDev {
   int32_t n;
-  int32_t pad_;
+  int32_t self;  // few tiles: its steps find |v| from the last step's partials themselves (no finish launch)
   const int32_t* rowptr;
   const int32_t* col;
   const double* A;
@@ -236,9 +236,15 @@ struct PowerDev {
   double* lam;   // 2 doubles: [0] the estimate, [1] the last norm
 };
 void launch_power_init(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int count);
-void launch_power_step(hipStream_t s, const PowerDev* jobs, const Tile* tiles, int ntiles, int step, double* partial,
-                       bool long_rows);
-void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial);
+// a graph normalises itself in the steps when it has at most this many tiles
+constexpr int kPowerSelfTiles = 256;
+// one step of every graph: tiles1 (S = 1) and tiles10 (S = 10) in one launch; partials by parity of step
+void launch_power_steps(hipStream_t s, const PowerDev* jobs, const Tile* tiles1, int n1, const Tile* tiles10, int n10,
+                        const int32_t* gtile, int step, double* partial);
+// the norm of the partials of parity `which`: the graphs of many tiles after each step, or (close) the
+// self-normalising ones after the last
+void launch_power_finish(hipStream_t s, const PowerDev* jobs, const int32_t* gtile, int count, const double* partial,
+                         int which, bool close);
 
 // the coarsest level's dense inverse (amg.cpp:535-578, then inverted)
 // the P, A P and P^T A P fills run a wavefront an entry (its lanes find the entry's terms together,
