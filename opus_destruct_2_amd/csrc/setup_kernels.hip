@@ -281,32 +281,44 @@ __global__ void k_diag_trace(const AssemblyDev* __restrict__ J, const Tile* __re
 }
 
 // id_scale = mean of tr(K_ii[0:3,0:3]) / 3 over rows with a positive trace, summed in row order
-// (stress.cpp:111-122). One wavefront a graph: 64 traces are loaded and divided by 3 at a time
-// (rows without a positive trace contribute +0.0, which leaves the running sum's bits unchanged: it
-// starts at +0.0 and only grows), then added in row order from scalar reads of the 64 lanes.
-__global__ void k_id_scale(const AssemblyDev* __restrict__ J, int count) {
+// (stress.cpp:111-122). One workgroup a graph: fifteen wavefronts load a chunk of 960 traces and
+// divide them by 3 into LDS (rows without a positive trace give +0.0, which leaves the running sum's
+// bits unchanged: it starts at +0.0 and only grows) while lane 0 of the first adds the previous
+// chunk's in row order - the sequential sum, with its operands ready in LDS.
+constexpr int kIdWG = 1024, kIdRows = kIdWG - 64;
+__global__ void __launch_bounds__(kIdWG) k_id_scale(const AssemblyDev* __restrict__ J, int count) {
   const int g = blockIdx.x;
   if (g >= count) return;
   const AssemblyDev& a = J[g];
-  const int l = threadIdx.x;
+  const int t = int(threadIdx.x), w = t >> 6;
+  __shared__ double buf[2][kIdRows];
+  __shared__ int cnts[kIdWG / 64];
+  const int nchunk = (a.nfree + kIdRows - 1) / kIdRows;
   double tr = 0.0;
-  int cnt = 0;
-  for (int base = 0; base < a.nfree; base += 64) {
-    const int i = base + l;
-    const double t = i < a.nfree ? a.trace[i] : 0.0;
-    const bool pos = t > 0;
-    const double t3 = pos ? t / 3.0 : 0.0;
-    cnt += __popcll(__ballot(pos));
-    const int2 w = *reinterpret_cast<const int2*>(&t3);
-#pragma unroll
-    for (int k = 0; k < 64; ++k) {
-      int2 v;
-      v.x = __builtin_amdgcn_readlane(w.x, k);
-      v.y = __builtin_amdgcn_readlane(w.y, k);
-      tr += *reinterpret_cast<const double*>(&v);
+  int cnt = 0;  // (a wavefront's count of positive traces)
+  for (int c = 0; c <= nchunk; ++c) {
+    if (w > 0) {
+      if (c < nchunk) {
+        const int i = c * kIdRows + (t - 64);
+        const double tv = i < a.nfree ? a.trace[i] : 0.0;
+        const bool pos = tv > 0;
+        buf[c & 1][t - 64] = pos ? tv / 3.0 : 0.0;
+        cnt += __popcll(__ballot(pos));
+      }
+    } else if (t == 0 && c > 0) {
+      const double* b = buf[(c - 1) & 1];
+      const int m = min(kIdRows, a.nfree - (c - 1) * kIdRows);
+      for (int k = 0; k < m; ++k) tr += b[k];
     }
+    __syncthreads();
   }
-  if (l == 0) *a.id_scale = cnt ? tr / cnt : 1.0;
+  if ((t & 63) == 0) cnts[w] = cnt;
+  __syncthreads();
+  if (t == 0) {
+    int n = 0;
+    for (int k = 1; k < kIdWG / 64; ++k) n += cnts[k];
+    *a.id_scale = n ? tr / n : 1.0;
+  }
 }
 
 __global__ void k_identity_rows(const AssemblyDev* __restrict__ J, const Tile* __restrict__ tiles, int ntiles) {
@@ -2018,7 +2030,7 @@ void launch_diag_trace(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles
   if (ntiles > 0) hipLaunchKernelGGL(k_diag_trace, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
 }
 void launch_id_scale(hipStream_t s, const AssemblyDev* jobs, int count) {
-  if (count > 0) hipLaunchKernelGGL(k_id_scale, dim3(count), dim3(64), 0, s, jobs, count);
+  if (count > 0) hipLaunchKernelGGL(k_id_scale, dim3(count), dim3(kIdWG), 0, s, jobs, count);
 }
 void launch_identity_rows(hipStream_t s, const AssemblyDev* jobs, const Tile* tiles, int ntiles) {
   if (ntiles > 0) hipLaunchKernelGGL(k_identity_rows, dim3(ntiles), dim3(kWG), 0, s, jobs, tiles, ntiles);
