@@ -571,6 +571,10 @@ struct Batch {
   int nt0 = 0;
   DevBuf t0run;    // those of the tiled graphs (what the tiled launches run over)
   int nt0run = 0;
+  // host copies of the tiled launches' tables (t0run, tj, and each level's below), from which the
+  // tables are cut to the running graphs in a batch's tail (compact_tiles) and made whole again
+  std::vector<Tile> h_t0run, h_tj;
+  bool compacted = false;
   DevBuf gtile;
   DevBuf tj;
   int ntj = 0;
@@ -587,10 +591,12 @@ struct Batch {
     int next = 0;        // the candidate the next launch takes
     DevBuf tiles[4];
     int nt[4] = {0, 0, 0, 0};
+    std::vector<Tile> htiles[4];
   };
   struct Lv {
     DevBuf ts[2], tp, tc[2], tstall[2], dense;
     int nts[2] = {0, 0}, ntp = 0, ntc[2] = {0, 0}, ntstall[2] = {0, 0}, ndense = 0;
+    std::vector<Tile> hts[2], htp, htc[2], htstall[2], hdense;
     DevBuf dpart, dcnt;  // (few dense chunks: the solve a wavefront a (chunk, span), its partials and counters)
     bool dsplit = false;  // the dense solve a wavefront a span (the tuned or default choice)
     Tune tune[4];
@@ -816,6 +822,9 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
   SVXG_TRY(up.add(B.t0, A, t0));
   SVXG_TRY(up.add(B.t0run, A, t0run));
   SVXG_TRY(up.add(B.tj, A, tj));
+  B.h_t0run = t0run;
+  B.h_tj = tj;
+  B.compacted = false;
   SVXG_TRY(up.add(B.gtile, A, gt));
   SVXG_TRY(up.add(B.res, A, res));
   SVXG_TRY(B.partial.make(A, sizeof(double) * 2 * size_t(std::max(1, B.nt0))));
@@ -906,6 +915,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
           else add_tiles(ct, g, P->lv[size_t(l) + 1].n, lpb_tile_rows(H.np, P->lv[size_t(l) + 1].n, v64, tg));
         }
         tu.nt[ci] = int(ct.size());
+        tu.htiles[ci] = ct;
         SVXG_TRY(up.add(tu.tiles[ci], A, ct));
       }
     }
@@ -922,6 +932,9 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
       }
     }
     for (int v = 0; v < 2; ++v) {
+      V.hts[v] = T.ts[v];
+      V.htc[v] = T.tc[v];
+      V.htstall[v] = T.tstall[v];
       V.nts[v] = int(T.ts[v].size());
       V.ntc[v] = int(T.tc[v].size());
       V.ntstall[v] = int(T.tstall[v].size());
@@ -929,6 +942,8 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
       SVXG_TRY(up.add(V.tc[v], A, T.tc[v]));
       SVXG_TRY(up.add(V.tstall[v], A, T.tstall[v]));
     }
+    V.htp = T.tp;
+    V.hdense = T.dense;
     V.ntp = int(T.tp.size());
     V.ndense = int(T.dense.size());
     SVXG_TRY(up.add(V.tp, A, T.tp));
@@ -1127,16 +1142,20 @@ static void tune_collect(Batch& B) {
       if (k == 0) {
         V.ts[0] = std::move(tu.tiles[choice]);
         V.nts[0] = tu.nt[choice];
+        V.hts[0] = std::move(tu.htiles[choice]);
       } else if (k == 1) {
         V.tp = std::move(tu.tiles[choice]);
         V.ntp = tu.nt[choice];
+        V.htp = std::move(tu.htiles[choice]);
       } else if (k == 2) {
         V.tc[0] = std::move(tu.tiles[choice]);
         V.ntc[0] = tu.nt[choice];
+        V.htc[0] = std::move(tu.htiles[choice]);
       } else {
         V.dsplit = choice == 1;
       }
       for (DevBuf& d : tu.tiles) d = DevBuf();
+      for (std::vector<Tile>& h : tu.htiles) h.clear();
     }
 }
 
@@ -1309,7 +1328,9 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
   if (!B.nt0run) return;
   auto lean_at = [&](int k) { return mixed && k < w - 1 && (lean || k > 0); };
   const svxg_ctx* c = B.probs[0]->ctx;
-  const bool record = c->step_graphs == 2 || (c->step_graphs == 1 && B.nt0run >= Batch::kGraphMinTiles && ++B.windows > 2);
+  // (a batch's size before its tables were cut: the cut tables are recorded again)
+  const bool record = c->step_graphs == 2 ||
+                      (c->step_graphs == 1 && int64_t(B.h_t0run.size()) >= Batch::kGraphMinTiles && ++B.windows > 2);
   if (record && !c->profiling && !B.graphs_failed && !B.tuning) {
     // (each step form recorded on first use; a failed recording leaves this batch to launch kernel
     // by kernel)
@@ -1344,12 +1365,69 @@ static void run_window(Batch& B, hipStream_t s, int w, bool mixed, bool p32, boo
   }
 }
 
+// The tiled launches' tables cut to the graphs still running (filter), or made whole again. In the
+// tail of a batch of uneven graphs most graphs have finished, and a launch over every tile pays for
+// their workgroups (which only exit) - about 10 us a level-0 launch of the 4096-piece batch. The
+// tables are rebuilt from their host copies in the same order, each graph's tiles as they were
+// (their slots too), so no graph's arithmetic changes. Called after a synchronisation (no launch
+// uses the old tables); the recorded step graphs, which hold them, are dropped.
+static int compact_tiles(Batch& B, hipStream_t s, bool filter) {
+  svxg_ctx* c = B.probs[0]->ctx;
+  Arena* A = c->arena.get();
+  std::vector<char> keep(size_t(B.G), 1);
+  if (filter)
+    for (int g = 0; g < B.G; ++g) keep[size_t(g)] = B.hstate[size_t(g)].state == ST_RUNNING ? 1 : 0;
+  std::vector<std::vector<Tile>> hold;  // (the cut tables, alive until the flush)
+  hold.reserve(2 + 8 * B.lvl.size());
+  Stager up(c, s);
+  auto put = [&](DevBuf& d, int& n, const std::vector<Tile>& h) -> int {
+    hold.emplace_back();
+    std::vector<Tile>& o = hold.back();
+    for (const Tile& t : h)
+      if (keep[size_t(t.g)]) o.push_back(t);
+    n = int(o.size());
+    return up.add(d, A, o);
+  };
+  SVXG_TRY(put(B.t0run, B.nt0run, B.h_t0run));
+  SVXG_TRY(put(B.tj, B.ntj, B.h_tj));
+  for (Batch::Lv& V : B.lvl) {
+    for (int v = 0; v < 2; ++v) {
+      SVXG_TRY(put(V.ts[v], V.nts[v], V.hts[v]));
+      SVXG_TRY(put(V.tc[v], V.ntc[v], V.htc[v]));
+      SVXG_TRY(put(V.tstall[v], V.ntstall[v], V.htstall[v]));
+    }
+    SVXG_TRY(put(V.tp, V.ntp, V.htp));
+    SVXG_TRY(put(V.dense, V.ndense, V.hdense));
+  }
+  SVXG_TRY(up.flush());
+  for (hipGraphExec_t& g : B.step_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  B.compacted = filter;
+  return SVXG_OK;
+}
+
+// at a look (states just read): the tables cut whenever at most half of their level-0 tiles belong
+// to running graphs (while at least kCompactMinTiles remain: a finished graph's workgroup costs a
+// launch about 1 ns, a cut about 0.1-0.2 ms; not while launch shapes are tuned)
+static constexpr int kCompactMinTiles = 512;
+static int maybe_compact(Batch& B, hipStream_t s) {
+  if (B.tuning || !B.tune_pending.empty() || B.nt0run < kCompactMinTiles) return SVXG_OK;
+  int64_t live = 0;
+  for (const Tile& t : B.h_t0run) live += B.hstate[size_t(t.g)].state == ST_RUNNING ? 1 : 0;
+  if (2 * live > B.nt0run) return SVXG_OK;
+  return compact_tiles(B, s, true);
+}
+
 // the FP64 PCG over a batch (after run_begin)
 static int pcg_fp64_iterate(Batch& B, hipStream_t s) {
   PollPlan plan;
   for (;;) {
     SVXG_TRY(read_states(B, s));
     if (!any_running(B, false)) break;
+    SVXG_TRY(maybe_compact(B, s));
     run_window(B, s, plan.next(B, false), false, false, false);
   }
   SVXG_HIP(hipGetLastError());
@@ -1425,6 +1503,7 @@ static int pcg_mixed_iterate(Batch& B, hipStream_t s) {
   for (;;) {
     SVXG_TRY(read_states(B, s));
     if (!any_running(B, false)) break;
+    SVXG_TRY(maybe_compact(B, s));
     const int w = plan.next(B, true);
     mixed_window(B, s, p32, p64);
     bool lean = true;  // (every running graph iterates an inner solve)

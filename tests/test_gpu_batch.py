@@ -135,3 +135,44 @@ def test_step_graphs_equal_kernel_launches_bitwise(prec, mats):
         assert np.array_equal(bits(a[k]), bits(b[k])), SIZES[k]
     for (ua, qa), (ub, qb) in zip(sa, sb):
         assert qa == qb and np.array_equal(bits(ua), bits(ub))
+
+
+@pytest.mark.parametrize("prec", PRECISIONS[:2], ids=PREC_IDS[:2])
+@pytest.mark.parametrize("step_graphs", [2, 1], ids=["recorded", "default"])
+def test_batch_tail_compaction_bitwise(prec, step_graphs, mats):
+    """A batch whose running graphs hold at most half of its level-0 tiles runs its launches over
+    their tiles only (the tables cut at looks): 48 pieces of 86k nodes (2,200 tiles), the
+    larger ones warm-started at their solutions (converged at the start), the rest cold; each
+    piece's result and iterations are bitwise its own solve alone from the same start, with steps
+    recorded as graphs (dropped when the tables are cut) and launched one by one."""
+    sizes = [4000, 3000, 2500, 2000, 1500, 900, 400, 100] * 6
+    gs = [gen.piece(s, pinned=bool(k & 1), seed=500 + k) for k, s in enumerate(sizes)]
+    c = stress.Context(0)
+    c.set_step_graphs(step_graphs)
+    opts = stress.default_options(precision=prec[0], amg_min_nodes=0, **prec[1])
+    u0 = []
+    for k, g in enumerate(gs):
+        if sizes[k] >= 2500:
+            p = stress.StressProblem(g.nodes, g.bonds, mats, c, options=opts)
+            assert p.assemble()
+            u, r = p.solve(g.f, rtol=1e-9, maxit=5000)
+            assert r["converged"], r
+            u0.append(u)
+            p.close()
+        else:
+            u0.append(np.zeros(6 * len(g.nodes)))
+    alone = []
+    for k, g in enumerate(gs):
+        p = stress.StressProblem(g.nodes, g.bonds, mats, c, options=opts)
+        us, rs = stress.solve_batch([p], [g.f], us=[u0[k]], rtol=1e-9, maxit=5000, warm=True)
+        assert rs[0]["converged"], rs[0]
+        alone.append((us[0], rs[0]))
+        p.close()
+    probs = [stress.StressProblem(g.nodes, g.bonds, mats, c, options=opts) for g in gs]
+    us, rs = stress.solve_batch(probs, [g.f for g in gs], us=u0, rtol=1e-9, maxit=5000, warm=True)
+    for k in range(len(gs)):
+        assert rs[k] == alone[k][1], (k, sizes[k], rs[k], alone[k][1])
+        assert np.array_equal(bits(us[k]), bits(alone[k][0])), (k, sizes[k])
+    for p in probs:
+        p.close()
+    c.close()
