@@ -329,6 +329,8 @@ typedef struct svxg_tune_record {
   int32_t samples;           /* launches timed a candidate */
   int32_t cand[4];           /* kinds 0-2: blocks a pass; kind 3: 0 a workgroup a chunk, 1 a wavefront a span */
   double us[4];              /* each candidate's mean launch time (microseconds) */
+  int32_t graphs_log2;       /* the graphs the level's launch spans, log2 */
+  int32_t pad_;
 } svxg_tune_record;
 int svxg_ctx_set_autotune(svxg_ctx* ctx, int32_t on);
 /* The shapes tuned so far: min(count, cap) records to out, *count the number tuned. */

@@ -95,13 +95,14 @@ class TuneRecord(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("fp64", ctypes.c_int32), ("half", ctypes.c_int32),
                 ("rows_log2", ctypes.c_int32), ("per_row", ctypes.c_int32), ("ncand", ctypes.c_int32),
                 ("choice", ctypes.c_int32), ("samples", ctypes.c_int32), ("cand", ctypes.c_int32 * 4),
-                ("us", ctypes.c_double * 4)]
+                ("us", ctypes.c_double * 4), ("graphs_log2", ctypes.c_int32), ("pad_", ctypes.c_int32)]
     KINDS = ("smoothing", "prolongation", "restriction", "dense")
 
     def as_dict(self):
         n = self.ncand
         return {"kind": self.KINDS[self.kind], "fp64": bool(self.fp64), "half": bool(self.half),
-                "rows_log2": self.rows_log2, "per_row": self.per_row, "candidates": list(self.cand[:n]),
+                "rows_log2": self.rows_log2, "per_row": self.per_row, "graphs_log2": self.graphs_log2,
+                "candidates": list(self.cand[:n]),
                 "us": [round(x, 2) for x in self.us[:n]], "choice": self.cand[self.choice]}
 
 

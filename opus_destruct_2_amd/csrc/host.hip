@@ -678,9 +678,13 @@ static int ilog2(int64_t v) {
   }
   return k;
 }
-static uint64_t tune_key(int kind, bool fp64, bool half, int64_t rows, int64_t entries) {
+// a launch shape: its kind, precision and storage, its rows over the batch (log2), entries a row, and
+// the graphs it spans (log2: one graph's level and a batch's many small ones of the same total rows
+// are different launches - their tiles' sizes and counts differ)
+static uint64_t tune_key(int kind, bool fp64, bool half, int64_t rows, int64_t entries, int64_t graphs) {
   const int64_t per = rows > 0 ? std::min<int64_t>(255, (entries + rows / 2) / rows) : 0;
-  return uint64_t(kind) | uint64_t(fp64) << 4 | uint64_t(half) << 5 | uint64_t(ilog2(rows)) << 8 | uint64_t(per) << 16;
+  return uint64_t(kind) | uint64_t(fp64) << 4 | uint64_t(half) << 5 | uint64_t(ilog2(rows)) << 8 | uint64_t(per) << 16 |
+         uint64_t(ilog2(graphs)) << 24;
 }
 static svxg_tune_record tune_record_of(uint64_t key) {
   svxg_tune_record r;
@@ -690,6 +694,7 @@ static svxg_tune_record tune_record_of(uint64_t key) {
   r.half = int32_t(key >> 5 & 1);
   r.rows_log2 = int32_t(key >> 8 & 255);
   r.per_row = int32_t(key >> 16 & 255);
+  r.graphs_log2 = int32_t(key >> 24 & 255);
   return r;
 }
 // the cached choice of a shape (-1: none yet)
@@ -868,7 +873,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     for (int k = 0; k < 3; ++k) {
       Batch::Tune& tu = V.tune[k];
       if (rows[k] == 0 || (k == 2 && v64)) continue;
-      tu.key = tune_key(k, v64, half, rows[k], ent[k]);
+      tu.key = tune_key(k, v64, half, rows[k], ent[k], int64_t(members.size()));
       const int ch = c ? tune_choice(c, tu.key) : -1;
       if (ch >= 0) {
         target[k] = kTuneTargets[ch];
@@ -922,7 +927,7 @@ static int build_batch(Batch& B, svxg_problem* const* probs, int count, Arena* A
     // the dense solve: a wavefront a span for few chunks (tuned against a workgroup a chunk)
     if (!T.dense.empty() && int(T.dense.size()) <= Batch::kDenseSplitChunks) {
       Batch::Tune& tu = V.tune[3];
-      tu.key = tune_key(3, v64, false, dense_rows, int64_t(T.dense.size()) * dense_rows);
+      tu.key = tune_key(3, v64, false, dense_rows, int64_t(T.dense.size()) * dense_rows, int64_t(members.size()));
       const int ch = c ? tune_choice(c, tu.key) : -1;
       V.dsplit = ch >= 0 ? ch == 1 : true;
       if (ch < 0 && c && c->autotune) {
