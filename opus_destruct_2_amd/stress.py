@@ -30,6 +30,16 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise SvxgError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        # A server runs its engine and its setup workers on streams of their own, beside the
+        # context's; with HIP's default of 4 hardware queues a process's streams beyond 4 share
+        # queues, and setup work then waits behind solves (a 60 Hz stream's p50 tick 13.3-13.8 ->
+        # 11.5 ms with 8). The HIP runtime reads this when it starts: before the first HIP call.
+        try:
+            queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        except ValueError:
+            queues = 4
+        if queues < 8:
+            os.environ["GPU_MAX_HW_QUEUES"] = "8"
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
         sig = {
