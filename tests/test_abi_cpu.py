@@ -189,3 +189,12 @@ def test_city_shapes_and_ground_storey_collapse():
     broken = np.zeros(len(g.bonds), np.uint8)
     broken[cut] = 1
     assert _grounded(g, broken)
+
+
+def test_hardware_queues_raised_before_the_library_loads():
+    """A server's engine and setup workers each run on a stream of their own: the package asks HIP for
+    at least 8 hardware queues (read when the runtime starts) before it loads the library."""
+    from opus_destruct_2_amd import stress
+
+    stress.lib()
+    assert int(os.environ["GPU_MAX_HW_QUEUES"]) >= 8
